@@ -1,0 +1,169 @@
+/*
+ * yee.h — C ABI of libyee.so, the MI355X (gfx950) FDTD Yee stepper.
+ *
+ * The library steps caller-owned device buffers (raw HBM pointers, e.g. from torch tensors): it never allocates or
+ * frees device memory, never synchronises inside a step, and reads every per-step scalar (injected source value, DFT
+ * phase, step index) from device tables, so a step sequence can be replayed from a HIP graph.
+ *
+ * State is double-buffered: after n steps the fields and CPML psi live in set n & 1. The absolute step index n of an
+ * instance is its device clock (*clock), advanced by the library after each batch of launches.
+ *
+ * Layouts: 2D arrays are (nx, ny) row-major (y contiguous, index i*ny + j); 3D arrays (nx, ny, nz) with z contiguous.
+ * E_z polarisation: E_z at cell centres, H_x on hi-y faces, H_y on hi-x faces. H_z polarisation: H_z at centres,
+ * E_x on hi-y faces, E_y on hi-x faces. 3D: E_x/E_y/E_z on the hi-x/y/z faces of cell (i, j, k), H on the edges.
+ *
+ * Every function returns 0 on success or a negative YEE_E* code; yee_last_error() gives a thread-local message.
+ * A context is used from one host thread; distinct contexts are independent.
+ */
+#ifndef OPENSOL_AMD_YEE_H
+#define OPENSOL_AMD_YEE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define YEE_ABI_VERSION 1
+#define YEE_MAX_SOURCES 4
+#define YEE_MAX_MONITORS 4
+
+#define YEE_OK 0
+#define YEE_EINVAL (-1)   /* invalid argument or descriptor */
+#define YEE_EHIP (-2)     /* a HIP runtime call failed */
+#define YEE_ESTATE (-3)   /* call out of order (e.g. stepping a destroyed context) */
+
+/* Polarisations of a 2D instance. */
+#define YEE_EZ 0   /* (E_z, H_x, H_y) */
+#define YEE_HZ 1   /* (H_z, E_x, E_y) */
+
+/*
+ * Soft line (sheet) source: field[line cells] += wave[n] * profile after the E update of step n.
+ * Replaces core/sources.py:221-229 (LineSource.inject -> _add_scaled_profile :146-155) and, for H_z,
+ * physics/em/sources.py:469-480 (TangentialSheetCurrentSource.inject_electric on E_x / E_y).
+ * axis 0: line along x at y-cell `line` (E_z, or E_x for H_z); axis 1: line along y at x-cell `line` (E_z, or E_y).
+ */
+typedef struct yee_line_source {
+    int32_t axis;
+    int32_t line;
+    int32_t lo, hi;          /* half-open cell span along the line */
+    const float* profile;    /* (hi - lo) calibrated float32 profile */
+    const float* wave;       /* float32 amplitude * s(t_n + dt/2) per absolute step n */
+    int64_t wave_len;        /* entries in wave; steps n >= wave_len inject nothing */
+} yee_line_source;
+
+/*
+ * Running-DFT flux line with FP64 Kahan-compensated accumulators, sampled after step n when (n + 1) % m == 0.
+ * Replaces physics/em/monitors.py:22-34 (_dft_phase), :86-90 (_dft_accum_line), :203-224 (_LineSpectra.update)
+ * for FluxMonitor (:325-369). normal 0: line along y at x-cell `position`; normal 1: line along x at y-cell.
+ * E_z samples (E_z, H_y) or (E_z, H_x); H_z samples (E_y, H_z) or (E_x, H_z).
+ */
+typedef struct yee_flux_line {
+    int32_t normal;
+    int32_t position;
+    int32_t lo, hi;          /* half-open cells along the line */
+    int32_t n_freq;
+    int32_t pad_;
+    const double* phase;     /* [sample s][n_freq][2]: exp(-j omega t) m dt at t = t_{(s+1) m} */
+    int64_t n_samples;
+    double* acc;             /* [2 (E, H)][n_freq][4 (sum re, sum im, comp re, comp im)][hi - lo] */
+} yee_flux_line;
+
+/*
+ * CPML of one axis: psi rows of the cell-centre ('e') and hi-face ('h') kinds. Replaces
+ * physics/em/boundaries.py:689-772 (CPMLSided: psi = b psi + a (f_hi - f_lo)/d; field += sign coefficient psi),
+ * with gain g = a / d of the bulk spacing per psi row.
+ */
+typedef struct yee_cpml_axis {
+    int32_t n_rows;          /* psi rows per kind (0: no CPML on this axis) */
+    int32_t pad_;
+    const int32_t* row_e;    /* (n) psi row of each centre row, -1 outside the layers */
+    const int32_t* row_h;    /* (n) psi row of each hi-face row, -1 outside */
+    const float* b_e;
+    const float* g_e;
+    const float* b_h;
+    const float* g_h;
+    float* psi_e[2];         /* ping-pong psi of the centre rows */
+    float* psi_h[2];         /* ping-pong psi of the hi-face rows */
+} yee_cpml_axis;
+
+/*
+ * One 2D instance. Replaces the 2D engine step of physics/em/solvers/fdtd_ez_2d.py:71-119 and
+ * fdtd_hz_2d.py:144-189 (Yee kernels fdtd_ez_2d_kernels.py:23-167, fdtd_hz_2d_kernels.py:28-180, CPML, sources,
+ * monitors) with one fused launch per step.
+ * table per material id: E_z (ca, cb, da, db); H_z (da, db, eps_xx, eps_yy) and table2 (ca_x, cb_x, ca_y, cb_y of a
+ * face between two cells of that id, sigma_e, 0, 0, 0); a face between two ids takes the mean eps / sigma (shares 1/2
+ * on uniform grids) and forms ca, cb in the kernel (fdtd_hz_2d_kernels.py:190-278).
+ */
+typedef struct yee2d_desc {
+    int32_t nx, ny;
+    int32_t polarization;    /* YEE_EZ or YEE_HZ */
+    int32_t periodic_x, periodic_y;
+    int32_t rows_per_tile;   /* x rows marched by one wavefront tile (0: library default) */
+    float* field[2][3];      /* [set][component]: E_z (ez, hx, hy) or H_z (hz, ex, ey) */
+    const uint16_t* material;
+    const float* table;      /* (n_materials, 4) */
+    const float* table2;     /* H_z: (n_materials, 8) */
+    int32_t n_materials;
+    float dt;                /* float32 time step (H_z face coefficients) */
+    float dt_over_eps0;      /* float32 dt / eps0 (H_z face coefficients) */
+    int32_t pad_;
+    const float* inv_dx_e;   /* (nx) 1/width at centres, CPML kappa folded in */
+    const float* inv_dx_h;   /* (nx) 1/spacing at hi faces (last: periodic wrap) */
+    const float* inv_dy_e;   /* (ny) */
+    const float* inv_dy_h;   /* (ny) */
+    yee_cpml_axis cpml[2];   /* x, y */
+    int32_t n_sources;
+    int32_t n_monitors;
+    int32_t sample_interval; /* DFT decimation m (physics/em/monitors.py:177-191) */
+    int32_t pad2_;
+    yee_line_source source[YEE_MAX_SOURCES];
+    yee_flux_line monitor[YEE_MAX_MONITORS];
+    int64_t* clock;          /* device int64: absolute step index n of the next step */
+    int64_t step_limit;      /* steps n >= step_limit are skipped (batched instances); INT64_MAX for none */
+} yee2d_desc;
+
+typedef struct yee_ctx yee_ctx;
+
+/* Library ABI version (YEE_ABI_VERSION) and sizeof checks for bindings. */
+int32_t yee_abi_version(void);
+size_t yee_sizeof(int32_t which);   /* 0: yee2d_desc, 1: yee_line_source, 2: yee_flux_line, 3: yee_cpml_axis */
+const char* yee_last_error(void);
+
+/*
+ * Create a stepping context for one 2D instance (host memory only; the descriptor is copied).
+ * Replaces the engine construction of physics/em/solvers/fdtd_2d_base.py:120-223 for the stepping part.
+ */
+int32_t yee2d_create(const yee2d_desc* desc, yee_ctx** out);
+
+/*
+ * Advance by n_steps steps on `stream` (hipStream_t; 0 = the null stream). Launches only: no allocation, no sync.
+ * Whole chunks of graph_steps steps replay a cached HIP graph when graph_steps > 0.
+ * Replaces coupling/runner.py:174-204 (advance) -> EMPhysics.step (physics/em/__init__.py:163-168).
+ */
+int32_t yee_step(yee_ctx* ctx, int64_t n_steps, void* stream);
+
+/* Steps per replayed HIP graph (0 disables graphs; default 0). Drops cached graphs. */
+int32_t yee_set_graph_steps(yee_ctx* ctx, int32_t graph_steps);
+
+/* Replace the descriptor (new buffers or tables); drops cached graphs. */
+int32_t yee2d_update(yee_ctx* ctx, const yee2d_desc* desc);
+
+int32_t yee_destroy(yee_ctx* ctx);
+
+/*
+ * Batched 2D instances (mixed E_z / H_z, uneven sizes) stepped by one launch per step over their tiles.
+ * descs_dev: caller device buffer of yee2d_batch_bytes(n) bytes that receives the descriptors and the tile prefix
+ * (written with hipMemcpyAsync on `stream`). Each instance keeps its own clock and step_limit, so instances of
+ * different lengths share launches and finished ones skip their tiles. No counterpart in the reference, which steps
+ * one model at a time (model/study.py:437-447 builds one model per sweep value).
+ */
+size_t yee2d_batch_bytes(int32_t n);
+int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, void* stream, yee_ctx** out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENSOL_AMD_YEE_H */

@@ -1,0 +1,90 @@
+"""ctypes binding of libyee.so (include/yee.h). Importing the product never falls back to a CPU path: a missing or
+mismatched library raises here."""
+import ctypes
+import os
+
+LIBRARY_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libyee.so')
+ABI_VERSION = 1
+MAX_SOURCES = 4
+MAX_MONITORS = 4
+YEE_EZ, YEE_HZ = 0, 1
+INT64_MAX = (1 << 63) - 1
+
+c_i32, c_i64, c_f32, c_ptr = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_void_p
+
+
+class LineSourceC(ctypes.Structure):
+    _fields_ = [('axis', c_i32), ('line', c_i32), ('lo', c_i32), ('hi', c_i32),
+                ('profile', c_ptr), ('wave', c_ptr), ('wave_len', c_i64)]
+
+
+class FluxLineC(ctypes.Structure):
+    _fields_ = [('normal', c_i32), ('position', c_i32), ('lo', c_i32), ('hi', c_i32), ('n_freq', c_i32),
+                ('pad_', c_i32), ('phase', c_ptr), ('n_samples', c_i64), ('acc', c_ptr)]
+
+
+class CpmlAxisC(ctypes.Structure):
+    _fields_ = [('n_rows', c_i32), ('pad_', c_i32), ('row_e', c_ptr), ('row_h', c_ptr),
+                ('b_e', c_ptr), ('g_e', c_ptr), ('b_h', c_ptr), ('g_h', c_ptr),
+                ('psi_e', c_ptr * 2), ('psi_h', c_ptr * 2)]
+
+
+class Desc2DC(ctypes.Structure):
+    _fields_ = [('nx', c_i32), ('ny', c_i32), ('polarization', c_i32), ('periodic_x', c_i32), ('periodic_y', c_i32),
+                ('rows_per_tile', c_i32), ('field', (c_ptr * 3) * 2), ('material', c_ptr), ('table', c_ptr),
+                ('table2', c_ptr), ('n_materials', c_i32), ('dt', c_f32), ('dt_over_eps0', c_f32), ('pad_', c_i32),
+                ('inv_dx_e', c_ptr), ('inv_dx_h', c_ptr), ('inv_dy_e', c_ptr), ('inv_dy_h', c_ptr),
+                ('cpml', CpmlAxisC * 2), ('n_sources', c_i32), ('n_monitors', c_i32), ('sample_interval', c_i32),
+                ('pad2_', c_i32), ('source', LineSourceC * MAX_SOURCES), ('monitor', FluxLineC * MAX_MONITORS),
+                ('clock', c_ptr), ('step_limit', c_i64)]
+
+
+# Every entry point include/yee.h declares: (name, restype, argtypes)
+SIGNATURES = (
+    ('yee_abi_version', c_i32, ()),
+    ('yee_sizeof', ctypes.c_size_t, (c_i32,)),
+    ('yee_last_error', ctypes.c_char_p, ()),
+    ('yee2d_create', c_i32, (ctypes.POINTER(Desc2DC), ctypes.POINTER(c_ptr))),
+    ('yee_step', c_i32, (c_ptr, c_i64, c_ptr)),
+    ('yee_set_graph_steps', c_i32, (c_ptr, c_i32)),
+    ('yee2d_update', c_i32, (c_ptr, ctypes.POINTER(Desc2DC))),
+    ('yee_destroy', c_i32, (c_ptr,)),
+    ('yee2d_batch_bytes', ctypes.c_size_t, (c_i32,)),
+    ('yee2d_batch_create', c_i32, (ctypes.POINTER(Desc2DC), c_i32, c_ptr, c_ptr, ctypes.POINTER(c_ptr))),
+)
+
+_LIB = None
+
+
+class YeeError(RuntimeError):
+    """A libyee.so call returned a non-zero status."""
+
+
+def load():
+    """The loaded library (once per process); RuntimeError when it is missing or its ABI differs."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBRARY_PATH):
+        raise RuntimeError(f'{LIBRARY_PATH} is missing: build the HIP library first (python -m opensol_amd.build); '
+                           f'there is no CPU fallback')
+    lib = ctypes.CDLL(LIBRARY_PATH)
+    for name, restype, argtypes in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = list(argtypes)
+    if lib.yee_abi_version() != ABI_VERSION:
+        raise RuntimeError(f'libyee.so ABI {lib.yee_abi_version()} != binding ABI {ABI_VERSION}')
+    for which, struct in enumerate((Desc2DC, LineSourceC, FluxLineC, CpmlAxisC)):
+        if lib.yee_sizeof(which) != ctypes.sizeof(struct):
+            raise RuntimeError(f'libyee.so struct {struct.__name__} is {lib.yee_sizeof(which)} bytes, the binding '
+                               f'{ctypes.sizeof(struct)}: rebuild the library')
+    _LIB = lib
+    return lib
+
+
+def check(status, what):
+    """Raise YeeError with the library's message for a non-zero status."""
+    if status != 0:
+        message = load().yee_last_error()
+        raise YeeError(f'{what} failed ({status}): {message.decode() if message else ""}')

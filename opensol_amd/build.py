@@ -1,0 +1,41 @@
+"""Build libyee.so (HIP, gfx950) in-tree: python -m opensol_amd.build"""
+import os
+import shutil
+import subprocess
+import sys
+
+PACKAGE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PACKAGE)
+CSRC = os.path.join(PACKAGE, 'csrc')
+INCLUDE = os.path.join(ROOT, 'include')
+LIBRARY = os.path.join(PACKAGE, 'libyee.so')
+SOURCES = ('yee2d.hip', 'yee_api.hip')
+ARCH = 'gfx950'
+# Packed FP32 instructions are disabled for every HIP source; hipcc -O3 otherwise forms them from float code.
+FLAGS = ['-O3', '-std=c++17', '-fPIC', '-shared', f'--offload-arch={ARCH}', '-munsafe-fp-atomics',
+         '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-Wall', '-Wno-unused-function']
+
+
+def hipcc():
+    path = shutil.which('hipcc') or '/opt/rocm/bin/hipcc'
+    if not os.path.exists(path):
+        raise RuntimeError('hipcc not found: the HIP toolchain (ROCm) is required to build libyee.so')
+    return path
+
+
+def build(verbose=False):
+    """Compile every HIP source into opensol_amd/libyee.so (always recompiles)."""
+    sources = [os.path.join(CSRC, name) for name in SOURCES]
+    tmp = LIBRARY + '.tmp'
+    cmd = [hipcc(), *FLAGS, f'-I{INCLUDE}', f'-I{CSRC}', *sources, '-o', tmp]
+    if verbose:
+        print(' '.join(cmd))
+    result = subprocess.run(cmd, capture_output=True, text=True)
+    if result.returncode != 0:
+        raise RuntimeError(f'libyee.so build failed:\n{result.stdout}\n{result.stderr}')
+    os.replace(tmp, LIBRARY)
+    return LIBRARY
+
+
+if __name__ == '__main__':
+    print(build(verbose='-v' in sys.argv))

@@ -1,0 +1,305 @@
+// C ABI of libyee.so: stepping contexts over caller-owned device buffers (see include/yee.h).
+// The library allocates no device memory; a context holds the host copy of its descriptor(s), the launch geometry
+// and (optionally) an instantiated HIP graph that replays graph_steps steps plus the clock advance.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "yee.h"
+
+__global__ void yee2d_step_kernel(yee2d_desc d, int32_t offset);
+__global__ void yee2d_batch_kernel(const yee2d_desc* __restrict__ descs, const int32_t* __restrict__ prefix,
+                                   int32_t n_inst, int32_t offset);
+__global__ void yee_clock_advance(int64_t* clock, int64_t k, int64_t limit);
+__global__ void yee_clock_advance_batch(const yee2d_desc* __restrict__ descs, int32_t n_inst, int64_t k);
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int32_t fail(int32_t code, const std::string& message)
+{
+    g_last_error = message;
+    return code;
+}
+
+#define YEE_HIP(call)                                                                                  \
+    do {                                                                                               \
+        hipError_t err_ = (call);                                                                      \
+        if (err_ != hipSuccess)                                                                        \
+            return fail(YEE_EHIP, std::string(#call) + ": " + hipGetErrorString(err_));                \
+    } while (0)
+
+constexpr int kDefaultRows2D = 64;
+constexpr int kOwn = 63;   // owned columns per wavefront tile (yee2d.hip)
+
+enum CtxKind { kSingle2D = 1, kBatch2D = 2 };
+
+}  // namespace
+
+struct yee_ctx {
+    int kind = 0;
+    yee2d_desc d2{};
+    // batch
+    std::vector<yee2d_desc> descs;
+    yee2d_desc* descs_dev = nullptr;
+    int32_t* prefix_dev = nullptr;
+    int32_t n_inst = 0;
+    // launch geometry
+    int64_t tiles = 0;
+    int blocks = 0;
+    // graph replay
+    int32_t graph_steps = 0;
+    hipGraphExec_t graph = nullptr;
+    hipStream_t capture_stream = nullptr;
+};
+
+namespace {
+
+int32_t check_2d(const yee2d_desc* d)
+{
+    if (!d) return fail(YEE_EINVAL, "null descriptor");
+    if (d->nx < 2 || d->ny < 2) return fail(YEE_EINVAL, "2D grid needs nx, ny >= 2");
+    if (d->polarization != YEE_EZ && d->polarization != YEE_HZ) return fail(YEE_EINVAL, "polarization must be 0 or 1");
+    if (d->rows_per_tile < 1) return fail(YEE_EINVAL, "rows_per_tile must be >= 1");
+    for (int s = 0; s < 2; ++s)
+        for (int c = 0; c < 3; ++c)
+            if (!d->field[s][c]) return fail(YEE_EINVAL, "null field buffer");
+    if (!d->material || !d->table || d->n_materials < 1) return fail(YEE_EINVAL, "material ids / table missing");
+    if (d->polarization == YEE_HZ && !d->table2) return fail(YEE_EINVAL, "H_z needs table2");
+    if (!d->inv_dx_e || !d->inv_dx_h || !d->inv_dy_e || !d->inv_dy_h) return fail(YEE_EINVAL, "spacings missing");
+    const int n_axis[2] = {d->nx, d->ny};
+    for (int a = 0; a < 2; ++a) {
+        const yee_cpml_axis& c = d->cpml[a];
+        if (c.n_rows < 0) return fail(YEE_EINVAL, "negative CPML rows");
+        if (c.n_rows > 0) {
+            if (!c.row_e || !c.row_h || !c.b_e || !c.g_e || !c.b_h || !c.g_h || !c.psi_e[0] || !c.psi_e[1] ||
+                !c.psi_h[0] || !c.psi_h[1])
+                return fail(YEE_EINVAL, "CPML arrays missing");
+            if (c.n_rows > n_axis[a]) return fail(YEE_EINVAL, "more CPML rows than cells");
+        }
+    }
+    if (d->n_sources < 0 || d->n_sources > YEE_MAX_SOURCES) return fail(YEE_EINVAL, "n_sources out of range");
+    for (int s = 0; s < d->n_sources; ++s) {
+        const yee_line_source& src = d->source[s];
+        if (src.axis != 0 && src.axis != 1) return fail(YEE_EINVAL, "source axis must be 0 or 1");
+        const int n_line = src.axis == 1 ? d->nx : d->ny;   // the fixed coordinate's extent
+        const int n_along = src.axis == 1 ? d->ny : d->nx;
+        if (src.line < 0 || src.line >= n_line) return fail(YEE_EINVAL, "source line outside the grid");
+        if (src.lo < 0 || src.hi > n_along || src.lo >= src.hi) return fail(YEE_EINVAL, "source span out of range");
+        if (!src.profile || !src.wave) return fail(YEE_EINVAL, "source profile / wave missing");
+    }
+    if (d->n_monitors < 0 || d->n_monitors > YEE_MAX_MONITORS) return fail(YEE_EINVAL, "n_monitors out of range");
+    if (d->n_monitors > 0 && d->sample_interval < 1) return fail(YEE_EINVAL, "sample_interval must be >= 1");
+    for (int m = 0; m < d->n_monitors; ++m) {
+        const yee_flux_line& mon = d->monitor[m];
+        if (mon.normal != 0 && mon.normal != 1) return fail(YEE_EINVAL, "monitor normal must be 0 or 1");
+        const int n_pos = mon.normal == 0 ? d->nx : d->ny;
+        const int n_along = mon.normal == 0 ? d->ny : d->nx;
+        if (mon.position < 0 || mon.position >= n_pos) return fail(YEE_EINVAL, "monitor line outside the grid");
+        if (mon.lo < 0 || mon.hi > n_along || mon.lo >= mon.hi) return fail(YEE_EINVAL, "monitor span out of range");
+        if (mon.n_freq < 1 || !mon.phase || !mon.acc) return fail(YEE_EINVAL, "monitor arrays missing");
+    }
+    if (!d->clock) return fail(YEE_EINVAL, "clock missing");
+    return YEE_OK;
+}
+
+int64_t tiles_2d(const yee2d_desc& d)
+{
+    const int64_t tj = (d.ny + kOwn - 1) / kOwn;
+    const int64_t ti = (d.nx + d.rows_per_tile - 1) / d.rows_per_tile;
+    return tj * ti;
+}
+
+void drop_graph(yee_ctx* ctx)
+{
+    if (ctx->graph) {
+        (void)hipGraphExecDestroy(ctx->graph);
+        ctx->graph = nullptr;
+    }
+}
+
+// Launch `count` steps (offsets 0..count-1 from the device clock) followed by the clock advance.
+int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
+{
+    if (count <= 0) return YEE_OK;
+    if (ctx->kind == kSingle2D) {
+        for (int32_t k = 0; k < count; ++k)
+            hipLaunchKernelGGL(yee2d_step_kernel, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d2, k);
+        YEE_HIP(hipGetLastError());
+        hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, stream, ctx->d2.clock, (int64_t)count,
+                           ctx->d2.step_limit);
+        YEE_HIP(hipGetLastError());
+        return YEE_OK;
+    }
+    if (ctx->kind == kBatch2D) {
+        for (int32_t k = 0; k < count; ++k)
+            hipLaunchKernelGGL(yee2d_batch_kernel, dim3(ctx->blocks), dim3(256), 0, stream, ctx->descs_dev,
+                               ctx->prefix_dev, ctx->n_inst, k);
+        YEE_HIP(hipGetLastError());
+        const int threads = 256;
+        hipLaunchKernelGGL(yee_clock_advance_batch, dim3((ctx->n_inst + threads - 1) / threads), dim3(threads), 0,
+                           stream, ctx->descs_dev, ctx->n_inst, (int64_t)count);
+        YEE_HIP(hipGetLastError());
+        return YEE_OK;
+    }
+    return fail(YEE_ESTATE, "context has no instances");
+}
+
+int32_t build_graph(yee_ctx* ctx)
+{
+    if (!ctx->capture_stream) YEE_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+    YEE_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
+    int32_t status = launch_steps(ctx, ctx->graph_steps, ctx->capture_stream);
+    hipGraph_t graph = nullptr;
+    hipError_t end = hipStreamEndCapture(ctx->capture_stream, &graph);
+    if (status != YEE_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return status;
+    }
+    if (end != hipSuccess) return fail(YEE_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(end));
+    hipError_t inst = hipGraphInstantiate(&ctx->graph, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (inst != hipSuccess) {
+        ctx->graph = nullptr;
+        return fail(YEE_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(inst));
+    }
+    return YEE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t yee_abi_version(void) { return YEE_ABI_VERSION; }
+
+size_t yee_sizeof(int32_t which)
+{
+    switch (which) {
+        case 0: return sizeof(yee2d_desc);
+        case 1: return sizeof(yee_line_source);
+        case 2: return sizeof(yee_flux_line);
+        case 3: return sizeof(yee_cpml_axis);
+        default: return 0;
+    }
+}
+
+const char* yee_last_error(void) { return g_last_error.c_str(); }
+
+int32_t yee2d_create(const yee2d_desc* desc, yee_ctx** out)
+{
+    if (!out) return fail(YEE_EINVAL, "null output pointer");
+    *out = nullptr;
+    yee2d_desc d = desc ? *desc : yee2d_desc{};
+    if (desc && d.rows_per_tile == 0) d.rows_per_tile = kDefaultRows2D;
+    int32_t status = check_2d(desc ? &d : nullptr);
+    if (status != YEE_OK) return status;
+    yee_ctx* ctx = new yee_ctx();
+    ctx->kind = kSingle2D;
+    ctx->d2 = d;
+    ctx->tiles = tiles_2d(d);
+    ctx->blocks = (int)((ctx->tiles + 3) / 4);
+    *out = ctx;
+    return YEE_OK;
+}
+
+int32_t yee2d_update(yee_ctx* ctx, const yee2d_desc* desc)
+{
+    if (!ctx || ctx->kind != kSingle2D) return fail(YEE_ESTATE, "not a single 2D context");
+    yee2d_desc d = desc ? *desc : yee2d_desc{};
+    if (desc && d.rows_per_tile == 0) d.rows_per_tile = kDefaultRows2D;
+    int32_t status = check_2d(desc ? &d : nullptr);
+    if (status != YEE_OK) return status;
+    drop_graph(ctx);
+    ctx->d2 = d;
+    ctx->tiles = tiles_2d(d);
+    ctx->blocks = (int)((ctx->tiles + 3) / 4);
+    return YEE_OK;
+}
+
+int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, void* stream, yee_ctx** out)
+{
+    if (!out) return fail(YEE_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (!descs || n < 1 || !descs_dev) return fail(YEE_EINVAL, "batch needs descriptors and a device buffer");
+    std::vector<yee2d_desc> host(descs, descs + n);
+    std::vector<int32_t> prefix(n + 1, 0);
+    for (int32_t k = 0; k < n; ++k) {
+        if (host[k].rows_per_tile == 0) host[k].rows_per_tile = kDefaultRows2D;
+        int32_t status = check_2d(&host[k]);
+        if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
+        const int64_t next = prefix[k] + tiles_2d(host[k]);
+        if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
+        prefix[k + 1] = (int32_t)next;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    auto* dev_descs = reinterpret_cast<yee2d_desc*>(descs_dev);
+    auto* dev_prefix = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(descs_dev) + sizeof(yee2d_desc) * n);
+    YEE_HIP(hipMemcpyAsync(dev_descs, host.data(), sizeof(yee2d_desc) * n, hipMemcpyHostToDevice, s));
+    YEE_HIP(hipMemcpyAsync(dev_prefix, prefix.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
+    yee_ctx* ctx = new yee_ctx();
+    ctx->kind = kBatch2D;
+    ctx->descs = std::move(host);
+    ctx->descs_dev = dev_descs;
+    ctx->prefix_dev = dev_prefix;
+    ctx->n_inst = n;
+    ctx->tiles = prefix[n];
+    ctx->blocks = (int)((ctx->tiles + 3) / 4);
+    *out = ctx;
+    return YEE_OK;
+}
+
+size_t yee2d_batch_bytes(int32_t n)
+{
+    return n < 1 ? 0 : sizeof(yee2d_desc) * (size_t)n + sizeof(int32_t) * (size_t)(n + 1);
+}
+
+int32_t yee_set_graph_steps(yee_ctx* ctx, int32_t graph_steps)
+{
+    if (!ctx) return fail(YEE_ESTATE, "null context");
+    if (graph_steps < 0) return fail(YEE_EINVAL, "graph_steps must be >= 0");
+    drop_graph(ctx);
+    ctx->graph_steps = graph_steps;
+    return YEE_OK;
+}
+
+int32_t yee_step(yee_ctx* ctx, int64_t n_steps, void* stream)
+{
+    if (!ctx || !ctx->kind) return fail(YEE_ESTATE, "null or destroyed context");
+    if (n_steps < 0) return fail(YEE_EINVAL, "n_steps must be >= 0");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    int64_t remaining = n_steps;
+    if (ctx->graph_steps > 0 && remaining >= ctx->graph_steps) {
+        if (!ctx->graph) {
+            int32_t status = build_graph(ctx);
+            if (status != YEE_OK) return status;
+        }
+        while (remaining >= ctx->graph_steps) {
+            YEE_HIP(hipGraphLaunch(ctx->graph, s));
+            remaining -= ctx->graph_steps;
+        }
+    }
+    while (remaining > 0) {
+        const int32_t chunk = remaining > 4096 ? 4096 : (int32_t)remaining;
+        int32_t status = launch_steps(ctx, chunk, s);
+        if (status != YEE_OK) return status;
+        remaining -= chunk;
+    }
+    return YEE_OK;
+}
+
+int32_t yee_destroy(yee_ctx* ctx)
+{
+    if (!ctx) return YEE_OK;
+    drop_graph(ctx);
+    if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
+    ctx->kind = 0;
+    delete ctx;
+    return YEE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,431 @@
+"""2D FDTD models (host setup) and their device engine over libyee.so.
+
+Model2D gathers what the reference spreads over Domain / MaterialGrid / EMPhysics / BoundaryManager / CPMLSided and
+the source and monitor objects (physics/em/__init__.py:63-557): geometry, a material-index grid, CPML tables, curl
+spacings, calibrated sheet sources and flux monitors, all host float64 rounded once to float32. Engine2D owns the
+device buffers (torch tensors on the ROCm device: memory only) and steps them with one fused HIP launch per step.
+"""
+import math
+
+import numpy as np
+
+from . import _lib
+from .boundaries import CPMLAxisTables, curl_spacings, resolve_boundaries
+from .constants import C0, DTYPE, EPS0, MU0
+from .domain import Domain
+from .materials import MaterialMap, semi_implicit
+from .monitors import FluxMonitor, dft_sample_interval
+from .sources import LineSource, complex_permittivity, resolve_frequencies, resolve_frequency
+from .waveforms import make_temporal
+
+POLARIZATIONS = ('ez', 'hz')
+COMPONENTS = {'ez': ('ez', 'hx', 'hy'), 'hz': ('hz', 'ex', 'ey')}
+
+
+class Model2D:
+    """Host description of one 2D FDTD model (E_z or H_z polarisation) on a uniform grid."""
+
+    def __init__(self, size_x, size_y, dx, npml=15, courant=None, boundaries=None, polarization='ez'):
+        if polarization not in POLARIZATIONS:
+            raise ValueError(f"polarization must be 'ez' (E_z, H_x, H_y) or 'hz' (H_z, E_x, E_y), got "
+                             f"{polarization!r}")
+        if size_x is None or size_y is None or dx is None:
+            raise ValueError('a 2D model needs size_x, size_y and dx')
+        self.polarization = polarization
+        self.bc = resolve_boundaries(boundaries, npml)
+        if self.bc.dimensions != 2:
+            raise ValueError('a 2D model needs a 2D BoundaryConfig')
+        self.domain = Domain((size_x, size_y), dx, self.bc.npml_sides, courant)
+        self.grid = self.domain.grid
+        self.nx, self.ny = self.grid.shape
+        self.dt = self.grid.dt
+        self.materials = MaterialMap((self.nx, self.ny))
+        self.sources = []
+        self.monitors = []
+        dx32 = float(self.grid.dx)
+        self.cpml = {axis: CPMLAxisTables(self.bc, axis, n, dx32, float(self.dt))
+                     for axis, n in (('x', self.nx), ('y', self.ny))}
+        self.inv_dx_e, self.inv_dx_h = curl_spacings(self.nx, dx32, self.cpml['x'])
+        self.inv_dy_e, self.inv_dy_h = curl_spacings(self.ny, dx32, self.cpml['y'])
+        self.sample_interval = 1
+
+    @property
+    def shape(self):
+        return (self.nx, self.ny)
+
+    @property
+    def components(self):
+        return COMPONENTS[self.polarization]
+
+    @property
+    def cell_count(self):
+        return self.nx * self.ny
+
+    # ---- materials ------------------------------------------------------------------------------------------------
+    def add_dielectric_rectangle(self, x0, y0, x1, y1, eps_r, sigma_e=0.0):
+        """Cells whose float32 centre lies in [x_min, x_max) x [y_min, y_max) get eps_r (every permittivity array
+        of the polarisation) and sigma_e (physics/em/__init__.py:317-336; core/domain.py:255-260)."""
+        ranges = self.domain.box_ranges((x0, y0), (x1, y1))
+        self.materials.paint_box(ranges, self._material(eps_r, sigma_e))
+
+    def add_dielectric_circle(self, cx, cy, radius, eps_r, sigma_e=0.0):
+        """Cells whose float32 centre lies within `radius` of (cx, cy) (core/domain.py:262-268)."""
+        if not radius >= 0:
+            raise ValueError(f'circle radius must be >= 0, got {radius}')
+        x, y = (c.astype(DTYPE) for c in self.domain.cell_centers())
+        mask = (x[:, None] - DTYPE(cx)) ** 2 + (y[None, :] - DTYPE(cy)) ** 2 <= DTYPE(radius) ** 2
+        self.materials.paint_mask(mask, self._material(eps_r, sigma_e))
+
+    def _material(self, eps_r, sigma_e):
+        return self.materials.material_id(eps_r, eps_r, eps_r, sigma_e, 1.0, 0.0)
+
+    def check_courant(self):
+        """ValueError when eps * mu_r < (C0 dt)^2 (1/dx^2 + 1/dy^2) for a material in use (physics/em/__init__.py:
+        242-281; eps = eps_r for E_z, min(eps_xx, eps_yy) for H_z)."""
+        values = self.materials.values()[self.materials.used()]
+        eps = values[:, 0] if self.polarization == 'ez' else np.minimum(values[:, 0], values[:, 1])
+        dx2 = float(self.domain.dx) ** 2
+        required = np.float64(DTYPE((C0 * float(self.dt)) ** 2 / dx2)) * 2.0
+        ratio = eps * values[:, 4] / required
+        worst = float(ratio.min())
+        if worst < 1.0 - 1e-6:
+            factor = math.floor(float(self.grid.courant) * math.sqrt(max(worst, 0.0)) * 1e3) / 1e3
+            raise ValueError(f'eps * mu_r = {float((eps * values[:, 4]).min()):.4g} is below (C0 dt)^2 (1/dx^2 + '
+                             f'1/dy^2) = {required:.4g}, so the time step is unstable; lower courant_factor to <= '
+                             f'{factor:.3f}')
+
+    # ---- sources and monitors -------------------------------------------------------------------------------------
+    def add_line_source(self, axis, position, wavelength=None, frequency=None, bandwidth=0.5, amplitude=1.0,
+                        waveform='gaussian', span=None, waist=None, center=None):
+        """Sheet current radiating `amplitude` [V/m] to each side (physics/em/__init__.py:442-473,
+        placement.py:223-251): axis 'x' = line along x at y = position; span None = the whole grid line; a span end
+        on a cpml window edge continues through the PML."""
+        temporal = make_temporal(waveform, resolve_frequency(wavelength, frequency), bandwidth)
+        domain, bc = self.domain, self.bc
+        if axis not in ('x', 'y'):
+            raise ValueError(f"Line source axis must be 'x' or 'y', got {axis!r}")
+        along_axis, normal_axis = (0, 1) if axis == 'x' else (1, 0)
+        point = [0.0, 0.0]
+        point[normal_axis] = position
+        domain.validate_physical_pos(point, f"Line source ({'xy'[normal_axis]})")
+        line_cell = domain.pos_to_cell_axis(normal_axis, position)
+        n_along = self.grid.shape[along_axis]
+        if span is None:
+            span_cells = (0, n_along)
+        else:
+            start, end = sorted((float(span[0]), float(span[1])))
+            for coord in (start, end):
+                p = list(point)
+                p[along_axis] = coord
+                domain.validate_physical_pos(p, 'Line source span')
+            lo, hi = domain.pos_to_cell_axis(along_axis, start), domain.pos_to_cell_axis(along_axis, end) + 1
+            window = domain.physical_window()[along_axis]
+            if lo == window.start and bc.faces[f'{axis}_lo'] == 'cpml':
+                lo = 0
+            if hi == window.stop and bc.faces[f'{axis}_hi'] == 'cpml':
+                hi = n_along
+            span_cells = (lo, hi)
+        if self.polarization == 'ez':
+            normal_width = float(domain.cell_widths()[normal_axis][line_cell])
+            component = 'ez'
+        else:
+            n_normal = self.grid.shape[normal_axis]
+            periodic = bc.periodic('xy'[normal_axis])
+            if line_cell == n_normal - 1 and not periodic:
+                raise ValueError(f'Line source: the H_z sheet sits on the hi face of cell {line_cell}, the outer grid '
+                                 f'wall; move the line at least one cell inside')
+            # dual spacing of the hi face = 1 / the float64 inverse face spacing (placement.py:253-265)
+            w = float(self.grid.dx)
+            normal_width = 1.0 / (1.0 / (0.5 * (w + w)))
+            component = 'ex' if axis == 'x' else 'ey'
+        shape = None
+        if waist is not None:
+            centres = domain.cell_centers()[along_axis][span_cells[0]:span_cells[1]]
+            shape = np.exp(-((centres - float(center)) / float(waist)) ** 2)
+        source = LineSource(axis, line_cell, temporal, amplitude, span_cells, normal_width, component, shape)
+        self.sources.append(source)
+        return source
+
+    def add_flux_monitor(self, axis, position, span, wavelengths=None, frequencies=None):
+        """Poynting flux through a line (axis 'x' = along x at y = position, flux > 0 toward +y); span = (start, end)
+        along it [m] (placement.py:55-70, physics/em/__init__.py:511-518)."""
+        if span is None or len(span) != 2:
+            raise ValueError(f'Flux monitor needs span = (start, end) along the line [m], got {span!r}')
+        freqs = resolve_frequencies(wavelengths, frequencies)
+        lo, hi = sorted((float(span[0]), float(span[1])))
+        domain = self.domain
+        widths = domain.cell_widths()
+        if axis == 'x':
+            domain.validate_physical_pos((lo, position), 'Flux monitor start')
+            domain.validate_physical_pos((hi, position), 'Flux monitor end')
+            normal, line_cell = 'y', domain.pos_to_cell_axis(1, position)
+            cells = (domain.pos_to_cell_axis(0, lo), domain.pos_to_cell_axis(0, hi) + 1)
+            along_widths = widths[0]
+        elif axis == 'y':
+            domain.validate_physical_pos((position, lo), 'Flux monitor start')
+            domain.validate_physical_pos((position, hi), 'Flux monitor end')
+            normal, line_cell = 'x', domain.pos_to_cell_axis(0, position)
+            cells = (domain.pos_to_cell_axis(1, lo), domain.pos_to_cell_axis(1, hi) + 1)
+            along_widths = widths[1]
+        else:
+            raise ValueError(f"Flux monitor axis (along the line) must be 'x' or 'y', got {axis!r}")
+        monitor = FluxMonitor(normal, line_cell, cells, freqs, float(self.dt), along_widths, self.polarization)
+        self.monitors.append(monitor)
+        return monitor
+
+    def finalize(self):
+        """Courant check, sheet calibration and the DFT sample interval (physics/em/__init__.py:181-203)."""
+        if len(self.sources) > _lib.MAX_SOURCES:
+            raise ValueError(f'at most {_lib.MAX_SOURCES} line sources per 2D model')
+        if len(self.monitors) > _lib.MAX_MONITORS:
+            raise ValueError(f'at most {_lib.MAX_MONITORS} flux monitors per 2D model')
+        self.check_courant()
+        self.calibrate()
+        self.sample_interval = dft_sample_interval(float(self.dt), self.sources,
+                                                   [m.frequencies for m in self.monitors])
+        for monitor in self.monitors:
+            monitor.sample_interval = self.sample_interval
+
+    def calibrate(self):
+        """Sheet profiles from the float64 coefficients of the line cells (physics/em/sources.py:582-602)."""
+        values = self.materials.values()
+        dt = float(self.dt)
+        for source in self.sources:
+            omega = 2.0 * np.pi * float(source.temporal.f0)
+            ids = source.line(self.materials.ids).astype(np.int64)
+            mu, sigma_m = values[ids, 4], values[ids, 5]
+            if self.polarization == 'ez':
+                _, cb = semi_implicit(values[ids, 0], values[ids, 3], dt, EPS0)
+                line_eps = complex_permittivity(values[ids, 0], values[ids, 3], omega)
+            else:
+                column = 0 if source.component == 'ex' else 1
+                normal_axis = 1 if source.axis == 'x' else 0
+                n_normal = self.grid.shape[normal_axis]
+                upper = (source.position + 1) % n_normal
+                if upper == 0 and not self.bc.periodic('xy'[normal_axis]):
+                    upper = source.position
+                lo, hi = source.span
+                upper_ids = (self.materials.ids[lo:hi, upper] if source.axis == 'x'
+                             else self.materials.ids[upper, lo:hi]).astype(np.int64)
+                eps_face = values[ids, column] * 0.5 + values[upper_ids, column] * 0.5
+                sig_face = values[ids, 3] * 0.5 + values[upper_ids, 3] * 0.5
+                _, cb = semi_implicit(eps_face, sig_face, dt, EPS0)
+                w = float(self.grid.dx)
+                line_eps = (complex_permittivity(values[ids, column], values[ids, 3], omega) * w
+                            + complex_permittivity(values[upper_ids, column], values[upper_ids, 3], omega) * w
+                            ) / (w + w)
+            source.calibrate(cb, line_eps, mu, sigma_m, dt)
+
+
+def clock_times(dt, n_steps):
+    """float64 times after 0..n_steps steps from t = 0, accumulated t += dt as the reference's clocks do
+    (solvers/fdtd_2d_base.py:470-488, coupling/runner.py:191-203); cumsum adds sequentially."""
+    return np.concatenate(([0.0], np.cumsum(np.full(int(n_steps), float(dt)))))
+
+
+def steps_for_duration(time, dt, duration):
+    """Ticks the runner takes from `time` for `duration`: it stops once t >= t_end - dt/2 (coupling/runner.py:182-203)."""
+    dt = float(dt)
+    stop = float(time) + float(duration) - 0.5 * dt
+    t, n = float(time), 0
+    while t < stop:
+        t = t + dt
+        n += 1
+    return n
+
+
+class Engine2D:
+    """Device state and libyee context of one Model2D. torch provides the HBM buffers and the stream."""
+
+    def __init__(self, model, device=None, rows_per_tile=64, graph_steps=0, capacity=1024):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError('Engine2D needs a ROCm GPU (torch.cuda.is_available() is False); there is no CPU path')
+        self.torch = torch
+        self.lib = _lib.load()
+        self.model = model
+        self.device = torch.device('cuda', torch.cuda.current_device() if device is None else device)
+        self.rows_per_tile = int(rows_per_tile)
+        self.graph_steps = int(graph_steps)
+        self.n_step = 0
+        self.capacity = 0
+        self._times = None
+        nx, ny = model.shape
+        dev, f32 = self.device, torch.float32
+        self.fields = torch.zeros((2, 3, nx, ny), dtype=f32, device=dev)
+        self.clock = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.inv = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in
+                    (model.inv_dx_e, model.inv_dx_h, model.inv_dy_e, model.inv_dy_h)]
+        self.cpml = {}
+        for axis, other in (('x', ny), ('y', nx)):
+            tables = model.cpml[axis]
+            if tables.n_rows == 0:
+                self.cpml[axis] = None
+                continue
+            shape = (2, tables.n_rows, other) if axis == 'x' else (2, nx, tables.n_rows)
+            self.cpml[axis] = dict(
+                row_e=torch.from_numpy(tables.row_e).to(dev), row_h=torch.from_numpy(tables.row_h).to(dev),
+                b_e=torch.from_numpy(tables.b_e).to(dev), g_e=torch.from_numpy(tables.g_e).to(dev),
+                b_h=torch.from_numpy(tables.b_h).to(dev), g_h=torch.from_numpy(tables.g_h).to(dev),
+                psi_e=torch.zeros(shape, dtype=f32, device=dev), psi_h=torch.zeros(shape, dtype=f32, device=dev))
+        self.profiles = [torch.from_numpy(s.profile.copy()).to(dev) for s in model.sources]
+        self.accumulators = [torch.zeros(m.accumulator_shape(), dtype=torch.float64, device=dev)
+                             for m in model.monitors]
+        for index, monitor in enumerate(model.monitors):
+            monitor.engine, monitor.index = self, index
+        self.waves = [None] * len(model.sources)
+        self.phases = [None] * len(model.monitors)
+        self._upload_materials()
+        self._ctx = _lib.c_ptr()
+        self._ensure_capacity(capacity, create=True)
+        if self.graph_steps:
+            _lib.check(self.lib.yee_set_graph_steps(self._ctx, self.graph_steps), 'yee_set_graph_steps')
+
+    # ---- device tables --------------------------------------------------------------------------------------------
+    def _upload_materials(self):
+        torch, model = self.torch, self.model
+        ids = np.ascontiguousarray(model.materials.ids).view(np.int16)
+        self.material = torch.from_numpy(ids).to(self.device)
+        if model.polarization == 'ez':
+            self.table = torch.from_numpy(model.materials.table_ez(model.dt)).to(self.device)
+            self.table2 = None
+        else:
+            table, table2 = model.materials.tables_hz(model.dt)
+            self.table = torch.from_numpy(table).to(self.device)
+            self.table2 = torch.from_numpy(table2).to(self.device)
+        self._materials_version = model.materials.version
+
+    def _ensure_capacity(self, steps, create=False):
+        """Waveform and DFT phase tables covering absolute steps [0, steps)."""
+        if steps <= self.capacity and not create:
+            return
+        torch, model = self.torch, self.model
+        capacity = max(int(steps), 2 * self.capacity, 1024)
+        times = clock_times(model.dt, capacity)
+        half = times[:-1] + 0.5 * float(model.dt)
+        self.waves = [torch.from_numpy(s.wave_table(half)).to(self.device) for s in model.sources]
+        m = model.sample_interval
+        n_samples = capacity // m
+        sample_times = times[m * np.arange(1, n_samples + 1)]
+        self.phases = [torch.from_numpy(np.ascontiguousarray(mon.phase_table(sample_times))).to(self.device)
+                       for mon in model.monitors]
+        self._times = times
+        self.capacity = capacity
+        self._rebind(create)
+
+    def descriptor(self):
+        """The yee2d_desc of the current buffers (for the single-instance context or a batch)."""
+        model = self.model
+        d = _lib.Desc2DC()
+        d.nx, d.ny = model.nx, model.ny
+        d.polarization = _lib.YEE_EZ if model.polarization == 'ez' else _lib.YEE_HZ
+        d.periodic_x, d.periodic_y = int(model.bc.periodic('x')), int(model.bc.periodic('y'))
+        d.rows_per_tile = self.rows_per_tile
+        for s in range(2):
+            for c in range(3):
+                d.field[s][c] = self.fields[s, c].data_ptr()
+        d.material = self.material.data_ptr()
+        d.table = self.table.data_ptr()
+        d.table2 = self.table2.data_ptr() if self.table2 is not None else None
+        d.n_materials = len(model.materials.materials)
+        d.dt = float(model.dt)
+        d.dt_over_eps0 = float(np.float32(float(model.dt) / EPS0))
+        d.inv_dx_e, d.inv_dx_h, d.inv_dy_e, d.inv_dy_h = (t.data_ptr() for t in self.inv)
+        for a, axis in enumerate('xy'):
+            c = self.cpml[axis]
+            if c is None:
+                d.cpml[a].n_rows = 0
+                continue
+            d.cpml[a].n_rows = model.cpml[axis].n_rows
+            for name in ('row_e', 'row_h', 'b_e', 'g_e', 'b_h', 'g_h'):
+                setattr(d.cpml[a], name, c[name].data_ptr())
+            for s in range(2):
+                d.cpml[a].psi_e[s] = c['psi_e'][s].data_ptr()
+                d.cpml[a].psi_h[s] = c['psi_h'][s].data_ptr()
+        d.n_sources = len(model.sources)
+        for k, source in enumerate(model.sources):
+            src = d.source[k]
+            src.axis = 0 if source.axis == 'x' else 1
+            src.line = source.position
+            src.lo, src.hi = source.span
+            src.profile = self.profiles[k].data_ptr()
+            src.wave = self.waves[k].data_ptr()
+            src.wave_len = self.waves[k].numel()
+        d.n_monitors = len(model.monitors)
+        d.sample_interval = model.sample_interval
+        for k, monitor in enumerate(model.monitors):
+            mon = d.monitor[k]
+            mon.normal = 0 if monitor.normal == 'x' else 1
+            mon.position = monitor.position
+            mon.lo, mon.hi = monitor.span
+            mon.n_freq = monitor.n_freqs
+            mon.phase = self.phases[k].data_ptr()
+            mon.n_samples = self.phases[k].shape[0]
+            mon.acc = self.accumulators[k].data_ptr()
+        d.clock = self.clock.data_ptr()
+        d.step_limit = _lib.INT64_MAX
+        return d
+
+    def _rebind(self, create=False):
+        d = self.descriptor()
+        if create:
+            _lib.check(self.lib.yee2d_create(ctypes_byref(d), ctypes_byref(self._ctx)), 'yee2d_create')
+        else:
+            _lib.check(self.lib.yee2d_update(self._ctx, ctypes_byref(d)), 'yee2d_update')
+
+    def materials_changed(self):
+        """Re-upload the id grid and coefficient tables after the model's materials changed."""
+        self._upload_materials()
+        self.profiles = [self.torch.from_numpy(s.profile.copy()).to(self.device) for s in self.model.sources]
+        self._rebind()
+
+    # ---- stepping -------------------------------------------------------------------------------------------------
+    def stream_handle(self):
+        return self.torch.cuda.current_stream(self.device).cuda_stream
+
+    def step(self, n_steps):
+        """Enqueue n_steps steps on the current stream (asynchronous)."""
+        n_steps = int(n_steps)
+        if n_steps <= 0:
+            return
+        if self.model.materials.version != self._materials_version:
+            self.materials_changed()
+        self._ensure_capacity(self.n_step + n_steps)
+        _lib.check(self.lib.yee_step(self._ctx, n_steps, self.stream_handle()), 'yee_step')
+        self.n_step += n_steps
+
+    @property
+    def time(self):
+        return float(self._times[self.n_step]) if self.n_step < self._times.size else float(
+            clock_times(self.model.dt, self.n_step)[-1])
+
+    def synchronize(self):
+        self.torch.cuda.synchronize(self.device)
+
+    def field(self, name):
+        """Host float32 copy of a field component of the current state set."""
+        index = self.model.components.index(name)
+        return self.fields[self.n_step & 1, index].cpu().numpy()
+
+    def set_field(self, name, values):
+        index = self.model.components.index(name)
+        self.fields[self.n_step & 1, index].copy_(self.torch.as_tensor(np.asarray(values, dtype=np.float32)))
+
+    def read_accumulator(self, index):
+        return self.accumulators[index].cpu().numpy()
+
+    def close(self):
+        if self._ctx:
+            self.lib.yee_destroy(self._ctx)
+            self._ctx = _lib.c_ptr()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ctypes_byref(obj):
+    import ctypes
+    return ctypes.byref(obj)

@@ -1,0 +1,167 @@
+"""Simulation: the reference's 2D EM-FDTD scripting API (simulation.py:18-399, physics/em/__init__.py:63-557) on the
+MI355X Yee stepper.
+
+    sim = Simulation(size_x=6e-6, size_y=3e-6, dx=25e-9, npml=15, boundaries='periodic_y')
+    sim.add_dielectric_rectangle(2.75e-6, 0, 3.25e-6, 3e-6, eps_r=2.25)
+    source = sim.add_line_source(axis='y', position=1.2e-6, wavelength=600e-9, bandwidth=0.8)
+    monitor = sim.add_flux_monitor(axis='y', position=4.5e-6, span=(0, 3e-6), wavelengths=[500e-9, 600e-9])
+    sim.run(duration=80e-15)
+    spectrum = monitor.flux()
+
+Stepping runs on the GPU only (libyee.so through ctypes); the host keeps the float64 clock, the calibration and the
+flux readout, as the reference's host code does.
+"""
+import time as _time
+
+import numpy as np
+
+from .constants import DTYPE
+from .fdtd2d import Engine2D, Model2D, steps_for_duration
+
+
+class Simulation:
+    """2D E_z / H_z FDTD model with the reference's method names; attributes of the model fall through."""
+
+    def __init__(self, size_x: float = None, size_y: float = None, dx: float = None, npml: int = 15,
+                 courant: float = None, boundaries=None, *, polarization: str = 'ez', device=None,
+                 rows_per_tile: int = 64, graph_steps: int = 0):
+        if size_x is None or size_y is None or dx is None:
+            raise ValueError('Simulation() needs (size_x, size_y, dx, ...) for the 2D EM-FDTD path')
+        self.model = Model2D(size_x, size_y, dx, npml=npml, courant=courant, boundaries=boundaries,
+                             polarization=polarization)
+        self.domain = self.model.domain
+        self.boundary_config = self.model.bc
+        self.polarization = polarization
+        self.engine = None
+        self._engine_options = dict(device=device, rows_per_tile=rows_per_tile, graph_steps=graph_steps)
+        self._materials_version = None
+        self.callbacks = []
+
+    # ---- accessors ------------------------------------------------------------------------------------------------
+    @property
+    def grid(self):
+        return self.domain.grid
+
+    @property
+    def sources(self):
+        return self.model.sources
+
+    @property
+    def monitors(self):
+        return self.model.monitors
+
+    @property
+    def n_step(self):
+        return 0 if self.engine is None else self.engine.n_step
+
+    @property
+    def time(self):
+        return 0.0 if self.engine is None else self.engine.time
+
+    @property
+    def t(self):
+        return self.time
+
+    @property
+    def tick(self):
+        return self.n_step
+
+    # ---- model building -------------------------------------------------------------------------------------------
+    def add_dielectric_rectangle(self, x0, y0, x1, y1, eps_r, sigma_e=0.0):
+        self.model.add_dielectric_rectangle(x0, y0, x1, y1, eps_r, sigma_e)
+
+    def add_dielectric_circle(self, cx, cy, radius, eps_r, sigma_e=0.0):
+        self.model.add_dielectric_circle(cx, cy, radius, eps_r, sigma_e)
+
+    def add_line_source(self, axis, position, wavelength=None, frequency=None, bandwidth=0.5, amplitude=1.0,
+                        waveform='gaussian', span=None):
+        self._require_unbuilt('add_line_source')
+        return self.model.add_line_source(axis, position, wavelength, frequency, bandwidth, amplitude, waveform, span)
+
+    def add_gaussian_beam(self, axis, position, waist, center, wavelength=None, frequency=None, bandwidth=0.5,
+                          amplitude=1.0, waveform='gaussian', span=None):
+        """Line source with transverse factor exp(-((s - center)/waist)^2) (physics/em/__init__.py:450-457)."""
+        self._require_unbuilt('add_gaussian_beam')
+        if not waist > 0:
+            raise ValueError(f'Gaussian beam waist must be > 0, got {waist}')
+        return self.model.add_line_source(axis, position, wavelength, frequency, bandwidth, amplitude, waveform, span,
+                                          waist=waist, center=center)
+
+    def add_flux_monitor(self, axis, position, span, wavelengths=None, frequencies=None):
+        self._require_unbuilt('add_flux_monitor')
+        return self.model.add_flux_monitor(axis, position, span, wavelengths, frequencies)
+
+    def _require_unbuilt(self, what):
+        if self.engine is not None:
+            raise ValueError(f'{what}: add sources and monitors before the first run()')
+
+    # ---- running --------------------------------------------------------------------------------------------------
+    def finalize(self):
+        """Courant check, calibration, DFT interval and (first call) the device engine."""
+        model = self.model
+        if self.engine is None:
+            model.finalize()
+            self.engine = Engine2D(model, **self._engine_options)
+        elif model.materials.version != self._materials_version:
+            model.check_courant()
+            model.calibrate()
+            self.engine.materials_changed()
+        self._materials_version = model.materials.version
+        return {'em': []}
+
+    def run(self, duration: float = None, n_steps: int = None, print_every: int = 500, flush: bool = True):
+        """Advance by `duration` [s] (stopping once t >= t_end - dt/2) or exactly `n_steps` steps
+        (simulation.py:272-327, coupling/runner.py:174-204)."""
+        if (duration is None) == (n_steps is None):
+            raise ValueError('Simulation.run needs exactly one of duration [s] or n_steps')
+        if n_steps is not None and int(n_steps) < 0:
+            raise ValueError(f'n_steps must be >= 0, got {n_steps}')
+        if duration is not None and not float(duration) >= 0:
+            raise ValueError(f'duration must be >= 0, got {duration}')
+        self.finalize()
+        engine = self.engine
+        count = int(n_steps) if n_steps is not None else steps_for_duration(engine.time, self.model.dt, duration)
+        if print_every > 0:
+            span = f'n_steps={count}' if n_steps is not None else f'duration={float(duration):.4e} s'
+            print(f'\nOpen_sol (MI355X): physics=[em], dt_sync={float(self.model.dt):.4e} s, {span}')
+        wall_start = _time.perf_counter()
+        if self.callbacks:
+            for _ in range(count):
+                engine.step(1)
+                for callback in self.callbacks:
+                    callback(self)
+        else:
+            engine.step(count)
+        if flush:
+            engine.synchronize()
+        if print_every > 0:
+            engine.synchronize()
+            print(f'Done: {count} ticks in {_time.perf_counter() - wall_start:.2f} s (t = {engine.time:.4e} s)')
+        return count
+
+    # ---- readout --------------------------------------------------------------------------------------------------
+    def get_field_snapshot(self, component=None):
+        """float32 field over the physical window (viz/visualization.py:617-620, renderer.py:78-99)."""
+        if self.engine is None:
+            self.finalize()
+        component = component or self.model.components[0]
+        if component in ('eps_r', 'eps_xx', 'eps_yy', 'sigma_e', 'mu_r', 'sigma_m'):
+            column = {'eps_r': 0, 'eps_xx': 0, 'eps_yy': 1, 'sigma_e': 3, 'mu_r': 4, 'sigma_m': 5}[component]
+            return self.model.materials.property_array(column)[self.domain.physical_window()]
+        if component not in self.model.components:
+            raise KeyError(f"no field '{component}'; available: {self.model.components}")
+        return np.asarray(self.engine.field(component)[self.domain.physical_window()], dtype=DTYPE)
+
+    def get_eps_r_snapshot(self):
+        return self.get_field_snapshot('eps_r' if self.polarization == 'ez' else 'eps_xx')
+
+    def state(self):
+        """Host copies of the current fields."""
+        if self.engine is None:
+            self.finalize()
+        return {name: self.engine.field(name) for name in self.model.components}
+
+    def close(self):
+        if self.engine is not None:
+            self.engine.close()
+            self.engine = None

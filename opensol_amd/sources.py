@@ -1,0 +1,94 @@
+"""Calibrated sheet (line) sources and their per-step waveform tables.
+
+A line source adds float32(amplitude * s(t + dt/2)) * profile to its field line after the E update of every step
+(core/sources.py:146-155, 190-229; physics/em/sources.py:469-480 for the H_z in-plane sheet). The profile is the
+reference's sheet calibration (physics/em/sources.py:383-436, 562-602): cb of the line cells times the local admittance
+|sqrt(eps/mu) cos(kappa0/2)| times 2 shape / (eta0 w_perp), rounded to float32. The kernel reads the per-step value
+from a device table built here from the same float64 clock the reference accumulates.
+"""
+import math
+
+import numpy as np
+
+from .constants import C0, DTYPE, EPS0, ETA0, MU0
+
+
+def resolve_frequency(wavelength, frequency) -> float:
+    """Source frequency [Hz] from exactly one of wavelength [m] / frequency [Hz] (physics/em/__init__.py:560-568)."""
+    if (wavelength is None) == (frequency is None):
+        raise ValueError(f'pass exactly one of wavelength or frequency, got wavelength={wavelength}, '
+                         f'frequency={frequency}')
+    value = float(frequency if frequency is not None else wavelength)
+    if not (np.isfinite(value) and value > 0):
+        raise ValueError(f'source wavelength / frequency must be finite and > 0, got {value}')
+    return value if frequency is not None else C0 / value
+
+
+def resolve_frequencies(wavelengths, frequencies):
+    """Monitor frequencies [Hz] (1D float64) from exactly one of wavelengths / frequencies
+    (physics/em/__init__.py:571-577)."""
+    if (wavelengths is None) == (frequencies is None):
+        raise ValueError('pass exactly one of wavelengths or frequencies')
+    values = np.atleast_1d(np.array(frequencies if frequencies is not None else wavelengths, dtype=np.float64))
+    if values.ndim != 1 or values.size == 0 or not np.all(np.isfinite(values) & (values > 0)):
+        raise ValueError(f'monitor {"frequencies" if frequencies is not None else "wavelengths"} must be a non-empty '
+                         f'1D list of finite positive values, got {values}')
+    return values if frequencies is not None else C0 / values
+
+
+def sheet_admittance(line_eps, mu_r, sigma_m, omega, normal_spacing, dt):
+    """float32 |sqrt(eps/mu) sqrt(1 - eps mu (w sin(omega dt/2) / (C0 dt))^2)| of the line cells
+    (physics/em/sources.py:430-436, physics/em/yee_grid.py:58-65)."""
+    permeability = np.asarray(mu_r, dtype=np.complex128) - 1j * np.asarray(sigma_m, dtype=np.float64) / (omega * MU0)
+    half_phase = float(normal_spacing) * math.sin(0.5 * float(omega) * float(dt)) / (C0 * float(dt))
+    launch = np.sqrt(1.0 - np.asarray(line_eps * permeability, dtype=np.complex128) * half_phase ** 2)
+    return np.abs(np.sqrt(line_eps / permeability) * launch).astype(DTYPE)
+
+
+class LineSource:
+    """Sheet current on one grid line: axis 'x' = line along x at y-cell `position` (axis 'y': along y at x-cell).
+    component: 'ez' (E_z sheet through the line-cell centres) or 'ex' / 'ey' (H_z: hi faces of the line cells)."""
+
+    def __init__(self, axis, position, temporal, amplitude, span, normal_width, component, transverse_shape=None):
+        if axis not in ('x', 'y'):
+            raise ValueError(f"LineSource axis must be 'x' or 'y', got {axis!r}")
+        self.axis = axis
+        self.position = int(position)
+        self.temporal = temporal
+        self.amplitude = DTYPE(amplitude)
+        self.span = (int(span[0]), int(span[1]))
+        if not 0 <= self.span[0] < self.span[1]:
+            raise ValueError(f'LineSource span must be a half-open cell range lo < hi, got {span}')
+        self.normal_width = float(normal_width)
+        self.component = component
+        n_line = self.span[1] - self.span[0]
+        shape = np.ones(n_line) if transverse_shape is None else np.asarray(transverse_shape, dtype=np.float64)
+        self.weight = (2.0 * shape / (ETA0 * self.normal_width)).astype(DTYPE)
+        self.profile = np.zeros(n_line, dtype=DTYPE)
+
+    @property
+    def n_cells(self):
+        return self.span[1] - self.span[0]
+
+    def value(self, t):
+        """float32(amplitude * s(t)) in float32 arithmetic (core/sources.py:221-229)."""
+        return DTYPE(self.amplitude * DTYPE(self.temporal(float(t))))
+
+    def wave_table(self, half_step_times):
+        """float32 value per step from the times t_n + dt/2."""
+        return np.array([self.value(t) for t in half_step_times], dtype=DTYPE)
+
+    def line(self, array):
+        lo, hi = self.span
+        return array[lo:hi, self.position] if self.axis == 'x' else array[self.position, lo:hi]
+
+    def calibrate(self, cb_line, line_eps, mu_line, sigma_m_line, dt):
+        """profile = float32(cb * admittance(omega0) * weight) (physics/em/sources.py:582-602)."""
+        omega = 2.0 * np.pi * float(self.temporal.f0)
+        admittance = sheet_admittance(line_eps, mu_line, sigma_m_line, omega, self.normal_width, dt)
+        self.profile[...] = np.asarray(cb_line, dtype=np.float64) * admittance * self.weight
+
+
+def complex_permittivity(eps_rel, sigma_e, omega):
+    """eps_rel - j sigma_e / (omega eps0) (physics/em/dispersion.py:10-21, no poles)."""
+    return np.asarray(eps_rel, dtype=np.complex128) - 1j * np.asarray(sigma_e, dtype=np.float64) / (omega * EPS0)

@@ -1,0 +1,669 @@
+"""ORACLE (test infrastructure only): numpy restatement of the reference's 2D FDTD hot path.
+
+E_z (E_z, H_x, H_y) and H_z (H_z, E_x, E_y) Yee steppers with CPML psi layers, periodic wraps, calibrated sheet
+(line) sources and running-DFT flux monitors, with the reference's geometry, clock and quantisation points:
+
+  * dt and the kernel cell width are float32 values (core/grid.py:9-17, physics/em/yee_grid.py:11-16);
+  * material arrays are float32 (core/material_grid.py:38-46), rectangle masks compare float32 cell centres
+    (core/domain.py:90-97, 255-260);
+  * the sheet weight and calibrated profile are float32 (physics/em/__init__.py:459-473, em/sources.py:383-392);
+  * the injected value is float32(amplitude * s(t + dt/2)) times the float32 profile (core/sources.py:146-155,
+    221-229);
+  * DFT phases are complex64 by default (em/monitors.py:22-34), spectra complex128 (em/monitors.py:86-90).
+
+dtype=np.float64 restates the reference's precision='double' stepper (physics/em/__init__.py:60-86); dtype=np.float32
+runs the same array passes as the reference's numpy twins in single precision (the CPU baseline). Everything else is
+float64 host arithmetic.
+"""
+import math
+
+import numpy as np
+
+# core/constants.py:6-9
+C0 = 299792458.0
+EPS0 = 8.8541878128e-12
+MU0 = 1.2566370614e-6
+ETA0 = float(np.sqrt(MU0 / EPS0))
+F32 = np.float32
+FACES = ('x_lo', 'x_hi', 'y_lo', 'y_hi')
+
+
+# ---------------------------------------------------------------------------------------------------------- grid
+
+def courant_time_step(dx_min, dy_min, courant=None):
+    """float32 dt = courant / (C0 sqrt(1/dx^2 + 1/dy^2)) (core/grid.py:9-17)."""
+    courant = 0.99 if courant is None else float(courant)
+    if not 0.0 < courant <= 1.0:
+        raise ValueError(f'courant must be in (0, 1], got {courant}')
+    return F32(courant / (C0 * np.sqrt(1.0 / float(dx_min) ** 2 + 1.0 / float(dy_min) ** 2)))
+
+
+def uniform_cell_count(length, dx):
+    """ceil(length / dx - 1e-9) (core/grid.py:20-22)."""
+    return int(np.ceil(length / dx - 1e-9))
+
+
+def containing_cell(coord, edges, first, last):
+    """Cell with edges[i] <= coord < edges[i+1] after a 1e-9-cell snap, clamped (core/domain.py:275-282)."""
+    coord = float(coord)
+    snap = 1e-9 * (edges[first + 1] - edges[first])
+    index = int(np.searchsorted(edges, coord + snap, side='right')) - 1
+    return min(max(index, first), last)
+
+
+class Boundaries:
+    """Face types and CPML settings (physics/em/boundaries.py:194-356): cpml faces are padded with npml cells."""
+
+    def __init__(self, faces, npml=15, order=3, sigma_factor=1.0, kappa_max=15.0, alpha_max=0.0):
+        self.faces = dict(faces)
+        for face, kind in self.faces.items():
+            if kind not in ('cpml', 'periodic'):
+                raise ValueError(f'oracle supports cpml and periodic faces, got {face}={kind}')
+        for axis in 'xy':
+            if (self.faces[f'{axis}_lo'] == 'periodic') != (self.faces[f'{axis}_hi'] == 'periodic'):
+                raise ValueError(f'{axis}: periodic faces must be paired')
+        self.npml, self.order, self.sigma_factor = int(npml), int(order), float(sigma_factor)
+        self.kappa_max, self.alpha_max = float(kappa_max), float(alpha_max)
+
+    @classmethod
+    def preset(cls, name, npml):
+        """Simulation boundaries presets (simulation.py:388-400)."""
+        presets = {None: ('cpml',) * 4, 'pml': ('cpml',) * 4,
+                   'periodic_y': ('cpml', 'cpml', 'periodic', 'periodic'),
+                   'periodic_x': ('periodic', 'periodic', 'cpml', 'cpml'),
+                   'periodic': ('periodic',) * 4}
+        if name not in presets:
+            raise ValueError(f'unknown boundary preset {name!r}')
+        return cls(dict(zip(FACES, presets[name])), npml=npml)
+
+    def cpml(self, face):
+        return self.faces[face] == 'cpml'
+
+    def periodic(self, axis):
+        return self.faces[f'{axis}_lo'] == 'periodic'
+
+    def npml_sides(self):
+        return {face: (self.npml if self.cpml(face) else 0) for face in FACES}
+
+
+class Geometry2D:
+    """Uniform window + CPML padding (core/domain.py:180-196): float64 widths/edges/centres, float32 dt and dx."""
+
+    def __init__(self, size_x, size_y, dx, boundaries, courant=None):
+        sides = boundaries.npml_sides()
+        self.sides = sides
+        self.size_x, self.size_y, self.dx = float(size_x), float(size_y), float(dx)
+        self.nx = uniform_cell_count(size_x + (sides['x_lo'] + sides['x_hi']) * dx, dx)
+        self.ny = uniform_cell_count(size_y + (sides['y_lo'] + sides['y_hi']) * dx, dx)
+        self.dx32 = F32(dx)
+        self.dt = courant_time_step(dx, dx, courant)
+        self.courant = 0.99 if courant is None else float(courant)
+        self.origins = (sides['x_lo'], sides['y_lo'])
+        self.shape = (self.nx, self.ny)
+        self.windows = (slice(sides['x_lo'], self.nx - sides['x_hi']), slice(sides['y_lo'], self.ny - sides['y_hi']))
+        # float64 geometry (core/domain.py:54-81, core/grid.py:25-31)
+        self.widths = tuple(np.full(n, self.dx) for n in self.shape)
+        self.edges = tuple((np.arange(n + 1) - o) * self.dx for n, o in zip(self.shape, self.origins))
+        self.centres = tuple(0.5 * (e[:-1] + e[1:]) for e in self.edges)
+
+    def validate(self, x, y, label):
+        """core/domain.py:126-133."""
+        tol = 1e-9 * max(self.size_x, self.size_y)
+        if not (-tol <= x <= self.size_x + tol and -tol <= y <= self.size_y + tol):
+            raise ValueError(f'{label} position ({x}, {y}) outside the physical domain')
+
+    def cell(self, axis, coord):
+        """core/domain.py:120-124."""
+        window = self.windows[axis]
+        return containing_cell(coord, self.edges[axis], window.start, window.stop - 1)
+
+    def rect_mask(self, x0, y0, x1, y1):
+        """Cells whose float32 centre lies in [x_min, x_max) x [y_min, y_max) (core/domain.py:255-260)."""
+        cx, cy = (c.astype(F32) for c in self.centres)
+        x_min, x_max = F32(min(x0, x1)), F32(max(x0, x1))
+        y_min, y_max = F32(min(y0, y1)), F32(max(y0, y1))
+        mx = (cx >= x_min) & (cx < x_max)
+        my = (cy >= y_min) & (cy < y_max)
+        return mx[:, None] & my[None, :]
+
+
+# ----------------------------------------------------------------------------------------------------------- CPML
+
+def cpml_side_rows(side, n, npml):
+    """(E rows, H rows) of one side (physics/em/boundaries.py:570-576)."""
+    if side == 'lo':
+        return slice(1, npml + 1), slice(0, npml)
+    return slice(n - npml - 1, n - 1), slice(n - npml - 1, n - 1)
+
+
+def cpml_axis_profiles(bc, axis, n, pml_dx):
+    """float64 (sigma, kappa, alpha) at E rows (centres) and H rows (hi faces) (physics/em/boundaries.py:551-631)."""
+    lo, hi = bc.cpml(f'{axis}_lo'), bc.cpml(f'{axis}_hi')
+    npml = bc.npml
+    sigma_e, kappa_e, alpha_e = np.zeros(n), np.ones(n), np.zeros(n)
+    sigma_h, kappa_h, alpha_h = np.zeros(n), np.ones(n), np.zeros(n)
+    if not (lo or hi):
+        return (sigma_e, kappa_e, alpha_e), (sigma_h, kappa_h, alpha_h)
+    if n < (int(lo) + int(hi)) * npml + 2:
+        raise ValueError(f'{axis} axis: {n} cells cannot hold the CPML layers')
+    sigma_max = bc.sigma_factor * (bc.order + 1) / (150.0 * np.pi * float(pml_dx))
+
+    def graded(depth):
+        return (sigma_max * depth ** bc.order, 1.0 + (bc.kappa_max - 1.0) * depth ** bc.order,
+                bc.alpha_max * (1.0 - depth))
+
+    if lo:
+        e_rows, h_rows = cpml_side_rows('lo', n, npml)
+        sigma_e[e_rows], kappa_e[e_rows], alpha_e[e_rows] = graded(np.arange(npml - 1, -1, -1.0) / npml)
+        sigma_h[h_rows], kappa_h[h_rows], alpha_h[h_rows] = graded((np.arange(npml, 0, -1.0) - 0.5) / npml)
+    if hi:
+        e_rows, h_rows = cpml_side_rows('hi', n, npml)
+        sigma_e[e_rows], kappa_e[e_rows], alpha_e[e_rows] = graded(np.arange(0, npml, 1.0) / npml)
+        sigma_h[h_rows], kappa_h[h_rows], alpha_h[h_rows] = graded((np.arange(1, npml + 1, 1.0) - 0.5) / npml)
+    return (sigma_e, kappa_e, alpha_e), (sigma_h, kappa_h, alpha_h)
+
+
+def cpml_recursion(sigma, kappa, alpha, dt, dtype):
+    """(a, b): b = exp(-(sigma/kappa + alpha) dt/eps0), a = sigma (b - 1) / (sigma kappa + kappa^2 alpha)
+    (physics/em/boundaries.py:560-567)."""
+    b = np.exp(-(sigma / kappa + alpha) * float(dt) / EPS0).astype(dtype)
+    denom = sigma * kappa + kappa ** 2 * alpha
+    a = np.divide(sigma * (b - 1.0), denom, out=np.zeros_like(denom), where=denom > 1e-30).astype(dtype)
+    return a, b
+
+
+class _Side:
+    """One active CPML side (physics/em/boundaries.py:671-730): per row kind its b, a, bulk inverse spacing, rows and
+    psi slab."""
+
+    def __init__(self, axis, side, sign, profiles, inv_bulk, dt, n, npml, other_n, dtype):
+        self.axis, self.side, self.sign = axis, side, dtype(sign)
+        self.rows = {}
+        e_rows, h_rows = cpml_side_rows(side, n, npml)
+        for kind, rows, profile in (('e', e_rows, profiles[0]), ('h', h_rows, profiles[1])):
+            sigma, kappa, alpha = (p[rows] for p in profile)
+            a, b = cpml_recursion(sigma, kappa, alpha, dt, dtype)
+            inv = np.asarray(inv_bulk[kind][rows], dtype=dtype)
+            shape = (npml, other_n) if axis == 'x' else (other_n, npml)
+            bcast = (lambda v: v.reshape(-1, 1)) if axis == 'x' else (lambda v: v.reshape(1, -1))
+            self.rows[kind] = dict(rows=rows, a=bcast(a), b=bcast(b), inv=bcast(inv), kappa=kappa.astype(dtype),
+                                   psi=np.zeros(shape, dtype=dtype))
+
+    def view(self, array, rows):
+        return array[rows, :] if self.axis == 'x' else array[:, rows]
+
+    def apply(self, kind, source, coefficient, field):
+        """psi = b psi + a (f_hi - f_lo) inv; field += sign coefficient psi on the rows of `kind`
+        (physics/em/boundaries.py:27-37, 732-750): 'e' rows difference r - (r-1), 'h' rows (r+1) - r."""
+        rs = self.rows[kind]
+        rows = rs['rows']
+        shift = -1 if kind == 'e' else 1
+        other = slice(rows.start + shift, rows.stop + shift)
+        upper, lower = (rows, other) if kind == 'e' else (other, rows)
+        psi = rs['psi']
+        psi *= rs['b']
+        psi += rs['a'] * (self.view(source, upper) - self.view(source, lower)) * rs['inv']
+        target = self.view(field, rows)
+        target += self.sign * self.view(coefficient, rows) * psi
+
+
+# -------------------------------------------------------------------------------------------------------- sources
+
+class GaussianPulse:
+    """exp(-((t-t0)/tau)^2) sin(2 pi f0 (t-t0)), tau = 1/(bw f0 pi), t0 = 4.5 tau (core/sources.py:23-53)."""
+
+    def __init__(self, f0, bandwidth=0.5, t0_factor=4.5):
+        self.f0, self.bandwidth = float(f0), float(bandwidth)
+        self.tau = 1.0 / (self.bandwidth * self.f0 * np.pi)
+        self.t0 = t0_factor * self.tau
+
+    def __call__(self, t):
+        delay = float(t) - self.t0
+        return math.exp(-(delay / self.tau) ** 2) * math.sin(2.0 * math.pi * self.f0 * delay)
+
+    def spectral_limit(self):
+        return self.f0 * (1.0 + 3.72 * self.bandwidth)
+
+
+class ContinuousWave:
+    """(1 - exp(-t/tau_ramp)) sin(2 pi f0 t), tau_ramp = 3/f0 (core/sources.py:56-78)."""
+
+    def __init__(self, f0, ramp_periods=3.0):
+        self.f0 = float(f0)
+        self.tau_ramp = ramp_periods / self.f0
+
+    def __call__(self, t):
+        t = float(t)
+        return (1.0 - math.exp(-t / self.tau_ramp)) * math.sin(2.0 * math.pi * self.f0 * t)
+
+    def spectral_limit(self):
+        return None
+
+
+class RickerWavelet:
+    """(1 - 2 a^2) exp(-a^2), a = pi f (t - t0), t0 = 4/f (core/sources.py:81-100)."""
+
+    def __init__(self, f_peak, t0_factor=4.0):
+        self.f_peak = float(f_peak)
+        self.f0 = self.f_peak
+        self.t0 = t0_factor / self.f_peak
+
+    def __call__(self, t):
+        arg = math.pi * self.f_peak * (float(t) - self.t0)
+        return (1.0 - 2.0 * arg * arg) * math.exp(-arg * arg)
+
+    def spectral_limit(self):
+        return 4.2 * self.f_peak
+
+
+def make_temporal(waveform, frequency, bandwidth):
+    """physics/em/__init__.py:580-587."""
+    if waveform == 'gaussian':
+        return GaussianPulse(frequency, bandwidth)
+    if waveform == 'cw':
+        return ContinuousWave(frequency)
+    if waveform == 'ricker':
+        return RickerWavelet(frequency)
+    raise ValueError(f'unknown waveform {waveform!r}')
+
+
+def resolve_frequency(wavelength, frequency):
+    """physics/em/__init__.py:560-568."""
+    if (wavelength is None) == (frequency is None):
+        raise ValueError('pass exactly one of wavelength or frequency')
+    value = float(frequency if frequency is not None else wavelength)
+    return value if frequency is not None else C0 / value
+
+
+def sheet_admittance(line_eps, mu_r, sigma_m, omega, normal_spacing, dt):
+    """float32 |sqrt(eps/mu) sqrt(1 - eps mu (w sin(omega dt/2)/(C0 dt))^2)| (em/sources.py:430-436,
+    physics/em/yee_grid.py:58-65)."""
+    permeability = mu_r.astype(np.complex128) - 1j * sigma_m.astype(np.float64) / (omega * MU0)
+    half_phase = float(normal_spacing) * math.sin(0.5 * omega * float(dt)) / (C0 * float(dt))
+    launch = np.sqrt(1.0 - np.asarray(line_eps * permeability, dtype=np.complex128) * half_phase ** 2)
+    return np.abs(np.sqrt(line_eps / permeability) * launch).astype(F32)
+
+
+class LineSource:
+    """Calibrated sheet current (physics/em/__init__.py:442-473; core/sources.py:190-229)."""
+
+    def __init__(self, axis, line_cell, span_cells, temporal, amplitude, normal_width, component):
+        self.axis, self.position, self.span = axis, int(line_cell), span_cells
+        self.temporal, self.amplitude = temporal, F32(amplitude)
+        self.normal_width = float(normal_width)
+        self.component = component
+        n_line = span_cells[1] - span_cells[0]
+        self.weight = np.full(n_line, 2.0 / (ETA0 * self.normal_width)).astype(F32)
+        self.profile = np.zeros(n_line, dtype=F32)
+
+    def line(self, array):
+        lo, hi = self.span
+        return array[lo:hi, self.position] if self.axis == 'x' else array[self.position, lo:hi]
+
+    def value(self, t):
+        """float32(amplitude * s(t)) with float32 arithmetic (core/sources.py:221-229)."""
+        return F32(self.amplitude * F32(self.temporal(t)))
+
+    def inject(self, fields, t):
+        value = self.value(t)
+        if value == 0:
+            return
+        target = self.line(fields[self.component])
+        target += value * self.profile
+
+
+class FluxMonitor:
+    """Flux line with complex128 E and H spectra phased at the E sample time (em/monitors.py:203-245, 325-379)."""
+
+    def __init__(self, normal, position, span, frequencies, dt, widths, polarization, phase_dtype):
+        self.normal, self.position, self.span = normal, int(position), (int(span[0]), int(span[1]))
+        self.frequencies = np.asarray(frequencies, dtype=np.float64)
+        self.dt = float(dt)
+        if polarization == 'ez':
+            self.e_component, self.h_component = 'ez', ('hy' if normal == 'x' else 'hx')
+            self.sign = -1.0 if normal == 'x' else 1.0
+        else:
+            self.e_component, self.h_component = ('ey' if normal == 'x' else 'ex'), 'hz'
+            self.sign = 1.0 if normal == 'x' else -1.0
+        n = self.span[1] - self.span[0]
+        self.e_spectrum = np.zeros((self.frequencies.size, n), dtype=np.complex128)
+        self.h_spectrum = np.zeros((self.frequencies.size, n), dtype=np.complex128)
+        self.widths = np.asarray(widths, dtype=np.float64)[self.span[0]:self.span[1]].copy()
+        self.omega = 2.0 * np.pi * self.frequencies
+        self.phase_dtype = phase_dtype
+        self.sample_interval = 1
+
+    def line(self, array):
+        lo, hi = self.span
+        return array[self.position, lo:hi] if self.normal == 'x' else array[lo:hi, self.position]
+
+    def phases(self, t):
+        """exp(-j omega t) m dt rounded to phase_dtype (em/monitors.py:22-34)."""
+        p = -self.omega * float(t)
+        scale = self.dt * self.sample_interval
+        if self.phase_dtype == np.complex64:
+            return ((np.cos(p) * scale).astype(F32) + 1j * (np.sin(p) * scale).astype(F32)).astype(np.complex128)
+        return np.cos(p) * scale + 1j * (np.sin(p) * scale)
+
+    def update(self, fields, t):
+        ph = self.phases(t)[:, None]
+        self.e_spectrum += ph * self.line(fields[self.e_component]).astype(np.float64)
+        self.h_spectrum += ph * self.line(fields[self.h_component]).astype(np.float64)
+
+    def flux(self):
+        """sign cos(pi f dt) Re(E conj(H e^{j pi f dt})) summed with the cell widths (em/monitors.py:371-379)."""
+        h = self.h_spectrum * np.exp(1j * np.pi * self.frequencies * self.dt)[:, None]
+        integrand = self.sign * np.cos(np.pi * self.frequencies * self.dt)[:, None] * np.real(
+            self.e_spectrum * np.conj(h))
+        return integrand @ self.widths
+
+
+def dft_sample_interval(dt, sources, monitor_frequencies):
+    """m = max(1, floor(1 / ((max source limit + max monitor frequency) dt))) (em/monitors.py:177-191)."""
+    if not sources or not monitor_frequencies:
+        return 1
+    limits = []
+    for source in sources:
+        limit = source.temporal.spectral_limit()
+        if limit is None:
+            return 1
+        limits.append(float(limit))
+    rate = max(limits) + max(float(np.max(f)) for f in monitor_frequencies)
+    return max(1, int(np.floor(1.0 / (rate * float(dt)))))
+
+
+# --------------------------------------------------------------------------------------------------------- stepper
+
+class Yee2D:
+    """2D FDTD model with the reference's Simulation/EMPhysics vocabulary (simulation.py:21-32,
+    physics/em/__init__.py:317-518), stepped on the CPU in `dtype`."""
+
+    def __init__(self, size_x, size_y, dx, npml=15, courant=None, boundaries=None, polarization='ez',
+                 dtype=np.float64, phase_dtype=np.complex64):
+        if polarization not in ('ez', 'hz'):
+            raise ValueError(f"polarization must be 'ez' or 'hz', got {polarization!r}")
+        self.bc = boundaries if isinstance(boundaries, Boundaries) else Boundaries.preset(boundaries, npml)
+        self.geo = geo = Geometry2D(size_x, size_y, dx, self.bc, courant)
+        self.polarization = polarization
+        self.dtype = np.dtype(dtype).type
+        self.phase_dtype = phase_dtype
+        self.nx, self.ny, self.dt = geo.nx, geo.ny, geo.dt
+        shape = geo.shape
+        names = ('ez', 'hx', 'hy') if polarization == 'ez' else ('hz', 'ex', 'ey')
+        self.fields = {name: np.zeros(shape, dtype=self.dtype) for name in names}
+        if polarization == 'ez':
+            self.materials = {'eps_r': np.ones(shape, F32), 'sigma_e': np.zeros(shape, F32)}
+        else:
+            self.materials = {'eps_xx': np.ones(shape, F32), 'eps_yy': np.ones(shape, F32),
+                              'sigma_e': np.zeros(shape, F32)}
+        self.materials.update(mu_r=np.ones(shape, F32), sigma_m=np.zeros(shape, F32))
+        self.sources, self.monitors = [], []
+        self.n_step, self.time = 0, 0.0
+        self._finalized = False
+        self._build_spacings_and_cpml()
+
+    # ---- setup ------------------------------------------------------------------------------------------------
+    def _build_spacings_and_cpml(self):
+        """Kernel widths = float32 grid.dx (yee_grid.py:11-45); kappa folded into the curl rows
+        (fdtd_2d_base.py:76-84, boundaries.py:778-798); CPML sides (boundaries.py:694-730)."""
+        geo, real, bc = self.geo, self.dtype, self.bc
+        w = float(geo.dx32)
+        inv = {}
+        for axis, n in (('x', geo.nx), ('y', geo.ny)):
+            widths = np.full(n, w)
+            inv[axis] = {'e': 1.0 / widths, 'h': 1.0 / (0.5 * (widths + np.roll(widths, -1)))}
+        self._inv_bulk = inv
+        plus, minus = 1.0, -1.0
+        x_sign, y_sign = (plus, minus) if self.polarization == 'ez' else (minus, plus)
+        self.sides = []
+        curl = {f'{axis}_{kind}': inv[axis][kind].astype(real) for axis in 'xy' for kind in 'eh'}
+        for face in FACES:
+            if not bc.cpml(face):
+                continue
+            axis, side = face[0], face[2:]
+            n, other = (geo.nx, geo.ny) if axis == 'x' else (geo.ny, geo.nx)
+            profiles = cpml_axis_profiles(bc, axis, n, w)
+            s = _Side(axis, side, x_sign if axis == 'x' else y_sign, profiles, inv[axis], float(geo.dt), n, bc.npml,
+                      other, real)
+            self.sides.append(s)
+            for kind in 'eh':
+                rs = s.rows[kind]
+                curl[f'{axis}_{kind}'][rs['rows']] /= rs['kappa']
+        self.inv_dx_e, self.inv_dx_h = curl['x_e'], curl['x_h']
+        self.inv_dy_e, self.inv_dy_h = curl['y_e'], curl['y_h']
+
+    def add_dielectric_rectangle(self, x0, y0, x1, y1, eps_r, sigma_e=0.0):
+        """physics/em/__init__.py:317-327, core/material_grid.py:56-83 (later writes win)."""
+        mask = self.geo.rect_mask(x0, y0, x1, y1)
+        names = ('eps_r',) if self.polarization == 'ez' else ('eps_xx', 'eps_yy')
+        for name in names:
+            self.materials[name][mask] = F32(eps_r)
+        self.materials['sigma_e'][mask] = F32(sigma_e)
+        self._finalized = False
+
+    def add_line_source(self, axis, position, wavelength=None, frequency=None, bandwidth=0.5, amplitude=1.0,
+                        waveform='gaussian', span=None):
+        """Sheet current of amplitude [V/m] per side (physics/em/__init__.py:442-473, placement.py:223-251)."""
+        geo = self.geo
+        temporal = make_temporal(waveform, resolve_frequency(wavelength, frequency), bandwidth)
+        along_axis = 0 if axis == 'x' else 1
+        if axis == 'x':
+            geo.validate(0.0, position, 'Line source (y)')
+            line_cell = geo.cell(1, position)
+        elif axis == 'y':
+            geo.validate(position, 0.0, 'Line source (x)')
+            line_cell = geo.cell(0, position)
+        else:
+            raise ValueError(f"axis must be 'x' or 'y', got {axis!r}")
+        n_along = geo.shape[along_axis]
+        if span is None:
+            span_cells = (0, n_along)
+        else:
+            start, end = sorted((float(span[0]), float(span[1])))
+            lo, hi = geo.cell(along_axis, start), geo.cell(along_axis, end) + 1
+            window = geo.windows[along_axis]
+            if lo == window.start and self.bc.cpml(f'{axis}_lo'):
+                lo = 0
+            if hi == window.stop and self.bc.cpml(f'{axis}_hi'):
+                hi = n_along
+            span_cells = (lo, hi)
+        if self.polarization == 'ez':
+            normal_width = float(geo.widths[1 - along_axis][line_cell])
+            component = 'ez'
+        else:
+            inv_face = self._inv_bulk['y' if axis == 'x' else 'x']['h']
+            n_normal = geo.shape[1 - along_axis]
+            if line_cell == n_normal - 1 and not self.bc.periodic('y' if axis == 'x' else 'x'):
+                raise ValueError('H_z sheet on the outer grid wall')
+            normal_width = 1.0 / float(inv_face[line_cell])
+            component = 'ex' if axis == 'x' else 'ey'
+        source = LineSource(axis, line_cell, span_cells, temporal, amplitude, normal_width, component)
+        self.sources.append(source)
+        self._finalized = False
+        return source
+
+    def add_flux_monitor(self, axis, position, span, wavelengths=None, frequencies=None):
+        """placement.py:55-70, physics/em/__init__.py:511-518."""
+        geo = self.geo
+        if (wavelengths is None) == (frequencies is None):
+            raise ValueError('pass exactly one of wavelengths or frequencies')
+        freqs = (np.asarray(frequencies, dtype=np.float64) if frequencies is not None
+                 else C0 / np.asarray(wavelengths, dtype=np.float64))
+        lo, hi = sorted((float(span[0]), float(span[1])))
+        if axis == 'x':
+            normal, line_cell, cells = 'y', geo.cell(1, position), (geo.cell(0, lo), geo.cell(0, hi) + 1)
+            widths = geo.widths[0]
+        else:
+            normal, line_cell, cells = 'x', geo.cell(0, position), (geo.cell(1, lo), geo.cell(1, hi) + 1)
+            widths = geo.widths[1]
+        monitor = FluxMonitor(normal, line_cell, cells, np.atleast_1d(freqs), float(geo.dt), widths,
+                              self.polarization, self.phase_dtype)
+        self.monitors.append(monitor)
+        self._finalized = False
+        return monitor
+
+    def finalize(self):
+        """Coefficients (fdtd_ez_2d_kernels.py:310-353, fdtd_hz_2d_kernels.py:190-278), source calibration
+        (em/sources.py:562-602) and the DFT sample interval (physics/em/__init__.py:181-203)."""
+        real, m = self.dtype, self.materials
+        dt, eps0, mu0 = real(self.dt), real(EPS0), real(MU0)
+
+        def coefficients(eps_rel, sigma, permittivity):
+            eps = eps_rel.astype(real) * permittivity
+            k = sigma.astype(real) * dt / (real(2.0) * eps)
+            denom = real(1.0) + k
+            return (real(1.0) - k) / denom, (dt / eps) / denom
+
+        self.da, self.db = coefficients(m['mu_r'], m['sigma_m'], mu0)
+        if self.polarization == 'ez':
+            self.ca, self.cb = coefficients(m['eps_r'], m['sigma_e'], eps0)
+        else:
+            half = real(0.5)
+            pj = np.roll(np.arange(self.ny), -1) if self.bc.periodic('y') else np.minimum(np.arange(self.ny) + 1,
+                                                                                         self.ny - 1)
+            pi = np.roll(np.arange(self.nx), -1) if self.bc.periodic('x') else np.minimum(np.arange(self.nx) + 1,
+                                                                                         self.nx - 1)
+            exx, eyy, sig = (m[name].astype(real) for name in ('eps_xx', 'eps_yy', 'sigma_e'))
+            self.ca_x, self.cb_x = coefficients(exx * half + exx[:, pj] * half, sig * half + sig[:, pj] * half, eps0)
+            self.ca_y, self.cb_y = coefficients(eyy * half + eyy[pi] * half, sig * half + sig[pi] * half, eps0)
+        self._calibrate()
+        if self.n_step == 0:
+            interval = dft_sample_interval(self.dt, self.sources, [mon.frequencies for mon in self.monitors])
+            for monitor in self.monitors:
+                monitor.sample_interval = interval
+        self._finalized = True
+
+    def _calibrate(self):
+        """profile = float32(cb_line admittance weight) (em/sources.py:383-392, 582-602)."""
+        m = self.materials
+        for source in self.sources:
+            omega = 2.0 * np.pi * source.temporal.f0
+            if self.polarization == 'ez':
+                cb_line = source.line(self.cb)
+                eps = source.line(m['eps_r']).astype(np.complex128) - 1j * source.line(m['sigma_e']).astype(
+                    np.float64) / (omega * EPS0)
+            else:
+                cb_face, cell_eps = (self.cb_x, m['eps_xx']) if source.component == 'ex' else (self.cb_y, m['eps_yy'])
+                cb_line = source.line(cb_face)
+                n_normal = self.geo.shape[1 if source.axis == 'x' else 0]
+                lower_row, upper_row = source.position, (source.position + 1) % n_normal
+                lo, hi = source.span
+
+                def row_eps(row):
+                    line = (slice(lo, hi), row) if source.axis == 'x' else (row, slice(lo, hi))
+                    return cell_eps[line].astype(np.complex128) - 1j * m['sigma_e'][line].astype(np.float64) / (
+                        omega * EPS0)
+                w = float(self.geo.dx32)
+                eps = (row_eps(lower_row) * w + row_eps(upper_row) * w) / (w + w)
+            admittance = sheet_admittance(eps, source.line(m['mu_r']), source.line(m['sigma_m']), omega,
+                                          source.normal_width, float(self.dt))
+            source.profile[...] = cb_line * admittance * source.weight
+
+    # ---- stepping ---------------------------------------------------------------------------------------------
+    def step(self):
+        """One Yee step: H, H CPML, E, E CPML, electric sources at t + dt/2, clock, monitors at t + dt
+        (fdtd_ez_2d.py:71-119, fdtd_hz_2d.py:144-189, fdtd_2d_base.py:470-488)."""
+        if not self._finalized:
+            self.finalize()
+        t_start = float(self.time)
+        t_half = t_start + 0.5 * float(self.dt)
+        t_full = t_start + float(self.dt)
+        if self.polarization == 'ez':
+            self._step_ez()
+        else:
+            self._step_hz()
+        for source in self.sources:
+            source.inject(self.fields, t_half)
+        self.n_step += 1
+        self.time = t_full
+        for monitor in self.monitors:
+            if self.n_step % monitor.sample_interval == 0:
+                monitor.update(self.fields, t_full)
+
+    def _step_ez(self):
+        f = self.fields
+        ez, hx, hy = f['ez'], f['hx'], f['hy']
+        da, db, ca, cb = self.da, self.db, self.ca, self.cb
+        px, py = self.bc.periodic('x'), self.bc.periodic('y')
+        # update_h (fdtd_ez_2d_kernels.py:23-60)
+        hx[:, :-1] = da[:, :-1] * hx[:, :-1] - db[:, :-1] * (ez[:, 1:] - ez[:, :-1]) * self.inv_dy_h[:-1]
+        if py:
+            hx[:, -1] = da[:, -1] * hx[:, -1] - db[:, -1] * (ez[:, 0] - ez[:, -1]) * self.inv_dy_h[-1]
+        hy[:-1] = da[:-1] * hy[:-1] + db[:-1] * (ez[1:] - ez[:-1]) * self.inv_dx_h[:-1, None]
+        if px:
+            hy[-1] = da[-1] * hy[-1] + db[-1] * (ez[0] - ez[-1]) * self.inv_dx_h[-1]
+        for side in self.sides:   # apply_h: x sides (ez, db, hy), y sides (ez, db, hx) (boundaries.py:752-756)
+            side.apply('h', ez, db, hy if side.axis == 'x' else hx)
+        # update_e (fdtd_ez_2d_kernels.py:125-140)
+        curl = np.empty_like(ez)
+        curl[1:] = (hy[1:] - hy[:-1]) * self.inv_dx_e[1:, None]
+        curl[0] = ((hy[0] - hy[-1]) if px else hy[0]) * self.inv_dx_e[0]
+        curl[:, 1:] -= (hx[:, 1:] - hx[:, :-1]) * self.inv_dy_e[1:]
+        curl[:, 0] -= ((hx[:, 0] - hx[:, -1]) if py else hx[:, 0]) * self.inv_dy_e[0]
+        ez *= ca
+        curl *= cb
+        ez += curl
+        for side in self.sides:   # apply_e: x sides (hy, cb, ez), y sides (hx, cb, ez) (boundaries.py:758-762)
+            side.apply('e', hy if side.axis == 'x' else hx, cb, ez)
+
+    def _step_hz(self):
+        f = self.fields
+        hz, ex, ey = f['hz'], f['ex'], f['ey']
+        px, py = self.bc.periodic('x'), self.bc.periodic('y')
+        # update_hz (fdtd_hz_2d_kernels.py:65-80)
+        curl = np.empty_like(hz)
+        curl[:, 1:] = (ex[:, 1:] - ex[:, :-1]) * self.inv_dy_e[1:]
+        curl[:, 0] = ((ex[:, 0] - ex[:, -1]) if py else ex[:, 0]) * self.inv_dy_e[0]
+        curl[1:] -= (ey[1:] - ey[:-1]) * self.inv_dx_e[1:, None]
+        curl[0] -= ((ey[0] - ey[-1]) if px else ey[0]) * self.inv_dx_e[0]
+        hz *= self.da
+        curl *= self.db
+        hz += curl
+        for side in self.sides:   # apply_hz: x sides (ey, db, hz), y sides (ex, db, hz) (boundaries.py:766-768)
+            side.apply('e', ey if side.axis == 'x' else ex, self.db, hz)
+        # update_exy (fdtd_hz_2d_kernels.py:146-155)
+        ca_x, cb_x, ca_y, cb_y = self.ca_x, self.cb_x, self.ca_y, self.cb_y
+        ex[:, :-1] = ca_x[:, :-1] * ex[:, :-1] + cb_x[:, :-1] * (hz[:, 1:] - hz[:, :-1]) * self.inv_dy_h[:-1]
+        if py:
+            ex[:, -1] = ca_x[:, -1] * ex[:, -1] + cb_x[:, -1] * (hz[:, 0] - hz[:, -1]) * self.inv_dy_h[-1]
+        ey[:-1] = ca_y[:-1] * ey[:-1] - cb_y[:-1] * (hz[1:] - hz[:-1]) * self.inv_dx_h[:-1, None]
+        if px:
+            ey[-1] = ca_y[-1] * ey[-1] - cb_y[-1] * (hz[0] - hz[-1]) * self.inv_dx_h[-1]
+        for side in self.sides:   # apply_exy: x sides (hz, cb_y, ey), y sides (hz, cb_x, ex) (boundaries.py:770-772)
+            if side.axis == 'x':
+                side.apply('h', hz, cb_y, ey)
+            else:
+                side.apply('h', hz, cb_x, ex)
+
+    def run(self, duration=None, n_steps=None):
+        """Tick loop stopping at t >= t_end - dt/2 (coupling/runner.py:174-204); returns the steps run."""
+        if (duration is None) == (n_steps is None):
+            raise ValueError('pass exactly one of duration or n_steps')
+        if not self._finalized:
+            self.finalize()
+        dt = float(self.dt)
+        if n_steps is not None:
+            for _ in range(int(n_steps)):
+                self.step()
+            return int(n_steps)
+        stop = self.time + float(duration) - 0.5 * dt
+        done = 0
+        while self.time < stop:
+            self.step()
+            done += 1
+        return done
+
+    def field_window(self, name):
+        """Physical-window slice of a field (viz/visualization.py:617-620)."""
+        return self.fields[name][self.geo.windows]
+
+
+def steps_for_duration(dt, duration, t_start=0.0):
+    """Number of ticks the runner takes for `duration` from t_start (coupling/runner.py:182-203)."""
+    dt = float(dt)
+    stop = t_start + float(duration) - 0.5 * dt
+    t, n = float(t_start), 0
+    while t < stop:
+        t = t + dt
+        n += 1
+    return n
