@@ -109,7 +109,8 @@ def main():
     parser.add_argument('--steps', type=int, default=2000)
     parser.add_argument('--warmup', type=int, default=50)
     parser.add_argument('--graph-steps', type=int, default=50)
-    parser.add_argument('--rows-per-tile', type=int, default=64)
+    parser.add_argument('--rows-per-tile', type=int, default=0, help='0: fill the device wave slots')
+    parser.add_argument('--lane-columns', type=int, default=4, choices=(2, 4))
     parser.add_argument('--cpu-steps', type=int, default=8, help='steps of the CPU baseline sample (0: skip)')
     args = parser.parse_args()
     if args.gpus != 1:
@@ -122,9 +123,11 @@ def main():
         print('bench.py needs a GPU', file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
-    sim = build_ez_workload(graph_steps=args.graph_steps, rows_per_tile=args.rows_per_tile)
+    sim = build_ez_workload(graph_steps=args.graph_steps, rows_per_tile=args.rows_per_tile,
+                            lane_columns=args.lane_columns)
     sim.finalize()
     engine = sim.engine
+    rows_per_tile, lane_columns = engine.launch_geometry()
     cells = sim.model.nx * sim.model.ny
     print(f'# workload ready: {sim.model.nx}x{sim.model.ny}, {len(sim.model.materials.materials)} materials',
           file=sys.stderr, flush=True)
@@ -166,7 +169,7 @@ def main():
         'data': 'synthetic (seeded rectangles, Gaussian line source)',
         'config': {'workload': workload, 'grid': [sim.model.nx, sim.model.ny], 'cpml': 20, 'dx_m': 20e-9,
                    'rectangles': 400, 'seed': SEED, 'polarization': 'ez', 'graph_steps': args.graph_steps,
-                   'rows_per_tile': args.rows_per_tile},
+                   'rows_per_tile': rows_per_tile, 'lane_columns': lane_columns},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
                      'bytes_per_cell_update': BYTES_PER_CELL_2D},

@@ -8,7 +8,8 @@
  * State is double-buffered: after n steps the fields and CPML psi live in set n & 1. The absolute step index n of an
  * instance is its device clock (*clock), advanced by the library after each batch of launches.
  *
- * Layouts: 2D arrays are (nx, ny) row-major (y contiguous, index i*ny + j); 3D arrays (nx, ny, nz) with z contiguous.
+ * Layouts: 2D arrays are (nx, pitch) row-major (y contiguous, index i*pitch + j, columns ny..pitch-1 are padding that
+ * stays 0); 3D arrays (nx, ny, nz) with z contiguous.
  * E_z polarisation: E_z at cell centres, H_x on hi-y faces, H_y on hi-x faces. H_z polarisation: H_z at centres,
  * E_x on hi-y faces, E_y on hi-x faces. 3D: E_x/E_y/E_z on the hi-x/y/z faces of cell (i, j, k), H on the edges.
  *
@@ -84,8 +85,8 @@ typedef struct yee_cpml_axis {
     const float* g_e;
     const float* b_h;
     const float* g_h;
-    float* psi_e[2];         /* ping-pong psi of the centre rows */
-    float* psi_h[2];         /* ping-pong psi of the hi-face rows */
+    float* psi_e[2];         /* ping-pong psi of the centre rows: x axis (n_rows, pitch), y axis (nx, n_rows) */
+    float* psi_h[2];         /* ping-pong psi of the hi-face rows, same layouts */
 } yee_cpml_axis;
 
 /*
@@ -98,9 +99,11 @@ typedef struct yee_cpml_axis {
  */
 typedef struct yee2d_desc {
     int32_t nx, ny;
+    int32_t pitch;           /* row stride in elements of every (nx, *) array: a multiple of 4, >= ny */
     int32_t polarization;    /* YEE_EZ or YEE_HZ */
     int32_t periodic_x, periodic_y;
-    int32_t rows_per_tile;   /* x rows marched by one wavefront tile (0: library default) */
+    int32_t rows_per_tile;   /* x rows marched by one wavefront tile (0: chosen from the device's wave slots) */
+    int32_t lane_columns;    /* y columns per lane: 2 or 4 (0: 4); a tile spans 64 * lane_columns columns */
     float* field[2][3];      /* [set][component]: E_z (ez, hx, hy) or H_z (hz, ex, ey) */
     const uint16_t* material;
     const float* table;      /* (n_materials, 4) */
@@ -149,6 +152,9 @@ int32_t yee_set_graph_steps(yee_ctx* ctx, int32_t graph_steps);
 
 /* Replace the descriptor (new buffers or tables); drops cached graphs. */
 int32_t yee2d_update(yee_ctx* ctx, const yee2d_desc* desc);
+
+/* The descriptor a single 2D context steps, with the defaults it resolved (rows_per_tile, lane_columns). */
+int32_t yee2d_get_desc(const yee_ctx* ctx, yee2d_desc* out);
 
 int32_t yee_destroy(yee_ctx* ctx);
 

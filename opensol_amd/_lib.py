@@ -30,9 +30,9 @@ class CpmlAxisC(ctypes.Structure):
 
 
 class Desc2DC(ctypes.Structure):
-    _fields_ = [('nx', c_i32), ('ny', c_i32), ('polarization', c_i32), ('periodic_x', c_i32), ('periodic_y', c_i32),
-                ('rows_per_tile', c_i32), ('field', (c_ptr * 3) * 2), ('material', c_ptr), ('table', c_ptr),
-                ('table2', c_ptr), ('n_materials', c_i32), ('dt', c_f32), ('dt_over_eps0', c_f32), ('pad_', c_i32),
+    _fields_ = [('nx', c_i32), ('ny', c_i32), ('pitch', c_i32), ('polarization', c_i32), ('periodic_x', c_i32),
+                ('periodic_y', c_i32), ('rows_per_tile', c_i32), ('lane_columns', c_i32), ('field', (c_ptr * 3) * 2),
+                ('material', c_ptr), ('table', c_ptr), ('table2', c_ptr), ('n_materials', c_i32), ('dt', c_f32), ('dt_over_eps0', c_f32), ('pad_', c_i32),
                 ('inv_dx_e', c_ptr), ('inv_dx_h', c_ptr), ('inv_dy_e', c_ptr), ('inv_dy_h', c_ptr),
                 ('cpml', CpmlAxisC * 2), ('n_sources', c_i32), ('n_monitors', c_i32), ('sample_interval', c_i32),
                 ('pad2_', c_i32), ('source', LineSourceC * MAX_SOURCES), ('monitor', FluxLineC * MAX_MONITORS),
@@ -48,6 +48,7 @@ SIGNATURES = (
     ('yee_step', c_i32, (c_ptr, c_i64, c_ptr)),
     ('yee_set_graph_steps', c_i32, (c_ptr, c_i32)),
     ('yee2d_update', c_i32, (c_ptr, ctypes.POINTER(Desc2DC))),
+    ('yee2d_get_desc', c_i32, (c_ptr, ctypes.POINTER(Desc2DC))),
     ('yee_destroy', c_i32, (c_ptr,)),
     ('yee2d_batch_bytes', ctypes.c_size_t, (c_i32,)),
     ('yee2d_batch_create', c_i32, (ctypes.POINTER(Desc2DC), c_i32, c_ptr, c_ptr, ctypes.POINTER(c_ptr))),

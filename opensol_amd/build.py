@@ -12,7 +12,8 @@ LIBRARY = os.path.join(PACKAGE, 'libyee.so')
 SOURCES = ('yee2d.hip', 'yee_api.hip')
 ARCH = 'gfx950'
 # Packed FP32 instructions are disabled for every HIP source; hipcc -O3 otherwise forms them from float code.
-FLAGS = ['-O3', '-std=c++17', '-fPIC', '-shared', f'--offload-arch={ARCH}', '-munsafe-fp-atomics',
+# -ffp-contract=on: FMAs only within one expression, so every copy of a cell update rounds identically.
+FLAGS = ['-O3', '-std=c++17', '-fPIC', '-shared', f'--offload-arch={ARCH}', '-munsafe-fp-atomics', '-ffp-contract=on',
          '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops', '-Wall', '-Wno-unused-function']
 
 

@@ -1,17 +1,22 @@
 // Fused 2D Yee step for gfx950: one launch advances E_z (E_z, H_x, H_y) or H_z (H_z, E_x, E_y) instances by one
 // full step (H update, H CPML, E update, E CPML, sheet-source injection, FP64 Kahan DFT flux samples).
 //
-// Work decomposition. A wavefront tile owns OWN = 63 consecutive y columns (one lane per column, y is contiguous in
-// HBM so every row access is one coalesced 256 B request) and marches rows_per_tile rows along x, carrying the
-// previous row in registers (register blocking along the slow axis). The 64th lane is a halo column: E_z tiles place
-// it on the left (H_x of column j0-1 feeds the E update of column j0), H_z tiles on the right (H_z of column j0+63
-// feeds E_x of column j0+62). Halo lanes run the same code as owned lanes but store nothing, so cross-lane neighbours
-// come from wave shuffles and no LDS or barrier is needed. Neighbouring tiles read each other's halo rows/columns,
-// which is race-free because every state array (fields and CPML psi) is double-buffered: step n reads set n & 1 and
-// writes set (n & 1) ^ 1.
+// Work decomposition. A wavefront tile owns 64 * VEC consecutive y columns, VEC per lane (y is contiguous in HBM:
+// rows are padded to a multiple of 4 floats, so every lane access is an aligned 8 or 16 B vector), and marches
+// rows_per_tile rows along x with register blocking along that slow axis. Cross-lane neighbours come from wave
+// shuffles; the one neighbour across the tile edge (H_x of column j0-1 for E_z, H_z of column j0+64*VEC for H_z) is
+// recomputed by the edge lane from direct reads. Neighbouring tiles read each other's edge rows/columns, which is
+// race-free because every state array (fields and CPML psi) is double-buffered: step n reads set n & 1 and writes set
+// (n & 1) ^ 1. Every copy of a cell goes through the same update helper (and the library is built with
+// -ffp-contract=on), so a recomputed halo equals its owner's value bit for bit.
 //
-// HBM traffic per cell-update: read E, H_a, H_b (12 B) + uint16 material id (2 B), write 12 B = 26 B; the per-id
-// coefficient table stays in L1/L2. Tile-boundary rows/columns are re-read (1/rows_per_tile and 1/63 extra) from L2.
+// Memory pipeline. The row index is wave-uniform (scalar registers), so per-row constants come from scalar loads,
+// and the per-id coefficient table sits in LDS: the only vector-memory operations are the field / id rows, the CPML
+// psi rows and the stores. Each iteration first issues the loads its own row depends on (psi), then the next row's
+// field loads, then computes; the loop is unrolled over the rotating row buffers so no register copy forces a wait
+// on the prefetch (gfx9 counts vector loads and stores in one in-order vmcnt).
+//
+// HBM traffic per cell-update: read E, H_a, H_b (12 B) + uint16 material id (2 B), write 12 B = 26 B.
 //
 // Reference semantics (file:line of the reference package open_sol/):
 //   E_z H update       physics/em/solvers/fdtd_ez_2d_kernels.py:23-60   (update_h_kernel / _update_h_host)
@@ -27,391 +32,823 @@
 
 namespace yee {
 
-constexpr int OWN = YEE_WAVE - 1;   // owned columns per wavefront tile
-
-__host__ __device__ inline int tiles_j_2d(int ny) { return (ny + OWN - 1) / OWN; }
+__host__ __device__ inline int tiles_j_2d(int ny, int vec) { return (ny + vec * YEE_WAVE - 1) / (vec * YEE_WAVE); }
 __host__ __device__ inline int tiles_i_2d(int nx, int rows) { return (nx + rows - 1) / rows; }
 
-// ------------------------------------------------------------------------------------------------------------ E_z
-__device__ __forceinline__ void ez_tile(const yee2d_desc& d, int tj, int ti, int64_t n)
+// ------------------------------------------------------------------------------------------------ LDS tables
+constexpr int LDS_IDS_EZ = 1024;   // E_z: (ca, cb, da, db) per id, 16 KiB
+constexpr int LDS_IDS_HZ = 512;    // H_z: (da, db, eps_xx, eps_yy), (ca_x, cb_x, ca_y, cb_y), sigma_e per id, 18 KiB
+
+__shared__ float4 lds_tab_a[LDS_IDS_EZ];
+__shared__ float4 lds_tab_b[LDS_IDS_HZ];
+__shared__ float lds_tab_sig[LDS_IDS_HZ];
+
+// Stage the coefficient table of one instance in LDS (whole block; every thread reaches the barrier).
+template <int POL>
+__device__ __forceinline__ bool stage_tables(const yee2d_desc& d)
 {
-    const int lane = threadIdx.x & (YEE_WAVE - 1);
-    const int nx = d.nx, ny = d.ny;
-    const int i0 = ti * d.rows_per_tile;
-    const int i1 = min(nx, i0 + d.rows_per_tile);
-    const bool px = d.periodic_x != 0, py = d.periodic_y != 0;
-    const int par = (int)(n & 1);
-
-    // lane -> column; lane 0 is the halo column j0 - 1 (wrapped on a periodic axis, absent otherwise)
-    const int col = tj * OWN - 1 + lane;
-    const bool owner = lane > 0 && col < ny;
-    int j = col;
-    if (col < 0) j = py ? ny - 1 : -1;
-    const bool valid = j >= 0 && j < ny;
-    const int jc = valid ? j : 0;
-
-    const float* __restrict__ ez_o = d.field[par][0];
-    const float* __restrict__ hx_o = d.field[par][1];
-    const float* __restrict__ hy_o = d.field[par][2];
-    float* __restrict__ ez_n = d.field[par ^ 1][0];
-    float* __restrict__ hx_n = d.field[par ^ 1][1];
-    float* __restrict__ hy_n = d.field[par ^ 1][2];
-    const float4* __restrict__ tab = reinterpret_cast<const float4*>(d.table);
-    const uint16_t* __restrict__ mat = d.material;
-
-    // column constants: H_x is updated on columns < ny-1 (every column when periodic), its hi neighbour wraps to 0
-    const bool hx_updated = valid && (j < ny - 1 || py);
-    const int jr = (j + 1 < ny) ? j + 1 : 0;
-    const bool direct_right = hx_updated && (lane == YEE_WAVE - 1 || j == ny - 1);
-    const float idy_h = d.inv_dy_h[jc];
-    const float idy_e = d.inv_dy_e[jc];
-
-    const yee_cpml_axis& cx = d.cpml[0];
-    const yee_cpml_axis& cy = d.cpml[1];
-    const int nry = cy.n_rows;
-    int ry_h = -1, ry_e = -1;
-    float by_h = 0.f, gy_h = 0.f, by_e = 0.f, gy_e = 0.f;
-    if (nry > 0 && valid) {
-        ry_h = cy.row_h[jc];
-        ry_e = cy.row_e[jc];
-        if (ry_h >= 0) { by_h = cy.b_h[ry_h]; gy_h = cy.g_h[ry_h]; }
-        if (ry_e >= 0) { by_e = cy.b_e[ry_e]; gy_e = cy.g_e[ry_e]; }
+    const int cap = POL == YEE_EZ ? LDS_IDS_EZ : LDS_IDS_HZ;
+    const bool use = d.n_materials <= cap;
+    if (use) {
+        const float4* a = reinterpret_cast<const float4*>(d.table);
+        for (int k = threadIdx.x; k < d.n_materials; k += blockDim.x) {
+            lds_tab_a[k] = a[k];
+            if (POL == YEE_HZ) {
+                const float* t2 = d.table2 + 8 * k;
+                lds_tab_b[k] = make_float4(t2[0], t2[1], t2[2], t2[3]);
+                lds_tab_sig[k] = t2[4];
+            }
+        }
     }
-    const float* __restrict__ psi_hy_o = cy.psi_h[par];
-    float* __restrict__ psi_hy_n = cy.psi_h[par ^ 1];
-    const float* __restrict__ psi_ey_o = cy.psi_e[par];
-    float* __restrict__ psi_ey_n = cy.psi_e[par ^ 1];
+    __syncthreads();
+    return use;
+}
 
-    const bool sample = d.n_monitors > 0 && ((n + 1) % d.sample_interval) == 0;
-    const int64_t sample_index = (n + 1) / (d.sample_interval > 0 ? d.sample_interval : 1) - 1;
+template <bool LDS>
+__device__ __forceinline__ float4 tab_a(const yee2d_desc& d, int id)
+{
+    if constexpr (LDS) return lds_tab_a[id];
+    else return reinterpret_cast<const float4*>(d.table)[id];
+}
 
-    // prologue: E of row i0, H_y of row i0-1 after this step (0 below a non-periodic grid edge)
-    float e_cur = valid ? ez_o[(int64_t)i0 * ny + jc] : 0.f;
-    float hy_prev = 0.f;
-    if (i0 > 0 || px) {
-        const int ip = i0 > 0 ? i0 - 1 : nx - 1;
-        const int64_t p0 = (int64_t)ip * ny + jc;
-        const float e_p = valid ? ez_o[p0] : 0.f;
-        const float4 c = tab[mat[p0]];
-        const float de = e_cur - e_p;
-        float v = c.z * hy_o[p0] + c.w * de * d.inv_dx_h[ip];
-        const int rx = cx.n_rows > 0 ? cx.row_h[ip] : -1;
-        if (rx >= 0) {
-            const float psi = cx.b_h[rx] * cx.psi_h[par][(int64_t)rx * ny + jc] + cx.g_h[rx] * de;
-            v += c.w * psi;
-        }
-        hy_prev = valid ? v : 0.f;
-    }
+template <bool LDS>
+__device__ __forceinline__ float4 tab_b(const yee2d_desc& d, int id)
+{
+    if constexpr (LDS) return lds_tab_b[id];
+    else return *reinterpret_cast<const float4*>(d.table2 + 8 * id);
+}
 
-    for (int i = i0; i < i1; ++i) {
-        const int64_t r0 = (int64_t)i * ny;
-        const bool hy_updated = (i < nx - 1) || px;
-        const int inext = (i + 1 < nx) ? i + 1 : 0;
-        const float e_next = (valid && hy_updated) ? ez_o[(int64_t)inext * ny + jc] : 0.f;
-        const float hx_old = hx_o[r0 + jc];
-        const float hy_old = hy_o[r0 + jc];
-        const float4 c = tab[mat[r0 + jc]];   // (ca, cb, da, db)
+template <bool LDS>
+__device__ __forceinline__ float tab_sig(const yee2d_desc& d, int id)
+{
+    if constexpr (LDS) return lds_tab_sig[id];
+    else return d.table2[8 * id + 4];
+}
 
-        // E of the hi-y neighbour: the next lane, or a direct read at the tile edge / periodic wrap
-        float e_right = shfl_from_next(e_cur);
-        if (direct_right) e_right = ez_o[r0 + jr];
-
-        // H_y on the hi-x face (+ x CPML, sign +)
-        float hy_v = hy_old;
-        if (hy_updated) {
-            const float de = e_next - e_cur;
-            hy_v = c.z * hy_old + c.w * de * d.inv_dx_h[i];
-            const int rx = cx.n_rows > 0 ? cx.row_h[i] : -1;
-            if (rx >= 0) {
-                const int64_t p = (int64_t)rx * ny + jc;
-                const float psi = cx.b_h[rx] * cx.psi_h[par][p] + cx.g_h[rx] * de;
-                if (owner) cx.psi_h[par ^ 1][p] = psi;
-                hy_v += c.w * psi;
-            }
-        }
-        // H_x on the hi-y face (+ y CPML, sign -)
-        float hx_v = valid ? hx_old : 0.f;
-        if (hx_updated) {
-            const float de = e_right - e_cur;
-            hx_v = c.z * hx_old - c.w * de * idy_h;
-            if (ry_h >= 0) {
-                const int64_t p = (int64_t)i * nry + ry_h;
-                const float psi = by_h * psi_hy_o[p] + gy_h * de;
-                if (owner) psi_hy_n[p] = psi;
-                hx_v -= c.w * psi;
-            }
-        }
-        // H_x of the lo-y neighbour after this step (lane 0 of a periodic/interior tile holds column j0-1)
-        const float hx_left = shfl_from_prev(hx_v);
-
-        // E_z at the centre (+ x CPML sign +, y CPML sign -)
-        const float dhy = hy_v - hy_prev;
-        const float dhx = hx_v - hx_left;
-        float ez_v = c.x * e_cur + c.y * (dhy * d.inv_dx_e[i] - dhx * idy_e);
-        const int rxe = cx.n_rows > 0 ? cx.row_e[i] : -1;
-        if (rxe >= 0) {
-            const int64_t p = (int64_t)rxe * ny + jc;
-            const float psi = cx.b_e[rxe] * cx.psi_e[par][p] + cx.g_e[rxe] * dhy;
-            if (owner) cx.psi_e[par ^ 1][p] = psi;
-            ez_v += c.y * psi;
-        }
-        if (ry_e >= 0) {
-            const int64_t p = (int64_t)i * nry + ry_e;
-            const float psi = by_e * psi_ey_o[p] + gy_e * dhx;
-            if (owner) psi_ey_n[p] = psi;
-            ez_v -= c.y * psi;
-        }
-        // sheet sources at t + dt/2: E_z += float32(value * profile)
-        for (int s = 0; s < d.n_sources; ++s) {
-            const yee_line_source& src = d.source[s];
-            const bool on_line = src.axis == 1 ? (i == src.line && j >= src.lo && j < src.hi)
-                                               : (j == src.line && i >= src.lo && i < src.hi);
-            if (owner && on_line) {
-                const int k = src.axis == 1 ? j - src.lo : i - src.lo;
-                ez_v += __fmul_rn(wave_value(src, n), src.profile[k]);
-            }
-        }
-        if (owner) {
-            ez_n[r0 + j] = ez_v;
-            hx_n[r0 + j] = hx_v;
-            hy_n[r0 + j] = hy_v;
-        }
-        if (sample) {
-            for (int m = 0; m < d.n_monitors; ++m) {
-                const yee_flux_line& mon = d.monitor[m];
-                if (mon.normal == 0) {
-                    if (owner && i == mon.position && j >= mon.lo && j < mon.hi)
-                        dft_accumulate(mon, sample_index, j - mon.lo, ez_v, hy_v);
-                } else if (owner && j == mon.position && i >= mon.lo && i < mon.hi) {
-                    dft_accumulate(mon, sample_index, i - mon.lo, ez_v, hx_v);
-                }
-            }
-        }
-        hy_prev = hy_v;
-        e_cur = e_next;
+// ------------------------------------------------------------------------------------------------ vector access
+template <int VEC>
+__device__ __forceinline__ void ldv(const float* __restrict__ p, float out[VEC])
+{
+    if constexpr (VEC == 4) {
+        const float4 v = *reinterpret_cast<const float4*>(p);
+        out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+    } else {
+        const float2 v = *reinterpret_cast<const float2*>(p);
+        out[0] = v.x; out[1] = v.y;
     }
 }
 
-// ------------------------------------------------------------------------------------------------------------ H_z
-// E face coefficients of a face between cells of ids a and b: the per-id table holds the coefficients of a face
-// between two cells of that id (computed in FP64 on the host); a mixed face takes the mean eps / sigma (shares 1/2,
-// fdtd_hz_2d_kernels.py:190-278) and forms ca, cb here in float32.
+template <int VEC>
+__device__ __forceinline__ void stv(float* __restrict__ p, const float v[VEC])
+{
+    if constexpr (VEC == 4)
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    else
+        *reinterpret_cast<float2*>(p) = make_float2(v[0], v[1]);
+}
+
+// VEC uint16 ids kept packed (two per register) until use, so a prefetch is not waited on early
+template <int VEC>
+__device__ __forceinline__ void ldids(const uint16_t* __restrict__ p, uint32_t out[VEC / 2])
+{
+    if constexpr (VEC == 4) {
+        const uint2 v = *reinterpret_cast<const uint2*>(p);
+        out[0] = v.x; out[1] = v.y;
+    } else {
+        out[0] = *reinterpret_cast<const uint32_t*>(p);
+    }
+}
+
+__device__ __forceinline__ int id_at(const uint32_t* packed, int k)
+{
+    return (packed[k >> 1] >> (16 * (k & 1))) & 0xffff;
+}
+
+// element k of a register array without dynamic indexing (which would move the array to scratch memory)
+template <int VEC>
+__device__ __forceinline__ float pickv(const float v[VEC], int k)
+{
+    float out = v[0];
+#pragma unroll
+    for (int q = 1; q < VEC; ++q) out = k == q ? v[q] : out;
+    return out;
+}
+
+// ------------------------------------------------------------------------------------------ single-cell updates
+// E_z polarisation, c = (ca, cb, da, db): H_y on a hi-x face = da H_y + db dE_z/dx (x CPML: psi = b psi + g dE,
+// H_y += db psi); H_x on a hi-y face = da H_x - db dE_z/dy (y CPML: H_x -= db psi).
+__device__ __forceinline__ float ez_hy(float4 c, float hy, float de, float idx, bool layer, float b, float g,
+                                       float psi_old, float* psi_new)
+{
+    float v = c.z * hy + c.w * de * idx;
+    if (layer) {
+        const float psi = b * psi_old + g * de;
+        *psi_new = psi;
+        v += c.w * psi;
+    }
+    return v;
+}
+
+__device__ __forceinline__ float ez_hx(float4 c, float hx, float de, float idy, bool layer, float b, float g,
+                                       float psi_old, float* psi_new)
+{
+    float v = c.z * hx - c.w * de * idy;
+    if (layer) {
+        const float psi = b * psi_old + g * de;
+        *psi_new = psi;
+        v -= c.w * psi;
+    }
+    return v;
+}
+
+// H_z polarisation, c = (da, db, eps_xx, eps_yy): H_z at a centre = da H_z + db (dE_x/dy - dE_y/dx)
+// (x CPML: H_z -= db psi_x, y CPML: H_z += db psi_y).
+__device__ __forceinline__ float hz_centre(float4 c, float hz, float dex, float dey, float idy, float idx,
+                                           bool layer_x, float bx, float gx, float psi_x_old, float* psi_x_new,
+                                           bool layer_y, float by, float gy, float psi_y_old, float* psi_y_new)
+{
+    float v = c.x * hz + c.y * (dex * idy - dey * idx);
+    if (layer_x) {
+        const float psi = bx * psi_x_old + gx * dey;
+        *psi_x_new = psi;
+        v -= c.y * psi;
+    }
+    if (layer_y) {
+        const float psi = by * psi_y_old + gy * dex;
+        *psi_y_new = psi;
+        v += c.y * psi;
+    }
+    return v;
+}
+
+// E face coefficients between cells of ids a and b (comp 0: E_x face, 1: E_y face). Same id: the table's FP64
+// coefficients; mixed: mean eps / sigma (shares 1/2, fdtd_hz_2d_kernels.py:190-278) formed here in float32.
+template <bool LDS>
 __device__ __forceinline__ void face_coefficients(const yee2d_desc& d, int ida, int idb, int comp, float* ca,
                                                   float* cb)
 {
-    const float* __restrict__ t2 = d.table2;
     if (ida == idb) {
-        *ca = t2[8 * ida + 2 * comp];
-        *cb = t2[8 * ida + 2 * comp + 1];
+        const float4 s = tab_b<LDS>(d, ida);
+        *ca = comp == 0 ? s.x : s.z;
+        *cb = comp == 0 ? s.y : s.w;
         return;
     }
-    const float4* __restrict__ tab = reinterpret_cast<const float4*>(d.table);
-    const float ea = comp == 0 ? tab[ida].z : tab[ida].w;
-    const float eb = comp == 0 ? tab[idb].z : tab[idb].w;
-    const float sa = t2[8 * ida + 4], sb = t2[8 * idb + 4];
+    const float4 a = tab_a<LDS>(d, ida), b = tab_a<LDS>(d, idb);
+    const float ea = comp == 0 ? a.z : a.w;
+    const float eb = comp == 0 ? b.z : b.w;
     const float eps_face = 0.5f * ea + 0.5f * eb;
-    const float sig_face = 0.5f * sa + 0.5f * sb;
+    const float sig_face = 0.5f * tab_sig<LDS>(d, ida) + 0.5f * tab_sig<LDS>(d, idb);
     const float k = sig_face * (0.5f * d.dt_over_eps0) / eps_face;
     const float den = 1.0f + k;
     *ca = (1.0f - k) / den;
     *cb = (d.dt_over_eps0 / eps_face) / den;
 }
 
-__device__ __forceinline__ void hz_tile(const yee2d_desc& d, int tj, int ti, int64_t n)
+// Per-column constants of one lane's columns: curl spacings and the y CPML psi row of each column (-1 outside).
+template <int VEC>
+struct Columns {
+    int jb;                 // first column of the lane
+    bool any;               // the lane holds at least one real column
+    bool y_layer;           // some column of the lane lies in a y CPML layer
+    float idy_e[VEC], idy_h[VEC];
+    int ry_e[VEC], ry_h[VEC];
+
+    __device__ __forceinline__ void load(const yee2d_desc& d, int first)
+    {
+        jb = first;
+        any = jb < d.ny;
+        y_layer = false;
+        const yee_cpml_axis& cy = d.cpml[1];
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const int j = jb + k;
+            const int jc = j < d.ny ? j : d.ny - 1;
+            idy_e[k] = d.inv_dy_e[jc];
+            idy_h[k] = d.inv_dy_h[jc];
+            ry_e[k] = (cy.n_rows > 0 && j < d.ny) ? cy.row_e[j] : -1;
+            ry_h[k] = (cy.n_rows > 0 && j < d.ny) ? cy.row_h[j] : -1;
+            y_layer = y_layer || ry_e[k] >= 0 || ry_h[k] >= 0;
+        }
+    }
+};
+
+// Per-row x CPML constants (wave-uniform: scalar loads).
+struct RowCpml {
+    int r;          // psi row, -1 outside the x layers
+    float b, g;
+};
+
+__device__ __forceinline__ RowCpml row_cpml(const yee_cpml_axis& c, int i, bool centre)
 {
-    const int lane = threadIdx.x & (YEE_WAVE - 1);
-    const int nx = d.nx, ny = d.ny;
-    const int i0 = ti * d.rows_per_tile;
-    const int i1 = min(nx, i0 + d.rows_per_tile);
-    const bool px = d.periodic_x != 0, py = d.periodic_y != 0;
-    const int par = (int)(n & 1);
+    RowCpml out{-1, 0.f, 0.f};
+    if (c.n_rows > 0) {
+        out.r = centre ? c.row_e[i] : c.row_h[i];
+        if (out.r >= 0) {
+            out.b = centre ? c.b_e[out.r] : c.b_h[out.r];
+            out.g = centre ? c.g_e[out.r] : c.g_h[out.r];
+        }
+    }
+    return out;
+}
 
-    // lane -> column; lane 63 is the halo column j0 + 63 (column 0 after the last one on a periodic axis)
-    const int col = tj * OWN + lane;
-    const bool owner = lane < YEE_WAVE - 1 && col < ny;
-    int j = col;
-    if (col >= ny) j = (py && col == ny) ? 0 : -1;
-    const bool valid = j >= 0;
-    const int jc = valid ? j : 0;
+__device__ __forceinline__ int wave_index()
+{
+    return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
 
-    const float* __restrict__ hz_o = d.field[par][0];
-    const float* __restrict__ ex_o = d.field[par][1];
-    const float* __restrict__ ey_o = d.field[par][2];
-    float* __restrict__ hz_n = d.field[par ^ 1][0];
-    float* __restrict__ ex_n = d.field[par ^ 1][1];
-    float* __restrict__ ey_n = d.field[par ^ 1][2];
-    const float4* __restrict__ tab = reinterpret_cast<const float4*>(d.table);
-    const uint16_t* __restrict__ mat = d.material;
+// ------------------------------------------------------------------------------------------------------------ E_z
+// Loads of row i issued one row ahead: E of row i+1 (the H_y difference), H_x, H_y, ids of row i, and the edge
+// values: lane 0 reads column jl (E, H_x, id), the right-edge lane reads E of column jr.
+template <int VEC>
+struct EzIn {
+    float e_next[VEC], hx[VEC], hy[VEC];
+    uint32_t ids[VEC / 2];
+    float e_l, hx_l, e_r;
+    uint32_t id_l;
+};
 
-    const bool ex_updated = valid && (j < ny - 1 || py);
-    const int jl = j > 0 ? j - 1 : (py ? ny - 1 : -1);
-    const bool direct_left = valid && lane == 0 && jl >= 0;
-    const float idy_h = d.inv_dy_h[jc];
-    const float idy_e = d.inv_dy_e[jc];
+template <int VEC, bool LDS>
+struct EzTile {
+    const yee2d_desc& d;
+    int64_t n;
+    int par, lane, nx, ny, P, jb_c, jl, jr, nry;
+    bool px, py, left_halo, right_halo, sample;
+    int64_t sample_index;
+    Columns<VEC> col;
+    float idy_h_l;
+    int ry_l;
+    float e_cur[VEC], hy_prev[VEC];
 
-    const yee_cpml_axis& cx = d.cpml[0];
-    const yee_cpml_axis& cy = d.cpml[1];
-    const int nry = cy.n_rows;
-    int ry_h = -1, ry_e = -1;
-    float by_h = 0.f, gy_h = 0.f, by_e = 0.f, gy_e = 0.f;
-    if (nry > 0 && valid) {
-        ry_h = cy.row_h[jc];
-        ry_e = cy.row_e[jc];
-        if (ry_h >= 0) { by_h = cy.b_h[ry_h]; gy_h = cy.g_h[ry_h]; }
-        if (ry_e >= 0) { by_e = cy.b_e[ry_e]; gy_e = cy.g_e[ry_e]; }
+    __device__ __forceinline__ EzTile(const yee2d_desc& desc, int64_t step) : d(desc), n(step) {}
+
+    __device__ __forceinline__ void load(int i, EzIn<VEC>& r) const
+    {
+        const bool hy_updated = (i < nx - 1) || px;
+        const int inext = (i + 1 < nx) ? i + 1 : 0;
+        const int64_t row = (int64_t)i * P;
+        if (hy_updated) ldv<VEC>(d.field[par][0] + (int64_t)inext * P + jb_c, r.e_next);
+        else {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) r.e_next[k] = 0.f;
+        }
+        ldv<VEC>(d.field[par][1] + row + jb_c, r.hx);
+        ldv<VEC>(d.field[par][2] + row + jb_c, r.hy);
+        ldids<VEC>(d.material + row + jb_c, r.ids);
+        r.e_l = r.hx_l = r.e_r = 0.f;
+        r.id_l = 0;
+        if (left_halo) {
+            r.e_l = d.field[par][0][row + jl];
+            r.hx_l = d.field[par][1][row + jl];
+            r.id_l = d.material[row + jl];
+        }
+        if (right_halo) r.e_r = d.field[par][0][row + jr];
     }
 
-    const bool sample = d.n_monitors > 0 && ((n + 1) % d.sample_interval) == 0;
-    const int64_t sample_index = (n + 1) / (d.sample_interval > 0 ? d.sample_interval : 1) - 1;
+    // one row: the row's own psi reads first, then the next row's prefetch, then the update
+    __device__ __forceinline__ void step(int i, const EzIn<VEC>& cur, EzIn<VEC>& nxt, bool prefetch)
+    {
+        const yee_cpml_axis& cx = d.cpml[0];
+        const yee_cpml_axis& cy = d.cpml[1];
+        const int64_t row = (int64_t)i * P;
+        const bool hy_updated = (i < nx - 1) || px;
+        const RowCpml xh = row_cpml(cx, i, false);
+        const RowCpml xe = row_cpml(cx, i, true);
 
-    // H_z of row k after this step from the old E (ey_below = E_y of row k-1, 0 below a non-periodic edge).
-    // store: write H_z and its psi (the row is owned by this tile).
-    auto hz_row = [&](int k, float hz_old, float ex_k, float ey_k, float ey_below, int id_k, bool store) -> float {
-        const int64_t rk = (int64_t)k * ny;
-        float ex_left = shfl_from_prev(ex_k);
-        if (direct_left) ex_left = ex_o[rk + jl];
-        if (jl < 0) ex_left = 0.f;
-        const float dex = ex_k - ex_left;
-        const float dey = ey_k - ey_below;
-        const float4 c = tab[id_k];   // (da, db, eps_xx, eps_yy)
-        float v = c.x * hz_old + c.y * (dex * idy_e - dey * d.inv_dx_e[k]);
-        const int rx = cx.n_rows > 0 ? cx.row_e[k] : -1;
-        if (rx >= 0) {   // x sides: H_z += -db psi
-            const int64_t p = (int64_t)rx * ny + jc;
-            const float psi = cx.b_e[rx] * cx.psi_e[par][p] + cx.g_e[rx] * dey;
-            if (store && owner) cx.psi_e[par ^ 1][p] = psi;
-            v -= c.y * psi;
-        }
-        if (ry_e >= 0) {   // y sides: H_z += +db psi
-            const int64_t p = (int64_t)k * nry + ry_e;
-            const float psi = by_e * cy.psi_e[par][p] + gy_e * dex;
-            if (store && owner) cy.psi_e[par ^ 1][p] = psi;
-            v += c.y * psi;
-        }
-        if (store && owner) hz_n[rk + j] = v;
-        return valid ? v : 0.f;
-    };
-
-    // prologue: row i0 and E_y of row i0-1
-    int64_t r0 = (int64_t)i0 * ny;
-    float hz_old_cur = hz_o[r0 + jc];
-    float ex_cur = valid ? ex_o[r0 + jc] : 0.f;
-    float ey_cur = valid ? ey_o[r0 + jc] : 0.f;
-    int id_cur = mat[r0 + jc];
-    float ey_below = 0.f;
-    if ((i0 > 0 || px) && valid) ey_below = ey_o[(int64_t)(i0 > 0 ? i0 - 1 : nx - 1) * ny + jc];
-    float hz_cur = hz_row(i0, hz_old_cur, ex_cur, ey_cur, ey_below, id_cur, true);
-
-    for (int i = i0; i < i1; ++i) {
-        r0 = (int64_t)i * ny;
-        const bool ey_updated = (i < nx - 1) || px;
-        // H_z of the next row (row 0 after the last one on a periodic axis); owned when inside the tile
-        float hz_next = 0.f;
-        int id_next = id_cur;
-        float ex_next = 0.f, ey_next = 0.f;
-        if (ey_updated) {
-            const int k = (i + 1 < nx) ? i + 1 : 0;
-            const int64_t rk = (int64_t)k * ny;
-            const float hz_old_next = hz_o[rk + jc];
-            ex_next = valid ? ex_o[rk + jc] : 0.f;
-            ey_next = valid ? ey_o[rk + jc] : 0.f;
-            id_next = mat[rk + jc];
-            hz_next = hz_row(k, hz_old_next, ex_next, ey_next, ey_cur, id_next, i + 1 < i1);
-        }
-        // E_x on the hi-y face: H_z of the next lane (+ y CPML 'h' rows, sign +)
-        const float hz_right = shfl_from_next(hz_cur);
-        const int id_right = __shfl_down(id_cur, 1, YEE_WAVE);
-        float ex_v = ex_cur;
-        if (ex_updated) {
-            float ca, cb;
-            face_coefficients(d, id_cur, id_right, 0, &ca, &cb);
-            const float dh = hz_right - hz_cur;
-            ex_v = ca * ex_cur + cb * dh * idy_h;
-            if (ry_h >= 0) {
-                const int64_t p = (int64_t)i * nry + ry_h;
-                const float psi = by_h * cy.psi_h[par][p] + gy_h * dh;
-                if (owner) cy.psi_h[par ^ 1][p] = psi;
-                ex_v += cb * psi;
+        // (1) psi of this row
+        float psi_xh[VEC] = {}, psi_xe[VEC] = {}, psi_yh[VEC] = {}, psi_ye[VEC] = {};
+        float psi_l = 0.f;
+        if (xh.r >= 0 && hy_updated) ldv<VEC>(cx.psi_h[par] + (int64_t)xh.r * P + jb_c, psi_xh);
+        if (xe.r >= 0) ldv<VEC>(cx.psi_e[par] + (int64_t)xe.r * P + jb_c, psi_xe);
+        if (col.y_layer) {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) {
+                if (col.ry_h[k] >= 0) psi_yh[k] = cy.psi_h[par][(int64_t)i * nry + col.ry_h[k]];
+                if (col.ry_e[k] >= 0) psi_ye[k] = cy.psi_e[par][(int64_t)i * nry + col.ry_e[k]];
             }
         }
-        // E_y on the hi-x face: H_z of the next row (+ x CPML 'h' rows, sign -)
-        float ey_v = ey_cur;
-        if (ey_updated) {
-            float ca, cb;
-            face_coefficients(d, id_cur, id_next, 1, &ca, &cb);
-            const float dh = hz_next - hz_cur;
-            ey_v = ca * ey_cur - cb * dh * d.inv_dx_h[i];
-            const int rx = cx.n_rows > 0 ? cx.row_h[i] : -1;
-            if (rx >= 0) {
-                const int64_t p = (int64_t)rx * ny + jc;
-                const float psi = cx.b_h[rx] * cx.psi_h[par][p] + cx.g_h[rx] * dh;
-                if (owner) cx.psi_h[par ^ 1][p] = psi;
-                ey_v -= cb * psi;
+        if (ry_l >= 0) psi_l = cy.psi_h[par][(int64_t)i * nry + ry_l];
+        // (2) next row
+        if (prefetch) load(i + 1, nxt);
+
+        // (3) update
+        const float idx_h = d.inv_dx_h[i];
+        const float idx_e = d.inv_dx_e[i];
+        float4 c[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) c[k] = tab_a<LDS>(d, id_at(cur.ids, k));   // (ca, cb, da, db)
+
+        const float e_sh = shfl_from_next(e_cur[0]);
+        float e_right[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const int j = col.jb + k;
+            if (k < VEC - 1 && j + 1 < ny) e_right[k] = e_cur[k + 1];
+            else if (j + 1 < ny) e_right[k] = lane == YEE_WAVE - 1 ? cur.e_r : e_sh;
+            else e_right[k] = cur.e_r;   // periodic wrap to column 0 (unused otherwise)
+        }
+
+        float hy_v[VEC], hx_v[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            hy_v[k] = cur.hy[k];
+            if (hy_updated)
+                hy_v[k] = ez_hy(c[k], cur.hy[k], cur.e_next[k] - e_cur[k], idx_h, xh.r >= 0, xh.b, xh.g, psi_xh[k],
+                                &psi_xh[k]);
+            const int j = col.jb + k;
+            hx_v[k] = j < ny ? cur.hx[k] : 0.f;
+            if (j < ny && (j < ny - 1 || py)) {
+                const bool layer = col.ry_h[k] >= 0;
+                hx_v[k] = ez_hx(c[k], cur.hx[k], e_right[k] - e_cur[k], col.idy_h[k], layer,
+                                layer ? cy.b_h[col.ry_h[k]] : 0.f, layer ? cy.g_h[col.ry_h[k]] : 0.f, psi_yh[k],
+                                &psi_yh[k]);
+            }
+        }
+        // H_x left of the first column: the previous lane, or lane 0's recompute of column jl
+        float hx_left = shfl_from_prev(hx_v[VEC - 1]);
+        if (left_halo) {
+            float unused;
+            hx_left = ez_hx(tab_a<LDS>(d, cur.id_l), cur.hx_l, e_cur[0] - cur.e_l, idy_h_l, ry_l >= 0,
+                            ry_l >= 0 ? cy.b_h[ry_l] : 0.f, ry_l >= 0 ? cy.g_h[ry_l] : 0.f, psi_l, &unused);
+        } else if (lane == 0) {
+            hx_left = 0.f;   // column 0 of a non-periodic axis reads H_x = 0 below it
+        }
+
+        float ez_v[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const int j = col.jb + k;
+            const float dhy = hy_v[k] - hy_prev[k];
+            const float dhx = hx_v[k] - (k == 0 ? hx_left : hx_v[k - 1]);
+            float v = c[k].x * e_cur[k] + c[k].y * (dhy * idx_e - dhx * col.idy_e[k]);
+            if (xe.r >= 0) {
+                psi_xe[k] = xe.b * psi_xe[k] + xe.g * dhy;
+                v += c[k].y * psi_xe[k];
+            }
+            if (col.ry_e[k] >= 0) {
+                psi_ye[k] = cy.b_e[col.ry_e[k]] * psi_ye[k] + cy.g_e[col.ry_e[k]] * dhx;
+                v -= c[k].y * psi_ye[k];
+            }
+            ez_v[k] = j < ny ? v : 0.f;
+        }
+        // sheet sources at t + dt/2: E_z += float32(value * profile)
+        for (int s = 0; s < d.n_sources; ++s) {
+            const yee_line_source& src = d.source[s];
+            const float w = wave_value(src, n);
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) {
+                const int j = col.jb + k;
+                const bool on_line = src.axis == 1 ? (i == src.line && j >= src.lo && j < src.hi)
+                                                   : (j == src.line && i >= src.lo && i < src.hi);
+                if (on_line) ez_v[k] += __fmul_rn(w, src.profile[src.axis == 1 ? j - src.lo : i - src.lo]);
+            }
+        }
+        // (4) stores
+        if (col.any) {
+            const int jb = col.jb;
+            stv<VEC>(d.field[par ^ 1][0] + row + jb, ez_v);
+            stv<VEC>(d.field[par ^ 1][1] + row + jb, hx_v);
+            stv<VEC>(d.field[par ^ 1][2] + row + jb, hy_v);
+            if (xh.r >= 0 && hy_updated) stv<VEC>(cx.psi_h[par ^ 1] + (int64_t)xh.r * P + jb, psi_xh);
+            if (xe.r >= 0) stv<VEC>(cx.psi_e[par ^ 1] + (int64_t)xe.r * P + jb, psi_xe);
+            if (col.y_layer) {
+#pragma unroll
+                for (int k = 0; k < VEC; ++k) {
+                    const int j = jb + k;
+                    if (col.ry_h[k] >= 0 && (j < ny - 1 || py))
+                        cy.psi_h[par ^ 1][(int64_t)i * nry + col.ry_h[k]] = psi_yh[k];
+                    if (col.ry_e[k] >= 0) cy.psi_e[par ^ 1][(int64_t)i * nry + col.ry_e[k]] = psi_ye[k];
+                }
+            }
+        }
+        if (sample) {
+            for (int m = 0; m < d.n_monitors; ++m) {
+                const yee_flux_line& mon = d.monitor[m];
+#pragma unroll
+                for (int k = 0; k < VEC; ++k) {
+                    const int j = col.jb + k;
+                    if (mon.normal == 0) {
+                        if (i == mon.position && j >= mon.lo && j < mon.hi)
+                            dft_accumulate(mon, sample_index, j - mon.lo, ez_v[k], hy_v[k]);
+                    } else if (j == mon.position && i >= mon.lo && i < mon.hi) {
+                        dft_accumulate(mon, sample_index, i - mon.lo, ez_v[k], hx_v[k]);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            hy_prev[k] = hy_v[k];
+            e_cur[k] = cur.e_next[k];
+        }
+    }
+
+    __device__ __forceinline__ void run(int tj, int ti)
+    {
+        lane = threadIdx.x & (YEE_WAVE - 1);
+        nx = d.nx; ny = d.ny; P = d.pitch;
+        px = d.periodic_x != 0; py = d.periodic_y != 0;
+        par = (int)(n & 1);
+        const int i0 = ti * d.rows_per_tile;
+        const int i1 = min(nx, i0 + d.rows_per_tile);
+        col.load(d, tj * VEC * YEE_WAVE + VEC * lane);
+        const int jb = col.jb;
+        jb_c = col.any ? jb : 0;
+        const yee_cpml_axis& cx = d.cpml[0];
+        const yee_cpml_axis& cy = d.cpml[1];
+        nry = cy.n_rows;
+
+        // left edge: H_x of column jl feeds the E update of column jb (lane 0 only)
+        jl = jb > 0 ? jb - 1 : (py ? ny - 1 : -1);
+        left_halo = lane == 0 && col.any && jl >= 0;
+        idy_h_l = 0.f;
+        ry_l = -1;
+        if (left_halo) {
+            idy_h_l = d.inv_dy_h[jl];
+            if (nry > 0) ry_l = cy.row_h[jl];
+        }
+        // right edge: E of column jr for the H_x of the lane's last real column (tile edge or periodic wrap)
+        const int last = min(jb + VEC - 1, ny - 1);
+        const bool wraps = py && last == ny - 1 && col.any;
+        jr = wraps ? 0 : jb + VEC;
+        right_halo = col.any && (wraps || (lane == YEE_WAVE - 1 && jb + VEC < ny));
+
+        sample = d.n_monitors > 0 && ((n + 1) % d.sample_interval) == 0;
+        sample_index = (n + 1) / (d.sample_interval > 0 ? d.sample_interval : 1) - 1;
+
+        // prologue: E of row i0, H_y of row i0-1 after this step (0 below a non-periodic grid edge)
+        ldv<VEC>(d.field[par][0] + (int64_t)i0 * P + jb_c, e_cur);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) hy_prev[k] = 0.f;
+        if ((i0 > 0 || px) && col.any) {
+            const int ip = i0 > 0 ? i0 - 1 : nx - 1;
+            const int64_t p0 = (int64_t)ip * P + jb_c;
+            float e_p[VEC], hy_o[VEC], psi_o[VEC] = {};
+            uint32_t idp[VEC / 2];
+            ldv<VEC>(d.field[par][0] + p0, e_p);
+            ldv<VEC>(d.field[par][2] + p0, hy_o);
+            ldids<VEC>(d.material + p0, idp);
+            const RowCpml xh = row_cpml(cx, ip, false);
+            if (xh.r >= 0) ldv<VEC>(cx.psi_h[par] + (int64_t)xh.r * P + jb_c, psi_o);
+            const float idx = d.inv_dx_h[ip];
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) {
+                float unused;
+                hy_prev[k] = ez_hy(tab_a<LDS>(d, id_at(idp, k)), hy_o[k], e_cur[k] - e_p[k], idx, xh.r >= 0, xh.b,
+                                   xh.g, psi_o[k], &unused);
+            }
+        }
+
+        EzIn<VEC> a, b;
+        load(i0, a);
+        int i = i0;
+        for (; i + 1 < i1; i += 2) {
+            step(i, a, b, true);
+            step(i + 1, b, a, i + 2 < i1);
+        }
+        if (i < i1) step(i, a, b, false);
+    }
+};
+
+// ------------------------------------------------------------------------------------------------------------ H_z
+// Loads of row k issued ahead: H_z, E_x, E_y, ids, lane 0's E_x of column jl, and the right-edge lane's values of
+// column jr (H_z, E_x, E_y, id) for recomputing H_z there.
+template <int VEC>
+struct HzIn {
+    float hz[VEC], ex[VEC], ey[VEC];
+    uint32_t ids[VEC / 2];
+    float ex_l, hz_r, ex_r, ey_r;
+    uint32_t id_r;
+};
+
+// psi reads of one H_z row (x row of centres, y columns, right-edge column)
+template <int VEC>
+struct HzPsi {
+    float x[VEC], y[VEC], x_r, y_r;
+};
+
+template <int VEC, bool LDS>
+struct HzTile {
+    const yee2d_desc& d;
+    int64_t n;
+    int par, lane, nx, ny, P, jb_c, jl, jr, kr, nry, i0, i1;
+    bool px, py, left_halo, right_halo, sample;
+    int64_t sample_index;
+    Columns<VEC> col;
+    float idy_e_r;
+    int ry_e_r;
+    float hz_cur[VEC], hz_cur_r;   // H_z after this step of the current row (and of the right-edge column)
+
+    __device__ __forceinline__ HzTile(const yee2d_desc& desc, int64_t step) : d(desc), n(step) {}
+
+    // the row after r, wrapped on a periodic axis; -1 past a non-periodic edge
+    __device__ __forceinline__ int row_after(int r) const { return r < nx ? r : (px ? r - nx : -1); }
+
+    __device__ __forceinline__ void load(int k, HzIn<VEC>& r) const
+    {
+        const int64_t row = (int64_t)k * P;
+        ldv<VEC>(d.field[par][0] + row + jb_c, r.hz);
+        ldv<VEC>(d.field[par][1] + row + jb_c, r.ex);
+        ldv<VEC>(d.field[par][2] + row + jb_c, r.ey);
+        ldids<VEC>(d.material + row + jb_c, r.ids);
+        r.ex_l = r.hz_r = r.ex_r = r.ey_r = 0.f;
+        r.id_r = 0;
+        if (left_halo) r.ex_l = d.field[par][1][row + jl];
+        if (right_halo) {
+            r.hz_r = d.field[par][0][row + jr];
+            r.ex_r = d.field[par][1][row + jr];
+            r.ey_r = d.field[par][2][row + jr];
+            r.id_r = d.material[row + jr];
+        }
+    }
+
+    __device__ __forceinline__ void load_hz_psi(int k, const RowCpml& xe, HzPsi<VEC>& p) const
+    {
+        const yee_cpml_axis& cx = d.cpml[0];
+        const yee_cpml_axis& cy = d.cpml[1];
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) { p.x[q] = 0.f; p.y[q] = 0.f; }
+        p.x_r = p.y_r = 0.f;
+        if (xe.r >= 0) {
+            ldv<VEC>(cx.psi_e[par] + (int64_t)xe.r * P + jb_c, p.x);
+            if (right_halo) p.x_r = cx.psi_e[par][(int64_t)xe.r * P + jr];
+        }
+        if (col.y_layer) {
+#pragma unroll
+            for (int q = 0; q < VEC; ++q)
+                if (col.ry_e[q] >= 0) p.y[q] = cy.psi_e[par][(int64_t)k * nry + col.ry_e[q]];
+        }
+        if (ry_e_r >= 0) p.y_r = cy.psi_e[par][(int64_t)k * nry + ry_e_r];
+    }
+
+    // H_z of row k after this step from its old values r (ey_below: E_y of row k-1, 0 below a non-periodic edge);
+    // store: write H_z and its psi (the row belongs to this tile)
+    __device__ __forceinline__ void hz_row(int k, const HzIn<VEC>& r, const float ey_below[VEC], float ey_below_r,
+                                           const RowCpml& xe, HzPsi<VEC>& p, bool store, float out[VEC],
+                                           float* out_r)
+    {
+        const yee_cpml_axis& cx = d.cpml[0];
+        const yee_cpml_axis& cy = d.cpml[1];
+        const float idx_e = d.inv_dx_e[k];
+        float ex_left = shfl_from_prev(r.ex[VEC - 1]);
+        if (lane == 0) ex_left = left_halo ? r.ex_l : 0.f;
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) {
+            const int j = col.jb + q;
+            const bool layer_y = col.ry_e[q] >= 0;
+            const float v = hz_centre(tab_a<LDS>(d, id_at(r.ids, q)), r.hz[q],
+                                      r.ex[q] - (q == 0 ? ex_left : r.ex[q - 1]), r.ey[q] - ey_below[q],
+                                      col.idy_e[q], idx_e, xe.r >= 0, xe.b, xe.g, p.x[q], &p.x[q], layer_y,
+                                      layer_y ? cy.b_e[col.ry_e[q]] : 0.f, layer_y ? cy.g_e[col.ry_e[q]] : 0.f,
+                                      p.y[q], &p.y[q]);
+            out[q] = j < ny ? v : 0.f;
+        }
+        if (store && col.any) {
+            const int64_t row = (int64_t)k * P + col.jb;
+            stv<VEC>(d.field[par ^ 1][0] + row, out);
+            if (xe.r >= 0) stv<VEC>(cx.psi_e[par ^ 1] + (int64_t)xe.r * P + col.jb, p.x);
+            if (col.y_layer) {
+#pragma unroll
+                for (int q = 0; q < VEC; ++q)
+                    if (col.ry_e[q] >= 0) cy.psi_e[par ^ 1][(int64_t)k * nry + col.ry_e[q]] = p.y[q];
+            }
+        }
+        *out_r = 0.f;
+        if (right_halo) {
+            float unused_x, unused_y;
+            *out_r = hz_centre(tab_a<LDS>(d, r.id_r), r.hz_r, r.ex_r - pickv<VEC>(r.ex, kr), r.ey_r - ey_below_r,
+                               idy_e_r, idx_e, xe.r >= 0, xe.b, xe.g, p.x_r, &unused_x, ry_e_r >= 0,
+                               ry_e_r >= 0 ? cy.b_e[ry_e_r] : 0.f, ry_e_r >= 0 ? cy.g_e[ry_e_r] : 0.f, p.y_r,
+                               &unused_y);
+        }
+    }
+
+    // one row i: psi reads of H_z row i+1 and of the faces of row i, the prefetch of row i+2, then H_z of row i+1
+    // (from nxt) and the faces of row i (from cur)
+    __device__ __forceinline__ void step(int i, const HzIn<VEC>& cur, const HzIn<VEC>& nxt, HzIn<VEC>& pre)
+    {
+        const yee_cpml_axis& cx = d.cpml[0];
+        const yee_cpml_axis& cy = d.cpml[1];
+        const int64_t row = (int64_t)i * P;
+        const int k1 = row_after(i + 1);
+        const bool ey_updated = k1 >= 0;
+        const RowCpml xe = ey_updated ? row_cpml(cx, k1, true) : RowCpml{-1, 0.f, 0.f};
+        const RowCpml xh = row_cpml(cx, i, false);
+
+        // (1) psi of H_z row k1 and of the faces of row i
+        HzPsi<VEC> p;
+        if (ey_updated) load_hz_psi(k1, xe, p);
+        float psi_xh[VEC] = {}, psi_yh[VEC] = {};
+        if (xh.r >= 0 && ey_updated) ldv<VEC>(cx.psi_h[par] + (int64_t)xh.r * P + jb_c, psi_xh);
+        if (col.y_layer) {
+#pragma unroll
+            for (int q = 0; q < VEC; ++q)
+                if (col.ry_h[q] >= 0) psi_yh[q] = cy.psi_h[par][(int64_t)i * nry + col.ry_h[q]];
+        }
+        // (2) the row after the lookahead row
+        const int k2 = (i + 1 < i1) ? row_after(i + 2) : -1;
+        if (k2 >= 0) load(k2, pre);
+
+        // (3) H_z of row k1 (stored when inside the tile), then the faces of row i
+        float hz_next[VEC] = {}, hz_next_r = 0.f;
+        if (ey_updated) hz_row(k1, nxt, cur.ey, cur.ey_r, xe, p, i + 1 < i1, hz_next, &hz_next_r);
+
+        const float hz_sh = shfl_from_next(hz_cur[0]);
+        const int id_sh = __shfl_down(id_at(cur.ids, 0), 1, YEE_WAVE);
+        float ex_v[VEC], ey_v[VEC];
+        const float idx_h = d.inv_dx_h[i];
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) {
+            const int j = col.jb + q;
+            const int id_q = id_at(cur.ids, q);
+            // E_x on the hi-y face: H_z of the next column (+ y CPML 'h' rows, sign +)
+            ex_v[q] = j < ny ? cur.ex[q] : 0.f;
+            if (j < ny && (j < ny - 1 || py)) {
+                float hz_right;
+                int id_right;
+                if (q < VEC - 1 && j + 1 < ny) { hz_right = hz_cur[q + 1]; id_right = id_at(cur.ids, q + 1); }
+                else if (j + 1 < ny && lane != YEE_WAVE - 1) { hz_right = hz_sh; id_right = id_sh; }
+                else { hz_right = hz_cur_r; id_right = (int)cur.id_r; }   // tile edge or periodic wrap
+                float ca, cb;
+                face_coefficients<LDS>(d, id_q, id_right, 0, &ca, &cb);
+                const float dh = hz_right - hz_cur[q];
+                ex_v[q] = ca * cur.ex[q] + cb * dh * col.idy_h[q];
+                if (col.ry_h[q] >= 0) {
+                    psi_yh[q] = cy.b_h[col.ry_h[q]] * psi_yh[q] + cy.g_h[col.ry_h[q]] * dh;
+                    ex_v[q] += cb * psi_yh[q];
+                }
+            }
+            // E_y on the hi-x face: H_z of the next row (+ x CPML 'h' rows, sign -)
+            ey_v[q] = j < ny ? cur.ey[q] : 0.f;
+            if (ey_updated && j < ny) {
+                float ca, cb;
+                face_coefficients<LDS>(d, id_q, id_at(nxt.ids, q), 1, &ca, &cb);
+                const float dh = hz_next[q] - hz_cur[q];
+                ey_v[q] = ca * cur.ey[q] - cb * dh * idx_h;
+                if (xh.r >= 0) {
+                    psi_xh[q] = xh.b * psi_xh[q] + xh.g * dh;
+                    ey_v[q] -= cb * psi_xh[q];
+                }
             }
         }
         // sheet current on the hi faces at t + dt/2 (E_y for a line along y, E_x for a line along x)
         for (int s = 0; s < d.n_sources; ++s) {
             const yee_line_source& src = d.source[s];
-            if (!owner) continue;
             const float w = wave_value(src, n);
-            if (src.axis == 1) {
-                if (i == src.line && j >= src.lo && j < src.hi) ey_v += __fmul_rn(w, src.profile[j - src.lo]);
-            } else if (j == src.line && i >= src.lo && i < src.hi) {
-                ex_v += __fmul_rn(w, src.profile[i - src.lo]);
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) {
+                const int j = col.jb + q;
+                if (src.axis == 1) {
+                    if (i == src.line && j >= src.lo && j < src.hi) ey_v[q] += __fmul_rn(w, src.profile[j - src.lo]);
+                } else if (j == src.line && i >= src.lo && i < src.hi) {
+                    ex_v[q] += __fmul_rn(w, src.profile[i - src.lo]);
+                }
             }
         }
-        if (owner) {
-            ex_n[r0 + j] = ex_v;
-            ey_n[r0 + j] = ey_v;
+        // (4) stores
+        if (col.any) {
+            const int jb = col.jb;
+            stv<VEC>(d.field[par ^ 1][1] + row + jb, ex_v);
+            stv<VEC>(d.field[par ^ 1][2] + row + jb, ey_v);
+            if (xh.r >= 0 && ey_updated) stv<VEC>(cx.psi_h[par ^ 1] + (int64_t)xh.r * P + jb, psi_xh);
+            if (col.y_layer) {
+#pragma unroll
+                for (int q = 0; q < VEC; ++q) {
+                    const int j = jb + q;
+                    if (col.ry_h[q] >= 0 && (j < ny - 1 || py))
+                        cy.psi_h[par ^ 1][(int64_t)i * nry + col.ry_h[q]] = psi_yh[q];
+                }
+            }
         }
         if (sample) {
             for (int m = 0; m < d.n_monitors; ++m) {
                 const yee_flux_line& mon = d.monitor[m];
-                if (mon.normal == 0) {
-                    if (owner && i == mon.position && j >= mon.lo && j < mon.hi)
-                        dft_accumulate(mon, sample_index, j - mon.lo, ey_v, hz_cur);
-                } else if (owner && j == mon.position && i >= mon.lo && i < mon.hi) {
-                    dft_accumulate(mon, sample_index, i - mon.lo, ex_v, hz_cur);
+#pragma unroll
+                for (int q = 0; q < VEC; ++q) {
+                    const int j = col.jb + q;
+                    if (mon.normal == 0) {
+                        if (i == mon.position && j >= mon.lo && j < mon.hi)
+                            dft_accumulate(mon, sample_index, j - mon.lo, ey_v[q], hz_cur[q]);
+                    } else if (j == mon.position && i >= mon.lo && i < mon.hi) {
+                        dft_accumulate(mon, sample_index, i - mon.lo, ex_v[q], hz_cur[q]);
+                    }
                 }
             }
         }
-        hz_cur = hz_next;
-        ex_cur = ex_next;
-        ey_cur = ey_next;
-        id_cur = id_next;
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) hz_cur[q] = hz_next[q];
+        hz_cur_r = hz_next_r;
     }
-}
 
+    __device__ __forceinline__ void run(int tj, int ti)
+    {
+        lane = threadIdx.x & (YEE_WAVE - 1);
+        nx = d.nx; ny = d.ny; P = d.pitch;
+        px = d.periodic_x != 0; py = d.periodic_y != 0;
+        par = (int)(n & 1);
+        i0 = ti * d.rows_per_tile;
+        i1 = min(nx, i0 + d.rows_per_tile);
+        col.load(d, tj * VEC * YEE_WAVE + VEC * lane);
+        const int jb = col.jb;
+        jb_c = col.any ? jb : 0;
+        const yee_cpml_axis& cx = d.cpml[0];
+        const yee_cpml_axis& cy = d.cpml[1];
+        nry = cy.n_rows;
+
+        jl = jb > 0 ? jb - 1 : (py ? ny - 1 : -1);
+        left_halo = lane == 0 && col.any && jl >= 0;
+        const int last = min(jb + VEC - 1, ny - 1);
+        const bool wraps = py && last == ny - 1 && col.any;
+        jr = wraps ? 0 : jb + VEC;
+        kr = wraps ? last - jb : VEC - 1;
+        right_halo = col.any && (wraps || (lane == YEE_WAVE - 1 && jb + VEC < ny));
+        idy_e_r = 0.f;
+        ry_e_r = -1;
+        if (right_halo) {
+            idy_e_r = d.inv_dy_e[jr];
+            if (nry > 0) ry_e_r = cy.row_e[jr];
+        }
+        sample = d.n_monitors > 0 && ((n + 1) % d.sample_interval) == 0;
+        sample_index = (n + 1) / (d.sample_interval > 0 ? d.sample_interval : 1) - 1;
+
+        // prologue: row i0 (and E_y of row i0-1), H_z of row i0, the loads of row i0+1
+        HzIn<VEC> a, b, c;
+        load(i0, a);
+        float ey_below[VEC] = {};
+        float ey_below_r = 0.f;
+        if (i0 > 0 || px) {
+            const int ip = i0 > 0 ? i0 - 1 : nx - 1;
+            ldv<VEC>(d.field[par][2] + (int64_t)ip * P + jb_c, ey_below);
+            if (right_halo) ey_below_r = d.field[par][2][(int64_t)ip * P + jr];
+        }
+        {
+            const RowCpml xe = row_cpml(cx, i0, true);
+            HzPsi<VEC> p;
+            load_hz_psi(i0, xe, p);
+            hz_row(i0, a, ey_below, ey_below_r, xe, p, true, hz_cur, &hz_cur_r);
+        }
+        const int r1 = row_after(i0 + 1);
+        if (r1 >= 0) load(r1, b);
+
+        int i = i0;
+        for (; i + 2 < i1; i += 3) {
+            step(i, a, b, c);
+            step(i + 1, b, c, a);
+            step(i + 2, c, a, b);
+        }
+        if (i < i1) step(i, a, b, c);
+        if (i + 1 < i1) step(i + 1, b, c, a);
+    }
+};
+
+constexpr int POL_ANY = 2;   // polarisation read from the descriptor (batched launches mixing E_z and H_z)
+
+template <int POL, int VEC, bool LDS>
 __device__ __forceinline__ void tile_2d(const yee2d_desc& d, int local_tile, int64_t n)
 {
-    const int tj_count = tiles_j_2d(d.ny);
+    const int tj_count = tiles_j_2d(d.ny, VEC);
     const int tj = local_tile % tj_count;
     const int ti = local_tile / tj_count;
-    if (d.polarization == YEE_EZ)
-        ez_tile(d, tj, ti, n);
-    else
-        hz_tile(d, tj, ti, n);
+    if (POL == YEE_EZ || (POL == POL_ANY && d.polarization == YEE_EZ)) {
+        EzTile<VEC, LDS> t(d, n);
+        t.run(tj, ti);
+    } else {
+        HzTile<VEC, LDS> t(d, n);
+        t.run(tj, ti);
+    }
 }
 
 }  // namespace yee
 
-// One instance: grid = ceil(tiles / 4) blocks of 4 independent wavefront tiles.
+// One instance: grid = ceil(tiles / 4) blocks of 4 independent wavefront tiles; one kernel per polarisation and
+// columns-per-lane. The block stages the coefficient table in LDS first (when it fits).
+template <int POL, int VEC>
 __global__ void __launch_bounds__(256) yee2d_step_kernel(yee2d_desc d, int32_t offset)
 {
+    const bool lds = yee::stage_tables<POL>(d);
     const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
-    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int total = yee::tiles_j_2d(d.ny) * yee::tiles_i_2d(d.nx, d.rows_per_tile);
+    const int tile = blockIdx.x * 4 + yee::wave_index();
+    const int total = yee::tiles_j_2d(d.ny, VEC) * yee::tiles_i_2d(d.nx, d.rows_per_tile);
     if (tile >= total) return;
-    yee::tile_2d(d, tile, n);
+    if (lds)
+        yee::tile_2d<POL, VEC, true>(d, tile, n);
+    else
+        yee::tile_2d<POL, VEC, false>(d, tile, n);
 }
 
-// Many instances in one launch: tile t belongs to the instance whose tile range [prefix[k], prefix[k+1]) holds it.
+template __global__ void yee2d_step_kernel<YEE_EZ, 2>(yee2d_desc, int32_t);
+template __global__ void yee2d_step_kernel<YEE_EZ, 4>(yee2d_desc, int32_t);
+template __global__ void yee2d_step_kernel<YEE_HZ, 2>(yee2d_desc, int32_t);
+template __global__ void yee2d_step_kernel<YEE_HZ, 4>(yee2d_desc, int32_t);
+
+// Many instances (E_z and H_z mixed) in one launch: tile t belongs to the instance whose tile range
+// [prefix[k], prefix[k+1]) holds it. Tables are read from global memory (they stay in L1/L2).
+template <int VEC>
 __global__ void __launch_bounds__(256) yee2d_batch_kernel(const yee2d_desc* __restrict__ descs,
                                                          const int32_t* __restrict__ prefix, int32_t n_inst,
                                                          int32_t offset)
 {
-    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int tile = blockIdx.x * 4 + yee::wave_index();
     if (tile >= prefix[n_inst]) return;
     int lo = 0, hi = n_inst;
     while (hi - lo > 1) {
@@ -421,8 +858,13 @@ __global__ void __launch_bounds__(256) yee2d_batch_kernel(const yee2d_desc* __re
     const yee2d_desc& d = descs[lo];
     const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
-    yee::tile_2d(d, tile - prefix[lo], n);
+    yee::tile_2d<yee::POL_ANY, VEC, false>(d, tile - prefix[lo], n);
 }
+
+template __global__ void yee2d_batch_kernel<2>(const yee2d_desc* __restrict__, const int32_t* __restrict__, int32_t,
+                                              int32_t);
+template __global__ void yee2d_batch_kernel<4>(const yee2d_desc* __restrict__, const int32_t* __restrict__, int32_t,
+                                              int32_t);
 
 // Advance one clock by k steps (bounded by the instance's step limit); a single lane writes with a vector store.
 __global__ void yee_clock_advance(int64_t* clock, int64_t k, int64_t limit)

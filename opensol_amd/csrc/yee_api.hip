@@ -11,7 +11,9 @@
 
 #include "yee.h"
 
+template <int POL, int VEC>
 __global__ void yee2d_step_kernel(yee2d_desc d, int32_t offset);
+template <int VEC>
 __global__ void yee2d_batch_kernel(const yee2d_desc* __restrict__ descs, const int32_t* __restrict__ prefix,
                                    int32_t n_inst, int32_t offset);
 __global__ void yee_clock_advance(int64_t* clock, int64_t k, int64_t limit);
@@ -34,8 +36,7 @@ int32_t fail(int32_t code, const std::string& message)
             return fail(YEE_EHIP, std::string(#call) + ": " + hipGetErrorString(err_));                \
     } while (0)
 
-constexpr int kDefaultRows2D = 64;
-constexpr int kOwn = 63;   // owned columns per wavefront tile (yee2d.hip)
+constexpr int kMinAutoRows = 8;   // shortest tile picked automatically (each tile re-reads one edge row)
 
 enum CtxKind { kSingle2D = 1, kBatch2D = 2 };
 
@@ -64,8 +65,10 @@ int32_t check_2d(const yee2d_desc* d)
 {
     if (!d) return fail(YEE_EINVAL, "null descriptor");
     if (d->nx < 2 || d->ny < 2) return fail(YEE_EINVAL, "2D grid needs nx, ny >= 2");
+    if (d->pitch < d->ny || d->pitch % 4 != 0) return fail(YEE_EINVAL, "pitch must be a multiple of 4 and >= ny");
     if (d->polarization != YEE_EZ && d->polarization != YEE_HZ) return fail(YEE_EINVAL, "polarization must be 0 or 1");
     if (d->rows_per_tile < 1) return fail(YEE_EINVAL, "rows_per_tile must be >= 1");
+    if (d->lane_columns != 2 && d->lane_columns != 4) return fail(YEE_EINVAL, "lane_columns must be 2 or 4");
     for (int s = 0; s < 2; ++s)
         for (int c = 0; c < 3; ++c)
             if (!d->field[s][c]) return fail(YEE_EINVAL, "null field buffer");
@@ -108,11 +111,50 @@ int32_t check_2d(const yee2d_desc* d)
     return YEE_OK;
 }
 
+int64_t tiles_j(const yee2d_desc& d)
+{
+    const int cols = 64 * d.lane_columns;
+    return (d.ny + cols - 1) / cols;
+}
+
 int64_t tiles_2d(const yee2d_desc& d)
 {
-    const int64_t tj = (d.ny + kOwn - 1) / kOwn;
-    const int64_t ti = (d.nx + d.rows_per_tile - 1) / d.rows_per_tile;
-    return tj * ti;
+    return tiles_j(d) * ((d.nx + d.rows_per_tile - 1) / d.rows_per_tile);
+}
+
+using StepKernel = void (*)(yee2d_desc, int32_t);
+
+StepKernel step_kernel(const yee2d_desc& d)
+{
+    if (d.polarization == YEE_EZ) return d.lane_columns == 2 ? yee2d_step_kernel<YEE_EZ, 2> : yee2d_step_kernel<YEE_EZ, 4>;
+    return d.lane_columns == 2 ? yee2d_step_kernel<YEE_HZ, 2> : yee2d_step_kernel<YEE_HZ, 4>;
+}
+
+// Wavefront slots of the current device for a 256-thread kernel: resident blocks per CU x CUs x 4 waves.
+int64_t wave_slots(const void* kernel)
+{
+    int device = 0, cus = 0, blocks = 0;
+    if (hipGetDevice(&device) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, 256, 0) != hipSuccess) return 0;
+    return (int64_t)cus * blocks * 4;
+}
+
+// Defaults of a 2D descriptor: 4 columns per lane; rows_per_tile so the tiles fill the device's wave slots in whole
+// rounds (one wavefront tile per slot when the grid is large enough).
+void apply_defaults(yee2d_desc& d)
+{
+    if (d.lane_columns == 0) d.lane_columns = 4;
+    if (d.rows_per_tile == 0 && d.nx > 0 && d.ny > 0 && (d.lane_columns == 2 || d.lane_columns == 4)) {
+        const int64_t slots = wave_slots(reinterpret_cast<const void*>(step_kernel(d)));
+        const int64_t columns = tiles_j(d);
+        int64_t rows = slots > 0 ? (d.nx * columns + slots - 1) / slots : 64;
+        if (rows < kMinAutoRows) rows = kMinAutoRows;
+        if (rows > d.nx) rows = d.nx;
+        // even out the tiles: the same count with the shortest rows
+        const int64_t tiles_x = (d.nx + rows - 1) / rows;
+        d.rows_per_tile = (int32_t)((d.nx + tiles_x - 1) / tiles_x);
+    }
 }
 
 void drop_graph(yee_ctx* ctx)
@@ -128,8 +170,9 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
 {
     if (count <= 0) return YEE_OK;
     if (ctx->kind == kSingle2D) {
+        const StepKernel kernel = step_kernel(ctx->d2);
         for (int32_t k = 0; k < count; ++k)
-            hipLaunchKernelGGL(yee2d_step_kernel, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d2, k);
+            hipLaunchKernelGGL(kernel, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d2, k);
         YEE_HIP(hipGetLastError());
         hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, stream, ctx->d2.clock, (int64_t)count,
                            ctx->d2.step_limit);
@@ -137,9 +180,10 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
         return YEE_OK;
     }
     if (ctx->kind == kBatch2D) {
+        auto kernel = ctx->descs[0].lane_columns == 2 ? yee2d_batch_kernel<2> : yee2d_batch_kernel<4>;
         for (int32_t k = 0; k < count; ++k)
-            hipLaunchKernelGGL(yee2d_batch_kernel, dim3(ctx->blocks), dim3(256), 0, stream, ctx->descs_dev,
-                               ctx->prefix_dev, ctx->n_inst, k);
+            hipLaunchKernelGGL(kernel, dim3(ctx->blocks), dim3(256), 0, stream, ctx->descs_dev, ctx->prefix_dev,
+                               ctx->n_inst, k);
         YEE_HIP(hipGetLastError());
         const int threads = 256;
         hipLaunchKernelGGL(yee_clock_advance_batch, dim3((ctx->n_inst + threads - 1) / threads), dim3(threads), 0,
@@ -195,7 +239,7 @@ int32_t yee2d_create(const yee2d_desc* desc, yee_ctx** out)
     if (!out) return fail(YEE_EINVAL, "null output pointer");
     *out = nullptr;
     yee2d_desc d = desc ? *desc : yee2d_desc{};
-    if (desc && d.rows_per_tile == 0) d.rows_per_tile = kDefaultRows2D;
+    if (desc) apply_defaults(d);
     int32_t status = check_2d(desc ? &d : nullptr);
     if (status != YEE_OK) return status;
     yee_ctx* ctx = new yee_ctx();
@@ -211,7 +255,7 @@ int32_t yee2d_update(yee_ctx* ctx, const yee2d_desc* desc)
 {
     if (!ctx || ctx->kind != kSingle2D) return fail(YEE_ESTATE, "not a single 2D context");
     yee2d_desc d = desc ? *desc : yee2d_desc{};
-    if (desc && d.rows_per_tile == 0) d.rows_per_tile = kDefaultRows2D;
+    if (desc) apply_defaults(d);
     int32_t status = check_2d(desc ? &d : nullptr);
     if (status != YEE_OK) return status;
     drop_graph(ctx);
@@ -229,8 +273,11 @@ int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, 
     std::vector<yee2d_desc> host(descs, descs + n);
     std::vector<int32_t> prefix(n + 1, 0);
     for (int32_t k = 0; k < n; ++k) {
-        if (host[k].rows_per_tile == 0) host[k].rows_per_tile = kDefaultRows2D;
+        if (host[k].lane_columns == 0) host[k].lane_columns = host[0].lane_columns ? host[0].lane_columns : 4;
+        if (host[k].rows_per_tile == 0) host[k].rows_per_tile = 32;
         int32_t status = check_2d(&host[k]);
+        if (status == YEE_OK && host[k].lane_columns != host[0].lane_columns)
+            status = fail(YEE_EINVAL, "batched instances must share lane_columns");
         if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
         const int64_t next = prefix[k] + tiles_2d(host[k]);
         if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
@@ -256,6 +303,13 @@ int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, 
 size_t yee2d_batch_bytes(int32_t n)
 {
     return n < 1 ? 0 : sizeof(yee2d_desc) * (size_t)n + sizeof(int32_t) * (size_t)(n + 1);
+}
+
+int32_t yee2d_get_desc(const yee_ctx* ctx, yee2d_desc* out)
+{
+    if (!ctx || ctx->kind != kSingle2D || !out) return fail(YEE_ESTATE, "not a single 2D context");
+    *out = ctx->d2;
+    return YEE_OK;
 }
 
 int32_t yee_set_graph_steps(yee_ctx* ctx, int32_t graph_steps)

@@ -237,7 +237,7 @@ def steps_for_duration(time, dt, duration):
 class Engine2D:
     """Device state and libyee context of one Model2D. torch provides the HBM buffers and the stream."""
 
-    def __init__(self, model, device=None, rows_per_tile=64, graph_steps=0, capacity=1024):
+    def __init__(self, model, device=None, rows_per_tile=0, graph_steps=0, capacity=1024, lane_columns=4):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError('Engine2D needs a ROCm GPU (torch.cuda.is_available() is False); there is no CPU path')
@@ -246,13 +246,15 @@ class Engine2D:
         self.model = model
         self.device = torch.device('cuda', torch.cuda.current_device() if device is None else device)
         self.rows_per_tile = int(rows_per_tile)
+        self.lane_columns = int(lane_columns)
         self.graph_steps = int(graph_steps)
         self.n_step = 0
         self.capacity = 0
         self._times = None
         nx, ny = model.shape
+        self.pitch = pitch = -(-ny // 4) * 4   # rows padded to whole 16 B groups; padding columns stay 0
         dev, f32 = self.device, torch.float32
-        self.fields = torch.zeros((2, 3, nx, ny), dtype=f32, device=dev)
+        self.fields = torch.zeros((2, 3, nx, pitch), dtype=f32, device=dev)
         self.clock = torch.zeros(1, dtype=torch.int64, device=dev)
         self.inv = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in
                     (model.inv_dx_e, model.inv_dx_h, model.inv_dy_e, model.inv_dy_h)]
@@ -262,7 +264,7 @@ class Engine2D:
             if tables.n_rows == 0:
                 self.cpml[axis] = None
                 continue
-            shape = (2, tables.n_rows, other) if axis == 'x' else (2, nx, tables.n_rows)
+            shape = (2, tables.n_rows, pitch) if axis == 'x' else (2, nx, tables.n_rows)
             self.cpml[axis] = dict(
                 row_e=torch.from_numpy(tables.row_e).to(dev), row_h=torch.from_numpy(tables.row_h).to(dev),
                 b_e=torch.from_numpy(tables.b_e).to(dev), g_e=torch.from_numpy(tables.g_e).to(dev),
@@ -284,8 +286,9 @@ class Engine2D:
     # ---- device tables --------------------------------------------------------------------------------------------
     def _upload_materials(self):
         torch, model = self.torch, self.model
-        ids = np.ascontiguousarray(model.materials.ids).view(np.int16)
-        self.material = torch.from_numpy(ids).to(self.device)
+        ids = np.zeros((model.nx, self.pitch), dtype=np.uint16)
+        ids[:, :model.ny] = model.materials.ids
+        self.material = torch.from_numpy(ids.view(np.int16)).to(self.device)
         if model.polarization == 'ez':
             self.table = torch.from_numpy(model.materials.table_ez(model.dt)).to(self.device)
             self.table2 = None
@@ -317,10 +320,11 @@ class Engine2D:
         """The yee2d_desc of the current buffers (for the single-instance context or a batch)."""
         model = self.model
         d = _lib.Desc2DC()
-        d.nx, d.ny = model.nx, model.ny
+        d.nx, d.ny, d.pitch = model.nx, model.ny, self.pitch
         d.polarization = _lib.YEE_EZ if model.polarization == 'ez' else _lib.YEE_HZ
         d.periodic_x, d.periodic_y = int(model.bc.periodic('x')), int(model.bc.periodic('y'))
         d.rows_per_tile = self.rows_per_tile
+        d.lane_columns = self.lane_columns
         for s in range(2):
             for c in range(3):
                 d.field[s][c] = self.fields[s, c].data_ptr()
@@ -373,6 +377,12 @@ class Engine2D:
         else:
             _lib.check(self.lib.yee2d_update(self._ctx, ctypes_byref(d)), 'yee2d_update')
 
+    def launch_geometry(self):
+        """(rows_per_tile, lane_columns) the library resolved for this instance."""
+        d = _lib.Desc2DC()
+        _lib.check(self.lib.yee2d_get_desc(self._ctx, ctypes_byref(d)), 'yee2d_get_desc')
+        return d.rows_per_tile, d.lane_columns
+
     def materials_changed(self):
         """Re-upload the id grid and coefficient tables after the model's materials changed."""
         self._upload_materials()
@@ -405,11 +415,12 @@ class Engine2D:
     def field(self, name):
         """Host float32 copy of a field component of the current state set."""
         index = self.model.components.index(name)
-        return self.fields[self.n_step & 1, index].cpu().numpy()
+        return self.fields[self.n_step & 1, index, :, :self.model.ny].cpu().numpy()
 
     def set_field(self, name, values):
         index = self.model.components.index(name)
-        self.fields[self.n_step & 1, index].copy_(self.torch.as_tensor(np.asarray(values, dtype=np.float32)))
+        self.fields[self.n_step & 1, index, :, :self.model.ny].copy_(
+            self.torch.as_tensor(np.asarray(values, dtype=np.float32)))
 
     def read_accumulator(self, index):
         return self.accumulators[index].cpu().numpy()

@@ -24,7 +24,7 @@ class Simulation:
 
     def __init__(self, size_x: float = None, size_y: float = None, dx: float = None, npml: int = 15,
                  courant: float = None, boundaries=None, *, polarization: str = 'ez', device=None,
-                 rows_per_tile: int = 64, graph_steps: int = 0):
+                 rows_per_tile: int = 0, graph_steps: int = 0, lane_columns: int = 4):
         if size_x is None or size_y is None or dx is None:
             raise ValueError('Simulation() needs (size_x, size_y, dx, ...) for the 2D EM-FDTD path')
         self.model = Model2D(size_x, size_y, dx, npml=npml, courant=courant, boundaries=boundaries,
@@ -33,7 +33,8 @@ class Simulation:
         self.boundary_config = self.model.bc
         self.polarization = polarization
         self.engine = None
-        self._engine_options = dict(device=device, rows_per_tile=rows_per_tile, graph_steps=graph_steps)
+        self._engine_options = dict(device=device, rows_per_tile=rows_per_tile, graph_steps=graph_steps,
+                                    lane_columns=lane_columns)
         self._materials_version = None
         self.callbacks = []
 
