@@ -110,7 +110,7 @@ def main():
     parser.add_argument('--warmup', type=int, default=50)
     parser.add_argument('--graph-steps', type=int, default=50)
     parser.add_argument('--rows-per-tile', type=int, default=0, help='0: fill the device wave slots')
-    parser.add_argument('--lane-columns', type=int, default=4, choices=(2, 4))
+    parser.add_argument('--lane-columns', type=int, default=0, choices=(0, 2, 4), help='0: library default')
     parser.add_argument('--cpu-steps', type=int, default=8, help='steps of the CPU baseline sample (0: skip)')
     args = parser.parse_args()
     if args.gpus != 1:

@@ -116,6 +116,12 @@ typedef struct yee2d_desc {
     const float* inv_dx_h;   /* (nx) 1/spacing at hi faces (last: periodic wrap) */
     const float* inv_dy_e;   /* (ny) */
     const float* inv_dy_h;   /* (ny) */
+    float inv_dx_bulk;       /* inv_d* outside the CPML layers (uniform grid) */
+    float inv_dy_bulk;
+    int32_t layer_x_lo;      /* rows [0, layer_x_lo) and [layer_x_hi, nx) hold every x CPML row (both kinds) */
+    int32_t layer_x_hi;
+    int32_t layer_y_lo;      /* columns, likewise for y */
+    int32_t layer_y_hi;
     yee_cpml_axis cpml[2];   /* x, y */
     int32_t n_sources;
     int32_t n_monitors;

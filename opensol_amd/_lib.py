@@ -3,7 +3,9 @@ mismatched library raises here."""
 import ctypes
 import os
 
-LIBRARY_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libyee.so')
+# OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
+LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                                     'libyee.so')
 ABI_VERSION = 1
 MAX_SOURCES = 4
 MAX_MONITORS = 4
@@ -34,7 +36,8 @@ class Desc2DC(ctypes.Structure):
                 ('periodic_y', c_i32), ('rows_per_tile', c_i32), ('lane_columns', c_i32), ('field', (c_ptr * 3) * 2),
                 ('material', c_ptr), ('table', c_ptr), ('table2', c_ptr), ('n_materials', c_i32), ('dt', c_f32), ('dt_over_eps0', c_f32), ('pad_', c_i32),
                 ('inv_dx_e', c_ptr), ('inv_dx_h', c_ptr), ('inv_dy_e', c_ptr), ('inv_dy_h', c_ptr),
-                ('cpml', CpmlAxisC * 2), ('n_sources', c_i32), ('n_monitors', c_i32), ('sample_interval', c_i32),
+                ('inv_dx_bulk', c_f32), ('inv_dy_bulk', c_f32), ('layer_x_lo', c_i32), ('layer_x_hi', c_i32),
+                ('layer_y_lo', c_i32), ('layer_y_hi', c_i32), ('cpml', CpmlAxisC * 2), ('n_sources', c_i32), ('n_monitors', c_i32), ('sample_interval', c_i32),
                 ('pad2_', c_i32), ('source', LineSourceC * MAX_SOURCES), ('monitor', FluxLineC * MAX_MONITORS),
                 ('clock', c_ptr), ('step_limit', c_i64)]
 

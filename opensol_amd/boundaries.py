@@ -193,6 +193,16 @@ class CPMLAxisTables:
         self.b_e, self.g_e = b_e.astype(DTYPE), g_e.astype(DTYPE)
         self.b_h, self.g_h = b_h.astype(DTYPE), g_h.astype(DTYPE)
 
+    def layer_bounds(self):
+        """(lo, hi): every row with a psi row of either kind lies in [0, lo) or [hi, n) (lo side: rows 0..npml,
+        hi side: rows n-npml-1..n-2); (0, n) without layers."""
+        rows = np.flatnonzero((self.row_e >= 0) | (self.row_h >= 0))
+        lo_rows = rows[rows < self.n // 2]
+        hi_rows = rows[rows >= self.n // 2]
+        lo = int(lo_rows.max()) + 1 if lo_rows.size else 0
+        hi = int(hi_rows.min()) if hi_rows.size else self.n
+        return lo, hi
+
 
 def curl_spacings(n, dx, cpml):
     """float32 (inv_e, inv_h) of one axis: 1/w at centres, 1/mean(w_i, w_(i+1)) at hi faces (last = periodic wrap),

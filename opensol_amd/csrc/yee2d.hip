@@ -161,6 +161,12 @@ __device__ __forceinline__ float ez_hx(float4 c, float hx, float de, float idy, 
     return v;
 }
 
+// E_z at a centre = ca E_z + cb (dH_y/dx - dH_x/dy), before its CPML corrections.
+__device__ __forceinline__ float ez_centre(float4 c, float e, float dhy, float dhx, float idx, float idy)
+{
+    return c.x * e + c.y * (dhy * idx - dhx * idy);
+}
+
 // H_z polarisation, c = (da, db, eps_xx, eps_yy): H_z at a centre = da H_z + db (dE_x/dy - dE_y/dx)
 // (x CPML: H_z -= db psi_x, y CPML: H_z += db psi_y).
 __device__ __forceinline__ float hz_centre(float4 c, float hz, float dex, float dey, float idy, float idx,
@@ -180,6 +186,10 @@ __device__ __forceinline__ float hz_centre(float4 c, float hz, float dex, float 
     }
     return v;
 }
+
+// E_x on a hi-y face = ca E_x + cb dH_z/dy; E_y on a hi-x face = ca E_y - cb dH_z/dx (before CPML).
+__device__ __forceinline__ float hz_ex(float ca, float cb, float ex, float dh, float idy) { return ca * ex + cb * dh * idy; }
+__device__ __forceinline__ float hz_ey(float ca, float cb, float ey, float dh, float idx) { return ca * ey - cb * dh * idx; }
 
 // E face coefficients between cells of ids a and b (comp 0: E_x face, 1: E_y face). Same id: the table's FP64
 // coefficients; mixed: mean eps / sigma (shares 1/2, fdtd_hz_2d_kernels.py:190-278) formed here in float32.
@@ -379,7 +389,7 @@ struct EzTile {
             const int j = col.jb + k;
             const float dhy = hy_v[k] - hy_prev[k];
             const float dhx = hx_v[k] - (k == 0 ? hx_left : hx_v[k - 1]);
-            float v = c[k].x * e_cur[k] + c[k].y * (dhy * idx_e - dhx * col.idy_e[k]);
+            float v = ez_centre(c[k], e_cur[k], dhy, dhx, idx_e, col.idy_e[k]);
             if (xe.r >= 0) {
                 psi_xe[k] = xe.b * psi_xe[k] + xe.g * dhy;
                 v += c[k].y * psi_xe[k];
@@ -442,14 +452,12 @@ struct EzTile {
         }
     }
 
-    __device__ __forceinline__ void run(int tj, int ti)
+    __device__ __forceinline__ void run(int tj, int i0, int i1)
     {
         lane = threadIdx.x & (YEE_WAVE - 1);
         nx = d.nx; ny = d.ny; P = d.pitch;
         px = d.periodic_x != 0; py = d.periodic_y != 0;
         par = (int)(n & 1);
-        const int i0 = ti * d.rows_per_tile;
-        const int i1 = min(nx, i0 + d.rows_per_tile);
         col.load(d, tj * VEC * YEE_WAVE + VEC * lane);
         const int jb = col.jb;
         jb_c = col.any ? jb : 0;
@@ -671,7 +679,7 @@ struct HzTile {
                 float ca, cb;
                 face_coefficients<LDS>(d, id_q, id_right, 0, &ca, &cb);
                 const float dh = hz_right - hz_cur[q];
-                ex_v[q] = ca * cur.ex[q] + cb * dh * col.idy_h[q];
+                ex_v[q] = hz_ex(ca, cb, cur.ex[q], dh, col.idy_h[q]);
                 if (col.ry_h[q] >= 0) {
                     psi_yh[q] = cy.b_h[col.ry_h[q]] * psi_yh[q] + cy.g_h[col.ry_h[q]] * dh;
                     ex_v[q] += cb * psi_yh[q];
@@ -683,7 +691,7 @@ struct HzTile {
                 float ca, cb;
                 face_coefficients<LDS>(d, id_q, id_at(nxt.ids, q), 1, &ca, &cb);
                 const float dh = hz_next[q] - hz_cur[q];
-                ey_v[q] = ca * cur.ey[q] - cb * dh * idx_h;
+                ey_v[q] = hz_ey(ca, cb, cur.ey[q], dh, idx_h);
                 if (xh.r >= 0) {
                     psi_xh[q] = xh.b * psi_xh[q] + xh.g * dh;
                     ey_v[q] -= cb * psi_xh[q];
@@ -739,14 +747,14 @@ struct HzTile {
         hz_cur_r = hz_next_r;
     }
 
-    __device__ __forceinline__ void run(int tj, int ti)
+    __device__ __forceinline__ void run(int tj, int row0, int row1)
     {
         lane = threadIdx.x & (YEE_WAVE - 1);
         nx = d.nx; ny = d.ny; P = d.pitch;
         px = d.periodic_x != 0; py = d.periodic_y != 0;
         par = (int)(n & 1);
-        i0 = ti * d.rows_per_tile;
-        i1 = min(nx, i0 + d.rows_per_tile);
+        i0 = row0;
+        i1 = row1;
         col.load(d, tj * VEC * YEE_WAVE + VEC * lane);
         const int jb = col.jb;
         jb_c = col.any ? jb : 0;
@@ -800,46 +808,413 @@ struct HzTile {
     }
 };
 
+// ------------------------------------------------------------------------------------------------- bulk tiles
+// A tile whose rows and columns (with the halo row/column on each side) lie outside every CPML layer, and that no
+// source or monitor line crosses (the host lists such tiles), runs these lean loops: bulk spacings as scalars, no
+// CPML, source or monitor code, seven live pointers. They evaluate the same helpers with the same operands as the
+// general path, so both classes of tile produce identical bits.
+template <int VEC>
+struct EzBulkIn {
+    float e_next[VEC], hx[VEC], hy[VEC];
+    uint32_t ids[VEC / 2];
+    float e_l, hx_l, e_r;
+    uint32_t id_l;
+};
+
+template <int VEC, bool LDS>
+__device__ __forceinline__ void ez_bulk_tile(const yee2d_desc& d, int tj, int ti, int64_t n)
+{
+    const int lane = threadIdx.x & (YEE_WAVE - 1);
+    const int nx = d.nx, ny = d.ny, P = d.pitch;
+    const bool px = d.periodic_x != 0, py = d.periodic_y != 0;
+    const int par = (int)(n & 1);
+    const int i0 = ti * d.rows_per_tile;
+    const int i1 = min(nx, i0 + d.rows_per_tile);
+    const int jb = tj * VEC * YEE_WAVE + VEC * lane;
+    const bool any = jb < ny;
+    const int jb_c = any ? jb : 0;
+    const float* __restrict__ e_o = d.field[par][0];
+    const float* __restrict__ hx_o = d.field[par][1];
+    const float* __restrict__ hy_o = d.field[par][2];
+    float* __restrict__ e_n = d.field[par ^ 1][0];
+    float* __restrict__ hx_n = d.field[par ^ 1][1];
+    float* __restrict__ hy_n = d.field[par ^ 1][2];
+    const uint16_t* __restrict__ mat = d.material;
+    const float idx = d.inv_dx_bulk, idy = d.inv_dy_bulk;
+    const int jl = jb > 0 ? jb - 1 : (py ? ny - 1 : -1);
+    const bool left_halo = lane == 0 && any && jl >= 0;
+    const int last = min(jb + VEC - 1, ny - 1);
+    const bool wraps = py && last == ny - 1 && any;
+    const int jr = wraps ? 0 : jb + VEC;
+    const bool right_halo = any && (wraps || (lane == YEE_WAVE - 1 && jb + VEC < ny));
+
+    auto load = [&](int i, EzBulkIn<VEC>& r) {
+        const bool hy_updated = (i < nx - 1) || px;
+        const int inext = (i + 1 < nx) ? i + 1 : 0;
+        const int64_t row = (int64_t)i * P;
+        if (hy_updated) ldv<VEC>(e_o + (int64_t)inext * P + jb_c, r.e_next);
+        else {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) r.e_next[k] = 0.f;
+        }
+        ldv<VEC>(hx_o + row + jb_c, r.hx);
+        ldv<VEC>(hy_o + row + jb_c, r.hy);
+        ldids<VEC>(mat + row + jb_c, r.ids);
+        r.e_l = r.hx_l = r.e_r = 0.f;
+        r.id_l = 0;
+        if (left_halo) { r.e_l = e_o[row + jl]; r.hx_l = hx_o[row + jl]; r.id_l = mat[row + jl]; }
+        if (right_halo) r.e_r = e_o[row + jr];
+    };
+
+    float e_cur[VEC], hy_prev[VEC];
+    ldv<VEC>(e_o + (int64_t)i0 * P + jb_c, e_cur);
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) hy_prev[k] = 0.f;
+    if ((i0 > 0 || px) && any) {
+        const int ip = i0 > 0 ? i0 - 1 : nx - 1;
+        const int64_t p0 = (int64_t)ip * P + jb_c;
+        float e_p[VEC], hy_o_p[VEC];
+        uint32_t idp[VEC / 2];
+        ldv<VEC>(e_o + p0, e_p);
+        ldv<VEC>(hy_o + p0, hy_o_p);
+        ldids<VEC>(mat + p0, idp);
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            float unused;
+            hy_prev[k] = ez_hy(tab_a<LDS>(d, id_at(idp, k)), hy_o_p[k], e_cur[k] - e_p[k], idx, false, 0.f, 0.f,
+                               0.f, &unused);
+        }
+    }
+
+    auto step = [&](int i, const EzBulkIn<VEC>& cur, EzBulkIn<VEC>& nxt, bool prefetch) {
+        if (prefetch) load(i + 1, nxt);
+        const int64_t row = (int64_t)i * P;
+        const bool hy_updated = (i < nx - 1) || px;
+        float4 c[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) c[k] = tab_a<LDS>(d, id_at(cur.ids, k));
+        const float e_sh = shfl_from_next(e_cur[0]);
+        float hy_v[VEC], hx_v[VEC], ez_v[VEC];
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const int j = jb + k;
+            float e_right;
+            if (k < VEC - 1 && j + 1 < ny) e_right = e_cur[k + 1];
+            else if (j + 1 < ny) e_right = lane == YEE_WAVE - 1 ? cur.e_r : e_sh;
+            else e_right = cur.e_r;
+            float unused;
+            hy_v[k] = hy_updated ? ez_hy(c[k], cur.hy[k], cur.e_next[k] - e_cur[k], idx, false, 0.f, 0.f, 0.f,
+                                         &unused)
+                                 : cur.hy[k];
+            hx_v[k] = j < ny ? cur.hx[k] : 0.f;
+            if (j < ny && (j < ny - 1 || py))
+                hx_v[k] = ez_hx(c[k], cur.hx[k], e_right - e_cur[k], idy, false, 0.f, 0.f, 0.f, &unused);
+        }
+        float hx_left = shfl_from_prev(hx_v[VEC - 1]);
+        if (left_halo) {
+            float unused;
+            hx_left = ez_hx(tab_a<LDS>(d, cur.id_l), cur.hx_l, e_cur[0] - cur.e_l, idy, false, 0.f, 0.f, 0.f,
+                            &unused);
+        } else if (lane == 0) {
+            hx_left = 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const float dhx = hx_v[k] - (k == 0 ? hx_left : hx_v[k - 1]);
+            ez_v[k] = jb + k < ny ? ez_centre(c[k], e_cur[k], hy_v[k] - hy_prev[k], dhx, idx, idy) : 0.f;
+        }
+        if (any) {
+            stv<VEC>(e_n + row + jb, ez_v);
+            stv<VEC>(hx_n + row + jb, hx_v);
+            stv<VEC>(hy_n + row + jb, hy_v);
+        }
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            hy_prev[k] = hy_v[k];
+            e_cur[k] = cur.e_next[k];
+        }
+    };
+
+    EzBulkIn<VEC> a, b;
+    load(i0, a);
+    int i = i0;
+    for (; i + 1 < i1; i += 2) {
+        step(i, a, b, true);
+        step(i + 1, b, a, i + 2 < i1);
+    }
+    if (i < i1) step(i, a, b, false);
+}
+
+template <int VEC>
+struct HzBulkIn {
+    float hz[VEC], ex[VEC], ey[VEC];
+    uint32_t ids[VEC / 2];
+    float ex_l, hz_r, ex_r, ey_r;
+    uint32_t id_r;
+};
+
+template <int VEC, bool LDS>
+__device__ __forceinline__ void hz_bulk_tile(const yee2d_desc& d, int tj, int ti, int64_t n)
+{
+    const int lane = threadIdx.x & (YEE_WAVE - 1);
+    const int nx = d.nx, ny = d.ny, P = d.pitch;
+    const bool px = d.periodic_x != 0, py = d.periodic_y != 0;
+    const int par = (int)(n & 1);
+    const int i0 = ti * d.rows_per_tile;
+    const int i1 = min(nx, i0 + d.rows_per_tile);
+    const int jb = tj * VEC * YEE_WAVE + VEC * lane;
+    const bool any = jb < ny;
+    const int jb_c = any ? jb : 0;
+    const float* __restrict__ hz_o = d.field[par][0];
+    const float* __restrict__ ex_o = d.field[par][1];
+    const float* __restrict__ ey_o = d.field[par][2];
+    float* __restrict__ hz_n = d.field[par ^ 1][0];
+    float* __restrict__ ex_n = d.field[par ^ 1][1];
+    float* __restrict__ ey_n = d.field[par ^ 1][2];
+    const uint16_t* __restrict__ mat = d.material;
+    const float idx = d.inv_dx_bulk, idy = d.inv_dy_bulk;
+    const int jl = jb > 0 ? jb - 1 : (py ? ny - 1 : -1);
+    const bool left_halo = lane == 0 && any && jl >= 0;
+    const int last = min(jb + VEC - 1, ny - 1);
+    const bool wraps = py && last == ny - 1 && any;
+    const int jr = wraps ? 0 : jb + VEC;
+    const int kr = wraps ? last - jb : VEC - 1;
+    const bool right_halo = any && (wraps || (lane == YEE_WAVE - 1 && jb + VEC < ny));
+    auto row_after = [&](int r) -> int { return r < nx ? r : (px ? r - nx : -1); };
+
+    auto load = [&](int k, HzBulkIn<VEC>& r) {
+        const int64_t row = (int64_t)k * P;
+        ldv<VEC>(hz_o + row + jb_c, r.hz);
+        ldv<VEC>(ex_o + row + jb_c, r.ex);
+        ldv<VEC>(ey_o + row + jb_c, r.ey);
+        ldids<VEC>(mat + row + jb_c, r.ids);
+        r.ex_l = r.hz_r = r.ex_r = r.ey_r = 0.f;
+        r.id_r = 0;
+        if (left_halo) r.ex_l = ex_o[row + jl];
+        if (right_halo) { r.hz_r = hz_o[row + jr]; r.ex_r = ex_o[row + jr]; r.ey_r = ey_o[row + jr]; r.id_r = mat[row + jr]; }
+    };
+    auto hz_row = [&](int k, const HzBulkIn<VEC>& r, const float ey_below[VEC], float ey_below_r, bool store,
+                      float out[VEC], float* out_r) {
+        float ex_left = shfl_from_prev(r.ex[VEC - 1]);
+        if (lane == 0) ex_left = left_halo ? r.ex_l : 0.f;
+        float unused_x, unused_y;
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) {
+            const float v = hz_centre(tab_a<LDS>(d, id_at(r.ids, q)), r.hz[q],
+                                      r.ex[q] - (q == 0 ? ex_left : r.ex[q - 1]), r.ey[q] - ey_below[q], idy, idx,
+                                      false, 0.f, 0.f, 0.f, &unused_x, false, 0.f, 0.f, 0.f, &unused_y);
+            out[q] = jb + q < ny ? v : 0.f;
+        }
+        if (store && any) stv<VEC>(hz_n + (int64_t)k * P + jb, out);
+        *out_r = 0.f;
+        if (right_halo)
+            *out_r = hz_centre(tab_a<LDS>(d, r.id_r), r.hz_r, r.ex_r - pickv<VEC>(r.ex, kr), r.ey_r - ey_below_r, idy,
+                               idx, false, 0.f, 0.f, 0.f, &unused_x, false, 0.f, 0.f, 0.f, &unused_y);
+    };
+
+    float hz_cur[VEC], hz_cur_r;
+    HzBulkIn<VEC> a, b, c;
+    load(i0, a);
+    {
+        float ey_below[VEC] = {};
+        float ey_below_r = 0.f;
+        if (i0 > 0 || px) {
+            const int ip = i0 > 0 ? i0 - 1 : nx - 1;
+            ldv<VEC>(ey_o + (int64_t)ip * P + jb_c, ey_below);
+            if (right_halo) ey_below_r = ey_o[(int64_t)ip * P + jr];
+        }
+        hz_row(i0, a, ey_below, ey_below_r, true, hz_cur, &hz_cur_r);
+    }
+    const int r1 = row_after(i0 + 1);
+    if (r1 >= 0) load(r1, b);
+
+    auto step = [&](int i, const HzBulkIn<VEC>& cur, const HzBulkIn<VEC>& nxt, HzBulkIn<VEC>& pre) {
+        const int64_t row = (int64_t)i * P;
+        const int k1 = row_after(i + 1);
+        const bool ey_updated = k1 >= 0;
+        const int k2 = (i + 1 < i1) ? row_after(i + 2) : -1;
+        if (k2 >= 0) load(k2, pre);
+        float hz_next[VEC] = {}, hz_next_r = 0.f;
+        if (ey_updated) hz_row(k1, nxt, cur.ey, cur.ey_r, i + 1 < i1, hz_next, &hz_next_r);
+        const float hz_sh = shfl_from_next(hz_cur[0]);
+        const int id_sh = __shfl_down(id_at(cur.ids, 0), 1, YEE_WAVE);
+        float ex_v[VEC], ey_v[VEC];
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) {
+            const int j = jb + q;
+            const int id_q = id_at(cur.ids, q);
+            ex_v[q] = j < ny ? cur.ex[q] : 0.f;
+            if (j < ny && (j < ny - 1 || py)) {
+                float hz_right;
+                int id_right;
+                if (q < VEC - 1 && j + 1 < ny) { hz_right = hz_cur[q + 1]; id_right = id_at(cur.ids, q + 1); }
+                else if (j + 1 < ny && lane != YEE_WAVE - 1) { hz_right = hz_sh; id_right = id_sh; }
+                else { hz_right = hz_cur_r; id_right = (int)cur.id_r; }
+                float ca, cb;
+                face_coefficients<LDS>(d, id_q, id_right, 0, &ca, &cb);
+                ex_v[q] = hz_ex(ca, cb, cur.ex[q], hz_right - hz_cur[q], idy);
+            }
+            ey_v[q] = j < ny ? cur.ey[q] : 0.f;
+            if (ey_updated && j < ny) {
+                float ca, cb;
+                face_coefficients<LDS>(d, id_q, id_at(nxt.ids, q), 1, &ca, &cb);
+                ey_v[q] = hz_ey(ca, cb, cur.ey[q], hz_next[q] - hz_cur[q], idx);
+            }
+        }
+        if (any) {
+            stv<VEC>(ex_n + row + jb, ex_v);
+            stv<VEC>(ey_n + row + jb, ey_v);
+        }
+#pragma unroll
+        for (int q = 0; q < VEC; ++q) hz_cur[q] = hz_next[q];
+        hz_cur_r = hz_next_r;
+    };
+
+    int i = i0;
+    for (; i + 2 < i1; i += 3) {
+        step(i, a, b, c);
+        step(i + 1, b, c, a);
+        step(i + 2, c, a, b);
+    }
+    if (i < i1) step(i, a, b, c);
+    if (i + 1 < i1) step(i + 1, b, c, a);
+}
+
 constexpr int POL_ANY = 2;   // polarisation read from the descriptor (batched launches mixing E_z and H_z)
 
-template <int POL, int VEC, bool LDS>
-__device__ __forceinline__ void tile_2d(const yee2d_desc& d, int local_tile, int64_t n)
+// Tile geometry of a local tile index (tiles run along y first, then x).
+struct TileBox {
+    int tj, ti, i0, i1, j0, j1;
+};
+
+template <int VEC>
+__device__ __forceinline__ TileBox tile_box(const yee2d_desc& d, int local_tile)
 {
+    TileBox t;
     const int tj_count = tiles_j_2d(d.ny, VEC);
-    const int tj = local_tile % tj_count;
-    const int ti = local_tile / tj_count;
-    if (POL == YEE_EZ || (POL == POL_ANY && d.polarization == YEE_EZ)) {
-        EzTile<VEC, LDS> t(d, n);
-        t.run(tj, ti);
+    t.tj = local_tile % tj_count;
+    t.ti = local_tile / tj_count;
+    t.i0 = t.ti * d.rows_per_tile;
+    t.i1 = min(d.nx, t.i0 + d.rows_per_tile);
+    t.j0 = t.tj * VEC * YEE_WAVE;
+    t.j1 = min(d.ny, t.j0 + VEC * YEE_WAVE);
+    return t;
+}
+
+// The k-th index of [0, n) not in the sorted list `skip` (length m).
+__device__ __forceinline__ int nth_outside(const int32_t* skip, int m, int k)
+{
+    int v = k;
+    for (int q = 0; q < m; ++q)
+        if (skip[q] <= v) ++v;
+    return v;
+}
+
+// Edge tile `item`: the full edge rows first (rows x all columns), then the edge columns of the other rows.
+__device__ __forceinline__ bool edge_tile(const yee2d_tile_sets& s, int item, int* ti, int* tj)
+{
+    if (s.all_edge) {
+        *ti = item / s.tiles_j;
+        *tj = item % s.tiles_j;
+        return item < s.tiles_i * s.tiles_j;
+    }
+    const int full = s.n_rows * s.tiles_j;
+    if (item < full) {
+        *ti = s.rows[item / s.tiles_j];
+        *tj = item % s.tiles_j;
+        return true;
+    }
+    const int k = item - full;
+    if (s.n_cols == 0 || k >= (s.tiles_i - s.n_rows) * s.n_cols) return false;
+    *ti = nth_outside(s.rows, s.n_rows, k / s.n_cols);
+    *tj = s.cols[k % s.n_cols];
+    return true;
+}
+
+// Bulk tile `item`: row-major over the rows and columns outside the edge lists.
+__device__ __forceinline__ bool bulk_tile(const yee2d_tile_sets& s, int item, int* ti, int* tj)
+{
+    const int cols = s.tiles_j - s.n_cols;
+    if (s.all_edge || cols <= 0 || item >= (s.tiles_i - s.n_rows) * cols) return false;
+    *ti = nth_outside(s.rows, s.n_rows, item / cols);
+    *tj = nth_outside(s.cols, s.n_cols, item % cols);
+    return true;
+}
+
+// Edge tiles are split along x into EDGE_SPLIT row chunks, one wavefront each (the general path is latency-bound:
+// few, slow tiles would otherwise finish long after the bulk).
+constexpr int EDGE_SPLIT = 8;
+
+template <int POL, int VEC, bool EDGE, bool LDS>
+__device__ __forceinline__ void tile_2d(const yee2d_desc& d, const TileBox& t, int64_t n, int part = 0,
+                                        int parts = 1)
+{
+    const bool ez = POL == YEE_EZ || (POL == POL_ANY && d.polarization == YEE_EZ);
+    if constexpr (EDGE) {
+        const int chunk = (t.i1 - t.i0 + parts - 1) / parts;
+        const int r0 = t.i0 + part * chunk;
+        const int r1 = min(t.i1, r0 + chunk);
+        if (r0 >= r1) return;
+        if (ez) {
+            EzTile<VEC, LDS> tile(d, n);
+            tile.run(t.tj, r0, r1);
+        } else {
+            HzTile<VEC, LDS> tile(d, n);
+            tile.run(t.tj, r0, r1);
+        }
     } else {
-        HzTile<VEC, LDS> t(d, n);
-        t.run(tj, ti);
+        if (ez)
+            ez_bulk_tile<VEC, LDS>(d, t.tj, t.ti, n);
+        else
+            hz_bulk_tile<VEC, LDS>(d, t.tj, t.ti, n);
     }
 }
 
 }  // namespace yee
 
-// One instance: grid = ceil(tiles / 4) blocks of 4 independent wavefront tiles; one kernel per polarisation and
-// columns-per-lane. The block stages the coefficient table in LDS first (when it fits).
-template <int POL, int VEC>
-__global__ void __launch_bounds__(256) yee2d_step_kernel(yee2d_desc d, int32_t offset)
+namespace yee {
+
+// One block of the bulk (EDGE = false) or edge (EDGE = true) tile set; `block` counts blocks within that set.
+template <int POL, int VEC, bool EDGE>
+__device__ __forceinline__ void step_block(const yee2d_desc& d, const yee2d_tile_sets& sets, int64_t n, int block)
 {
-    const bool lds = yee::stage_tables<POL>(d);
-    const int64_t n = *d.clock + offset;
-    if (n >= d.step_limit) return;
-    const int tile = blockIdx.x * 4 + yee::wave_index();
-    const int total = yee::tiles_j_2d(d.ny, VEC) * yee::tiles_i_2d(d.nx, d.rows_per_tile);
-    if (tile >= total) return;
+    const int parts = EDGE ? EDGE_SPLIT : 1;
+    const int slot = block * 4 + wave_index();
+    const int item = slot / parts;
+    int ti = 0, tj = 0;
+    const bool mine = EDGE ? edge_tile(sets, item, &ti, &tj) : bulk_tile(sets, item, &ti, &tj);
+    if (!__syncthreads_or(mine)) return;
+    const bool lds = stage_tables<POL>(d);
+    if (!mine) return;
+    const TileBox t = tile_box<VEC>(d, ti * sets.tiles_j + tj);
     if (lds)
-        yee::tile_2d<POL, VEC, true>(d, tile, n);
+        tile_2d<POL, VEC, EDGE, true>(d, t, n, slot % parts, parts);
     else
-        yee::tile_2d<POL, VEC, false>(d, tile, n);
+        tile_2d<POL, VEC, EDGE, false>(d, t, n, slot % parts, parts);
 }
 
-template __global__ void yee2d_step_kernel<YEE_EZ, 2>(yee2d_desc, int32_t);
-template __global__ void yee2d_step_kernel<YEE_EZ, 4>(yee2d_desc, int32_t);
-template __global__ void yee2d_step_kernel<YEE_HZ, 2>(yee2d_desc, int32_t);
-template __global__ void yee2d_step_kernel<YEE_HZ, 4>(yee2d_desc, int32_t);
+}  // namespace yee
+
+// One instance, two kernels over disjoint tile sets. The host lists the edge tile rows and columns (those touching a
+// CPML layer, with the halo row / column, or a source or monitor line); every other tile is a bulk tile. EDGE = false
+// steps the bulk tiles with the lean loop, EDGE = true the edge tiles with the general one, each edge tile split into
+// EDGE_SPLIT row chunks. Tiles are enumerated compactly: no block is launched without work. (One launch holding both
+// paths runs slower: it carries the general path's registers into the bulk tiles.)
+template <int POL, int VEC, bool EDGE>
+__global__ void __launch_bounds__(256) yee2d_step_kernel(yee2d_desc d, yee2d_tile_sets sets, int32_t offset)
+{
+    const int64_t n = *d.clock + offset;
+    if (n >= d.step_limit) return;
+    yee::step_block<POL, VEC, EDGE>(d, sets, n, blockIdx.x);
+}
+
+#define YEE_STEP_KERNEL(POL, VEC, EDGE) \
+    template __global__ void yee2d_step_kernel<POL, VEC, EDGE>(yee2d_desc, yee2d_tile_sets, int32_t);
+#define YEE_STEP_KERNELS(POL, VEC) YEE_STEP_KERNEL(POL, VEC, false) YEE_STEP_KERNEL(POL, VEC, true)
+YEE_STEP_KERNELS(YEE_EZ, 2)
+YEE_STEP_KERNELS(YEE_EZ, 4)
+YEE_STEP_KERNELS(YEE_HZ, 2)
+YEE_STEP_KERNELS(YEE_HZ, 4)
 
 // Many instances (E_z and H_z mixed) in one launch: tile t belongs to the instance whose tile range
 // [prefix[k], prefix[k+1]) holds it. Tables are read from global memory (they stay in L1/L2).
@@ -858,7 +1233,8 @@ __global__ void __launch_bounds__(256) yee2d_batch_kernel(const yee2d_desc* __re
     const yee2d_desc& d = descs[lo];
     const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
-    yee::tile_2d<yee::POL_ANY, VEC, false>(d, tile - prefix[lo], n);
+    const yee::TileBox t = yee::tile_box<VEC>(d, tile - prefix[lo]);
+    yee::tile_2d<yee::POL_ANY, VEC, true, false>(d, t, n);
 }
 
 template __global__ void yee2d_batch_kernel<2>(const yee2d_desc* __restrict__, const int32_t* __restrict__, int32_t,

@@ -3,6 +3,7 @@
 // and (optionally) an instantiated HIP graph that replays graph_steps steps plus the clock advance.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cstdio>
 #include <cstring>
@@ -10,9 +11,10 @@
 #include <vector>
 
 #include "yee.h"
+#include "yee_launch.h"
 
-template <int POL, int VEC>
-__global__ void yee2d_step_kernel(yee2d_desc d, int32_t offset);
+template <int POL, int VEC, bool EDGE>
+__global__ void yee2d_step_kernel(yee2d_desc d, yee2d_tile_sets sets, int32_t offset);
 template <int VEC>
 __global__ void yee2d_batch_kernel(const yee2d_desc* __restrict__ descs, const int32_t* __restrict__ prefix,
                                    int32_t n_inst, int32_t offset);
@@ -36,7 +38,9 @@ int32_t fail(int32_t code, const std::string& message)
             return fail(YEE_EHIP, std::string(#call) + ": " + hipGetErrorString(err_));                \
     } while (0)
 
-constexpr int kMinAutoRows = 8;   // shortest tile picked automatically (each tile re-reads one edge row)
+constexpr int kMinAutoRows = 8;    // shortest tile picked automatically (each tile re-reads one edge row)
+constexpr int kMaxAutoRows = 32;   // longest: many short tiles keep every CU busy to the end of the step
+constexpr int kEdgeSplit = 8;     // row chunks per edge tile (yee2d.hip EDGE_SPLIT)
 
 enum CtxKind { kSingle2D = 1, kBatch2D = 2 };
 
@@ -52,7 +56,9 @@ struct yee_ctx {
     int32_t n_inst = 0;
     // launch geometry
     int64_t tiles = 0;
-    int blocks = 0;
+    int blocks = 0;        // blocks of the bulk kernel (4 tiles each)
+    int edge_blocks = 0;   // blocks of the edge kernel (kEdgeSplit wavefronts per tile, 4 per block)
+    yee2d_tile_sets sets{};
     // graph replay
     int32_t graph_steps = 0;
     hipGraphExec_t graph = nullptr;
@@ -122,12 +128,17 @@ int64_t tiles_2d(const yee2d_desc& d)
     return tiles_j(d) * ((d.nx + d.rows_per_tile - 1) / d.rows_per_tile);
 }
 
-using StepKernel = void (*)(yee2d_desc, int32_t);
+using StepKernel = void (*)(yee2d_desc, yee2d_tile_sets, int32_t);
 
-StepKernel step_kernel(const yee2d_desc& d)
+// Bulk-tile (edge = false) or edge-tile kernel of a descriptor.
+StepKernel step_kernel(const yee2d_desc& d, bool edge)
 {
-    if (d.polarization == YEE_EZ) return d.lane_columns == 2 ? yee2d_step_kernel<YEE_EZ, 2> : yee2d_step_kernel<YEE_EZ, 4>;
-    return d.lane_columns == 2 ? yee2d_step_kernel<YEE_HZ, 2> : yee2d_step_kernel<YEE_HZ, 4>;
+    if (d.polarization == YEE_EZ) {
+        if (d.lane_columns == 2) return edge ? yee2d_step_kernel<YEE_EZ, 2, true> : yee2d_step_kernel<YEE_EZ, 2, false>;
+        return edge ? yee2d_step_kernel<YEE_EZ, 4, true> : yee2d_step_kernel<YEE_EZ, 4, false>;
+    }
+    if (d.lane_columns == 2) return edge ? yee2d_step_kernel<YEE_HZ, 2, true> : yee2d_step_kernel<YEE_HZ, 2, false>;
+    return edge ? yee2d_step_kernel<YEE_HZ, 4, true> : yee2d_step_kernel<YEE_HZ, 4, false>;
 }
 
 // Wavefront slots of the current device for a 256-thread kernel: resident blocks per CU x CUs x 4 waves.
@@ -140,21 +151,75 @@ int64_t wave_slots(const void* kernel)
     return (int64_t)cus * blocks * 4;
 }
 
-// Defaults of a 2D descriptor: 4 columns per lane; rows_per_tile so the tiles fill the device's wave slots in whole
-// rounds (one wavefront tile per slot when the grid is large enough).
+// Defaults of a 2D descriptor: 2 columns per lane; rows_per_tile so the tiles fill the device's wave slots in whole
+// rounds, capped at kMaxAutoRows rows.
 void apply_defaults(yee2d_desc& d)
 {
-    if (d.lane_columns == 0) d.lane_columns = 4;
+    if (d.lane_columns == 0) d.lane_columns = 2;
     if (d.rows_per_tile == 0 && d.nx > 0 && d.ny > 0 && (d.lane_columns == 2 || d.lane_columns == 4)) {
-        const int64_t slots = wave_slots(reinterpret_cast<const void*>(step_kernel(d)));
+        const int64_t slots = wave_slots(reinterpret_cast<const void*>(step_kernel(d, false)));
         const int64_t columns = tiles_j(d);
         int64_t rows = slots > 0 ? (d.nx * columns + slots - 1) / slots : 64;
+        if (rows > kMaxAutoRows) rows = kMaxAutoRows;
         if (rows < kMinAutoRows) rows = kMinAutoRows;
         if (rows > d.nx) rows = d.nx;
         // even out the tiles: the same count with the shortest rows
         const int64_t tiles_x = (d.nx + rows - 1) / rows;
         d.rows_per_tile = (int32_t)((d.nx + tiles_x - 1) / tiles_x);
     }
+}
+
+// Edge tile rows and columns of a descriptor: tile rows whose rows (with the row before and the row after) reach an x
+// CPML layer or hold a source / monitor line along y, tile columns likewise for y. Too many lines: every tile is
+// an edge tile.
+yee2d_tile_sets tile_sets(const yee2d_desc& d)
+{
+    yee2d_tile_sets s{};
+    const int cols = 64 * d.lane_columns;
+    s.tiles_j = (d.ny + cols - 1) / cols;
+    s.tiles_i = (d.nx + d.rows_per_tile - 1) / d.rows_per_tile;
+    std::vector<int32_t> rows, columns;
+    for (int ti = 0; ti < s.tiles_i; ++ti) {
+        const int i0 = ti * d.rows_per_tile, i1 = std::min(d.nx, i0 + d.rows_per_tile);
+        const int a = i0 > 0 ? i0 - 1 : 0, b = i1 < d.nx ? i1 : d.nx - 1;
+        bool edge = a < d.layer_x_lo || b >= d.layer_x_hi;
+        for (int k = 0; k < d.n_sources; ++k)
+            edge = edge || (d.source[k].axis == 1 && d.source[k].line >= a && d.source[k].line <= b);
+        for (int k = 0; k < d.n_monitors; ++k)
+            edge = edge || (d.monitor[k].normal == 0 && d.monitor[k].position >= a && d.monitor[k].position <= b);
+        if (edge) rows.push_back(ti);
+    }
+    for (int tj = 0; tj < s.tiles_j; ++tj) {
+        const int j0 = tj * cols, j1 = std::min(d.ny, j0 + cols);
+        const int a = j0 > 0 ? j0 - 1 : 0, b = j1 < d.ny ? j1 : d.ny - 1;
+        bool edge = a < d.layer_y_lo || b >= d.layer_y_hi;
+        for (int k = 0; k < d.n_sources; ++k)
+            edge = edge || (d.source[k].axis == 0 && d.source[k].line >= a && d.source[k].line <= b);
+        for (int k = 0; k < d.n_monitors; ++k)
+            edge = edge || (d.monitor[k].normal == 1 && d.monitor[k].position >= a && d.monitor[k].position <= b);
+        if (edge) columns.push_back(tj);
+    }
+    if ((int)rows.size() > YEE_MAX_EDGE_LINES || (int)columns.size() > YEE_MAX_EDGE_LINES) {
+        s.all_edge = 1;
+        return s;
+    }
+    s.n_rows = (int32_t)rows.size();
+    s.n_cols = (int32_t)columns.size();
+    for (int k = 0; k < s.n_rows; ++k) s.rows[k] = rows[k];
+    for (int k = 0; k < s.n_cols; ++k) s.cols[k] = columns[k];
+    return s;
+}
+
+void set_geometry(yee_ctx* ctx)
+{
+    const yee2d_desc& d = ctx->d2;
+    ctx->sets = tile_sets(d);
+    const yee2d_tile_sets& s = ctx->sets;
+    ctx->tiles = tiles_2d(d);
+    const int64_t bulk = s.all_edge ? 0 : (int64_t)(s.tiles_i - s.n_rows) * std::max(0, s.tiles_j - s.n_cols);
+    const int64_t edge = ctx->tiles - bulk;
+    ctx->blocks = (int)((bulk + 3) / 4);
+    ctx->edge_blocks = (int)((edge * kEdgeSplit + 3) / 4);
 }
 
 void drop_graph(yee_ctx* ctx)
@@ -170,9 +235,14 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
 {
     if (count <= 0) return YEE_OK;
     if (ctx->kind == kSingle2D) {
-        const StepKernel kernel = step_kernel(ctx->d2);
-        for (int32_t k = 0; k < count; ++k)
-            hipLaunchKernelGGL(kernel, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d2, k);
+        const StepKernel bulk = step_kernel(ctx->d2, false);
+        const StepKernel edge = step_kernel(ctx->d2, true);
+        for (int32_t k = 0; k < count; ++k) {
+            if (ctx->blocks > 0)
+                hipLaunchKernelGGL(bulk, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
+            if (ctx->edge_blocks > 0)
+                hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
+        }
         YEE_HIP(hipGetLastError());
         hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, stream, ctx->d2.clock, (int64_t)count,
                            ctx->d2.step_limit);
@@ -245,8 +315,7 @@ int32_t yee2d_create(const yee2d_desc* desc, yee_ctx** out)
     yee_ctx* ctx = new yee_ctx();
     ctx->kind = kSingle2D;
     ctx->d2 = d;
-    ctx->tiles = tiles_2d(d);
-    ctx->blocks = (int)((ctx->tiles + 3) / 4);
+    set_geometry(ctx);
     *out = ctx;
     return YEE_OK;
 }
@@ -260,8 +329,7 @@ int32_t yee2d_update(yee_ctx* ctx, const yee2d_desc* desc)
     if (status != YEE_OK) return status;
     drop_graph(ctx);
     ctx->d2 = d;
-    ctx->tiles = tiles_2d(d);
-    ctx->blocks = (int)((ctx->tiles + 3) / 4);
+    set_geometry(ctx);
     return YEE_OK;
 }
 
@@ -273,7 +341,7 @@ int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, 
     std::vector<yee2d_desc> host(descs, descs + n);
     std::vector<int32_t> prefix(n + 1, 0);
     for (int32_t k = 0; k < n; ++k) {
-        if (host[k].lane_columns == 0) host[k].lane_columns = host[0].lane_columns ? host[0].lane_columns : 4;
+        if (host[k].lane_columns == 0) host[k].lane_columns = host[0].lane_columns ? host[0].lane_columns : 2;
         if (host[k].rows_per_tile == 0) host[k].rows_per_tile = 32;
         int32_t status = check_2d(&host[k]);
         if (status == YEE_OK && host[k].lane_columns != host[0].lane_columns)

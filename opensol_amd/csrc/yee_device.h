@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "yee.h"
+#include "yee_launch.h"
 
 #define YEE_WAVE 64
 

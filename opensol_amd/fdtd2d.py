@@ -47,6 +47,8 @@ class Model2D:
                      for axis, n in (('x', self.nx), ('y', self.ny))}
         self.inv_dx_e, self.inv_dx_h = curl_spacings(self.nx, dx32, self.cpml['x'])
         self.inv_dy_e, self.inv_dy_h = curl_spacings(self.ny, dx32, self.cpml['y'])
+        # the curl spacing outside the CPML layers (uniform grid: every bulk entry of all four arrays)
+        self.inv_bulk = tuple(float(a[b]) for a, b in ((self.inv_dx_e, self.nx // 2), (self.inv_dy_e, self.ny // 2)))
         self.sample_interval = 1
 
     @property
@@ -237,7 +239,7 @@ def steps_for_duration(time, dt, duration):
 class Engine2D:
     """Device state and libyee context of one Model2D. torch provides the HBM buffers and the stream."""
 
-    def __init__(self, model, device=None, rows_per_tile=0, graph_steps=0, capacity=1024, lane_columns=4):
+    def __init__(self, model, device=None, rows_per_tile=0, graph_steps=0, capacity=1024, lane_columns=0):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError('Engine2D needs a ROCm GPU (torch.cuda.is_available() is False); there is no CPU path')
@@ -335,6 +337,9 @@ class Engine2D:
         d.dt = float(model.dt)
         d.dt_over_eps0 = float(np.float32(float(model.dt) / EPS0))
         d.inv_dx_e, d.inv_dx_h, d.inv_dy_e, d.inv_dy_h = (t.data_ptr() for t in self.inv)
+        d.inv_dx_bulk, d.inv_dy_bulk = model.inv_bulk
+        d.layer_x_lo, d.layer_x_hi = model.cpml['x'].layer_bounds()
+        d.layer_y_lo, d.layer_y_hi = model.cpml['y'].layer_bounds()
         for a, axis in enumerate('xy'):
             c = self.cpml[axis]
             if c is None:

@@ -24,7 +24,7 @@ class Simulation:
 
     def __init__(self, size_x: float = None, size_y: float = None, dx: float = None, npml: int = 15,
                  courant: float = None, boundaries=None, *, polarization: str = 'ez', device=None,
-                 rows_per_tile: int = 0, graph_steps: int = 0, lane_columns: int = 4):
+                 rows_per_tile: int = 0, graph_steps: int = 0, lane_columns: int = 0):
         if size_x is None or size_y is None or dx is None:
             raise ValueError('Simulation() needs (size_x, size_y, dx, ...) for the 2D EM-FDTD path')
         self.model = Model2D(size_x, size_y, dx, npml=npml, courant=courant, boundaries=boundaries,
