@@ -112,6 +112,8 @@ def main():
     parser.add_argument('--rows-per-tile', type=int, default=0, help='0: fill the device wave slots')
     parser.add_argument('--lane-columns', type=int, default=0, choices=(0, 2, 4), help='0: library default')
     parser.add_argument('--cpu-steps', type=int, default=8, help='steps of the CPU baseline sample (0: skip)')
+    parser.add_argument('--kernel-steps', type=int, default=200,
+                        help='steps timed per kernel with HIP events for the roofline (0: skip)')
     args = parser.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU: this FDTD path runs on a single MI355X (got --gpus {args.gpus}); '
@@ -143,11 +145,24 @@ def main():
     elapsed_s = start.elapsed_time(stop) / 1e3
     per_step = elapsed_s / args.steps
     value = cells * args.steps / elapsed_s
-    achieved = BYTES_PER_CELL_2D * cells / per_step / 1e9
+    step_achieved = BYTES_PER_CELL_2D * cells / per_step / 1e9
     workload = 'yee2d_ez_8192x8192_cpml20_400rect'
-    traffic = measured_traffic(workload)
-    print(f'# GPU: {per_step * 1e3:.4f} ms/step, {value:.4e} cell-updates/s, {achieved:.1f} GB/s algorithmic',
+    print(f'# GPU: {per_step * 1e3:.4f} ms/step, {value:.4e} cell-updates/s, {step_achieved:.1f} GB/s algorithmic',
           file=sys.stderr, flush=True)
+    # the dominant kernel (bulk tiles): its own launch time from HIP events on the library's stream
+    kernels = engine.step_timed(args.kernel_steps) if args.kernel_steps > 0 else None
+    roofline = {'bound': 'hbm', 'achieved': None, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': None,
+                'traffic': measured_traffic(workload), 'kernel': 'yee2d_step_kernel<EZ, bulk tiles>',
+                'bytes_per_cell_update': BYTES_PER_CELL_2D, 'step_achieved': step_achieved,
+                'step_frac': step_achieved / HBM_PEAK_GBS}
+    if kernels is not None:
+        bulk_ms, bulk_cells = kernels['bulk']
+        achieved = BYTES_PER_CELL_2D * bulk_cells / (bulk_ms * 1e-3) / 1e9
+        roofline.update(achieved=achieved, frac=achieved / HBM_PEAK_GBS, launch_ms=bulk_ms, cells_per_launch=bulk_cells,
+                        algorithmic_bytes_per_launch=BYTES_PER_CELL_2D * bulk_cells,
+                        edge_launch_ms=kernels['edge'][0], edge_cells_per_launch=kernels['edge'][1])
+        print(f'# kernels: bulk {bulk_ms * 1e3:.1f} us for {bulk_cells} cells ({achieved:.1f} GB/s), edge '
+              f'{kernels["edge"][0] * 1e3:.1f} us for {kernels["edge"][1]} cells', file=sys.stderr, flush=True)
 
     baseline = None
     if args.cpu_steps > 0:
@@ -170,9 +185,7 @@ def main():
         'config': {'workload': workload, 'grid': [sim.model.nx, sim.model.ny], 'cpml': 20, 'dx_m': 20e-9,
                    'rectangles': 400, 'seed': SEED, 'polarization': 'ez', 'graph_steps': args.graph_steps,
                    'rows_per_tile': rows_per_tile, 'lane_columns': lane_columns},
-        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
-                     'bytes_per_cell_update': BYTES_PER_CELL_2D},
+        'roofline': roofline,
         'cpu_baseline': baseline,
     }
     print(json.dumps(result))

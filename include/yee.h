@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 1
+#define YEE_ABI_VERSION 2
 #define YEE_MAX_SOURCES 4
 #define YEE_MAX_MONITORS 4
 
@@ -103,7 +103,7 @@ typedef struct yee2d_desc {
     int32_t polarization;    /* YEE_EZ or YEE_HZ */
     int32_t periodic_x, periodic_y;
     int32_t rows_per_tile;   /* x rows marched by one wavefront tile (0: chosen from the device's wave slots) */
-    int32_t lane_columns;    /* y columns per lane: 2 or 4 (0: 4); a tile spans 64 * lane_columns columns */
+    int32_t lane_columns;    /* y columns per lane: 2 or 4 (0: 2); a tile spans 64 * lane_columns columns */
     float* field[2][3];      /* [set][component]: E_z (ez, hx, hy) or H_z (hz, ex, ey) */
     const uint16_t* material;
     const float* table;      /* (n_materials, 4) */
@@ -133,11 +133,100 @@ typedef struct yee2d_desc {
     int64_t step_limit;      /* steps n >= step_limit are skipped (batched instances); INT64_MAX for none */
 } yee2d_desc;
 
+/*
+ * Point dipole of a 3D instance: E_component[i, j, k] += float32(weight * wave[n]) after the E update of step n.
+ * Replaces physics/em3d/sources3d.py:41-75 (DipoleSource3D.inject; weight = face cb x moment / dual volume, the
+ * calibration of :56-61 and physics/em3d/engine3d.py:299-311 done on the host).
+ */
+typedef struct yee_point_source {
+    int32_t component;       /* 0 E_x, 1 E_y, 2 E_z */
+    int32_t i, j, k;
+    float weight;
+    int32_t pad_;
+    const float* wave;       /* float32 s(t_n + dt/2) per absolute step n */
+    int64_t wave_len;
+} yee_point_source;
+
+/*
+ * Running-DFT flux plane of a 3D instance (FP64 Kahan accumulators), sampled after step n when (n + 1) % m == 0.
+ * Replaces physics/em3d/monitors3d.py:90-101 (PlaneFlux3D.update: E = mean of the two layers across the plane,
+ * H on the plane). Pair p samples E component e_comp[p] on layers edge - 1 and edge along `normal` and H component
+ * h_comp[p] on layer edge - 1, over index ranges [lo[p][a], hi[p][a]) of the two tangential axes a (in grid axis
+ * order: the lower-numbered tangential axis first).
+ */
+typedef struct yee_flux_plane {
+    int32_t normal;
+    int32_t edge;
+    int32_t e_comp[2];       /* 0..2 = E_x..E_z */
+    int32_t h_comp[2];       /* 0..2 = H_x..H_z */
+    int32_t lo[2][2];        /* [pair][tangential axis in grid order] */
+    int32_t hi[2][2];
+    int32_t n_freq;
+    int32_t pad_;
+    const double* phase;     /* [sample s][n_freq][2]: exp(-j omega t) m dt at t = t_{(s+1) m} */
+    int64_t n_samples;
+    double* acc[2];          /* per pair: [2 (E, H)][n_freq][4 (sum re, sum im, comp re, comp im)][cells a][cells b] */
+} yee_flux_plane;
+
+/*
+ * CPML of one axis of a 3D instance (physics/em3d/boundaries3d.py:208-241): psi rows of the E ('e', cell centres)
+ * and H ('h', hi faces) kinds, recursion b and gain = a x bulk inverse spacing per psi row, psi of the axis' first
+ * and second component. psi layouts: x axis (n_rows, ny, nz), y axis (nx, n_rows, nz), z axis (nx, ny, n_rows).
+ * E psi is updated in place; H psi is ping-pong ([set] = step parity, like the fields).
+ */
+typedef struct yee3d_cpml_axis {
+    int32_t n_rows;          /* psi rows per kind (0: no CPML on this axis) */
+    int32_t pad_;
+    const int32_t* row_e;
+    const int32_t* row_h;
+    const float* b_e;
+    const float* g_e;
+    const float* b_h;
+    const float* g_h;
+    float* psi_e[2];         /* [first, second component] */
+    float* psi_h[2][2];      /* [set][first, second component] */
+} yee3d_cpml_axis;
+
+#define YEE3D_MAX_SOURCES 4
+#define YEE3D_MAX_PLANES 4
+
+/*
+ * One 3D instance. Replaces the 3D engine step of physics/em3d/engine3d.py:497-554 (update_h_3d / update_e_3d with
+ * their fused CPML, kernels3d.py:335-400, 538-610; dipole sources; plane flux monitors) with one fused E/H launch
+ * per step plus a plane-sample launch. table per material id: (da, db, ca, cb, eps_r, sigma_e, 0, 0): H edge
+ * coefficients of the id, the E face coefficients of a face between two cells of the id, and the float32 eps_r and
+ * sigma_e from which a face between two ids forms its mean coefficients (kernels3d.py:61-85, shares 1/2).
+ */
+typedef struct yee3d_desc {
+    int32_t nx, ny, nz;
+    int32_t periodic[3];
+    int32_t rows_per_tile;   /* y rows per wavefront tile: 4 (0: 4) */
+    int32_t planes_per_tile; /* x planes marched by one wavefront (0: chosen from the device's wave slots) */
+    float* field[2][6];      /* [set][ex, ey, ez, hx, hy, hz], (nx, ny, nz) z contiguous */
+    const uint16_t* material;
+    const float* table;      /* (n_materials, 8) */
+    int32_t n_materials;
+    float dt;
+    const float* inv_e[3];   /* per axis (n): 1 / width at centres, CPML kappa folded in */
+    const float* inv_h[3];   /* per axis (n): 1 / spacing at hi faces, kappa folded in; last 0 on non-periodic axes */
+    yee3d_cpml_axis cpml[3];
+    int32_t n_sources;
+    int32_t n_planes;
+    int32_t sample_interval;
+    int32_t pad_;
+    yee_point_source source[YEE3D_MAX_SOURCES];
+    yee_flux_plane plane[YEE3D_MAX_PLANES];
+    int64_t* clock;
+    int64_t step_limit;
+} yee3d_desc;
+
 typedef struct yee_ctx yee_ctx;
 
 /* Library ABI version (YEE_ABI_VERSION) and sizeof checks for bindings. */
 int32_t yee_abi_version(void);
-size_t yee_sizeof(int32_t which);   /* 0: yee2d_desc, 1: yee_line_source, 2: yee_flux_line, 3: yee_cpml_axis */
+/* 0: yee2d_desc, 1: yee_line_source, 2: yee_flux_line, 3: yee_cpml_axis, 4: yee3d_desc, 5: yee_point_source,
+ * 6: yee_flux_plane, 7: yee3d_cpml_axis */
+size_t yee_sizeof(int32_t which);
 const char* yee_last_error(void);
 
 /*
@@ -153,6 +242,15 @@ int32_t yee2d_create(const yee2d_desc* desc, yee_ctx** out);
  */
 int32_t yee_step(yee_ctx* ctx, int64_t n_steps, void* stream);
 
+/*
+ * Profiling form of yee_step: the same launches, one step at a time and never from a graph, with a HIP event pair
+ * around every kernel. kernel_ms[c] receives the summed kernel time of class c over the n_steps steps and cells[c]
+ * the cells one launch of that class updates (single 2D instance: class 0 = bulk tiles, 1 = edge tiles; 3D: class 0
+ * = the fused step kernel, every cell, 1 = the plane-sample kernel). Synchronizes `stream`. No counterpart in the
+ * reference (its timing is wall clock around Simulation.run).
+ */
+int32_t yee_step_timed(yee_ctx* ctx, int64_t n_steps, void* stream, double* kernel_ms, int64_t* cells);
+
 /* Steps per replayed HIP graph (0 disables graphs; default 0). Drops cached graphs. */
 int32_t yee_set_graph_steps(yee_ctx* ctx, int32_t graph_steps);
 
@@ -161,6 +259,14 @@ int32_t yee2d_update(yee_ctx* ctx, const yee2d_desc* desc);
 
 /* The descriptor a single 2D context steps, with the defaults it resolved (rows_per_tile, lane_columns). */
 int32_t yee2d_get_desc(const yee_ctx* ctx, yee2d_desc* out);
+
+/*
+ * Create a stepping context for one 3D instance (host memory only; the descriptor is copied). yee_step advances it.
+ * Replaces the stepping part of physics/em3d/engine3d.py:162-263 (FDTDEngine3D construction).
+ */
+int32_t yee3d_create(const yee3d_desc* desc, yee_ctx** out);
+int32_t yee3d_update(yee_ctx* ctx, const yee3d_desc* desc);
+int32_t yee3d_get_desc(const yee_ctx* ctx, yee3d_desc* out);
 
 int32_t yee_destroy(yee_ctx* ctx);
 

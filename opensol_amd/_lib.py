@@ -6,9 +6,11 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 1
+ABI_VERSION = 2
 MAX_SOURCES = 4
 MAX_MONITORS = 4
+MAX_SOURCES_3D = 4
+MAX_PLANES_3D = 4
 YEE_EZ, YEE_HZ = 0, 1
 INT64_MAX = (1 << 63) - 1
 
@@ -42,6 +44,34 @@ class Desc2DC(ctypes.Structure):
                 ('clock', c_ptr), ('step_limit', c_i64)]
 
 
+class PointSourceC(ctypes.Structure):
+    _fields_ = [('component', c_i32), ('i', c_i32), ('j', c_i32), ('k', c_i32), ('weight', c_f32), ('pad_', c_i32),
+                ('wave', c_ptr), ('wave_len', c_i64)]
+
+
+class FluxPlaneC(ctypes.Structure):
+    _fields_ = [('normal', c_i32), ('edge', c_i32), ('e_comp', c_i32 * 2), ('h_comp', c_i32 * 2),
+                ('lo', (c_i32 * 2) * 2), ('hi', (c_i32 * 2) * 2), ('n_freq', c_i32), ('pad_', c_i32),
+                ('phase', c_ptr), ('n_samples', c_i64), ('acc', c_ptr * 2)]
+
+
+class Cpml3DAxisC(ctypes.Structure):
+    _fields_ = [('n_rows', c_i32), ('pad_', c_i32), ('row_e', c_ptr), ('row_h', c_ptr),
+                ('b_e', c_ptr), ('g_e', c_ptr), ('b_h', c_ptr), ('g_h', c_ptr),
+                ('psi_e', c_ptr * 2), ('psi_h', (c_ptr * 2) * 2)]
+
+
+class Desc3DC(ctypes.Structure):
+    _fields_ = [('nx', c_i32), ('ny', c_i32), ('nz', c_i32), ('periodic', c_i32 * 3), ('rows_per_tile', c_i32),
+                ('planes_per_tile', c_i32), ('field', (c_ptr * 6) * 2), ('material', c_ptr), ('table', c_ptr),
+                ('n_materials', c_i32), ('dt', c_f32), ('inv_e', c_ptr * 3), ('inv_h', c_ptr * 3),
+                ('cpml', Cpml3DAxisC * 3), ('n_sources', c_i32), ('n_planes', c_i32), ('sample_interval', c_i32),
+                ('pad_', c_i32), ('source', PointSourceC * MAX_SOURCES_3D), ('plane', FluxPlaneC * MAX_PLANES_3D),
+                ('clock', c_ptr), ('step_limit', c_i64)]
+
+
+STRUCTS = (Desc2DC, LineSourceC, FluxLineC, CpmlAxisC, Desc3DC, PointSourceC, FluxPlaneC, Cpml3DAxisC)
+
 # Every entry point include/yee.h declares: (name, restype, argtypes)
 SIGNATURES = (
     ('yee_abi_version', c_i32, ()),
@@ -49,9 +79,13 @@ SIGNATURES = (
     ('yee_last_error', ctypes.c_char_p, ()),
     ('yee2d_create', c_i32, (ctypes.POINTER(Desc2DC), ctypes.POINTER(c_ptr))),
     ('yee_step', c_i32, (c_ptr, c_i64, c_ptr)),
+    ('yee_step_timed', c_i32, (c_ptr, c_i64, c_ptr, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_i64))),
     ('yee_set_graph_steps', c_i32, (c_ptr, c_i32)),
     ('yee2d_update', c_i32, (c_ptr, ctypes.POINTER(Desc2DC))),
     ('yee2d_get_desc', c_i32, (c_ptr, ctypes.POINTER(Desc2DC))),
+    ('yee3d_create', c_i32, (ctypes.POINTER(Desc3DC), ctypes.POINTER(c_ptr))),
+    ('yee3d_update', c_i32, (c_ptr, ctypes.POINTER(Desc3DC))),
+    ('yee3d_get_desc', c_i32, (c_ptr, ctypes.POINTER(Desc3DC))),
     ('yee_destroy', c_i32, (c_ptr,)),
     ('yee2d_batch_bytes', ctypes.c_size_t, (c_i32,)),
     ('yee2d_batch_create', c_i32, (ctypes.POINTER(Desc2DC), c_i32, c_ptr, c_ptr, ctypes.POINTER(c_ptr))),
@@ -79,7 +113,7 @@ def load():
         fn.argtypes = list(argtypes)
     if lib.yee_abi_version() != ABI_VERSION:
         raise RuntimeError(f'libyee.so ABI {lib.yee_abi_version()} != binding ABI {ABI_VERSION}')
-    for which, struct in enumerate((Desc2DC, LineSourceC, FluxLineC, CpmlAxisC)):
+    for which, struct in enumerate(STRUCTS):
         if lib.yee_sizeof(which) != ctypes.sizeof(struct):
             raise RuntimeError(f'libyee.so struct {struct.__name__} is {lib.yee_sizeof(which)} bytes, the binding '
                                f'{ctypes.sizeof(struct)}: rebuild the library')

@@ -204,13 +204,16 @@ class CPMLAxisTables:
         return lo, hi
 
 
-def curl_spacings(n, dx, cpml):
+def curl_spacings(n, dx, cpml, zero_last_h=False):
     """float32 (inv_e, inv_h) of one axis: 1/w at centres, 1/mean(w_i, w_(i+1)) at hi faces (last = periodic wrap),
     with CPML kappa divided in (physics/em/yee_grid.py:11-45, solvers/fdtd_2d_base.py:76-84,
-    boundaries.py:778-798). Kernel widths are the float32 grid dx (yee_grid.py:11-16)."""
+    boundaries.py:778-798). Kernel widths are the float32 grid dx (yee_grid.py:11-16). zero_last_h: the 3D engine's
+    inv_h[n-1] = 0 on a non-periodic axis (physics/em3d/engine3d.py:201-210)."""
     widths = np.full(n, float(DTYPE(dx)))
     inv_e = 1.0 / widths
     inv_h = 1.0 / (0.5 * (widths + np.roll(widths, -1)))
+    if zero_last_h:
+        inv_h[-1] = 0.0
     if cpml is not None:
         inv_e = inv_e / cpml.kappa_e
         inv_h = inv_h / cpml.kappa_h

@@ -1,0 +1,513 @@
+// Fused 3D Yee step for gfx950: one launch advances a 3D instance by one full step (H update with its CPML on every
+// axis, E update with its CPML, point dipole injection); a second, small launch takes the FP64 Kahan DFT samples of
+// the flux planes on sample steps.
+//
+// Work decomposition. Arrays are (nx, ny, nz) with z contiguous. A wavefront tile spans 64 consecutive z cells, one
+// per lane, of which lanes 1..62 own their cell: lane 0 holds the cell below (its H feeds the E of lane 1) and lane 63
+// the cell above (its E feeds the H of lane 62), so every z neighbour is a wave shuffle and no lane runs extra code.
+// A lane carries J rows along y in registers (rows j0..j0+J-1 owned, plus the halo row j0-1 whose H_x, H_z feed the
+// E of row j0, plus row j0+J whose E feeds the H of row j0+J-1), and the tile marches planes i0..i1-1 along x with the
+// new H of the previous plane kept in registers (the plane i0-1 is recomputed once at the start). Halo values are
+// recomputed from the previous state, which is race-free because the fields and the H CPML psi are double-buffered
+// (step n reads set n & 1 and writes set (n & 1) ^ 1); E psi is read and written only by the owner of its cell.
+// Every copy of an H value goes through the same update helper (the library is built with -ffp-contract=on), so a
+// recomputed halo equals its owner's value bit for bit.
+//
+// Boundaries: a periodic axis wraps its indices; on a non-periodic axis indices are clamped to the grid, which makes
+// the hi neighbour of the last row the row itself (face partner = the cell itself, and inv_h[n-1] = 0 cancels the
+// curl term), and the lo neighbour of row 0 is forced to 0.
+//
+// HBM traffic per cell-update: read six fields (24 B) + uint16 material id (2 B), write six fields (24 B) = 50 B.
+//
+// Reference semantics (file:line of the reference package open_sol/):
+//   H update + CPML    physics/em3d/kernels3d.py:335-400 (update_h_3d), :44-56 (edge_coefficients)
+//   E update + CPML    physics/em3d/kernels3d.py:538-610 (update_e_3d), :61-85 (face_coefficients)
+//   CPML rows          physics/em3d/boundaries3d.py:208-241 (cpml_slabs)
+//   dipole             physics/em3d/sources3d.py:41-75
+//   plane flux         physics/em3d/monitors3d.py:90-101; physics/em/monitors.py:22-34, 92-96
+//   step order         physics/em3d/engine3d.py:497-554
+#include "yee_device.h"
+
+namespace yee3 {
+
+constexpr int LANES = 64;
+constexpr int OWNED = 62;          // z cells a wavefront tile owns (lanes 1..62)
+constexpr int LDS_IDS = 512;       // ids whose table is staged in LDS: (da, db, ca, cb) + (eps_r, sigma_e), 12 KiB
+constexpr float EPS0F = 8.8541878128e-12f;
+
+__shared__ float4 lds3_coef[LDS_IDS];
+__shared__ float2 lds3_mat[LDS_IDS];
+
+__device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+// Stage the table of one instance in LDS (whole block; every thread reaches the barrier).
+__device__ __forceinline__ bool stage_tables(const yee3d_desc& d)
+{
+    const bool use = d.n_materials <= LDS_IDS;
+    if (use) {
+        for (int k = threadIdx.x; k < d.n_materials; k += blockDim.x) {
+            const float* t = d.table + 8 * k;
+            lds3_coef[k] = make_float4(t[0], t[1], t[2], t[3]);
+            lds3_mat[k] = make_float2(t[4], t[5]);
+        }
+    }
+    __syncthreads();
+    return use;
+}
+
+template <bool LDS>
+__device__ __forceinline__ float4 coef(const yee3d_desc& d, int id)
+{
+    if constexpr (LDS) return lds3_coef[id];
+    else return *reinterpret_cast<const float4*>(d.table + 8 * id);
+}
+
+template <bool LDS>
+__device__ __forceinline__ float2 mat(const yee3d_desc& d, int id)
+{
+    if constexpr (LDS) return lds3_mat[id];
+    else return *reinterpret_cast<const float2*>(d.table + 8 * id + 4);
+}
+
+// (ca, cb) of an E face between cells of ids a and b: the id's own entry when they agree, else the float32 mean of
+// eps_r and sigma_e (shares 1/2) and the semi-implicit coefficients (kernels3d.py:61-85).
+template <bool LDS>
+__device__ __forceinline__ float2 face(const yee3d_desc& d, int a, int b)
+{
+    if (a == b) {
+        const float4 c = coef<LDS>(d, a);
+        return make_float2(c.z, c.w);
+    }
+    const float2 ma = mat<LDS>(d, a), mb = mat<LDS>(d, b);
+    const float eps = (0.5f * ma.x + 0.5f * mb.x) * EPS0F;
+    const float sig = 0.5f * ma.y + 0.5f * mb.y;
+    const float k = sig * d.dt / (2.0f * eps);
+    const float den = 1.0f + k;
+    return make_float2((1.0f - k) / den, (d.dt / eps) / den);
+}
+
+__device__ __forceinline__ float from_next(float v) { return __shfl_down(v, 1, LANES); }
+__device__ __forceinline__ float from_prev(float v) { return __shfl_up(v, 1, LANES); }
+__device__ __forceinline__ int from_next_i(int v) { return __shfl_down(v, 1, LANES); }
+
+// Wrapped (periodic) or clamped index along an axis of n cells.
+__device__ __forceinline__ int fold(int v, int n, bool periodic)
+{
+    if (periodic) return v < 0 ? v + n : (v >= n ? v - n : v);
+    return v < 0 ? 0 : (v >= n ? n - 1 : v);
+}
+
+// psi recursion of one layer cell: b psi + g dF.
+__device__ __forceinline__ float psi_next(float b, float psi, float g, float difference) { return b * psi + g * difference; }
+
+// ---- H components (shared by owners and halo copies) ----
+// H_x: -db (dEz/dy - dEy/dz); CPML y first (-), z first (+).
+__device__ __forceinline__ float hx_yee(float da, float db, float hx, float dez_y, float ihy, float dey_z, float ihz)
+{
+    return da * hx - db * (dez_y * ihy - dey_z * ihz);
+}
+// H_y: -db (dEx/dz - dEz/dx); CPML x first (+), z second (-).
+__device__ __forceinline__ float hy_yee(float da, float db, float hy, float dex_z, float ihz, float dez_x, float ihx)
+{
+    return da * hy - db * (dex_z * ihz - dez_x * ihx);
+}
+// H_z: -db (dEy/dx - dEx/dy); CPML x second (-), y second (+).
+__device__ __forceinline__ float hz_yee(float da, float db, float hz, float dey_x, float ihx, float dex_y, float ihy)
+{
+    return da * hz - db * (dey_x * ihx - dex_y * ihy);
+}
+
+// CPML row of one axis at one index: psi row (-1 outside the layers), b, gain.
+struct Row {
+    int r;
+    float b, g;
+};
+
+__device__ __forceinline__ Row cpml_row(const yee3d_cpml_axis& a, bool h, int index)
+{
+    Row row{-1, 0.f, 0.f};
+    if (a.n_rows == 0) return row;
+    row.r = h ? a.row_h[index] : a.row_e[index];
+    if (row.r >= 0) {
+        row.b = h ? a.b_h[row.r] : a.b_e[row.r];
+        row.g = h ? a.g_h[row.r] : a.g_e[row.r];
+    }
+    return row;
+}
+
+__device__ __forceinline__ int64_t psi_x(const yee3d_desc& d, int r, int j, int k)
+{
+    return ((int64_t)r * d.ny + j) * d.nz + k;
+}
+__device__ __forceinline__ int64_t psi_y(const yee3d_desc& d, int i, int r, int k)
+{
+    return ((int64_t)i * d.cpml[1].n_rows + r) * d.nz + k;
+}
+__device__ __forceinline__ int64_t psi_z(const yee3d_desc& d, int i, int j, int r)
+{
+    return ((int64_t)i * d.ny + j) * d.cpml[2].n_rows + r;
+}
+
+// Per-row (y) constants of a tile row.
+struct RowY {
+    int j;          // folded index
+    bool own;       // an owned row (stores E and H)
+    bool zero;      // the lo halo of row 0 on a non-periodic axis: its H reads 0
+    float ih, ie;
+    Row ch, ce;     // y CPML rows of the H and E kinds
+};
+
+// Values of one cell column the H update needs from the current plane.
+struct Cell {
+    float ex, ey, ez;
+    int id;
+};
+
+// New H of one cell.
+struct HNew {
+    float hx, hy, hz;
+};
+
+// H update of one cell at (i, j, k) from E at the cell (ex, ey, ez), at j+1 (ez_yp, ex_yp), at k+1 (ey_zp, ex_zp) and
+// at i+1 (ey_xp, ez_xp), with the psi of every layer the cell lies in (read from set `par`; the owner, store = true,
+// writes the new psi to set par ^ 1).
+__device__ __forceinline__ HNew h_cell(const yee3d_desc& d, int par, bool store, float da, float db, float hx,
+                                       float hy, float hz, float ex, float ey, float ez, float ez_yp, float ex_yp,
+                                       float ey_zp, float ex_zp, float ey_xp, float ez_xp, float ihx, float ihy,
+                                       float ihz, const Row& rx, const Row& ry, const Row& rz, int i, int j, int k)
+{
+    const float dez_y = ez_yp - ez, dey_z = ey_zp - ey;
+    const float dex_z = ex_zp - ex, dez_x = ez_xp - ez;
+    const float dey_x = ey_xp - ey, dex_y = ex_yp - ex;
+    float vx = hx_yee(da, db, hx, dez_y, ihy, dey_z, ihz);
+    float vy = hy_yee(da, db, hy, dex_z, ihz, dez_x, ihx);
+    float vz = hz_yee(da, db, hz, dey_x, ihx, dex_y, ihy);
+    const int out = par ^ 1;
+    if (rx.r >= 0) {   // x layers: H_y from E_z (+), H_z from E_y (-)
+        const int64_t p = psi_x(d, rx.r, j, k);
+        const float a = psi_next(rx.b, d.cpml[0].psi_h[par][0][p], rx.g, dez_x);
+        const float b = psi_next(rx.b, d.cpml[0].psi_h[par][1][p], rx.g, dey_x);
+        if (store) {
+            d.cpml[0].psi_h[out][0][p] = a;
+            d.cpml[0].psi_h[out][1][p] = b;
+        }
+        vy += db * a;
+        vz -= db * b;
+    }
+    if (ry.r >= 0) {   // y layers: H_x from E_z (-), H_z from E_x (+)
+        const int64_t p = psi_y(d, i, ry.r, k);
+        const float a = psi_next(ry.b, d.cpml[1].psi_h[par][0][p], ry.g, dez_y);
+        const float b = psi_next(ry.b, d.cpml[1].psi_h[par][1][p], ry.g, dex_y);
+        if (store) {
+            d.cpml[1].psi_h[out][0][p] = a;
+            d.cpml[1].psi_h[out][1][p] = b;
+        }
+        vx -= db * a;
+        vz += db * b;
+    }
+    if (rz.r >= 0) {   // z layers: H_x from E_y (+), H_y from E_x (-)
+        const int64_t p = psi_z(d, i, j, rz.r);
+        const float a = psi_next(rz.b, d.cpml[2].psi_h[par][0][p], rz.g, dey_z);
+        const float b = psi_next(rz.b, d.cpml[2].psi_h[par][1][p], rz.g, dex_z);
+        if (store) {
+            d.cpml[2].psi_h[out][0][p] = a;
+            d.cpml[2].psi_h[out][1][p] = b;
+        }
+        vx += db * a;
+        vy -= db * b;
+    }
+    return HNew{vx, vy, vz};
+}
+
+// E update of one owned cell with its CPML (psi in place) and point sources.
+template <bool LDS>
+__device__ __forceinline__ void e_cell(const yee3d_desc& d, int64_t n, float& ex, float& ey, float& ez, float2 fx,
+                                       float2 fy, float2 fz, float dhz_y, float dhy_z, float dhx_z, float dhz_x,
+                                       float dhy_x, float dhx_y, float iex, float iey, float iez, const Row& rx,
+                                       const Row& ry, const Row& rz, int i, int j, int k)
+{
+    float vx = fx.x * ex + fx.y * (dhz_y * iey - dhy_z * iez);
+    float vy = fy.x * ey + fy.y * (dhx_z * iez - dhz_x * iex);
+    float vz = fz.x * ez + fz.y * (dhy_x * iex - dhx_y * iey);
+    if (rx.r >= 0) {   // x layers: E_y from H_z (-), E_z from H_y (+)
+        const int64_t p = psi_x(d, rx.r, j, k);
+        float* pf = d.cpml[0].psi_e[0];
+        float* ps = d.cpml[0].psi_e[1];
+        const float a = psi_next(rx.b, pf[p], rx.g, dhz_x);
+        const float b = psi_next(rx.b, ps[p], rx.g, dhy_x);
+        pf[p] = a;
+        ps[p] = b;
+        vy -= fy.y * a;
+        vz += fz.y * b;
+    }
+    if (ry.r >= 0) {   // y layers: E_x from H_z (+), E_z from H_x (-)
+        const int64_t p = psi_y(d, i, ry.r, k);
+        float* pf = d.cpml[1].psi_e[0];
+        float* ps = d.cpml[1].psi_e[1];
+        const float a = psi_next(ry.b, pf[p], ry.g, dhz_y);
+        const float b = psi_next(ry.b, ps[p], ry.g, dhx_y);
+        pf[p] = a;
+        ps[p] = b;
+        vx += fx.y * a;
+        vz -= fz.y * b;
+    }
+    if (rz.r >= 0) {   // z layers: E_x from H_y (-), E_y from H_x (+)
+        const int64_t p = psi_z(d, i, j, rz.r);
+        float* pf = d.cpml[2].psi_e[0];
+        float* ps = d.cpml[2].psi_e[1];
+        const float a = psi_next(rz.b, pf[p], rz.g, dhy_z);
+        const float b = psi_next(rz.b, ps[p], rz.g, dhx_z);
+        pf[p] = a;
+        ps[p] = b;
+        vx -= fx.y * a;
+        vy += fy.y * b;
+    }
+    for (int s = 0; s < d.n_sources; ++s) {
+        const yee_point_source& src = d.source[s];
+        if (src.i == i && src.j == j && src.k == k && n < src.wave_len) {
+            const float add = __fmul_rn(src.weight, src.wave[n]);
+            if (src.component == 0) vx += add;
+            else if (src.component == 1) vy += add;
+            else vz += add;
+        }
+    }
+    ex = vx;
+    ey = vy;
+    ez = vz;
+}
+
+// One wavefront tile: planes [i0, i1), rows j0..j0+J-1, z cells tk*OWNED .. +OWNED-1.
+template <int J, bool LDS>
+__device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, int tk)
+{
+    const int lane = threadIdx.x & (LANES - 1);
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const bool px = d.periodic[0] != 0, py = d.periodic[1] != 0, pz = d.periodic[2] != 0;
+    const int64_t sy = nz, sx = (int64_t)ny * nz;
+    const int par = (int)(n & 1);
+    const float* __restrict__ e_o[3] = {d.field[par][0], d.field[par][1], d.field[par][2]};
+    const float* __restrict__ h_o[3] = {d.field[par][3], d.field[par][4], d.field[par][5]};
+    float* __restrict__ e_n[3] = {d.field[par ^ 1][0], d.field[par ^ 1][1], d.field[par ^ 1][2]};
+    float* __restrict__ h_n[3] = {d.field[par ^ 1][3], d.field[par ^ 1][4], d.field[par ^ 1][5]};
+    const uint16_t* __restrict__ ids = d.material;
+
+    // z of this lane
+    const int kraw = tk * OWNED - 1 + lane;
+    const int k = fold(kraw, nz, pz);
+    const bool k_zero = !pz && kraw < 0;                      // lo neighbour of cell 0: H reads 0
+    const bool k_own = lane >= 1 && lane <= OWNED && kraw < nz;
+    const float ihz = d.inv_h[2][k], iez = d.inv_e[2][k];
+    const Row rz_h = cpml_row(d.cpml[2], true, k), rz_e = cpml_row(d.cpml[2], false, k);
+
+    // rows: 0 = halo j0-1, 1..J owned, J+1 = the row above (E only)
+    RowY rows[J + 2];
+#pragma unroll
+    for (int r = 0; r < J + 2; ++r) {
+        const int jraw = tj * J - 1 + r;
+        rows[r].j = fold(jraw, ny, py);
+        rows[r].own = r >= 1 && r <= J && jraw < ny;
+        rows[r].zero = !py && jraw < 0;
+        rows[r].ih = d.inv_h[1][rows[r].j];
+        rows[r].ie = d.inv_e[1][rows[r].j];
+        rows[r].ch = cpml_row(d.cpml[1], true, rows[r].j);
+        rows[r].ce = cpml_row(d.cpml[1], false, rows[r].j);
+    }
+
+    auto load_plane = [&](int i, Cell (&c)[J + 2]) {
+        const int64_t base = (int64_t)i * sx + k;
+#pragma unroll
+        for (int r = 0; r < J + 2; ++r) {
+            const int64_t o = base + (int64_t)rows[r].j * sy;
+            c[r].ex = e_o[0][o];
+            c[r].ey = e_o[1][o];
+            c[r].ez = e_o[2][o];
+            c[r].id = ids[o];
+        }
+    };
+
+    // H of plane i for rows lo..J (the E-feeding components are all computed; the callers use what they need).
+    // cur: plane i, nxt: plane i+1 (folded). Returns new H per row.
+    auto h_plane = [&](int i, const Cell (&cur)[J + 2], const Cell (&nxt)[J + 2], HNew (&out)[J + 1],
+                       const float (&hold)[J + 1][3], int r_lo) {
+        const float ihx = d.inv_h[0][i];
+        const Row rx = cpml_row(d.cpml[0], true, i);
+#pragma unroll
+        for (int r = 0; r < J + 1; ++r) {
+            if (r < r_lo) continue;
+            const float ey_zp = from_next(cur[r].ey);
+            const float ex_zp = from_next(cur[r].ex);
+            const float4 c = coef<LDS>(d, cur[r].id);
+            out[r] = h_cell(d, par, rows[r].own && k_own, c.x, c.y, hold[r][0], hold[r][1], hold[r][2], cur[r].ex,
+                            cur[r].ey, cur[r].ez, cur[r + 1].ez, cur[r + 1].ex, ey_zp, ex_zp, nxt[r].ey, nxt[r].ez,
+                            ihx, rows[r].ih, ihz, rx, rows[r].ch, rz_h, i, rows[r].j, k);
+        }
+    };
+
+    auto load_hold = [&](int i, float (&hold)[J + 1][3], int r_lo) {
+        const int64_t base = (int64_t)i * sx + k;
+#pragma unroll
+        for (int r = 0; r < J + 1; ++r) {
+            if (r < r_lo) continue;
+            const int64_t o = base + (int64_t)rows[r].j * sy;
+            hold[r][0] = h_o[0][o];
+            hold[r][1] = h_o[1][o];
+            hold[r][2] = h_o[2][o];
+        }
+    };
+
+    Cell cur[J + 2], nxt[J + 2];
+    float hold[J + 1][3];
+    HNew hn[J + 1];
+    float hy_prev[J + 1], hz_prev[J + 1];   // new H_y, H_z of plane i-1, rows 1..J
+
+    // prologue: new H of plane i0 - 1 (rows 1..J), or 0 below cell 0 of a non-periodic x axis
+    load_plane(i0, cur);
+    if (i0 > 0 || px) {
+        const int im = i0 > 0 ? i0 - 1 : nx - 1;
+        Cell prev[J + 2];
+        load_plane(im, prev);
+        load_hold(im, hold, 1);
+        const float ihx = d.inv_h[0][im];
+        const Row rx = cpml_row(d.cpml[0], true, im);
+#pragma unroll
+        for (int r = 1; r < J + 1; ++r) {
+            const float ey_zp = from_next(prev[r].ey);
+            const float ex_zp = from_next(prev[r].ex);
+            const float4 c = coef<LDS>(d, prev[r].id);
+            const HNew h = h_cell(d, par, false, c.x, c.y, hold[r][0], hold[r][1], hold[r][2], prev[r].ex, prev[r].ey,
+                                  prev[r].ez, prev[r + 1].ez, prev[r + 1].ex, ey_zp, ex_zp, cur[r].ey, cur[r].ez, ihx,
+                                  rows[r].ih, ihz, rx, rows[r].ch, rz_h, im, rows[r].j, k);
+            hy_prev[r] = h.hy;
+            hz_prev[r] = h.hz;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < J + 1; ++r) hy_prev[r] = hz_prev[r] = 0.f;
+    }
+
+    for (int i = i0; i < i1; ++i) {
+        const int ip = fold(i + 1, nx, px);
+        load_plane(ip, nxt);
+        load_hold(i, hold, 0);
+        h_plane(i, cur, nxt, hn, hold, 0);
+        // zero the H that stands for "below cell 0" on non-periodic axes
+        if (rows[0].zero) { hn[0].hx = 0.f; hn[0].hz = 0.f; }
+        if (k_zero) {
+#pragma unroll
+            for (int r = 0; r < J + 1; ++r) { hn[r].hx = 0.f; hn[r].hy = 0.f; }
+        }
+        const float iex = d.inv_e[0][i];
+        const Row rx_e = cpml_row(d.cpml[0], false, i);
+        const int64_t plane = (int64_t)i * sx + k;
+#pragma unroll
+        for (int r = 1; r < J + 1; ++r) {
+            const float hx_km = from_prev(hn[r].hx);
+            const float hy_km = from_prev(hn[r].hy);
+            const int id_zp = from_next_i(cur[r].id);
+            const float dhz_y = hn[r].hz - hn[r - 1].hz;
+            const float dhy_z = hn[r].hy - hy_km;
+            const float dhx_z = hn[r].hx - hx_km;
+            const float dhz_x = hn[r].hz - hz_prev[r];
+            const float dhy_x = hn[r].hy - hy_prev[r];
+            const float dhx_y = hn[r].hx - hn[r - 1].hx;
+            if (rows[r].own && k_own) {
+                const int id = cur[r].id;
+                const float2 fx = face<LDS>(d, id, nxt[r].id);
+                const float2 fy = face<LDS>(d, id, cur[r + 1].id);
+                const float2 fz = face<LDS>(d, id, id_zp);
+                float ex = cur[r].ex, ey = cur[r].ey, ez = cur[r].ez;
+                e_cell<LDS>(d, n, ex, ey, ez, fx, fy, fz, dhz_y, dhy_z, dhx_z, dhz_x, dhy_x, dhx_y, iex, rows[r].ie,
+                            iez, rx_e, rows[r].ce, rz_e, i, rows[r].j, k);
+                const int64_t o = plane + (int64_t)rows[r].j * sy;
+                e_n[0][o] = ex;
+                e_n[1][o] = ey;
+                e_n[2][o] = ez;
+                h_n[0][o] = hn[r].hx;
+                h_n[1][o] = hn[r].hy;
+                h_n[2][o] = hn[r].hz;
+            }
+            hy_prev[r] = hn[r].hy;
+            hz_prev[r] = hn[r].hz;
+        }
+#pragma unroll
+        for (int r = 0; r < J + 2; ++r) cur[r] = nxt[r];
+    }
+}
+
+}  // namespace yee3
+
+// One 3D instance: tile t = (i chunk, j tile, k tile), k fastest, one per wavefront.
+template <int J>
+__global__ void __launch_bounds__(256) yee3d_step_kernel(yee3d_desc d, int32_t tiles_j, int32_t tiles_k,
+                                                        int32_t tiles, int32_t offset)
+{
+    const int64_t n = *d.clock + offset;
+    if (n >= d.step_limit) return;
+    const int tile = blockIdx.x * 4 + yee3::wave_index();
+    const bool mine = tile < tiles;
+    if (!__syncthreads_or(mine)) return;
+    const bool lds = yee3::stage_tables(d);
+    if (!mine) return;
+    const int tk = tile % tiles_k;
+    const int tj = (tile / tiles_k) % tiles_j;
+    const int ic = tile / (tiles_k * tiles_j);
+    const int i0 = ic * d.planes_per_tile;
+    const int i1 = min(d.nx, i0 + d.planes_per_tile);
+    if (lds)
+        yee3::tile3d<J, true>(d, n, i0, i1, tj, tk);
+    else
+        yee3::tile3d<J, false>(d, n, i0, i1, tj, tk);
+}
+
+template __global__ void yee3d_step_kernel<4>(yee3d_desc, int32_t, int32_t, int32_t, int32_t);
+
+// FP64 Kahan DFT samples of the flux planes after step n (when (n + 1) % m == 0): one thread per plane cell and pair;
+// E = mean of the layers edge - 1 and edge along the normal, H on layer edge - 1 (monitors3d.py:90-101).
+__global__ void __launch_bounds__(256) yee3d_sample_kernel(yee3d_desc d, int64_t cells_total, int32_t offset)
+{
+    const int64_t n = *d.clock + offset;
+    if (n >= d.step_limit || d.n_planes == 0) return;
+    const int m = d.sample_interval > 0 ? d.sample_interval : 1;
+    if ((n + 1) % m != 0) return;
+    const int64_t sample = (n + 1) / m - 1;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cells_total) return;
+    const int set = (int)((n & 1) ^ 1);
+    for (int p = 0; p < d.n_planes; ++p) {
+        const yee_flux_plane& pl = d.plane[p];
+        for (int q = 0; q < 2; ++q) {
+            const int na = pl.hi[q][0] - pl.lo[q][0], nb = pl.hi[q][1] - pl.lo[q][1];
+            const int64_t count = (int64_t)na * nb;
+            if (t >= count) {
+                t -= count;
+                continue;
+            }
+            if (sample >= pl.n_samples) return;
+            const int a = pl.lo[q][0] + (int)(t / nb), b = pl.lo[q][1] + (int)(t % nb);
+            const int ta = pl.normal == 0 ? 1 : 0, tb = pl.normal == 2 ? 1 : 2;   // tangential axes in grid order
+            int idx[3];
+            idx[ta] = a;
+            idx[tb] = b;
+            idx[pl.normal] = pl.edge - 1;
+            const int64_t lower = ((int64_t)idx[0] * d.ny + idx[1]) * d.nz + idx[2];
+            idx[pl.normal] = pl.edge;
+            const int64_t upper = ((int64_t)idx[0] * d.ny + idx[1]) * d.nz + idx[2];
+            const float* e = d.field[set][pl.e_comp[q]];
+            const float* h = d.field[set][3 + pl.h_comp[q]];
+            const double e_value = 0.5 * ((double)e[lower] + (double)e[upper]);
+            const double h_value = (double)h[lower];
+            const int nf = pl.n_freq;
+            const double* ph = pl.phase + sample * (int64_t)nf * 2;
+            for (int f = 0; f < nf; ++f) {
+                const double pr = ph[2 * f], pi = ph[2 * f + 1];
+                double* acc_e = pl.acc[q] + ((int64_t)(0 * nf + f) * 4) * count + t;
+                double* acc_h = pl.acc[q] + ((int64_t)(1 * nf + f) * 4) * count + t;
+                yee::kahan_add(acc_e, acc_e + 2 * count, pr * e_value);
+                yee::kahan_add(acc_e + count, acc_e + 3 * count, pi * e_value);
+                yee::kahan_add(acc_h, acc_h + 2 * count, pr * h_value);
+                yee::kahan_add(acc_h + count, acc_h + 3 * count, pi * h_value);
+            }
+            return;
+        }
+    }
+}

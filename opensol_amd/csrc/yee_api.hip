@@ -18,6 +18,9 @@ __global__ void yee2d_step_kernel(yee2d_desc d, yee2d_tile_sets sets, int32_t of
 template <int VEC>
 __global__ void yee2d_batch_kernel(const yee2d_desc* __restrict__ descs, const int32_t* __restrict__ prefix,
                                    int32_t n_inst, int32_t offset);
+template <int J>
+__global__ void yee3d_step_kernel(yee3d_desc d, int32_t tiles_j, int32_t tiles_k, int32_t tiles, int32_t offset);
+__global__ void yee3d_sample_kernel(yee3d_desc d, int64_t cells_total, int32_t offset);
 __global__ void yee_clock_advance(int64_t* clock, int64_t k, int64_t limit);
 __global__ void yee_clock_advance_batch(const yee2d_desc* __restrict__ descs, int32_t n_inst, int64_t k);
 
@@ -42,7 +45,12 @@ constexpr int kMinAutoRows = 8;    // shortest tile picked automatically (each t
 constexpr int kMaxAutoRows = 32;   // longest: many short tiles keep every CU busy to the end of the step
 constexpr int kEdgeSplit = 8;     // row chunks per edge tile (yee2d.hip EDGE_SPLIT)
 
-enum CtxKind { kSingle2D = 1, kBatch2D = 2 };
+enum CtxKind { kSingle2D = 1, kBatch2D = 2, kSingle3D = 3 };
+
+constexpr int kRows3D = 4;         // y rows per 3D wavefront tile (yee3d.hip J)
+constexpr int kOwned3D = 62;       // z cells a 3D wavefront tile owns (yee3d.hip OWNED)
+constexpr int kMinAutoPlanes = 4;
+constexpr int kMaxAutoPlanes = 32;
 
 }  // namespace
 
@@ -59,6 +67,10 @@ struct yee_ctx {
     int blocks = 0;        // blocks of the bulk kernel (4 tiles each)
     int edge_blocks = 0;   // blocks of the edge kernel (kEdgeSplit wavefronts per tile, 4 per block)
     yee2d_tile_sets sets{};
+    // 3D
+    yee3d_desc d3{};
+    int32_t tiles_j3 = 0, tiles_k3 = 0;
+    int64_t sample_cells = 0;   // plane cells x pairs of the sample launch
     // graph replay
     int32_t graph_steps = 0;
     hipGraphExec_t graph = nullptr;
@@ -112,6 +124,60 @@ int32_t check_2d(const yee2d_desc* d)
         if (mon.position < 0 || mon.position >= n_pos) return fail(YEE_EINVAL, "monitor line outside the grid");
         if (mon.lo < 0 || mon.hi > n_along || mon.lo >= mon.hi) return fail(YEE_EINVAL, "monitor span out of range");
         if (mon.n_freq < 1 || !mon.phase || !mon.acc) return fail(YEE_EINVAL, "monitor arrays missing");
+    }
+    if (!d->clock) return fail(YEE_EINVAL, "clock missing");
+    return YEE_OK;
+}
+
+int32_t check_3d(const yee3d_desc* d)
+{
+    if (!d) return fail(YEE_EINVAL, "null descriptor");
+    const int n[3] = {d->nx, d->ny, d->nz};
+    for (int a = 0; a < 3; ++a)
+        if (n[a] < 2) return fail(YEE_EINVAL, "3D grid needs nx, ny, nz >= 2");
+    if ((int64_t)d->nx * d->ny * d->nz > ((int64_t)1 << 40)) return fail(YEE_EINVAL, "3D grid too large");
+    if (d->rows_per_tile != kRows3D) return fail(YEE_EINVAL, "rows_per_tile must be 4 (or 0)");
+    if (d->planes_per_tile < 1) return fail(YEE_EINVAL, "planes_per_tile must be >= 1");
+    for (int s = 0; s < 2; ++s)
+        for (int c = 0; c < 6; ++c)
+            if (!d->field[s][c]) return fail(YEE_EINVAL, "null field buffer");
+    if (!d->material || !d->table || d->n_materials < 1) return fail(YEE_EINVAL, "material ids / table missing");
+    for (int a = 0; a < 3; ++a) {
+        if (!d->inv_e[a] || !d->inv_h[a]) return fail(YEE_EINVAL, "spacings missing");
+        const yee3d_cpml_axis& c = d->cpml[a];
+        if (c.n_rows < 0) return fail(YEE_EINVAL, "negative CPML rows");
+        if (c.n_rows > 0) {
+            if (d->periodic[a]) return fail(YEE_EINVAL, "a periodic axis has no CPML");
+            if (!c.row_e || !c.row_h || !c.b_e || !c.g_e || !c.b_h || !c.g_h || !c.psi_e[0] || !c.psi_e[1] ||
+                !c.psi_h[0][0] || !c.psi_h[0][1] || !c.psi_h[1][0] || !c.psi_h[1][1])
+                return fail(YEE_EINVAL, "CPML arrays missing");
+            if (c.n_rows > n[a]) return fail(YEE_EINVAL, "more CPML rows than cells");
+        }
+    }
+    if (d->n_sources < 0 || d->n_sources > YEE3D_MAX_SOURCES) return fail(YEE_EINVAL, "n_sources out of range");
+    for (int s = 0; s < d->n_sources; ++s) {
+        const yee_point_source& src = d->source[s];
+        if (src.component < 0 || src.component > 2) return fail(YEE_EINVAL, "source component must be 0..2");
+        if (src.i < 0 || src.i >= d->nx || src.j < 0 || src.j >= d->ny || src.k < 0 || src.k >= d->nz)
+            return fail(YEE_EINVAL, "source cell outside the grid");
+        if (!src.wave) return fail(YEE_EINVAL, "source wave missing");
+    }
+    if (d->n_planes < 0 || d->n_planes > YEE3D_MAX_PLANES) return fail(YEE_EINVAL, "n_planes out of range");
+    if (d->n_planes > 0 && d->sample_interval < 1) return fail(YEE_EINVAL, "sample_interval must be >= 1");
+    for (int p = 0; p < d->n_planes; ++p) {
+        const yee_flux_plane& pl = d->plane[p];
+        if (pl.normal < 0 || pl.normal > 2) return fail(YEE_EINVAL, "plane normal must be 0..2");
+        if (pl.edge < 1 || pl.edge > n[pl.normal] - 1) return fail(YEE_EINVAL, "plane edge needs a cell on both sides");
+        const int axes[2] = {pl.normal == 0 ? 1 : 0, pl.normal == 2 ? 1 : 2};
+        for (int q = 0; q < 2; ++q) {
+            if (pl.e_comp[q] < 0 || pl.e_comp[q] > 2 || pl.h_comp[q] < 0 || pl.h_comp[q] > 2)
+                return fail(YEE_EINVAL, "plane components must be 0..2");
+            for (int t = 0; t < 2; ++t)
+                if (pl.lo[q][t] < 0 || pl.hi[q][t] > n[axes[t]] || pl.lo[q][t] >= pl.hi[q][t])
+                    return fail(YEE_EINVAL, "plane range out of the grid");
+            if (!pl.acc[q]) return fail(YEE_EINVAL, "plane accumulators missing");
+        }
+        if (pl.n_freq < 1 || !pl.phase) return fail(YEE_EINVAL, "plane phases missing");
     }
     if (!d->clock) return fail(YEE_EINVAL, "clock missing");
     return YEE_OK;
@@ -222,6 +288,39 @@ void set_geometry(yee_ctx* ctx)
     ctx->edge_blocks = (int)((edge * kEdgeSplit + 3) / 4);
 }
 
+int64_t wave_slots(const void* kernel);
+
+// Defaults of a 3D descriptor: 4 rows per tile; planes_per_tile so the tiles fill the device's wave slots in whole
+// rounds, clamped to [kMinAutoPlanes, kMaxAutoPlanes].
+void apply_defaults_3d(yee3d_desc& d)
+{
+    if (d.rows_per_tile == 0) d.rows_per_tile = kRows3D;
+    if (d.planes_per_tile == 0 && d.nx > 0 && d.ny > 0 && d.nz > 0) {
+        const int64_t slots = wave_slots(reinterpret_cast<const void*>(yee3d_step_kernel<kRows3D>));
+        const int64_t columns = (int64_t)((d.ny + kRows3D - 1) / kRows3D) * ((d.nz + kOwned3D - 1) / kOwned3D);
+        int64_t planes = slots > 0 ? (d.nx * columns + slots - 1) / slots : 16;
+        planes = std::max<int64_t>(kMinAutoPlanes, std::min<int64_t>(kMaxAutoPlanes, planes));
+        if (planes > d.nx) planes = d.nx;
+        const int64_t chunks = (d.nx + planes - 1) / planes;
+        d.planes_per_tile = (int32_t)((d.nx + chunks - 1) / chunks);
+    }
+}
+
+void set_geometry_3d(yee_ctx* ctx)
+{
+    const yee3d_desc& d = ctx->d3;
+    ctx->tiles_j3 = (d.ny + kRows3D - 1) / kRows3D;
+    ctx->tiles_k3 = (d.nz + kOwned3D - 1) / kOwned3D;
+    const int64_t chunks = (d.nx + d.planes_per_tile - 1) / d.planes_per_tile;
+    ctx->tiles = chunks * ctx->tiles_j3 * ctx->tiles_k3;
+    ctx->blocks = (int)((ctx->tiles + 3) / 4);
+    int64_t cells = 0;
+    for (int p = 0; p < d.n_planes; ++p)
+        for (int q = 0; q < 2; ++q)
+            cells += (int64_t)(d.plane[p].hi[q][0] - d.plane[p].lo[q][0]) * (d.plane[p].hi[q][1] - d.plane[p].lo[q][1]);
+    ctx->sample_cells = cells;
+}
+
 void drop_graph(yee_ctx* ctx)
 {
     if (ctx->graph) {
@@ -246,6 +345,21 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
         YEE_HIP(hipGetLastError());
         hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, stream, ctx->d2.clock, (int64_t)count,
                            ctx->d2.step_limit);
+        YEE_HIP(hipGetLastError());
+        return YEE_OK;
+    }
+    if (ctx->kind == kSingle3D) {
+        const int sample_blocks = (int)((ctx->sample_cells + 255) / 256);
+        for (int32_t k = 0; k < count; ++k) {
+            hipLaunchKernelGGL(yee3d_step_kernel<kRows3D>, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d3,
+                               ctx->tiles_j3, ctx->tiles_k3, (int32_t)ctx->tiles, k);
+            if (sample_blocks > 0)
+                hipLaunchKernelGGL(yee3d_sample_kernel, dim3(sample_blocks), dim3(256), 0, stream, ctx->d3,
+                                   ctx->sample_cells, k);
+        }
+        YEE_HIP(hipGetLastError());
+        hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, stream, ctx->d3.clock, (int64_t)count,
+                           ctx->d3.step_limit);
         YEE_HIP(hipGetLastError());
         return YEE_OK;
     }
@@ -298,6 +412,10 @@ size_t yee_sizeof(int32_t which)
         case 1: return sizeof(yee_line_source);
         case 2: return sizeof(yee_flux_line);
         case 3: return sizeof(yee_cpml_axis);
+        case 4: return sizeof(yee3d_desc);
+        case 5: return sizeof(yee_point_source);
+        case 6: return sizeof(yee_flux_plane);
+        case 7: return sizeof(yee3d_cpml_axis);
         default: return 0;
     }
 }
@@ -389,6 +507,101 @@ int32_t yee_set_graph_steps(yee_ctx* ctx, int32_t graph_steps)
     return YEE_OK;
 }
 
+// Cells in the bulk tile set: the non-edge tile rows times the non-edge tile columns (the set is their product).
+int64_t bulk_cells(const yee2d_desc& d, const yee2d_tile_sets& s)
+{
+    if (s.all_edge) return 0;
+    const int cols = 64 * d.lane_columns;
+    int64_t rows = 0, columns = 0;
+    for (int ti = 0, q = 0; ti < s.tiles_i; ++ti) {
+        if (q < s.n_rows && s.rows[q] == ti) { ++q; continue; }
+        rows += std::min(d.nx, (ti + 1) * d.rows_per_tile) - ti * d.rows_per_tile;
+    }
+    for (int tj = 0, q = 0; tj < s.tiles_j; ++tj) {
+        if (q < s.n_cols && s.cols[q] == tj) { ++q; continue; }
+        columns += std::min(d.ny, (tj + 1) * cols) - tj * cols;
+    }
+    return rows * columns;
+}
+
+// yee_step_timed of a 3D context: class 0 = the fused step kernel (every cell), 1 = the plane-sample kernel.
+int32_t step_timed_3d(yee_ctx* ctx, int64_t n_steps, hipStream_t s, double* kernel_ms, int64_t* cells)
+{
+    const yee3d_desc& d = ctx->d3;
+    cells[0] = (int64_t)d.nx * d.ny * d.nz;
+    cells[1] = ctx->sample_cells;
+    kernel_ms[0] = kernel_ms[1] = 0.0;
+    const int sample_blocks = (int)((ctx->sample_cells + 255) / 256);
+    hipEvent_t ev[3];
+    for (auto& e : ev) YEE_HIP(hipEventCreate(&e));
+    int32_t status = YEE_OK;
+    for (int64_t k = 0; k < n_steps && status == YEE_OK; ++k) {
+        (void)hipEventRecord(ev[0], s);
+        hipLaunchKernelGGL(yee3d_step_kernel<kRows3D>, dim3(ctx->blocks), dim3(256), 0, s, d, ctx->tiles_j3,
+                           ctx->tiles_k3, (int32_t)ctx->tiles, 0);
+        (void)hipEventRecord(ev[1], s);
+        if (sample_blocks > 0)
+            hipLaunchKernelGGL(yee3d_sample_kernel, dim3(sample_blocks), dim3(256), 0, s, d, ctx->sample_cells, 0);
+        (void)hipEventRecord(ev[2], s);
+        hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, s, d.clock, (int64_t)1, d.step_limit);
+        hipError_t e = hipEventSynchronize(ev[2]);
+        float a = 0.f, b = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&a, ev[0], ev[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
+        if (e != hipSuccess) {
+            status = fail(YEE_EHIP, hipGetErrorString(e));
+            break;
+        }
+        kernel_ms[0] += a;
+        kernel_ms[1] += b;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (status == YEE_OK) YEE_HIP(hipStreamSynchronize(s));
+    return status;
+}
+
+int32_t yee_step_timed(yee_ctx* ctx, int64_t n_steps, void* stream, double* kernel_ms, int64_t* cells)
+{
+    if (!ctx || !ctx->kind) return fail(YEE_ESTATE, "null or destroyed context");
+    if (ctx->kind != kSingle2D && ctx->kind != kSingle3D)
+        return fail(YEE_EINVAL, "yee_step_timed times a single-instance context");
+    if (n_steps < 0 || !kernel_ms || !cells) return fail(YEE_EINVAL, "n_steps must be >= 0 and outputs non-null");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (ctx->kind == kSingle3D) return step_timed_3d(ctx, n_steps, s, kernel_ms, cells);
+    const StepKernel bulk = step_kernel(ctx->d2, false);
+    const StepKernel edge = step_kernel(ctx->d2, true);
+    cells[0] = bulk_cells(ctx->d2, ctx->sets);
+    cells[1] = (int64_t)ctx->d2.nx * ctx->d2.ny - cells[0];
+    kernel_ms[0] = kernel_ms[1] = 0.0;
+    hipEvent_t ev[3];
+    for (auto& e : ev) YEE_HIP(hipEventCreate(&e));
+    int32_t status = YEE_OK;
+    for (int64_t k = 0; k < n_steps && status == YEE_OK; ++k) {
+        (void)hipEventRecord(ev[0], s);
+        if (ctx->blocks > 0)
+            hipLaunchKernelGGL(bulk, dim3(ctx->blocks), dim3(256), 0, s, ctx->d2, ctx->sets, 0);
+        (void)hipEventRecord(ev[1], s);
+        if (ctx->edge_blocks > 0)
+            hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, s, ctx->d2, ctx->sets, 0);
+        (void)hipEventRecord(ev[2], s);
+        hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, s, ctx->d2.clock, (int64_t)1,
+                           ctx->d2.step_limit);
+        hipError_t e = hipEventSynchronize(ev[2]);
+        float a = 0.f, b = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&a, ev[0], ev[1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&b, ev[1], ev[2]);
+        if (e != hipSuccess) {
+            status = fail(YEE_EHIP, hipGetErrorString(e));
+            break;
+        }
+        kernel_ms[0] += a;
+        kernel_ms[1] += b;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (status == YEE_OK) YEE_HIP(hipStreamSynchronize(s));
+    return status;
+}
+
 int32_t yee_step(yee_ctx* ctx, int64_t n_steps, void* stream)
 {
     if (!ctx || !ctx->kind) return fail(YEE_ESTATE, "null or destroyed context");
@@ -411,6 +624,42 @@ int32_t yee_step(yee_ctx* ctx, int64_t n_steps, void* stream)
         if (status != YEE_OK) return status;
         remaining -= chunk;
     }
+    return YEE_OK;
+}
+
+int32_t yee3d_create(const yee3d_desc* desc, yee_ctx** out)
+{
+    if (!out) return fail(YEE_EINVAL, "null output pointer");
+    *out = nullptr;
+    yee3d_desc d = desc ? *desc : yee3d_desc{};
+    if (desc) apply_defaults_3d(d);
+    int32_t status = check_3d(desc ? &d : nullptr);
+    if (status != YEE_OK) return status;
+    yee_ctx* ctx = new yee_ctx();
+    ctx->kind = kSingle3D;
+    ctx->d3 = d;
+    set_geometry_3d(ctx);
+    *out = ctx;
+    return YEE_OK;
+}
+
+int32_t yee3d_update(yee_ctx* ctx, const yee3d_desc* desc)
+{
+    if (!ctx || ctx->kind != kSingle3D) return fail(YEE_ESTATE, "not a 3D context");
+    yee3d_desc d = desc ? *desc : yee3d_desc{};
+    if (desc) apply_defaults_3d(d);
+    int32_t status = check_3d(desc ? &d : nullptr);
+    if (status != YEE_OK) return status;
+    drop_graph(ctx);
+    ctx->d3 = d;
+    set_geometry_3d(ctx);
+    return YEE_OK;
+}
+
+int32_t yee3d_get_desc(const yee_ctx* ctx, yee3d_desc* out)
+{
+    if (!ctx || ctx->kind != kSingle3D || !out) return fail(YEE_ESTATE, "not a 3D context");
+    *out = ctx->d3;
     return YEE_OK;
 }
 

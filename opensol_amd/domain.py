@@ -51,7 +51,10 @@ class UniformGrid:
         self.shape = tuple(uniform_cell_count(total, dx) for total in totals)
         if min(self.shape) < 1:
             raise ValueError(f'grid has no cells: size {totals} m, dx {dx} m')
-        self.dt = courant_time_step([dx] * len(totals), courant)
+        # the 2D step takes the float64 width (core/grid.py:34-60); the 3D one the finest float32 cell width
+        # (core/grid3d.py:44-56)
+        spacing = float(dx) if len(totals) == 2 else float(self.dx)
+        self.dt = courant_time_step([spacing] * len(totals), courant)
         self.courant = DTYPE(DEFAULT_COURANT_FACTOR if courant is None else courant)
 
     @property
@@ -143,12 +146,22 @@ class Domain:
     def pos_to_cell(self, *coords):
         return tuple(self.pos_to_cell_axis(a, c) for a, c in enumerate(coords))
 
-    def box_ranges(self, lows, highs):
+    def host_centres(self, clamp_to_physical=True):
+        """Per-axis float32 centres; clamped: PML cells take the nearest window centre (core/domain.py:90-98)."""
+        if not clamp_to_physical:
+            return self._centres32
+        out = []
+        for centres, window in zip(self._centres, self.physical_window()):
+            out.append(np.clip(centres, centres[window.start], centres[window.stop - 1]).astype(DTYPE))
+        return tuple(out)
+
+    def box_ranges(self, lows, highs, clamp_to_physical=False):
         """Per-axis half-open index ranges of the cells whose float32 centre lies in [min, max) (the reference's
-        rect_mask / box mask compares float32 centres, core/domain.py:90-97, 255-260): a box is a product of
-        ranges, so it is painted in O(area)."""
+        rect_mask / box mask compares float32 centres, core/domain.py:90-97, 255-260; the 3D model builder's box raster
+        compares the centres clamped to the window, model/geometry.py:1762-1788): a box is a product of ranges, so it
+        is painted in O(area)."""
         ranges = []
-        for centres, lo, hi in zip(self._centres32, lows, highs):
+        for centres, lo, hi in zip(self.host_centres(clamp_to_physical), lows, highs):
             a, b = DTYPE(min(lo, hi)), DTYPE(max(lo, hi))
             ranges.append((int(np.searchsorted(centres, a, side='left')),
                            int(np.searchsorted(centres, b, side='left'))))

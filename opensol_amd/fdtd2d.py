@@ -5,6 +5,7 @@ the source and monitor objects (physics/em/__init__.py:63-557): geometry, a mate
 spacings, calibrated sheet sources and flux monitors, all host float64 rounded once to float32. Engine2D owns the
 device buffers (torch tensors on the ROCm device: memory only) and steps them with one fused HIP launch per step.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -408,6 +409,20 @@ class Engine2D:
         self._ensure_capacity(self.n_step + n_steps)
         _lib.check(self.lib.yee_step(self._ctx, n_steps, self.stream_handle()), 'yee_step')
         self.n_step += n_steps
+
+    def step_timed(self, n_steps):
+        """Run n_steps steps with a HIP event pair around every kernel (synchronous): {'bulk' / 'edge': (average
+        ms per launch, cells per launch)}."""
+        n_steps = int(n_steps)
+        if self.model.materials.version != self._materials_version:
+            self.materials_changed()
+        self._ensure_capacity(self.n_step + n_steps)
+        kernel_ms = (ctypes.c_double * 2)()
+        cells = (ctypes.c_int64 * 2)()
+        _lib.check(self.lib.yee_step_timed(self._ctx, n_steps, self.stream_handle(), kernel_ms, cells),
+                   'yee_step_timed')
+        self.n_step += n_steps
+        return {name: (kernel_ms[k] / max(n_steps, 1), int(cells[k])) for k, name in enumerate(('bulk', 'edge'))}
 
     @property
     def time(self):

@@ -82,3 +82,82 @@ class FluxMonitor:
         e_time_average = np.cos(np.pi * self.frequencies * self.dt)
         integrand = self.sign * e_time_average[:, None] * np.real(e_spectra * np.conj(h_spectra))
         return integrand @ self.cell_widths
+
+
+def face_samples(widths, first_edge, last_edge, wrap=False):
+    """(first face, last face + 1, trapezoid weights) of the samples on cell edges first..last: face f is the hi face
+    of cell f (edge f + 1), edge 0 has no face; wrap (a whole periodic axis): every face, dual widths
+    (physics/em3d/monitors3d.py:22-34)."""
+    widths = np.asarray(widths, dtype=np.float64)
+    if wrap:
+        return 0, widths.size, 0.5 * (widths + np.roll(widths, -1))
+    edges = np.arange(max(first_edge, 1), last_edge + 1)
+    weights = np.empty(edges.size)
+    for position, edge in enumerate(edges):
+        below = widths[edge - 1] if edge > first_edge else 0.0
+        above = widths[edge] if edge < last_edge else 0.0
+        weights[position] = 0.5 * (below + above)
+    return int(edges[0]) - 1, int(edges[-1]), weights
+
+
+class FluxPlane:
+    """Spectral Poynting flux through the cell-edge plane `edge` normal to axis `normal` (0 / 1 / 2) over the edge
+    ranges `bounds` of the tangential axes, > 0 toward +normal (physics/em3d/monitors3d.py:42-128). Two pairs: (E of
+    the first tangential axis, H of the second) with sign +1 and (E second, H first) with sign -1, first = (normal +
+    1) % 3. E is the mean of the two layers across the plane, H the layer below it; both are phased at the E time."""
+
+    def __init__(self, normal, edge, bounds, frequencies, dt, widths, shape, periodic):
+        if not 1 <= edge <= shape[normal] - 1:
+            raise ValueError(f"flux plane at {'xyz'[normal]} edge {edge} needs a cell on both sides (edges 1 to "
+                             f"{shape[normal] - 1})")
+        self.normal, self.edge = int(normal), int(edge)
+        self.frequencies = np.atleast_1d(np.asarray(frequencies, dtype=np.float64))
+        self.omega = 2.0 * np.pi * self.frequencies
+        self.n_freqs = self.frequencies.size
+        self.dt = float(dt)
+        self.sample_interval = 1
+        self.tangential = tuple(a for a in range(3) if a != normal)   # grid order
+        first, second = (normal + 1) % 3, (normal + 2) % 3
+        self.pairs = []
+        for e_axis, h_axis, face_axis, cell_axis, sign in ((first, second, first, second, 1.0),
+                                                           (second, first, second, first, -1.0)):
+            wrap = bool(periodic[face_axis]) and tuple(bounds[face_axis]) == (0, shape[face_axis])
+            f_lo, f_hi, face_weights = face_samples(widths[face_axis], *bounds[face_axis], wrap=wrap)
+            c_lo, c_hi = bounds[cell_axis]
+            cell_weights = np.asarray(widths[cell_axis][c_lo:c_hi], dtype=np.float64)
+            ranges = {face_axis: (f_lo, f_hi), cell_axis: (c_lo, c_hi)}
+            weights = {face_axis: face_weights, cell_axis: cell_weights}
+            a, b = self.tangential
+            self.pairs.append(dict(e_comp=e_axis, h_comp=h_axis, sign=sign, lo=(ranges[a][0], ranges[b][0]),
+                                   hi=(ranges[a][1], ranges[b][1]), area=np.outer(weights[a], weights[b])))
+        self.engine = None
+        self.index = None
+
+    def accumulator_shape(self, pair):
+        """(2 (E, H), n_freq, 4 (sum re, sum im, comp re, comp im), cells a, cells b) float64 on the device."""
+        p = self.pairs[pair]
+        return (2, self.n_freqs, 4, p['hi'][0] - p['lo'][0], p['hi'][1] - p['lo'][1])
+
+    def phase_table(self, sample_times):
+        """(n_samples, n_freq, 2) float64: exp(-j omega t) m dt at each sample time."""
+        phase = -np.outer(np.asarray(sample_times, dtype=np.float64), self.omega)
+        scale = self.dt * self.sample_interval
+        return np.stack([np.cos(phase) * scale, np.sin(phase) * scale], axis=-1)
+
+    def spectra(self):
+        """Per pair (pair, E spectrum, H spectrum re-timed to the E time), complex128 (n_freq, cells a, cells b)
+        (monitors3d.py:103-110)."""
+        retime = np.exp(1j * np.pi * self.frequencies * self.dt)[:, None, None]
+        out = []
+        for q, pair in enumerate(self.pairs):
+            acc = self.engine.read_plane_accumulator(self.index, q)
+            values = acc[:, :, 0] - acc[:, :, 2] + 1j * (acc[:, :, 1] - acc[:, :, 3])
+            out.append((pair, values[0], values[1] * retime))
+        return out
+
+    def flux(self):
+        """Spectral power (n_freqs,) float64: cos(pi f dt) sum sign Re(E conj(H)) area (monitors3d.py:112-118)."""
+        total = np.zeros(self.n_freqs)
+        for pair, e_spectrum, h_spectrum in self.spectra():
+            total += pair['sign'] * np.einsum('fij,ij->f', np.real(e_spectrum * np.conj(h_spectrum)), pair['area'])
+        return np.cos(np.pi * self.frequencies * self.dt) * total

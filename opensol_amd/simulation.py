@@ -166,3 +166,107 @@ class Simulation:
         if self.engine is not None:
             self.engine.close()
             self.engine = None
+
+
+class Simulation3D:
+    """3D FDTD model with the reference's EMPhysics3D method names (physics/em3d/physics3d.py:144-445) and the
+    Simulation run loop; size_* is the physical window, every cpml face adds npml cells.
+
+        sim = Simulation3D(12.2e-6, 12.2e-6, 12.2e-6, dx=25e-9, npml=12)
+        sim.add_dielectric_box((1e-6, 1e-6, 1e-6), (3e-6, 2e-6, 4e-6), eps_r=4.0)
+        sim.add_dipole(6.1e-6, 6.1e-6, 6.1e-6, orientation='z', wavelength=1e-6, bandwidth=0.5)
+        plane = sim.add_flux_plane('x', 10.1e-6, wavelengths=[0.8e-6, 1e-6])
+        sim.run(n_steps=500)
+        spectrum = plane.flux()
+    """
+
+    def __init__(self, size_x, size_y, size_z, dx, npml: int = 12, courant: float = None, boundaries=None, *,
+                 device=None, planes_per_tile: int = 0, graph_steps: int = 0):
+        from .fdtd3d import Model3D
+        self.model = Model3D(size_x, size_y, size_z, dx, npml=npml, courant=courant, boundaries=boundaries)
+        self.domain = self.model.domain
+        self.boundary_config = self.model.bc
+        self.engine = None
+        self._engine_options = dict(device=device, planes_per_tile=planes_per_tile, graph_steps=graph_steps)
+        self._materials_version = None
+
+    @property
+    def grid(self):
+        return self.domain.grid
+
+    @property
+    def sources(self):
+        return self.model.sources
+
+    @property
+    def monitors(self):
+        return self.model.monitors
+
+    @property
+    def n_step(self):
+        return 0 if self.engine is None else self.engine.n_step
+
+    @property
+    def time(self):
+        return 0.0 if self.engine is None else self.engine.time
+
+    def add_dielectric_box(self, lo, hi, eps_r, sigma_e=0.0):
+        self.model.add_dielectric_box(lo, hi, eps_r, sigma_e)
+
+    def add_dipole(self, x, y, z, orientation='z', dipole_moment=1e-12, wavelength=None, frequency=None,
+                   bandwidth=0.5, waveform='gaussian'):
+        self._require_unbuilt('add_dipole')
+        return self.model.add_dipole(x, y, z, orientation, dipole_moment, wavelength, frequency, bandwidth, waveform)
+
+    def add_flux_plane(self, normal, position, wavelengths=None, frequencies=None):
+        self._require_unbuilt('add_flux_plane')
+        return self.model.add_flux_plane(normal, position, wavelengths, frequencies)
+
+    def _require_unbuilt(self, what):
+        if self.engine is not None:
+            raise ValueError(f'{what}: add sources and monitors before the first run()')
+
+    def finalize(self):
+        from .fdtd3d import Engine3D
+        model = self.model
+        if self.engine is None:
+            model.finalize()
+            self.engine = Engine3D(model, **self._engine_options)
+        elif model.materials.version != self._materials_version:
+            model.finalize()
+            self.engine.materials_changed()
+        self._materials_version = model.materials.version
+        return {'em': []}
+
+    def run(self, duration: float = None, n_steps: int = None, print_every: int = 0, flush: bool = True):
+        """Advance by `duration` [s] (stopping once t >= t_end - dt/2) or exactly `n_steps` steps."""
+        if (duration is None) == (n_steps is None):
+            raise ValueError('Simulation3D.run needs exactly one of duration [s] or n_steps')
+        if n_steps is not None and int(n_steps) < 0:
+            raise ValueError(f'n_steps must be >= 0, got {n_steps}')
+        self.finalize()
+        engine = self.engine
+        count = int(n_steps) if n_steps is not None else steps_for_duration(engine.time, self.model.dt, duration)
+        wall_start = _time.perf_counter()
+        engine.step(count)
+        if flush or print_every > 0:
+            engine.synchronize()
+        if print_every > 0:
+            print(f'Done: {count} ticks in {_time.perf_counter() - wall_start:.2f} s (t = {engine.time:.4e} s)')
+        return count
+
+    def get_field_snapshot(self, component='ez'):
+        """float32 field over the physical window."""
+        if self.engine is None:
+            self.finalize()
+        return np.asarray(self.engine.field(component)[self.domain.physical_window()], dtype=DTYPE)
+
+    def state(self):
+        if self.engine is None:
+            self.finalize()
+        return {name: self.engine.field(name) for name in ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')}
+
+    def close(self):
+        if self.engine is not None:
+            self.engine.close()
+            self.engine = None

@@ -92,3 +92,27 @@ class LineSource:
 def complex_permittivity(eps_rel, sigma_e, omega):
     """eps_rel - j sigma_e / (omega eps0) (physics/em/dispersion.py:10-21, no poles)."""
     return np.asarray(eps_rel, dtype=np.complex128) - 1j * np.asarray(sigma_e, dtype=np.float64) / (omega * EPS0)
+
+
+class PointDipole:
+    """Electric point dipole of moment I l [A m] on the E face `cell` along `component` ('ex' / 'ey' / 'ez'):
+    E[cell] += float32(weight * float32(s(t + dt/2))), weight = float32(cb of the face * moment / dual volume)
+    (physics/em3d/sources3d.py:41-75, physics/em3d/engine3d.py:299-311)."""
+
+    def __init__(self, component, cell, temporal, moment, volume):
+        if component not in ('ex', 'ey', 'ez'):
+            raise ValueError(f"dipole component must be 'ex', 'ey' or 'ez', got {component!r}")
+        self.component = component
+        self.axis = 'xyz'.index(component[1])
+        self.cell = tuple(int(v) for v in cell)
+        self.temporal = temporal
+        self.moment = float(moment)
+        self.volume = float(volume)
+        self.weight = DTYPE(0.0)
+
+    def calibrate(self, cb):
+        self.weight = DTYPE(float(cb) * self.moment / self.volume)
+
+    def wave_table(self, half_step_times):
+        """float32 s(t_n + dt/2) per step."""
+        return np.array([self.temporal(t) for t in half_step_times], dtype=np.float64).astype(DTYPE)

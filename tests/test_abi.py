@@ -29,7 +29,8 @@ def lib():
 def test_header_declares_the_expected_entry_points():
     names = declared_functions()
     for name in ('yee_abi_version', 'yee_sizeof', 'yee_last_error', 'yee2d_create', 'yee_step', 'yee_destroy',
-                 'yee2d_batch_create', 'yee2d_batch_bytes', 'yee2d_update', 'yee_set_graph_steps'):
+                 'yee2d_batch_create', 'yee2d_batch_bytes', 'yee2d_update', 'yee_set_graph_steps', 'yee3d_create',
+                 'yee3d_update', 'yee3d_get_desc', 'yee_step_timed'):
         assert name in names
 
 
@@ -42,7 +43,7 @@ def test_library_exports_every_declared_symbol(lib):
 
 def test_struct_layouts_match_binding(lib):
     assert lib.yee_abi_version() == _lib.ABI_VERSION
-    for which, struct in enumerate((_lib.Desc2DC, _lib.LineSourceC, _lib.FluxLineC, _lib.CpmlAxisC)):
+    for which, struct in enumerate(_lib.STRUCTS):
         assert lib.yee_sizeof(which) == ctypes.sizeof(struct)
 
 
@@ -57,3 +58,9 @@ def test_invalid_descriptors_are_rejected_without_a_gpu(lib):
     assert b'pitch' in lib.yee_last_error()
     assert lib.yee_step(None, 1, None) == -3
     assert lib.yee_destroy(None) == 0
+    d3 = _lib.Desc3DC()
+    assert lib.yee3d_create(ctypes.byref(d3), ctypes.byref(ctx)) == -1 and not ctx
+    assert b'3D grid' in lib.yee_last_error()
+    d3.nx, d3.ny, d3.nz, d3.rows_per_tile, d3.planes_per_tile = 8, 8, 8, 3, 4
+    assert lib.yee3d_create(ctypes.byref(d3), ctypes.byref(ctx)) == -1
+    assert b'rows_per_tile' in lib.yee_last_error()

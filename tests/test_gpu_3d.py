@@ -1,0 +1,149 @@
+"""3D HIP Yee step (libyee.so through the C ABI) against the FP64 3D oracle: the golden dipole cases end to end (fields
+and plane-flux spectra), random initial fields over tile shapes that put tile edges, ragged tiles and wraps
+everywhere, exact invariance along a periodic axis (halo copies equal their owners bit for bit), graph replay and
+launch-geometry invariance (bit-identical)."""
+import numpy as np
+import pytest
+
+from oracle.yee3d import Yee3D
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')
+GOLDEN_WINDOW = (0.7e-6, 0.6e-6, 0.55e-6)
+GOLDEN_BOXES = (((0.05e-6, 0.05e-6, 0.00e-6), (0.30e-6, 0.35e-6, 0.25e-6), 4.0),
+                ((0.40e-6, 0.20e-6, 0.15e-6), (0.70e-6, 0.60e-6, 0.40e-6), 2.25),
+                ((0.20e-6, 0.40e-6, 0.30e-6), (0.50e-6, 0.55e-6, 0.55e-6), 6.5))
+GOLDEN_CASES = {
+    'cpml': dict(faces=None, dipole=(0.36e-6, 0.29e-6, 0.27e-6), orientation='z',
+                 planes=(('x', 0.55e-6), ('z', 0.1e-6))),
+    'periodic_y': dict(faces={'y_lo': 'periodic', 'y_hi': 'periodic'}, dipole=(0.33e-6, 0.21e-6, 0.26e-6),
+                       orientation='x', planes=(('x', 0.15e-6), ('z', 0.45e-6))),
+}
+
+
+def _build(target, spec, **options):
+    sim = target(*GOLDEN_WINDOW, 25e-9, npml=8, boundaries=spec['faces'], **options)
+    for lo, hi, eps in GOLDEN_BOXES:
+        sim.add_dielectric_box(lo, hi, eps)
+    sim.add_dipole(*spec['dipole'], orientation=spec['orientation'], dipole_moment=1e-12, wavelength=0.6e-6,
+                   bandwidth=0.5)
+    planes = [sim.add_flux_plane(n, p, wavelengths=[0.5e-6, 0.6e-6, 0.7e-6]) for n, p in spec['planes']]
+    return sim, planes
+
+
+@pytest.mark.parametrize('name', sorted(GOLDEN_CASES))
+def test_dipole_case_matches_oracle(gpu, name):
+    """Fields within 1e-5 (relative L2, normalised by the field norm while the pulse is in the box) after 120 and 300
+    steps, flux spectra within 1e-5 of the FP64 oracle."""
+    from opensol_amd import Simulation3D
+    spec = GOLDEN_CASES[name]
+    sim, planes = _build(Simulation3D, spec)
+    ref, rplanes = _build(lambda *a, **k: Yee3D(*a, **k), spec)
+    ref.finalize()
+    sim.run(n_steps=120)
+    ref.run(n_steps=120)
+    norms = {}
+    for f in FIELDS:
+        want = ref.fields[f]
+        norms[f] = np.linalg.norm(want)
+        err = np.linalg.norm(sim.engine.field(f) - want) / norms[f]
+        print(f'{name} step 120 {f}: {err:.2e}')
+        assert err < 1e-5, f
+    sim.run(n_steps=180)
+    ref.run(n_steps=180)
+    for f in FIELDS:
+        err = np.linalg.norm(sim.engine.field(f) - ref.fields[f]) / norms[f]
+        print(f'{name} step 300 {f}: {err:.2e}')
+        assert err < 1e-5, f
+    for plane, rplane in zip(planes, rplanes):
+        got, want = plane.flux(), rplane.flux()
+        rel = np.max(np.abs(got - want) / np.abs(want))
+        print(f'{name} flux {got} oracle {want} rel {rel:.2e}')
+        assert rel < 1e-5
+
+
+# (window cells x, y, z, npml, faces, planes_per_tile)
+CASES = [
+    ((8, 9, 20), 4, None, 3),                   # nz below one tile, ragged y tile
+    ((10, 7, 62), 3, None, 1),                  # exactly one z tile
+    ((9, 8, 63), 3, {'z_lo': 'periodic', 'z_hi': 'periodic'}, 4),   # one cell over a tile, z wrap
+    ((6, 11, 130), 2, {'x_lo': 'periodic', 'x_hi': 'periodic'}, 2),
+    ((7, 6, 70), 3, {'y_lo': 'periodic', 'y_hi': 'periodic'}, 0),
+    ((12, 10, 40), 0, {f: 'periodic' for f in ('x_lo', 'x_hi', 'y_lo', 'y_hi', 'z_lo', 'z_hi')}, 5),
+]
+
+
+def _random_pair(case, seed, **options):
+    from opensol_amd import Simulation3D
+    cells, npml, faces, planes = case
+    dx = 25e-9
+    size = tuple(c * dx for c in cells)
+    sim = Simulation3D(*size, dx, npml=npml, boundaries=faces, planes_per_tile=planes, **options)
+    ref = Yee3D(*size, dx, npml=npml, boundaries=faces)
+    rng = np.random.default_rng(seed)
+    for _ in range(6):
+        lo = rng.uniform(-0.2, 1.0, 3) * np.array(size)
+        hi = lo + rng.uniform(0.1, 0.6, 3) * np.array(size)
+        eps, sig = rng.uniform(1.0, 9.0), (rng.uniform(0, 2e4) if rng.random() < 0.3 else 0.0)
+        sim.add_dielectric_box(tuple(lo), tuple(hi), eps, sig)
+        ref.add_dielectric_box(tuple(lo), tuple(hi), eps, sig)
+    sim.finalize()
+    ref.finalize()
+    for f in FIELDS:
+        values = rng.standard_normal(ref.shape) * (1.0 if f[0] == 'e' else 1.0 / 376.7)
+        ref.fields[f][...] = values
+        sim.engine.set_field(f, values)
+    return sim, ref
+
+
+@pytest.mark.parametrize('case', CASES, ids=[str(c[0]) + str(c[2] and sorted(c[2])[0]) for c in CASES])
+def test_random_fields_match_oracle(gpu, case):
+    sim, ref = _random_pair(case, seed=11)
+    sim.run(n_steps=12)
+    ref.run(n_steps=12)
+    for f in FIELDS:
+        want = ref.fields[f]
+        err = np.max(np.abs(sim.engine.field(f) - want)) / np.max(np.abs(want))
+        assert err < 5e-6, (f, err)
+
+
+@pytest.mark.parametrize('axis', [0, 1, 2])
+def test_invariance_along_periodic_axis_is_exact(gpu, axis):
+    """Fields and materials invariant along a periodic axis stay exactly invariant: every halo copy (plane i0-1, row
+    j0-1, lanes 0 and 63) rounds as its owner does."""
+    from opensol_amd import Simulation3D
+    dx = 25e-9
+    cells = [14, 13, 70]
+    cells[axis] = 5
+    faces = {f'{"xyz"[axis]}_lo': 'periodic', f'{"xyz"[axis]}_hi': 'periodic'}
+    size = [c * dx for c in cells]
+    sim = Simulation3D(*size, dx, npml=3, boundaries=faces, planes_per_tile=2)
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        lo = rng.uniform(0, 1.0, 3) * np.array(size)
+        hi = lo + rng.uniform(0.1, 0.5, 3) * np.array(size)
+        lo[axis], hi[axis] = -1.0, 1.0
+        sim.add_dielectric_box(tuple(lo), tuple(hi), rng.uniform(1, 8))
+    sim.finalize()
+    shape = sim.model.shape
+    for f in FIELDS:
+        line = rng.standard_normal([n if a != axis else 1 for a, n in enumerate(shape)])
+        sim.engine.set_field(f, np.broadcast_to(line, shape))
+    sim.run(n_steps=10)
+    for f in FIELDS:
+        values = sim.engine.field(f)
+        first = np.take(values, [0], axis=axis)
+        assert np.array_equal(values, np.broadcast_to(first, values.shape)), f
+
+
+def test_graph_replay_and_launch_geometry_are_bit_identical(gpu):
+    case = CASES[4]
+    runs = []
+    for options, planes in (({}, 0), ({'graph_steps': 7}, 0), ({}, 1), ({}, 3)):
+        sim, _ = _random_pair(case[:3] + (planes,), seed=3, **options)
+        sim.run(n_steps=23)
+        runs.append({f: sim.engine.field(f) for f in FIELDS})
+    for other in runs[1:]:
+        for f in FIELDS:
+            assert np.array_equal(runs[0][f], other[f]), f
