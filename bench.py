@@ -1,12 +1,17 @@
-"""Benchmark of the MI355X FDTD hot path: Yee cell-updates/s on one GPU, with the HBM roofline of the fused step
+"""Benchmark of the MI355X FDTD hot path: Yee cell-updates/s on one GPU, with the HBM roofline of the dominant step
 kernel and the numpy CPU baseline timed on this host.
 
-Workload (one step = one full Yee step of the whole grid, CPML included): 2D E_z, 8192 x 8192 cells with a 20-cell
-CPML on every side (window 8152^2 cells), dx = 20 nm, 400 seeded axis-aligned dielectric rectangles (sides uniform
-0.1-2 um, eps_r uniform in [1, 12], painted in draw order), one Gaussian line source (lambda0 = 1.55 um, bandwidth 0.5)
-along y at x = 2 um, 2000 steps by default. Inputs are resident in HBM before timing starts.
+Default workload (one step = one full Yee step of the whole grid, CPML included): 2D E_z, 8192 x 8192 cells with a
+20-cell CPML on every side (window 8152^2 cells), dx = 20 nm, 400 seeded axis-aligned dielectric rectangles (sides
+uniform 0.1-2 um, eps_r uniform in [1, 12], painted in draw order), one Gaussian line source (lambda0 = 1.55 um,
+bandwidth 0.5) along y at x = 2 um, 2000 steps by default. Inputs are resident in HBM before timing starts.
 
-    python bench.py [--gpus 1] [--steps 2000] [--warmup 50]
+--workload yee3d_512: 3D, 512^3 cells with a 12-cell CPML on every face (window 488^3, 12.2 um), dx = 25 nm, 200
+seeded dielectric boxes (edges uniform 0.2-3 um, eps_r uniform in [1, 12]), a Gaussian E_z point dipole at the centre
+(lambda0 = 1 um, bandwidth 0.5), two flux planes normal to x at the centre -+ 4 um at 0.8 / 0.9 / 1.0 / 1.1 um;
+500 steps by default.
+
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 50] [--workload ez8192 | yee3d_512]
 """
 import argparse
 import json
@@ -20,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 BYTES_PER_CELL_2D = 26          # read E, H_a, H_b + uint16 id, write E, H_a, H_b (fp32)
+BYTES_PER_CELL_3D = 50          # read the six fields + uint16 id, write the six fields (fp32)
 HBM_PEAK_GBS = 8000.0           # MI355X HBM3E peak
 SEED = 20260512
 TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'traffic.json')
@@ -52,18 +58,44 @@ def build_ez_workload(n=8192, npml=20, dx=20e-9, rectangles=400, seed=SEED, **op
     return sim
 
 
-def cpu_baseline(sim, steps):
-    """The numpy restatement of the reference's numpy-twin step (oracle, float32) on the same grid, materials and
-    source, timed for `steps` steps on one host core."""
-    from oracle.yee2d import Yee2D
+def build_3d_workload(n=512, npml=12, dx=25e-9, boxes=200, seed=SEED, **options):
+    """The 3D roofline workload as a Simulation3D (not yet finalized)."""
+    from opensol_amd import Simulation3D
+    size = window_for_cells(n, npml, dx)
+    sim = Simulation3D(size, size, size, dx, npml=npml, **options)
+    rng = np.random.default_rng(seed)
+    for _ in range(boxes):
+        centre = rng.uniform(0, size, 3)
+        edges = rng.uniform(0.2e-6, 3e-6, 3)
+        sim.add_dielectric_box(tuple(centre - edges / 2), tuple(centre + edges / 2), eps_r=rng.uniform(1.0, 12.0))
+    c = size / 2
+    sim.add_dipole(c, c, c, orientation='z', dipole_moment=1e-12, wavelength=1e-6, bandwidth=0.5)
+    for x in (c - 4e-6, c + 4e-6):
+        sim.add_flux_plane('x', x, wavelengths=[0.8e-6, 0.9e-6, 1.0e-6, 1.1e-6])
+    assert sim.model.shape == (n, n, n), sim.model.shape
+    return sim
+
+
+def cpu_baseline_3d(sim, steps):
+    """The 3D oracle in float32 (the reference's numpy-twin array passes) on the same grid, materials, dipole and
+    planes, timed for `steps` steps on one host core."""
+    from oracle.yee3d import Yee3D
     model = sim.model
-    size = model.domain.size_x
-    ref = Yee2D(size, model.domain.size_y, model.domain.dx, npml=model.bc.cpml_npml, dtype=np.float32)
-    assert (ref.nx, ref.ny) == model.shape
+    size = model.domain.sizes
+    ref = Yee3D(*size, model.domain.dx, npml=model.bc.cpml_npml, dtype=np.float32)
+    assert ref.shape == model.shape
     ref.materials['eps_r'][...] = model.materials.property_array(0)
-    ref.add_line_source(axis='y', position=2e-6, wavelength=1.55e-6, bandwidth=0.5)
+    c = size[0] / 2
+    ref.add_dipole(c, c, c, orientation='z', dipole_moment=1e-12, wavelength=1e-6, bandwidth=0.5)
+    for x in (c - 4e-6, c + 4e-6):
+        ref.add_flux_plane('x', x, wavelengths=[0.8e-6, 0.9e-6, 1.0e-6, 1.1e-6])
     ref.finalize()
-    ref.step()
+    return _time_on_one_core(ref, steps, model.cell_count,
+                             f'{{steps}} steps of the same {model.nx}x{model.ny}x{model.nz} grid (numpy float32, '
+                             f'reference numpy-twin array passes)')
+
+
+def _time_on_one_core(ref, steps, cells, sample):
     affinity = None
     if hasattr(os, 'sched_getaffinity'):
         affinity = os.sched_getaffinity(0)
@@ -76,10 +108,25 @@ def cpu_baseline(sim, steps):
     finally:
         if affinity is not None:
             os.sched_setaffinity(0, affinity)
-    cells = model.nx * model.ny
     return {'value': cells * steps / elapsed, 'unit': 'cell-updates/s', 'cores': 1, 'kind': 'port',
-            'sample': f'{steps} steps of the same {model.nx}x{model.ny} E_z grid (numpy float32, reference numpy-twin '
-                      f'array passes), {elapsed:.1f} s on 1 core of {_cpu_model()}'}
+            'sample': sample.format(steps=steps) + f', {elapsed:.1f} s on 1 core of {_cpu_model()}'}
+
+
+def cpu_baseline(sim, steps):
+    """The numpy restatement of the reference's numpy-twin step (oracle, float32) on the same grid, materials and
+    source, timed for `steps` steps on one host core."""
+    from oracle.yee2d import Yee2D
+    model = sim.model
+    size = model.domain.size_x
+    ref = Yee2D(size, model.domain.size_y, model.domain.dx, npml=model.bc.cpml_npml, dtype=np.float32)
+    assert (ref.nx, ref.ny) == model.shape
+    ref.materials['eps_r'][...] = model.materials.property_array(0)
+    ref.add_line_source(axis='y', position=2e-6, wavelength=1.55e-6, bandwidth=0.5)
+    ref.finalize()
+    ref.step()
+    return _time_on_one_core(ref, steps, model.nx * model.ny,
+                             f'{{steps}} steps of the same {model.nx}x{model.ny} E_z grid (numpy float32, reference '
+                             f'numpy-twin array passes)')
 
 
 def _cpu_model():
@@ -103,28 +150,24 @@ def measured_traffic(workload):
         return None
 
 
-def main():
-    parser = argparse.ArgumentParser(description=__doc__.split('\n')[0])
-    parser.add_argument('--gpus', type=int, default=1)
-    parser.add_argument('--steps', type=int, default=2000)
-    parser.add_argument('--warmup', type=int, default=50)
-    parser.add_argument('--graph-steps', type=int, default=50)
-    parser.add_argument('--rows-per-tile', type=int, default=0, help='0: fill the device wave slots')
-    parser.add_argument('--lane-columns', type=int, default=0, choices=(0, 2, 4), help='0: library default')
-    parser.add_argument('--cpu-steps', type=int, default=8, help='steps of the CPU baseline sample (0: skip)')
-    parser.add_argument('--kernel-steps', type=int, default=200,
-                        help='steps timed per kernel with HIP events for the roofline (0: skip)')
-    args = parser.parse_args()
-    if args.gpus != 1:
-        print(f'bench.py measures one GPU: this FDTD path runs on a single MI355X (got --gpus {args.gpus}); '
-              f'run it with --gpus 1', file=sys.stderr)
-        sys.exit(2)
+def _common_fields(args, value, per_step):
+    return {'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': per_step * 1e3,
+            'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32'}
 
-    import torch
-    if not torch.cuda.is_available():
-        print('bench.py needs a GPU', file=sys.stderr)
-        sys.exit(1)
-    torch.cuda.set_device(0)
+
+def _time_steps(engine, steps, torch):
+    """Seconds for `steps` steps on the library's stream, HIP events on both sides after a device sync."""
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    engine.step(steps)
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / 1e3
+
+
+def run_2d(args, torch):
     sim = build_ez_workload(graph_steps=args.graph_steps, rows_per_tile=args.rows_per_tile,
                             lane_columns=args.lane_columns)
     sim.finalize()
@@ -134,15 +177,7 @@ def main():
     print(f'# workload ready: {sim.model.nx}x{sim.model.ny}, {len(sim.model.materials.materials)} materials',
           file=sys.stderr, flush=True)
     engine.step(args.warmup)
-    torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    start.record()
-    engine.step(args.steps)
-    stop.record()
-    torch.cuda.synchronize()
-    elapsed_s = start.elapsed_time(stop) / 1e3
+    elapsed_s = _time_steps(engine, args.steps, torch)
     per_step = elapsed_s / args.steps
     value = cells * args.steps / elapsed_s
     step_achieved = BYTES_PER_CELL_2D * cells / per_step / 1e9
@@ -163,31 +198,94 @@ def main():
                         edge_launch_ms=kernels['edge'][0], edge_cells_per_launch=kernels['edge'][1])
         print(f'# kernels: bulk {bulk_ms * 1e3:.1f} us for {bulk_cells} cells ({achieved:.1f} GB/s), edge '
               f'{kernels["edge"][0] * 1e3:.1f} us for {kernels["edge"][1]} cells', file=sys.stderr, flush=True)
-
     baseline = None
     if args.cpu_steps > 0:
         baseline = cpu_baseline(sim, args.cpu_steps)
         print(f'# CPU baseline: {baseline["value"]:.4e} cell-updates/s', file=sys.stderr, flush=True)
+    result = {'metric': 'Yee cell-updates/s (HBM BW fraction in roofline)', 'value': value, 'unit': 'cell-updates/s',
+              **_common_fields(args, value, per_step),
+              'data': 'synthetic (seeded rectangles, Gaussian line source)',
+              'config': {'workload': workload, 'grid': [sim.model.nx, sim.model.ny], 'cpml': 20, 'dx_m': 20e-9,
+                         'rectangles': 400, 'seed': SEED, 'polarization': 'ez', 'graph_steps': args.graph_steps,
+                         'rows_per_tile': rows_per_tile, 'lane_columns': lane_columns},
+              'roofline': roofline, 'cpu_baseline': baseline}
+    return result
 
-    result = {
-        'metric': 'Yee cell-updates/s (HBM BW fraction in roofline)',
-        'value': value,
-        'unit': 'cell-updates/s',
-        'n_gpus': 1,
-        'steps': args.steps,
-        'warmup': args.warmup,
-        'ms_per_step': per_step * 1e3,
-        'higher_is_better': True,
-        'scaling': 'weak',
-        'vs_baseline': None,
-        'dtype': 'f32',
-        'data': 'synthetic (seeded rectangles, Gaussian line source)',
-        'config': {'workload': workload, 'grid': [sim.model.nx, sim.model.ny], 'cpml': 20, 'dx_m': 20e-9,
-                   'rectangles': 400, 'seed': SEED, 'polarization': 'ez', 'graph_steps': args.graph_steps,
-                   'rows_per_tile': rows_per_tile, 'lane_columns': lane_columns},
-        'roofline': roofline,
-        'cpu_baseline': baseline,
-    }
+
+def run_3d(args, torch):
+    sim = build_3d_workload(graph_steps=args.graph_steps, planes_per_tile=args.planes_per_tile)
+    sim.finalize()
+    engine = sim.engine
+    rows_per_tile, planes_per_tile = engine.launch_geometry()
+    cells = sim.model.cell_count
+    print(f'# workload ready: {sim.model.shape}, {len(sim.model.materials.materials)} materials, '
+          f'm = {sim.model.sample_interval}', file=sys.stderr, flush=True)
+    engine.step(args.warmup)
+    elapsed_s = _time_steps(engine, args.steps, torch)
+    per_step = elapsed_s / args.steps
+    value = cells * args.steps / elapsed_s
+    step_achieved = BYTES_PER_CELL_3D * cells / per_step / 1e9
+    workload = 'yee3d_512cubed_cpml12_200box_dipole_2planes'
+    print(f'# GPU: {per_step * 1e3:.4f} ms/step, {value:.4e} cell-updates/s, {step_achieved:.1f} GB/s algorithmic',
+          file=sys.stderr, flush=True)
+    kernels = engine.step_timed(args.kernel_steps) if args.kernel_steps > 0 else None
+    roofline = {'bound': 'hbm', 'achieved': None, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': None,
+                'traffic': measured_traffic(workload), 'kernel': 'yee3d_step_kernel<4>',
+                'bytes_per_cell_update': BYTES_PER_CELL_3D, 'step_achieved': step_achieved,
+                'step_frac': step_achieved / HBM_PEAK_GBS}
+    if kernels is not None:
+        step_ms, step_cells = kernels['step']
+        achieved = BYTES_PER_CELL_3D * step_cells / (step_ms * 1e-3) / 1e9
+        roofline.update(achieved=achieved, frac=achieved / HBM_PEAK_GBS, launch_ms=step_ms,
+                        cells_per_launch=step_cells, algorithmic_bytes_per_launch=BYTES_PER_CELL_3D * step_cells,
+                        sample_launch_ms=kernels['sample'][0], sample_cells_per_launch=kernels['sample'][1])
+        print(f'# kernels: step {step_ms * 1e3:.1f} us for {step_cells} cells ({achieved:.1f} GB/s), sample '
+              f'{kernels["sample"][0] * 1e3:.1f} us', file=sys.stderr, flush=True)
+    baseline = None
+    if args.cpu_steps > 0:
+        baseline = cpu_baseline_3d(sim, args.cpu_steps)
+        print(f'# CPU baseline: {baseline["value"]:.4e} cell-updates/s', file=sys.stderr, flush=True)
+    result = {'metric': 'Yee cell-updates/s (HBM BW fraction in roofline)', 'value': value, 'unit': 'cell-updates/s',
+              **_common_fields(args, value, per_step),
+              'data': 'synthetic (seeded boxes, Gaussian point dipole, two flux planes)',
+              'config': {'workload': workload, 'grid': list(sim.model.shape), 'cpml': 12, 'dx_m': 25e-9, 'boxes': 200,
+                         'seed': SEED, 'graph_steps': args.graph_steps, 'rows_per_tile': rows_per_tile,
+                         'planes_per_tile': planes_per_tile, 'sample_interval': sim.model.sample_interval},
+              'roofline': roofline, 'cpu_baseline': baseline}
+    return result
+
+
+def main():
+    parser = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    parser.add_argument('--gpus', type=int, default=1)
+    parser.add_argument('--workload', default='ez8192', choices=('ez8192', 'yee3d_512'))
+    parser.add_argument('--steps', type=int, default=None, help='timed steps (default 2000 2D, 500 3D)')
+    parser.add_argument('--warmup', type=int, default=None, help='untimed steps (default 50 2D, 20 3D)')
+    parser.add_argument('--graph-steps', type=int, default=None, help='steps per HIP graph (default 50 2D, 10 3D)')
+    parser.add_argument('--rows-per-tile', type=int, default=0, help='2D: 0 fills the device wave slots')
+    parser.add_argument('--lane-columns', type=int, default=0, choices=(0, 2, 4), help='2D: 0 = library default')
+    parser.add_argument('--planes-per-tile', type=int, default=0, help='3D: 0 fills the device wave slots')
+    parser.add_argument('--cpu-steps', type=int, default=None,
+                        help='steps of the CPU baseline sample (default 8 2D, 1 3D; 0 skips it)')
+    parser.add_argument('--kernel-steps', type=int, default=None,
+                        help='steps timed per kernel with HIP events for the roofline (default 200 2D, 50 3D; 0 skips)')
+    args = parser.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU: this FDTD path runs on a single MI355X (got --gpus {args.gpus}); '
+              f'run it with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+    three = args.workload == 'yee3d_512'
+    defaults = dict(steps=(500, 2000), warmup=(20, 50), graph_steps=(10, 50), cpu_steps=(1, 8), kernel_steps=(50, 200))
+    for name, (d3, d2) in defaults.items():
+        if getattr(args, name) is None:
+            setattr(args, name, d3 if three else d2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print('bench.py needs a GPU', file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    result = run_3d(args, torch) if three else run_2d(args, torch)
     print(json.dumps(result))
 
 
