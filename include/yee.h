@@ -273,9 +273,9 @@ int32_t yee_destroy(yee_ctx* ctx);
 /*
  * Batched 2D instances (mixed E_z / H_z, uneven sizes) stepped by one launch per step over their tiles.
  * descs_dev: caller device buffer of yee2d_batch_bytes(n) bytes that receives the descriptors and the tile prefix
- * (written with hipMemcpyAsync on `stream`). Each instance keeps its own clock and step_limit, so instances of
- * different lengths share launches and finished ones skip their tiles. No counterpart in the reference, which steps
- * one model at a time (model/study.py:437-447 builds one model per sweep value).
+ * (copied on `stream`, which is synchronized once before the call returns). Each instance keeps its own clock and
+ * step_limit, so instances of different lengths share launches and finished ones skip their tiles. No counterpart in
+ * the reference, which steps one model at a time (model/study.py:437-447 builds one model per sweep value).
  */
 size_t yee2d_batch_bytes(int32_t n);
 int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, void* stream, yee_ctx** out);

@@ -474,6 +474,7 @@ int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, 
     auto* dev_prefix = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(descs_dev) + sizeof(yee2d_desc) * n);
     YEE_HIP(hipMemcpyAsync(dev_descs, host.data(), sizeof(yee2d_desc) * n, hipMemcpyHostToDevice, s));
     YEE_HIP(hipMemcpyAsync(dev_prefix, prefix.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
+    YEE_HIP(hipStreamSynchronize(s));   // the host vectors are the copies' sources
     yee_ctx* ctx = new yee_ctx();
     ctx->kind = kBatch2D;
     ctx->descs = std::move(host);

@@ -1,0 +1,77 @@
+"""The serving loop: a stream of 2D and 3D jobs admitted as they arrive (continuous batching of the 2D jobs, resident
+state under an HBM budget) gives every job the same flux spectra, bit for bit, as running it alone; the plain FIFO form
+gives them too."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _job_2d(k):
+    def build():
+        from opensol_amd import Simulation
+        rng = np.random.default_rng(100 + k)
+        dx = 25e-9
+        sx, sy = rng.integers(30, 80) * dx, rng.integers(20, 60) * dx
+        sim = Simulation(sx, sy, dx, npml=8, polarization='ez' if k % 2 == 0 else 'hz')
+        for _ in range(4):
+            x0, y0 = rng.uniform(0, sx), rng.uniform(0, sy)
+            sim.add_dielectric_rectangle(x0, y0, x0 + 0.3 * sx, y0 + 0.3 * sy, eps_r=rng.uniform(1, 8))
+        sim.add_line_source(axis='y', position=0.25 * sx, wavelength=0.6e-6, bandwidth=0.5)
+        sim.add_flux_monitor(axis='y', position=0.75 * sx, span=(0.0, sy), wavelengths=[0.55e-6, 0.65e-6])
+        return sim
+    return build
+
+
+def _job_3d(k):
+    def build():
+        from opensol_amd import Simulation3D
+        dx = 25e-9
+        size = (0.5e-6, 0.45e-6, 0.4e-6)
+        sim = Simulation3D(*size, dx, npml=6)
+        sim.add_dielectric_box((0.1e-6, 0.1e-6, 0.0), (0.3e-6, 0.3e-6, 0.2e-6), eps_r=3.0 + k)
+        sim.add_dipole(0.25e-6, 0.22e-6, 0.2e-6, orientation='z', wavelength=0.6e-6)
+        sim.add_flux_plane('x', 0.4e-6, wavelengths=[0.55e-6, 0.65e-6])
+        return sim
+    return build
+
+
+def _jobs():
+    from opensol_amd.serve import Job
+    jobs = []
+    for k in range(6):
+        jobs.append(Job(build=_job_2d(k), steps=60 + 23 * k, arrival=0.03 * k, name=f'2d{k}'))
+    for k in range(2):
+        jobs.append(Job(build=_job_3d(k), steps=70 + 40 * k, arrival=0.02 + 0.05 * k, name=f'3d{k}'))
+    return jobs
+
+
+def test_served_results_equal_solo_results(gpu):
+    from opensol_amd.serve import FDTDServer
+    server = FDTDServer(hbm_budget=1e9, chunk_steps=25)
+    served = server.run(_jobs())
+    plain = server.run_plain(_jobs())
+    assert served['jobs'] == plain['jobs'] == 8
+    assert served['cell_updates'] == plain['cell_updates']
+    for job in _jobs():
+        sim = job.build()
+        sim.run(n_steps=job.steps, print_every=0)
+        solo = [np.asarray(m.flux()) for m in sim.model.monitors]
+        sim.close()
+        for a, b, c in zip(solo, served['results'][job.name], plain['results'][job.name]):
+            assert np.array_equal(a, b), job.name
+            assert np.array_equal(a, c), job.name
+    assert served['p50_latency_s'] is not None and served['p99_latency_s'] >= served['p50_latency_s']
+
+
+def test_budget_holds_jobs_until_memory_frees(gpu):
+    from opensol_amd.serve import FDTDServer, state_bytes
+    jobs = _jobs()[:4]
+    need = max(state_bytes(job.build()) for job in jobs)
+    budget = need * 1.5
+    server = FDTDServer(hbm_budget=budget, chunk_steps=40)
+    stats = server.run(jobs)
+    assert stats['jobs'] == 4
+    for job in jobs:   # the jobs resident when `job` was admitted fit the budget together
+        resident = [o for o in jobs if o.admitted <= job.admitted < o.finished]
+        assert sum(o.bytes for o in resident) <= budget
