@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 2
+#define YEE_ABI_VERSION 3
 #define YEE_MAX_SOURCES 4
 #define YEE_MAX_MONITORS 4
 
@@ -169,20 +169,30 @@ typedef struct yee_flux_plane {
 } yee_flux_plane;
 
 /*
- * CPML of one axis of a 3D instance (physics/em3d/boundaries3d.py:208-241): psi rows of the E ('e', cell centres)
- * and H ('h', hi faces) kinds, recursion b and gain = a x bulk inverse spacing per psi row, psi of the axis' first
- * and second component. psi layouts: x axis (n_rows, ny, nz), y axis (nx, n_rows, nz), z axis (nx, ny, n_rows).
- * E psi is updated in place; H psi is ping-pong ([set] = step parity, like the fields).
+ * Per-index constants of one axis of a 3D instance, one 32-byte record per grid index (cell centre rows 'e', hi-face
+ * rows 'h'): the kappa-folded inverse spacings (inv_h[n-1] = 0 on a non-periodic axis, physics/em3d/engine3d.py:
+ * 201-216) and the CPML psi row of each kind (-1 outside the layers) with its recursion b and gain = a x bulk inverse
+ * spacing (physics/em3d/boundaries3d.py:208-241).
+ */
+typedef struct yee3d_axis_row {
+    float inv_h;
+    float inv_e;
+    int32_t psi_row_h;
+    float b_h;
+    float g_h;
+    int32_t psi_row_e;
+    float b_e;
+    float g_e;
+} yee3d_axis_row;
+
+/*
+ * CPML psi of one axis of a 3D instance: psi of the axis' first and second component. Layouts: x axis (n_rows, ny,
+ * nz), y axis (nx, n_rows, nz), z axis (nx, ny, n_rows). E psi is updated in place; H psi is ping-pong ([set] = step
+ * parity, like the fields).
  */
 typedef struct yee3d_cpml_axis {
     int32_t n_rows;          /* psi rows per kind (0: no CPML on this axis) */
     int32_t pad_;
-    const int32_t* row_e;
-    const int32_t* row_h;
-    const float* b_e;
-    const float* g_e;
-    const float* b_h;
-    const float* g_h;
     float* psi_e[2];         /* [first, second component] */
     float* psi_h[2][2];      /* [set][first, second component] */
 } yee3d_cpml_axis;
@@ -207,8 +217,7 @@ typedef struct yee3d_desc {
     const float* table;      /* (n_materials, 8) */
     int32_t n_materials;
     float dt;
-    const float* inv_e[3];   /* per axis (n): 1 / width at centres, CPML kappa folded in */
-    const float* inv_h[3];   /* per axis (n): 1 / spacing at hi faces, kappa folded in; last 0 on non-periodic axes */
+    const yee3d_axis_row* axis_rows[3];   /* per axis (n) records */
     yee3d_cpml_axis cpml[3];
     int32_t n_sources;
     int32_t n_planes;
@@ -225,7 +234,7 @@ typedef struct yee_ctx yee_ctx;
 /* Library ABI version (YEE_ABI_VERSION) and sizeof checks for bindings. */
 int32_t yee_abi_version(void);
 /* 0: yee2d_desc, 1: yee_line_source, 2: yee_flux_line, 3: yee_cpml_axis, 4: yee3d_desc, 5: yee_point_source,
- * 6: yee_flux_plane, 7: yee3d_cpml_axis */
+ * 6: yee_flux_plane, 7: yee3d_cpml_axis, 8: yee3d_axis_row */
 size_t yee_sizeof(int32_t which);
 const char* yee_last_error(void);
 

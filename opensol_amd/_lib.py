@@ -6,7 +6,7 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 2
+ABI_VERSION = 3
 MAX_SOURCES = 4
 MAX_MONITORS = 4
 MAX_SOURCES_3D = 4
@@ -56,21 +56,23 @@ class FluxPlaneC(ctypes.Structure):
 
 
 class Cpml3DAxisC(ctypes.Structure):
-    _fields_ = [('n_rows', c_i32), ('pad_', c_i32), ('row_e', c_ptr), ('row_h', c_ptr),
-                ('b_e', c_ptr), ('g_e', c_ptr), ('b_h', c_ptr), ('g_h', c_ptr),
-                ('psi_e', c_ptr * 2), ('psi_h', (c_ptr * 2) * 2)]
+    _fields_ = [('n_rows', c_i32), ('pad_', c_i32), ('psi_e', c_ptr * 2), ('psi_h', (c_ptr * 2) * 2)]
+
+
+class AxisRow3DC(ctypes.Structure):
+    _fields_ = [('inv_h', c_f32), ('inv_e', c_f32), ('psi_row_h', c_i32), ('b_h', c_f32), ('g_h', c_f32),
+                ('psi_row_e', c_i32), ('b_e', c_f32), ('g_e', c_f32)]
 
 
 class Desc3DC(ctypes.Structure):
     _fields_ = [('nx', c_i32), ('ny', c_i32), ('nz', c_i32), ('periodic', c_i32 * 3), ('rows_per_tile', c_i32),
                 ('planes_per_tile', c_i32), ('field', (c_ptr * 6) * 2), ('material', c_ptr), ('table', c_ptr),
-                ('n_materials', c_i32), ('dt', c_f32), ('inv_e', c_ptr * 3), ('inv_h', c_ptr * 3),
-                ('cpml', Cpml3DAxisC * 3), ('n_sources', c_i32), ('n_planes', c_i32), ('sample_interval', c_i32),
+                ('n_materials', c_i32), ('dt', c_f32), ('axis_rows', c_ptr * 3), ('cpml', Cpml3DAxisC * 3), ('n_sources', c_i32), ('n_planes', c_i32), ('sample_interval', c_i32),
                 ('pad_', c_i32), ('source', PointSourceC * MAX_SOURCES_3D), ('plane', FluxPlaneC * MAX_PLANES_3D),
                 ('clock', c_ptr), ('step_limit', c_i64)]
 
 
-STRUCTS = (Desc2DC, LineSourceC, FluxLineC, CpmlAxisC, Desc3DC, PointSourceC, FluxPlaneC, Cpml3DAxisC)
+STRUCTS = (Desc2DC, LineSourceC, FluxLineC, CpmlAxisC, Desc3DC, PointSourceC, FluxPlaneC, Cpml3DAxisC, AxisRow3DC)
 
 # Every entry point include/yee.h declares: (name, restype, argtypes)
 SIGNATURES = (

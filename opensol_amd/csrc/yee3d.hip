@@ -40,6 +40,21 @@ __shared__ float2 lds3_mat[LDS_IDS];
 
 __device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
+#ifndef YEE3D_XCD_ORDER
+#define YEE3D_XCD_ORDER 1
+#endif
+
+// Blocks are dispatched to the 8 XCDs (each with its own L2) round-robin by block index. Renumber them so that each
+// XCD takes one contiguous run of tiles: neighbouring tiles, which read each other's halo rows, then share an L2.
+__device__ __forceinline__ int block_order(int b, int n)
+{
+    if (!YEE3D_XCD_ORDER) return b;
+    constexpr int XCDS = 8;
+    const int per = n / XCDS, extra = n % XCDS;
+    const int xcd = b % XCDS, slot = b / XCDS;
+    return xcd < extra ? xcd * (per + 1) + slot : extra * (per + 1) + (xcd - extra) * per + slot;
+}
+
 // Stage the table of one instance in LDS (whole block; every thread reaches the barrier).
 __device__ __forceinline__ bool stage_tables(const yee3d_desc& d)
 {
@@ -90,10 +105,14 @@ __device__ __forceinline__ float from_next(float v) { return __shfl_down(v, 1, L
 __device__ __forceinline__ float from_prev(float v) { return __shfl_up(v, 1, LANES); }
 __device__ __forceinline__ int from_next_i(int v) { return __shfl_down(v, 1, LANES); }
 
-// Wrapped (periodic) or clamped index along an axis of n cells.
+// Wrapped (periodic) or clamped index along an axis of n cells. A tile can span several periods of a short periodic
+// axis (64 lanes over nz = 5), so the wrap is a full modulo.
 __device__ __forceinline__ int fold(int v, int n, bool periodic)
 {
-    if (periodic) return v < 0 ? v + n : (v >= n ? v - n : v);
+    if (periodic) {
+        const int m = v % n;
+        return m < 0 ? m + n : m;
+    }
     return v < 0 ? 0 : (v >= n ? n - 1 : v);
 }
 
@@ -123,17 +142,28 @@ struct Row {
     float b, g;
 };
 
-__device__ __forceinline__ Row cpml_row(const yee3d_cpml_axis& a, bool h, int index)
+// The per-index record of an axis read at a wave-uniform index: through the constant address space, so it is a scalar
+// load (counted apart from the vector loads of the field rows, which it would otherwise wait behind).
+__device__ __forceinline__ yee3d_axis_row uniform_row(const yee3d_axis_row* rows, int index)
 {
-    Row row{-1, 0.f, 0.f};
-    if (a.n_rows == 0) return row;
-    row.r = h ? a.row_h[index] : a.row_e[index];
-    if (row.r >= 0) {
-        row.b = h ? a.b_h[row.r] : a.b_e[row.r];
-        row.g = h ? a.g_h[row.r] : a.g_e[row.r];
-    }
-    return row;
+    typedef const __attribute__((address_space(4))) float ConstF;
+    typedef const __attribute__((address_space(4))) int32_t ConstI;
+    ConstF* f = (ConstF*)(rows) + 8 * index;
+    ConstI* n = (ConstI*)(rows) + 8 * index;
+    yee3d_axis_row out;
+    out.inv_h = f[0];
+    out.inv_e = f[1];
+    out.psi_row_h = n[2];
+    out.b_h = f[3];
+    out.g_h = f[4];
+    out.psi_row_e = n[5];
+    out.b_e = f[6];
+    out.g_e = f[7];
+    return out;
 }
+
+__device__ __forceinline__ Row h_row(const yee3d_axis_row& a) { return Row{a.psi_row_h, a.b_h, a.g_h}; }
+__device__ __forceinline__ Row e_row(const yee3d_axis_row& a) { return Row{a.psi_row_e, a.b_e, a.g_e}; }
 
 __device__ __forceinline__ int64_t psi_x(const yee3d_desc& d, int r, int j, int k)
 {
@@ -296,8 +326,9 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
     const int k = fold(kraw, nz, pz);
     const bool k_zero = !pz && kraw < 0;                      // lo neighbour of cell 0: H reads 0
     const bool k_own = lane >= 1 && lane <= OWNED && kraw < nz;
-    const float ihz = d.inv_h[2][k], iez = d.inv_e[2][k];
-    const Row rz_h = cpml_row(d.cpml[2], true, k), rz_e = cpml_row(d.cpml[2], false, k);
+    const yee3d_axis_row az = d.axis_rows[2][k];
+    const float ihz = az.inv_h, iez = az.inv_e;
+    const Row rz_h = h_row(az), rz_e = e_row(az);
 
     // rows: 0 = halo j0-1, 1..J owned, J+1 = the row above (E only)
     RowY rows[J + 2];
@@ -307,10 +338,11 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
         rows[r].j = fold(jraw, ny, py);
         rows[r].own = r >= 1 && r <= J && jraw < ny;
         rows[r].zero = !py && jraw < 0;
-        rows[r].ih = d.inv_h[1][rows[r].j];
-        rows[r].ie = d.inv_e[1][rows[r].j];
-        rows[r].ch = cpml_row(d.cpml[1], true, rows[r].j);
-        rows[r].ce = cpml_row(d.cpml[1], false, rows[r].j);
+        const yee3d_axis_row ay = uniform_row(d.axis_rows[1], rows[r].j);
+        rows[r].ih = ay.inv_h;
+        rows[r].ie = ay.inv_e;
+        rows[r].ch = h_row(ay);
+        rows[r].ce = e_row(ay);
     }
 
     auto load_plane = [&](int i, Cell (&c)[J + 2]) {
@@ -327,10 +359,10 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
 
     // H of plane i for rows lo..J (the E-feeding components are all computed; the callers use what they need).
     // cur: plane i, nxt: plane i+1 (folded). Returns new H per row.
-    auto h_plane = [&](int i, const Cell (&cur)[J + 2], const Cell (&nxt)[J + 2], HNew (&out)[J + 1],
-                       const float (&hold)[J + 1][3], int r_lo) {
-        const float ihx = d.inv_h[0][i];
-        const Row rx = cpml_row(d.cpml[0], true, i);
+    auto h_plane = [&](int i, const yee3d_axis_row& ax, const Cell (&cur)[J + 2], const Cell (&nxt)[J + 2],
+                       HNew (&out)[J + 1], const float (&hold)[J + 1][3], int r_lo) {
+        const float ihx = ax.inv_h;
+        const Row rx = h_row(ax);
 #pragma unroll
         for (int r = 0; r < J + 1; ++r) {
             if (r < r_lo) continue;
@@ -367,8 +399,9 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
         Cell prev[J + 2];
         load_plane(im, prev);
         load_hold(im, hold, 1);
-        const float ihx = d.inv_h[0][im];
-        const Row rx = cpml_row(d.cpml[0], true, im);
+        const yee3d_axis_row ax = uniform_row(d.axis_rows[0], im);
+        const float ihx = ax.inv_h;
+        const Row rx = h_row(ax);
 #pragma unroll
         for (int r = 1; r < J + 1; ++r) {
             const float ey_zp = from_next(prev[r].ey);
@@ -387,17 +420,18 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
 
     for (int i = i0; i < i1; ++i) {
         const int ip = fold(i + 1, nx, px);
+        const yee3d_axis_row ax = uniform_row(d.axis_rows[0], i);
         load_plane(ip, nxt);
         load_hold(i, hold, 0);
-        h_plane(i, cur, nxt, hn, hold, 0);
+        h_plane(i, ax, cur, nxt, hn, hold, 0);
         // zero the H that stands for "below cell 0" on non-periodic axes
         if (rows[0].zero) { hn[0].hx = 0.f; hn[0].hz = 0.f; }
         if (k_zero) {
 #pragma unroll
             for (int r = 0; r < J + 1; ++r) { hn[r].hx = 0.f; hn[r].hy = 0.f; }
         }
-        const float iex = d.inv_e[0][i];
-        const Row rx_e = cpml_row(d.cpml[0], false, i);
+        const float iex = ax.inv_e;
+        const Row rx_e = e_row(ax);
         const int64_t plane = (int64_t)i * sx + k;
 #pragma unroll
         for (int r = 1; r < J + 1; ++r) {
@@ -443,7 +477,7 @@ __global__ void __launch_bounds__(256) yee3d_step_kernel(yee3d_desc d, int32_t t
 {
     const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
-    const int tile = blockIdx.x * 4 + yee3::wave_index();
+    const int tile = yee3::block_order(blockIdx.x, gridDim.x) * 4 + yee3::wave_index();
     const bool mine = tile < tiles;
     if (!__syncthreads_or(mine)) return;
     const bool lds = yee3::stage_tables(d);

@@ -143,13 +143,13 @@ int32_t check_3d(const yee3d_desc* d)
             if (!d->field[s][c]) return fail(YEE_EINVAL, "null field buffer");
     if (!d->material || !d->table || d->n_materials < 1) return fail(YEE_EINVAL, "material ids / table missing");
     for (int a = 0; a < 3; ++a) {
-        if (!d->inv_e[a] || !d->inv_h[a]) return fail(YEE_EINVAL, "spacings missing");
+        if (!d->axis_rows[a]) return fail(YEE_EINVAL, "axis rows missing");
+        if (reinterpret_cast<uintptr_t>(d->axis_rows[a]) % 32 != 0) return fail(YEE_EINVAL, "axis rows not 32 B aligned");
         const yee3d_cpml_axis& c = d->cpml[a];
         if (c.n_rows < 0) return fail(YEE_EINVAL, "negative CPML rows");
         if (c.n_rows > 0) {
             if (d->periodic[a]) return fail(YEE_EINVAL, "a periodic axis has no CPML");
-            if (!c.row_e || !c.row_h || !c.b_e || !c.g_e || !c.b_h || !c.g_h || !c.psi_e[0] || !c.psi_e[1] ||
-                !c.psi_h[0][0] || !c.psi_h[0][1] || !c.psi_h[1][0] || !c.psi_h[1][1])
+            if (!c.psi_e[0] || !c.psi_e[1] || !c.psi_h[0][0] || !c.psi_h[0][1] || !c.psi_h[1][0] || !c.psi_h[1][1])
                 return fail(YEE_EINVAL, "CPML arrays missing");
             if (c.n_rows > n[a]) return fail(YEE_EINVAL, "more CPML rows than cells");
         }
@@ -416,6 +416,7 @@ size_t yee_sizeof(int32_t which)
         case 5: return sizeof(yee_point_source);
         case 6: return sizeof(yee_flux_plane);
         case 7: return sizeof(yee3d_cpml_axis);
+        case 8: return sizeof(yee3d_axis_row);
         default: return 0;
     }
 }

@@ -71,6 +71,22 @@ class Model3D:
             self.inv_h.append(inv_h)
         self.sample_interval = 1
 
+    def axis_rows(self, axis):
+        """(n, 8) float32 records of one axis (the yee3d_axis_row layout): inv_h, inv_e, psi row of the hi-face
+        kind (int32 bits, -1 outside the layers), its b and gain, psi row of the centre kind, its b and gain."""
+        name = AXES[axis]
+        tables = self.cpml[name]
+        n = self.shape[axis]
+        out = np.zeros((n, 8), dtype=DTYPE)
+        ints = out.view(np.int32)
+        out[:, 0], out[:, 1] = self.inv_h[axis], self.inv_e[axis]
+        ints[:, 2], ints[:, 5] = tables.row_h, tables.row_e
+        for column, rows, b, g in ((3, tables.row_h, tables.b_h, tables.g_h), (6, tables.row_e, tables.b_e, tables.g_e)):
+            inside = rows >= 0
+            out[inside, column] = b[rows[inside]]
+            out[inside, column + 1] = g[rows[inside]]
+        return out
+
     @property
     def cell_count(self):
         return self.nx * self.ny * self.nz
@@ -206,8 +222,7 @@ class Engine3D:
         shape = model.shape
         self.fields = torch.zeros((2, 6) + shape, dtype=f32, device=dev)
         self.clock = torch.zeros(1, dtype=torch.int64, device=dev)
-        self.inv_e = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in model.inv_e]
-        self.inv_h = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in model.inv_h]
+        self.axis_rows = [torch.from_numpy(model.axis_rows(a)).to(dev) for a in range(3)]
         self.cpml = {}
         for a, axis in enumerate(AXES):
             tables = model.cpml[axis]
@@ -217,9 +232,6 @@ class Engine3D:
             psi_shape = list(shape)
             psi_shape[a] = tables.n_rows
             self.cpml[axis] = dict(
-                row_e=torch.from_numpy(tables.row_e).to(dev), row_h=torch.from_numpy(tables.row_h).to(dev),
-                b_e=torch.from_numpy(tables.b_e).to(dev), g_e=torch.from_numpy(tables.g_e).to(dev),
-                b_h=torch.from_numpy(tables.b_h).to(dev), g_h=torch.from_numpy(tables.g_h).to(dev),
                 psi_e=torch.zeros([2] + psi_shape, dtype=f32, device=dev),
                 psi_h=torch.zeros([2, 2] + psi_shape, dtype=f32, device=dev))
         self.accumulators = [[torch.zeros(m.accumulator_shape(q), dtype=torch.float64, device=dev) for q in range(2)]
@@ -275,15 +287,12 @@ class Engine3D:
         d.n_materials = len(model.materials.materials)
         d.dt = float(model.dt)
         for a, axis in enumerate(AXES):
-            d.inv_e[a] = self.inv_e[a].data_ptr()
-            d.inv_h[a] = self.inv_h[a].data_ptr()
+            d.axis_rows[a] = self.axis_rows[a].data_ptr()
             c = self.cpml[axis]
             if c is None:
                 d.cpml[a].n_rows = 0
                 continue
             d.cpml[a].n_rows = model.cpml[axis].n_rows
-            for name in ('row_e', 'row_h', 'b_e', 'g_e', 'b_h', 'g_h'):
-                setattr(d.cpml[a], name, c[name].data_ptr())
             for q in range(2):
                 d.cpml[a].psi_e[q] = c['psi_e'][q].data_ptr()
                 for s in range(2):
