@@ -1141,22 +1141,6 @@ __device__ __forceinline__ bool bulk_tile(const yee2d_tile_sets& s, int item, in
     return true;
 }
 
-#ifndef YEE2D_XCD_ORDER
-#define YEE2D_XCD_ORDER 1
-#endif
-
-// Blocks are dispatched to the 8 XCDs (each with its own L2) round-robin by block index. Renumber them so that each
-// XCD takes one contiguous run of tiles: a tile and the tile below it, which read each other's edge rows, then share
-// an L2.
-__device__ __forceinline__ int block_order(int b, int n)
-{
-    if (!YEE2D_XCD_ORDER) return b;
-    constexpr int XCDS = 8;
-    const int per = n / XCDS, extra = n % XCDS;
-    const int xcd = b % XCDS, slot = b / XCDS;
-    return xcd < extra ? xcd * (per + 1) + slot : extra * (per + 1) + (xcd - extra) * per + slot;
-}
-
 // Edge tiles are split along x into EDGE_SPLIT row chunks, one wavefront each (the general path is latency-bound:
 // few, slow tiles would otherwise finish long after the bulk).
 constexpr int EDGE_SPLIT = 8;
@@ -1221,7 +1205,7 @@ __global__ void __launch_bounds__(256) yee2d_step_kernel(yee2d_desc d, yee2d_til
 {
     const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
-    yee::step_block<POL, VEC, EDGE>(d, sets, n, yee::block_order(blockIdx.x, gridDim.x));
+    yee::step_block<POL, VEC, EDGE>(d, sets, n, blockIdx.x);
 }
 
 #define YEE_STEP_KERNEL(POL, VEC, EDGE) \

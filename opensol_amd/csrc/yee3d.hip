@@ -40,21 +40,6 @@ __shared__ float2 lds3_mat[LDS_IDS];
 
 __device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
-#ifndef YEE3D_XCD_ORDER
-#define YEE3D_XCD_ORDER 1
-#endif
-
-// Blocks are dispatched to the 8 XCDs (each with its own L2) round-robin by block index. Renumber them so that each
-// XCD takes one contiguous run of tiles: neighbouring tiles, which read each other's halo rows, then share an L2.
-__device__ __forceinline__ int block_order(int b, int n)
-{
-    if (!YEE3D_XCD_ORDER) return b;
-    constexpr int XCDS = 8;
-    const int per = n / XCDS, extra = n % XCDS;
-    const int xcd = b % XCDS, slot = b / XCDS;
-    return xcd < extra ? xcd * (per + 1) + slot : extra * (per + 1) + (xcd - extra) * per + slot;
-}
-
 // Stage the table of one instance in LDS (whole block; every thread reaches the barrier).
 __device__ __forceinline__ bool stage_tables(const yee3d_desc& d)
 {
@@ -84,8 +69,10 @@ __device__ __forceinline__ float2 mat(const yee3d_desc& d, int id)
     else return *reinterpret_cast<const float2*>(d.table + 8 * id + 4);
 }
 
-// (ca, cb) of an E face between cells of ids a and b: the id's own entry when they agree, else the float32 mean of
-// eps_r and sigma_e (shares 1/2) and the semi-implicit coefficients (kernels3d.py:61-85).
+// (ca, cb) of an E face between cells of ids a and b: the id's own entry when they agree, else the coefficients of the
+// float32 mean eps_r and sigma_e (shares 1/2; kernels3d.py:61-85). With k = sigma dt / (2 eps):
+// ca = (1 - k) / (1 + k) = (eps - sigma dt/2) / (eps + sigma dt/2) and cb = (dt / eps) / (1 + k) = dt / (eps + sigma
+// dt/2), formed with one reciprocal (within 2 ulp of the reference's float32 divisions).
 template <bool LDS>
 __device__ __forceinline__ float2 face(const yee3d_desc& d, int a, int b)
 {
@@ -95,10 +82,9 @@ __device__ __forceinline__ float2 face(const yee3d_desc& d, int a, int b)
     }
     const float2 ma = mat<LDS>(d, a), mb = mat<LDS>(d, b);
     const float eps = (0.5f * ma.x + 0.5f * mb.x) * EPS0F;
-    const float sig = 0.5f * ma.y + 0.5f * mb.y;
-    const float k = sig * d.dt / (2.0f * eps);
-    const float den = 1.0f + k;
-    return make_float2((1.0f - k) / den, (d.dt / eps) / den);
+    const float loss = (0.5f * ma.y + 0.5f * mb.y) * (0.5f * d.dt);
+    const float inv = __builtin_amdgcn_rcpf(eps + loss);
+    return make_float2((eps - loss) * inv, d.dt * inv);
 }
 
 __device__ __forceinline__ float from_next(float v) { return __shfl_down(v, 1, LANES); }
@@ -177,15 +163,6 @@ __device__ __forceinline__ int64_t psi_z(const yee3d_desc& d, int i, int j, int 
 {
     return ((int64_t)i * d.ny + j) * d.cpml[2].n_rows + r;
 }
-
-// Per-row (y) constants of a tile row.
-struct RowY {
-    int j;          // folded index
-    bool own;       // an owned row (stores E and H)
-    bool zero;      // the lo halo of row 0 on a non-periodic axis: its H reads 0
-    float ih, ie;
-    Row ch, ce;     // y CPML rows of the H and E kinds
-};
 
 // Values of one cell column the H update needs from the current plane.
 struct Cell {
@@ -306,6 +283,29 @@ __device__ __forceinline__ void e_cell(const yee3d_desc& d, int64_t n, float& ex
     ez = vz;
 }
 
+// Raw buffer access to one x plane of an array: the plane's base and size sit in a buffer resource (scalar
+// registers), the row offset in a scalar register and the lane's z offset in a vector register, so a load or store
+// needs no vector address arithmetic; an access outside the plane reads 0 and writes nothing.
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr int RSRC_DWORD3 = 0x00020000;   // 32-bit data format, no swizzle (gfx9 buffer resource word 3)
+
+__device__ __forceinline__ Rsrc plane_rsrc(const void* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, RSRC_DWORD3);
+}
+__device__ __forceinline__ float ldf(Rsrc r, uint32_t lane_bytes, uint32_t row_bytes)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)lane_bytes, (int)row_bytes, 0));
+}
+__device__ __forceinline__ int ldid(Rsrc r, uint32_t lane_bytes, uint32_t row_bytes)
+{
+    return (int)__builtin_amdgcn_raw_buffer_load_b16(r, (int)lane_bytes, (int)row_bytes, 0);
+}
+__device__ __forceinline__ void stf(Rsrc r, uint32_t lane_bytes, uint32_t row_bytes, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)lane_bytes, (int)row_bytes, 0);
+}
+
 // One wavefront tile: planes [i0, i1), rows j0..j0+J-1, z cells tk*OWNED .. +OWNED-1.
 template <int J, bool LDS>
 __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, int tk)
@@ -313,17 +313,14 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
     const int lane = threadIdx.x & (LANES - 1);
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     const bool px = d.periodic[0] != 0, py = d.periodic[1] != 0, pz = d.periodic[2] != 0;
-    const int64_t sy = nz, sx = (int64_t)ny * nz;
+    const int64_t sx = (int64_t)ny * nz;
+    const uint32_t plane_bytes = (uint32_t)(sx * 4), plane_id_bytes = (uint32_t)(sx * 2);
     const int par = (int)(n & 1);
-    const float* __restrict__ e_o[3] = {d.field[par][0], d.field[par][1], d.field[par][2]};
-    const float* __restrict__ h_o[3] = {d.field[par][3], d.field[par][4], d.field[par][5]};
-    float* __restrict__ e_n[3] = {d.field[par ^ 1][0], d.field[par ^ 1][1], d.field[par ^ 1][2]};
-    float* __restrict__ h_n[3] = {d.field[par ^ 1][3], d.field[par ^ 1][4], d.field[par ^ 1][5]};
-    const uint16_t* __restrict__ ids = d.material;
 
-    // z of this lane
+    // z of this lane: byte offsets within a row
     const int kraw = tk * OWNED - 1 + lane;
     const int k = fold(kraw, nz, pz);
+    const uint32_t vk4 = (uint32_t)k * 4u, vk2 = (uint32_t)k * 2u;
     const bool k_zero = !pz && kraw < 0;                      // lo neighbour of cell 0: H reads 0
     const bool k_own = lane >= 1 && lane <= OWNED && kraw < nz;
     const yee3d_axis_row az = d.axis_rows[2][k];
@@ -331,59 +328,68 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
     const Row rz_h = h_row(az), rz_e = e_row(az);
 
     // rows: 0 = halo j0-1, 1..J owned, J+1 = the row above (E only)
-    RowY rows[J + 2];
+    int jr[J + 2];
+    uint32_t roff[J + 2];   // row byte offsets j * nz * 4 within a field plane (wave-uniform)
+    float ihy[J + 2], iey[J + 2];
+    bool own[J + 2];
 #pragma unroll
     for (int r = 0; r < J + 2; ++r) {
         const int jraw = tj * J - 1 + r;
-        rows[r].j = fold(jraw, ny, py);
-        rows[r].own = r >= 1 && r <= J && jraw < ny;
-        rows[r].zero = !py && jraw < 0;
-        const yee3d_axis_row ay = uniform_row(d.axis_rows[1], rows[r].j);
-        rows[r].ih = ay.inv_h;
-        rows[r].ie = ay.inv_e;
-        rows[r].ch = h_row(ay);
-        rows[r].ce = e_row(ay);
+        jr[r] = fold(jraw, ny, py);
+        roff[r] = (uint32_t)jr[r] * (uint32_t)nz * 4u;
+        own[r] = r >= 1 && r <= J && jraw < ny;
+        const yee3d_axis_row ay = uniform_row(d.axis_rows[1], jr[r]);
+        ihy[r] = ay.inv_h;
+        iey[r] = ay.inv_e;
     }
+    const bool row0_zero = !py && tj == 0;   // the halo row below j = 0 of a non-periodic y axis: its H reads 0
 
+    // E (and ids) of plane i, rows 0..J+1 (the row above needs only E_x, E_z)
     auto load_plane = [&](int i, Cell (&c)[J + 2]) {
-        const int64_t base = (int64_t)i * sx + k;
+        const int64_t po = (int64_t)i * sx;
+        const Rsrc rx = plane_rsrc(d.field[par][0] + po, plane_bytes);
+        const Rsrc ry = plane_rsrc(d.field[par][1] + po, plane_bytes);
+        const Rsrc rz = plane_rsrc(d.field[par][2] + po, plane_bytes);
+        const Rsrc rid = plane_rsrc(d.material + po, plane_id_bytes);
 #pragma unroll
         for (int r = 0; r < J + 2; ++r) {
-            const int64_t o = base + (int64_t)rows[r].j * sy;
-            c[r].ex = e_o[0][o];
-            c[r].ey = e_o[1][o];
-            c[r].ez = e_o[2][o];
-            c[r].id = ids[o];
+            c[r].ex = ldf(rx, vk4, roff[r]);
+            c[r].ey = r <= J ? ldf(ry, vk4, roff[r]) : 0.f;
+            c[r].ez = ldf(rz, vk4, roff[r]);
+            c[r].id = ldid(rid, vk2, roff[r] >> 1);
         }
     };
 
-    // H of plane i for rows lo..J (the E-feeding components are all computed; the callers use what they need).
-    // cur: plane i, nxt: plane i+1 (folded). Returns new H per row.
-    auto h_plane = [&](int i, const yee3d_axis_row& ax, const Cell (&cur)[J + 2], const Cell (&nxt)[J + 2],
-                       HNew (&out)[J + 1], const float (&hold)[J + 1][3], int r_lo) {
-        const float ihx = ax.inv_h;
+    // old H of plane i, rows r_lo..J (the halo row feeds E through H_x and H_z only)
+    auto load_hold = [&](int i, float (&hold)[J + 1][3], int r_lo) {
+        const int64_t po = (int64_t)i * sx;
+        const Rsrc rx = plane_rsrc(d.field[par][3] + po, plane_bytes);
+        const Rsrc ry = plane_rsrc(d.field[par][4] + po, plane_bytes);
+        const Rsrc rz = plane_rsrc(d.field[par][5] + po, plane_bytes);
+#pragma unroll
+        for (int r = 0; r < J + 1; ++r) {
+            if (r < r_lo) continue;
+            hold[r][0] = ldf(rx, vk4, roff[r]);
+            hold[r][1] = r > 0 ? ldf(ry, vk4, roff[r]) : 0.f;
+            hold[r][2] = ldf(rz, vk4, roff[r]);
+        }
+    };
+
+    // new H of plane i (cur) from E of plane i+1 (nxt), rows r_lo..J
+    auto h_plane = [&](int i, const Cell (&cur)[J + 2], const Cell (&nxt)[J + 2], HNew (&out)[J + 1],
+                       const float (&hold)[J + 1][3], int r_lo, bool store) {
+        const yee3d_axis_row ax = uniform_row(d.axis_rows[0], i);
         const Row rx = h_row(ax);
 #pragma unroll
         for (int r = 0; r < J + 1; ++r) {
             if (r < r_lo) continue;
+            const Row ry = h_row(uniform_row(d.axis_rows[1], jr[r]));
             const float ey_zp = from_next(cur[r].ey);
             const float ex_zp = from_next(cur[r].ex);
             const float4 c = coef<LDS>(d, cur[r].id);
-            out[r] = h_cell(d, par, rows[r].own && k_own, c.x, c.y, hold[r][0], hold[r][1], hold[r][2], cur[r].ex,
+            out[r] = h_cell(d, par, store && own[r] && k_own, c.x, c.y, hold[r][0], hold[r][1], hold[r][2], cur[r].ex,
                             cur[r].ey, cur[r].ez, cur[r + 1].ez, cur[r + 1].ex, ey_zp, ex_zp, nxt[r].ey, nxt[r].ez,
-                            ihx, rows[r].ih, ihz, rx, rows[r].ch, rz_h, i, rows[r].j, k);
-        }
-    };
-
-    auto load_hold = [&](int i, float (&hold)[J + 1][3], int r_lo) {
-        const int64_t base = (int64_t)i * sx + k;
-#pragma unroll
-        for (int r = 0; r < J + 1; ++r) {
-            if (r < r_lo) continue;
-            const int64_t o = base + (int64_t)rows[r].j * sy;
-            hold[r][0] = h_o[0][o];
-            hold[r][1] = h_o[1][o];
-            hold[r][2] = h_o[2][o];
+                            ax.inv_h, ihy[r], ihz, rx, ry, rz_h, i, jr[r], k);
         }
     };
 
@@ -399,19 +405,11 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
         Cell prev[J + 2];
         load_plane(im, prev);
         load_hold(im, hold, 1);
-        const yee3d_axis_row ax = uniform_row(d.axis_rows[0], im);
-        const float ihx = ax.inv_h;
-        const Row rx = h_row(ax);
+        h_plane(im, prev, cur, hn, hold, 1, false);
 #pragma unroll
         for (int r = 1; r < J + 1; ++r) {
-            const float ey_zp = from_next(prev[r].ey);
-            const float ex_zp = from_next(prev[r].ex);
-            const float4 c = coef<LDS>(d, prev[r].id);
-            const HNew h = h_cell(d, par, false, c.x, c.y, hold[r][0], hold[r][1], hold[r][2], prev[r].ex, prev[r].ey,
-                                  prev[r].ez, prev[r + 1].ez, prev[r + 1].ex, ey_zp, ex_zp, cur[r].ey, cur[r].ez, ihx,
-                                  rows[r].ih, ihz, rx, rows[r].ch, rz_h, im, rows[r].j, k);
-            hy_prev[r] = h.hy;
-            hz_prev[r] = h.hz;
+            hy_prev[r] = hn[r].hy;
+            hz_prev[r] = hn[r].hz;
         }
     } else {
 #pragma unroll
@@ -420,19 +418,24 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
 
     for (int i = i0; i < i1; ++i) {
         const int ip = fold(i + 1, nx, px);
-        const yee3d_axis_row ax = uniform_row(d.axis_rows[0], i);
         load_plane(ip, nxt);
         load_hold(i, hold, 0);
-        h_plane(i, ax, cur, nxt, hn, hold, 0);
+        h_plane(i, cur, nxt, hn, hold, 0, true);
         // zero the H that stands for "below cell 0" on non-periodic axes
-        if (rows[0].zero) { hn[0].hx = 0.f; hn[0].hz = 0.f; }
+        if (row0_zero) { hn[0].hx = 0.f; hn[0].hz = 0.f; }
         if (k_zero) {
 #pragma unroll
             for (int r = 0; r < J + 1; ++r) { hn[r].hx = 0.f; hn[r].hy = 0.f; }
         }
-        const float iex = ax.inv_e;
+        const yee3d_axis_row ax = uniform_row(d.axis_rows[0], i);
         const Row rx_e = e_row(ax);
-        const int64_t plane = (int64_t)i * sx + k;
+        const int64_t po = (int64_t)i * sx;
+        const Rsrc wex = plane_rsrc(d.field[par ^ 1][0] + po, plane_bytes);
+        const Rsrc wey = plane_rsrc(d.field[par ^ 1][1] + po, plane_bytes);
+        const Rsrc wez = plane_rsrc(d.field[par ^ 1][2] + po, plane_bytes);
+        const Rsrc whx = plane_rsrc(d.field[par ^ 1][3] + po, plane_bytes);
+        const Rsrc why = plane_rsrc(d.field[par ^ 1][4] + po, plane_bytes);
+        const Rsrc whz = plane_rsrc(d.field[par ^ 1][5] + po, plane_bytes);
 #pragma unroll
         for (int r = 1; r < J + 1; ++r) {
             const float hx_km = from_prev(hn[r].hx);
@@ -444,21 +447,21 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
             const float dhz_x = hn[r].hz - hz_prev[r];
             const float dhy_x = hn[r].hy - hy_prev[r];
             const float dhx_y = hn[r].hx - hn[r - 1].hx;
-            if (rows[r].own && k_own) {
+            if (own[r] && k_own) {
                 const int id = cur[r].id;
                 const float2 fx = face<LDS>(d, id, nxt[r].id);
                 const float2 fy = face<LDS>(d, id, cur[r + 1].id);
                 const float2 fz = face<LDS>(d, id, id_zp);
+                const Row ry_e = e_row(uniform_row(d.axis_rows[1], jr[r]));
                 float ex = cur[r].ex, ey = cur[r].ey, ez = cur[r].ez;
-                e_cell<LDS>(d, n, ex, ey, ez, fx, fy, fz, dhz_y, dhy_z, dhx_z, dhz_x, dhy_x, dhx_y, iex, rows[r].ie,
-                            iez, rx_e, rows[r].ce, rz_e, i, rows[r].j, k);
-                const int64_t o = plane + (int64_t)rows[r].j * sy;
-                e_n[0][o] = ex;
-                e_n[1][o] = ey;
-                e_n[2][o] = ez;
-                h_n[0][o] = hn[r].hx;
-                h_n[1][o] = hn[r].hy;
-                h_n[2][o] = hn[r].hz;
+                e_cell<LDS>(d, n, ex, ey, ez, fx, fy, fz, dhz_y, dhy_z, dhx_z, dhz_x, dhy_x, dhx_y, ax.inv_e, iey[r],
+                            iez, rx_e, ry_e, rz_e, i, jr[r], k);
+                stf(wex, vk4, roff[r], ex);
+                stf(wey, vk4, roff[r], ey);
+                stf(wez, vk4, roff[r], ez);
+                stf(whx, vk4, roff[r], hn[r].hx);
+                stf(why, vk4, roff[r], hn[r].hy);
+                stf(whz, vk4, roff[r], hn[r].hz);
             }
             hy_prev[r] = hn[r].hy;
             hz_prev[r] = hn[r].hz;
@@ -477,7 +480,7 @@ __global__ void __launch_bounds__(256) yee3d_step_kernel(yee3d_desc d, int32_t t
 {
     const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
-    const int tile = yee3::block_order(blockIdx.x, gridDim.x) * 4 + yee3::wave_index();
+    const int tile = blockIdx.x * 4 + yee3::wave_index();
     const bool mine = tile < tiles;
     if (!__syncthreads_or(mine)) return;
     const bool lds = yee3::stage_tables(d);

@@ -50,7 +50,7 @@ enum CtxKind { kSingle2D = 1, kBatch2D = 2, kSingle3D = 3 };
 constexpr int kRows3D = 4;         // y rows per 3D wavefront tile (yee3d.hip J)
 constexpr int kOwned3D = 62;       // z cells a 3D wavefront tile owns (yee3d.hip OWNED)
 constexpr int kMinAutoPlanes = 4;
-constexpr int kMaxAutoPlanes = 32;
+constexpr int kMaxAutoPlanes = 8;    // short tiles: measured fastest at 512^3
 
 }  // namespace
 
@@ -136,6 +136,7 @@ int32_t check_3d(const yee3d_desc* d)
     for (int a = 0; a < 3; ++a)
         if (n[a] < 2) return fail(YEE_EINVAL, "3D grid needs nx, ny, nz >= 2");
     if ((int64_t)d->nx * d->ny * d->nz > ((int64_t)1 << 40)) return fail(YEE_EINVAL, "3D grid too large");
+    if ((int64_t)d->ny * d->nz >= ((int64_t)1 << 29)) return fail(YEE_EINVAL, "3D plane (ny * nz) too large");
     if (d->rows_per_tile != kRows3D) return fail(YEE_EINVAL, "rows_per_tile must be 4 (or 0)");
     if (d->planes_per_tile < 1) return fail(YEE_EINVAL, "planes_per_tile must be >= 1");
     for (int s = 0; s < 2; ++s)
