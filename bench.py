@@ -16,7 +16,12 @@ seeded dielectric boxes (edges uniform 0.2-3 um, eps_r uniform in [1, 12]), a Ga
 [0.6, 1.6] um) and one flux line at 3 wavelengths each. Reports the cell-updates/s of one batched launch per step
 (Batch2D) and of the plain form (each instance run alone, one after another, with the same kernels and graphs).
 
-    python bench.py [--gpus 1] [--steps 2000] [--warmup 50] [--workload ez8192 | yee3d_512 | batch256]
+--workload serve: a seeded stream of jobs with Poisson arrivals at 50, 100, 200 and 400 jobs/s (--jobs per rate): 80 %
+2D (as batch256, 500-5000 steps) and 20 % 3D (64^3-192^3 cells, 8-cell CPML, 5-40 boxes, one dipole, one flux plane,
+500-5000 steps), served by FDTDServer (continuous batching, HBM budget 200 GB) and by the plain FIFO form (one job at
+a time). Reports p50 / p99 latency and aggregate cell-updates/s of both.
+
+    python bench.py [--gpus 1] [--steps 2000] [--warmup 50] [--workload ez8192 | yee3d_512 | batch256 | serve]
 """
 import argparse
 import json
@@ -85,6 +90,87 @@ def build_batch_instances(count=256, seed=SEED, **options):
         sims.append(sim)
         steps.append(int(rng.integers(1000, 4001)))
     return sims, steps
+
+
+def serve_jobs(rate, count, seed=SEED):
+    """Seeded job stream: Poisson arrivals at `rate` jobs/s, 80 % 2D and 20 % 3D jobs."""
+    from opensol_amd.serve import Job
+    rng = np.random.default_rng(seed + int(rate))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    jobs = []
+    for k, arrival in enumerate(arrivals):
+        job_seed = int(rng.integers(1 << 30))
+        steps = int(rng.integers(500, 5001))
+        if rng.random() < 0.8:
+            jobs.append(Job(build=_serve_2d(job_seed), steps=steps, arrival=float(arrival), name=f'2d-{k}'))
+        else:
+            jobs.append(Job(build=_serve_3d(job_seed), steps=steps, arrival=float(arrival), name=f'3d-{k}'))
+    return jobs
+
+
+def _serve_2d(seed):
+    def build():
+        from opensol_amd import Simulation
+        rng = np.random.default_rng(seed)
+        dx, npml = 25e-9, 12
+        nx, ny = (int(round(np.exp(rng.uniform(np.log(128), np.log(1024))))) for _ in range(2))
+        sx, sy = window_for_cells(nx, npml, dx), window_for_cells(ny, npml, dx)
+        sim = Simulation(sx, sy, dx, npml=npml, polarization='ez' if rng.random() < 0.5 else 'hz')
+        for _ in range(int(rng.integers(20, 101))):
+            cx, cy = rng.uniform(0, sx), rng.uniform(0, sy)
+            wx, wy = rng.uniform(0.05, 0.4) * sx, rng.uniform(0.05, 0.4) * sy
+            sim.add_dielectric_rectangle(cx - wx / 2, cy - wy / 2, cx + wx / 2, cy + wy / 2,
+                                         eps_r=rng.uniform(1.0, 12.0))
+        wavelength = rng.uniform(0.6e-6, 1.6e-6)
+        sim.add_line_source(axis='y', position=0.25 * sx, wavelength=wavelength, bandwidth=0.5)
+        sim.add_flux_monitor(axis='y', position=0.75 * sx, span=(0.0, sy),
+                             wavelengths=[0.9 * wavelength, wavelength, 1.1 * wavelength])
+        return sim
+    return build
+
+
+def _serve_3d(seed):
+    def build():
+        from opensol_amd import Simulation3D
+        rng = np.random.default_rng(seed)
+        dx, npml = 25e-9, 8
+        n = int(rng.integers(64, 193))
+        size = window_for_cells(n, npml, dx)
+        sim = Simulation3D(size, size, size, dx, npml=npml)
+        for _ in range(int(rng.integers(5, 41))):
+            centre = rng.uniform(0, size, 3)
+            edges = rng.uniform(0.1, 0.5, 3) * size
+            sim.add_dielectric_box(tuple(centre - edges / 2), tuple(centre + edges / 2), eps_r=rng.uniform(1.0, 12.0))
+        c = size / 2
+        wavelength = rng.uniform(0.6e-6, 1.6e-6)
+        sim.add_dipole(c, c, c, orientation='z', wavelength=wavelength, bandwidth=0.5)
+        sim.add_flux_plane('x', c + 0.3 * size, wavelengths=[0.9 * wavelength, wavelength, 1.1 * wavelength])
+        return sim
+    return build
+
+
+def run_serve(args, torch):
+    from opensol_amd.serve import FDTDServer
+    server = FDTDServer(hbm_budget=200e9, chunk_steps=100, graph_steps=0)
+    rates = {}
+    for rate in (50, 100, 200, 400):
+        served = server.run(serve_jobs(rate, args.jobs))
+        plain = server.run_plain(serve_jobs(rate, args.jobs))
+        for name, stats in (('served', served), ('plain', plain)):
+            stats.pop('results')
+        rates[str(rate)] = {'served': served, 'plain': plain,
+                            'throughput_ratio': served['cell_updates_per_s'] / plain['cell_updates_per_s']}
+        print(f'# rate {rate}/s: served p50 {served["p50_latency_s"]:.3f} s p99 {served["p99_latency_s"]:.3f} s '
+              f'{served["cell_updates_per_s"]:.3e}/s | plain p50 {plain["p50_latency_s"]:.3f} s p99 '
+              f'{plain["p99_latency_s"]:.3f} s {plain["cell_updates_per_s"]:.3e}/s', file=sys.stderr, flush=True)
+    top = rates['400']['served']
+    return {'metric': 'served FDTD stream: aggregate Yee cell-updates/s, p50/p99 latency vs plain FIFO',
+            'value': top['cell_updates_per_s'], 'unit': 'cell-updates/s', 'n_gpus': 1, 'steps': None, 'warmup': 0,
+            'ms_per_step': None, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
+            'data': 'synthetic (seeded Poisson job stream, 80 % 2D / 20 % 3D)',
+            'config': {'workload': 'serve_2d3d_poisson', 'jobs_per_rate': args.jobs, 'hbm_budget_B': 200e9,
+                       'chunk_steps': 100, 'seed': SEED},
+            'rates': rates, 'roofline': None, 'cpu_baseline': None}
 
 
 def build_3d_workload(n=512, npml=12, dx=25e-9, boxes=200, seed=SEED, **options):
@@ -333,7 +419,8 @@ def run_batch(args, torch):
 def main():
     parser = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     parser.add_argument('--gpus', type=int, default=1)
-    parser.add_argument('--workload', default='ez8192', choices=('ez8192', 'yee3d_512', 'batch256'))
+    parser.add_argument('--workload', default='ez8192', choices=('ez8192', 'yee3d_512', 'batch256', 'serve'))
+    parser.add_argument('--jobs', type=int, default=100, help='serve: jobs per arrival rate')
     parser.add_argument('--steps', type=int, default=None, help='timed steps (default 2000 2D, 500 3D)')
     parser.add_argument('--warmup', type=int, default=None, help='untimed steps (default 50 2D, 20 3D)')
     parser.add_argument('--graph-steps', type=int, default=None, help='steps per HIP graph (default 50 2D, 10 3D)')
@@ -362,6 +449,8 @@ def main():
     torch.cuda.set_device(0)
     if args.workload == 'batch256':
         result = run_batch(args, torch)
+    elif args.workload == 'serve':
+        result = run_serve(args, torch)
     else:
         result = run_3d(args, torch) if three else run_2d(args, torch)
     print(json.dumps(result))

@@ -102,7 +102,7 @@ typedef struct yee2d_desc {
     int32_t pitch;           /* row stride in elements of every (nx, *) array: a multiple of 4, >= ny */
     int32_t polarization;    /* YEE_EZ or YEE_HZ */
     int32_t periodic_x, periodic_y;
-    int32_t rows_per_tile;   /* x rows marched by one wavefront tile (0: chosen from the device's wave slots) */
+    int32_t rows_per_tile;   /* x rows marched by one wavefront tile (0: 4, balanced over nx) */
     int32_t lane_columns;    /* y columns per lane: 2 or 4 (0: 2); a tile spans 64 * lane_columns columns */
     float* field[2][3];      /* [set][component]: E_z (ez, hx, hy) or H_z (hz, ex, ey) */
     const uint16_t* material;
@@ -211,7 +211,7 @@ typedef struct yee3d_desc {
     int32_t nx, ny, nz;
     int32_t periodic[3];
     int32_t rows_per_tile;   /* y rows per wavefront tile: 4 (0: 4) */
-    int32_t planes_per_tile; /* x planes marched by one wavefront (0: chosen from the device's wave slots) */
+    int32_t planes_per_tile; /* x planes marched by one wavefront (0: 4, balanced over nx) */
     float* field[2][6];      /* [set][ex, ey, ez, hx, hy, hz], (nx, ny, nz) z contiguous */
     const uint16_t* material;
     const float* table;      /* (n_materials, 8) */
