@@ -162,7 +162,9 @@ def run_serve(args, torch):
                             'throughput_ratio': served['cell_updates_per_s'] / plain['cell_updates_per_s']}
         print(f'# rate {rate}/s: served p50 {served["p50_latency_s"]:.3f} s p99 {served["p99_latency_s"]:.3f} s '
               f'{served["cell_updates_per_s"]:.3e}/s | plain p50 {plain["p50_latency_s"]:.3f} s p99 '
-              f'{plain["p99_latency_s"]:.3f} s {plain["cell_updates_per_s"]:.3e}/s', file=sys.stderr, flush=True)
+              f'{plain["p99_latency_s"]:.3f} s {plain["cell_updates_per_s"]:.3e}/s | ratio '
+              f'{rates[str(rate)]["throughput_ratio"]:.2f} | served host phases {served["host_phases_s"]}',
+              file=sys.stderr, flush=True)
     top = rates['400']['served']
     return {'metric': 'served FDTD stream: aggregate Yee cell-updates/s, p50/p99 latency vs plain FIFO',
             'value': top['cell_updates_per_s'], 'unit': 'cell-updates/s', 'n_gpus': 1, 'steps': None, 'warmup': 0,
@@ -420,7 +422,7 @@ def main():
     parser = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     parser.add_argument('--gpus', type=int, default=1)
     parser.add_argument('--workload', default='ez8192', choices=('ez8192', 'yee3d_512', 'batch256', 'serve'))
-    parser.add_argument('--jobs', type=int, default=100, help='serve: jobs per arrival rate')
+    parser.add_argument('--jobs', type=int, default=200, help='serve: jobs per arrival rate')
     parser.add_argument('--steps', type=int, default=None, help='timed steps (default 2000 2D, 500 3D)')
     parser.add_argument('--warmup', type=int, default=None, help='untimed steps (default 50 2D, 20 3D)')
     parser.add_argument('--graph-steps', type=int, default=None, help='steps per HIP graph (default 50 2D, 10 3D)')

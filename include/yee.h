@@ -289,6 +289,16 @@ int32_t yee_destroy(yee_ctx* ctx);
 size_t yee2d_batch_bytes(int32_t n);
 int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, void* stream, yee_ctx** out);
 
+/*
+ * Batched 3D instances of uneven sizes stepped by one launch per step over their tiles (plus one sample launch over
+ * all their flux-plane cells). descs_dev: caller device buffer of yee3d_batch_bytes(n) bytes that receives the
+ * descriptors and the block and plane-cell prefixes (copied on `stream`, which is synchronized once before the call
+ * returns). Each instance keeps its own clock, step_limit and planes_per_tile. No counterpart in the reference, which
+ * steps one model at a time (model/study.py:437-447).
+ */
+size_t yee3d_batch_bytes(int32_t n);
+int32_t yee3d_batch_create(const yee3d_desc* descs, int32_t n, void* descs_dev, void* stream, yee_ctx** out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -91,6 +91,8 @@ SIGNATURES = (
     ('yee_destroy', c_i32, (c_ptr,)),
     ('yee2d_batch_bytes', ctypes.c_size_t, (c_i32,)),
     ('yee2d_batch_create', c_i32, (ctypes.POINTER(Desc2DC), c_i32, c_ptr, c_ptr, ctypes.POINTER(c_ptr))),
+    ('yee3d_batch_bytes', ctypes.c_size_t, (c_i32,)),
+    ('yee3d_batch_create', c_i32, (ctypes.POINTER(Desc3DC), c_i32, c_ptr, c_ptr, ctypes.POINTER(c_ptr))),
 )
 
 _LIB = None

@@ -1230,7 +1230,9 @@ __global__ void __launch_bounds__(256) yee2d_batch_kernel(const yee2d_desc* __re
         const int mid = (lo + hi) >> 1;
         if (prefix[mid] <= tile) lo = mid; else hi = mid;
     }
-    const yee2d_desc& d = descs[lo];
+    // Read the descriptor through the constant address space: its fields become scalar loads, as a kernel argument's.
+    typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
+    const yee2d_desc& d = *(const yee2d_desc*)((ConstDesc*)descs + lo);
     const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
     const yee::TileBox t = yee::tile_box<VEC>(d, tile - prefix[lo]);

@@ -473,17 +473,17 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
 
 }  // namespace yee3
 
-// One 3D instance: tile t = (i chunk, j tile, k tile), k fastest, one per wavefront.
+namespace yee3 {
+
+// The tiles of one block of one instance: tile t = (i chunk, j tile, k tile), k fastest, one per wavefront. Every
+// thread of the block reaches the table barrier before a wave without a tile leaves.
 template <int J>
-__global__ void __launch_bounds__(256) yee3d_step_kernel(yee3d_desc d, int32_t tiles_j, int32_t tiles_k,
-                                                        int32_t tiles, int32_t offset)
+__device__ __forceinline__ void step_block(const yee3d_desc& d, int64_t n, int tile, int tiles, int tiles_j,
+                                           int tiles_k)
 {
-    const int64_t n = *d.clock + offset;
-    if (n >= d.step_limit) return;
-    const int tile = blockIdx.x * 4 + yee3::wave_index();
     const bool mine = tile < tiles;
     if (!__syncthreads_or(mine)) return;
-    const bool lds = yee3::stage_tables(d);
+    const bool lds = stage_tables(d);
     if (!mine) return;
     const int tk = tile % tiles_k;
     const int tj = (tile / tiles_k) % tiles_j;
@@ -491,24 +491,63 @@ __global__ void __launch_bounds__(256) yee3d_step_kernel(yee3d_desc d, int32_t t
     const int i0 = ic * d.planes_per_tile;
     const int i1 = min(d.nx, i0 + d.planes_per_tile);
     if (lds)
-        yee3::tile3d<J, true>(d, n, i0, i1, tj, tk);
+        tile3d<J, true>(d, n, i0, i1, tj, tk);
     else
-        yee3::tile3d<J, false>(d, n, i0, i1, tj, tk);
+        tile3d<J, false>(d, n, i0, i1, tj, tk);
+}
+
+}  // namespace yee3
+
+// One 3D instance, four tiles per block.
+template <int J>
+__global__ void __launch_bounds__(256) yee3d_step_kernel(yee3d_desc d, int32_t tiles_j, int32_t tiles_k,
+                                                        int32_t tiles, int32_t offset)
+{
+    const int64_t n = *d.clock + offset;
+    if (n >= d.step_limit) return;
+    yee3::step_block<J>(d, n, blockIdx.x * 4 + yee3::wave_index(), tiles, tiles_j, tiles_k);
+}
+
+// Many 3D instances of uneven sizes in one launch: block b belongs to the instance whose block range
+// [prefix[k], prefix[k+1]) holds it, so a block stages one instance's table. Each instance runs on its own clock
+// and stops at its own step limit.
+template <int J>
+__global__ void __launch_bounds__(256) yee3d_batch_kernel(const yee3d_desc* __restrict__ descs,
+                                                         const int32_t* __restrict__ prefix, int32_t n_inst,
+                                                         int32_t offset)
+{
+    const int block = blockIdx.x;
+    int lo = 0, hi = n_inst;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= block) lo = mid; else hi = mid;
+    }
+    // Read the descriptor through the constant address space: its fields become scalar loads, as a kernel argument's.
+    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
+    const yee3d_desc& d = *(const yee3d_desc*)((ConstDesc*)descs + lo);
+    const int64_t n = *d.clock + offset;
+    if (n >= d.step_limit) return;
+    const int tiles_j = (d.ny + J - 1) / J;
+    const int tiles_k = (d.nz + yee3::OWNED - 1) / yee3::OWNED;
+    const int tiles = ((d.nx + d.planes_per_tile - 1) / d.planes_per_tile) * tiles_j * tiles_k;
+    yee3::step_block<J>(d, n, (block - prefix[lo]) * 4 + yee3::wave_index(), tiles, tiles_j, tiles_k);
 }
 
 template __global__ void yee3d_step_kernel<4>(yee3d_desc, int32_t, int32_t, int32_t, int32_t);
+template __global__ void yee3d_batch_kernel<4>(const yee3d_desc* __restrict__, const int32_t* __restrict__, int32_t,
+                                               int32_t);
 
 // FP64 Kahan DFT samples of the flux planes after step n (when (n + 1) % m == 0): one thread per plane cell and pair;
 // E = mean of the layers edge - 1 and edge along the normal, H on layer edge - 1 (monitors3d.py:90-101).
-__global__ void __launch_bounds__(256) yee3d_sample_kernel(yee3d_desc d, int64_t cells_total, int32_t offset)
+namespace yee3 {
+
+// Sample of plane cell t (cells of plane 0 pair 0, then pair 1, then plane 1 ...) after step n.
+__device__ __forceinline__ void sample_cell(const yee3d_desc& d, int64_t n, int64_t t)
 {
-    const int64_t n = *d.clock + offset;
     if (n >= d.step_limit || d.n_planes == 0) return;
     const int m = d.sample_interval > 0 ? d.sample_interval : 1;
     if ((n + 1) % m != 0) return;
     const int64_t sample = (n + 1) / m - 1;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cells_total) return;
     const int set = (int)((n & 1) ^ 1);
     for (int p = 0; p < d.n_planes; ++p) {
         const yee_flux_plane& pl = d.plane[p];
@@ -546,5 +585,43 @@ __global__ void __launch_bounds__(256) yee3d_sample_kernel(yee3d_desc d, int64_t
             }
             return;
         }
+    }
+}
+
+}  // namespace yee3
+
+__global__ void __launch_bounds__(256) yee3d_sample_kernel(yee3d_desc d, int64_t cells_total, int32_t offset)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cells_total) return;
+    yee3::sample_cell(d, *d.clock + offset, t);
+}
+
+// Samples of many instances in one launch: thread t belongs to the instance whose plane-cell range
+// [prefix[k], prefix[k+1]) holds it.
+__global__ void __launch_bounds__(256) yee3d_batch_sample_kernel(const yee3d_desc* __restrict__ descs,
+                                                                const int64_t* __restrict__ prefix, int32_t n_inst,
+                                                                int32_t offset)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= prefix[n_inst]) return;
+    int lo = 0, hi = n_inst;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= t) lo = mid; else hi = mid;
+    }
+    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
+    const yee3d_desc& d = *(const yee3d_desc*)((ConstDesc*)descs + lo);
+    yee3::sample_cell(d, *d.clock + offset, t - prefix[lo]);
+}
+
+// Advance the clocks of a 3D batch by k steps, each bounded by its instance's step limit.
+__global__ void yee_clock_advance_batch3d(const yee3d_desc* __restrict__ descs, int32_t n_inst, int64_t k)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_inst) {
+        int64_t* clock = descs[i].clock;
+        const int64_t v = *clock + k;
+        *clock = v < descs[i].step_limit ? v : descs[i].step_limit;
     }
 }

@@ -20,7 +20,13 @@ __global__ void yee2d_batch_kernel(const yee2d_desc* __restrict__ descs, const i
                                    int32_t n_inst, int32_t offset);
 template <int J>
 __global__ void yee3d_step_kernel(yee3d_desc d, int32_t tiles_j, int32_t tiles_k, int32_t tiles, int32_t offset);
+template <int J>
+__global__ void yee3d_batch_kernel(const yee3d_desc* __restrict__ descs, const int32_t* __restrict__ prefix,
+                                   int32_t n_inst, int32_t offset);
 __global__ void yee3d_sample_kernel(yee3d_desc d, int64_t cells_total, int32_t offset);
+__global__ void yee3d_batch_sample_kernel(const yee3d_desc* __restrict__ descs, const int64_t* __restrict__ prefix,
+                                          int32_t n_inst, int32_t offset);
+__global__ void yee_clock_advance_batch3d(const yee3d_desc* __restrict__ descs, int32_t n_inst, int64_t k);
 __global__ void yee_clock_advance(int64_t* clock, int64_t k, int64_t limit);
 __global__ void yee_clock_advance_batch(const yee2d_desc* __restrict__ descs, int32_t n_inst, int64_t k);
 
@@ -45,7 +51,7 @@ constexpr int kMinAutoRows = 8;    // shortest tile picked automatically (each t
 constexpr int kMaxAutoRows = 32;   // longest: many short tiles keep every CU busy to the end of the step
 constexpr int kEdgeSplit = 8;     // row chunks per edge tile (yee2d.hip EDGE_SPLIT)
 
-enum CtxKind { kSingle2D = 1, kBatch2D = 2, kSingle3D = 3 };
+enum CtxKind { kSingle2D = 1, kBatch2D = 2, kSingle3D = 3, kBatch3D = 4 };
 
 constexpr int kRows3D = 4;         // y rows per 3D wavefront tile (yee3d.hip J)
 constexpr int kOwned3D = 62;       // z cells a 3D wavefront tile owns (yee3d.hip OWNED)
@@ -70,6 +76,11 @@ struct yee_ctx {
     yee3d_desc d3{};
     int32_t tiles_j3 = 0, tiles_k3 = 0;
     int64_t sample_cells = 0;   // plane cells x pairs of the sample launch
+    // 3D batch: descriptors, block prefix and plane-cell prefix in the caller's device buffer
+    std::vector<yee3d_desc> descs3;
+    yee3d_desc* descs3_dev = nullptr;
+    int32_t* block_prefix_dev = nullptr;
+    int64_t* sample_prefix_dev = nullptr;
     // graph replay
     int32_t graph_steps = 0;
     hipGraphExec_t graph = nullptr;
@@ -301,19 +312,31 @@ void apply_defaults_3d(yee3d_desc& d)
     }
 }
 
+// Wavefront tiles of a 3D instance: x chunks x y tiles x z tiles.
+int64_t tiles_3d(const yee3d_desc& d)
+{
+    const int64_t chunks = (d.nx + d.planes_per_tile - 1) / d.planes_per_tile;
+    return chunks * ((d.ny + kRows3D - 1) / kRows3D) * ((d.nz + kOwned3D - 1) / kOwned3D);
+}
+
+// Plane cells x pairs sampled by the sample launch of a 3D instance.
+int64_t sample_cells_3d(const yee3d_desc& d)
+{
+    int64_t cells = 0;
+    for (int p = 0; p < d.n_planes; ++p)
+        for (int q = 0; q < 2; ++q)
+            cells += (int64_t)(d.plane[p].hi[q][0] - d.plane[p].lo[q][0]) * (d.plane[p].hi[q][1] - d.plane[p].lo[q][1]);
+    return cells;
+}
+
 void set_geometry_3d(yee_ctx* ctx)
 {
     const yee3d_desc& d = ctx->d3;
     ctx->tiles_j3 = (d.ny + kRows3D - 1) / kRows3D;
     ctx->tiles_k3 = (d.nz + kOwned3D - 1) / kOwned3D;
-    const int64_t chunks = (d.nx + d.planes_per_tile - 1) / d.planes_per_tile;
-    ctx->tiles = chunks * ctx->tiles_j3 * ctx->tiles_k3;
+    ctx->tiles = tiles_3d(d);
     ctx->blocks = (int)((ctx->tiles + 3) / 4);
-    int64_t cells = 0;
-    for (int p = 0; p < d.n_planes; ++p)
-        for (int q = 0; q < 2; ++q)
-            cells += (int64_t)(d.plane[p].hi[q][0] - d.plane[p].lo[q][0]) * (d.plane[p].hi[q][1] - d.plane[p].lo[q][1]);
-    ctx->sample_cells = cells;
+    ctx->sample_cells = sample_cells_3d(d);
 }
 
 void drop_graph(yee_ctx* ctx)
@@ -355,6 +378,22 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
         YEE_HIP(hipGetLastError());
         hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, stream, ctx->d3.clock, (int64_t)count,
                            ctx->d3.step_limit);
+        YEE_HIP(hipGetLastError());
+        return YEE_OK;
+    }
+    if (ctx->kind == kBatch3D) {
+        const int sample_blocks = (int)((ctx->sample_cells + 255) / 256);
+        for (int32_t k = 0; k < count; ++k) {
+            hipLaunchKernelGGL(yee3d_batch_kernel<kRows3D>, dim3(ctx->blocks), dim3(256), 0, stream, ctx->descs3_dev,
+                               ctx->block_prefix_dev, ctx->n_inst, k);
+            if (sample_blocks > 0)
+                hipLaunchKernelGGL(yee3d_batch_sample_kernel, dim3(sample_blocks), dim3(256), 0, stream,
+                                   ctx->descs3_dev, ctx->sample_prefix_dev, ctx->n_inst, k);
+        }
+        YEE_HIP(hipGetLastError());
+        const int threads = 256;
+        hipLaunchKernelGGL(yee_clock_advance_batch3d, dim3((ctx->n_inst + threads - 1) / threads), dim3(threads), 0,
+                           stream, ctx->descs3_dev, ctx->n_inst, (int64_t)count);
         YEE_HIP(hipGetLastError());
         return YEE_OK;
     }
@@ -651,6 +690,52 @@ int32_t yee3d_update(yee_ctx* ctx, const yee3d_desc* desc)
     ctx->d3 = d;
     set_geometry_3d(ctx);
     return YEE_OK;
+}
+
+int32_t yee3d_batch_create(const yee3d_desc* descs, int32_t n, void* descs_dev, void* stream, yee_ctx** out)
+{
+    if (!out) return fail(YEE_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (!descs || n < 1 || !descs_dev) return fail(YEE_EINVAL, "batch needs descriptors and a device buffer");
+    std::vector<yee3d_desc> host(descs, descs + n);
+    std::vector<int64_t> sample_prefix(n + 1, 0);
+    std::vector<int32_t> block_prefix(n + 1, 0);
+    for (int32_t k = 0; k < n; ++k) {
+        apply_defaults_3d(host[k]);
+        const int32_t status = check_3d(&host[k]);
+        if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
+        if (host[k].rows_per_tile != kRows3D) return fail(YEE_EINVAL, "batched 3D instances use 4 rows per tile");
+        const int64_t next = block_prefix[k] + (tiles_3d(host[k]) + 3) / 4;
+        if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
+        block_prefix[k + 1] = (int32_t)next;
+        sample_prefix[k + 1] = sample_prefix[k] + sample_cells_3d(host[k]);
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    char* base = reinterpret_cast<char*>(descs_dev);
+    auto* dev_descs = reinterpret_cast<yee3d_desc*>(base);
+    auto* dev_samples = reinterpret_cast<int64_t*>(base + sizeof(yee3d_desc) * n);
+    auto* dev_blocks = reinterpret_cast<int32_t*>(base + sizeof(yee3d_desc) * n + sizeof(int64_t) * (n + 1));
+    YEE_HIP(hipMemcpyAsync(dev_descs, host.data(), sizeof(yee3d_desc) * n, hipMemcpyHostToDevice, s));
+    YEE_HIP(hipMemcpyAsync(dev_samples, sample_prefix.data(), sizeof(int64_t) * (n + 1), hipMemcpyHostToDevice, s));
+    YEE_HIP(hipMemcpyAsync(dev_blocks, block_prefix.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
+    YEE_HIP(hipStreamSynchronize(s));   // the host vectors are the copies' sources
+    yee_ctx* ctx = new yee_ctx();
+    ctx->kind = kBatch3D;
+    ctx->descs3 = std::move(host);
+    ctx->descs3_dev = dev_descs;
+    ctx->sample_prefix_dev = dev_samples;
+    ctx->block_prefix_dev = dev_blocks;
+    ctx->n_inst = n;
+    ctx->blocks = block_prefix[n];
+    ctx->tiles = (int64_t)block_prefix[n] * 4;
+    ctx->sample_cells = sample_prefix[n];
+    *out = ctx;
+    return YEE_OK;
+}
+
+size_t yee3d_batch_bytes(int32_t n)
+{
+    return n < 1 ? 0 : sizeof(yee3d_desc) * (size_t)n + (sizeof(int64_t) + sizeof(int32_t)) * (size_t)(n + 1);
 }
 
 int32_t yee3d_get_desc(const yee_ctx* ctx, yee3d_desc* out)

@@ -71,3 +71,61 @@ def test_autotuned_shape_gives_the_same_bits(gpu):
     for name in a[0]:
         assert np.array_equal(a[0][name], b[0][name])
     assert np.array_equal(a[1], b[1])
+
+
+# 3D: (window cells x, y, z, npml, boundaries, steps, planes per tile in the batch)
+PERIODIC_Y = {'y_lo': 'periodic', 'y_hi': 'periodic'}
+PERIODIC_X = {'x_lo': 'periodic', 'x_hi': 'periodic'}
+INSTANCES_3D = [((20, 14, 18), 4, None, 60), ((9, 22, 12), 3, PERIODIC_Y, 41), ((16, 16, 70), 4, None, 75),
+                ((30, 8, 11), 3, PERIODIC_X, 33), ((12, 12, 12), 2, None, 90)]
+
+
+def _make_3d(spec, seed):
+    from opensol_amd import Simulation3D
+    (cx, cy, cz), npml, bnd, _ = spec
+    dx = 25e-9
+    size = np.array([cx, cy, cz]) * dx
+    sim = Simulation3D(*size, dx, npml=npml, boundaries=bnd)
+    rng = np.random.default_rng(100 + seed)
+    for _ in range(4):
+        lo = rng.uniform(0, 1, 3) * size
+        sim.add_dielectric_box(tuple(lo), tuple(lo + rng.uniform(0.2, 0.6, 3) * size), eps_r=rng.uniform(1.0, 9.0))
+    centre = size / 2
+    sim.add_dipole(*centre, orientation='xyz'[seed % 3], wavelength=0.5e-6, bandwidth=0.6)
+    plane = sim.add_flux_plane('x', 0.75 * size[0], wavelengths=[0.45e-6, 0.55e-6])
+    return sim, plane
+
+
+def _snapshot_3d(sim, plane):
+    sim.engine.synchronize()
+    return {name: sim.engine.field(name) for name in ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')}, plane.flux()
+
+
+@pytest.mark.parametrize('planes', [0, 3])
+def test_batch3d_matches_solo_runs_bit_for_bit(gpu, planes):
+    from opensol_amd.batch import Batch3D
+    solo = []
+    for k, spec in enumerate(INSTANCES_3D):
+        sim, plane = _make_3d(spec, k)
+        sim.run(n_steps=spec[3])
+        first = _snapshot_3d(sim, plane)
+        sim.run(n_steps=17)
+        solo.append((first, _snapshot_3d(sim, plane)))
+        sim.close()
+    made = [_make_3d(spec, k) for k, spec in enumerate(INSTANCES_3D)]
+    batch = Batch3D([sim for sim, _ in made], graph_steps=8, planes_per_tile=planes)
+    batch.run([spec[3] for spec in INSTANCES_3D])
+    for (sim, plane), (first, _) in zip(made, solo):
+        fields, flux = _snapshot_3d(sim, plane)
+        for name, values in first[0].items():
+            assert np.array_equal(fields[name], values), name
+        assert np.array_equal(flux, first[1])
+    batch.run(17)
+    for (sim, plane), (_, second) in zip(made, solo):
+        fields, flux = _snapshot_3d(sim, plane)
+        for name, values in second[0].items():
+            assert np.array_equal(fields[name], values), name
+        assert np.array_equal(flux, second[1])
+    batch.close()
+    for sim, _ in made:
+        sim.close()

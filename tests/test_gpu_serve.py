@@ -1,6 +1,6 @@
-"""The serving loop: a stream of 2D and 3D jobs admitted as they arrive (continuous batching of the 2D jobs, resident
-state under an HBM budget) gives every job the same flux spectra, bit for bit, as running it alone; the plain FIFO form
-gives them too."""
+"""The serving loop: a stream of 2D and 3D jobs admitted as they arrive (continuous batching of the 2D jobs and of the
+3D jobs on two streams, resident state under an HBM budget) gives every job the same flux spectra, bit for bit, as
+running it alone; the plain FIFO form gives them too."""
 import numpy as np
 import pytest
 
@@ -27,7 +27,7 @@ def _job_3d(k):
     def build():
         from opensol_amd import Simulation3D
         dx = 25e-9
-        size = (0.5e-6, 0.45e-6, 0.4e-6)
+        size = (0.5e-6 + 0.1e-6 * k, 0.45e-6, 0.4e-6 + 0.2e-6 * (k % 2))
         sim = Simulation3D(*size, dx, npml=6)
         sim.add_dielectric_box((0.1e-6, 0.1e-6, 0.0), (0.3e-6, 0.3e-6, 0.2e-6), eps_r=3.0 + k)
         sim.add_dipole(0.25e-6, 0.22e-6, 0.2e-6, orientation='z', wavelength=0.6e-6)
@@ -41,7 +41,7 @@ def _jobs():
     jobs = []
     for k in range(6):
         jobs.append(Job(build=_job_2d(k), steps=60 + 23 * k, arrival=0.03 * k, name=f'2d{k}'))
-    for k in range(2):
+    for k in range(3):
         jobs.append(Job(build=_job_3d(k), steps=70 + 40 * k, arrival=0.02 + 0.05 * k, name=f'3d{k}'))
     return jobs
 
@@ -51,7 +51,7 @@ def test_served_results_equal_solo_results(gpu):
     server = FDTDServer(hbm_budget=1e9, chunk_steps=25)
     served = server.run(_jobs())
     plain = server.run_plain(_jobs())
-    assert served['jobs'] == plain['jobs'] == 8
+    assert served['jobs'] == plain['jobs'] == 9
     assert served['cell_updates'] == plain['cell_updates']
     for job in _jobs():
         sim = job.build()
