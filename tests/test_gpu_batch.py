@@ -129,3 +129,23 @@ def test_batch3d_matches_solo_runs_bit_for_bit(gpu, planes):
     batch.close()
     for sim, _ in made:
         sim.close()
+
+
+def test_tall_grid_single_and_batched_kernels_agree(gpu):
+    """A tall grid (77 y tiles, 2 z tiles) stepped by the single-instance kernel and by the batched kernel gives the
+    same bits."""
+    from opensol_amd.batch import Batch3D
+    spec = ((10, 300, 70), 4, None, 45)
+    solo, solo_plane = _make_3d(spec, 7)
+    solo.run(n_steps=spec[3])
+    expected = _snapshot_3d(solo, solo_plane)
+    solo.close()
+    sim, plane = _make_3d(spec, 7)
+    batch = Batch3D([sim])
+    batch.run(spec[3])
+    fields, flux = _snapshot_3d(sim, plane)
+    for name, values in expected[0].items():
+        assert np.array_equal(fields[name], values), name
+    assert np.array_equal(flux, expected[1])
+    batch.close()
+    sim.close()
