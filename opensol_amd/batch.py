@@ -63,9 +63,8 @@ class _Batch:
             sim.finalize()
             engine = sim.engine
             engine._ensure_capacity(engine.n_step + count)
-            d = self._descriptor(engine)
-            d.step_limit = engine.n_step + count
-            descs[k] = d
+            descs[k] = self._cached_descriptor(engine)
+            descs[k].step_limit = engine.n_step + count
         size = self._bytes(len(self.simulations))
         torch = self.torch
         device = self.simulations[0].engine.device
@@ -76,6 +75,16 @@ class _Batch:
         if self.graph_steps:
             _lib.check(self.lib.yee_set_graph_steps(self._ctx, self.graph_steps), 'yee_set_graph_steps')
         return longest
+
+    def _cached_descriptor(self, engine):
+        """The engine's batch descriptor, rebuilt only when its tables or options change (a chunked caller prepares
+        the same instances many times)."""
+        key = (type(self).__name__, engine.capacity, engine._materials_version, self._options_key())
+        cached = getattr(engine, '_batch_descriptor', None)
+        if cached is None or cached[0] != key:
+            cached = (key, self._descriptor(engine))
+            engine._batch_descriptor = cached
+        return cached[1]
 
     def launch(self):
         """Enqueue the steps prepared by prepare() on the current stream (asynchronous); returns the launches."""
@@ -130,6 +139,9 @@ class Batch2D(_Batch):
         d.rows_per_tile = resolved.rows_per_tile
         return d
 
+    def _options_key(self):
+        return (self.lane_columns, self.rows_per_tile)
+
     def _bytes(self, n):
         return self.lib.yee2d_batch_bytes(n)
 
@@ -156,6 +168,9 @@ class Batch3D(_Batch):
         if self.planes_per_tile:
             d.planes_per_tile = self.planes_per_tile
         return d
+
+    def _options_key(self):
+        return self.planes_per_tile
 
     def _bytes(self, n):
         return self.lib.yee3d_batch_bytes(n)
