@@ -11,13 +11,16 @@ import numpy as np
 
 from .constants import DTYPE, EPS0, FACES, FACES_3D
 
-CPML, PERIODIC = 'cpml', 'periodic'
-_KINDS = (CPML, PERIODIC)
+CPML, PERIODIC, PEC, PMC = 'cpml', 'periodic', 'pec', 'pmc'
+_KINDS = (CPML, PERIODIC, PEC, PMC)
 
 
 class BoundaryConfig:
-    """Per-face types ('cpml' or 'periodic') and CPML settings (npml 15, order 3, sigma_factor 1, kappa_max 15,
-    alpha_max 0 S/m). Walls (pec / pmc / mirrors) and Bloch phases are not part of this FDTD path."""
+    """Per-face types and CPML settings (npml 15, order 3, sigma_factor 1, kappa_max 15, alpha_max 0 S/m).
+
+    Face types: 'cpml', 'periodic', 'pec' (the outermost window cell layer is a perfect electric conductor: 3D and 2D
+    E_z) and 'pmc' (3D: the natural grid edge, H tangential 0 half a cell beyond the outermost centres,
+    physics/em3d/engine3d.py:12-13). Mirrors and Bloch phases are not part of this FDTD path."""
 
     def __init__(self, x_lo='cpml', x_hi='cpml', y_lo='cpml', y_hi='cpml', z_lo=None, z_hi=None):
         faces = {'x_lo': x_lo, 'x_hi': x_hi, 'y_lo': y_lo, 'y_hi': y_hi}
@@ -27,7 +30,8 @@ class BoundaryConfig:
         for face, kind in faces.items():
             kind = getattr(kind, 'value', kind)
             if kind not in _KINDS:
-                raise ValueError(f"face {face}: type {kind!r} is not supported by this FDTD path (cpml, periodic)")
+                raise ValueError(f"face {face}: type {kind!r} is not supported by this FDTD path "
+                                 f"(cpml, periodic, pec, pmc)")
             self.faces[face] = kind
         self.cpml_npml = 15
         self.cpml_order = 3
@@ -82,6 +86,10 @@ class BoundaryConfig:
     @property
     def periodic_y(self):
         return self.periodic('y')
+
+    def pec_faces(self):
+        """Faces of type 'pec'."""
+        return [face for face, kind in self.faces.items() if kind == PEC]
 
     @property
     def npml_sides(self):

@@ -73,6 +73,8 @@ __device__ __forceinline__ float2 mat(const yee3d_desc& d, int id)
 // float32 mean eps_r and sigma_e (shares 1/2; kernels3d.py:61-85). With k = sigma dt / (2 eps):
 // ca = (1 - k) / (1 + k) = (eps - sigma dt/2) / (eps + sigma dt/2) and cb = (dt / eps) / (1 + k) = dt / (eps + sigma
 // dt/2), formed with one reciprocal (within 2 ulp of the reference's float32 divisions).
+// PEC ids carry eps_r = -1 and (ca, cb) = (0, 0): a face between two PEC cells is pinned by the id's own entry, and a
+// face between a PEC cell and another cell takes the other cell's values (kernels3d.py:61-85), i.e. its own entry.
 template <bool LDS>
 __device__ __forceinline__ float2 face(const yee3d_desc& d, int a, int b)
 {
@@ -81,6 +83,10 @@ __device__ __forceinline__ float2 face(const yee3d_desc& d, int a, int b)
         return make_float2(c.z, c.w);
     }
     const float2 ma = mat<LDS>(d, a), mb = mat<LDS>(d, b);
+    if (ma.x < 0.f || mb.x < 0.f) {
+        const float4 c = coef<LDS>(d, ma.x < 0.f ? b : a);
+        return make_float2(c.z, c.w);
+    }
     const float eps = (0.5f * ma.x + 0.5f * mb.x) * EPS0F;
     const float loss = (0.5f * ma.y + 0.5f * mb.y) * (0.5f * d.dt);
     const float inv = __builtin_amdgcn_rcpf(eps + loss);

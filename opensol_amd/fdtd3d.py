@@ -26,8 +26,8 @@ COMPONENTS = ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')
 
 
 def resolve_boundaries_3d(boundaries, npml):
-    """None: CPML on every face; a dict of face -> 'cpml' / 'periodic' (missing faces: cpml); or a BoundaryConfig
-    with z faces (physics/em3d/boundaries3d.py:17-60)."""
+    """None: CPML on every face; a dict of face -> 'cpml' / 'periodic' / 'pec' / 'pmc' (missing faces: cpml); or a
+    BoundaryConfig with z faces (physics/em3d/boundaries3d.py:17-60)."""
     if boundaries is None:
         return BoundaryConfig.all_pml(npml, dimensions=3)
     if isinstance(boundaries, BoundaryConfig):
@@ -70,6 +70,7 @@ class Model3D:
             self.inv_e.append(inv_e)
             self.inv_h.append(inv_h)
         self.sample_interval = 1
+        self.pec_boxes = []   # index ranges of PEC boxes, painted over the dielectrics at finalize
 
     def axis_rows(self, axis):
         """(n, 8) float32 records of one axis (the yee3d_axis_row layout): inv_h, inv_e, psi row of the hi-face
@@ -98,9 +99,54 @@ class Model3D:
         ranges = self.domain.box_ranges(lo, hi, clamp_to_physical=True)
         self.materials.paint_box(ranges, self.materials.material_id(eps_r, eps_r, eps_r, sigma_e, 1.0, 0.0))
 
+    def add_pec_box(self, lo, hi):
+        """PEC cells on the box raster of add_dielectric_box (physics/em3d/physics3d.py:351-353). PEC regions are
+        applied over every dielectric, whatever the order of the calls (engine3d.py:339-362)."""
+        self.pec_boxes.append(self.domain.box_ranges(lo, hi, clamp_to_physical=True))
+        self.materials.version += 1   # a built engine re-applies the regions on its next step
+
+    def pec_wall_layers(self):
+        """Index tuples of the outermost window cell layer of every 'pec' face (physics/em3d/boundaries3d.py:149-158)."""
+        window = self.domain.physical_window()
+        layers = []
+        for face in self.bc.pec_faces():
+            axis = AXES.index(face[0])
+            index = window[axis].start if face.endswith('lo') else window[axis].stop - 1
+            key = [slice(None)] * 3
+            key[axis] = slice(index, index + 1)
+            layers.append(tuple(key))
+        return layers
+
+    def pec_wall_cell(self, cell, axes):
+        """Whether a cell lies in the pec wall layer of a face normal to one of `axes` (engine3d.py:379-382)."""
+        for face in self.bc.pec_faces():
+            axis = AXES.index(face[0])
+            window = self.domain.physical_window()[axis]
+            index = window.start if face.endswith('lo') else window.stop - 1
+            if axis in axes and cell[axis] == index:
+                return True
+        return False
+
+    def apply_pec(self):
+        """Paint the PEC boxes, then the pec wall layers, over the dielectrics (engine3d.py:358-362); the material
+        version changes only when a cell changes."""
+        if not self.pec_boxes and not self.bc.pec_faces():
+            return
+        pec = self.materials.pec_id()
+        keys = [tuple(slice(lo, hi) for lo, hi in ranges) for ranges in self.pec_boxes] + self.pec_wall_layers()
+        ids = self.materials.ids
+        if any(ids[key].size and np.any(ids[key] != pec) for key in keys):
+            for key in keys:
+                ids[key] = pec
+            self.materials.version += 1
+
     def check_courant(self):
-        """ValueError when eps_r mu_r < (C0 dt)^2 sum 1/d^2 for a material in use (physics/em3d/engine3d.py:824-852)."""
-        values = self.materials.values()[self.materials.used()]
+        """ValueError when eps_r mu_r < (C0 dt)^2 sum 1/d^2 for a material in use (physics/em3d/engine3d.py:824-852);
+        PEC cells do not step E and set no limit."""
+        used = self.materials.used()
+        values = self.materials.values()[used[~self.materials.pec_flags()[used]]]
+        if values.size == 0:
+            return
         d = float(self.grid.dx)
         required = (C0 * float(self.dt)) ** 2 * 3.0 / d ** 2
         worst = float((values[:, 0] * values[:, 4]).min() / required)
@@ -140,6 +186,9 @@ class Model3D:
         axis = AXES.index(orientation)
         cell = list(self.domain.pos_to_cell(x, y, z))
         cell[axis] = self.nearest_face(axis, (x, y, z)[axis])
+        if self.pec_wall_cell(cell, tuple(other for other in range(3) if other != axis)):
+            raise ValueError(f'dipole at ({x:.4g}, {y:.4g}, {z:.4g}) m lies in the outermost cell layer of a pec face '
+                             f'parallel to it, where its tangential E is pinned to 0; move it inside')
         source = PointDipole('e' + orientation, cell, temporal, dipole_moment, self.face_dual_volume(axis, cell))
         self.sources.append(source)
         return source
@@ -173,6 +222,8 @@ class Model3D:
         dt = float(self.dt)
         da, db = semi_implicit(v[:, 4], v[:, 5], dt, MU0)
         ca, cb = semi_implicit(v[:, 0], v[:, 3], dt, EPS0)
+        pec = self.materials.pec_flags()
+        ca[pec] = cb[pec] = 0.0   # a face between two PEC cells is pinned; eps_r = -1 marks the id for the kernel
         zeros = np.zeros_like(da)
         return np.stack([da, db, ca, cb, v[:, 0], v[:, 3], zeros, zeros], axis=1).astype(DTYPE)
 
@@ -184,6 +235,12 @@ class Model3D:
         partner = list(cell)
         partner[axis] = (cell[axis] + 1) % n if self.periodic[axis] else min(cell[axis] + 1, n - 1)
         a, b = values[int(self.materials.ids[cell])], values[int(self.materials.ids[tuple(partner)])]
+        if a[0] < 0 and b[0] < 0:   # PEC rule (kernels3d.py:61-85): two PEC cells pin the face
+            return 0.0
+        if a[0] < 0:   # one PEC cell: the face takes the other cell's values
+            a = b
+        elif b[0] < 0:
+            b = a
         _, cb = semi_implicit(0.5 * a[0] + 0.5 * b[0], 0.5 * a[3] + 0.5 * b[3], float(self.dt), EPS0)
         return float(cb)
 
@@ -193,6 +250,7 @@ class Model3D:
             raise ValueError(f'at most {_lib.MAX_SOURCES_3D} dipoles per 3D model')
         if len(self.monitors) > _lib.MAX_PLANES_3D:
             raise ValueError(f'at most {_lib.MAX_PLANES_3D} flux planes per 3D model')
+        self.apply_pec()
         self.check_courant()
         for source in self.sources:
             source.calibrate(self.face_cb(source.axis, source.cell))

@@ -11,6 +11,10 @@ import numpy as np
 from .constants import DTYPE, EPS0, MU0
 
 MAX_MATERIALS = 65535
+# The perfect electric conductor: a material entry with eps = -1 on every axis, mu_r = 1 and no losses (the reference's
+# last table id, physics/em3d/engine3d.py:339-362). Its E coefficients are pinned to 0 in every table; a face between
+# a PEC cell and another cell follows the rule of each polarisation (face coefficients in the kernels).
+PEC_VALUES = (-1.0, -1.0, -1.0, 0.0, 1.0, 0.0)
 
 
 def semi_implicit(eps_rel, sigma, dt, vacuum):
@@ -49,6 +53,22 @@ class MaterialMap:
             self._lookup[key] = index
         return index
 
+    def pec_id(self):
+        """Id of the PEC entry (created on first use)."""
+        key = tuple(DTYPE(v) for v in PEC_VALUES)
+        index = self._lookup.get(key)
+        if index is None:
+            if len(self.materials) >= MAX_MATERIALS:
+                raise ValueError(f'more than {MAX_MATERIALS} distinct materials')
+            index = len(self.materials)
+            self.materials.append(key)
+            self._lookup[key] = index
+        return index
+
+    def pec_flags(self):
+        """bool (n_materials,): the PEC entries."""
+        return self.values()[:, 0] < 0.0
+
     def paint_box(self, ranges, material):
         """Write `material` into the half-open index box `ranges` (one (lo, hi) per axis)."""
         if any(hi <= lo for lo, hi in ranges):
@@ -80,6 +100,8 @@ class MaterialMap:
         dt = float(dt)
         ca, cb = semi_implicit(v[:, 0], v[:, 3], dt, EPS0)
         da, db = semi_implicit(v[:, 4], v[:, 5], dt, MU0)
+        pec = self.pec_flags()
+        ca[pec] = cb[pec] = 0.0   # E_z pinned to 0 in PEC cells (physics/em/solvers/fdtd_ez_2d.py:223-233)
         return np.stack([ca, cb, da, db], axis=1).astype(DTYPE)
 
     def tables_hz(self, dt):
@@ -90,6 +112,9 @@ class MaterialMap:
         da, db = semi_implicit(v[:, 4], v[:, 5], dt, MU0)
         ca_x, cb_x = semi_implicit(v[:, 0], v[:, 3], dt, EPS0)
         ca_y, cb_y = semi_implicit(v[:, 1], v[:, 3], dt, EPS0)
+        pec = self.pec_flags()
+        for column in (ca_x, cb_x, ca_y, cb_y):
+            column[pec] = 0.0   # every E face touching a PEC cell is pinned (fdtd_hz_2d.py:475-495)
         table = np.stack([da, db, v[:, 0], v[:, 1]], axis=1).astype(DTYPE)
         zeros = np.zeros_like(da)
         table2 = np.stack([ca_x, cb_x, ca_y, cb_y, v[:, 3], zeros, zeros, zeros], axis=1).astype(DTYPE)

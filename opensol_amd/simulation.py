@@ -213,6 +213,11 @@ class Simulation3D:
     def add_dielectric_box(self, lo, hi, eps_r, sigma_e=0.0):
         self.model.add_dielectric_box(lo, hi, eps_r, sigma_e)
 
+    def add_pec_box(self, lo, hi):
+        """PEC cells on the box raster of add_dielectric_box, applied over every dielectric
+        (physics/em3d/physics3d.py:351-353)."""
+        self.model.add_pec_box(lo, hi)
+
     def add_dipole(self, x, y, z, orientation='z', dipole_moment=1e-12, wavelength=None, frequency=None,
                    bandwidth=0.5, waveform='gaussian'):
         self._require_unbuilt('add_dipole')

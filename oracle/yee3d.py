@@ -46,14 +46,16 @@ def courant_time_step_3d(dx, dy, dz, courant=None):
 
 
 class Boundaries3D:
-    """Face types ('cpml' / 'periodic') and CPML settings (physics/em3d/boundaries3d.py:17-60)."""
+    """Face types ('cpml' / 'periodic' / 'pec' / 'pmc') and CPML settings (physics/em3d/boundaries3d.py:17-60). A
+    'pmc' face is the bare grid edge (engine3d.py:12-13); a 'pec' face makes the outermost window cell layer PEC
+    (engine3d.py:13, boundaries3d.py:149-158)."""
 
     def __init__(self, faces=None, npml=12, order=3, sigma_factor=1.0, kappa_max=15.0, alpha_max=0.0):
         faces = dict(faces or {})
         self.faces = {face: faces.get(face, 'cpml') for face in FACES_3D}
         for face, kind in self.faces.items():
-            if kind not in ('cpml', 'periodic'):
-                raise ValueError(f'oracle supports cpml and periodic faces, got {face}={kind}')
+            if kind not in ('cpml', 'periodic', 'pec', 'pmc'):
+                raise ValueError(f'oracle supports cpml, periodic, pec and pmc faces, got {face}={kind}')
         for axis in AXES:
             if (self.faces[f'{axis}_lo'] == 'periodic') != (self.faces[f'{axis}_hi'] == 'periodic'):
                 raise ValueError(f'{axis}: periodic faces must be paired')
@@ -257,6 +259,7 @@ class Yee3D:
         self.materials = {'eps_r': np.ones(self.shape, F32), 'sigma_e': np.zeros(self.shape, F32),
                           'mu_r': np.ones(self.shape, F32)}
         self.sources, self.monitors = [], []
+        self.pec_boxes = []
         self.n_step, self.time = 0, 0.0
         self._finalized = False
         self._build_spacings_and_cpml()
@@ -312,6 +315,31 @@ class Yee3D:
         self.materials['sigma_e'][mask] = F32(sigma_e)
         self._finalized = False
 
+    def add_pec_box(self, lo, hi):
+        """PEC cells on the box raster, kept over every dielectric (physics3d.py:351-353, engine3d.py:358-360)."""
+        self.pec_boxes.append(self.box_mask(lo, hi))
+        self._finalized = False
+
+    def pec_wall_layers(self):
+        """Outermost window cell layer of every 'pec' face (boundaries3d.py:149-158)."""
+        layers = []
+        for axis, name in enumerate(AXES):
+            for side in ('lo', 'hi'):
+                if self.bc.faces[f'{name}_{side}'] == 'pec':
+                    window = self.windows[axis]
+                    index = window.start if side == 'lo' else window.stop - 1
+                    layers.append(along(axis, slice(index, index + 1)))
+        return layers
+
+    def pec_mask(self):
+        """bool per cell: PEC boxes and pec wall layers."""
+        mask = np.zeros(self.shape, dtype=bool)
+        for box in self.pec_boxes:
+            mask |= box
+        for layer in self.pec_wall_layers():
+            mask[layer] = True
+        return mask
+
     def validate(self, position, label):
         """core/domain.py:125-133."""
         tol = 1e-9 * max(self.sizes)
@@ -355,6 +383,10 @@ class Yee3D:
         axis = AXES.index(orientation)
         cell = [self.cell(a, v) for a, v in enumerate((x, y, z))]
         cell[axis] = self.nearest_face(axis, (x, y, z)[axis])
+        for layer in self.pec_wall_layers():   # engine3d.py:379-382, physics3d.py:415-417
+            wall_axis = next(a for a, key in enumerate(layer) if key != slice(None))
+            if wall_axis != axis and cell[wall_axis] == layer[wall_axis].start:
+                raise ValueError('dipole lies in the outermost cell layer of a pec face parallel to it')
         source = DipoleSource('e' + orientation, cell, temporal, dipole_moment, self.face_dual_volume(axis, cell))
         self.sources.append(source)
         self._finalized = False
@@ -380,14 +412,25 @@ class Yee3D:
         """(ca, cb) of the E component on the hi face along axis: mean eps / sigma of the cell and its hi partner
         (kernels3d.py:61-85; uniform widths: shares 0.5 / 0.5)."""
         eps, sig = (self.materials[name].astype(np.float64) for name in ('eps_r', 'sigma_e'))
+        pec = self.pec_mask()
         n = self.shape[axis]
         partner = np.arange(1, n + 1)
         partner[-1] = 0 if self.periodic[axis] else n - 1
-        eps_face = (0.5 * eps + 0.5 * np.take(eps, partner, axis=axis)) * EPS0
-        sig_face = 0.5 * sig + 0.5 * np.take(sig, partner, axis=axis)
+        eps_b, sig_b, pec_b = (np.take(a, partner, axis=axis) for a in (eps, sig, pec))
+        # PEC rule (kernels3d.py:61-85): a face touching one PEC cell takes the other cell's values on both sides;
+        # a face between two PEC cells is pinned (ca = cb = 0)
+        eps_a = np.where(pec, eps_b, eps)
+        sig_a = np.where(pec, sig_b, sig)
+        eps_b = np.where(pec_b & ~pec, eps, eps_b)
+        sig_b = np.where(pec_b & ~pec, sig, sig_b)
+        eps_face = (0.5 * eps_a + 0.5 * eps_b) * EPS0
+        sig_face = 0.5 * sig_a + 0.5 * sig_b
+        pinned = pec & pec_b
+        eps_face = np.where(pinned, 1.0, eps_face)   # any finite value: the face is zeroed below
         loss = sig_face * float(self.dt) / (2.0 * eps_face)
-        return ((1.0 - loss) / (1.0 + loss)).astype(self.dtype), ((float(self.dt) / eps_face) / (1.0 + loss)).astype(
-            self.dtype)
+        ca = np.where(pinned, 0.0, (1.0 - loss) / (1.0 + loss))
+        cb = np.where(pinned, 0.0, (float(self.dt) / eps_face) / (1.0 + loss))
+        return ca.astype(self.dtype), cb.astype(self.dtype)
 
     def finalize(self):
         """Coefficients, dipole calibration and the DFT sample interval (physics3d.py:192-217)."""

@@ -1,4 +1,4 @@
-"""Write tests/golden/dipole3d_{cpml,periodic_y}.npz: golden vectors of a small 3D FDTD case from the reference's own
+"""Write tests/golden/dipole3d_{cpml,periodic_y,pec}.npz: golden vectors of a small 3D FDTD case from the reference's own
 numpy path (EMPhysics3D, precision 'single', CPU backend: the float32 numpy twins of the 3D kernels). They pin the 3D
 stepping, CPML, dipole calibration and plane-flux composition that no closed form pins end to end.
 
@@ -8,8 +8,10 @@ Gaussian point dipole (lambda0 0.6 um, bandwidth 0.5, moment 1e-12 A m) near the
 and z) at 3 wavelengths; 300 steps. All six fields are recorded after the last step, and their three planes through the
 dipole cell after step 120.
 
+Case 'pec' adds a PEC z_lo face, a bare (pmc) z_hi face and a PEC box over the dielectrics.
+
 Run in this container only (the reference is not present on the GPU box):
-    python tests/golden/make_dipole3d_golden.py
+    python tests/golden/make_dipole3d_golden.py [case ...]
 The fixture holds inputs (scalars and the eps_r array of the case) and outputs (field arrays, flux spectra); no source
 text of the reference.
 """
@@ -35,6 +37,10 @@ CASES = {
                  planes=(('x', 0.55e-6), ('z', 0.1e-6))),
     'periodic_y': dict(faces={'y_lo': 'periodic', 'y_hi': 'periodic'}, dipole=(0.33e-6, 0.21e-6, 0.26e-6),
                        orientation='x', planes=(('x', 0.15e-6), ('z', 0.45e-6))),
+    # a PEC ground face (z_lo), the bare grid edge (z_hi pmc) and a PEC box over the dielectrics
+    'pec': dict(faces={'z_lo': 'pec', 'z_hi': 'pmc'}, dipole=(0.36e-6, 0.29e-6, 0.27e-6), orientation='z',
+                planes=(('x', 0.2e-6), ('z', 0.45e-6)),
+                pec_box=((0.45e-6, 0.10e-6, 0.30e-6), (0.60e-6, 0.35e-6, 0.45e-6))),
 }
 FIELDS = ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')
 
@@ -66,12 +72,19 @@ def run_case(spec):
         domain.materials.add_property(name, default)
     domain.materials.eps_r[...] = eps
     em = EMPhysics3D(domain, precision='single')
+    pec = np.zeros(domain.grid.shape, dtype=bool)
+    if 'pec_box' in spec:
+        lo, hi = spec['pec_box']
+        axis_masks = [(c >= np.float32(a)) & (c < np.float32(b)) for c, a, b in zip(centres, lo, hi)]
+        pec = axis_masks[0][:, None, None] & axis_masks[1][None, :, None] & axis_masks[2][None, None, :]
+        em.add_pec(tuple(slice(0, n) for n in domain.grid.shape), pec)
     source = em.add_dipole(*spec['dipole'], orientation=spec['orientation'], dipole_moment=MOMENT,
                            wavelength=LAMBDA0, bandwidth=BANDWIDTH)
     planes = [em.add_flux_plane(normal, position, wavelengths=list(WAVELENGTHS)) for normal, position in spec['planes']]
     em.finalize()
     engine = em.engine
-    out = {'shape': np.array(engine.shape), 'dt': np.float64(engine.dt), 'eps_r': eps,
+    out = {'shape': np.array(engine.shape), 'dt': np.float64(engine.dt), 'eps_r': eps, 'pec_box_mask': pec,
+           'pec_cells': np.asarray(np.asarray(engine.material_id) == engine.pec_id),
            'dipole_cell': np.array(source.cell), 'dipole_weight': np.asarray(source.weight, dtype=np.float64),
            'sample_interval': np.int64(planes[0].sample_interval),
            'plane_edges': np.array([plane.edge for plane in planes])}
@@ -96,8 +109,10 @@ def run_case(spec):
     return out
 
 
-def main():
+def main(names=None):
     for name, spec in CASES.items():
+        if names and name not in names:
+            continue
         out = run_case(spec)
         path = os.path.join(HERE, f'dipole3d_{name}.npz')
         np.savez_compressed(path, **out)
@@ -105,4 +120,4 @@ def main():
 
 
 if __name__ == '__main__':
-    main()
+    main(sys.argv[1:])

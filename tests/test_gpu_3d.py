@@ -1,5 +1,5 @@
 """3D HIP Yee step (libyee.so through the C ABI) against the FP64 3D oracle: the golden dipole cases end to end (fields
-and plane-flux spectra), random initial fields over tile shapes that put tile edges, ragged tiles and wraps
+and plane-flux spectra; one with a PEC face, a pmc face and a PEC box), random initial fields over tile shapes that put tile edges, ragged tiles and wraps
 everywhere, exact invariance along a periodic axis (halo copies equal their owners bit for bit), graph replay and
 launch-geometry invariance (bit-identical)."""
 import numpy as np
@@ -19,6 +19,9 @@ GOLDEN_CASES = {
                  planes=(('x', 0.55e-6), ('z', 0.1e-6))),
     'periodic_y': dict(faces={'y_lo': 'periodic', 'y_hi': 'periodic'}, dipole=(0.33e-6, 0.21e-6, 0.26e-6),
                        orientation='x', planes=(('x', 0.15e-6), ('z', 0.45e-6))),
+    'pec': dict(faces={'z_lo': 'pec', 'z_hi': 'pmc'}, dipole=(0.36e-6, 0.29e-6, 0.27e-6), orientation='z',
+                planes=(('x', 0.2e-6), ('z', 0.45e-6)),
+                pec_box=((0.45e-6, 0.10e-6, 0.30e-6), (0.60e-6, 0.35e-6, 0.45e-6))),
 }
 
 
@@ -26,6 +29,8 @@ def _build(target, spec, **options):
     sim = target(*GOLDEN_WINDOW, 25e-9, npml=8, boundaries=spec['faces'], **options)
     for lo, hi, eps in GOLDEN_BOXES:
         sim.add_dielectric_box(lo, hi, eps)
+    if 'pec_box' in spec:
+        sim.add_pec_box(*spec['pec_box'])
     sim.add_dipole(*spec['dipole'], orientation=spec['orientation'], dipole_moment=1e-12, wavelength=0.6e-6,
                    bandwidth=0.5)
     planes = [sim.add_flux_plane(n, p, wavelengths=[0.5e-6, 0.6e-6, 0.7e-6]) for n, p in spec['planes']]

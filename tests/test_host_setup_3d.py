@@ -136,3 +136,57 @@ def test_model3d_validation():
         model.add_dipole(0.1e-6, 0.1e-6, 0.1e-6, orientation='w', wavelength=1e-6)
     with pytest.raises(ValueError):
         model.add_flux_plane('q', 0.1e-6, wavelengths=[1e-6])
+
+
+def test_pec_faces_and_boxes_paint_over_dielectrics():
+    """A 'pec' face makes the outermost window layer PEC; a PEC box keeps its cells whatever is painted after it
+    (physics/em3d/engine3d.py:339-362, boundaries3d.py:149-158); 'pmc' is the bare grid edge (no CPML, no layer)."""
+    from opensol_amd.fdtd3d import Model3D
+    dx = 25e-9
+    m = Model3D(20 * dx, 16 * dx, 12 * dx, dx, npml=4, boundaries={'z_lo': 'pec', 'z_hi': 'pmc', 'x_hi': 'pec'})
+    assert m.shape == (24 + 0, 24, 12)   # x: cpml lo only; z: no CPML
+    m.add_pec_box((5 * dx, 5 * dx, 5 * dx), (9 * dx, 9 * dx, 9 * dx))
+    m.add_dielectric_box((0, 0, 0), (20 * dx, 16 * dx, 12 * dx), eps_r=3.0)   # painted after the PEC box
+    m.finalize()
+    ids = m.materials.ids
+    pec = m.materials.pec_id()
+    assert np.all(ids[:, :, 0] == pec) and np.all(ids[-1] == pec)       # z_lo and x_hi wall layers
+    assert not np.any(ids[:-1, :, -1] == pec)                            # pmc: no layer (off the x_hi wall)
+    box = ids[4 + 5:4 + 9, 4 + 5:4 + 9, 5:9]
+    assert np.all(box == pec)
+    assert int((ids == pec).sum()) == 24 * 24 + 24 * 12 - 24 + 4 * 4 * 4
+    row = m.table()[pec]
+    assert row[2] == 0.0 and row[3] == 0.0 and row[4] == -1.0           # ca = cb = 0, the kernel's PEC mark
+
+
+def test_pec_face_rule_of_the_dipole_weight():
+    """The dipole cb follows the PEC face rule: a face between two PEC cells is pinned, a face touching one PEC cell
+    takes the other cell's values (kernels3d.py:61-85)."""
+    from opensol_amd.fdtd3d import Model3D
+    from opensol_amd.materials import semi_implicit
+    from opensol_amd.constants import EPS0
+    dx = 25e-9
+    m = Model3D(12 * dx, 12 * dx, 12 * dx, dx, npml=3)
+    m.add_dielectric_box((0, 0, 0), (12 * dx, 12 * dx, 12 * dx), eps_r=2.0)
+    m.add_pec_box((6 * dx, 0, 0), (12 * dx, 12 * dx, 12 * dx))   # window cells x >= 6 (grid x >= 9)
+    m.apply_pec()
+    _, cb_dielectric = semi_implicit(2.0, 0.0, float(m.dt), EPS0)
+    assert m.face_cb(0, (8, 6, 6)) == pytest.approx(float(cb_dielectric), rel=1e-15)   # dielectric | PEC
+    assert m.face_cb(0, (10, 6, 6)) == 0.0                                             # PEC | PEC
+    assert m.face_cb(1, (10, 6, 6)) == 0.0
+
+
+def test_dipole_in_a_pec_wall_layer_is_rejected():
+    from opensol_amd.fdtd3d import Model3D
+    dx = 25e-9
+    m = Model3D(12 * dx, 12 * dx, 12 * dx, dx, npml=3, boundaries={'z_lo': 'pec'})
+    with pytest.raises(ValueError, match='pec face'):
+        m.add_dipole(6 * dx, 6 * dx, 0.2 * dx, orientation='x', wavelength=0.6e-6)
+    m.add_dipole(6 * dx, 6 * dx, 0.2 * dx, orientation='z', wavelength=0.6e-6)   # normal to the wall: allowed
+
+
+def test_courant_check_skips_pec_cells():
+    from opensol_amd.fdtd3d import Model3D
+    dx = 25e-9
+    m = Model3D(8 * dx, 8 * dx, 8 * dx, dx, npml=2, boundaries={'x_lo': 'pec', 'x_hi': 'pec'})
+    m.finalize()   # the eps = -1 entry sets no limit

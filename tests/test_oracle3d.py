@@ -1,5 +1,5 @@
 """The 3D oracle against what pins it: golden vectors of the reference's own 3D numpy path (float32 twins, so the
-tolerances sit at float32 round-off), the exact discrete dispersion of periodic-box eigenmodes of each E component,
+tolerances sit at float32 round-off; one case with a PEC face, a bare pmc face and a PEC box), the exact discrete dispersion of periodic-box eigenmodes of each E component,
 and the reduction of a 2D-invariant 3D run to the 2D oracle (which the 2D golden vectors pin bit for bit) along each
 axis, CPML included."""
 import os
@@ -11,7 +11,7 @@ from oracle.yee2d import EPS0, MU0, Yee2D
 from oracle.yee3d import Yee3D
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
-GOLDEN_CASES = ('cpml', 'periodic_y')
+GOLDEN_CASES = ('cpml', 'periodic_y', 'pec')
 
 
 def _spec():
@@ -27,6 +27,8 @@ def _golden_model(name, dtype=np.float64, phase_dtype=np.complex128):
     sim = Yee3D(*G.WINDOW, G.DX, npml=G.NPML, boundaries=faces, dtype=dtype, phase_dtype=phase_dtype)
     for lo, hi, eps in G.BOXES:
         sim.add_dielectric_box(lo, hi, eps)
+    if 'pec_box' in spec:
+        sim.add_pec_box(*spec['pec_box'])
     source = sim.add_dipole(*spec['dipole'], orientation=spec['orientation'], dipole_moment=G.MOMENT,
                             wavelength=G.LAMBDA0, bandwidth=G.BANDWIDTH)
     planes = [sim.add_flux_plane(normal, position, wavelengths=list(G.WAVELENGTHS))
@@ -47,6 +49,8 @@ def test_setup_matches_golden(name):
     assert sim.shape == tuple(golden['shape'])
     assert float(sim.dt) == float(golden['dt'])
     np.testing.assert_array_equal(sim.materials['eps_r'], golden['eps_r'])
+    if 'pec_cells' in golden:   # PEC box raster and the pec face's wall layer
+        np.testing.assert_array_equal(sim.pec_mask(), golden['pec_cells'])
     assert source.cell == tuple(golden['dipole_cell'])
     # the reference rounds cb and moment / volume to float32 before the product
     assert abs(source.weight / float(golden['dipole_weight'][0]) - 1.0) < 3e-7
