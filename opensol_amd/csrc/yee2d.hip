@@ -192,7 +192,8 @@ __device__ __forceinline__ float hz_ex(float ca, float cb, float ex, float dh, f
 __device__ __forceinline__ float hz_ey(float ca, float cb, float ey, float dh, float idx) { return ca * ey - cb * dh * idx; }
 
 // E face coefficients between cells of ids a and b (comp 0: E_x face, 1: E_y face). Same id: the table's FP64
-// coefficients; mixed: mean eps / sigma (shares 1/2, fdtd_hz_2d_kernels.py:190-278) formed here in float32.
+// coefficients (zero for the PEC id); mixed: mean eps / sigma (shares 1/2, fdtd_hz_2d_kernels.py:190-278) formed here
+// in float32, or zero when either cell is PEC.
 template <bool LDS>
 __device__ __forceinline__ void face_coefficients(const yee2d_desc& d, int ida, int idb, int comp, float* ca,
                                                   float* cb)
@@ -206,6 +207,11 @@ __device__ __forceinline__ void face_coefficients(const yee2d_desc& d, int ida, 
     const float4 a = tab_a<LDS>(d, ida), b = tab_a<LDS>(d, idb);
     const float ea = comp == 0 ? a.z : a.w;
     const float eb = comp == 0 ? b.z : b.w;
+    if (ea < 0.f || eb < 0.f) {   // a PEC id (eps = -1): every E face touching a PEC cell is pinned
+        *ca = 0.f;                // (physics/em/solvers/fdtd_hz_2d.py:475-495, yee_grid.py:145-150)
+        *cb = 0.f;
+        return;
+    }
     const float eps_face = 0.5f * ea + 0.5f * eb;
     const float sig_face = 0.5f * tab_sig<LDS>(d, ida) + 0.5f * tab_sig<LDS>(d, idb);
     const float k = sig_face * (0.5f * d.dt_over_eps0) / eps_face;

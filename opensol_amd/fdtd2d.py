@@ -36,6 +36,10 @@ class Model2D:
         self.bc = resolve_boundaries(boundaries, npml)
         if self.bc.dimensions != 2:
             raise ValueError('a 2D model needs a 2D BoundaryConfig')
+        walls = [face for face, kind in self.bc.faces.items() if kind in ('pec', 'pmc')]
+        if walls and (polarization != 'ez' or any(self.bc.faces[face] == 'pmc' for face in walls)):
+            raise ValueError(f"2D faces {walls}: this FDTD path has 'pec' faces for the E_z polarisation only "
+                             f"(the E_z row of the wall is pinned to 0, physics/em/boundaries.py:57-61)")
         self.domain = Domain((size_x, size_y), dx, self.bc.npml_sides, courant)
         self.grid = self.domain.grid
         self.nx, self.ny = self.grid.shape
@@ -51,6 +55,7 @@ class Model2D:
         # the curl spacing outside the CPML layers (uniform grid: every bulk entry of all four arrays)
         self.inv_bulk = tuple(float(a[b]) for a, b in ((self.inv_dx_e, self.nx // 2), (self.inv_dy_e, self.ny // 2)))
         self.sample_interval = 1
+        self.pec_regions = []   # bool (nx, ny) masks, applied over the dielectrics at finalize
 
     @property
     def shape(self):
@@ -82,10 +87,54 @@ class Model2D:
     def _material(self, eps_r, sigma_e):
         return self.materials.material_id(eps_r, eps_r, eps_r, sigma_e, 1.0, 0.0)
 
+    def add_pec_region(self, mask):
+        """Perfect electric conductor on the bool (nx, ny) cell mask (physics/em/__init__.py:372-376): E_z pinned to 0
+        in its cells (E_z), every E face touching it pinned to 0 (H_z); kept over every dielectric."""
+        mask = np.asarray(mask)
+        if mask.dtype != np.bool_ or mask.shape != self.shape:
+            raise ValueError(f'PEC mask must be a bool array of shape {self.shape}, got {mask.dtype} {mask.shape}')
+        self.pec_regions.append(mask.copy())
+        self.materials.version += 1
+
+    def add_pec_rectangle(self, x0, y0, x1, y1):
+        """PEC on the cells whose float32 centre lies in [x_min, x_max) x [y_min, y_max) (the raster of
+        add_dielectric_rectangle)."""
+        mask = np.zeros(self.shape, dtype=bool)
+        mask[tuple(slice(lo, hi) for lo, hi in self.domain.box_ranges((x0, y0), (x1, y1)))] = True
+        self.add_pec_region(mask)
+
+    def pec_mask(self):
+        """bool (nx, ny): PEC regions and the outermost window row of every 'pec' face."""
+        mask = np.zeros(self.shape, dtype=bool)
+        for region in self.pec_regions:
+            mask |= region
+        window = self.domain.physical_window()
+        for face in self.bc.pec_faces():
+            axis = 'xy'.index(face[0])
+            index = window[axis].start if face.endswith('lo') else window[axis].stop - 1
+            if axis == 0:
+                mask[index, :] = True
+            else:
+                mask[:, index] = True
+        return mask
+
+    def apply_pec(self):
+        """Paint the PEC cells over the dielectrics; the material version changes only when a cell changes."""
+        mask = self.pec_mask()
+        if not mask.any():
+            return
+        pec = self.materials.pec_id()
+        if np.any(self.materials.ids[mask] != pec):
+            self.materials.ids[mask] = pec
+            self.materials.version += 1
+
     def check_courant(self):
         """ValueError when eps * mu_r < (C0 dt)^2 (1/dx^2 + 1/dy^2) for a material in use (physics/em/__init__.py:
         242-281; eps = eps_r for E_z, min(eps_xx, eps_yy) for H_z)."""
-        values = self.materials.values()[self.materials.used()]
+        used = self.materials.used()
+        values = self.materials.values()[used[~self.materials.pec_flags()[used]]]   # PEC cells set no limit
+        if values.size == 0:
+            return
         eps = values[:, 0] if self.polarization == 'ez' else np.minimum(values[:, 0], values[:, 1])
         dx2 = float(self.domain.dx) ** 2
         required = np.float64(DTYPE((C0 * float(self.dt)) ** 2 / dx2)) * 2.0
@@ -182,6 +231,7 @@ class Model2D:
             raise ValueError(f'at most {_lib.MAX_SOURCES} line sources per 2D model')
         if len(self.monitors) > _lib.MAX_MONITORS:
             raise ValueError(f'at most {_lib.MAX_MONITORS} flux monitors per 2D model')
+        self.apply_pec()
         self.check_courant()
         self.calibrate()
         self.sample_interval = dft_sample_interval(float(self.dt), self.sources,
@@ -190,8 +240,11 @@ class Model2D:
             monitor.sample_interval = self.sample_interval
 
     def calibrate(self):
-        """Sheet profiles from the float64 coefficients of the line cells (physics/em/sources.py:582-602)."""
-        values = self.materials.values()
+        """Sheet profiles from the float64 coefficients of the line cells (physics/em/sources.py:582-602); PEC cells
+        (E_z) and faces touching PEC (H_z) inject nothing (fdtd_ez_2d.py:175-177: the weight times the keep mask)."""
+        values = self.materials.values().copy()
+        pec = self.materials.pec_flags()
+        values[pec] = (1.0, 1.0, 1.0, 0.0, 1.0, 0.0)   # calibrated as vacuum, then zeroed by the keep mask
         dt = float(self.dt)
         for source in self.sources:
             omega = 2.0 * np.pi * float(source.temporal.f0)
@@ -200,6 +253,7 @@ class Model2D:
             if self.polarization == 'ez':
                 _, cb = semi_implicit(values[ids, 0], values[ids, 3], dt, EPS0)
                 line_eps = complex_permittivity(values[ids, 0], values[ids, 3], omega)
+                keep = ~pec[ids]
             else:
                 column = 0 if source.component == 'ex' else 1
                 normal_axis = 1 if source.axis == 'x' else 0
@@ -217,7 +271,8 @@ class Model2D:
                 line_eps = (complex_permittivity(values[ids, column], values[ids, 3], omega) * w
                             + complex_permittivity(values[upper_ids, column], values[upper_ids, 3], omega) * w
                             ) / (w + w)
-            source.calibrate(cb, line_eps, mu, sigma_m, dt)
+                keep = ~(pec[ids] | pec[upper_ids])
+            source.calibrate(np.where(keep, cb, 0.0), line_eps, mu, sigma_m, dt)
 
 
 def clock_times(dt, n_steps):

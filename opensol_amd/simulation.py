@@ -74,6 +74,14 @@ class Simulation:
     def add_dielectric_circle(self, cx, cy, radius, eps_r, sigma_e=0.0):
         self.model.add_dielectric_circle(cx, cy, radius, eps_r, sigma_e)
 
+    def add_pec_region(self, mask):
+        """PEC on a bool (nx, ny) cell mask (physics/em/__init__.py:372-376), kept over every dielectric."""
+        self.model.add_pec_region(mask)
+
+    def add_pec_rectangle(self, x0, y0, x1, y1):
+        """PEC on the rectangle raster of add_dielectric_rectangle (this build's convenience over add_pec_region)."""
+        self.model.add_pec_rectangle(x0, y0, x1, y1)
+
     def add_line_source(self, axis, position, wavelength=None, frequency=None, bandwidth=0.5, amplitude=1.0,
                         waveform='gaussian', span=None):
         self._require_unbuilt('add_line_source')
@@ -104,6 +112,7 @@ class Simulation:
             model.finalize()
             self.engine = Engine2D(model, **self._engine_options)
         elif model.materials.version != self._materials_version:
+            model.apply_pec()
             model.check_courant()
             model.calibrate()
             self.engine.materials_changed()

@@ -57,8 +57,8 @@ class Boundaries:
     def __init__(self, faces, npml=15, order=3, sigma_factor=1.0, kappa_max=15.0, alpha_max=0.0):
         self.faces = dict(faces)
         for face, kind in self.faces.items():
-            if kind not in ('cpml', 'periodic'):
-                raise ValueError(f'oracle supports cpml and periodic faces, got {face}={kind}')
+            if kind not in ('cpml', 'periodic', 'pec'):
+                raise ValueError(f'oracle supports cpml, periodic and pec faces, got {face}={kind}')
         for axis in 'xy':
             if (self.faces[f'{axis}_lo'] == 'periodic') != (self.faces[f'{axis}_hi'] == 'periodic'):
                 raise ValueError(f'{axis}: periodic faces must be paired')
@@ -398,6 +398,9 @@ class Yee2D:
                               'sigma_e': np.zeros(shape, F32)}
         self.materials.update(mu_r=np.ones(shape, F32), sigma_m=np.zeros(shape, F32))
         self.sources, self.monitors = [], []
+        self.pec_regions = []
+        if polarization == 'hz' and any(kind == 'pec' for kind in self.bc.faces.values()):
+            raise ValueError("the oracle's 'pec' faces are the E_z wall rows")
         self.n_step, self.time = 0, 0.0
         self._finalized = False
         self._build_spacings_and_cpml()
@@ -440,6 +443,28 @@ class Yee2D:
             self.materials[name][mask] = F32(eps_r)
         self.materials['sigma_e'][mask] = F32(sigma_e)
         self._finalized = False
+
+    def add_pec_region(self, mask):
+        """PEC cells (physics/em/__init__.py:372-376): E_z pinned to 0 in them (ca = cb = 0,
+        solvers/fdtd_ez_2d.py:180-233), or every E face touching them pinned (H_z, solvers/fdtd_hz_2d.py:475-495,
+        yee_grid.py:145-150); sources inject weight x keep."""
+        self.pec_regions.append(np.asarray(mask, dtype=bool).copy())
+        self._finalized = False
+
+    def pec_mask(self):
+        """PEC regions plus the outermost row of every 'pec' face: E_z wall rows pinned to 0 x the inner row
+        (boundaries.py:57-61, 105-128)."""
+        mask = np.zeros(self.geo.shape, dtype=bool)
+        for region in self.pec_regions:
+            mask |= region
+        for face, kind in self.bc.faces.items():
+            if kind == 'pec':
+                index = 0 if face.endswith('lo') else -1
+                if face[0] == 'x':
+                    mask[index, :] = True
+                else:
+                    mask[:, index] = True
+        return mask
 
     def add_line_source(self, axis, position, wavelength=None, frequency=None, bandwidth=0.5, amplitude=1.0,
                         waveform='gaussian', span=None):
@@ -515,8 +540,12 @@ class Yee2D:
             return (real(1.0) - k) / denom, (dt / eps) / denom
 
         self.da, self.db = coefficients(m['mu_r'], m['sigma_m'], mu0)
+        pec = self.pec_mask()
         if self.polarization == 'ez':
             self.ca, self.cb = coefficients(m['eps_r'], m['sigma_e'], eps0)
+            self.keep = ~pec
+            self.ca = self.ca * self.keep
+            self.cb = self.cb * self.keep
         else:
             half = real(0.5)
             pj = np.roll(np.arange(self.ny), -1) if self.bc.periodic('y') else np.minimum(np.arange(self.ny) + 1,
@@ -526,6 +555,9 @@ class Yee2D:
             exx, eyy, sig = (m[name].astype(real) for name in ('eps_xx', 'eps_yy', 'sigma_e'))
             self.ca_x, self.cb_x = coefficients(exx * half + exx[:, pj] * half, sig * half + sig[:, pj] * half, eps0)
             self.ca_y, self.cb_y = coefficients(eyy * half + eyy[pi] * half, sig * half + sig[pi] * half, eps0)
+            self.keep_x, self.keep_y = ~(pec | pec[:, pj]), ~(pec | pec[pi])   # faces touching a PEC cell
+            self.ca_x, self.cb_x = self.ca_x * self.keep_x, self.cb_x * self.keep_x
+            self.ca_y, self.cb_y = self.ca_y * self.keep_y, self.cb_y * self.keep_y
         self._calibrate()
         if self.n_step == 0:
             interval = dft_sample_interval(self.dt, self.sources, [mon.frequencies for mon in self.monitors])
@@ -557,7 +589,7 @@ class Yee2D:
                 eps = (row_eps(lower_row) * w + row_eps(upper_row) * w) / (w + w)
             admittance = sheet_admittance(eps, source.line(m['mu_r']), source.line(m['sigma_m']), omega,
                                           source.normal_width, float(self.dt))
-            source.profile[...] = cb_line * admittance * source.weight
+            source.profile[...] = cb_line * admittance * source.weight   # cb_line is 0 on PEC cells / faces
 
     # ---- stepping ---------------------------------------------------------------------------------------------
     def step(self):

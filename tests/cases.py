@@ -8,6 +8,43 @@ THIN_FILM = dict(size_x=6e-6, size_y=3e-6, dx=25e-9, npml=15, boundaries='period
                  duration=80e-15)
 SNAPSHOT_STEPS = (60, 165, 400)
 
+# 2D cases with perfect electric conductors (the golden script tests/golden/make_pec2d_golden.py writes the same
+# cases from the reference): a PEC rectangle under a dielectric painted after it, a line source crossing the PEC, and
+# for E_z a 'pec' x_lo face.
+PEC2D = dict(size_x=2e-6, size_y=1e-6, dx=25e-9, npml=10, pec_rect=(0.9e-6, 0.3e-6, 1.1e-6, 0.65e-6),
+             rectangles=((1.0e-6, 0.0, 1.6e-6, 1e-6, 3.0),),
+             source=dict(axis='y', position=1.0e-6, wavelength=600e-9, bandwidth=0.8),
+             monitor=dict(axis='y', position=1.7e-6, span=(0.0, 1e-6), wavelengths=[500e-9, 600e-9, 700e-9]),
+             faces={'ez': ('pec', 'cpml', 'periodic', 'periodic'), 'hz': ('cpml', 'cpml', 'periodic', 'periodic')},
+             steps=(150, 500))
+
+
+def build_pec2d(target, polarization, boundary_type, **options):
+    """The PEC case on `target` (oracle Yee2D with oracle Boundaries, or Simulation with BoundaryConfig);
+    boundary_type builds the face configuration from (faces, npml)."""
+    spec = PEC2D
+    bc = boundary_type(spec['faces'][polarization], spec['npml'])
+    sim = target(spec['size_x'], spec['size_y'], spec['dx'], npml=spec['npml'], boundaries=bc,
+                 polarization=polarization, **options)
+    x0, y0, x1, y1 = spec['pec_rect']
+    sim.add_pec_region(pec_rect_mask(sim, x0, y0, x1, y1))
+    for x0, y0, x1, y1, eps in spec['rectangles']:
+        sim.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
+    source = sim.add_line_source(**spec['source'])
+    monitor = sim.add_flux_monitor(**spec['monitor'])
+    return sim, source, monitor
+
+
+def pec_rect_mask(sim, x0, y0, x1, y1):
+    """Cells whose float64 centre lies in [x0, x1) x [y0, y1) (the reference's Domain.rect_mask, core/domain.py:
+    255-260), on either target."""
+    geo = getattr(sim, 'geo', None)
+    if geo is not None:
+        xc, yc = geo.centres
+    else:
+        xc, yc = sim.domain.cell_centers()
+    return ((xc[:, None] >= x0) & (xc[:, None] < x1)) & ((yc[None, :] >= y0) & (yc[None, :] < y1))
+
 
 def build(target, spec, polarization, with_slab=True, **options):
     """Build spec on `target` (the oracle Yee2D class or opensol_amd.Simulation); returns (sim, source, monitor)."""

@@ -150,3 +150,43 @@ def test_launch_geometry_does_not_change_results(gpu, polarization):
     for other in results[1:]:
         for a, b in zip(results[0], other):
             np.testing.assert_array_equal(a, b)
+
+
+def _product_boundaries(faces, npml):
+    from opensol_amd.boundaries import BoundaryConfig
+    bc = BoundaryConfig(*faces)
+    bc.cpml_npml = npml
+    return bc
+
+
+def _oracle_boundaries(faces, npml):
+    from oracle.yee2d import FACES, Boundaries
+    return Boundaries(dict(zip(FACES, faces)), npml=npml)
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_pec_case_matches_oracle(gpu, polarization):
+    """PEC region under a later dielectric, a source line crossing it and (E_z) a 'pec' x_lo face against the FP64
+    oracle (which reproduces the reference's own run to 1e-12): fields <= 1e-5 relative L2, flux <= 1e-5; E_z stays
+    exactly 0 in PEC cells (E_z) and the E faces touching PEC stay exactly 0 (H_z)."""
+    from tests.cases import PEC2D, build_pec2d
+    Simulation = simulation_class()
+    ref, _, ref_monitor = build_pec2d(Yee2D, polarization, _oracle_boundaries, phase_dtype=np.complex128)
+    sim, _, monitor = build_pec2d(Simulation, polarization, _product_boundaries)
+    done = 0
+    for step in PEC2D['steps']:
+        ref.run(n_steps=step - done)
+        sim.run(n_steps=step - done, print_every=0)
+        done = step
+        for name in sim.model.components:
+            error = rel_l2(sim.engine.field(name), ref.fields[name])
+            print(f'{polarization} step {step} {name}: rel L2 {error:.3e}')
+            assert error <= FIELD_TOL, (step, name, error)
+    pec = ref.pec_mask()
+    if polarization == 'ez':
+        assert not np.any(sim.engine.field('ez')[pec])
+    else:
+        assert not np.any(sim.engine.field('ex')[~ref.keep_x]) and not np.any(sim.engine.field('ey')[~ref.keep_y])
+    flux_error = np.max(np.abs(monitor.flux() / ref_monitor.flux() - 1.0))
+    print(f'{polarization} PEC flux rel error {flux_error:.3e}')
+    assert flux_error <= 1e-5

@@ -136,3 +136,34 @@ def test_model_validation_errors():
         model.add_line_source(axis='y', position=0.5e-6)   # neither wavelength nor frequency
     with pytest.raises(ValueError):
         Model2D(1e-6, 1e-6, 25e-9, npml=5, courant=1.5)
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_pec_cells_and_source_profile_match_the_oracle(polarization):
+    """The PEC cells (region + E_z 'pec' face) and the calibrated source profile (zero on PEC cells / faces) equal the
+    oracle's, which reproduces the reference's run."""
+    from opensol_amd.boundaries import BoundaryConfig
+    from oracle.yee2d import FACES, Boundaries, Yee2D
+    from tests.cases import build_pec2d
+
+    def product_bc(faces, npml):
+        bc = BoundaryConfig(*faces)
+        bc.cpml_npml = npml
+        return bc
+
+    model, source, _ = build_pec2d(Model2D, polarization, product_bc)
+    ref, ref_source, _ = build_pec2d(Yee2D, polarization, lambda f, n: Boundaries(dict(zip(FACES, f)), npml=n))
+    model.finalize()
+    ref.finalize()
+    np.testing.assert_array_equal(model.materials.ids == model.materials.pec_id(), ref.pec_mask())
+    np.testing.assert_allclose(source.profile, ref_source.profile.astype(np.float32), rtol=1e-6, atol=0)
+    assert np.all(source.profile[ref_source.profile == 0] == 0)
+
+
+def test_pec_faces_are_e_z_only():
+    from opensol_amd.boundaries import BoundaryConfig
+    bc = BoundaryConfig('pec', 'cpml', 'periodic', 'periodic')
+    with pytest.raises(ValueError, match='E_z polarisation only'):
+        Model2D(1e-6, 1e-6, 25e-9, npml=5, boundaries=bc, polarization='hz')
+    with pytest.raises(ValueError, match='E_z polarisation only'):
+        Model2D(1e-6, 1e-6, 25e-9, npml=5, boundaries=BoundaryConfig('pmc', 'cpml', 'cpml', 'cpml'))

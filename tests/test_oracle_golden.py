@@ -58,3 +58,30 @@ def test_golden_transmission_is_physical():
     r = ((1 - n) / (1 + n)) ** 2
     analytic = (1 - r) ** 2 / (1 - 2 * r * np.cos(2 * delta) + r ** 2)
     assert np.max(np.abs(golden['transmission'] - analytic)) < 1.5e-2
+
+
+def _oracle_boundaries(faces, npml):
+    from oracle.yee2d import FACES, Boundaries
+    return Boundaries(dict(zip(FACES, faces)), npml=npml)
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_oracle_reproduces_reference_pec_case(polarization):
+    """PEC region under a later dielectric, a source line crossing it and (E_z) a 'pec' x_lo face: the FP64 oracle
+    reproduces the reference's FP64 run (fields at two steps, the source profile, the flux spectrum)."""
+    from tests.cases import PEC2D, build_pec2d
+    golden = np.load(os.path.join(GOLDEN, f'pec2d_{polarization}.npz'))
+    sim, source, monitor = build_pec2d(Yee2D, polarization, _oracle_boundaries, phase_dtype=np.complex64)
+    assert (sim.nx, sim.ny) == (int(golden['nx']), int(golden['ny']))
+    np.testing.assert_array_equal(sim.pec_regions[0], golden['pec_mask'])
+    done = 0
+    for step in PEC2D['steps']:
+        sim.run(n_steps=step - done)
+        done = step
+        if step == PEC2D['steps'][0]:
+            np.testing.assert_array_equal(source.profile.astype(np.float64), golden['source_profile'])
+        for name, values in sim.fields.items():
+            want = golden[f'{name}_{step}']
+            np.testing.assert_allclose(values, want, rtol=0, atol=1e-12 * np.abs(want).max() + 1e-300)
+    assert monitor.sample_interval == int(golden['sample_interval'])
+    np.testing.assert_allclose(monitor.flux(), golden['flux'], rtol=1e-12)
