@@ -1206,9 +1206,17 @@ __device__ __forceinline__ void step_block(const yee2d_desc& d, const yee2d_tile
 // steps the bulk tiles with the lean loop, EDGE = true the edge tiles with the general one, each edge tile split into
 // EDGE_SPLIT row chunks. Tiles are enumerated compactly: no block is launched without work. (One launch holding both
 // paths runs slower: it carries the general path's registers into the bulk tiles.)
+//
+// The descriptor is read in place in the kernel argument segment (the first argument sits at offset 0) through the
+// constant address space. Bound to the argument itself, the general H_z path took its address in a way that made the
+// compiler copy all 736 bytes to scratch and turn every field access into a flat load (168 VGPRs + 744 B/lane of
+// scratch); read in place, the fields are scalar loads (127 VGPRs, no scratch).
 template <int POL, int VEC, bool EDGE>
-__global__ void __launch_bounds__(256) yee2d_step_kernel(yee2d_desc d, yee2d_tile_sets sets, int32_t offset)
+__global__ void __launch_bounds__(256) yee2d_step_kernel(yee2d_desc desc, yee2d_tile_sets sets, int32_t offset)
 {
+    typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
+    const yee2d_desc& d = *(const yee2d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
+    if (d.clock != desc.clock) return;   // the argument layout is not the one assumed: do nothing
     const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
     yee::step_block<POL, VEC, EDGE>(d, sets, n, blockIdx.x);
