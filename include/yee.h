@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 3
+#define YEE_ABI_VERSION 4
 #define YEE_MAX_SOURCES 4
 #define YEE_MAX_MONITORS 4
 
@@ -97,6 +97,22 @@ typedef struct yee_cpml_axis {
  * face between two cells of that id, sigma_e, 0, 0, 0); a face between two ids takes the mean eps / sigma (shares 1/2
  * on uniform grids) and forms ca, cb in the kernel (fdtd_hz_2d_kernels.py:190-278).
  */
+/*
+ * Field probes of an instance: after step n (on the device clock), samples[n * n_probes + k] = the field component
+ * component[k] at the flat element index cell[k] of the new state set, for n < capacity (NaN for a record outside
+ * the grid or the components). Batched instances take no probes. One small launch per step
+ * (also inside the step graphs). Replaces the FieldProbe rows of the reference (core/monitors.py:42-130: one
+ * probe_rows launch per step; physics/em/__init__.py:493-498, physics/em3d/physics3d.py:425-433).
+ */
+typedef struct yee_probes {
+    int32_t n_probes;
+    int32_t pad_;
+    const int64_t* cell;        /* 2D: i * pitch + j; 3D: (i * ny + j) * nz + k */
+    const int32_t* component;   /* index into the instance's field components */
+    float* samples;             /* (capacity, n_probes) float32 */
+    int64_t capacity;           /* rows of samples: steps n >= capacity record nothing */
+} yee_probes;
+
 typedef struct yee2d_desc {
     int32_t nx, ny;
     int32_t pitch;           /* row stride in elements of every (nx, *) array: a multiple of 4, >= ny */
@@ -129,6 +145,7 @@ typedef struct yee2d_desc {
     int32_t pad2_;
     yee_line_source source[YEE_MAX_SOURCES];
     yee_flux_line monitor[YEE_MAX_MONITORS];
+    yee_probes probes;
     int64_t* clock;          /* device int64: absolute step index n of the next step */
     int64_t step_limit;      /* steps n >= step_limit are skipped (batched instances); INT64_MAX for none */
 } yee2d_desc;
@@ -225,6 +242,7 @@ typedef struct yee3d_desc {
     int32_t pad_;
     yee_point_source source[YEE3D_MAX_SOURCES];
     yee_flux_plane plane[YEE3D_MAX_PLANES];
+    yee_probes probes;
     int64_t* clock;
     int64_t step_limit;
 } yee3d_desc;
@@ -234,7 +252,7 @@ typedef struct yee_ctx yee_ctx;
 /* Library ABI version (YEE_ABI_VERSION) and sizeof checks for bindings. */
 int32_t yee_abi_version(void);
 /* 0: yee2d_desc, 1: yee_line_source, 2: yee_flux_line, 3: yee_cpml_axis, 4: yee3d_desc, 5: yee_point_source,
- * 6: yee_flux_plane, 7: yee3d_cpml_axis, 8: yee3d_axis_row */
+ * 6: yee_flux_plane, 7: yee3d_cpml_axis, 8: yee3d_axis_row, 9: yee_probes */
 size_t yee_sizeof(int32_t which);
 const char* yee_last_error(void);
 

@@ -6,7 +6,7 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 3
+ABI_VERSION = 4
 MAX_SOURCES = 4
 MAX_MONITORS = 4
 MAX_SOURCES_3D = 4
@@ -33,6 +33,11 @@ class CpmlAxisC(ctypes.Structure):
                 ('psi_e', c_ptr * 2), ('psi_h', c_ptr * 2)]
 
 
+class ProbesC(ctypes.Structure):
+    _fields_ = [('n_probes', c_i32), ('pad_', c_i32), ('cell', c_ptr), ('component', c_ptr), ('samples', c_ptr),
+                ('capacity', c_i64)]
+
+
 class Desc2DC(ctypes.Structure):
     _fields_ = [('nx', c_i32), ('ny', c_i32), ('pitch', c_i32), ('polarization', c_i32), ('periodic_x', c_i32),
                 ('periodic_y', c_i32), ('rows_per_tile', c_i32), ('lane_columns', c_i32), ('field', (c_ptr * 3) * 2),
@@ -41,7 +46,7 @@ class Desc2DC(ctypes.Structure):
                 ('inv_dx_bulk', c_f32), ('inv_dy_bulk', c_f32), ('layer_x_lo', c_i32), ('layer_x_hi', c_i32),
                 ('layer_y_lo', c_i32), ('layer_y_hi', c_i32), ('cpml', CpmlAxisC * 2), ('n_sources', c_i32), ('n_monitors', c_i32), ('sample_interval', c_i32),
                 ('pad2_', c_i32), ('source', LineSourceC * MAX_SOURCES), ('monitor', FluxLineC * MAX_MONITORS),
-                ('clock', c_ptr), ('step_limit', c_i64)]
+                ('probes', ProbesC), ('clock', c_ptr), ('step_limit', c_i64)]
 
 
 class PointSourceC(ctypes.Structure):
@@ -69,10 +74,11 @@ class Desc3DC(ctypes.Structure):
                 ('planes_per_tile', c_i32), ('field', (c_ptr * 6) * 2), ('material', c_ptr), ('table', c_ptr),
                 ('n_materials', c_i32), ('dt', c_f32), ('axis_rows', c_ptr * 3), ('cpml', Cpml3DAxisC * 3), ('n_sources', c_i32), ('n_planes', c_i32), ('sample_interval', c_i32),
                 ('pad_', c_i32), ('source', PointSourceC * MAX_SOURCES_3D), ('plane', FluxPlaneC * MAX_PLANES_3D),
-                ('clock', c_ptr), ('step_limit', c_i64)]
+                ('probes', ProbesC), ('clock', c_ptr), ('step_limit', c_i64)]
 
 
-STRUCTS = (Desc2DC, LineSourceC, FluxLineC, CpmlAxisC, Desc3DC, PointSourceC, FluxPlaneC, Cpml3DAxisC, AxisRow3DC)
+STRUCTS = (Desc2DC, LineSourceC, FluxLineC, CpmlAxisC, Desc3DC, PointSourceC, FluxPlaneC, Cpml3DAxisC, AxisRow3DC,
+           ProbesC)
 
 # Every entry point include/yee.h declares: (name, restype, argtypes)
 SIGNATURES = (

@@ -1258,6 +1258,27 @@ template __global__ void yee2d_batch_kernel<2>(const yee2d_desc* __restrict__, c
 template __global__ void yee2d_batch_kernel<4>(const yee2d_desc* __restrict__, const int32_t* __restrict__, int32_t,
                                               int32_t);
 
+// Field probes after step n = *clock + offset: samples[n * n_probes + k] = field[new set][component[k]][cell[k]]
+// (core/monitors.py:42-130), cell = i * pitch + j. One block; the descriptor is read in place as in yee2d_step_kernel.
+__global__ void __launch_bounds__(256) yee2d_probe_kernel(yee2d_desc desc, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
+    const yee2d_desc& d = *(const yee2d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
+    if (d.clock != desc.clock) return;
+    const int64_t n = *d.clock + offset;
+    const yee_probes& p = d.probes;
+    if (n >= d.step_limit || n >= p.capacity) return;
+    const int set = (int)((n & 1) ^ 1);
+    const int64_t cells = (int64_t)d.nx * d.pitch;
+    for (int k = threadIdx.x; k < p.n_probes; k += blockDim.x) {
+        const int32_t c = p.component[k];
+        const int64_t cell = p.cell[k];
+        // a record outside the grid or the components samples NaN rather than reading out of bounds
+        p.samples[n * p.n_probes + k] = (c >= 0 && c < 3 && cell >= 0 && cell < cells)
+                                            ? d.field[set][c][cell] : __builtin_nanf("");
+    }
+}
+
 // Advance one clock by k steps (bounded by the instance's step limit); a single lane writes with a vector store.
 __global__ void yee_clock_advance(int64_t* clock, int64_t k, int64_t limit)
 {

@@ -621,6 +621,27 @@ __global__ void __launch_bounds__(256) yee3d_batch_sample_kernel(const yee3d_des
     yee3::sample_cell(d, *d.clock + offset, t - prefix[lo]);
 }
 
+// Field probes after step n = *clock + offset: samples[n * n_probes + k] = field[new set][component[k]][cell[k]]
+// (core/monitors.py:42-130, physics/em3d/physics3d.py:425-433). One block; the descriptor is read in place.
+__global__ void __launch_bounds__(256) yee3d_probe_kernel(yee3d_desc desc, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
+    const yee3d_desc& d = *(const yee3d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
+    if (d.clock != desc.clock) return;
+    const int64_t n = *d.clock + offset;
+    const yee_probes& p = d.probes;
+    if (n >= d.step_limit || n >= p.capacity) return;
+    const int set = (int)((n & 1) ^ 1);
+    const int64_t cells = (int64_t)d.nx * d.ny * d.nz;
+    for (int k = threadIdx.x; k < p.n_probes; k += blockDim.x) {
+        const int32_t c = p.component[k];
+        const int64_t cell = p.cell[k];
+        // a record outside the grid or the components samples NaN rather than reading out of bounds
+        p.samples[n * p.n_probes + k] = (c >= 0 && c < 6 && cell >= 0 && cell < cells)
+                                            ? d.field[set][c][cell] : __builtin_nanf("");
+    }
+}
+
 // Advance the clocks of a 3D batch by k steps, each bounded by its instance's step limit.
 __global__ void yee_clock_advance_batch3d(const yee3d_desc* __restrict__ descs, int32_t n_inst, int64_t k)
 {

@@ -28,6 +28,8 @@ __global__ void yee3d_batch_sample_kernel(const yee3d_desc* __restrict__ descs, 
                                           int32_t n_inst, int32_t offset);
 __global__ void yee_clock_advance_batch3d(const yee3d_desc* __restrict__ descs, int32_t n_inst, int64_t k);
 __global__ void yee_clock_advance(int64_t* clock, int64_t k, int64_t limit);
+__global__ void yee2d_probe_kernel(yee2d_desc d, int32_t offset);
+__global__ void yee3d_probe_kernel(yee3d_desc d, int32_t offset);
 __global__ void yee_clock_advance_batch(const yee2d_desc* __restrict__ descs, int32_t n_inst, int64_t k);
 
 namespace {
@@ -89,9 +91,20 @@ struct yee_ctx {
 
 namespace {
 
+// Field probe table: the counts and pointers are checked here. The device cells and components are not read on the
+// host; the probe kernels sample NaN for a record outside the grid or the components.
+int32_t check_probes(const yee_probes& p)
+{
+    if (p.n_probes < 0) return fail(YEE_EINVAL, "n_probes must be >= 0");
+    if (p.n_probes > 0 && (!p.cell || !p.component || !p.samples || p.capacity < 0))
+        return fail(YEE_EINVAL, "field probes need cell, component and samples arrays and a capacity >= 0");
+    return YEE_OK;
+}
+
 int32_t check_2d(const yee2d_desc* d)
 {
     if (!d) return fail(YEE_EINVAL, "null descriptor");
+    if (check_probes(d->probes) != YEE_OK) return YEE_EINVAL;
     if (d->nx < 2 || d->ny < 2) return fail(YEE_EINVAL, "2D grid needs nx, ny >= 2");
     if (d->pitch < d->ny || d->pitch % 4 != 0) return fail(YEE_EINVAL, "pitch must be a multiple of 4 and >= ny");
     if (d->polarization != YEE_EZ && d->polarization != YEE_HZ) return fail(YEE_EINVAL, "polarization must be 0 or 1");
@@ -142,6 +155,7 @@ int32_t check_2d(const yee2d_desc* d)
 int32_t check_3d(const yee3d_desc* d)
 {
     if (!d) return fail(YEE_EINVAL, "null descriptor");
+    if (check_probes(d->probes) != YEE_OK) return YEE_EINVAL;
     const int n[3] = {d->nx, d->ny, d->nz};
     for (int a = 0; a < 3; ++a)
         if (n[a] < 2) return fail(YEE_EINVAL, "3D grid needs nx, ny, nz >= 2");
@@ -359,6 +373,8 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
                 hipLaunchKernelGGL(bulk, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
             if (ctx->edge_blocks > 0)
                 hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
+            if (ctx->d2.probes.n_probes > 0)
+                hipLaunchKernelGGL(yee2d_probe_kernel, dim3(1), dim3(256), 0, stream, ctx->d2, k);
         }
         YEE_HIP(hipGetLastError());
         hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, stream, ctx->d2.clock, (int64_t)count,
@@ -374,6 +390,8 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
             if (sample_blocks > 0)
                 hipLaunchKernelGGL(yee3d_sample_kernel, dim3(sample_blocks), dim3(256), 0, stream, ctx->d3,
                                    ctx->sample_cells, k);
+            if (ctx->d3.probes.n_probes > 0)
+                hipLaunchKernelGGL(yee3d_probe_kernel, dim3(1), dim3(256), 0, stream, ctx->d3, k);
         }
         YEE_HIP(hipGetLastError());
         hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, stream, ctx->d3.clock, (int64_t)count,
@@ -451,6 +469,7 @@ size_t yee_sizeof(int32_t which)
         case 6: return sizeof(yee_flux_plane);
         case 7: return sizeof(yee3d_cpml_axis);
         case 8: return sizeof(yee3d_axis_row);
+        case 9: return sizeof(yee_probes);
         default: return 0;
     }
 }
@@ -499,6 +518,8 @@ int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, 
         int32_t status = check_2d(&host[k]);
         if (status == YEE_OK && host[k].lane_columns != host[0].lane_columns)
             status = fail(YEE_EINVAL, "batched instances must share lane_columns");
+        if (status == YEE_OK && host[k].probes.n_probes > 0)
+            status = fail(YEE_EINVAL, "batched instances take no field probes");
         if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
         const int64_t next = prefix[k] + tiles_2d(host[k]);
         if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
@@ -579,6 +600,7 @@ int32_t step_timed_3d(yee_ctx* ctx, int64_t n_steps, hipStream_t s, double* kern
         if (sample_blocks > 0)
             hipLaunchKernelGGL(yee3d_sample_kernel, dim3(sample_blocks), dim3(256), 0, s, d, ctx->sample_cells, 0);
         (void)hipEventRecord(ev[2], s);
+        if (d.probes.n_probes > 0) hipLaunchKernelGGL(yee3d_probe_kernel, dim3(1), dim3(256), 0, s, d, 0);
         hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, s, d.clock, (int64_t)1, d.step_limit);
         hipError_t e = hipEventSynchronize(ev[2]);
         float a = 0.f, b = 0.f;
@@ -620,6 +642,8 @@ int32_t yee_step_timed(yee_ctx* ctx, int64_t n_steps, void* stream, double* kern
         if (ctx->edge_blocks > 0)
             hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, s, ctx->d2, ctx->sets, 0);
         (void)hipEventRecord(ev[2], s);
+        if (ctx->d2.probes.n_probes > 0)
+            hipLaunchKernelGGL(yee2d_probe_kernel, dim3(1), dim3(256), 0, s, ctx->d2, 0);
         hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, s, ctx->d2.clock, (int64_t)1,
                            ctx->d2.step_limit);
         hipError_t e = hipEventSynchronize(ev[2]);
@@ -705,6 +729,7 @@ int32_t yee3d_batch_create(const yee3d_desc* descs, int32_t n, void* descs_dev, 
         const int32_t status = check_3d(&host[k]);
         if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
         if (host[k].rows_per_tile != kRows3D) return fail(YEE_EINVAL, "batched 3D instances use 4 rows per tile");
+        if (host[k].probes.n_probes > 0) return fail(YEE_EINVAL, "batched instances take no field probes");
         const int64_t next = block_prefix[k] + (tiles_3d(host[k]) + 3) / 4;
         if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
         block_prefix[k + 1] = (int32_t)next;

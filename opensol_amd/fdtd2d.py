@@ -15,7 +15,7 @@ from .boundaries import CPMLAxisTables, curl_spacings, resolve_boundaries
 from .constants import C0, DTYPE, EPS0, MU0
 from .domain import Domain
 from .materials import MaterialMap, semi_implicit
-from .monitors import FluxMonitor, dft_sample_interval
+from .monitors import FieldProbe, FluxMonitor, ProbeTable, dft_sample_interval
 from .sources import LineSource, complex_permittivity, resolve_frequencies, resolve_frequency
 from .waveforms import make_temporal
 
@@ -56,6 +56,7 @@ class Model2D:
         self.inv_bulk = tuple(float(a[b]) for a, b in ((self.inv_dx_e, self.nx // 2), (self.inv_dy_e, self.ny // 2)))
         self.sample_interval = 1
         self.pec_regions = []   # bool (nx, ny) masks, applied over the dielectrics at finalize
+        self.probes = []
 
     @property
     def shape(self):
@@ -86,6 +87,19 @@ class Model2D:
 
     def _material(self, eps_r, sigma_e):
         return self.materials.material_id(eps_r, eps_r, eps_r, sigma_e, 1.0, 0.0)
+
+    def add_field_probe(self, x, y, field_component=None):
+        """Raw Yee samples of one component at the cell containing (x, y); None = the out-of-plane field
+        (physics/em/__init__.py:493-498, placement.py:29-45)."""
+        components = COMPONENTS[self.polarization]
+        field_component = components[0] if field_component is None else field_component
+        if field_component not in components:
+            raise ValueError(f"field probe component of an {self.polarization!r} physics must be one of "
+                             f"{components}, got {field_component!r}")
+        self.domain.validate_physical_pos((x, y), 'Field probe')
+        probe = FieldProbe(self.domain.pos_to_cell(x, y), field_component)
+        self.probes.append(probe)
+        return probe
 
     def add_pec_region(self, mask):
         """Perfect electric conductor on the bool (nx, ny) cell mask (physics/em/__init__.py:372-376): E_z pinned to 0
@@ -335,6 +349,11 @@ class Engine2D:
             monitor.engine, monitor.index = self, index
         self.waves = [None] * len(model.sources)
         self.phases = [None] * len(model.monitors)
+        components = COMPONENTS[model.polarization]
+        self.probes = ProbeTable(model.probes, [p.cell[0] * pitch + p.cell[1] for p in model.probes],
+                                 [components.index(p.field_component) for p in model.probes], dev)
+        for index, probe in enumerate(model.probes):
+            probe.engine, probe.index = self, index
         self._upload_materials()
         self._ctx = _lib.c_ptr()
         self._ensure_capacity(capacity, create=True)
@@ -370,9 +389,14 @@ class Engine2D:
         sample_times = times[m * np.arange(1, n_samples + 1)]
         self.phases = [torch.from_numpy(np.ascontiguousarray(mon.phase_table(sample_times))).to(self.device)
                        for mon in model.monitors]
+        self.probes.ensure(capacity)
         self._times = times
         self.capacity = capacity
         self._rebind(create)
+
+    def probe_series(self, index):
+        """(times after each step [s], float32 values) of probe `index` over the steps run so far."""
+        return self._times[1:self.n_step + 1].copy(), self.probes.series(index, self.n_step)
 
     def descriptor(self):
         """The yee2d_desc of the current buffers (for the single-instance context or a batch)."""
@@ -427,6 +451,7 @@ class Engine2D:
             mon.phase = self.phases[k].data_ptr()
             mon.n_samples = self.phases[k].shape[0]
             mon.acc = self.accumulators[k].data_ptr()
+        self.probes.fill(d.probes)
         d.clock = self.clock.data_ptr()
         d.step_limit = _lib.INT64_MAX
         return d

@@ -17,7 +17,7 @@ from .constants import C0, DTYPE, EPS0, MU0
 from .domain import Domain
 from .fdtd2d import clock_times, steps_for_duration
 from .materials import MaterialMap, semi_implicit
-from .monitors import FluxPlane, dft_sample_interval
+from .monitors import FieldProbe, FluxPlane, ProbeTable, dft_sample_interval
 from .sources import PointDipole, resolve_frequencies, resolve_frequency
 from .waveforms import make_temporal
 
@@ -71,6 +71,7 @@ class Model3D:
             self.inv_h.append(inv_h)
         self.sample_interval = 1
         self.pec_boxes = []   # index ranges of PEC boxes, painted over the dielectrics at finalize
+        self.probes = []
 
     def axis_rows(self, axis):
         """(n, 8) float32 records of one axis (the yee3d_axis_row layout): inv_h, inv_e, psi row of the hi-face
@@ -193,6 +194,15 @@ class Model3D:
         self.sources.append(source)
         return source
 
+    def add_field_probe(self, x, y, z, component='ez'):
+        """Raw Yee samples of one component at the cell containing (x, y, z) (physics/em3d/physics3d.py:425-433)."""
+        if component not in COMPONENTS:
+            raise ValueError(f'probe component must be one of {list(COMPONENTS)}, got {component!r}')
+        self.domain.validate_physical_pos((x, y, z), 'Field probe')
+        probe = FieldProbe(self.domain.pos_to_cell(x, y, z), component)
+        self.probes.append(probe)
+        return probe
+
     def nearest_edge(self, axis, coordinate):
         """Grid edge nearest the coordinate among the window's edges (physics/em3d/physics3d.py:492-496)."""
         window = self.domain.physical_window()[axis]
@@ -298,6 +308,11 @@ class Engine3D:
             monitor.engine, monitor.index = self, index
         self.waves = [None] * len(model.sources)
         self.phases = [None] * len(model.monitors)
+        ny, nz = shape[1], shape[2]
+        self.probes = ProbeTable(model.probes, [(p.cell[0] * ny + p.cell[1]) * nz + p.cell[2] for p in model.probes],
+                                 [COMPONENTS.index(p.field_component) for p in model.probes], dev)
+        for index, probe in enumerate(model.probes):
+            probe.engine, probe.index = self, index
         self._upload_materials()
         self._ctx = _lib.c_ptr()
         self._ensure_capacity(capacity, create=True)
@@ -325,9 +340,14 @@ class Engine3D:
         sample_times = times[m * np.arange(1, n_samples + 1)]
         self.phases = [torch.from_numpy(np.ascontiguousarray(mon.phase_table(sample_times))).to(self.device)
                        for mon in model.monitors]
+        self.probes.ensure(capacity)
         self._times = times
         self.capacity = capacity
         self._rebind(create)
+
+    def probe_series(self, index):
+        """(times after each step [s], float32 values) of probe `index` over the steps run so far."""
+        return self._times[1:self.n_step + 1].copy(), self.probes.series(index, self.n_step)
 
     def descriptor(self):
         model = self.model
@@ -376,6 +396,7 @@ class Engine3D:
             pl.n_freq = monitor.n_freqs
             pl.phase = self.phases[k].data_ptr()
             pl.n_samples = self.phases[k].shape[0]
+        self.probes.fill(d.probes)
         d.clock = self.clock.data_ptr()
         d.step_limit = _lib.INT64_MAX
         return d

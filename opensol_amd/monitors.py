@@ -161,3 +161,70 @@ class FluxPlane:
         for pair, e_spectrum, h_spectrum in self.spectra():
             total += pair['sign'] * np.einsum('fij,ij->f', np.real(e_spectrum * np.conj(h_spectrum)), pair['area'])
         return np.cos(np.pi * self.frequencies * self.dt) * total
+
+
+class FieldProbe:
+    """Raw Yee samples of one field component at one cell, one after every step (core/monitors.py:194-283; the
+    reference's FieldProbeRows sample every probe in one launch per step, :42-130). get_data() gives the times after
+    each recorded step [s] (float64) and the values (float32), as FieldProbe.get_data."""
+
+    def __init__(self, cell, field_component):
+        self.cell = tuple(int(v) for v in cell)
+        self.field_component = field_component
+        self.engine = None
+        self.index = None
+
+    @property
+    def ix(self):
+        return self.cell[0]
+
+    @property
+    def iy(self):
+        return self.cell[1]
+
+    @property
+    def iz(self):
+        return self.cell[2] if len(self.cell) > 2 else None
+
+    def get_data(self):
+        if self.engine is None:
+            return np.empty(0, dtype=np.float64), np.empty(0, dtype=np.float32)
+        return self.engine.probe_series(self.index)
+
+
+class ProbeTable:
+    """Device table of an engine's field probes: flat cell indices (int64), component indices (int32) and the
+    (capacity, n_probes) float32 samples the probe launch writes at row n after step n (the yee_probes record)."""
+
+    def __init__(self, probes, cells, components, device):
+        import torch
+        self.torch = torch
+        self.device = device
+        self.n = len(probes)
+        self.cell = torch.tensor(cells, dtype=torch.int64, device=device) if self.n else None
+        self.component = torch.tensor(components, dtype=torch.int32, device=device) if self.n else None
+        self.samples = None
+        self.capacity = 0
+
+    def ensure(self, capacity):
+        """Room for rows [0, capacity); the rows recorded so far are kept."""
+        if not self.n or capacity <= self.capacity:
+            return
+        grown = self.torch.zeros((int(capacity), self.n), dtype=self.torch.float32, device=self.device)
+        if self.samples is not None:
+            grown[:self.capacity] = self.samples
+        self.samples = grown
+        self.capacity = int(capacity)
+
+    def fill(self, probes_c):
+        """Fill a yee_probes record (ProbesC)."""
+        probes_c.n_probes = self.n
+        if not self.n:
+            return
+        probes_c.cell = self.cell.data_ptr()
+        probes_c.component = self.component.data_ptr()
+        probes_c.samples = self.samples.data_ptr()
+        probes_c.capacity = self.capacity
+
+    def series(self, index, n_step):
+        return self.samples[:n_step, index].cpu().numpy()

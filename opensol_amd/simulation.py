@@ -96,6 +96,12 @@ class Simulation:
         return self.model.add_line_source(axis, position, wavelength, frequency, bandwidth, amplitude, waveform, span,
                                           waist=waist, center=center)
 
+    def add_field_probe(self, x, y, field_component=None):
+        """Raw Yee samples of one component at the cell containing (x, y), one per step; get_data() -> (times,
+        values) (physics/em/__init__.py:493-498)."""
+        self._require_unbuilt('add_field_probe')
+        return self.model.add_field_probe(x, y, field_component)
+
     def add_flux_monitor(self, axis, position, span, wavelengths=None, frequencies=None):
         self._require_unbuilt('add_flux_monitor')
         return self.model.add_flux_monitor(axis, position, span, wavelengths, frequencies)
@@ -231,6 +237,12 @@ class Simulation3D:
                    bandwidth=0.5, waveform='gaussian'):
         self._require_unbuilt('add_dipole')
         return self.model.add_dipole(x, y, z, orientation, dipole_moment, wavelength, frequency, bandwidth, waveform)
+
+    def add_field_probe(self, x, y, z, component='ez'):
+        """Raw Yee samples of one component at the cell containing (x, y, z), one per step
+        (physics/em3d/physics3d.py:425-433)."""
+        self._require_unbuilt('add_field_probe')
+        return self.model.add_field_probe(x, y, z, component)
 
     def add_flux_plane(self, normal, position, wavelengths=None, frequencies=None):
         self._require_unbuilt('add_flux_plane')

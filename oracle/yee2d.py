@@ -358,6 +358,23 @@ class FluxMonitor:
         return integrand @ self.widths
 
 
+class PointProbe:
+    """Raw Yee samples of one component at one cell after every step, stamped with the time after the step
+    (core/monitors.py:194-283; engine3d.py:544-548 records the probe rows after the step and stamps self.time)."""
+
+    def __init__(self, cell, component):
+        self.cell = tuple(int(v) for v in cell)
+        self.component = component
+        self.times, self.values = [], []
+
+    def record(self, fields, t):
+        self.times.append(float(t))
+        self.values.append(fields[self.component][self.cell])
+
+    def get_data(self):
+        return np.array(self.times, dtype=np.float64), np.array(self.values, dtype=np.float64)
+
+
 def dft_sample_interval(dt, sources, monitor_frequencies):
     """m = max(1, floor(1 / ((max source limit + max monitor frequency) dt))) (em/monitors.py:177-191)."""
     if not sources or not monitor_frequencies:
@@ -397,7 +414,7 @@ class Yee2D:
             self.materials = {'eps_xx': np.ones(shape, F32), 'eps_yy': np.ones(shape, F32),
                               'sigma_e': np.zeros(shape, F32)}
         self.materials.update(mu_r=np.ones(shape, F32), sigma_m=np.zeros(shape, F32))
-        self.sources, self.monitors = [], []
+        self.sources, self.monitors, self.probes = [], [], []
         self.pec_regions = []
         if polarization == 'hz' and any(kind == 'pec' for kind in self.bc.faces.values()):
             raise ValueError("the oracle's 'pec' faces are the E_z wall rows")
@@ -507,6 +524,19 @@ class Yee2D:
         self._finalized = False
         return source
 
+    def add_field_probe(self, x, y, field_component=None):
+        """Samples of one component at the cell containing (x, y); None = the out-of-plane field
+        (physics/em/__init__.py:493-498, placement.py:16, 29-45)."""
+        names = tuple(self.fields)
+        component = names[0] if field_component is None else field_component
+        if component not in names:
+            raise ValueError(f'field probe component of an {self.polarization!r} physics must be one of {names}, '
+                             f'got {component!r}')
+        self.geo.validate(x, y, 'Field probe')
+        probe = PointProbe((self.geo.cell(0, x), self.geo.cell(1, y)), component)
+        self.probes.append(probe)
+        return probe
+
     def add_flux_monitor(self, axis, position, span, wavelengths=None, frequencies=None):
         """placement.py:55-70, physics/em/__init__.py:511-518."""
         geo = self.geo
@@ -608,6 +638,8 @@ class Yee2D:
             source.inject(self.fields, t_half)
         self.n_step += 1
         self.time = t_full
+        for probe in self.probes:
+            probe.record(self.fields, t_full)
         for monitor in self.monitors:
             if self.n_step % monitor.sample_interval == 0:
                 monitor.update(self.fields, t_full)

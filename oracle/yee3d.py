@@ -29,8 +29,8 @@ import math
 
 import numpy as np
 
-from .yee2d import (C0, EPS0, F32, MU0, containing_cell, cpml_recursion, cpml_side_rows, dft_sample_interval,
-                    make_temporal, resolve_frequency, uniform_cell_count)
+from .yee2d import (C0, EPS0, F32, MU0, PointProbe, containing_cell, cpml_recursion, cpml_side_rows,
+                    dft_sample_interval, make_temporal, resolve_frequency, uniform_cell_count)
 
 AXES = ('x', 'y', 'z')
 FACES_3D = ('x_lo', 'x_hi', 'y_lo', 'y_hi', 'z_lo', 'z_hi')
@@ -258,7 +258,7 @@ class Yee3D:
         self.fields = {name: np.zeros(self.shape, dtype=self.dtype) for name in ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')}
         self.materials = {'eps_r': np.ones(self.shape, F32), 'sigma_e': np.zeros(self.shape, F32),
                           'mu_r': np.ones(self.shape, F32)}
-        self.sources, self.monitors = [], []
+        self.sources, self.monitors, self.probes = [], [], []
         self.pec_boxes = []
         self.n_step, self.time = 0, 0.0
         self._finalized = False
@@ -392,6 +392,15 @@ class Yee3D:
         self._finalized = False
         return source
 
+    def add_field_probe(self, x, y, z, component='ez'):
+        """Samples of one component at the cell containing (x, y, z) after every step (physics3d.py:425-433)."""
+        if component not in ('ex', 'ey', 'ez', 'hx', 'hy', 'hz'):
+            raise ValueError(f'probe component must be one of ex, ey, ez, hx, hy, hz, got {component!r}')
+        self.validate((x, y, z), 'Field probe')
+        probe = PointProbe(tuple(self.cell(axis, v) for axis, v in enumerate((x, y, z))), component)
+        self.probes.append(probe)
+        return probe
+
     def add_flux_plane(self, normal, position, wavelengths=None, frequencies=None):
         """Flux through the cell-edge plane nearest `position` across the whole window (physics3d.py:434-445)."""
         if (wavelengths is None) == (frequencies is None):
@@ -503,6 +512,8 @@ class Yee3D:
             source.inject(F, t + 0.5 * float(self.dt), self.dtype)
         self.time = t + float(self.dt)
         self.n_step += 1
+        for probe in self.probes:
+            probe.record(F, self.time)
         for monitor in self.monitors:
             if self.n_step % monitor.sample_interval == 0:
                 monitor.update(F, self.time)

@@ -7,6 +7,11 @@ THIN_FILM = dict(size_x=6e-6, size_y=3e-6, dx=25e-9, npml=15, boundaries='period
                  monitor=dict(axis='y', position=4.5e-6, span=(0.0, 3e-6), wavelengths=[500e-9, 600e-9]),
                  duration=80e-15)
 SNAPSHOT_STEPS = (60, 165, 400)
+# field probes of the thin-film case: (x, y, index into the polarization's components); window corners, the source
+# line, the film's first cell and the flux line
+THIN_FILM_PROBES = ((1.2e-6, 1.5e-6, 0), (3.0e-6, 0.7e-6, 1), (4.5e-6, 2.9e-6, 2), (0.0, 0.0, 0), (6e-6, 3e-6, 0),
+                    (2.76e-6, 1.0e-6, 0), (4.5e-6, 1.2e-6, 1), (5.0e-6, 0.3e-6, 2))
+PROBE_COMPONENTS = {'ez': ('ez', 'hx', 'hy'), 'hz': ('hz', 'ex', 'ey')}
 
 # 2D cases with perfect electric conductors (the golden script tests/golden/make_pec2d_golden.py writes the same
 # cases from the reference): a PEC rectangle under a dielectric painted after it, a line source crossing the PEC, and
@@ -58,6 +63,10 @@ def build(target, spec, polarization, with_slab=True, **options):
     source = sim.add_line_source(**spec['source']) if spec.get('source') else None
     monitor = sim.add_flux_monitor(**spec['monitor']) if spec.get('monitor') else None
     return sim, source, monitor
+
+
+def add_thin_film_probes(sim, polarization):
+    return [sim.add_field_probe(x, y, PROBE_COMPONENTS[polarization][c]) for x, y, c in THIN_FILM_PROBES]
 
 
 def random_rectangles(rng, size_x, size_y, count, eps_range=(1.0, 12.0), side_range=(0.1, 0.5)):

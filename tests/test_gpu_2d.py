@@ -190,3 +190,38 @@ def test_pec_case_matches_oracle(gpu, polarization):
     flux_error = np.max(np.abs(monitor.flux() / ref_monitor.flux() - 1.0))
     print(f'{polarization} PEC flux rel error {flux_error:.3e}')
     assert flux_error <= 1e-5
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_field_probes_match_oracle(gpu, polarization):
+    """Field probes of the thin-film case: every sample within 1e-5 of the series' peak of the FP64 oracle's, times
+    equal to the oracle's, the last sample equal to the engine's field at the probe's cell bit for bit, rows kept
+    while the sample table grows, and graph replay bit-identical to eager stepping."""
+    from tests.cases import add_thin_film_probes
+    Simulation = simulation_class()
+    ref, _, _ = build(Yee2D, THIN_FILM, polarization)
+    sim, _, _ = build(Simulation, THIN_FILM, polarization)
+    graphed, _, _ = build(Simulation, THIN_FILM, polarization, graph_steps=16)
+    ref_probes, probes, graphed_probes = (add_thin_film_probes(s, polarization) for s in (ref, sim, graphed))
+    done = 0
+    for step in SNAPSHOT_STEPS:   # three runs: the table grows twice
+        ref.run(n_steps=step - done)
+        sim.run(n_steps=step - done, print_every=0)
+        graphed.run(n_steps=step - done, print_every=0)
+        done = step
+        for probe in probes:
+            times, values = probe.get_data()
+            assert len(values) == step and values.dtype == np.float32
+            assert values[-1] == sim.engine.field(probe.field_component)[probe.cell]
+    for probe, graphed_probe, ref_probe in zip(probes, graphed_probes, ref_probes):
+        times, values = probe.get_data()
+        ref_times, ref_values = ref_probe.get_data()
+        np.testing.assert_array_equal(times, ref_times)
+        np.testing.assert_array_equal(graphed_probe.get_data()[1], values)
+        peak = np.max(np.abs(ref_values))
+        if peak == 0.0:   # H_x (E_z) and E_x (H_z) stay exactly 0 in this y-invariant case
+            assert not np.any(values), probe.cell
+            continue
+        error = np.max(np.abs(values - ref_values)) / peak
+        print(f'{polarization} probe {probe.cell} {probe.field_component}: max err / peak {error:.3e}')
+        assert error <= 1e-5, (probe.cell, error)

@@ -167,3 +167,22 @@ def test_pec_faces_are_e_z_only():
         Model2D(1e-6, 1e-6, 25e-9, npml=5, boundaries=bc, polarization='hz')
     with pytest.raises(ValueError, match='E_z polarisation only'):
         Model2D(1e-6, 1e-6, 25e-9, npml=5, boundaries=BoundaryConfig('pmc', 'cpml', 'cpml', 'cpml'))
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_field_probe_cells_match_the_oracle(polarization):
+    """Probe placement (the cell containing the point, window edges included) and the component rule: None is the
+    out-of-plane field; only the polarization's own components are accepted (physics/em/placement.py:16, 29-45)."""
+    from tests.cases import PROBE_COMPONENTS, THIN_FILM_PROBES, add_thin_film_probes
+    model, _, _ = build(Model2D, THIN_FILM, polarization)
+    ref, _, _ = build(Yee2D, THIN_FILM, polarization)
+    got, want = add_thin_film_probes(model, polarization), add_thin_film_probes(ref, polarization)
+    assert [p.cell for p in got] == [p.cell for p in want]
+    assert [p.field_component for p in got] == [p.component for p in want]
+    assert model.add_field_probe(1e-6, 1e-6).field_component == PROBE_COMPONENTS[polarization][0]
+    other = PROBE_COMPONENTS['hz' if polarization == 'ez' else 'ez'][0]
+    with pytest.raises(ValueError):
+        model.add_field_probe(1e-6, 1e-6, other)
+    with pytest.raises(ValueError):
+        model.add_field_probe(THIN_FILM['size_x'] + 1e-7, 1e-6)
+    assert len(THIN_FILM_PROBES) + 1 == len(model.probes)

@@ -190,3 +190,16 @@ def test_courant_check_skips_pec_cells():
     dx = 25e-9
     m = Model3D(8 * dx, 8 * dx, 8 * dx, dx, npml=2, boundaries={'x_lo': 'pec', 'x_hi': 'pec'})
     m.finalize()   # the eps = -1 entry sets no limit
+
+
+def test_field_probe_cells_match_the_oracle():
+    model, ref = _pair({'y_lo': 'periodic', 'y_hi': 'periodic'})
+    points = ((0.0, 0.0, 0.0), (0.7e-6, 0.6e-6, 0.55e-6), (0.36e-6, 0.29e-6, 0.27e-6), (0.1e-6, 0.45e-6, 0.5e-6))
+    for point in points:
+        for component in ('ex', 'ey', 'ez', 'hx', 'hy', 'hz'):
+            got, want = model.add_field_probe(*point, component), ref.add_field_probe(*point, component)
+            assert got.cell == want.cell and got.field_component == want.component
+    with pytest.raises(ValueError):
+        model.add_field_probe(0.1e-6, 0.1e-6, 0.1e-6, 'e')
+    with pytest.raises(ValueError):
+        model.add_field_probe(0.1e-6, 0.1e-6, 0.6e-6)

@@ -65,6 +65,12 @@ def test_fields_and_flux_match_golden(name):
     G = _spec()
     golden = np.load(os.path.join(GOLDEN, f'dipole3d_{name}.npz'))
     sim, source, planes = _golden_model(name)
+    # field probes on the three slices through the dipole's cell (one per component) and at the window's corners
+    x, y, z = G.CASES[name]['dipole']
+    sizes = G.WINDOW
+    spots = [(x, y, z), (0.1e-6, y, 0.2e-6), (x, 0.5e-6, 0.1e-6), (0.6e-6, 0.05e-6, z), (0.0, 0.0, 0.0), sizes,
+             (x, y, 0.4e-6)]
+    probes = [sim.add_field_probe(*spot, component=f) for spot, f in zip(spots * 2, G.FIELDS * 2)]
     sim.run(n_steps=G.SLICE_STEP)
     i, j, k = source.cell
     for f in G.FIELDS:
@@ -74,6 +80,20 @@ def test_fields_and_flux_match_golden(name):
     assert sim.time == float(golden['time'])
     for f in G.FIELDS:
         assert _rel(golden[f'{f}_{G.N_STEPS}'], sim.fields[f]) < 5e-4, f
+    # each probe's samples after SLICE_STEP and N_STEPS are the reference's field at its cell (float32 run: within
+    # 1e-4 of the field's peak; 1e-3 in the dipole's own cell, where the float32 run accumulates the source's
+    # residue; a neighbouring cell is off by 5e-4 to 3e-2 of the peak)
+    for probe in probes:
+        times, values = probe.get_data()
+        assert len(values) == G.N_STEPS and times[-1] == sim.time
+        pi, pj, pk = probe.cell
+        final = golden[f'{probe.component}_{G.N_STEPS}']
+        tol = 1e-3 if probe.cell == source.cell else 1e-4
+        assert abs(values[-1] - final[probe.cell]) <= tol * np.abs(final).max(), probe.cell
+        for axis, hit, index in (('x', pi == i, (pj, pk)), ('y', pj == j, (pi, pk)), ('z', pk == k, (pi, pj))):
+            if hit:
+                early = golden[f'{probe.component}_{G.SLICE_STEP}_{axis}']
+                assert abs(values[G.SLICE_STEP - 1] - early[index]) <= 1e-4 * np.abs(early).max(), (probe.cell, axis)
     for index, plane in enumerate(planes):
         np.testing.assert_allclose(plane.flux(), golden[f'flux_{index}'], rtol=2e-6)
         for pair_index, (_, e_spectrum, h_spectrum) in enumerate(plane.spectra()):

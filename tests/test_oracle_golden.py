@@ -6,13 +6,14 @@ import numpy as np
 import pytest
 
 from oracle.yee2d import Yee2D
-from tests.cases import SNAPSHOT_STEPS, THIN_FILM, build
+from tests.cases import SNAPSHOT_STEPS, THIN_FILM, add_thin_film_probes, build
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
 
 
 def run_oracle(polarization, with_slab):
     sim, source, monitor = build(Yee2D, THIN_FILM, polarization, with_slab)
+    probes = add_thin_film_probes(sim, polarization)
     snapshots = {}
     done = 0
     for step in SNAPSHOT_STEPS:
@@ -21,7 +22,7 @@ def run_oracle(polarization, with_slab):
         for name, values in sim.fields.items():
             snapshots[f'{name}_{step}'] = values.copy()
     sim.run(duration=THIN_FILM['duration'] - sim.time)
-    return sim, source, monitor, snapshots
+    return sim, source, monitor, snapshots, probes
 
 
 @pytest.mark.parametrize('polarization', ['ez', 'hz'])
@@ -29,7 +30,7 @@ def test_oracle_reproduces_reference_thin_film(polarization):
     golden = np.load(os.path.join(GOLDEN, f'thin_film_{polarization}.npz'))
     flux = {}
     for tag, with_slab in (('film', True), ('empty', False)):
-        sim, source, monitor, snapshots = run_oracle(polarization, with_slab)
+        sim, source, monitor, snapshots, probes = run_oracle(polarization, with_slab)
         assert (sim.nx, sim.ny) == (int(golden[f'{tag}_nx']), int(golden[f'{tag}_ny']))
         assert float(sim.dt) == float(golden[f'{tag}_dt'])
         assert sim.n_step == int(golden[f'{tag}_n_steps'])
@@ -45,6 +46,16 @@ def test_oracle_reproduces_reference_thin_film(polarization):
         np.testing.assert_allclose(monitor.e_spectrum, golden[f'{tag}_e_spectrum'], rtol=1e-12, atol=0)
         flux[tag] = monitor.flux()
         np.testing.assert_allclose(flux[tag], golden[f'{tag}_flux'], rtol=1e-12)
+        # field probes: one sample per step at the time after it; the sample after step s is the reference's
+        # field at the probe's cell after step s
+        for probe in probes:
+            times, values = probe.get_data()
+            assert len(times) == len(values) == sim.n_step
+            assert times[-1] == sim.time
+            np.testing.assert_allclose(np.diff(times), sim.dt, rtol=1e-9)
+            for step in SNAPSHOT_STEPS:
+                want = golden[f'{tag}_{probe.component}_{step}']
+                assert abs(values[step - 1] - want[probe.cell]) <= 1e-12 * np.abs(want).max() + 1e-300
     np.testing.assert_allclose(flux['film'] / flux['empty'], golden['transmission'], rtol=1e-12)
 
 
