@@ -26,9 +26,10 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 4
+#define YEE_ABI_VERSION 5
 #define YEE_MAX_SOURCES 4
 #define YEE_MAX_MONITORS 4
+#define YEE_MAX_DFT 4
 
 #define YEE_OK 0
 #define YEE_EINVAL (-1)   /* invalid argument or descriptor */
@@ -113,6 +114,31 @@ typedef struct yee_probes {
     int64_t capacity;           /* rows of samples: steps n >= capacity record nothing */
 } yee_probes;
 
+/*
+ * Running DFT of up to three field components over a box of sampled cells, with FP64 accumulators, sampled after step
+ * n when (n + 1) % m == 0 (m = the instance's sample_interval). Sampled cell t of the box has grid index
+ * start[a] + stride[a] * t_a on each axis a (t_a < count[a]; a 2D instance uses axes 0 and 1 with count[2] = 1).
+ * Spectrum c samples component comp[c] of the new state set: the value at the cell, or (pair_axis[c] = a >= 0) the
+ * float32 mean 0.5 (f[cell] + f[lower]) with the lower neighbour along a, where the lower neighbour of index 0 is the
+ * last index (pair_wrap[c] = 1: periodic axis) or the cell itself (0). acc[c][f][t] += v exp(-j omega t_s) m dt.
+ * Replaces physics/em/monitors.py:248-322 (DFTMonitor: E_z, or H_z and E_x, E_y at the cell centres, with a
+ * downsample stride) and physics/em3d/monitors3d.py:165-238 (DFTPlane3D: E at the centres of one z layer). The
+ * reference accumulates complex64; H_z, sampled at t - dt/2 there, is re-timed by the caller at readout.
+ */
+typedef struct yee_dft_region {
+    int32_t n_comp;          /* 1..3 spectra */
+    int32_t n_freq;
+    int32_t start[3];
+    int32_t count[3];
+    int32_t stride[3];
+    int32_t comp[3];         /* index into the instance's field components */
+    int32_t pair_axis[3];    /* -1: the value at the cell; 0..2: mean with the lower neighbour along that axis */
+    int32_t pair_wrap[3];
+    const double* phase;     /* [sample s][n_freq][2]: exp(-j omega t) m dt at t = t_{(s+1) m} */
+    int64_t n_samples;
+    double* acc;             /* [n_comp][n_freq][count0 * count1 * count2][2 (re, im)] */
+} yee_dft_region;
+
 typedef struct yee2d_desc {
     int32_t nx, ny;
     int32_t pitch;           /* row stride in elements of every (nx, *) array: a multiple of 4, >= ny */
@@ -145,6 +171,9 @@ typedef struct yee2d_desc {
     int32_t pad2_;
     yee_line_source source[YEE_MAX_SOURCES];
     yee_flux_line monitor[YEE_MAX_MONITORS];
+    int32_t n_dft;
+    int32_t pad3_;
+    yee_dft_region dft[YEE_MAX_DFT];
     yee_probes probes;
     int64_t* clock;          /* device int64: absolute step index n of the next step */
     int64_t step_limit;      /* steps n >= step_limit are skipped (batched instances); INT64_MAX for none */
@@ -242,6 +271,9 @@ typedef struct yee3d_desc {
     int32_t pad_;
     yee_point_source source[YEE3D_MAX_SOURCES];
     yee_flux_plane plane[YEE3D_MAX_PLANES];
+    int32_t n_dft;
+    int32_t pad2_;
+    yee_dft_region dft[YEE_MAX_DFT];
     yee_probes probes;
     int64_t* clock;
     int64_t step_limit;
@@ -252,7 +284,7 @@ typedef struct yee_ctx yee_ctx;
 /* Library ABI version (YEE_ABI_VERSION) and sizeof checks for bindings. */
 int32_t yee_abi_version(void);
 /* 0: yee2d_desc, 1: yee_line_source, 2: yee_flux_line, 3: yee_cpml_axis, 4: yee3d_desc, 5: yee_point_source,
- * 6: yee_flux_plane, 7: yee3d_cpml_axis, 8: yee3d_axis_row, 9: yee_probes */
+ * 6: yee_flux_plane, 7: yee3d_cpml_axis, 8: yee3d_axis_row, 9: yee_probes, 10: yee_dft_region */
 size_t yee_sizeof(int32_t which);
 const char* yee_last_error(void);
 

@@ -6,9 +6,10 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 4
+ABI_VERSION = 5
 MAX_SOURCES = 4
 MAX_MONITORS = 4
+MAX_DFT = 4
 MAX_SOURCES_3D = 4
 MAX_PLANES_3D = 4
 YEE_EZ, YEE_HZ = 0, 1
@@ -38,6 +39,12 @@ class ProbesC(ctypes.Structure):
                 ('capacity', c_i64)]
 
 
+class DftRegionC(ctypes.Structure):
+    _fields_ = [('n_comp', c_i32), ('n_freq', c_i32), ('start', c_i32 * 3), ('count', c_i32 * 3), ('stride', c_i32 * 3),
+                ('comp', c_i32 * 3), ('pair_axis', c_i32 * 3), ('pair_wrap', c_i32 * 3), ('phase', c_ptr),
+                ('n_samples', c_i64), ('acc', c_ptr)]
+
+
 class Desc2DC(ctypes.Structure):
     _fields_ = [('nx', c_i32), ('ny', c_i32), ('pitch', c_i32), ('polarization', c_i32), ('periodic_x', c_i32),
                 ('periodic_y', c_i32), ('rows_per_tile', c_i32), ('lane_columns', c_i32), ('field', (c_ptr * 3) * 2),
@@ -46,6 +53,7 @@ class Desc2DC(ctypes.Structure):
                 ('inv_dx_bulk', c_f32), ('inv_dy_bulk', c_f32), ('layer_x_lo', c_i32), ('layer_x_hi', c_i32),
                 ('layer_y_lo', c_i32), ('layer_y_hi', c_i32), ('cpml', CpmlAxisC * 2), ('n_sources', c_i32), ('n_monitors', c_i32), ('sample_interval', c_i32),
                 ('pad2_', c_i32), ('source', LineSourceC * MAX_SOURCES), ('monitor', FluxLineC * MAX_MONITORS),
+                ('n_dft', c_i32), ('pad3_', c_i32), ('dft', DftRegionC * MAX_DFT),
                 ('probes', ProbesC), ('clock', c_ptr), ('step_limit', c_i64)]
 
 
@@ -74,11 +82,12 @@ class Desc3DC(ctypes.Structure):
                 ('planes_per_tile', c_i32), ('field', (c_ptr * 6) * 2), ('material', c_ptr), ('table', c_ptr),
                 ('n_materials', c_i32), ('dt', c_f32), ('axis_rows', c_ptr * 3), ('cpml', Cpml3DAxisC * 3), ('n_sources', c_i32), ('n_planes', c_i32), ('sample_interval', c_i32),
                 ('pad_', c_i32), ('source', PointSourceC * MAX_SOURCES_3D), ('plane', FluxPlaneC * MAX_PLANES_3D),
+                ('n_dft', c_i32), ('pad2_', c_i32), ('dft', DftRegionC * MAX_DFT),
                 ('probes', ProbesC), ('clock', c_ptr), ('step_limit', c_i64)]
 
 
 STRUCTS = (Desc2DC, LineSourceC, FluxLineC, CpmlAxisC, Desc3DC, PointSourceC, FluxPlaneC, Cpml3DAxisC, AxisRow3DC,
-           ProbesC)
+           ProbesC, DftRegionC)
 
 # Every entry point include/yee.h declares: (name, restype, argtypes)
 SIGNATURES = (

@@ -9,7 +9,7 @@ ROOT = os.path.dirname(PACKAGE)
 CSRC = os.path.join(PACKAGE, 'csrc')
 INCLUDE = os.path.join(ROOT, 'include')
 LIBRARY = os.path.join(PACKAGE, 'libyee.so')
-SOURCES = ('yee2d.hip', 'yee3d.hip', 'yee_api.hip')
+SOURCES = ('yee2d.hip', 'yee3d.hip', 'yee_dft.hip', 'yee_api.hip')
 ARCH = 'gfx950'
 # Packed FP32 instructions are disabled for every HIP source; hipcc -O3 otherwise forms them from float code.
 # -ffp-contract=on: FMAs only within one expression, so every copy of a cell update rounds identically.

@@ -30,6 +30,8 @@ __global__ void yee_clock_advance_batch3d(const yee3d_desc* __restrict__ descs, 
 __global__ void yee_clock_advance(int64_t* clock, int64_t k, int64_t limit);
 __global__ void yee2d_probe_kernel(yee2d_desc d, int32_t offset);
 __global__ void yee3d_probe_kernel(yee3d_desc d, int32_t offset);
+__global__ void yee2d_dft_kernel(yee2d_desc d, int32_t offset);
+__global__ void yee3d_dft_kernel(yee3d_desc d, int32_t offset);
 __global__ void yee_clock_advance_batch(const yee2d_desc* __restrict__ descs, int32_t n_inst, int64_t k);
 
 namespace {
@@ -101,11 +103,50 @@ int32_t check_probes(const yee_probes& p)
     return YEE_OK;
 }
 
+// DFT regions of an instance with grid extents n_axis (2D: n_axis[2] = 1) and n_fields components per state set.
+int32_t check_dft(const yee_dft_region* regions, int32_t n_dft, const int32_t* n_axis, int32_t dims, int32_t n_fields,
+                  int32_t sample_interval)
+{
+    if (n_dft < 0 || n_dft > YEE_MAX_DFT) return fail(YEE_EINVAL, "n_dft out of range");
+    if (n_dft > 0 && sample_interval < 1) return fail(YEE_EINVAL, "sample_interval must be >= 1");
+    for (int32_t k = 0; k < n_dft; ++k) {
+        const yee_dft_region& r = regions[k];
+        if (r.n_comp < 1 || r.n_comp > 3) return fail(YEE_EINVAL, "DFT region n_comp must be 1..3");
+        if (r.n_freq < 1 || !r.phase || !r.acc || r.n_samples < 0) return fail(YEE_EINVAL, "DFT region arrays missing");
+        int64_t cells = 1;
+        for (int a = 0; a < 3; ++a) {
+            if (r.count[a] < 1 || r.stride[a] < 1 || r.start[a] < 0)
+                return fail(YEE_EINVAL, "DFT region needs count, stride >= 1 and start >= 0");
+            if ((int64_t)r.start[a] + (int64_t)r.stride[a] * (r.count[a] - 1) >= n_axis[a])
+                return fail(YEE_EINVAL, "DFT region outside the grid");
+            cells *= r.count[a];
+        }
+        if (cells > ((int64_t)1 << 38)) return fail(YEE_EINVAL, "DFT region too large");
+        for (int c = 0; c < r.n_comp; ++c) {
+            if (r.comp[c] < 0 || r.comp[c] >= n_fields) return fail(YEE_EINVAL, "DFT component out of range");
+            if (r.pair_axis[c] < -1 || r.pair_axis[c] >= dims) return fail(YEE_EINVAL, "DFT pair axis out of range");
+            if (r.pair_wrap[c] != 0 && r.pair_wrap[c] != 1) return fail(YEE_EINVAL, "DFT pair wrap must be 0 or 1");
+        }
+    }
+    return YEE_OK;
+}
+
+// Blocks of 256 lanes along x of a DFT launch: enough for the largest region (blockIdx.y selects the region).
+int dft_blocks(const yee_dft_region* regions, int32_t n_dft)
+{
+    int64_t cells = 0;
+    for (int32_t k = 0; k < n_dft; ++k)
+        cells = std::max(cells, (int64_t)regions[k].count[0] * regions[k].count[1] * regions[k].count[2]);
+    return (int)((cells + 255) / 256);
+}
+
 int32_t check_2d(const yee2d_desc* d)
 {
     if (!d) return fail(YEE_EINVAL, "null descriptor");
     if (check_probes(d->probes) != YEE_OK) return YEE_EINVAL;
     if (d->nx < 2 || d->ny < 2) return fail(YEE_EINVAL, "2D grid needs nx, ny >= 2");
+    const int32_t extents[3] = {d->nx, d->ny, 1};
+    if (check_dft(d->dft, d->n_dft, extents, 2, 3, d->sample_interval) != YEE_OK) return YEE_EINVAL;
     if (d->pitch < d->ny || d->pitch % 4 != 0) return fail(YEE_EINVAL, "pitch must be a multiple of 4 and >= ny");
     if (d->polarization != YEE_EZ && d->polarization != YEE_HZ) return fail(YEE_EINVAL, "polarization must be 0 or 1");
     if (d->rows_per_tile < 1) return fail(YEE_EINVAL, "rows_per_tile must be >= 1");
@@ -159,6 +200,7 @@ int32_t check_3d(const yee3d_desc* d)
     const int n[3] = {d->nx, d->ny, d->nz};
     for (int a = 0; a < 3; ++a)
         if (n[a] < 2) return fail(YEE_EINVAL, "3D grid needs nx, ny, nz >= 2");
+    if (check_dft(d->dft, d->n_dft, n, 3, 6, d->sample_interval) != YEE_OK) return YEE_EINVAL;
     if ((int64_t)d->nx * d->ny * d->nz > ((int64_t)1 << 40)) return fail(YEE_EINVAL, "3D grid too large");
     if ((int64_t)d->ny * d->nz >= ((int64_t)1 << 29)) return fail(YEE_EINVAL, "3D plane (ny * nz) too large");
     if (d->rows_per_tile != kRows3D) return fail(YEE_EINVAL, "rows_per_tile must be 4 (or 0)");
@@ -373,6 +415,9 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
                 hipLaunchKernelGGL(bulk, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
             if (ctx->edge_blocks > 0)
                 hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
+            if (ctx->d2.n_dft > 0)
+                hipLaunchKernelGGL(yee2d_dft_kernel, dim3(dft_blocks(ctx->d2.dft, ctx->d2.n_dft), ctx->d2.n_dft),
+                                   dim3(256), 0, stream, ctx->d2, k);
             if (ctx->d2.probes.n_probes > 0)
                 hipLaunchKernelGGL(yee2d_probe_kernel, dim3(1), dim3(256), 0, stream, ctx->d2, k);
         }
@@ -390,6 +435,9 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
             if (sample_blocks > 0)
                 hipLaunchKernelGGL(yee3d_sample_kernel, dim3(sample_blocks), dim3(256), 0, stream, ctx->d3,
                                    ctx->sample_cells, k);
+            if (ctx->d3.n_dft > 0)
+                hipLaunchKernelGGL(yee3d_dft_kernel, dim3(dft_blocks(ctx->d3.dft, ctx->d3.n_dft), ctx->d3.n_dft),
+                                   dim3(256), 0, stream, ctx->d3, k);
             if (ctx->d3.probes.n_probes > 0)
                 hipLaunchKernelGGL(yee3d_probe_kernel, dim3(1), dim3(256), 0, stream, ctx->d3, k);
         }
@@ -470,6 +518,7 @@ size_t yee_sizeof(int32_t which)
         case 7: return sizeof(yee3d_cpml_axis);
         case 8: return sizeof(yee3d_axis_row);
         case 9: return sizeof(yee_probes);
+        case 10: return sizeof(yee_dft_region);
         default: return 0;
     }
 }
@@ -520,6 +569,8 @@ int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, 
             status = fail(YEE_EINVAL, "batched instances must share lane_columns");
         if (status == YEE_OK && host[k].probes.n_probes > 0)
             status = fail(YEE_EINVAL, "batched instances take no field probes");
+        if (status == YEE_OK && host[k].n_dft > 0)
+            status = fail(YEE_EINVAL, "batched instances take no DFT regions");
         if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
         const int64_t next = prefix[k] + tiles_2d(host[k]);
         if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
@@ -600,6 +651,8 @@ int32_t step_timed_3d(yee_ctx* ctx, int64_t n_steps, hipStream_t s, double* kern
         if (sample_blocks > 0)
             hipLaunchKernelGGL(yee3d_sample_kernel, dim3(sample_blocks), dim3(256), 0, s, d, ctx->sample_cells, 0);
         (void)hipEventRecord(ev[2], s);
+        if (d.n_dft > 0)
+            hipLaunchKernelGGL(yee3d_dft_kernel, dim3(dft_blocks(d.dft, d.n_dft), d.n_dft), dim3(256), 0, s, d, 0);
         if (d.probes.n_probes > 0) hipLaunchKernelGGL(yee3d_probe_kernel, dim3(1), dim3(256), 0, s, d, 0);
         hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, s, d.clock, (int64_t)1, d.step_limit);
         hipError_t e = hipEventSynchronize(ev[2]);
@@ -642,6 +695,9 @@ int32_t yee_step_timed(yee_ctx* ctx, int64_t n_steps, void* stream, double* kern
         if (ctx->edge_blocks > 0)
             hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, s, ctx->d2, ctx->sets, 0);
         (void)hipEventRecord(ev[2], s);
+        if (ctx->d2.n_dft > 0)
+            hipLaunchKernelGGL(yee2d_dft_kernel, dim3(dft_blocks(ctx->d2.dft, ctx->d2.n_dft), ctx->d2.n_dft),
+                               dim3(256), 0, s, ctx->d2, 0);
         if (ctx->d2.probes.n_probes > 0)
             hipLaunchKernelGGL(yee2d_probe_kernel, dim3(1), dim3(256), 0, s, ctx->d2, 0);
         hipLaunchKernelGGL(yee_clock_advance, dim3(1), dim3(64), 0, s, ctx->d2.clock, (int64_t)1,
@@ -730,6 +786,7 @@ int32_t yee3d_batch_create(const yee3d_desc* descs, int32_t n, void* descs_dev, 
         if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
         if (host[k].rows_per_tile != kRows3D) return fail(YEE_EINVAL, "batched 3D instances use 4 rows per tile");
         if (host[k].probes.n_probes > 0) return fail(YEE_EINVAL, "batched instances take no field probes");
+        if (host[k].n_dft > 0) return fail(YEE_EINVAL, "batched instances take no DFT regions");
         const int64_t next = block_prefix[k] + (tiles_3d(host[k]) + 3) / 4;
         if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
         block_prefix[k + 1] = (int32_t)next;

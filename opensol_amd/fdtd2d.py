@@ -15,7 +15,7 @@ from .boundaries import CPMLAxisTables, curl_spacings, resolve_boundaries
 from .constants import C0, DTYPE, EPS0, MU0
 from .domain import Domain
 from .materials import MaterialMap, semi_implicit
-from .monitors import FieldProbe, FluxMonitor, ProbeTable, dft_sample_interval
+from .monitors import DFTMonitor, FieldProbe, FluxMonitor, ProbeTable, dft_sample_interval
 from .sources import LineSource, complex_permittivity, resolve_frequencies, resolve_frequency
 from .waveforms import make_temporal
 
@@ -57,6 +57,7 @@ class Model2D:
         self.sample_interval = 1
         self.pec_regions = []   # bool (nx, ny) masks, applied over the dielectrics at finalize
         self.probes = []
+        self.dft_monitors = []
 
     @property
     def shape(self):
@@ -239,18 +240,36 @@ class Model2D:
         self.monitors.append(monitor)
         return monitor
 
+    def add_dft_monitor(self, x0, y0, x1, y1, wavelengths=None, frequencies=None, downsample=1):
+        """Running DFT over the cells [cell(x_lo), cell(x_hi)] x [cell(y_lo), cell(y_hi)] of a physical box; downsample
+        s > 1 samples every s-th cell along x and y (physics/em/__init__.py:500-509, placement.py:97-105)."""
+        domain = self.domain
+        domain.validate_physical_pos((x0, y0), 'DFT monitor corner 0')
+        domain.validate_physical_pos((x1, y1), 'DFT monitor corner 1')
+        x_lo, x_hi = sorted((float(x0), float(x1)))
+        y_lo, y_hi = sorted((float(y0), float(y1)))
+        monitor = DFTMonitor(slice(domain.pos_to_cell_axis(0, x_lo), domain.pos_to_cell_axis(0, x_hi) + 1),
+                             slice(domain.pos_to_cell_axis(1, y_lo), domain.pos_to_cell_axis(1, y_hi) + 1),
+                             resolve_frequencies(wavelengths, frequencies), float(self.dt), self.polarization,
+                             self.bc.periodic('x'), self.bc.periodic('y'), downsample)
+        self.dft_monitors.append(monitor)
+        return monitor
+
     def finalize(self):
-        """Courant check, sheet calibration and the DFT sample interval (physics/em/__init__.py:181-203)."""
+        """Courant check, sheet calibration and the DFT sample interval shared by every flux and DFT monitor
+        (physics/em/__init__.py:181-203)."""
         if len(self.sources) > _lib.MAX_SOURCES:
             raise ValueError(f'at most {_lib.MAX_SOURCES} line sources per 2D model')
         if len(self.monitors) > _lib.MAX_MONITORS:
             raise ValueError(f'at most {_lib.MAX_MONITORS} flux monitors per 2D model')
+        if len(self.dft_monitors) > _lib.MAX_DFT:
+            raise ValueError(f'at most {_lib.MAX_DFT} DFT monitors per 2D model')
         self.apply_pec()
         self.check_courant()
         self.calibrate()
-        self.sample_interval = dft_sample_interval(float(self.dt), self.sources,
-                                                   [m.frequencies for m in self.monitors])
-        for monitor in self.monitors:
+        spectral = self.monitors + self.dft_monitors
+        self.sample_interval = dft_sample_interval(float(self.dt), self.sources, [m.frequencies for m in spectral])
+        for monitor in spectral:
             monitor.sample_interval = self.sample_interval
 
     def calibrate(self):
@@ -347,8 +366,13 @@ class Engine2D:
                              for m in model.monitors]
         for index, monitor in enumerate(model.monitors):
             monitor.engine, monitor.index = self, index
+        self.dft_accumulators = [torch.zeros(m.accumulator_shape(), dtype=torch.float64, device=dev)
+                                 for m in model.dft_monitors]
+        for index, monitor in enumerate(model.dft_monitors):
+            monitor.engine, monitor.index = self, index
         self.waves = [None] * len(model.sources)
         self.phases = [None] * len(model.monitors)
+        self.dft_phases = [None] * len(model.dft_monitors)
         components = COMPONENTS[model.polarization]
         self.probes = ProbeTable(model.probes, [p.cell[0] * pitch + p.cell[1] for p in model.probes],
                                  [components.index(p.field_component) for p in model.probes], dev)
@@ -389,6 +413,8 @@ class Engine2D:
         sample_times = times[m * np.arange(1, n_samples + 1)]
         self.phases = [torch.from_numpy(np.ascontiguousarray(mon.phase_table(sample_times))).to(self.device)
                        for mon in model.monitors]
+        self.dft_phases = [torch.from_numpy(np.ascontiguousarray(mon.phase_table(sample_times))).to(self.device)
+                           for mon in model.dft_monitors]
         self.probes.ensure(capacity)
         self._times = times
         self.capacity = capacity
@@ -451,6 +477,9 @@ class Engine2D:
             mon.phase = self.phases[k].data_ptr()
             mon.n_samples = self.phases[k].shape[0]
             mon.acc = self.accumulators[k].data_ptr()
+        d.n_dft = len(model.dft_monitors)
+        for k, monitor in enumerate(model.dft_monitors):
+            monitor.fill(d.dft[k], COMPONENTS[model.polarization], self.dft_phases[k], self.dft_accumulators[k])
         self.probes.fill(d.probes)
         d.clock = self.clock.data_ptr()
         d.step_limit = _lib.INT64_MAX
@@ -524,6 +553,9 @@ class Engine2D:
 
     def read_accumulator(self, index):
         return self.accumulators[index].cpu().numpy()
+
+    def read_dft_accumulator(self, index):
+        return self.dft_accumulators[index].cpu().numpy()
 
     def close(self):
         if self._ctx:

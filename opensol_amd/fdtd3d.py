@@ -17,7 +17,7 @@ from .constants import C0, DTYPE, EPS0, MU0
 from .domain import Domain
 from .fdtd2d import clock_times, steps_for_duration
 from .materials import MaterialMap, semi_implicit
-from .monitors import FieldProbe, FluxPlane, ProbeTable, dft_sample_interval
+from .monitors import DFTPlane3D, FieldProbe, FluxPlane, ProbeTable, dft_sample_interval
 from .sources import PointDipole, resolve_frequencies, resolve_frequency
 from .waveforms import make_temporal
 
@@ -72,6 +72,7 @@ class Model3D:
         self.sample_interval = 1
         self.pec_boxes = []   # index ranges of PEC boxes, painted over the dielectrics at finalize
         self.probes = []
+        self.dft_monitors = []
 
     def axis_rows(self, axis):
         """(n, 8) float32 records of one axis (the yee3d_axis_row layout): inv_h, inv_e, psi row of the hi-face
@@ -223,6 +224,15 @@ class Model3D:
         self.monitors.append(monitor)
         return monitor
 
+    def add_dft_plane(self, z, wavelengths=None, frequencies=None):
+        """Running DFT of E at the cell centres of the window layer containing height z [m]
+        (physics/em3d/physics3d.py:478-486)."""
+        self.domain.validate_physical_pos((0.0, 0.0, z), 'DFT plane')
+        monitor = DFTPlane3D(self.domain.physical_window(), self.domain.pos_to_cell_axis(2, z),
+                             resolve_frequencies(wavelengths, frequencies), float(self.dt), self.periodic)
+        self.dft_monitors.append(monitor)
+        return monitor
+
     # ---- coefficients ---------------------------------------------------------------------------------------------
     def table(self):
         """(n_materials, 8) float32 per id: (da, db, ca, cb, eps_r, sigma_e, 0, 0) -- the H edge coefficients
@@ -260,13 +270,15 @@ class Model3D:
             raise ValueError(f'at most {_lib.MAX_SOURCES_3D} dipoles per 3D model')
         if len(self.monitors) > _lib.MAX_PLANES_3D:
             raise ValueError(f'at most {_lib.MAX_PLANES_3D} flux planes per 3D model')
+        if len(self.dft_monitors) > _lib.MAX_DFT:
+            raise ValueError(f'at most {_lib.MAX_DFT} DFT planes per 3D model')
         self.apply_pec()
         self.check_courant()
         for source in self.sources:
             source.calibrate(self.face_cb(source.axis, source.cell))
-        self.sample_interval = dft_sample_interval(float(self.dt), self.sources,
-                                                   [m.frequencies for m in self.monitors])
-        for monitor in self.monitors:
+        spectral = self.monitors + self.dft_monitors
+        self.sample_interval = dft_sample_interval(float(self.dt), self.sources, [m.frequencies for m in spectral])
+        for monitor in spectral:
             monitor.sample_interval = self.sample_interval
 
 
@@ -306,8 +318,13 @@ class Engine3D:
                              for m in model.monitors]
         for index, monitor in enumerate(model.monitors):
             monitor.engine, monitor.index = self, index
+        self.dft_accumulators = [torch.zeros(m.accumulator_shape(), dtype=torch.float64, device=dev)
+                                 for m in model.dft_monitors]
+        for index, monitor in enumerate(model.dft_monitors):
+            monitor.engine, monitor.index = self, index
         self.waves = [None] * len(model.sources)
         self.phases = [None] * len(model.monitors)
+        self.dft_phases = [None] * len(model.dft_monitors)
         ny, nz = shape[1], shape[2]
         self.probes = ProbeTable(model.probes, [(p.cell[0] * ny + p.cell[1]) * nz + p.cell[2] for p in model.probes],
                                  [COMPONENTS.index(p.field_component) for p in model.probes], dev)
@@ -340,6 +357,8 @@ class Engine3D:
         sample_times = times[m * np.arange(1, n_samples + 1)]
         self.phases = [torch.from_numpy(np.ascontiguousarray(mon.phase_table(sample_times))).to(self.device)
                        for mon in model.monitors]
+        self.dft_phases = [torch.from_numpy(np.ascontiguousarray(mon.phase_table(sample_times))).to(self.device)
+                           for mon in model.dft_monitors]
         self.probes.ensure(capacity)
         self._times = times
         self.capacity = capacity
@@ -396,6 +415,9 @@ class Engine3D:
             pl.n_freq = monitor.n_freqs
             pl.phase = self.phases[k].data_ptr()
             pl.n_samples = self.phases[k].shape[0]
+        d.n_dft = len(model.dft_monitors)
+        for k, monitor in enumerate(model.dft_monitors):
+            monitor.fill(d.dft[k], COMPONENTS, self.dft_phases[k], self.dft_accumulators[k])
         self.probes.fill(d.probes)
         d.clock = self.clock.data_ptr()
         d.step_limit = _lib.INT64_MAX
@@ -465,6 +487,9 @@ class Engine3D:
 
     def read_plane_accumulator(self, index, pair):
         return self.accumulators[index][pair].cpu().numpy()
+
+    def read_dft_accumulator(self, index):
+        return self.dft_accumulators[index].cpu().numpy()
 
     def close(self):
         if self._ctx:

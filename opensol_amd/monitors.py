@@ -228,3 +228,139 @@ class ProbeTable:
 
     def series(self, index, n_step):
         return self.samples[:n_step, index].cpu().numpy()
+
+
+def dft_sample_slice(start, stop, stride):
+    """Cells a DFT box samples along one axis of [start, stop): every stride-th cell from the middle of the first
+    stride-cell block (physics/em/monitors.py:152-155)."""
+    return slice(start + min(stride, stop - start) // 2, stop, stride)
+
+
+class _DFTRegion:
+    """Device record of a running DFT over a box of sampled cells (the yee_dft_region record): FP64 (re, im)
+    accumulators [component][frequency][cells], phases exp(-j omega t) m dt at the sample times (the flux monitors'
+    tables). Subclasses name the components and their pairing."""
+
+    names = ()
+
+    def _setup(self, frequencies, dt):
+        self.frequencies = np.atleast_1d(np.asarray(frequencies, dtype=np.float64))
+        if self.frequencies.ndim != 1 or not self.frequencies.size or not np.all(np.isfinite(self.frequencies)) \
+                or np.any(self.frequencies <= 0):
+            raise ValueError(f'DFT frequencies must be positive and finite, got {frequencies!r}')
+        self.omega = 2.0 * np.pi * self.frequencies
+        self.n_freqs = self.frequencies.size
+        self.dt = float(dt)
+        self.sample_interval = 1
+        self.engine = None
+        self.index = None
+
+    @property
+    def sample_counts(self):
+        return tuple(len(range(s.start, s.stop, s.step)) for s in self.samples)
+
+    def accumulator_shape(self):
+        return (len(self.names), self.n_freqs) + self.sample_counts + (2,)
+
+    def phase_table(self, sample_times):
+        phase = -np.outer(np.asarray(sample_times, dtype=np.float64), self.omega)
+        scale = self.dt * self.sample_interval
+        return np.stack([np.cos(phase) * scale, np.sin(phase) * scale], axis=-1)
+
+    def fill(self, region, components, phase, acc):
+        """Fill a yee_dft_region (DftRegionC) for an engine whose field components are `components`."""
+        samples = list(self.samples) + [slice(0, 1, 1)] * (3 - len(self.samples))
+        region.n_comp = len(self.names)
+        region.n_freq = self.n_freqs
+        for a, s in enumerate(samples):
+            region.start[a] = s.start
+            region.count[a] = len(range(s.start, s.stop, s.step))
+            region.stride[a] = s.step
+        for c, name in enumerate(self.names):
+            axis, wrap = self.pairs[c]
+            region.comp[c] = components.index(name)
+            region.pair_axis[c] = axis
+            region.pair_wrap[c] = int(wrap)
+        region.phase = phase.data_ptr()
+        region.n_samples = phase.shape[0]
+        region.acc = acc.data_ptr()
+
+    def _spectra(self):
+        """complex128 (component, n_freq, *sample counts) spectra from the device accumulators."""
+        if self.engine is None:
+            return np.zeros(self.accumulator_shape()[:-1], dtype=np.complex128)
+        acc = self.engine.read_dft_accumulator(self.index)
+        return acc[..., 0] + 1j * acc[..., 1]
+
+
+class DFTMonitor(_DFTRegion):
+    """Running DFT over the cell box (x_slice, y_slice) at each frequency (physics/em/monitors.py:248-322): E_z
+    ('ez'), or H_z plus E_x / E_y averaged to the cell centres ('hz': E_x with its lower y face, E_y with its lower x
+    face; at index 0 the wrapped last face on a periodic axis, else the face itself); downsample s > 1 samples every
+    s-th cell along x and y. ez_dft / hz_dft / ex_dft / ey_dft are complex128 (n_freq, mx, my) spectra (the
+    reference's are complex64); H_z is re-timed from the E sample time to its own, t - dt/2."""
+
+    def __init__(self, x_slice, y_slice, frequencies, dt, polarization='ez', periodic_x=False, periodic_y=False,
+                 downsample=1):
+        for label, cells in (('x_slice', x_slice), ('y_slice', y_slice)):
+            if cells.step not in (None, 1) or cells.stop is None or not (cells.start or 0) < cells.stop:
+                raise ValueError(f'DFTMonitor {label} must be a non-empty unit-step slice, got {cells}')
+        if isinstance(downsample, bool) or int(downsample) != downsample or downsample < 1:
+            raise ValueError(f'DFTMonitor downsample must be an int >= 1, got {downsample!r}')
+        if polarization not in ('ez', 'hz'):
+            raise ValueError(f"DFTMonitor polarization must be 'ez' or 'hz', got {polarization!r}")
+        self._setup(frequencies, dt)
+        self.polarization = polarization
+        self.x_slice = slice(x_slice.start or 0, x_slice.stop)
+        self.y_slice = slice(y_slice.start or 0, y_slice.stop)
+        self.downsample = int(downsample)
+        self.x_samples = dft_sample_slice(self.x_slice.start, self.x_slice.stop, self.downsample)
+        self.y_samples = dft_sample_slice(self.y_slice.start, self.y_slice.stop, self.downsample)
+        self.samples = (self.x_samples, self.y_samples)
+        self.mx, self.my = self.sample_counts
+        if polarization == 'ez':
+            self.names, self.pairs = ('ez',), ((-1, False),)
+        else:
+            self.names = ('hz', 'ex', 'ey')
+            self.pairs = ((-1, False), (1, bool(periodic_y)), (0, bool(periodic_x)))
+
+    def _component(self, name):
+        values = self._spectra()[self.names.index(name)]
+        if name == 'hz':   # sampled with the E-time phase; the reference's H_z phase is at t - dt/2
+            values = values * np.exp(1j * np.pi * self.frequencies * self.dt)[:, None, None]
+        return values
+
+    def __getattr__(self, name):
+        if name in ('ez_dft', 'hz_dft', 'ex_dft', 'ey_dft') and name[:2] in self.__dict__.get('names', ()):
+            return self._component(name[:2])
+        raise AttributeError(name)
+
+
+class DFTPlane3D(_DFTRegion):
+    """Running DFT of E_x, E_y, E_z averaged to the cell centres of z layer k_grid over the window
+    (physics/em3d/monitors3d.py:165-238): E_x with its lower x face, E_y with its lower y face, E_z with the layer
+    below (at index 0: the wrapped last face or layer on a periodic axis, else the face itself)."""
+
+    names = ('ex', 'ey', 'ez')
+
+    def __init__(self, window, k_grid, frequencies, dt, periodic):
+        self._setup(frequencies, dt)
+        self.window = (slice(window[0].start, window[0].stop), slice(window[1].start, window[1].stop))
+        self.k_grid = int(k_grid)
+        self.shape = tuple(w.stop - w.start for w in self.window)
+        self.samples = (slice(self.window[0].start, self.window[0].stop, 1),
+                        slice(self.window[1].start, self.window[1].stop, 1), slice(self.k_grid, self.k_grid + 1, 1))
+        self.pairs = ((0, bool(periodic[0])), (1, bool(periodic[1])), (2, bool(periodic[2])))
+
+    @property
+    def spectra(self):
+        """{'ex', 'ey', 'ez'}: complex128 (n_freq, nx window, ny window) centre phasors."""
+        values = self._spectra()[..., 0]
+        return {name: values[c] for c, name in enumerate(self.names)}
+
+    def spectrum(self, name, index):
+        return self.spectra[name][index]
+
+    def intensity(self, index):
+        """float64 |E_x|^2 + |E_y|^2 + |E_z|^2 at frequency `index`."""
+        return sum(np.abs(self.spectrum(name, index)) ** 2 for name in self.names)

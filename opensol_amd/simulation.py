@@ -102,6 +102,12 @@ class Simulation:
         self._require_unbuilt('add_field_probe')
         return self.model.add_field_probe(x, y, field_component)
 
+    def add_dft_monitor(self, x0, y0, x1, y1, wavelengths=None, frequencies=None, downsample=1):
+        """Running DFT over the cell box of (x0, y0)-(x1, y1); downsample s > 1 samples every s-th cell along x and y
+        (physics/em/__init__.py:500-509)."""
+        self._require_unbuilt('add_dft_monitor')
+        return self.model.add_dft_monitor(x0, y0, x1, y1, wavelengths, frequencies, downsample)
+
     def add_flux_monitor(self, axis, position, span, wavelengths=None, frequencies=None):
         self._require_unbuilt('add_flux_monitor')
         return self.model.add_flux_monitor(axis, position, span, wavelengths, frequencies)
@@ -243,6 +249,11 @@ class Simulation3D:
         (physics/em3d/physics3d.py:425-433)."""
         self._require_unbuilt('add_field_probe')
         return self.model.add_field_probe(x, y, z, component)
+
+    def add_dft_plane(self, z, wavelengths=None, frequencies=None):
+        """Running DFT of E at the cell centres of the window layer containing height z (physics3d.py:478-486)."""
+        self._require_unbuilt('add_dft_plane')
+        return self.model.add_dft_plane(z, wavelengths, frequencies)
 
     def add_flux_plane(self, normal, position, wavelengths=None, frequencies=None):
         self._require_unbuilt('add_flux_plane')

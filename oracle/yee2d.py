@@ -358,6 +358,54 @@ class FluxMonitor:
         return integrand @ self.widths
 
 
+def centre_partners(indices, n, periodic):
+    """Lower-face partner of each sampled index: index - 1, and for index 0 the wrapped last face (periodic) or the
+    face itself (em/monitors.py:158-169, yee_grid.lower_face)."""
+    partners = np.asarray(indices) - 1
+    partners[np.asarray(indices) == 0] = n - 1 if periodic else 0
+    return partners
+
+
+class DFTBox:
+    """Running DFT over a cell box (em/monitors.py:248-322): E_z, or H_z (phased at t - dt/2) with E_x, E_y averaged
+    to the cell centres; every downsample-th cell from the middle of the first block (:152-155). Spectra accumulate
+    in complex64, as the reference's, when phase_dtype is complex64, else in complex128."""
+
+    def __init__(self, x_cells, y_cells, frequencies, dt, polarization, periodic, downsample, shape, phase_dtype):
+        (x0, x1), (y0, y1) = x_cells, y_cells
+        s = int(downsample)
+        self.x_samples = slice(x0 + min(s, x1 - x0) // 2, x1, s)
+        self.y_samples = slice(y0 + min(s, y1 - y0) // 2, y1, s)
+        self.frequencies = np.asarray(frequencies, dtype=np.float64)
+        self.omega = 2.0 * np.pi * self.frequencies
+        self.dt = float(dt)
+        self.polarization = polarization
+        self.phase_dtype = phase_dtype
+        self.sample_interval = 1
+        xs = np.arange(shape[0])[self.x_samples]
+        ys = np.arange(shape[1])[self.y_samples]
+        self.xs, self.ys = xs, ys
+        self.x_partners = centre_partners(xs, shape[0], periodic[0])
+        self.y_partners = centre_partners(ys, shape[1], periodic[1])
+        names = ('ez',) if polarization == 'ez' else ('hz', 'ex', 'ey')
+        acc_dtype = np.complex64 if phase_dtype == np.complex64 else np.complex128
+        self.spectra = {name: np.zeros((self.frequencies.size, xs.size, ys.size), dtype=acc_dtype) for name in names}
+
+    phases = FluxMonitor.phases
+
+    def update(self, fields, t):
+        ph = self.phases(t)[:, None, None]
+        box = np.ix_(self.xs, self.ys)
+        half = F32(0.5)
+        if self.polarization == 'ez':
+            self.spectra['ez'] += ph * fields['ez'][box]
+            return
+        self.spectra['hz'] += self.phases(t - 0.5 * self.dt)[:, None, None] * fields['hz'][box]
+        ex, ey = fields['ex'], fields['ey']
+        self.spectra['ex'] += ph * (half * (ex[box] + ex[np.ix_(self.xs, self.y_partners)]))
+        self.spectra['ey'] += ph * (half * (ey[box] + ey[np.ix_(self.x_partners, self.ys)]))
+
+
 class PointProbe:
     """Raw Yee samples of one component at one cell after every step, stamped with the time after the step
     (core/monitors.py:194-283; engine3d.py:544-548 records the probe rows after the step and stamps self.time)."""
@@ -536,6 +584,25 @@ class Yee2D:
         probe = PointProbe((self.geo.cell(0, x), self.geo.cell(1, y)), component)
         self.probes.append(probe)
         return probe
+
+    def add_dft_monitor(self, x0, y0, x1, y1, wavelengths=None, frequencies=None, downsample=1):
+        """Cells [cell(x_lo), cell(x_hi)] x [cell(y_lo), cell(y_hi)] (placement.py:97-105, physics/em/__init__.py:
+        500-509)."""
+        geo = self.geo
+        if (wavelengths is None) == (frequencies is None):
+            raise ValueError('pass exactly one of wavelengths or frequencies')
+        freqs = (np.asarray(frequencies, dtype=np.float64) if frequencies is not None
+                 else C0 / np.asarray(wavelengths, dtype=np.float64))
+        geo.validate(x0, y0, 'DFT monitor corner 0')
+        geo.validate(x1, y1, 'DFT monitor corner 1')
+        x_lo, x_hi = sorted((x0, x1))
+        y_lo, y_hi = sorted((y0, y1))
+        monitor = DFTBox((geo.cell(0, x_lo), geo.cell(0, x_hi) + 1), (geo.cell(1, y_lo), geo.cell(1, y_hi) + 1),
+                         np.atleast_1d(freqs), float(geo.dt), self.polarization,
+                         (self.bc.periodic('x'), self.bc.periodic('y')), downsample, geo.shape, self.phase_dtype)
+        self.monitors.append(monitor)
+        self._finalized = False
+        return monitor
 
     def add_flux_monitor(self, axis, position, span, wavelengths=None, frequencies=None):
         """placement.py:55-70, physics/em/__init__.py:511-518."""

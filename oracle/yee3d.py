@@ -29,8 +29,8 @@ import math
 
 import numpy as np
 
-from .yee2d import (C0, EPS0, F32, MU0, PointProbe, containing_cell, cpml_recursion, cpml_side_rows,
-                    dft_sample_interval, make_temporal, resolve_frequency, uniform_cell_count)
+from .yee2d import (C0, EPS0, F32, MU0, FluxMonitor, PointProbe, centre_partners, containing_cell, cpml_recursion,
+                    cpml_side_rows, dft_sample_interval, make_temporal, resolve_frequency, uniform_cell_count)
 
 AXES = ('x', 'y', 'z')
 FACES_3D = ('x_lo', 'x_hi', 'y_lo', 'y_hi', 'z_lo', 'z_hi')
@@ -230,6 +230,42 @@ class PlaneFlux:
         return np.cos(np.pi * self.frequencies * self.dt) * total
 
 
+class DFTPlane:
+    """Running DFT of E_x, E_y, E_z at the cell centres of window layer k (monitors3d.py:165-238): E_x with its lower
+    x face, E_y with its lower y face, E_z with the layer below (index 0: the wrapped last one on a periodic axis,
+    else itself). complex64 accumulation, as the reference's, when phase_dtype is complex64."""
+
+    def __init__(self, windows, k, frequencies, dt, shape, periodic, phase_dtype):
+        self.xs = np.arange(windows[0].start, windows[0].stop)
+        self.ys = np.arange(windows[1].start, windows[1].stop)
+        self.k = int(k)
+        self.x_partners = centre_partners(self.xs, shape[0], periodic[0])
+        self.y_partners = centre_partners(self.ys, shape[1], periodic[1])
+        self.k_lower = int(centre_partners([self.k], shape[2], periodic[2])[0])
+        self.frequencies = np.asarray(frequencies, dtype=np.float64)
+        self.omega = 2.0 * np.pi * self.frequencies
+        self.dt = float(dt)
+        self.phase_dtype = phase_dtype
+        self.sample_interval = 1
+        acc_dtype = np.complex64 if phase_dtype == np.complex64 else np.complex128
+        self.spectra = {name: np.zeros((self.frequencies.size, self.xs.size, self.ys.size), dtype=acc_dtype)
+                        for name in ('ex', 'ey', 'ez')}
+
+    phases = FluxMonitor.phases
+
+    def update(self, fields, t):
+        ph = self.phases(t)[:, None, None]
+        half = F32(0.5)
+        box = np.ix_(self.xs, self.ys)
+        ex, ey, ez = (fields[name][:, :, self.k] for name in ('ex', 'ey', 'ez'))
+        self.spectra['ex'] += ph * (half * (ex[box] + ex[np.ix_(self.x_partners, self.ys)]))
+        self.spectra['ey'] += ph * (half * (ey[box] + ey[np.ix_(self.xs, self.y_partners)]))
+        self.spectra['ez'] += ph * (half * (ez[box] + fields['ez'][:, :, self.k_lower][box]))
+
+    def intensity(self, index):
+        return sum(np.abs(self.spectra[name][index].astype(np.complex128)) ** 2 for name in ('ex', 'ey', 'ez'))
+
+
 class Yee3D:
     """3D FDTD model (physics/em3d/physics3d.py:144-445, engine3d.py:162-554) stepped on the CPU in `dtype`.
     size_* is the physical window; every cpml face adds npml cells of width dx."""
@@ -400,6 +436,19 @@ class Yee3D:
         probe = PointProbe(tuple(self.cell(axis, v) for axis, v in enumerate((x, y, z))), component)
         self.probes.append(probe)
         return probe
+
+    def add_dft_plane(self, z, wavelengths=None, frequencies=None):
+        """DFT of E at the cell centres of the window layer containing z (physics3d.py:478-486)."""
+        if (wavelengths is None) == (frequencies is None):
+            raise ValueError('pass exactly one of wavelengths or frequencies')
+        freqs = (np.asarray(frequencies, dtype=np.float64) if frequencies is not None
+                 else C0 / np.asarray(wavelengths, dtype=np.float64))
+        self.validate((0.0, 0.0, z), 'DFT plane')
+        monitor = DFTPlane(self.windows, self.cell(2, z), np.atleast_1d(freqs), float(self.dt), self.shape,
+                           self.periodic, self.phase_dtype)
+        self.monitors.append(monitor)
+        self._finalized = False
+        return monitor
 
     def add_flux_plane(self, normal, position, wavelengths=None, frequencies=None):
         """Flux through the cell-edge plane nearest `position` across the whole window (physics3d.py:434-445)."""

@@ -21,7 +21,14 @@ PEC2D = dict(size_x=2e-6, size_y=1e-6, dx=25e-9, npml=10, pec_rect=(0.9e-6, 0.3e
              source=dict(axis='y', position=1.0e-6, wavelength=600e-9, bandwidth=0.8),
              monitor=dict(axis='y', position=1.7e-6, span=(0.0, 1e-6), wavelengths=[500e-9, 600e-9, 700e-9]),
              faces={'ez': ('pec', 'cpml', 'periodic', 'periodic'), 'hz': ('cpml', 'cpml', 'periodic', 'periodic')},
-             steps=(150, 500))
+             steps=(150, 500),
+             dft_boxes=(((0.2e-6, 0.0, 1.8e-6, 0.5e-6), 1), ((0.3e-6, 0.1e-6, 1.9e-6, 0.9e-6), 3)))
+
+
+def add_pec2d_dft_boxes(sim):
+    """The PEC case's two DFT box monitors (full resolution from y cell 0; downsampled by 3)."""
+    wavelengths = PEC2D['monitor']['wavelengths']
+    return [sim.add_dft_monitor(*box, wavelengths=wavelengths, downsample=s) for box, s in PEC2D['dft_boxes']]
 
 
 def build_pec2d(target, polarization, boundary_type, **options):

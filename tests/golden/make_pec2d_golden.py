@@ -7,7 +7,10 @@ Cases (window 2 um x 1 um, dx 25 nm, 10-cell CPML, y periodic):
   ez: x_lo 'pec' face, x_hi CPML; a PEC rectangle (add_pec_region on the reference's rectangle raster) partly covered
       by a dielectric painted after it; a Gaussian line source along y that crosses the PEC rectangle; a flux line.
   hz: CPML on both x faces; the same PEC rectangle and dielectric; the line source and flux line.
-Fields are recorded after 150 and 500 steps, with the flux spectra and the source profile.
+Fields are recorded after 150 and 500 steps, with the flux spectra and the source profile. Two running-DFT box
+monitors at the flux line's wavelengths are recorded after 500 steps: one over the whole lower half of the window at
+full resolution (it starts at y cell 0, where the periodic wrap pairs the H_z case's E_x with the last face), one
+downsampled by 3.
 
 Run in this container only (the reference is not present on the GPU box):
     python tests/golden/make_pec2d_golden.py
@@ -28,6 +31,7 @@ DIELECTRIC = ((1.0e-6, 0.0, 1.6e-6, 1e-6), 3.0)
 SOURCE_X, LAMBDA0, BANDWIDTH = 1.0e-6, 600e-9, 0.8
 MONITOR_X, WAVELENGTHS = 1.7e-6, (500e-9, 600e-9, 700e-9)
 STEPS = (150, 500)
+DFT_BOXES = (((0.2e-6, 0.0, 1.8e-6, 0.5e-6), 1), ((0.3e-6, 0.1e-6, 1.9e-6, 0.9e-6), 3))
 FACES = {'ez': ('pec', 'cpml', 'periodic', 'periodic'), 'hz': ('cpml', 'cpml', 'periodic', 'periodic')}
 
 
@@ -49,6 +53,8 @@ def run_case(polarization):
     em.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
     em.add_line_source(axis='y', position=SOURCE_X, wavelength=LAMBDA0, bandwidth=BANDWIDTH, waveform='gaussian')
     monitor = em.add_flux_monitor(axis='y', position=MONITOR_X, span=(0.0, WINDOW_Y), wavelengths=list(WAVELENGTHS))
+    boxes = [em.add_dft_monitor(*box, wavelengths=list(WAVELENGTHS), downsample=downsample)
+             for box, downsample in DFT_BOXES]
     names = ('ez', 'hx', 'hy') if polarization == 'ez' else ('hz', 'ex', 'ey')
     out = {'pec_mask': mask}
     done = 0
@@ -62,6 +68,11 @@ def run_case(polarization):
                 'sample_interval': monitor.sample_interval,
                 'flux': np.asarray(monitor.flux(), dtype=np.float64),
                 'source_profile': np.asarray(em.sources[0].profile, dtype=np.float64)})
+    for index, box in enumerate(boxes):
+        out[f'dft{index}_samples'] = np.array([[box.x_samples.start, box.x_samples.stop, box.x_samples.step],
+                                               [box.y_samples.start, box.y_samples.stop, box.y_samples.step]])
+        for name in (('ez',) if polarization == 'ez' else ('hz', 'ex', 'ey')):
+            out[f'dft{index}_{name}'] = np.array(getattr(box, f'{name}_dft'))
     return out
 
 

@@ -225,3 +225,30 @@ def test_field_probes_match_oracle(gpu, polarization):
         error = np.max(np.abs(values - ref_values)) / peak
         print(f'{polarization} probe {probe.cell} {probe.field_component}: max err / peak {error:.3e}')
         assert error <= 1e-5, (probe.cell, error)
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_dft_boxes_match_oracle(gpu, polarization):
+    """The PEC case's DFT boxes (full resolution from y cell 0, where H_z's E_x wraps to the last face; downsampled
+    by 3) against the FP64 oracle after 500 steps: every spectrum within 1e-5 of its peak (relative L2 <= 1e-5);
+    graph replay bit-identical to eager stepping; box cells and H_z re-timing as the reference's."""
+    from tests.cases import PEC2D, add_pec2d_dft_boxes, build_pec2d
+    Simulation = simulation_class()
+    ref, _, _ = build_pec2d(Yee2D, polarization, _oracle_boundaries, phase_dtype=np.complex128)
+    sim, _, _ = build_pec2d(Simulation, polarization, _product_boundaries)
+    graphed, _, _ = build_pec2d(Simulation, polarization, _product_boundaries, graph_steps=16)
+    ref_boxes, boxes, graphed_boxes = (add_pec2d_dft_boxes(s) for s in (ref, sim, graphed))
+    steps = PEC2D['steps'][-1]
+    ref.run(n_steps=steps)
+    sim.run(n_steps=137, print_every=0)   # two runs: the phase tables grow in between
+    sim.run(n_steps=steps - 137, print_every=0)
+    graphed.run(n_steps=steps, print_every=0)
+    assert sim.model.sample_interval == ref.monitors[0].sample_interval
+    for index, (box, graphed_box, ref_box) in enumerate(zip(boxes, graphed_boxes, ref_boxes)):
+        for name, want in ref_box.spectra.items():
+            got = getattr(box, f'{name}_dft')
+            np.testing.assert_array_equal(getattr(graphed_box, f'{name}_dft'), got)
+            peak_error = np.max(np.abs(got - want)) / np.max(np.abs(want))
+            l2 = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+            print(f'{polarization} box {index} {name}: max err / peak {peak_error:.3e}, rel L2 {l2:.3e}')
+            assert peak_error <= 1e-5 and l2 <= 1e-5, (index, name, peak_error, l2)

@@ -16,12 +16,12 @@ GOLDEN_BOXES = (((0.05e-6, 0.05e-6, 0.00e-6), (0.30e-6, 0.35e-6, 0.25e-6), 4.0),
                 ((0.20e-6, 0.40e-6, 0.30e-6), (0.50e-6, 0.55e-6, 0.55e-6), 6.5))
 GOLDEN_CASES = {
     'cpml': dict(faces=None, dipole=(0.36e-6, 0.29e-6, 0.27e-6), orientation='z',
-                 planes=(('x', 0.55e-6), ('z', 0.1e-6))),
+                 planes=(('x', 0.55e-6), ('z', 0.1e-6)), dft_planes=(0.27e-6,)),
     'periodic_y': dict(faces={'y_lo': 'periodic', 'y_hi': 'periodic'}, dipole=(0.33e-6, 0.21e-6, 0.26e-6),
-                       orientation='x', planes=(('x', 0.15e-6), ('z', 0.45e-6))),
+                       orientation='x', planes=(('x', 0.15e-6), ('z', 0.45e-6)), dft_planes=(0.26e-6, 0.5e-6)),
     'pec': dict(faces={'z_lo': 'pec', 'z_hi': 'pmc'}, dipole=(0.36e-6, 0.29e-6, 0.27e-6), orientation='z',
                 planes=(('x', 0.2e-6), ('z', 0.45e-6)),
-                pec_box=((0.45e-6, 0.10e-6, 0.30e-6), (0.60e-6, 0.35e-6, 0.45e-6))),
+                pec_box=((0.45e-6, 0.10e-6, 0.30e-6), (0.60e-6, 0.35e-6, 0.45e-6)), dft_planes=(0.0, 0.27e-6)),
 }
 
 
@@ -45,6 +45,8 @@ def test_dipole_case_matches_oracle(gpu, name):
     spec = GOLDEN_CASES[name]
     sim, planes = _build(Simulation3D, spec)
     ref, rplanes = _build(lambda *a, **k: Yee3D(*a, **k), spec)
+    dft_planes = [sim.add_dft_plane(z, wavelengths=[0.5e-6, 0.6e-6, 0.7e-6]) for z in spec['dft_planes']]
+    ref_dft_planes = [ref.add_dft_plane(z, wavelengths=[0.5e-6, 0.6e-6, 0.7e-6]) for z in spec['dft_planes']]
     ref.finalize()
     sim.run(n_steps=120)
     ref.run(n_steps=120)
@@ -66,6 +68,20 @@ def test_dipole_case_matches_oracle(gpu, name):
         rel = np.max(np.abs(got - want) / np.abs(want))
         print(f'{name} flux {got} oracle {want} rel {rel:.2e}')
         assert rel < 1e-5
+    # DFT planes of E at the cell centres: relative L2 <= 1e-5 per component (exactly 0 where the oracle's is, the
+    # tangential E of a PEC face)
+    for plane, rplane in zip(dft_planes, ref_dft_planes):
+        assert plane.k_grid == rplane.k
+        for component in ('ex', 'ey', 'ez'):
+            got, want = plane.spectra[component], rplane.spectra[component]
+            if not np.any(want):
+                assert not np.any(got), (plane.k_grid, component)
+                continue
+            err = np.linalg.norm(got - want) / np.linalg.norm(want)
+            print(f'{name} DFT plane k {plane.k_grid} {component}: rel L2 {err:.2e}')
+            assert err < 1e-5, (plane.k_grid, component, err)
+        np.testing.assert_allclose(plane.intensity(1), rplane.intensity(1), rtol=2e-5,
+                                   atol=2e-5 * np.max(rplane.intensity(1)))
 
 
 # (window cells x, y, z, npml, faces, planes_per_tile)

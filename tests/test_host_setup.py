@@ -186,3 +186,34 @@ def test_field_probe_cells_match_the_oracle(polarization):
     with pytest.raises(ValueError):
         model.add_field_probe(THIN_FILM['size_x'] + 1e-7, 1e-6)
     assert len(THIN_FILM_PROBES) + 1 == len(model.probes)
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_dft_monitor_cells_match_the_oracle(polarization):
+    """DFT box placement (the containing cells of both corners, in either order), the downsampled sample cells
+    (physics/em/monitors.py:152-155) and, for H_z, the lower-face partners of E_x and E_y (periodic wrap at index 0)."""
+    from tests.cases import PEC2D, add_pec2d_dft_boxes, build_pec2d
+    from oracle.yee2d import Boundaries, FACES
+    from opensol_amd.boundaries import BoundaryConfig
+
+    def product_boundaries(faces, npml):
+        bc = BoundaryConfig(*faces)
+        bc.cpml_npml = npml
+        return bc
+
+    model, _, _ = build_pec2d(Model2D, polarization, product_boundaries)
+    ref, _, _ = build_pec2d(Yee2D, polarization, lambda faces, npml: Boundaries(dict(zip(FACES, faces)), npml=npml))
+    boxes, ref_boxes = add_pec2d_dft_boxes(model), add_pec2d_dft_boxes(ref)
+    boxes.append(model.add_dft_monitor(1.5e-6, 1e-6, 0.4e-6, 0.0, frequencies=[4e14, 5e14], downsample=7))
+    ref_boxes.append(ref.add_dft_monitor(1.5e-6, 1e-6, 0.4e-6, 0.0, frequencies=[4e14, 5e14], downsample=7))
+    for box, ref_box in zip(boxes, ref_boxes):
+        assert (box.x_samples, box.y_samples) == (ref_box.x_samples, ref_box.y_samples)
+        assert box.accumulator_shape()[2:4] == (ref_box.xs.size, ref_box.ys.size)
+        assert box.names == tuple(ref_box.spectra)
+        if polarization == 'hz':
+            # E_x pairs along y, E_y along x; index 0 wraps on the periodic y axis only
+            assert box.pairs == ((-1, False), (1, True), (0, False))
+    with pytest.raises(ValueError):
+        model.add_dft_monitor(0.1e-6, 0.1e-6, 2.5e-6, 0.5e-6, wavelengths=[5e-7])
+    with pytest.raises(ValueError):
+        model.add_dft_monitor(0.1e-6, 0.1e-6, 0.5e-6, 0.5e-6, wavelengths=[5e-7], downsample=0)

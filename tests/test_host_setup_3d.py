@@ -203,3 +203,15 @@ def test_field_probe_cells_match_the_oracle():
         model.add_field_probe(0.1e-6, 0.1e-6, 0.1e-6, 'e')
     with pytest.raises(ValueError):
         model.add_field_probe(0.1e-6, 0.1e-6, 0.6e-6)
+
+
+@pytest.mark.parametrize('z', [0.0, 0.27e-6, 0.55e-6])
+def test_dft_plane_layer_matches_the_oracle(z):
+    faces = {'z_lo': 'pmc', 'y_lo': 'periodic', 'y_hi': 'periodic'}
+    model, ref = _pair(faces)
+    plane, ref_plane = model.add_dft_plane(z, wavelengths=[6e-7]), ref.add_dft_plane(z, wavelengths=[6e-7])
+    assert plane.k_grid == ref_plane.k
+    assert plane.accumulator_shape() == (3, 1, ref_plane.xs.size, ref_plane.ys.size, 1, 2)
+    assert plane.pairs == ((0, False), (1, True), (2, False))
+    with pytest.raises(ValueError):
+        model.add_dft_plane(0.7e-6, wavelengths=[6e-7])

@@ -71,6 +71,7 @@ def test_fields_and_flux_match_golden(name):
     spots = [(x, y, z), (0.1e-6, y, 0.2e-6), (x, 0.5e-6, 0.1e-6), (0.6e-6, 0.05e-6, z), (0.0, 0.0, 0.0), sizes,
              (x, y, 0.4e-6)]
     probes = [sim.add_field_probe(*spot, component=f) for spot, f in zip(spots * 2, G.FIELDS * 2)]
+    dft_planes = [sim.add_dft_plane(z, wavelengths=list(G.WAVELENGTHS)) for z in G.CASES[name]['dft_planes']]
     sim.run(n_steps=G.SLICE_STEP)
     i, j, k = source.cell
     for f in G.FIELDS:
@@ -80,6 +81,16 @@ def test_fields_and_flux_match_golden(name):
     assert sim.time == float(golden['time'])
     for f in G.FIELDS:
         assert _rel(golden[f'{f}_{G.N_STEPS}'], sim.fields[f]) < 5e-4, f
+    # DFT planes of E at the cell centres (the pec case's layer 0 pairs E_z with itself): within 1e-5 relative L2
+    # of the reference's complex64 spectra (measured 1.5e-7 to 2.1e-6)
+    for index, plane in enumerate(dft_planes):
+        assert plane.k == int(golden[f'dft{index}_k'])
+        for component in ('ex', 'ey', 'ez'):
+            want = golden[f'dft{index}_{component}'].astype(np.complex128)
+            if not np.any(want):   # tangential E on the PEC face
+                assert not np.any(plane.spectra[component]), (index, component)
+                continue
+            assert _rel(plane.spectra[component], want) < 1e-5, (index, component)
     # each probe's samples after SLICE_STEP and N_STEPS are the reference's field at its cell (float32 run: within
     # 1e-4 of the field's peak; 1e-3 in the dipole's own cell, where the float32 run accumulates the source's
     # residue; a neighbouring cell is off by 5e-4 to 3e-2 of the peak)

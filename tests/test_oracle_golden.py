@@ -80,9 +80,10 @@ def _oracle_boundaries(faces, npml):
 def test_oracle_reproduces_reference_pec_case(polarization):
     """PEC region under a later dielectric, a source line crossing it and (E_z) a 'pec' x_lo face: the FP64 oracle
     reproduces the reference's FP64 run (fields at two steps, the source profile, the flux spectrum)."""
-    from tests.cases import PEC2D, build_pec2d
+    from tests.cases import PEC2D, add_pec2d_dft_boxes, build_pec2d
     golden = np.load(os.path.join(GOLDEN, f'pec2d_{polarization}.npz'))
     sim, source, monitor = build_pec2d(Yee2D, polarization, _oracle_boundaries, phase_dtype=np.complex64)
+    boxes = add_pec2d_dft_boxes(sim)
     assert (sim.nx, sim.ny) == (int(golden['nx']), int(golden['ny']))
     np.testing.assert_array_equal(sim.pec_regions[0], golden['pec_mask'])
     done = 0
@@ -96,3 +97,12 @@ def test_oracle_reproduces_reference_pec_case(polarization):
             np.testing.assert_allclose(values, want, rtol=0, atol=1e-12 * np.abs(want).max() + 1e-300)
     assert monitor.sample_interval == int(golden['sample_interval'])
     np.testing.assert_allclose(monitor.flux(), golden['flux'], rtol=1e-12)
+    # DFT boxes: sampled cells, and the complex64 spectra of the same complex64 arithmetic within 1e-6 of the peak
+    for index, box in enumerate(boxes):
+        samples = golden[f'dft{index}_samples']
+        assert [[s.start, s.stop, s.step] for s in (box.x_samples, box.y_samples)] == samples.tolist()
+        for name, values in box.spectra.items():
+            want = golden[f'dft{index}_{name}']
+            assert values.shape == want.shape
+            error = np.max(np.abs(values - want)) / np.max(np.abs(want))
+            assert error <= 1e-6, (index, name, error)
