@@ -1,0 +1,555 @@
+"""Model-JSON front end of the FDTD path: a model file of the reference's headless API (the em_fdtd_ez, em_fdtd_hz and
+em_fdtd_3d physics types) is built onto Simulation / Simulation3D, its time-dependent study (with a parameter sweep)
+runs on the HIP path, and the results come back and are saved in the reference's results-directory layout.
+
+Covered, from the reference's model vocabulary (open_sol/model/spec.py, physics/em/definition_fdtd.py,
+physics/em3d/definition.py, physics/em/definition_common.py:1167-1287):
+
+  * parameters: expressions over the earlier parameters and the builtin constants and units
+    (model/expressions.py:26-32), evaluated in dependency order (:616-680);
+  * domain size_x, size_y (size_z), a uniform mesh of cell dx; physics settings bc_<face>, pml_cells, pml_order,
+    pml_sigma_factor, pml_kappa_max, pml_alpha_max, courant_factor (definition_common.py:139-190, 660-668);
+  * geometry kinds rectangle, circle, line, point and box (model/geometry.py:111-262); materials with eps_r and
+    sigma_e painted on rectangles, circles (2D) or boxes (3D), in the order the materials are listed;
+  * 2D features line_source, flux_monitor, field_probe, dft_monitor and pec_domain; 3D features dipole_source,
+    field_probe, flux_plane, dft_plane and pec_domain;
+  * the time_dependent study (duration; the runner stops at t >= t_end - dt/2) and its sweep over one parameter;
+  * results: flux_spectrum (with power_per_cw where every source shares one waveform, physics/em/results.py:
+    93-122), timeseries and dft_fields per feature, the last run's fields on the window, results.npz in the
+    'open_sol.results' version-1 layout (io/results_io.py:399-433, 517-532) and summary.json with the per-result
+    digests (:156-225).
+
+Not covered (a ValueError names them): other physics types, features, geometry kinds and study types, couplings,
+graded meshes, field snapshots during the run and derived values.
+"""
+import ast
+import json
+import math
+import operator
+import os
+import time
+
+import numpy as np
+
+from .constants import C0, EPS0, MU0
+
+# model/expressions.py:26-32
+CONSTANTS = {
+    'pi': math.pi, 'c_const': float(C0), 'eps0_const': float(EPS0), 'mu0_const': float(MU0),
+    'h_const': 6.62607015e-34, 'hbar_const': 6.62607015e-34 / (2.0 * math.pi), 'e_const': 1.602176634e-19,
+    'kB_const': 1.380649e-23, 'NA_const': 6.02214076e23,
+    'nm': 1e-9, 'um': 1e-6, 'mm': 1e-3, 'cm': 1e-2,
+    'fs': 1e-15, 'ps': 1e-12, 'ns': 1e-9, 'us': 1e-6, 'ms': 1e-3,
+    'THz': 1e12, 'GHz': 1e9, 'MHz': 1e6, 'kHz': 1e3,
+}
+_FUNCTIONS = {
+    'sin': math.sin, 'cos': math.cos, 'tan': math.tan, 'asin': math.asin, 'acos': math.acos, 'atan': math.atan,
+    'atan2': math.atan2, 'sinh': math.sinh, 'cosh': math.cosh, 'tanh': math.tanh, 'exp': math.exp, 'log': math.log,
+    'log10': math.log10, 'sqrt': math.sqrt, 'abs': abs, 'min': min, 'max': max, 'floor': math.floor,
+    'ceil': math.ceil, 'hypot': math.hypot, 'pow': math.pow, 'sign': lambda v: float(np.sign(v)),
+}
+_BINARY = {ast.Add: operator.add, ast.Sub: operator.sub, ast.Mult: operator.mul, ast.Div: operator.truediv,
+           ast.Pow: operator.pow, ast.Mod: operator.mod, ast.FloorDiv: operator.floordiv}
+_UNARY = {ast.UAdd: operator.pos, ast.USub: operator.neg}
+_ALIASES = {'μm': 'um', 'μs': 'us'}
+RESULTS_FORMAT, RESULTS_FORMAT_VERSION = 'open_sol.results', 1
+FACES_2D = ('x_lo', 'x_hi', 'y_lo', 'y_hi')
+FACES_3D = FACES_2D + ('z_lo', 'z_hi')
+_VALID_SPECTRUM_FRACTION = 1e-6   # physics/em/results.py:23
+
+
+# ---- expressions ----------------------------------------------------------------------------------------------------
+def _names(text):
+    tree = ast.parse(str(text).strip(), mode='eval')
+    return {node.id for node in ast.walk(tree) if isinstance(node, ast.Name)} - set(_FUNCTIONS)
+
+
+def evaluate(value, names):
+    """A number, or the float value of an expression over `names` (parameters) and the builtin constants; ValueError
+    for anything but arithmetic, the listed functions and known names."""
+    if isinstance(value, bool):
+        raise ValueError(f'expected a number or expression, got {value!r}')
+    if isinstance(value, (int, float)):
+        return float(value)
+    if not isinstance(value, str):
+        raise ValueError(f'expected a number or expression, got {type(value).__name__}')
+    text = value.strip()
+    for alias, name in _ALIASES.items():
+        text = text.replace(alias, name)
+    tree = ast.parse(text, mode='eval')
+
+    def walk(node):
+        if isinstance(node, ast.Expression):
+            return walk(node.body)
+        if isinstance(node, ast.Constant) and isinstance(node.value, (int, float)) and not isinstance(node.value,
+                                                                                                       bool):
+            return float(node.value)
+        if isinstance(node, ast.Name):
+            if node.id in names:
+                return float(names[node.id])
+            if node.id in CONSTANTS:
+                return CONSTANTS[node.id]
+            raise ValueError(f'unknown name {node.id!r} in {value!r}')
+        if isinstance(node, ast.BinOp) and type(node.op) in _BINARY:
+            return float(_BINARY[type(node.op)](walk(node.left), walk(node.right)))
+        if isinstance(node, ast.UnaryOp) and type(node.op) in _UNARY:
+            return float(_UNARY[type(node.op)](walk(node.operand)))
+        if isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id in _FUNCTIONS \
+                and not node.keywords:
+            return float(_FUNCTIONS[node.func.id](*(walk(arg) for arg in node.args)))
+        raise ValueError(f'{value!r}: only numbers, parameter names, + - * / ** % // and the functions '
+                         f'{sorted(_FUNCTIONS)} are allowed')
+
+    result = walk(tree)
+    if not math.isfinite(result):
+        raise ValueError(f'{value!r} evaluates to {result}')
+    return result
+
+
+def evaluate_parameters(parameters, overrides=None):
+    """{name: float} of the model's parameters in dependency order (model/expressions.py:616-680); overrides
+    {name: expression} replace expressions first (a sweep value)."""
+    raw = {}
+    for item in parameters:
+        name = item['name']
+        if name in raw:
+            raise ValueError(f'parameter {name!r} is defined more than once')
+        if name in CONSTANTS or name in _FUNCTIONS:
+            raise ValueError(f'parameter name {name!r} is reserved')
+        raw[name] = item['expression']
+    for name, expression in (overrides or {}).items():
+        if name not in raw:
+            raise KeyError(f'unknown parameter {name!r}')
+        raw[name] = expression
+    values, pending = {}, dict(raw)
+    while pending:
+        ready = [name for name, expr in pending.items()
+                 if isinstance(expr, (int, float)) or not (_names(expr) - set(values) - set(CONSTANTS))]
+        if not ready:
+            raise ValueError(f'parameters {sorted(pending)} depend on unknown names or on each other')
+        for name in ready:
+            values[name] = evaluate(pending.pop(name), values)
+    return {name: values[name] for name in raw}
+
+
+# ---- spec access ----------------------------------------------------------------------------------------------------
+def load(path):
+    """The model dict of a model JSON file."""
+    with open(path, encoding='utf-8-sig') as handle:
+        return json.load(handle)
+
+
+def _physics(spec):
+    physics = spec.get('physics') or []
+    if len(physics) != 1:
+        raise ValueError(f'this front end runs one physics per model, got {len(physics)}')
+    entry = physics[0]
+    if entry.get('type') not in ('em_fdtd_ez', 'em_fdtd_hz', 'em_fdtd_3d'):
+        raise ValueError(f"physics type {entry.get('type')!r} is not an FDTD type of this path "
+                         f"(em_fdtd_ez, em_fdtd_hz, em_fdtd_3d)")
+    if spec.get('couplings'):
+        raise ValueError('couplings are not part of the FDTD path')
+    return entry
+
+
+def _wavelengths(params, values):
+    """core/wavelengths.py:5-10: count wavelengths linearly from start to stop; one gives their centre."""
+    start = values('wavelength_start', 400e-9)
+    stop = values('wavelength_stop', 800e-9)
+    count = int(values('n_wavelengths', 100))
+    if count < 1:
+        raise ValueError('n_wavelengths must be >= 1')
+    if count == 1:
+        return np.array([0.5 * (start + stop)])
+    if not stop > start:
+        raise ValueError(f'wavelength_stop ({stop}) must be > wavelength_start ({start})')
+    return np.linspace(start, stop, count)
+
+
+class _Built:
+    """One build of the model at one set of parameter values: the simulation and its feature objects."""
+
+    def __init__(self, spec, names):
+        from .boundaries import BoundaryConfig
+        from .simulation import Simulation, Simulation3D
+        physics = _physics(spec)
+        self.physics_name = physics.get('name', 'em')
+        self.kind = physics['type']
+        self.three_d = self.kind == 'em_fdtd_3d'
+        value = lambda v: evaluate(v, names)   # noqa: E731
+        settings = physics.get('settings') or {}
+
+        def setting(key, default):
+            raw = settings.get(key, default)
+            return raw if isinstance(raw, str) and key.startswith('bc_') else value(raw)
+
+        mesh = spec.get('mesh') or {}
+        if mesh.get('mode', 'uniform') != 'uniform' or mesh.get('refinements'):
+            raise ValueError('only a uniform mesh is part of the FDTD path')
+        dx = value(mesh.get('dx', '25*nm'))
+        domain = spec['domain']
+        faces = FACES_3D if self.three_d else FACES_2D
+        bc = BoundaryConfig(**{face: setting(f'bc_{face}', 'cpml') for face in faces})
+        bc.cpml_npml = int(setting('pml_cells', 15))
+        bc.cpml_order = int(setting('pml_order', 3))
+        bc.cpml_sigma_factor = setting('pml_sigma_factor', 1.0)
+        bc.cpml_kappa_max = setting('pml_kappa_max', 15.0)
+        bc.cpml_alpha_max = setting('pml_alpha_max', 0.0)
+        courant = setting('courant_factor', 0.99)
+        sizes = [value(domain['size_x']), value(domain['size_y'])]
+        if self.three_d:
+            sizes.append(value(domain['size_z']))
+            self.sim = Simulation3D(*sizes, dx, npml=bc.cpml_npml, courant=courant, boundaries=bc)
+        else:
+            self.sim = Simulation(*sizes, dx, npml=bc.cpml_npml, courant=courant, boundaries=bc,
+                                  polarization=self.kind[-2:])
+        self.geometry = {}
+        for item in spec.get('geometry') or []:
+            self.geometry[item['name']] = (item['kind'], {k: value(v) for k, v in (item.get('params') or {}).items()})
+        for material in spec.get('materials') or []:
+            self._paint(material, value)
+        self.features = []
+        for feature in physics.get('features') or []:
+            if feature.get('enabled', True):
+                self.features.append((feature['name'], feature['type'], self._apply(feature, value)))
+
+    def _shape(self, name, kinds):
+        if name not in self.geometry:
+            raise ValueError(f'selection names unknown geometry {name!r}')
+        kind, params = self.geometry[name]
+        if kind not in kinds:
+            raise ValueError(f'geometry {name!r} is a {kind}; expected {" or ".join(kinds)}')
+        return kind, params
+
+    def _single(self, feature, kinds):
+        selection = feature.get('selection') or []
+        if len(selection) != 1:
+            raise ValueError(f"feature {feature['name']!r} needs exactly one selected object")
+        return self._shape(selection[0], kinds)
+
+    def _paint(self, material, value):
+        properties = material.get('properties') or {}
+        unknown = set(properties) - {'eps_r', 'sigma_e'}
+        if unknown or material.get('dispersion'):
+            raise ValueError(f"material {material.get('name')!r}: only eps_r and sigma_e are part of the FDTD path")
+        eps = value(properties.get('eps_r', 1.0))
+        sigma = value(properties.get('sigma_e', 0.0))
+        for name in material.get('selection') or []:
+            kind, p = self._shape(name, ('box',) if self.three_d else ('rectangle', 'circle'))
+            if kind == 'rectangle':
+                self.sim.add_dielectric_rectangle(p['x0'], p['y0'], p['x1'], p['y1'], eps_r=eps, sigma_e=sigma)
+            elif kind == 'circle':
+                self.sim.model.add_dielectric_circle(p['cx'], p['cy'], p['r'], eps_r=eps, sigma_e=sigma)
+            else:
+                self.sim.add_dielectric_box((p['x0'], p['y0'], p['z0']), (p['x1'], p['y1'], p['z1']), eps, sigma)
+
+    def _apply(self, feature, value):
+        """definition_common.py:1167-1287 (2D) and physics/em3d/definition.py:472-520 (3D)."""
+        params = feature.get('params') or {}
+        get = lambda key, default=None: value(params[key]) if key in params else default   # noqa: E731
+        text = lambda key, default: params.get(key, default)   # noqa: E731
+        kind, sim = feature['type'], self.sim
+        source_time = dict(wavelength=get('wavelength', 550e-9), bandwidth=get('bandwidth', 0.5),
+                           waveform=text('waveform', 'gaussian'))
+        if kind == 'pec_domain':
+            for name in feature.get('selection') or []:
+                shape, p = self._shape(name, ('box',) if self.three_d else ('rectangle',))
+                if self.three_d:
+                    sim.add_pec_box((p['x0'], p['y0'], p['z0']), (p['x1'], p['y1'], p['z1']))
+                else:
+                    sim.add_pec_rectangle(p['x0'], p['y0'], p['x1'], p['y1'])
+            return None
+        if kind == 'field_probe':
+            _, p = self._single(feature, ('point',))
+            if self.three_d:
+                return sim.add_field_probe(p['x'], p['y'], p.get('z', 0.0), text('component', 'ez'))
+            return sim.add_field_probe(p['x'], p['y'], text('component', self.kind[-2:]))
+        if not self.three_d:
+            if kind in ('line_source', 'flux_monitor'):
+                _, p = self._single(feature, ('line',))
+                if p['x0'] == p['x1']:
+                    axis, position, span = 'y', 0.5 * (p['x0'] + p['x1']), (p['y0'], p['y1'])
+                elif p['y0'] == p['y1']:
+                    axis, position, span = 'x', 0.5 * (p['y0'] + p['y1']), (p['x0'], p['x1'])
+                else:
+                    raise ValueError(f"feature {feature['name']!r}: its line must be horizontal or vertical")
+                if kind == 'line_source':
+                    return sim.add_line_source(axis, position, amplitude=get('amplitude', 1.0), span=span,
+                                               **source_time)
+                return sim.add_flux_monitor(axis, position, tuple(sorted(span)), wavelengths=_wavelengths(params, get))
+            if kind == 'dft_monitor':
+                _, p = self._single(feature, ('rectangle',))
+                return sim.add_dft_monitor(p['x0'], p['y0'], p['x1'], p['y1'], wavelengths=_wavelengths(params, get),
+                                           downsample=int(get('downsample', 1)))
+        else:
+            if kind == 'dipole_source':
+                _, p = self._single(feature, ('point',))
+                return sim.add_dipole(p['x'], p['y'], p.get('z', 0.0), orientation=text('dipole_axis', 'z'),
+                                      dipole_moment=get('current_moment', 1e-12), **source_time)
+            if kind == 'flux_plane':
+                return sim.add_flux_plane(text('normal', 'z'), get('position', 0.0),
+                                          wavelengths=_wavelengths(params, get))
+            if kind == 'dft_plane':
+                return sim.add_dft_plane(get('position', 0.0), wavelengths=_wavelengths(params, get))
+        raise ValueError(f"feature type {kind!r} is not part of the {self.kind} FDTD path here")
+
+    # ---- results ----
+    def results(self):
+        """{'<physics>.<feature>': result dict} of every result-bearing feature."""
+        out = {}
+        for name, kind, obj in self.features:
+            key = f'{self.physics_name}.{name}'
+            if kind in ('flux_monitor', 'flux_plane'):
+                out[key] = self._flux_result(obj)
+            elif kind == 'field_probe':
+                times, values = obj.get_data()
+                out[key] = {'kind': 'timeseries', 't': times, 'values': values, 'component': obj.field_component,
+                            'unit': 'A/m' if obj.field_component.startswith('h') else 'V/m'}
+            elif kind in ('dft_monitor', 'dft_plane'):
+                out[key] = self._dft_result(obj)
+        return out
+
+    def _source_spectrum(self, frequency, sample_interval):
+        """(S(omega), valid) of the one waveform every source shares, else None (physics/em/results.py:55-100;
+        core/sources.py:103-116: S = sum over the sampled steps n of s(n dt) exp(-j omega n dt) m dt)."""
+        sources = list(self.sim.model.sources)
+        if not sources:
+            return None
+        first = sources[0].temporal
+        if any(type(s.temporal) is not type(first) or vars(s.temporal) != vars(first) for s in sources):
+            return None
+        dt = float(self.sim.model.dt)
+        n_step, m = self.sim.engine.n_step, int(sample_interval)
+
+        def spectrum(freqs):
+            times = np.arange(m, n_step + 1, m, dtype=np.float64) * dt
+            weighted = np.asarray(first(times), dtype=np.float64) * (dt * m)
+            return np.exp(-2j * np.pi * np.outer(np.atleast_1d(freqs), times)) @ weighted
+
+        values = spectrum(frequency)
+        carrier = float(np.abs(spectrum([first.f0]))[0] ** 2)
+        valid = (np.abs(values) ** 2 >= _VALID_SPECTRUM_FRACTION * carrier) & (carrier > 0.0)
+        if type(first).__name__ == 'ContinuousWave':
+            valid &= np.abs(frequency - first.f0) * n_step * dt < 1.0
+        return values, valid
+
+    def _flux_result(self, monitor):
+        frequency = np.array(monitor.frequencies, dtype=np.float64)
+        result = {'kind': 'flux_spectrum', 'frequency': frequency, 'wavelength': C0 / frequency,
+                  'power': np.asarray(monitor.flux(), dtype=np.float64), 'unit': 'a.u.', 'mirror_factor': 1.0}
+        shared = self._source_spectrum(frequency, monitor.sample_interval)
+        if shared is not None:
+            values, valid = shared
+            scale = np.where(valid, 1.0 / (2.0 * np.where(valid, np.abs(values) ** 2, 1.0)), np.nan)
+            result['power_per_cw'] = result['power'] * scale
+        return result
+
+    def _dft_result(self, monitor):
+        frequency = np.array(monitor.frequencies, dtype=np.float64)
+        centres = self.sim.domain.cell_centers()
+        edges = self.sim.domain.cell_edges()
+        if self.three_d:
+            xs, ys = monitor.window
+            intensity = np.stack([monitor.intensity(i) for i in range(monitor.n_freqs)])
+            extent = (float(edges[0][xs.start]), float(edges[0][xs.stop]), float(edges[1][ys.start]),
+                      float(edges[1][ys.stop]))
+            return {'kind': 'dft_fields', 'frequency': frequency, 'wavelength': C0 / frequency,
+                    'x_centers': np.array(centres[0][xs]), 'y_centers': np.array(centres[1][ys]), 'extent': extent,
+                    'shape': tuple(monitor.shape), 'quantity': 'E intensity', 'z': float(centres[2][monitor.k_grid]),
+                    'intensity_all': intensity}
+        if monitor.polarization == 'ez':
+            intensity = np.abs(monitor.ez_dft) ** 2
+        else:
+            intensity = np.abs(monitor.ex_dft) ** 2 + np.abs(monitor.ey_dft) ** 2
+        xs, ys = monitor.x_slice, monitor.y_slice
+        extent = (float(edges[0][xs.start]), float(edges[0][xs.stop]), float(edges[1][ys.start]),
+                  float(edges[1][ys.stop]))
+        return {'kind': 'dft_fields', 'frequency': frequency, 'wavelength': C0 / frequency,
+                'x_centers': np.array(centres[0][monitor.x_samples]), 'y_centers': np.array(centres[1][monitor.y_samples]),
+                'extent': extent, 'shape': (int(monitor.mx), int(monitor.my)), 'downsample': int(monitor.downsample),
+                'quantity': 'E_z intensity' if monitor.polarization == 'ez' else 'in-plane E intensity',
+                'intensity_all': intensity}
+
+    def fields(self):
+        """{'<physics>.<component>': (window values, x edges, y edges)} of the final state (3D: full arrays)."""
+        sim = self.sim
+        window = sim.domain.physical_window()
+        edges = sim.domain.cell_edges()
+        out = {}
+        for name in sim.model.components if not self.three_d else ('ex', 'ey', 'ez', 'hx', 'hy', 'hz'):
+            values = sim.engine.field(name)[tuple(window)]
+            out[f'{self.physics_name}.{name}'] = (values, edges[0][window[0].start:window[0].stop + 1],
+                                                  edges[1][window[1].start:window[1].stop + 1])
+        return out
+
+
+# ---- digests (io/results_io.py:156-225) ----------------------------------------------------------------------------
+def _finite(values):
+    values = np.asarray(values, dtype=np.float64).ravel()
+    return values[np.isfinite(values)]
+
+
+def result_digest(result):
+    kind = result.get('kind')
+    if kind == 'timeseries':
+        times, values = np.asarray(result['t'], np.float64), np.asarray(result['values'], np.float64)
+        finite = _finite(values)
+        return {'kind': kind, 'component': result.get('component'), 'unit': result.get('unit'),
+                'n_samples': int(values.size), 't_start': float(times[0]) if times.size else None,
+                't_stop': float(times[-1]) if times.size else None,
+                'min': float(finite.min()) if finite.size else None, 'max': float(finite.max()) if finite.size else None,
+                'final': float(values[-1]) if values.size else None}
+    if kind == 'flux_spectrum':
+        wavelength, power = np.asarray(result['wavelength'], np.float64), np.asarray(result['power'], np.float64)
+        finite = _finite(power)
+        peak = int(np.nanargmax(np.where(np.isfinite(power), power, -np.inf))) if finite.size else None
+        return {'kind': kind, 'unit': result.get('unit'), 'n_wavelengths': int(wavelength.size),
+                'wavelength_min': float(wavelength.min()), 'wavelength_max': float(wavelength.max()),
+                'power_min': float(finite.min()) if finite.size else None,
+                'power_max': float(finite.max()) if finite.size else None,
+                'wavelength_at_max': None if peak is None else float(wavelength[peak])}
+    if kind == 'dft_fields':
+        wavelength = np.asarray(result['wavelength'], np.float64)
+        intensity_max = [float(_finite(frame).max()) if _finite(frame).size else None
+                         for frame in result.get('intensity_all', [])]
+        return {'kind': kind, 'quantity': result.get('quantity'), 'shape': [int(n) for n in result['shape']],
+                'n_wavelengths': int(wavelength.size), 'wavelengths': wavelength.tolist(),
+                'intensity_max': intensity_max or None}
+    return {'kind': kind, 'keys': sorted(str(key) for key in result)}
+
+
+def _json_safe(value):
+    if value is None or isinstance(value, (bool, int, str)):
+        return value
+    if isinstance(value, (float, np.floating)):
+        value = float(value)
+        return value if math.isfinite(value) else ('nan' if math.isnan(value) else ('inf' if value > 0 else '-inf'))
+    if isinstance(value, np.integer):
+        return int(value)
+    if isinstance(value, dict):
+        return {str(k): _json_safe(v) for k, v in value.items()}
+    if isinstance(value, (list, tuple)):
+        return [_json_safe(v) for v in value]
+    if isinstance(value, np.ndarray):
+        return _json_safe(value.tolist())
+    raise TypeError(f'{type(value).__name__} is not JSON serialisable')
+
+
+def _is_array(value):
+    return isinstance(value, np.ndarray) and value.ndim >= 1 and value.dtype.kind in 'biufc'
+
+
+def _split(result, prefix, arrays):
+    """Non-array entries of a result; its arrays go into `arrays` under prefix + key (io/results_io.py:517-532)."""
+    entries = {}
+    for key, value in result.items():
+        if '/' in str(key):
+            raise ValueError(f"result key {key!r} contains '/'")
+        if _is_array(value):
+            arrays[prefix + str(key)] = value
+        else:
+            entries[key] = value
+    return entries
+
+
+class Results:
+    """Outcome of run(): status, the last run's results and fields, every swept run's results, summary() and
+    save(). result(physics, feature) / sweep_result(physics, feature) as the reference's Results."""
+
+    def __init__(self, status, spec, results, sweep_results, sweep, fields, summary):
+        self.status = status
+        self.spec = spec
+        self.results = results
+        self.sweep_results = sweep_results
+        self.sweep = sweep
+        self.fields = fields
+        self._summary = summary
+
+    def summary(self):
+        return self._summary
+
+    def result(self, physics, feature=None):
+        return self.results[f'{physics}.{feature}' if feature is not None else physics]
+
+    def sweep_result(self, physics, feature=None):
+        key = f'{physics}.{feature}' if feature is not None else physics
+        return [run[key] for run in self.sweep_results]
+
+    def save(self, directory):
+        """results.npz (format 'open_sol.results' version 1), model.json and summary.json; returns the files dict."""
+        os.makedirs(directory, exist_ok=True)
+        arrays = {}
+        metadata = {'format': RESULTS_FORMAT, 'version': RESULTS_FORMAT_VERSION, 'fields': {}, 'results': {},
+                    'sweep_results': [], 'sector_results': [], 'sector_combined_results': {}, 'snapshots': [],
+                    'sweep_snapshots': []}
+        for key, (values, x_edges, y_edges) in self.fields.items():
+            arrays[f'field/{key}/values'] = np.asarray(values)
+            arrays[f'field/{key}/x_edges'] = np.asarray(x_edges, dtype=np.float64)
+            arrays[f'field/{key}/y_edges'] = np.asarray(y_edges, dtype=np.float64)
+            physics, component = key.split('.', 1)
+            metadata['fields'][key] = {'physics': physics, 'component': component, 'time': self._summary['t_end']}
+        for key, result in self.results.items():
+            metadata['results'][key] = _split(result, f'result/{key}/', arrays)
+        for index, run in enumerate(self.sweep_results if self.sweep else []):
+            metadata['sweep_results'].append({key: _split(result, f'sweep/{index}/{key}/', arrays)
+                                              for key, result in run.items()})
+        files = {'summary': 'summary.json', 'arrays': 'results.npz', 'model': 'model.json'}
+        with open(os.path.join(directory, 'results.npz'), 'wb') as handle:
+            np.savez(handle, **arrays, metadata=np.array(json.dumps(_json_safe(metadata))))
+        with open(os.path.join(directory, 'model.json'), 'w', encoding='utf-8') as handle:
+            json.dump(self.spec, handle, indent=2)
+        summary = dict(self._summary, files=files)
+        with open(os.path.join(directory, 'summary.json'), 'w', encoding='utf-8') as handle:
+            json.dump(_json_safe(summary), handle, indent=2, ensure_ascii=True, allow_nan=False)
+            handle.write('\n')
+        return files
+
+
+def run(spec, out_dir=None, parameters=None, progress=None):
+    """Build and run a model (a dict or a path to its JSON) on the HIP path; returns Results and, with out_dir, saves
+    them there. parameters {name: expression} override the model's. A study sweep runs one build per value."""
+    if isinstance(spec, (str, os.PathLike)):
+        spec = load(spec)
+    study = spec.get('study') or {'type': 'time_dependent', 'settings': {'duration': 200e-15}}
+    if study.get('type') != 'time_dependent':
+        raise ValueError(f"study type {study.get('type')!r}: only 'time_dependent' is part of the FDTD path")
+    if spec.get('derived_values'):
+        raise ValueError('derived values are not part of the FDTD path')
+    overrides = dict(parameters or {})
+    base = evaluate_parameters(spec.get('parameters') or [], overrides)
+    sweep = study.get('sweep')
+    runs = [overrides]
+    if sweep:
+        if sweep.get('parameter') not in base:
+            raise ValueError(f"sweep parameter {sweep.get('parameter')!r} is not a model parameter")
+        values = [evaluate(v, base) for v in sweep.get('values') or []]
+        runs = [dict(overrides, **{sweep['parameter']: v}) for v in values]
+        sweep = {'parameter': sweep['parameter'], 'values': values}
+    wall = time.perf_counter()
+    sweep_results, built, names = [], None, base
+    for index, run_overrides in enumerate(runs):
+        names = evaluate_parameters(spec.get('parameters') or [], run_overrides)
+        if built is not None:
+            built.sim.close()
+        built = _Built(spec, names)
+        duration = evaluate((study.get('settings') or {}).get('duration', 200e-15), names)
+        built.sim.run(duration=duration)
+        sweep_results.append(built.results())
+        if progress is not None:
+            progress({'run': index, 'of': len(runs), 'steps': built.sim.engine.n_step})
+    results = sweep_results[-1]
+    summary = {'status': 'completed', 'message': '', 'parameters': names,
+               'results': {key: result_digest(result) for key, result in results.items()},
+               't_end': float(built.sim.time), 'ticks': int(built.sim.engine.n_step),
+               'wall_time': time.perf_counter() - wall}
+    if sweep:
+        summary['sweep'] = dict(sweep, results=[{key: result_digest(r) for key, r in run.items()}
+                                                for run in sweep_results])
+    out = Results('completed', spec, results, sweep_results, sweep, built.fields(), summary)
+    built.sim.close()
+    if out_dir is not None:
+        out.save(out_dir)
+    return out
+
+
+__all__ = ['CONSTANTS', 'Results', 'evaluate', 'evaluate_parameters', 'load', 'result_digest', 'run']

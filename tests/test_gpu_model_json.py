@@ -1,0 +1,44 @@
+"""The reference's thin-film example model (examples/em/thin_film.json, E_z and H_z) run through the model-JSON front
+end on the HIP path, against what the reference's own headless API returned for it (tests/golden/
+make_model_json_golden.py, its float32 CPU path; fixtures in tests/fixtures): every swept run's flux spectra and per-cw powers, the transmission,
+and the results.npz array layout."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'fixtures')
+MONITORS = ('reflection_side_flux', 'transmitted_flux')
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_thin_film_model_matches_the_reference_run(gpu, polarization, tmp_path):
+    from opensol_amd import model as M
+    golden = np.load(os.path.join(FIXTURES, f'model_thin_film_{polarization}.npz'))
+    spec = json.loads(str(golden['model_json']))
+    results = M.run(spec, out_dir=tmp_path)
+    assert results.status == 'completed'
+    assert 2 * results.summary()['ticks'] == int(golden['summary_ticks'])
+    assert results.summary()['t_end'] == float(golden['summary_t_end'])
+    for name in MONITORS:
+        runs = results.sweep_result('em', name)
+        assert len(runs) == 2
+        for index, result in enumerate(runs):
+            np.testing.assert_allclose(result['wavelength'], golden[f'{name}/{index}/wavelength'], rtol=1e-15)
+            for key in ('power', 'power_per_cw'):
+                want = golden[f'{name}/{index}/{key}']
+                error = np.max(np.abs(result[key] - want) / np.abs(want))
+                print(f'{polarization} {name} run {index} {key}: max rel err {error:.2e}')
+                assert error <= 1e-4, (name, index, key, error)
+    # film run over the empty run: the reference's transmission
+    t_gpu = results.sweep_result('em', 'transmitted_flux')[1]['power'] / results.sweep_result(
+        'em', 'transmitted_flux')[0]['power']
+    t_ref = golden['transmitted_flux/1/power'] / golden['transmitted_flux/0/power']
+    print(f'{polarization} transmission max |diff| {np.max(np.abs(t_gpu - t_ref)):.2e}')
+    assert np.max(np.abs(t_gpu - t_ref)) <= 1e-5
+    with np.load(tmp_path / 'results.npz', allow_pickle=False) as npz:
+        ours = {n for n in npz.files if n.split('/')[0] in ('result', 'sweep')}
+    assert ours == {str(n) for n in golden['npz_names'] if str(n).split('/')[0] in ('result', 'sweep')}

@@ -1,0 +1,118 @@
+"""Model-JSON front end (opensol_amd.model): parameter expressions, the build of the reference's thin-film example
+model (examples/em/thin_film.json, held as text in the golden fixture) against the hand-built thin-film case, the
+study's step count, and the results-directory writer. GPU runs of the same model are in test_gpu_model_json.py."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from opensol_amd import model as M
+from opensol_amd.fdtd2d import Model2D, steps_for_duration
+
+FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'fixtures')
+
+
+def golden_spec(polarization):
+    golden = np.load(os.path.join(FIXTURES, f'model_thin_film_{polarization}.npz'))
+    return golden, json.loads(str(golden['model_json']))
+
+
+def test_expressions_follow_python_arithmetic_and_the_builtin_units():
+    names = M.evaluate_parameters([{'name': 'b', 'expression': 'a*2'}, {'name': 'a', 'expression': '600*nm'},
+                                   {'name': 'c', 'expression': 'sqrt(b**2) / c_const + 0*pi'}])
+    assert list(names) == ['b', 'a', 'c']
+    assert names['a'] == 600 * 1e-9 and names['b'] == 600 * 1e-9 * 2
+    assert names['c'] == (names['b'] ** 2) ** 0.5 / 299792458.0
+    assert M.evaluate('4*Lx/c_const', {'Lx': 6e-6}) == 4 * 6e-6 / 299792458.0
+    for bad in ('__import__("os")', 'a.b', '[1]', 'lambda: 1', 'x if 1 else 2', 'unknown*2', '1/0'):
+        with pytest.raises((ValueError, ZeroDivisionError)):
+            M.evaluate(bad, {'a': 1.0})
+    with pytest.raises(ValueError):
+        M.evaluate_parameters([{'name': 'a', 'expression': 'b'}, {'name': 'b', 'expression': 'a'}])
+    with pytest.raises(ValueError):
+        M.evaluate_parameters([{'name': 'pi', 'expression': 3}])
+    with pytest.raises(KeyError):
+        M.evaluate_parameters([{'name': 'a', 'expression': 1}], {'z': 2})
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_thin_film_model_builds_the_hand_built_case(polarization):
+    """The example model's build equals the same case built through the scripting API: grid, dt, materials, the
+    source line and profile, the monitor lines and wavelengths; the study runs the reference's step count."""
+    golden, spec = golden_spec(polarization)
+    names = M.evaluate_parameters(spec['parameters'])
+    built = M._Built(spec, dict(names))
+    model = built.sim.model
+    ref = Model2D(6e-6, 3e-6, 25e-9, npml=15, boundaries='periodic_y', polarization=polarization)
+    ref.add_dielectric_rectangle(3e-6 - 250e-9, 0.0, 3e-6 + 250e-9, 3e-6, eps_r=1.5 ** 2)
+    ref.add_line_source('y', 1.2e-6, wavelength=600e-9, bandwidth=0.8, span=(0.0, 3e-6))
+    wavelengths = np.linspace(400e-9, 800e-9, 50)
+    for x in (1.8e-6, 4.5e-6):
+        ref.add_flux_monitor('y', x, (0.0, 3e-6), wavelengths=wavelengths)
+    model.finalize()
+    ref.finalize()
+    assert model.shape == ref.shape and model.dt == ref.dt
+    np.testing.assert_array_equal(model.materials.ids, ref.materials.ids)
+    np.testing.assert_array_equal(model.materials.values(), ref.materials.values())
+    (source,), (ref_source,) = model.sources, ref.sources
+    assert (source.axis, source.position, source.span) == (ref_source.axis, ref_source.position, ref_source.span)
+    np.testing.assert_array_equal(source.profile, ref_source.profile)
+    for monitor, ref_monitor in zip(model.monitors, ref.monitors):
+        assert (monitor.normal, monitor.position, monitor.span) == (ref_monitor.normal, ref_monitor.position,
+                                                                    ref_monitor.span)
+        np.testing.assert_array_equal(monitor.frequencies, ref_monitor.frequencies)
+        np.testing.assert_allclose(M.C0 / monitor.frequencies, golden['transmitted_flux/0/wavelength'], rtol=1e-15)
+    assert model.sample_interval == ref.sample_interval
+    duration = M.evaluate(spec['study']['settings']['duration'], names)
+    # the reference's summary counts the ticks of both swept runs
+    assert 2 * steps_for_duration(0.0, model.dt, duration) == int(golden['summary_ticks'])
+    sweep = json.loads(str(golden['summary_sweep']))
+    assert [M.evaluate(v, names) for v in spec['study']['sweep']['values']] == sweep['values']
+
+
+def test_unsupported_models_are_refused():
+    _, spec = golden_spec('ez')
+    names = M.evaluate_parameters(spec['parameters'])
+    for change in ({'type': 'em_fdfd_ez'},):
+        other = json.loads(json.dumps(spec))
+        other['physics'][0].update(change)
+        with pytest.raises(ValueError):
+            M._Built(other, names)
+    other = json.loads(json.dumps(spec))
+    other['physics'][0]['features'].append({'name': 'beam', 'type': 'gaussian_beam', 'selection': ['source_line'],
+                                            'params': {}, 'enabled': True})
+    with pytest.raises(ValueError):
+        M._Built(other, names)
+    other = json.loads(json.dumps(spec))
+    other['study']['type'] = 'frequency_domain'
+    with pytest.raises(ValueError):
+        M.run(other)
+
+
+def test_results_directory_layout(tmp_path):
+    """results.npz holds 'result/<key>/<array>' and 'sweep/<i>/<key>/<array>' arrays with a 'metadata' JSON entry of
+    format 'open_sol.results' version 1 (io/results_io.py:399-433, 517-532); summary.json the per-result digests."""
+    golden, spec = golden_spec('ez')
+    result = {'kind': 'flux_spectrum', 'frequency': np.array([1.0, 2.0]), 'wavelength': np.array([2.0, 1.0]),
+              'power': np.array([3.0, 4.0]), 'unit': 'a.u.', 'mirror_factor': 1.0}
+    runs = [{'em.transmitted_flux': result, 'em.reflection_side_flux': result}] * 2
+    fields = {'em.ez': (np.zeros((3, 2), np.float32), np.arange(4.0), np.arange(3.0))}
+    summary = {'status': 'completed', 't_end': 1.0, 'results': {k: M.result_digest(v) for k, v in runs[-1].items()}}
+    out = M.Results('completed', spec, runs[-1], runs, {'parameter': 'n_film', 'values': [1.0, 1.5]}, fields, summary)
+    files = out.save(tmp_path)
+    with np.load(tmp_path / files['arrays'], allow_pickle=False) as npz:
+        names = set(npz.files)
+        metadata = json.loads(str(npz['metadata']))
+    reference_names = {str(n) for n in golden['npz_names'] if str(n).split('/')[0] in ('result', 'sweep')}
+    ours = {n for n in names if n.split('/')[0] in ('result', 'sweep')}
+    assert ours == {n for n in reference_names if not n.endswith('power_per_cw')}
+    reference_metadata = json.loads(str(golden['npz_metadata']))
+    assert metadata['format'] == reference_metadata['format'] and metadata['version'] == reference_metadata['version']
+    assert metadata['results'] == reference_metadata['results']
+    assert metadata['sweep_results'] == reference_metadata['sweep_results']
+    assert set(metadata) == set(reference_metadata)
+    saved = json.loads((tmp_path / 'summary.json').read_text())
+    reference_digest = json.loads(str(golden['summary_sweep']))['results'][1]['em.transmitted_flux']
+    assert set(saved['results']['em.transmitted_flux']) == set(reference_digest)
+    assert json.loads((tmp_path / 'model.json').read_text()) == spec
