@@ -1,7 +1,7 @@
-"""Write tests/fixtures/model_thin_film_{ez,hz}.npz (beside the GPU tests that read them): what the reference's headless model API (open_sol.api.run, CPU
+"""Write tests/fixtures/model_thin_film_{ez,hz}.npz and model_dipole_3d.npz (beside the GPU tests that read them): what the reference's headless model API (open_sol.api.run, CPU
 backend) returns and saves for its thin-film example model (examples/em/thin_film.json: a line source, two flux
 monitors of 50 wavelengths, a film swept over n_film = 1 then 1.5), once as written (E_z) and once with the physics
-type switched to em_fdtd_hz. They pin the model-JSON caller end to end: parameter evaluation, geometry and material
+type switched to em_fdtd_hz, and for a small 3D model written here (MODEL_3D). They pin the model-JSON caller end to end: parameter evaluation, geometry and material
 selection, feature placement, the study duration and sweep, and the results.npz layout.
 
 Saved per case: the model JSON text (input), each swept run's flux wavelengths, power and power_per_cw per monitor
@@ -24,6 +24,87 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURES = os.path.join(os.path.dirname(HERE), 'fixtures')
 EXAMPLE = os.path.join(REFERENCE, 'examples', 'em', 'thin_film.json')
 MONITORS = ('reflection_side_flux', 'transmitted_flux')
+
+
+# A small 3D model of the em_fdtd_3d vocabulary (no example of the reference uses only features of this path): a
+# z dipole over a glass block, a PEC box, flux planes normal to x and z, a DFT layer and an E_z probe; 8-cell CPML.
+MODEL_3D = {
+    'title': 'Dipole over a glass block (3D)', 'version': 1,
+    'parameters': [
+        {'name': 'L', 'expression': '0.6*um', 'description': 'domain width'},
+        {'name': 'H', 'expression': '0.55*um', 'description': 'domain height'},
+        {'name': 'lambda0', 'expression': '600*nm', 'description': 'pulse centre wavelength'},
+        {'name': 'n_glass', 'expression': 1.5, 'description': 'block index'},
+    ],
+    'domain': {'size_x': '1.2*L', 'size_y': 'L', 'size_z': 'H'},
+    'geometry': [
+        {'name': 'dipole_point', 'kind': 'point', 'params': {'x': '0.36*um', 'y': '0.29*um', 'z': '0.27*um'}},
+        {'name': 'probe_point', 'kind': 'point', 'params': {'x': '0.2*um', 'y': '0.45*um', 'z': '0.3*um'}},
+        {'name': 'block', 'kind': 'box', 'params': {'x0': '0.05*um', 'y0': '0.05*um', 'z0': 0.0, 'x1': '0.3*um',
+                                                    'y1': '0.35*um', 'z1': '0.25*um'}},
+        {'name': 'metal', 'kind': 'box', 'params': {'x0': '0.45*um', 'y0': '0.1*um', 'z0': '0.3*um', 'x1': '0.6*um',
+                                                    'y1': '0.35*um', 'z1': '0.45*um'}},
+    ],
+    'selections': [],
+    'materials': [{'name': 'glass', 'library': None, 'properties': {'eps_r': 'n_glass**2'}, 'dispersion': None,
+                   'selection': ['block']}],
+    'physics': [{
+        'name': 'em', 'type': 'em_fdtd_3d', 'settings': {'pml_cells': 8},
+        'features': [
+            {'name': 'dipole', 'type': 'dipole_source', 'selection': ['dipole_point'],
+             'params': {'dipole_axis': 'z', 'waveform': 'gaussian', 'wavelength': 'lambda0', 'bandwidth': 0.5,
+                        'current_moment': 1e-12}, 'enabled': True},
+            {'name': 'conductor', 'type': 'pec_domain', 'selection': ['metal'], 'params': {}, 'enabled': True},
+            {'name': 'flux_x', 'type': 'flux_plane', 'selection': [],
+             'params': {'normal': 'x', 'position': '0.55*um', 'wavelength_start': 5e-07, 'wavelength_stop': 7e-07,
+                        'n_wavelengths': 5}, 'enabled': True},
+            {'name': 'flux_z', 'type': 'flux_plane', 'selection': [],
+             'params': {'normal': 'z', 'position': '0.1*um', 'wavelength_start': 5e-07, 'wavelength_stop': 7e-07,
+                        'n_wavelengths': 5}, 'enabled': True},
+            {'name': 'layer', 'type': 'dft_plane', 'selection': [],
+             'params': {'position': '0.27*um', 'wavelength_start': 5e-07, 'wavelength_stop': 7e-07,
+                        'n_wavelengths': 3}, 'enabled': True},
+            {'name': 'probe', 'type': 'field_probe', 'selection': ['probe_point'], 'params': {'component': 'ez'},
+             'enabled': True},
+        ], 'every': 1}],
+    'couplings': [],
+    'mesh': {'mode': 'uniform', 'dx': '25*nm', 'max_ratio': 1.15, 'wavelength_min': None,
+             'points_per_wavelength': 20.0, 'refinements': []},
+    'study': {'type': 'time_dependent', 'settings': {'duration': '15*fs', 'output_interval': '5*fs'},
+              'stored_fields': [], 'sweep': None},
+    'derived_values': [],
+}
+
+
+def run_3d():
+    """The 3D model through the reference's api.run: flux, per-cw power, DFT layer intensity and the probe series."""
+    sys.path.insert(0, REFERENCE)
+    from open_sol import api
+
+    text = json.dumps(MODEL_3D)
+    out = {'model_json': np.array(text)}
+    with tempfile.TemporaryDirectory() as directory:
+        path = os.path.join(directory, 'model.json')
+        with open(path, 'w', encoding='utf-8') as handle:
+            handle.write(text)
+        results = api.run(api.load(path), backend='cpu')
+        if results.status != 'completed':
+            raise RuntimeError(f'reference run ended {results.status}')
+        for name in ('flux_x', 'flux_z'):
+            result = results.result('em', name)
+            for key in ('wavelength', 'power', 'power_per_cw'):
+                out[f'{name}/{key}'] = np.asarray(result[key], dtype=np.float64)
+        for index in range(3):
+            layer = results.result('em', 'layer', freq_index=index)
+            out[f'layer/intensity/{index}'] = np.asarray(layer['intensity'], dtype=np.float64)
+        out['layer/x_centers'] = np.asarray(layer['x_centers'], dtype=np.float64)
+        out['layer/z'] = np.float64(layer['z'])
+        probe = results.result('em', 'probe')
+        out['probe/t'] = np.asarray(probe['t'], dtype=np.float64)
+        out['probe/values'] = np.asarray(probe['values'], dtype=np.float64)
+        out['summary_ticks'] = np.int64(results.summary()['ticks'])
+        results.close()
+    return out
 
 
 def model_text(polarization):
@@ -65,6 +146,10 @@ def run_case(polarization):
 
 
 def main():
+    out = run_3d()
+    path = os.path.join(FIXTURES, 'model_dipole_3d.npz')
+    np.savez_compressed(path, **out)
+    print(path, 'ticks', int(out['summary_ticks']), 'flux_x', out['flux_x/power'])
     for polarization in ('ez', 'hz'):
         out = run_case(polarization)
         path = os.path.join(FIXTURES, f'model_thin_film_{polarization}.npz')

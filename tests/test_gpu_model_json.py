@@ -42,3 +42,35 @@ def test_thin_film_model_matches_the_reference_run(gpu, polarization, tmp_path):
     with np.load(tmp_path / 'results.npz', allow_pickle=False) as npz:
         ours = {n for n in npz.files if n.split('/')[0] in ('result', 'sweep')}
     assert ours == {str(n) for n in golden['npz_names'] if str(n).split('/')[0] in ('result', 'sweep')}
+
+
+def test_3d_model_matches_the_reference_run(gpu):
+    """A 3D model (dipole over a glass block, a PEC box, two flux planes, a DFT layer, a probe) against the
+    reference's api.run of the same file (float32 CPU path): flux and per-cw power within 1e-4, the layer intensity
+    within 1e-4 relative L2 per wavelength, the probe within 1e-4 of its peak."""
+    from opensol_amd import model as M
+    golden = np.load(os.path.join(FIXTURES, 'model_dipole_3d.npz'))
+    results = M.run(json.loads(str(golden['model_json'])))
+    assert results.summary()['ticks'] == int(golden['summary_ticks'])
+    for name in ('flux_x', 'flux_z'):
+        result = results.result('em', name)
+        for key in ('power', 'power_per_cw'):
+            want = golden[f'{name}/{key}']
+            error = np.max(np.abs(result[key] - want) / np.abs(want))
+            print(f'3D {name} {key}: max rel err {error:.2e}')
+            assert error <= 1e-4, (name, key, error)
+    layer = results.result('em', 'layer')
+    np.testing.assert_allclose(layer['x_centers'], golden['layer/x_centers'], rtol=1e-12)
+    assert layer['z'] == pytest.approx(float(golden['layer/z']), rel=1e-12)
+    for index in range(3):
+        want = golden[f'layer/intensity/{index}']
+        error = np.linalg.norm(layer['intensity_all'][index] - want) / np.linalg.norm(want)
+        print(f'3D layer intensity {index}: rel L2 {error:.2e}')
+        assert error <= 1e-4
+    probe = results.result('em', 'probe')
+    np.testing.assert_allclose(probe['t'], golden['probe/t'], rtol=1e-12)
+    peak = np.max(np.abs(golden['probe/values']))
+    assert peak > 0
+    error = np.max(np.abs(probe['values'] - golden['probe/values'])) / peak
+    print(f'3D probe: max err / peak {error:.2e}')
+    assert error <= 1e-4

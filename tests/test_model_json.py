@@ -116,3 +116,35 @@ def test_results_directory_layout(tmp_path):
     reference_digest = json.loads(str(golden['summary_sweep']))['results'][1]['em.transmitted_flux']
     assert set(saved['results']['em.transmitted_flux']) == set(reference_digest)
     assert json.loads((tmp_path / 'model.json').read_text()) == spec
+
+
+def test_3d_model_builds_the_scripted_case():
+    """The 3D model's build equals the same case through Simulation3D: materials (glass block, PEC box), the dipole,
+    the two flux planes, the DFT layer and the probe; the study runs the reference's step count."""
+    from opensol_amd.fdtd3d import Model3D
+    golden = np.load(os.path.join(FIXTURES, 'model_dipole_3d.npz'))
+    spec = json.loads(str(golden['model_json']))
+    names = M.evaluate_parameters(spec['parameters'])
+    model = M._Built(spec, names).sim.model
+    ref = Model3D(0.72e-6, 0.6e-6, 0.55e-6, 25e-9, npml=8)
+    ref.add_dielectric_box((0.05e-6, 0.05e-6, 0.0), (0.3e-6, 0.35e-6, 0.25e-6), 2.25)
+    ref.add_pec_box((0.45e-6, 0.1e-6, 0.3e-6), (0.6e-6, 0.35e-6, 0.45e-6))
+    ref.add_dipole(0.36e-6, 0.29e-6, 0.27e-6, orientation='z', dipole_moment=1e-12, wavelength=600e-9,
+                   bandwidth=0.5)
+    wavelengths = np.linspace(500e-9, 700e-9, 5)
+    ref.add_flux_plane('x', 0.55e-6, wavelengths=wavelengths)
+    ref.add_flux_plane('z', 0.1e-6, wavelengths=wavelengths)
+    ref.add_dft_plane(0.27e-6, wavelengths=np.linspace(500e-9, 700e-9, 3))
+    ref.add_field_probe(0.2e-6, 0.45e-6, 0.3e-6, 'ez')
+    model.finalize()
+    ref.finalize()
+    assert model.shape == ref.shape and model.dt == ref.dt
+    np.testing.assert_array_equal(model.materials.ids, ref.materials.ids)
+    np.testing.assert_array_equal(model.materials.values(), ref.materials.values())
+    assert [(s.cell, s.component) for s in model.sources] == [(s.cell, s.component) for s in ref.sources]
+    assert [(p.normal, p.edge) for p in model.monitors] == [(p.normal, p.edge) for p in ref.monitors]
+    assert [d.k_grid for d in model.dft_monitors] == [d.k_grid for d in ref.dft_monitors]
+    assert [p.cell for p in model.probes] == [p.cell for p in ref.probes]
+    duration = M.evaluate(spec['study']['settings']['duration'], names)
+    assert steps_for_duration(0.0, model.dt, duration) == int(golden['summary_ticks'])
+    np.testing.assert_allclose(M.C0 / model.monitors[0].frequencies, golden['flux_x/wavelength'], rtol=1e-15)
