@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 5
+#define YEE_ABI_VERSION 6
 #define YEE_MAX_SOURCES 4
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
@@ -244,7 +244,7 @@ typedef struct yee3d_cpml_axis {
 } yee3d_cpml_axis;
 
 #define YEE3D_MAX_SOURCES 4
-#define YEE3D_MAX_PLANES 4
+#define YEE3D_MAX_PLANES 8     /* a flux box is six bounded planes */
 
 /*
  * One 3D instance. Replaces the 3D engine step of physics/em3d/engine3d.py:497-554 (update_h_3d / update_e_3d with

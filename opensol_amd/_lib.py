@@ -6,12 +6,12 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 5
+ABI_VERSION = 6
 MAX_SOURCES = 4
 MAX_MONITORS = 4
 MAX_DFT = 4
 MAX_SOURCES_3D = 4
-MAX_PLANES_3D = 4
+MAX_PLANES_3D = 8
 YEE_EZ, YEE_HZ = 0, 1
 INT64_MAX = (1 << 63) - 1
 

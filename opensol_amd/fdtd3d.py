@@ -17,7 +17,7 @@ from .constants import C0, DTYPE, EPS0, MU0
 from .domain import Domain
 from .fdtd2d import clock_times, steps_for_duration
 from .materials import MaterialMap, semi_implicit
-from .monitors import DFTPlane3D, FieldProbe, FluxPlane, ProbeTable, dft_sample_interval
+from .monitors import DFTPlane3D, FieldProbe, FluxBox, FluxPlane, ProbeTable, dft_sample_interval
 from .sources import PointDipole, resolve_frequencies, resolve_frequency
 from .waveforms import make_temporal
 
@@ -223,6 +223,29 @@ class Model3D:
                             self.kernel_widths, self.shape, self.periodic)
         self.monitors.append(monitor)
         return monitor
+
+    def add_flux_box(self, lo, hi, wavelengths=None, frequencies=None):
+        """Net outward flux through the box between the window cell edges nearest the corners lo, hi [m]: six bounded
+        flux planes (physics/em3d/physics3d.py:447-476)."""
+        bounds = []
+        for axis in range(3):
+            first = self.nearest_edge(axis, min(lo[axis], hi[axis]))
+            last = self.nearest_edge(axis, max(lo[axis], hi[axis]))
+            if last <= first:
+                raise ValueError(f'flux box spans no cell along {AXES[axis]}: corners {lo[axis]:.4g} and '
+                                 f'{hi[axis]:.4g} m snap to the same cell edge')
+            bounds.append((first, last))
+        if len(self.monitors) + 6 > _lib.MAX_PLANES_3D:
+            raise ValueError(f'a flux box needs six of the {_lib.MAX_PLANES_3D} flux planes of a 3D model')
+        freqs = resolve_frequencies(wavelengths, frequencies)
+        faces = []
+        for axis in range(3):
+            for side, sign in ((0, -1.0), (1, 1.0)):
+                plane = FluxPlane(axis, bounds[axis][side], bounds, freqs, float(self.dt), self.kernel_widths,
+                                  self.shape, self.periodic)
+                self.monitors.append(plane)
+                faces.append((sign, plane))
+        return FluxBox(faces)
 
     def add_dft_plane(self, z, wavelengths=None, frequencies=None):
         """Running DFT of E at the cell centres of the window layer containing height z [m]

@@ -12,7 +12,7 @@ physics/em3d/definition.py, physics/em/definition_common.py:1167-1287):
   * geometry kinds rectangle, circle, line, point and box (model/geometry.py:111-262); materials with eps_r and
     sigma_e painted on rectangles, circles (2D) or boxes (3D), in the order the materials are listed;
   * 2D features line_source, flux_monitor, field_probe, dft_monitor and pec_domain; 3D features dipole_source,
-    field_probe, flux_plane, dft_plane and pec_domain;
+    field_probe, flux_plane, flux_box, dft_plane and pec_domain;
   * the time_dependent study (duration; the runner stops at t >= t_end - dt/2) and its sweep over one parameter;
   * results: flux_spectrum (with power_per_cw where every source shares one waveform, physics/em/results.py:
     93-122), timeseries and dft_fields per feature, the last run's fields on the window, results.npz in the
@@ -291,6 +291,10 @@ class _Built:
                                           wavelengths=_wavelengths(params, get))
             if kind == 'dft_plane':
                 return sim.add_dft_plane(get('position', 0.0), wavelengths=_wavelengths(params, get))
+            if kind == 'flux_box':
+                _, p = self._single(feature, ('box',))
+                return sim.add_flux_box((p['x0'], p['y0'], p['z0']), (p['x1'], p['y1'], p['z1']),
+                                        wavelengths=_wavelengths(params, get))
         raise ValueError(f"feature type {kind!r} is not part of the {self.kind} FDTD path here")
 
     # ---- results ----
@@ -299,7 +303,7 @@ class _Built:
         out = {}
         for name, kind, obj in self.features:
             key = f'{self.physics_name}.{name}'
-            if kind in ('flux_monitor', 'flux_plane'):
+            if kind in ('flux_monitor', 'flux_plane', 'flux_box'):
                 out[key] = self._flux_result(obj)
             elif kind == 'field_probe':
                 times, values = obj.get_data()

@@ -230,6 +230,24 @@ class ProbeTable:
         return self.samples[:n_step, index].cpu().numpy()
 
 
+class FluxBox:
+    """Net outward flux through the six cell-edge faces of a box (physics/em3d/monitors3d.py:131-162): one bounded
+    FluxPlane per face over the box's edge ranges, summed with sign -1 on the lo face and +1 on the hi face of each
+    axis."""
+
+    def __init__(self, faces):
+        self.faces = list(faces)   # [(sign, FluxPlane)], x lo, x hi, y lo, y hi, z lo, z hi
+        self.frequencies = self.faces[0][1].frequencies
+        self.n_freqs = self.faces[0][1].n_freqs
+
+    @property
+    def sample_interval(self):
+        return self.faces[0][1].sample_interval
+
+    def flux(self):
+        return sum(sign * plane.flux() for sign, plane in self.faces)
+
+
 def dft_sample_slice(start, stop, stride):
     """Cells a DFT box samples along one axis of [start, stop): every stride-th cell from the middle of the first
     stride-cell block (physics/em/monitors.py:152-155)."""

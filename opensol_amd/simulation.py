@@ -250,6 +250,11 @@ class Simulation3D:
         self._require_unbuilt('add_field_probe')
         return self.model.add_field_probe(x, y, z, component)
 
+    def add_flux_box(self, lo, hi, wavelengths=None, frequencies=None):
+        """Net outward flux through the box between the cell edges nearest the corners lo, hi (physics3d.py:447-451)."""
+        self._require_unbuilt('add_flux_box')
+        return self.model.add_flux_box(lo, hi, wavelengths, frequencies)
+
     def add_dft_plane(self, z, wavelengths=None, frequencies=None):
         """Running DFT of E at the cell centres of the window layer containing height z (physics3d.py:478-486)."""
         self._require_unbuilt('add_dft_plane')

@@ -230,6 +230,17 @@ class PlaneFlux:
         return np.cos(np.pi * self.frequencies * self.dt) * total
 
 
+class FluxBox:
+    """Net outward flux through the six faces of a box of edge ranges (monitors3d.py:131-162): the lo face of each
+    axis with sign -1, the hi face with +1, each a PlaneFlux over the box's ranges."""
+
+    def __init__(self, faces):
+        self.faces = faces
+
+    def flux(self):
+        return sum(sign * plane.flux() for sign, plane in self.faces)
+
+
 class DFTPlane:
     """Running DFT of E_x, E_y, E_z at the cell centres of window layer k (monitors3d.py:165-238): E_x with its lower
     x face, E_y with its lower y face, E_z with the layer below (index 0: the wrapped last one on a periodic axis,
@@ -436,6 +447,29 @@ class Yee3D:
         probe = PointProbe(tuple(self.cell(axis, v) for axis, v in enumerate((x, y, z))), component)
         self.probes.append(probe)
         return probe
+
+    def add_flux_box(self, lo, hi, wavelengths=None, frequencies=None):
+        """Box between the window edges nearest the corners (physics3d.py:447-476)."""
+        if (wavelengths is None) == (frequencies is None):
+            raise ValueError('pass exactly one of wavelengths or frequencies')
+        freqs = (np.asarray(frequencies, dtype=np.float64) if frequencies is not None
+                 else C0 / np.asarray(wavelengths, dtype=np.float64))
+        bounds = []
+        for axis in range(3):
+            first = self.nearest_edge(axis, min(lo[axis], hi[axis]))
+            last = self.nearest_edge(axis, max(lo[axis], hi[axis]))
+            if last <= first:
+                raise ValueError('flux box spans no cell')
+            bounds.append((first, last))
+        faces = []
+        for axis in range(3):
+            for side, sign in ((0, -1.0), (1, 1.0)):
+                plane = PlaneFlux(axis, bounds[axis][side], bounds, np.atleast_1d(freqs), float(self.dt),
+                                  self.widths, self.shape, self.periodic, self.phase_dtype)
+                self.monitors.append(plane)
+                faces.append((sign, plane))
+        self._finalized = False
+        return FluxBox(faces)
 
     def add_dft_plane(self, z, wavelengths=None, frequencies=None):
         """DFT of E at the cell centres of the window layer containing z (physics3d.py:478-486)."""

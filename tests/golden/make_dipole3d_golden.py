@@ -10,6 +10,9 @@ dipole cell after step 120.
 
 Case 'pec' adds a PEC z_lo face, a bare (pmc) z_hi face and a PEC box over the dielectrics.
 
+The cpml and pec cases also have a flux box around the dipole (net outward flux of its six faces), recorded after
+the last step.
+
 Every case also has running-DFT planes of E at the cell centres of one or two z layers (through the dipole; in the
 pec case also layer 0, whose E_z pairs with itself), at the flux planes' wavelengths, recorded after the last step.
 
@@ -37,13 +40,15 @@ SLICE_STEP = 120
 N_STEPS = 300
 CASES = {
     'cpml': dict(faces={}, dipole=(0.36e-6, 0.29e-6, 0.27e-6), orientation='z',
-                 planes=(('x', 0.55e-6), ('z', 0.1e-6)), dft_planes=(0.27e-6,)),
+                 planes=(('x', 0.55e-6), ('z', 0.1e-6)), dft_planes=(0.27e-6,),
+                 flux_box=((0.2e-6, 0.15e-6, 0.12e-6), (0.5e-6, 0.45e-6, 0.42e-6))),
     'periodic_y': dict(faces={'y_lo': 'periodic', 'y_hi': 'periodic'}, dipole=(0.33e-6, 0.21e-6, 0.26e-6),
                        orientation='x', planes=(('x', 0.15e-6), ('z', 0.45e-6)), dft_planes=(0.26e-6,)),
     # a PEC ground face (z_lo), the bare grid edge (z_hi pmc) and a PEC box over the dielectrics
     'pec': dict(faces={'z_lo': 'pec', 'z_hi': 'pmc'}, dipole=(0.36e-6, 0.29e-6, 0.27e-6), orientation='z',
                 planes=(('x', 0.2e-6), ('z', 0.45e-6)),
-                pec_box=((0.45e-6, 0.10e-6, 0.30e-6), (0.60e-6, 0.35e-6, 0.45e-6)), dft_planes=(0.0, 0.27e-6)),
+                pec_box=((0.45e-6, 0.10e-6, 0.30e-6), (0.60e-6, 0.35e-6, 0.45e-6)), dft_planes=(0.0, 0.27e-6),
+                flux_box=((0.25e-6, 0.2e-6, 0.05e-6), (0.45e-6, 0.4e-6, 0.40e-6))),
 }
 FIELDS = ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')
 
@@ -85,6 +90,7 @@ def run_case(spec):
                            wavelength=LAMBDA0, bandwidth=BANDWIDTH)
     planes = [em.add_flux_plane(normal, position, wavelengths=list(WAVELENGTHS)) for normal, position in spec['planes']]
     dft_planes = [em.add_dft_plane(z, wavelengths=list(WAVELENGTHS)) for z in spec['dft_planes']]
+    box = em.add_flux_box(*spec['flux_box'], wavelengths=list(WAVELENGTHS)) if 'flux_box' in spec else None
     em.finalize()
     engine = em.engine
     out = {'shape': np.array(engine.shape), 'dt': np.float64(engine.dt), 'eps_r': eps, 'pec_box_mask': pec,
@@ -105,6 +111,9 @@ def run_case(spec):
         out[f'{name}_{N_STEPS}'] = np.array(getattr(engine.fields, name), dtype=np.float32)
     out['time'] = np.float64(engine.time)
     out['n_steps'] = np.int64(engine.n_step)
+    if box is not None:
+        out['flux_box'] = np.asarray(box.flux(), dtype=np.float64)
+        out['flux_box_edges'] = np.array(box.edge_bounds)
     for index, plane in enumerate(dft_planes):
         out[f'dft{index}_k'] = np.int64(plane.k_grid)
         for name in ('ex', 'ey', 'ez'):

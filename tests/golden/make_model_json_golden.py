@@ -27,7 +27,8 @@ MONITORS = ('reflection_side_flux', 'transmitted_flux')
 
 
 # A small 3D model of the em_fdtd_3d vocabulary (no example of the reference uses only features of this path): a
-# z dipole over a glass block, a PEC box, flux planes normal to x and z, a DFT layer and an E_z probe; 8-cell CPML.
+# z dipole over a glass block, a PEC box, flux planes normal to x and z, a flux box around the dipole, a DFT layer
+# and an E_z probe; 8-cell CPML.
 MODEL_3D = {
     'title': 'Dipole over a glass block (3D)', 'version': 1,
     'parameters': [
@@ -44,6 +45,8 @@ MODEL_3D = {
                                                     'y1': '0.35*um', 'z1': '0.25*um'}},
         {'name': 'metal', 'kind': 'box', 'params': {'x0': '0.45*um', 'y0': '0.1*um', 'z0': '0.3*um', 'x1': '0.6*um',
                                                     'y1': '0.35*um', 'z1': '0.45*um'}},
+        {'name': 'surface', 'kind': 'box', 'params': {'x0': '0.2*um', 'y0': '0.15*um', 'z0': '0.12*um',
+                                                      'x1': '0.42*um', 'y1': '0.45*um', 'z1': '0.42*um'}},
     ],
     'selections': [],
     'materials': [{'name': 'glass', 'library': None, 'properties': {'eps_r': 'n_glass**2'}, 'dispersion': None,
@@ -61,6 +64,8 @@ MODEL_3D = {
             {'name': 'flux_z', 'type': 'flux_plane', 'selection': [],
              'params': {'normal': 'z', 'position': '0.1*um', 'wavelength_start': 5e-07, 'wavelength_stop': 7e-07,
                         'n_wavelengths': 5}, 'enabled': True},
+            {'name': 'box', 'type': 'flux_box', 'selection': ['surface'],
+             'params': {'wavelength_start': 5e-07, 'wavelength_stop': 7e-07, 'n_wavelengths': 5}, 'enabled': True},
             {'name': 'layer', 'type': 'dft_plane', 'selection': [],
              'params': {'position': '0.27*um', 'wavelength_start': 5e-07, 'wavelength_stop': 7e-07,
                         'n_wavelengths': 3}, 'enabled': True},
@@ -90,7 +95,7 @@ def run_3d():
         results = api.run(api.load(path), backend='cpu')
         if results.status != 'completed':
             raise RuntimeError(f'reference run ended {results.status}')
-        for name in ('flux_x', 'flux_z'):
+        for name in ('flux_x', 'flux_z', 'box'):
             result = results.result('em', name)
             for key in ('wavelength', 'power', 'power_per_cw'):
                 out[f'{name}/{key}'] = np.asarray(result[key], dtype=np.float64)

@@ -16,12 +16,14 @@ GOLDEN_BOXES = (((0.05e-6, 0.05e-6, 0.00e-6), (0.30e-6, 0.35e-6, 0.25e-6), 4.0),
                 ((0.20e-6, 0.40e-6, 0.30e-6), (0.50e-6, 0.55e-6, 0.55e-6), 6.5))
 GOLDEN_CASES = {
     'cpml': dict(faces=None, dipole=(0.36e-6, 0.29e-6, 0.27e-6), orientation='z',
-                 planes=(('x', 0.55e-6), ('z', 0.1e-6)), dft_planes=(0.27e-6,)),
+                 planes=(('x', 0.55e-6), ('z', 0.1e-6)), dft_planes=(0.27e-6,),
+                 flux_box=((0.2e-6, 0.15e-6, 0.12e-6), (0.5e-6, 0.45e-6, 0.42e-6))),
     'periodic_y': dict(faces={'y_lo': 'periodic', 'y_hi': 'periodic'}, dipole=(0.33e-6, 0.21e-6, 0.26e-6),
                        orientation='x', planes=(('x', 0.15e-6), ('z', 0.45e-6)), dft_planes=(0.26e-6, 0.5e-6)),
     'pec': dict(faces={'z_lo': 'pec', 'z_hi': 'pmc'}, dipole=(0.36e-6, 0.29e-6, 0.27e-6), orientation='z',
                 planes=(('x', 0.2e-6), ('z', 0.45e-6)),
-                pec_box=((0.45e-6, 0.10e-6, 0.30e-6), (0.60e-6, 0.35e-6, 0.45e-6)), dft_planes=(0.0, 0.27e-6)),
+                pec_box=((0.45e-6, 0.10e-6, 0.30e-6), (0.60e-6, 0.35e-6, 0.45e-6)), dft_planes=(0.0, 0.27e-6),
+                flux_box=((0.25e-6, 0.2e-6, 0.05e-6), (0.45e-6, 0.4e-6, 0.40e-6))),
 }
 
 
@@ -47,6 +49,8 @@ def test_dipole_case_matches_oracle(gpu, name):
     ref, rplanes = _build(lambda *a, **k: Yee3D(*a, **k), spec)
     dft_planes = [sim.add_dft_plane(z, wavelengths=[0.5e-6, 0.6e-6, 0.7e-6]) for z in spec['dft_planes']]
     ref_dft_planes = [ref.add_dft_plane(z, wavelengths=[0.5e-6, 0.6e-6, 0.7e-6]) for z in spec['dft_planes']]
+    boxes = [s.add_flux_box(*spec['flux_box'], wavelengths=[0.5e-6, 0.6e-6, 0.7e-6]) for s in (sim, ref)
+             if 'flux_box' in spec]
     ref.finalize()
     sim.run(n_steps=120)
     ref.run(n_steps=120)
@@ -67,6 +71,11 @@ def test_dipole_case_matches_oracle(gpu, name):
         got, want = plane.flux(), rplane.flux()
         rel = np.max(np.abs(got - want) / np.abs(want))
         print(f'{name} flux {got} oracle {want} rel {rel:.2e}')
+        assert rel < 1e-5
+    if boxes:   # flux box: six bounded planes, net outward flux within 1e-5 of the oracle's
+        got, want = boxes[0].flux(), boxes[1].flux()
+        rel = np.max(np.abs(got - want) / np.abs(want))
+        print(f'{name} flux box {got} oracle {want} rel {rel:.2e}')
         assert rel < 1e-5
     # DFT planes of E at the cell centres: relative L2 <= 1e-5 per component (exactly 0 where the oracle's is, the
     # tangential E of a PEC face)

@@ -45,14 +45,14 @@ def test_thin_film_model_matches_the_reference_run(gpu, polarization, tmp_path):
 
 
 def test_3d_model_matches_the_reference_run(gpu):
-    """A 3D model (dipole over a glass block, a PEC box, two flux planes, a DFT layer, a probe) against the
+    """A 3D model (dipole over a glass block, a PEC box, two flux planes, a flux box, a DFT layer, a probe) against the
     reference's api.run of the same file (float32 CPU path): flux and per-cw power within 1e-4, the layer intensity
     within 1e-4 relative L2 per wavelength, the probe within 1e-4 of its peak."""
     from opensol_amd import model as M
     golden = np.load(os.path.join(FIXTURES, 'model_dipole_3d.npz'))
     results = M.run(json.loads(str(golden['model_json'])))
     assert results.summary()['ticks'] == int(golden['summary_ticks'])
-    for name in ('flux_x', 'flux_z'):
+    for name in ('flux_x', 'flux_z', 'box'):
         result = results.result('em', name)
         for key in ('power', 'power_per_cw'):
             want = golden[f'{name}/{key}']

@@ -215,3 +215,22 @@ def test_dft_plane_layer_matches_the_oracle(z):
     assert plane.pairs == ((0, False), (1, True), (2, False))
     with pytest.raises(ValueError):
         model.add_dft_plane(0.7e-6, wavelengths=[6e-7])
+
+
+def test_flux_box_faces_match_the_oracle():
+    """Six bounded planes on the window edges nearest the corners, lo faces -1 and hi faces +1, with the oracle's
+    pair ranges (physics/em3d/physics3d.py:447-476, monitors3d.py:131-162)."""
+    model, ref = _pair({'y_lo': 'periodic', 'y_hi': 'periodic'})
+    box = model.add_flux_box((0.5e-6, 0.14e-6, 0.12e-6), (0.2e-6, 0.46e-6, 0.41e-6), wavelengths=[6e-7])
+    ref_box = ref.add_flux_box((0.5e-6, 0.14e-6, 0.12e-6), (0.2e-6, 0.46e-6, 0.41e-6), wavelengths=[6e-7])
+    assert [s for s, _ in box.faces] == [s for s, _ in ref_box.faces] == [-1.0, 1.0] * 3
+    for (_, plane), (_, ref_plane) in zip(box.faces, ref_box.faces):
+        assert (plane.normal, plane.edge) == (ref_plane.normal, ref_plane.edge)
+        for pair, ref_pair in zip(plane.pairs, ref_plane.pairs):
+            a, b = sorted((ref_pair['face_axis'], ref_pair['cell_axis']))
+            slices = {ref_pair['face_axis']: ref_pair['face_slice'], ref_pair['cell_axis']: ref_pair['cell_slice']}
+            assert pair['lo'] == (slices[a].start, slices[b].start)
+            assert pair['hi'] == (slices[a].stop, slices[b].stop)
+            np.testing.assert_array_equal(pair['area'], ref_pair['area'])
+    with pytest.raises(ValueError):
+        model.add_flux_box((0.2e-6, 0.1e-6, 0.1e-6), (0.21e-6, 0.3e-6, 0.3e-6), wavelengths=[6e-7])

@@ -134,6 +134,7 @@ def test_3d_model_builds_the_scripted_case():
     wavelengths = np.linspace(500e-9, 700e-9, 5)
     ref.add_flux_plane('x', 0.55e-6, wavelengths=wavelengths)
     ref.add_flux_plane('z', 0.1e-6, wavelengths=wavelengths)
+    ref.add_flux_box((0.2e-6, 0.15e-6, 0.12e-6), (0.42e-6, 0.45e-6, 0.42e-6), wavelengths=wavelengths)
     ref.add_dft_plane(0.27e-6, wavelengths=np.linspace(500e-9, 700e-9, 3))
     ref.add_field_probe(0.2e-6, 0.45e-6, 0.3e-6, 'ez')
     model.finalize()

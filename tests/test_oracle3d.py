@@ -72,6 +72,8 @@ def test_fields_and_flux_match_golden(name):
              (x, y, 0.4e-6)]
     probes = [sim.add_field_probe(*spot, component=f) for spot, f in zip(spots * 2, G.FIELDS * 2)]
     dft_planes = [sim.add_dft_plane(z, wavelengths=list(G.WAVELENGTHS)) for z in G.CASES[name]['dft_planes']]
+    box = (sim.add_flux_box(*G.CASES[name]['flux_box'], wavelengths=list(G.WAVELENGTHS))
+           if 'flux_box' in G.CASES[name] else None)
     sim.run(n_steps=G.SLICE_STEP)
     i, j, k = source.cell
     for f in G.FIELDS:
@@ -81,6 +83,11 @@ def test_fields_and_flux_match_golden(name):
     assert sim.time == float(golden['time'])
     for f in G.FIELDS:
         assert _rel(golden[f'{f}_{G.N_STEPS}'], sim.fields[f]) < 5e-4, f
+    # flux box: the six faces' edges and the net outward flux, within the planes' 2e-6
+    if box is not None:
+        assert [[face.edge for _, face in box.faces[2 * a:2 * a + 2]] for a in range(3)] == \
+            golden['flux_box_edges'].tolist()
+        np.testing.assert_allclose(box.flux(), golden['flux_box'], rtol=2e-6)
     # DFT planes of E at the cell centres (the pec case's layer 0 pairs E_z with itself): within 1e-5 relative L2
     # of the reference's complex64 spectra (measured 1.5e-7 to 2.1e-6)
     for index, plane in enumerate(dft_planes):
