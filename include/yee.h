@@ -26,10 +26,13 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 6
+#define YEE_ABI_VERSION 7
 #define YEE_MAX_SOURCES 4
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
+#define YEE_MAX_POLES 4
+#define YEE_DRUDE 1
+#define YEE_LORENTZ 2
 
 #define YEE_OK 0
 #define YEE_EINVAL (-1)   /* invalid argument or descriptor */
@@ -139,6 +142,25 @@ typedef struct yee_dft_region {
     double* acc;             /* [n_comp][n_freq][count0 * count1 * count2][2 (re, im)] */
 } yee_dft_region;
 
+/*
+ * Dispersive pole of a 2D E_z instance on the cell box [i0, i1) x [j0, j1), in the fused E update of the step kernel
+ * (auxiliary differential equations in velocity form, from E^n before the update; corrections after the Yee E update
+ * with its CPML, before the sources):
+ *   Drude (YEE_DRUDE):     J += damping J + drive E^n;                     E -= cb J
+ *   Lorentz (YEE_LORENTZ): dP += damping dP - restoring P + drive E^n; P += dP;  E -= cb inv_dt dP
+ * cb is the cell's E coefficient (0 in PEC cells); drive holds the pole's drive coefficient times its mask (0 outside
+ * the pole's cells, so their state stays 0). The state is read and written in place by the owner of its cell.
+ * Replaces physics/em/solvers/fdtd_ez_2d.py:123-163, 318-358 and the kernels of fdtd_ez_2d_kernels.py:176-264;
+ * coefficients physics/em/materials.py:132-190.
+ */
+typedef struct yee_pole {
+    int32_t kind;            /* YEE_DRUDE or YEE_LORENTZ */
+    int32_t i0, i1, j0, j1;
+    float damping, restoring, inv_dt;
+    const float* drive;      /* (i1 - i0) * (j1 - j0), row-major */
+    float* state[2];         /* Drude: J (state[1] unused); Lorentz: P, dP; (i1 - i0) * (j1 - j0) each */
+} yee_pole;
+
 typedef struct yee2d_desc {
     int32_t nx, ny;
     int32_t pitch;           /* row stride in elements of every (nx, *) array: a multiple of 4, >= ny */
@@ -174,6 +196,9 @@ typedef struct yee2d_desc {
     int32_t n_dft;
     int32_t pad3_;
     yee_dft_region dft[YEE_MAX_DFT];
+    int32_t n_poles;         /* E_z only */
+    int32_t pad4_;
+    yee_pole pole[YEE_MAX_POLES];
     yee_probes probes;
     int64_t* clock;          /* device int64: absolute step index n of the next step */
     int64_t step_limit;      /* steps n >= step_limit are skipped (batched instances); INT64_MAX for none */
@@ -284,7 +309,7 @@ typedef struct yee_ctx yee_ctx;
 /* Library ABI version (YEE_ABI_VERSION) and sizeof checks for bindings. */
 int32_t yee_abi_version(void);
 /* 0: yee2d_desc, 1: yee_line_source, 2: yee_flux_line, 3: yee_cpml_axis, 4: yee3d_desc, 5: yee_point_source,
- * 6: yee_flux_plane, 7: yee3d_cpml_axis, 8: yee3d_axis_row, 9: yee_probes, 10: yee_dft_region */
+ * 6: yee_flux_plane, 7: yee3d_cpml_axis, 8: yee3d_axis_row, 9: yee_probes, 10: yee_dft_region, 11: yee_pole */
 size_t yee_sizeof(int32_t which);
 const char* yee_last_error(void);
 

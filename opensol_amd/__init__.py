@@ -8,9 +8,10 @@ or without a GPU raises.
 from .constants import C0, DTYPE, EPS0, ETA0, MU0
 from .waveforms import ContinuousWave, GaussianPulse, RickerWavelet
 from .boundaries import BoundaryConfig
+from .dispersion import DrudePole, LorentzPole
 from .monitors import DFTMonitor, DFTPlane3D, FieldProbe, FluxMonitor, FluxPlane
 from .sources import LineSource, PointDipole
 from .simulation import Simulation, Simulation3D
 
 __all__ = ['C0', 'DTYPE', 'EPS0', 'ETA0', 'MU0', 'ContinuousWave', 'GaussianPulse', 'RickerWavelet',
-           'BoundaryConfig', 'DFTMonitor', 'DFTPlane3D', 'FieldProbe', 'FluxMonitor', 'FluxPlane', 'LineSource', 'PointDipole', 'Simulation', 'Simulation3D']
+           'BoundaryConfig', 'DrudePole', 'LorentzPole', 'DFTMonitor', 'DFTPlane3D', 'FieldProbe', 'FluxMonitor', 'FluxPlane', 'LineSource', 'PointDipole', 'Simulation', 'Simulation3D']

@@ -6,10 +6,12 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 6
+ABI_VERSION = 7
 MAX_SOURCES = 4
 MAX_MONITORS = 4
 MAX_DFT = 4
+MAX_POLES = 4
+YEE_DRUDE, YEE_LORENTZ = 1, 2
 MAX_SOURCES_3D = 4
 MAX_PLANES_3D = 8
 YEE_EZ, YEE_HZ = 0, 1
@@ -45,6 +47,11 @@ class DftRegionC(ctypes.Structure):
                 ('n_samples', c_i64), ('acc', c_ptr)]
 
 
+class PoleC(ctypes.Structure):
+    _fields_ = [('kind', c_i32), ('i0', c_i32), ('i1', c_i32), ('j0', c_i32), ('j1', c_i32), ('damping', c_f32),
+                ('restoring', c_f32), ('inv_dt', c_f32), ('drive', c_ptr), ('state', c_ptr * 2)]
+
+
 class Desc2DC(ctypes.Structure):
     _fields_ = [('nx', c_i32), ('ny', c_i32), ('pitch', c_i32), ('polarization', c_i32), ('periodic_x', c_i32),
                 ('periodic_y', c_i32), ('rows_per_tile', c_i32), ('lane_columns', c_i32), ('field', (c_ptr * 3) * 2),
@@ -54,6 +61,7 @@ class Desc2DC(ctypes.Structure):
                 ('layer_y_lo', c_i32), ('layer_y_hi', c_i32), ('cpml', CpmlAxisC * 2), ('n_sources', c_i32), ('n_monitors', c_i32), ('sample_interval', c_i32),
                 ('pad2_', c_i32), ('source', LineSourceC * MAX_SOURCES), ('monitor', FluxLineC * MAX_MONITORS),
                 ('n_dft', c_i32), ('pad3_', c_i32), ('dft', DftRegionC * MAX_DFT),
+                ('n_poles', c_i32), ('pad4_', c_i32), ('pole', PoleC * MAX_POLES),
                 ('probes', ProbesC), ('clock', c_ptr), ('step_limit', c_i64)]
 
 
@@ -87,7 +95,7 @@ class Desc3DC(ctypes.Structure):
 
 
 STRUCTS = (Desc2DC, LineSourceC, FluxLineC, CpmlAxisC, Desc3DC, PointSourceC, FluxPlaneC, Cpml3DAxisC, AxisRow3DC,
-           ProbesC, DftRegionC)
+           ProbesC, DftRegionC, PoleC)
 
 # Every entry point include/yee.h declares: (name, restype, argtypes)
 SIGNATURES = (

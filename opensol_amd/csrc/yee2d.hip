@@ -406,6 +406,32 @@ struct EzTile {
             }
             ez_v[k] = j < ny ? v : 0.f;
         }
+        // dispersive poles from E^n, then their corrections of the updated E (before the sources)
+        for (int p = 0; p < d.n_poles; ++p) {
+            const yee_pole& pl = d.pole[p];
+            if (i < pl.i0 || i >= pl.i1) continue;
+            const int64_t base = (int64_t)(i - pl.i0) * (pl.j1 - pl.j0) - pl.j0;
+#pragma unroll
+            for (int k = 0; k < VEC; ++k) {
+                const int j = col.jb + k;
+                if (j < pl.j0 || j >= pl.j1) continue;
+                const int64_t q = base + j;
+                const float drive = pl.drive[q];
+                if (pl.kind == YEE_DRUDE) {
+                    float jz = pl.state[0][q];
+                    jz += pl.damping * jz + drive * e_cur[k];
+                    pl.state[0][q] = jz;
+                    ez_v[k] -= c[k].y * jz;
+                } else {
+                    const float pz = pl.state[0][q];
+                    float dpz = pl.state[1][q];
+                    dpz += pl.damping * dpz - pl.restoring * pz + drive * e_cur[k];
+                    pl.state[0][q] = pz + dpz;
+                    pl.state[1][q] = dpz;
+                    ez_v[k] -= c[k].y * pl.inv_dt * dpz;
+                }
+            }
+        }
         // sheet sources at t + dt/2: E_z += float32(value * profile)
         for (int s = 0; s < d.n_sources; ++s) {
             const yee_line_source& src = d.source[s];

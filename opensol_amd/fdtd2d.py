@@ -15,6 +15,7 @@ from .boundaries import CPMLAxisTables, curl_spacings, resolve_boundaries
 from .constants import C0, DTYPE, EPS0, MU0
 from .domain import Domain
 from .materials import MaterialMap, semi_implicit
+from .dispersion import DrudePole, LorentzPole, bounding_box, pole_step_term
 from .monitors import DFTMonitor, FieldProbe, FluxMonitor, ProbeTable, dft_sample_interval
 from .sources import LineSource, complex_permittivity, resolve_frequencies, resolve_frequency
 from .waveforms import make_temporal
@@ -58,6 +59,7 @@ class Model2D:
         self.pec_regions = []   # bool (nx, ny) masks, applied over the dielectrics at finalize
         self.probes = []
         self.dft_monitors = []
+        self.poles = []   # (DrudePole or LorentzPole, bool (nx, ny) mask)
 
     @property
     def shape(self):
@@ -101,6 +103,62 @@ class Model2D:
         probe = FieldProbe(self.domain.pos_to_cell(x, y), field_component)
         self.probes.append(probe)
         return probe
+
+    # ---- dispersive poles (E_z) ----------------------------------------------------------------------------------
+    def rect_mask(self, x0, y0, x1, y1):
+        """bool (nx, ny): cells whose float32 centre lies in [x_min, x_max) x [y_min, y_max), compared in float32
+        (core/domain.py:100-112, 255-260)."""
+        xc, yc = (c.astype(DTYPE) for c in self.domain.cell_centers())
+        x_lo, x_hi = DTYPE(min(x0, x1)), DTYPE(max(x0, x1))
+        y_lo, y_hi = DTYPE(min(y0, y1)), DTYPE(max(y0, y1))
+        return ((xc >= x_lo) & (xc < x_hi))[:, None] & ((yc >= y_lo) & (yc < y_hi))[None, :]
+
+    def circle_mask(self, cx, cy, radius):
+        """bool (nx, ny): cells whose float32 centre lies within `radius` of (cx, cy), in float32 (core/domain.py:
+        262-268)."""
+        if not radius >= 0:
+            raise ValueError(f'circle radius must be >= 0, got {radius}')
+        xc, yc = (c.astype(DTYPE) for c in self.domain.cell_centers())
+        return (xc[:, None] - DTYPE(cx)) ** 2 + (yc[None, :] - DTYPE(cy)) ** 2 <= DTYPE(radius) ** 2
+
+    def add_drude_rectangle(self, x0, y0, x1, y1, drude):
+        """Drude metal on the rectangle's cells, its eps_inf written as their permittivity
+        (physics/em/__init__.py:329-340)."""
+        self._add_pole(drude, self.rect_mask(x0, y0, x1, y1), background=True)
+
+    def add_drude_circle(self, cx, cy, radius, drude):
+        self._add_pole(drude, self.circle_mask(cx, cy, radius), background=True)
+
+    def add_dispersive(self, pole, mask):
+        """Drude or Lorentz pole on the bool (nx, ny) mask over the cells' own permittivity
+        (physics/em/__init__.py:342-355)."""
+        self._add_pole(pole, mask, background=False)
+
+    def _add_pole(self, pole, mask, background):
+        if not isinstance(pole, (DrudePole, LorentzPole)):
+            raise TypeError(f'add_dispersive needs a DrudePole or LorentzPole, got {type(pole).__name__}')
+        if self.polarization != 'ez':
+            raise ValueError('dispersive poles of this FDTD path are for the E_z polarisation')
+        mask = np.asarray(mask)
+        if mask.dtype != np.bool_ or mask.shape != self.shape:
+            raise ValueError(f'pole mask must be a bool array of shape {self.shape}, got {mask.dtype} {mask.shape}')
+        if len(self.poles) >= _lib.MAX_POLES:
+            raise ValueError(f'at most {_lib.MAX_POLES} dispersive poles per 2D model')
+        if background:   # eps_r = eps_inf under the mask, the other material values kept
+            materials = self.materials
+            for old in np.unique(materials.ids[mask]):
+                values = materials.materials[int(old)]
+                new = materials.material_id(pole.eps_inf, pole.eps_inf, pole.eps_inf, *values[3:])
+                materials.ids[mask & (materials.ids == old)] = new
+        self.poles.append((pole, mask.copy()))
+        self.materials.version += 1
+
+    def pole_term(self):
+        """float64 (nx, ny): the summed stability terms of the poles on each cell (0 without)."""
+        term = np.zeros(self.shape)
+        for pole, mask in self.poles:
+            term[mask] += pole_step_term(pole, float(self.dt))
+        return term
 
     def add_pec_region(self, mask):
         """Perfect electric conductor on the bool (nx, ny) cell mask (physics/em/__init__.py:372-376): E_z pinned to 0
@@ -153,6 +211,15 @@ class Model2D:
         eps = values[:, 0] if self.polarization == 'ez' else np.minimum(values[:, 0], values[:, 1])
         dx2 = float(self.domain.dx) ** 2
         required = np.float64(DTYPE((C0 * float(self.dt)) ** 2 / dx2)) * 2.0
+        if self.poles:   # per cell: eps mu_r >= the light term + mu_r x the poles' terms (physics/em/__init__.py:242-270)
+            cells = self.materials.values()[self.materials.ids]
+            keep = ~self.materials.pec_flags()[self.materials.ids]
+            ratio = cells[..., 0] * cells[..., 4] / (required + cells[..., 4] * self.pole_term())
+            worst = float(ratio[keep].min()) if keep.any() else 1.0
+            if worst < 1.0 - 1e-6:
+                factor = math.floor(float(self.grid.courant) * math.sqrt(max(worst, 0.0)) * 1e3) / 1e3
+                raise ValueError(f'a cell with dispersive poles is unstable at this time step (eps mu_r / requirement '
+                                 f'= {worst:.4g}); lower courant_factor to <= {factor:.3f}')
         ratio = eps * values[:, 4] / required
         worst = float(ratio.min())
         if worst < 1.0 - 1e-6:
@@ -373,6 +440,19 @@ class Engine2D:
         self.waves = [None] * len(model.sources)
         self.phases = [None] * len(model.monitors)
         self.dft_phases = [None] * len(model.dft_monitors)
+        self.poles = []
+        pec_cells = model.materials.pec_flags()[model.materials.ids]
+        for pole, mask in model.poles:
+            box = bounding_box(mask)
+            if box is None:
+                continue
+            (i0, i1), (j0, j1) = box
+            coefficients = pole.ade_coefficients(float(model.dt))
+            weight = (mask & ~pec_cells)[i0:i1, j0:j1].astype(np.float32)
+            drive = torch.from_numpy(np.ascontiguousarray(weight * coefficients[-1])).to(dev)
+            states = [torch.zeros((i1 - i0, j1 - j0), dtype=f32, device=dev)
+                      for _ in range(1 if isinstance(pole, DrudePole) else 2)]
+            self.poles.append(dict(pole=pole, box=box, coefficients=coefficients, drive=drive, states=states))
         components = COMPONENTS[model.polarization]
         self.probes = ProbeTable(model.probes, [p.cell[0] * pitch + p.cell[1] for p in model.probes],
                                  [components.index(p.field_component) for p in model.probes], dev)
@@ -480,6 +560,18 @@ class Engine2D:
         d.n_dft = len(model.dft_monitors)
         for k, monitor in enumerate(model.dft_monitors):
             monitor.fill(d.dft[k], COMPONENTS[model.polarization], self.dft_phases[k], self.dft_accumulators[k])
+        d.n_poles = len(self.poles)
+        for k, entry in enumerate(self.poles):
+            pl = d.pole[k]
+            (pl.i0, pl.i1), (pl.j0, pl.j1) = entry['box']
+            drude = isinstance(entry['pole'], DrudePole)
+            pl.kind = _lib.YEE_DRUDE if drude else _lib.YEE_LORENTZ
+            pl.damping = float(entry['coefficients'][0])
+            pl.restoring = 0.0 if drude else float(entry['coefficients'][1])
+            pl.inv_dt = float(np.float32(1.0) / np.float32(model.dt))
+            pl.drive = entry['drive'].data_ptr()
+            for q, state in enumerate(entry['states']):
+                pl.state[q] = state.data_ptr()
         self.probes.fill(d.probes)
         d.clock = self.clock.data_ptr()
         d.step_limit = _lib.INT64_MAX

@@ -78,6 +78,22 @@ class Simulation:
         """PEC on a bool (nx, ny) cell mask (physics/em/__init__.py:372-376), kept over every dielectric."""
         self.model.add_pec_region(mask)
 
+    def add_drude_rectangle(self, x0, y0, x1, y1, drude):
+        """Drude metal on the rectangle (cell centres in [x0, x1) x [y0, y1)), eps_r = its eps_inf there
+        (physics/em/__init__.py:329-340); E_z only."""
+        self._require_unbuilt('add_drude_rectangle')
+        self.model.add_drude_rectangle(x0, y0, x1, y1, drude)
+
+    def add_drude_circle(self, cx, cy, radius, drude):
+        self._require_unbuilt('add_drude_circle')
+        self.model.add_drude_circle(cx, cy, radius, drude)
+
+    def add_dispersive(self, pole, mask):
+        """Drude or Lorentz pole on a bool (nx, ny) cell mask over the cells' own eps_r (physics/em/__init__.py:
+        342-355); E_z only."""
+        self._require_unbuilt('add_dispersive')
+        self.model.add_dispersive(pole, mask)
+
     def add_pec_rectangle(self, x0, y0, x1, y1):
         """PEC on the rectangle raster of add_dielectric_rectangle (this build's convenience over add_pec_region)."""
         self.model.add_pec_rectangle(x0, y0, x1, y1)

@@ -406,6 +406,21 @@ class DFTBox:
         self.spectra['ey'] += ph * (half * (ey[box] + ey[np.ix_(self.x_partners, self.ys)]))
 
 
+def drude_coefficients(omega_p, gamma, dt):
+    """(damping, drive) float32 of J += damping J + drive E^n (physics/em/materials.py:132-149)."""
+    dt, gamma, wp = float(dt), float(F32(gamma)), float(F32(omega_p))
+    denom = 1.0 + gamma * dt / 2.0
+    return F32(-gamma * dt / denom), F32((EPS0 * wp ** 2 * dt) / denom)
+
+
+def lorentz_coefficients(delta_eps, omega_0, gamma, dt):
+    """(damping, restoring, drive) float32 of dP += damping dP - restoring P + drive E^n, P += dP
+    (physics/em/materials.py:160-180)."""
+    dt, w0, g, de = float(dt), float(F32(omega_0)), float(F32(gamma)), float(F32(delta_eps))
+    denom = 1.0 + g * dt / 2.0
+    return F32(-g * dt / denom), F32(w0 ** 2 * dt ** 2 / denom), F32(EPS0 * de * w0 ** 2 * dt ** 2 / denom)
+
+
 class PointProbe:
     """Raw Yee samples of one component at one cell after every step, stamped with the time after the step
     (core/monitors.py:194-283; engine3d.py:544-548 records the probe rows after the step and stamps self.time)."""
@@ -464,6 +479,7 @@ class Yee2D:
         self.materials.update(mu_r=np.ones(shape, F32), sigma_m=np.zeros(shape, F32))
         self.sources, self.monitors, self.probes = [], [], []
         self.pec_regions = []
+        self.poles = []
         if polarization == 'hz' and any(kind == 'pec' for kind in self.bc.faces.values()):
             raise ValueError("the oracle's 'pec' faces are the E_z wall rows")
         self.n_step, self.time = 0, 0.0
@@ -507,6 +523,26 @@ class Yee2D:
         for name in names:
             self.materials[name][mask] = F32(eps_r)
         self.materials['sigma_e'][mask] = F32(sigma_e)
+        self._finalized = False
+
+    def add_drude_rectangle(self, x0, y0, x1, y1, omega_p, gamma, eps_inf=1.0):
+        """Drude pole on the rectangle with eps_r = eps_inf there (physics/em/__init__.py:329-340)."""
+        self._add_pole(('drude', omega_p, gamma), self.geo.rect_mask(x0, y0, x1, y1), eps_inf)
+
+    def add_dispersive_drude(self, mask, omega_p, gamma):
+        self._add_pole(('drude', omega_p, gamma), mask, None)
+
+    def add_dispersive_lorentz(self, mask, delta_eps, omega_0, gamma):
+        """Lorentz pole over the cells' own eps_r (physics/em/__init__.py:342-355)."""
+        self._add_pole(('lorentz', delta_eps, omega_0, gamma), mask, None)
+
+    def _add_pole(self, pole, mask, eps_inf):
+        if self.polarization != 'ez':
+            raise ValueError("the oracle's poles are E_z only")
+        mask = np.asarray(mask, dtype=bool)
+        if eps_inf is not None:
+            self.materials['eps_r'][mask] = F32(eps_inf)
+        self.poles.append({'pole': pole, 'mask': mask.copy()})
         self._finalized = False
 
     def add_pec_region(self, mask):
@@ -643,6 +679,15 @@ class Yee2D:
             self.keep = ~pec
             self.ca = self.ca * self.keep
             self.cb = self.cb * self.keep
+            # poles (fdtd_ez_2d.py:123-163): drive = mask x PEC keep x the float32 drive coefficient
+            for entry in self.poles:
+                kind = entry['pole'][0]
+                coefficients = (drude_coefficients(*entry['pole'][1:], self.dt) if kind == 'drude'
+                                else lorentz_coefficients(*entry['pole'][1:], self.dt))
+                entry['coefficients'] = tuple(real(c) for c in coefficients)
+                entry['drive'] = (entry['mask'] & self.keep).astype(real) * real(coefficients[-1])
+                if 'state' not in entry:
+                    entry['state'] = [np.zeros(self.geo.shape, dtype=real) for _ in range(1 if kind == 'drude' else 2)]
         else:
             half = real(0.5)
             pj = np.roll(np.arange(self.ny), -1) if self.bc.periodic('y') else np.minimum(np.arange(self.ny) + 1,
@@ -725,6 +770,17 @@ class Yee2D:
             hy[-1] = da[-1] * hy[-1] + db[-1] * (ez[0] - ez[-1]) * self.inv_dx_h[-1]
         for side in self.sides:   # apply_h: x sides (ez, db, hy), y sides (ez, db, hx) (boundaries.py:752-756)
             side.apply('h', ez, db, hy if side.axis == 'x' else hx)
+        # pole updates from E^n (fdtd_ez_2d.py:318-345)
+        for entry in self.poles:
+            if entry['pole'][0] == 'drude':
+                damping, _ = entry['coefficients']
+                j = entry['state'][0]
+                j += damping * j + entry['drive'] * ez
+            else:
+                damping, restoring, _ = entry['coefficients']
+                p, dp = entry['state']
+                dp += damping * dp - restoring * p + entry['drive'] * ez
+                p += dp
         # update_e (fdtd_ez_2d_kernels.py:125-140)
         curl = np.empty_like(ez)
         curl[1:] = (hy[1:] - hy[:-1]) * self.inv_dx_e[1:, None]
@@ -736,6 +792,14 @@ class Yee2D:
         ez += curl
         for side in self.sides:   # apply_e: x sides (hy, cb, ez), y sides (hx, cb, ez) (boundaries.py:758-762)
             side.apply('e', hy if side.axis == 'x' else hx, cb, ez)
+        # pole corrections (fdtd_ez_2d.py:347-358): Drude E -= cb J_total, Lorentz E -= cb / dt dP
+        drude = [entry['state'][0] for entry in self.poles if entry['pole'][0] == 'drude']
+        if drude:
+            ez -= cb * sum(drude[1:], drude[0].copy())
+        inv_dt = self.dtype(1.0) / self.dtype(self.dt)
+        for entry in self.poles:
+            if entry['pole'][0] == 'lorentz':
+                ez -= cb * inv_dt * entry['state'][1]
 
     def _step_hz(self):
         f = self.fields

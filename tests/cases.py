@@ -91,3 +91,39 @@ def rel_l2(a, b):
     b = np.asarray(b, dtype=np.float64)
     norm = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / norm) if norm > 0 else float(np.linalg.norm(a))
+
+
+# 2D E_z case with dispersive poles (tests/golden/make_poles2d_golden.py writes it from the reference): a Drude
+# rectangle (eps_inf 2) under a PEC corner, a Lorentz disc over a dielectric rectangle, a line source and a flux line.
+POLES2D = dict(size_x=2e-6, size_y=1e-6, dx=25e-9, npml=10, boundaries='periodic_y',
+               drude_rect=(1.0e-6, 0.2e-6, 1.3e-6, 0.7e-6), drude=dict(omega_p=5e15, gamma=1e14, eps_inf=2.0),
+               pec_rect=(1.2e-6, 0.6e-6, 1.4e-6, 0.8e-6), dielectric=((0.5e-6, 0.1e-6, 0.9e-6, 0.9e-6), 2.0),
+               lorentz_disc=((0.7e-6, 0.5e-6), 0.15e-6), lorentz=dict(delta_eps=1.5, omega_0=3e15, gamma=2e14),
+               source=dict(axis='y', position=0.35e-6, wavelength=600e-9, bandwidth=0.8),
+               monitor=dict(axis='y', position=1.6e-6, span=(0.0, 1e-6), wavelengths=[500e-9, 600e-9, 700e-9]),
+               steps=(300, 800))
+
+
+def build_poles2d(target, **options):
+    """The pole case on the oracle Yee2D, opensol_amd.Simulation or Model2D; returns (sim, monitor)."""
+    spec = POLES2D
+    sim = target(spec['size_x'], spec['size_y'], spec['dx'], npml=spec['npml'], boundaries=spec['boundaries'],
+                 polarization='ez', **options)
+    (x0, y0, x1, y1), eps = spec['dielectric']
+    sim.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
+    (cx, cy), radius = spec['lorentz_disc']
+    if hasattr(sim, 'geo'):   # the oracle
+        sim.add_drude_rectangle(*spec['drude_rect'], **spec['drude'])
+        xc, yc = (c.astype(np.float32) for c in sim.geo.centres)
+        disc = (xc[:, None] - np.float32(cx)) ** 2 + (yc[None, :] - np.float32(cy)) ** 2 <= np.float32(radius) ** 2
+        sim.add_dispersive_lorentz(disc, **spec['lorentz'])
+        sim.add_pec_region(sim.geo.rect_mask(*spec['pec_rect']))
+    else:
+        from opensol_amd import DrudePole, LorentzPole
+        sim.add_drude_rectangle(*spec['drude_rect'], DrudePole(**spec['drude']))
+        model = getattr(sim, 'model', sim)   # a Simulation or a bare Model2D
+        sim.add_dispersive(LorentzPole(**spec['lorentz']), model.circle_mask(cx, cy, radius))
+        sim.add_pec_region(model.rect_mask(*spec['pec_rect']))
+    sim.add_line_source(**spec['source'])
+    monitor = sim.add_flux_monitor(**spec['monitor'])
+    return sim, monitor

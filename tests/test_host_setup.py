@@ -217,3 +217,31 @@ def test_dft_monitor_cells_match_the_oracle(polarization):
         model.add_dft_monitor(0.1e-6, 0.1e-6, 2.5e-6, 0.5e-6, wavelengths=[5e-7])
     with pytest.raises(ValueError):
         model.add_dft_monitor(0.1e-6, 0.1e-6, 0.5e-6, 0.5e-6, wavelengths=[5e-7], downsample=0)
+
+
+def test_pole_masks_background_and_drive_match_the_oracle():
+    """The pole case: Drude rectangle and Lorentz disc masks (float32 centres, as the reference's rect_mask and
+    circle_mask), the eps_inf background under the Drude cells, the PEC cells over the Drude corner, and the Courant
+    bound with the poles' terms."""
+    from tests.cases import build_poles2d
+    from opensol_amd import DrudePole
+    from opensol_amd.dispersion import pole_step_term
+    model, _ = build_poles2d(Model2D)
+    ref, _ = build_poles2d(Yee2D)
+    model.finalize()
+    ref.finalize()
+    for (pole, mask), entry in zip(model.poles, ref.poles):
+        np.testing.assert_array_equal(mask, entry['mask'])
+        coefficients = pole.ade_coefficients(float(model.dt))
+        assert tuple(float(c) for c in coefficients) == tuple(float(c) for c in entry['coefficients'])
+    pec = model.materials.pec_flags()[model.materials.ids]
+    np.testing.assert_array_equal(pec, ref.pec_mask())
+    np.testing.assert_array_equal(model.materials.property_array(0)[~pec], ref.materials['eps_r'][~pec])
+    assert pole_step_term(DrudePole(5e15, 1e14), 1e-17) == (float(np.float32(5e15)) * 1e-17) ** 2 / 4
+    unstable = Model2D(1e-6, 1e-6, 25e-9, npml=5)
+    unstable.add_drude_rectangle(0.2e-6, 0.2e-6, 0.6e-6, 0.6e-6, DrudePole(1.37e16, 1e14, 1.0))
+    with pytest.raises(ValueError):
+        unstable.check_courant()
+    with pytest.raises(ValueError):
+        Model2D(1e-6, 1e-6, 25e-9, npml=5, polarization='hz').add_drude_rectangle(0, 0, 1e-7, 1e-7,
+                                                                                    DrudePole(1e15, 1e13, 2.0))

@@ -106,3 +106,51 @@ def test_oracle_reproduces_reference_pec_case(polarization):
             assert values.shape == want.shape
             error = np.max(np.abs(values - want)) / np.max(np.abs(want))
             assert error <= 1e-6, (index, name, error)
+
+
+def test_oracle_reproduces_reference_pole_case():
+    """Drude rectangle (eps_inf background) under a PEC corner and a Lorentz disc on a dielectric: the FP64 oracle
+    reproduces the reference's FP64 run (masks, eps_r, fields at two steps, flux spectrum)."""
+    from tests.cases import POLES2D, build_poles2d
+    golden = np.load(os.path.join(GOLDEN, 'poles2d_ez.npz'))
+    sim, monitor = build_poles2d(Yee2D, phase_dtype=np.complex64)
+    assert (sim.nx, sim.ny) == (int(golden['nx']), int(golden['ny']))
+    np.testing.assert_array_equal(sim.poles[0]['mask'], golden['drude_mask'])
+    np.testing.assert_array_equal(sim.poles[1]['mask'], golden['lorentz_mask'])
+    np.testing.assert_array_equal(sim.pec_regions[0], golden['pec_mask'])
+    np.testing.assert_array_equal(sim.materials['eps_r'].astype(np.float64), golden['eps_r'])
+    done = 0
+    for step in POLES2D['steps']:
+        sim.run(n_steps=step - done)
+        done = step
+        for name, values in sim.fields.items():
+            want = golden[f'{name}_{step}']
+            np.testing.assert_allclose(values, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    assert monitor.sample_interval == int(golden['sample_interval'])
+    np.testing.assert_allclose(monitor.flux(), golden['flux'], rtol=1e-12)
+
+
+def test_pole_oracle_bulk_plasma_oscillation():
+    """A periodic box filled with one Drude pole, uniform initial E_z (k = 0, no curl): E_z rings at the plasma
+    frequency omega_p / sqrt(eps_inf) (1 % for omega dt ~ 0.2) and decays at gamma / 2 (5 %), the closed form of
+    eps_inf eps0 dE/dt = -J, dJ/dt + gamma J = eps0 omega_p^2 E; a dropped drive or a sign error fails both."""
+    dx = 25e-9
+    sim = Yee2D(10 * dx, 8 * dx, dx, npml=1, boundaries='periodic')
+    omega_p, gamma, eps_inf = 3e15, 4e13, 2.0
+    sim.add_drude_rectangle(-1.0, -1.0, 1.0, 1.0, omega_p, gamma, eps_inf)
+    sim.finalize()
+    sim.fields['ez'][...] = 1.0
+    series = []
+    for _ in range(4000):
+        sim.step()
+        series.append(float(sim.fields['ez'][3, 4]))
+    series = np.array(series)
+    assert np.allclose(sim.fields['ez'], sim.fields['ez'][0, 0])   # stays uniform
+    crossings = np.nonzero(np.diff(np.sign(series)) != 0)[0]
+    period = 2.0 * float(sim.dt) * np.mean(np.diff(crossings))
+    omega = 2.0 * np.pi / period
+    assert abs(omega / (omega_p / np.sqrt(eps_inf)) - 1.0) < 1e-2
+    peaks = [np.max(np.abs(series[a:b])) for a, b in zip(crossings[:-2:2], crossings[2::2])]
+    t_span = (crossings[-3] - crossings[0]) * float(sim.dt)
+    decay = -np.log(peaks[-1] / peaks[0]) / t_span
+    assert abs(decay / (gamma / 2.0) - 1.0) < 5e-2
