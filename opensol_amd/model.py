@@ -206,8 +206,15 @@ class _Built:
         self.geometry = {}
         for item in spec.get('geometry') or []:
             self.geometry[item['name']] = (item['kind'], {k: value(v) for k, v in (item.get('params') or {}).items()})
-        for material in spec.get('materials') or []:
-            self._paint(material, value)
+        self._owner = np.full(self.sim.model.shape, -1, dtype=np.int32)   # the material each cell was painted with
+        dispersive = []
+        for index, material in enumerate(spec.get('materials') or []):
+            poles = self._paint(material, value, index)
+            if poles:
+                dispersive.append((index, poles))
+        for index, poles in dispersive:   # definition_common.py:1009-1041: each pole on its material's cells
+            for pole in poles:
+                self.sim.add_dispersive(pole, self._owner == index)
         self.features = []
         for feature in physics.get('features') or []:
             if feature.get('enabled', True):
@@ -227,21 +234,50 @@ class _Built:
             raise ValueError(f"feature {feature['name']!r} needs exactly one selected object")
         return self._shape(selection[0], kinds)
 
-    def _paint(self, material, value):
+    def _paint(self, material, value, index):
+        """Paint one material in list order; returns its dispersive poles (E_z: DrudePole / LorentzPole from the
+        'dispersion' list, definition_common.py:1079-1099), whose background permittivity is the eps_r property,
+        else the first Drude eps_inf, else 1 (:1102-1111)."""
+        from .dispersion import DrudePole, LorentzPole
         properties = material.get('properties') or {}
         unknown = set(properties) - {'eps_r', 'sigma_e'}
-        if unknown or material.get('dispersion'):
-            raise ValueError(f"material {material.get('name')!r}: only eps_r and sigma_e are part of the FDTD path")
-        eps = value(properties.get('eps_r', 1.0))
+        if unknown:
+            raise ValueError(f"material {material.get('name')!r}: only eps_r, sigma_e and dispersion are part of "
+                             f"the FDTD path")
+        dispersion = material.get('dispersion') or []
+        if dispersion and self.kind != 'em_fdtd_ez':
+            raise ValueError(f"material {material.get('name')!r}: dispersive poles are built for em_fdtd_ez only")
+        poles = []
+        for pole in dispersion:
+            model = pole.get('model') if isinstance(pole, dict) else None
+            keys = {'drude': (('omega_p', 'gamma'), ('eps_inf',)),
+                    'lorentz': (('delta_eps', 'omega_0', 'gamma'), ())}.get(model)
+            if keys is None:
+                raise ValueError(f"dispersion pole model must be 'drude' or 'lorentz', got {model!r}")
+            required, optional = keys
+            if set(pole) - {'model', *required, *optional} or any(k not in pole for k in required):
+                raise ValueError(f'{model} pole needs {required} (optional {optional}), got {sorted(pole)}')
+            numbers = {k: value(pole[k]) for k in (*required, *optional) if k in pole}
+            if any(v < 0 for v in numbers.values()) or numbers.get('eps_inf', 1.0) <= 0:
+                raise ValueError(f'{model} pole values must be >= 0 (eps_inf > 0), got {numbers}')
+            poles.append(DrudePole(**numbers) if model == 'drude' else LorentzPole(**numbers))
+        background = next((p.get('eps_inf', 1.0) for p in dispersion if p.get('model') == 'drude'), 1.0)
+        eps = value(properties.get('eps_r', background))
         sigma = value(properties.get('sigma_e', 0.0))
         for name in material.get('selection') or []:
             kind, p = self._shape(name, ('box',) if self.three_d else ('rectangle', 'circle'))
             if kind == 'rectangle':
                 self.sim.add_dielectric_rectangle(p['x0'], p['y0'], p['x1'], p['y1'], eps_r=eps, sigma_e=sigma)
+                ranges = self.sim.domain.box_ranges((p['x0'], p['y0']), (p['x1'], p['y1']))
+                self._owner[tuple(slice(lo, hi) for lo, hi in ranges)] = index
             elif kind == 'circle':
                 self.sim.model.add_dielectric_circle(p['cx'], p['cy'], p['r'], eps_r=eps, sigma_e=sigma)
+                x, y = (c.astype(np.float32) for c in self.sim.domain.cell_centers())
+                self._owner[(x[:, None] - np.float32(p['cx'])) ** 2 + (y[None, :] - np.float32(p['cy'])) ** 2
+                            <= np.float32(p['r']) ** 2] = index
             else:
                 self.sim.add_dielectric_box((p['x0'], p['y0'], p['z0']), (p['x1'], p['y1'], p['z1']), eps, sigma)
+        return poles
 
     def _apply(self, feature, value):
         """definition_common.py:1167-1287 (2D) and physics/em3d/definition.py:472-520 (3D)."""

@@ -1,7 +1,7 @@
-"""Write tests/fixtures/model_thin_film_{ez,hz}.npz and model_dipole_3d.npz (beside the GPU tests that read them): what the reference's headless model API (open_sol.api.run, CPU
+"""Write tests/fixtures/model_thin_film_{ez,hz,drude}.npz and model_dipole_3d.npz (beside the GPU tests that read them): what the reference's headless model API (open_sol.api.run, CPU
 backend) returns and saves for its thin-film example model (examples/em/thin_film.json: a line source, two flux
-monitors of 50 wavelengths, a film swept over n_film = 1 then 1.5), once as written (E_z) and once with the physics
-type switched to em_fdtd_hz, and for a small 3D model written here (MODEL_3D). They pin the model-JSON caller end to end: parameter evaluation, geometry and material
+monitors of 50 wavelengths, a film swept over n_film = 1 then 1.5), once as written (E_z), once with the physics
+type switched to em_fdtd_hz and once with the film a Drude medium (E_z), and for a small 3D model written here (MODEL_3D). They pin the model-JSON caller end to end: parameter evaluation, geometry and material
 selection, feature placement, the study duration and sweep, and the results.npz layout.
 
 Saved per case: the model JSON text (input), each swept run's flux wavelengths, power and power_per_cw per monitor
@@ -113,8 +113,15 @@ def run_3d():
 
 
 def model_text(polarization):
+    """The example as written (E_z), its H_z variant, or 'drude': E_z with the film a Drude medium (eps_inf 2.25,
+    omega_p 2e15 rad/s, gamma 1e14 1/s) in place of its eps_r."""
     with open(EXAMPLE, encoding='utf-8') as handle:
         spec = json.load(handle)
+    if polarization == 'drude':
+        film = spec['materials'][0]
+        film['properties'] = {}
+        film['dispersion'] = [{'model': 'drude', 'omega_p': '2e15', 'gamma': '1e14', 'eps_inf': 2.25}]
+        return json.dumps(spec)
     spec['physics'][0]['type'] = f'em_fdtd_{polarization}'
     return json.dumps(spec)
 
@@ -155,7 +162,7 @@ def main():
     path = os.path.join(FIXTURES, 'model_dipole_3d.npz')
     np.savez_compressed(path, **out)
     print(path, 'ticks', int(out['summary_ticks']), 'flux_x', out['flux_x/power'])
-    for polarization in ('ez', 'hz'):
+    for polarization in ('ez', 'hz', 'drude'):
         out = run_case(polarization)
         path = os.path.join(FIXTURES, f'model_thin_film_{polarization}.npz')
         np.savez_compressed(path, **out)

@@ -14,8 +14,9 @@ FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'fixtures')
 MONITORS = ('reflection_side_flux', 'transmitted_flux')
 
 
-@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+@pytest.mark.parametrize('polarization', ['ez', 'hz', 'drude'])
 def test_thin_film_model_matches_the_reference_run(gpu, polarization, tmp_path):
+    """'drude': the film is a Drude medium given as a material 'dispersion' (a pole in the fused E_z update)."""
     from opensol_amd import model as M
     golden = np.load(os.path.join(FIXTURES, f'model_thin_film_{polarization}.npz'))
     spec = json.loads(str(golden['model_json']))

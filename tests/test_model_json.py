@@ -149,3 +149,26 @@ def test_3d_model_builds_the_scripted_case():
     duration = M.evaluate(spec['study']['settings']['duration'], names)
     assert steps_for_duration(0.0, model.dt, duration) == int(golden['summary_ticks'])
     np.testing.assert_allclose(M.C0 / model.monitors[0].frequencies, golden['flux_x/wavelength'], rtol=1e-15)
+
+
+def test_dispersive_material_becomes_a_pole_on_its_cells():
+    """A material 'dispersion' list (definition_common.py:1009-1111): one DrudePole on the film's cells, the film's
+    permittivity its eps_inf; refused for H_z and with unknown pole keys."""
+    from opensol_amd import DrudePole
+    golden, spec = golden_spec('drude')
+    names = M.evaluate_parameters(spec['parameters'])
+    model = M._Built(spec, names).sim.model
+    (pole, mask), = model.poles
+    assert isinstance(pole, DrudePole) and float(pole.omega_p) == np.float32(2e15) and pole.eps_inf == np.float32(2.25)
+    film = np.zeros(model.shape, dtype=bool)
+    film[tuple(slice(lo, hi) for lo, hi in model.domain.box_ranges((2.75e-6, 0.0), (3.25e-6, 3e-6)))] = True
+    np.testing.assert_array_equal(mask, film)
+    np.testing.assert_array_equal(model.materials.property_array(0)[film], np.float32(2.25))
+    other = json.loads(json.dumps(spec))
+    other['physics'][0]['type'] = 'em_fdtd_hz'
+    with pytest.raises(ValueError):
+        M._Built(other, names)
+    other = json.loads(json.dumps(spec))
+    other['materials'][0]['dispersion'][0]['tau'] = 1.0
+    with pytest.raises(ValueError):
+        M._Built(other, names)
