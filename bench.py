@@ -329,6 +329,48 @@ def run_2d(args, torch):
     return result
 
 
+def run_features(args, torch):
+    """The 8192^2 E_z workload with the monitors and media of the wider path on top: a running-DFT box over the whole
+    window (3 wavelengths), a Drude rectangle over the middle third of the rows, 64 field probes. Reports the step time
+    with and without them; rocprofv3 gives each feature kernel's own time (profiles/)."""
+    from opensol_amd import DrudePole
+    plain = build_ez_workload(graph_steps=args.graph_steps)
+    plain.finalize()
+    plain.engine.step(args.warmup)
+    plain_s = _time_steps(plain.engine, args.steps, torch) / args.steps
+    plain.close()
+    del plain
+    sim = build_ez_workload(graph_steps=args.graph_steps)
+    size = sim.model.domain.sizes[0]
+    box = sim.add_dft_monitor(0.0, 0.0, size, size, wavelengths=[1.4e-6, 1.55e-6, 1.7e-6])
+    sim.add_drude_rectangle(size / 3, 0.0, 2 * size / 3, size, DrudePole(omega_p=1.5e15, gamma=1e14, eps_inf=4.0))
+    rng = np.random.default_rng(SEED)
+    for x, y in rng.uniform(0.0, size, (64, 2)):
+        sim.add_field_probe(x, y)
+    sim.finalize()
+    engine = sim.engine
+    cells = sim.model.nx * sim.model.ny
+    engine.step(args.warmup)
+    elapsed_s = _time_steps(engine, args.steps, torch)
+    per_step = elapsed_s / args.steps
+    m = sim.model.sample_interval
+    dft_cells = box.mx * box.my
+    dft_bytes = dft_cells * (4 + 32 * box.n_freqs)   # E_z read, (re, im) FP64 read + write per frequency
+    print(f'# features: {per_step * 1e3:.4f} ms/step against {plain_s * 1e3:.4f} plain; DFT box {box.mx}x{box.my} x '
+          f'{box.n_freqs} every {m} steps ({dft_bytes / 1e9:.2f} GB per sample)', file=sys.stderr, flush=True)
+    value = cells * args.steps / elapsed_s
+    return {'metric': 'Yee cell-updates/s (HBM BW fraction in roofline)', 'value': value, 'unit': 'cell-updates/s',
+            **_common_fields(args, value, per_step),
+            'data': 'synthetic (seeded rectangles, Gaussian line source, Drude rectangle, DFT box, probes)',
+            'config': {'workload': 'yee2d_ez_8192x8192_cpml20_400rect_dftbox3_drude_probes64', 'grid': [sim.model.nx,
+                       sim.model.ny], 'graph_steps': args.graph_steps, 'sample_interval': m},
+            'features': {'plain_ms_per_step': plain_s * 1e3, 'ms_per_step': per_step * 1e3,
+                         'overhead': per_step / plain_s - 1.0, 'dft_cells': dft_cells, 'dft_frequencies': box.n_freqs,
+                         'dft_bytes_per_sample': dft_bytes, 'drude_rows': int(sim.model.poles[0][1].any(axis=1).sum()),
+                         'probes': 64},
+            'roofline': None, 'cpu_baseline': None}
+
+
 def run_3d(args, torch):
     sim = build_3d_workload(graph_steps=args.graph_steps, planes_per_tile=args.planes_per_tile)
     sim.finalize()
@@ -421,7 +463,8 @@ def run_batch(args, torch):
 def main():
     parser = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     parser.add_argument('--gpus', type=int, default=1)
-    parser.add_argument('--workload', default='ez8192', choices=('ez8192', 'yee3d_512', 'batch256', 'serve'))
+    parser.add_argument('--workload', default='ez8192',
+                        choices=('ez8192', 'yee3d_512', 'batch256', 'serve', 'features8192'))
     parser.add_argument('--jobs', type=int, default=200, help='serve: jobs per arrival rate')
     parser.add_argument('--steps', type=int, default=None, help='timed steps (default 2000 2D, 500 3D)')
     parser.add_argument('--warmup', type=int, default=None, help='untimed steps (default 50 2D, 20 3D)')
@@ -453,6 +496,8 @@ def main():
         result = run_batch(args, torch)
     elif args.workload == 'serve':
         result = run_serve(args, torch)
+    elif args.workload == 'features8192':
+        result = run_features(args, torch)
     else:
         result = run_3d(args, torch) if three else run_2d(args, torch)
     print(json.dumps(result))

@@ -267,6 +267,40 @@ __device__ __forceinline__ RowCpml row_cpml(const yee_cpml_axis& c, int i, bool 
     return out;
 }
 
+// Dispersive poles of row i (E_z): each pole whose box holds the row updates its state from E^n (e_n) and corrects
+// the updated E (ez) of its cells, before the sources (physics/em/solvers/fdtd_ez_2d.py:262-358). The loop is
+// uniform; rows outside every pole box skip it.
+template <int VEC>
+__device__ __forceinline__ void apply_poles(const yee2d_desc& d, int i, int jb, const float e_n[VEC],
+                                            const float4 c[VEC], float ez[VEC])
+{
+    for (int p = 0; p < d.n_poles; ++p) {
+        const yee_pole& pl = d.pole[p];
+        if (i < pl.i0 || i >= pl.i1) continue;
+        const int64_t base = (int64_t)(i - pl.i0) * (pl.j1 - pl.j0) - pl.j0;
+#pragma unroll
+        for (int k = 0; k < VEC; ++k) {
+            const int j = jb + k;
+            if (j < pl.j0 || j >= pl.j1) continue;
+            const int64_t q = base + j;
+            const float drive = pl.drive[q];
+            if (pl.kind == YEE_DRUDE) {
+                float jz = pl.state[0][q];
+                jz += pl.damping * jz + drive * e_n[k];
+                pl.state[0][q] = jz;
+                ez[k] -= c[k].y * jz;
+            } else {
+                const float pz = pl.state[0][q];
+                float dpz = pl.state[1][q];
+                dpz += pl.damping * dpz - pl.restoring * pz + drive * e_n[k];
+                pl.state[0][q] = pz + dpz;
+                pl.state[1][q] = dpz;
+                ez[k] -= c[k].y * pl.inv_dt * dpz;
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ int wave_index()
 {
     return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -407,31 +441,7 @@ struct EzTile {
             ez_v[k] = j < ny ? v : 0.f;
         }
         // dispersive poles from E^n, then their corrections of the updated E (before the sources)
-        for (int p = 0; p < d.n_poles; ++p) {
-            const yee_pole& pl = d.pole[p];
-            if (i < pl.i0 || i >= pl.i1) continue;
-            const int64_t base = (int64_t)(i - pl.i0) * (pl.j1 - pl.j0) - pl.j0;
-#pragma unroll
-            for (int k = 0; k < VEC; ++k) {
-                const int j = col.jb + k;
-                if (j < pl.j0 || j >= pl.j1) continue;
-                const int64_t q = base + j;
-                const float drive = pl.drive[q];
-                if (pl.kind == YEE_DRUDE) {
-                    float jz = pl.state[0][q];
-                    jz += pl.damping * jz + drive * e_cur[k];
-                    pl.state[0][q] = jz;
-                    ez_v[k] -= c[k].y * jz;
-                } else {
-                    const float pz = pl.state[0][q];
-                    float dpz = pl.state[1][q];
-                    dpz += pl.damping * dpz - pl.restoring * pz + drive * e_cur[k];
-                    pl.state[0][q] = pz + dpz;
-                    pl.state[1][q] = dpz;
-                    ez_v[k] -= c[k].y * pl.inv_dt * dpz;
-                }
-            }
-        }
+        apply_poles<VEC>(d, i, col.jb, e_cur, c, ez_v);
         // sheet sources at t + dt/2: E_z += float32(value * profile)
         for (int s = 0; s < d.n_sources; ++s) {
             const yee_line_source& src = d.source[s];
@@ -853,7 +863,7 @@ struct EzBulkIn {
     uint32_t id_l;
 };
 
-template <int VEC, bool LDS>
+template <int VEC, bool LDS, bool POLES>
 __device__ __forceinline__ void ez_bulk_tile(const yee2d_desc& d, int tj, int ti, int64_t n)
 {
     const int lane = threadIdx.x & (YEE_WAVE - 1);
@@ -955,6 +965,7 @@ __device__ __forceinline__ void ez_bulk_tile(const yee2d_desc& d, int tj, int ti
             const float dhx = hx_v[k] - (k == 0 ? hx_left : hx_v[k - 1]);
             ez_v[k] = jb + k < ny ? ez_centre(c[k], e_cur[k], hy_v[k] - hy_prev[k], dhx, idx, idy) : 0.f;
         }
+        if constexpr (POLES) apply_poles<VEC>(d, i, jb, e_cur, c, ez_v);
         if (any) {
             stv<VEC>(e_n + row + jb, ez_v);
             stv<VEC>(hx_n + row + jb, hx_v);
@@ -1177,7 +1188,7 @@ __device__ __forceinline__ bool bulk_tile(const yee2d_tile_sets& s, int item, in
 // few, slow tiles would otherwise finish long after the bulk).
 constexpr int EDGE_SPLIT = 8;
 
-template <int POL, int VEC, bool EDGE, bool LDS>
+template <int POL, int VEC, bool EDGE, bool LDS, bool POLES = false>
 __device__ __forceinline__ void tile_2d(const yee2d_desc& d, const TileBox& t, int64_t n, int part = 0,
                                         int parts = 1)
 {
@@ -1196,7 +1207,7 @@ __device__ __forceinline__ void tile_2d(const yee2d_desc& d, const TileBox& t, i
         }
     } else {
         if (ez)
-            ez_bulk_tile<VEC, LDS>(d, t.tj, t.ti, n);
+            ez_bulk_tile<VEC, LDS, POLES>(d, t.tj, t.ti, n);
         else
             hz_bulk_tile<VEC, LDS>(d, t.tj, t.ti, n);
     }
@@ -1207,7 +1218,7 @@ __device__ __forceinline__ void tile_2d(const yee2d_desc& d, const TileBox& t, i
 namespace yee {
 
 // One block of the bulk (EDGE = false) or edge (EDGE = true) tile set; `block` counts blocks within that set.
-template <int POL, int VEC, bool EDGE>
+template <int POL, int VEC, bool EDGE, bool POLES>
 __device__ __forceinline__ void step_block(const yee2d_desc& d, const yee2d_tile_sets& sets, int64_t n, int block)
 {
     const int parts = EDGE ? EDGE_SPLIT : 1;
@@ -1220,9 +1231,9 @@ __device__ __forceinline__ void step_block(const yee2d_desc& d, const yee2d_tile
     if (!mine) return;
     const TileBox t = tile_box<VEC>(d, ti * sets.tiles_j + tj);
     if (lds)
-        tile_2d<POL, VEC, EDGE, true>(d, t, n, slot % parts, parts);
+        tile_2d<POL, VEC, EDGE, true, POLES>(d, t, n, slot % parts, parts);
     else
-        tile_2d<POL, VEC, EDGE, false>(d, t, n, slot % parts, parts);
+        tile_2d<POL, VEC, EDGE, false, POLES>(d, t, n, slot % parts, parts);
 }
 
 }  // namespace yee
@@ -1237,7 +1248,7 @@ __device__ __forceinline__ void step_block(const yee2d_desc& d, const yee2d_tile
 // constant address space. Bound to the argument itself, the general H_z path took its address in a way that made the
 // compiler copy all 736 bytes to scratch and turn every field access into a flat load (168 VGPRs + 744 B/lane of
 // scratch); read in place, the fields are scalar loads (127 VGPRs, no scratch).
-template <int POL, int VEC, bool EDGE>
+template <int POL, int VEC, bool EDGE, bool POLES>
 __global__ void __launch_bounds__(256) yee2d_step_kernel(yee2d_desc desc, yee2d_tile_sets sets, int32_t offset)
 {
     typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
@@ -1245,16 +1256,20 @@ __global__ void __launch_bounds__(256) yee2d_step_kernel(yee2d_desc desc, yee2d_
     if (d.clock != desc.clock) return;   // the argument layout is not the one assumed: do nothing
     const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
-    yee::step_block<POL, VEC, EDGE>(d, sets, n, blockIdx.x);
+    yee::step_block<POL, VEC, EDGE, POLES>(d, sets, n, blockIdx.x);
 }
 
-#define YEE_STEP_KERNEL(POL, VEC, EDGE) \
-    template __global__ void yee2d_step_kernel<POL, VEC, EDGE>(yee2d_desc, yee2d_tile_sets, int32_t);
-#define YEE_STEP_KERNELS(POL, VEC) YEE_STEP_KERNEL(POL, VEC, false) YEE_STEP_KERNEL(POL, VEC, true)
+// POLES: the E_z bulk loop with the dispersive-pole update (a separate instantiation, so the loop without poles keeps
+// its own register allocation); the general loop always carries it.
+#define YEE_STEP_KERNEL(POL, VEC, EDGE, POLES) \
+    template __global__ void yee2d_step_kernel<POL, VEC, EDGE, POLES>(yee2d_desc, yee2d_tile_sets, int32_t);
+#define YEE_STEP_KERNELS(POL, VEC) YEE_STEP_KERNEL(POL, VEC, false, false) YEE_STEP_KERNEL(POL, VEC, true, false)
 YEE_STEP_KERNELS(YEE_EZ, 2)
 YEE_STEP_KERNELS(YEE_EZ, 4)
 YEE_STEP_KERNELS(YEE_HZ, 2)
 YEE_STEP_KERNELS(YEE_HZ, 4)
+YEE_STEP_KERNEL(YEE_EZ, 2, false, true)
+YEE_STEP_KERNEL(YEE_EZ, 4, false, true)
 
 // Many instances (E_z and H_z mixed) in one launch: tile t belongs to the instance whose tile range
 // [prefix[k], prefix[k+1]) holds it. Tables are read from global memory (they stay in L1/L2).

@@ -13,7 +13,7 @@
 #include "yee.h"
 #include "yee_launch.h"
 
-template <int POL, int VEC, bool EDGE>
+template <int POL, int VEC, bool EDGE, bool POLES>
 __global__ void yee2d_step_kernel(yee2d_desc d, yee2d_tile_sets sets, int32_t offset);
 template <int VEC>
 __global__ void yee2d_batch_kernel(const yee2d_desc* __restrict__ descs, const int32_t* __restrict__ prefix,
@@ -131,13 +131,15 @@ int32_t check_dft(const yee_dft_region* regions, int32_t n_dft, const int32_t* n
     return YEE_OK;
 }
 
-// Blocks of 256 lanes along x of a DFT launch: enough for the largest region (blockIdx.y selects the region).
+// Blocks of 256 lanes along x of a DFT launch: enough for the largest region (blockIdx.y selects the region), at most
+// kDftBlocks (the kernel strides over larger regions).
+constexpr int64_t kDftBlocks = 2048;
 int dft_blocks(const yee_dft_region* regions, int32_t n_dft)
 {
     int64_t cells = 0;
     for (int32_t k = 0; k < n_dft; ++k)
         cells = std::max(cells, (int64_t)regions[k].count[0] * regions[k].count[1] * regions[k].count[2]);
-    return (int)((cells + 255) / 256);
+    return (int)std::min(kDftBlocks, (cells + 255) / 256);
 }
 
 int32_t check_2d(const yee2d_desc* d)
@@ -277,11 +279,16 @@ using StepKernel = void (*)(yee2d_desc, yee2d_tile_sets, int32_t);
 StepKernel step_kernel(const yee2d_desc& d, bool edge)
 {
     if (d.polarization == YEE_EZ) {
-        if (d.lane_columns == 2) return edge ? yee2d_step_kernel<YEE_EZ, 2, true> : yee2d_step_kernel<YEE_EZ, 2, false>;
-        return edge ? yee2d_step_kernel<YEE_EZ, 4, true> : yee2d_step_kernel<YEE_EZ, 4, false>;
+        const bool poles = d.n_poles > 0;
+        if (d.lane_columns == 2)
+            return edge ? yee2d_step_kernel<YEE_EZ, 2, true, false>
+                        : (poles ? yee2d_step_kernel<YEE_EZ, 2, false, true> : yee2d_step_kernel<YEE_EZ, 2, false, false>);
+        return edge ? yee2d_step_kernel<YEE_EZ, 4, true, false>
+                    : (poles ? yee2d_step_kernel<YEE_EZ, 4, false, true> : yee2d_step_kernel<YEE_EZ, 4, false, false>);
     }
-    if (d.lane_columns == 2) return edge ? yee2d_step_kernel<YEE_HZ, 2, true> : yee2d_step_kernel<YEE_HZ, 2, false>;
-    return edge ? yee2d_step_kernel<YEE_HZ, 4, true> : yee2d_step_kernel<YEE_HZ, 4, false>;
+    if (d.lane_columns == 2)
+        return edge ? yee2d_step_kernel<YEE_HZ, 2, true, false> : yee2d_step_kernel<YEE_HZ, 2, false, false>;
+    return edge ? yee2d_step_kernel<YEE_HZ, 4, true, false> : yee2d_step_kernel<YEE_HZ, 4, false, false>;
 }
 
 // Wavefront slots of the current device for a 256-thread kernel: resident blocks per CU x CUs x 4 waves.
@@ -313,8 +320,8 @@ void apply_defaults(yee2d_desc& d)
 }
 
 // Edge tile rows and columns of a descriptor: tile rows whose rows (with the row before and the row after) reach an x
-// CPML layer or hold a source / monitor line along y, or that hold rows of a dispersive pole's box; tile columns
-// likewise for y (sources and monitors). Too many lines: every tile is an edge tile.
+// CPML layer or hold a source / monitor line along y, tile columns likewise for y. Too many lines: every tile is
+// an edge tile. (Dispersive poles run in both loops.)
 yee2d_tile_sets tile_sets(const yee2d_desc& d)
 {
     yee2d_tile_sets s{};
@@ -330,8 +337,6 @@ yee2d_tile_sets tile_sets(const yee2d_desc& d)
             edge = edge || (d.source[k].axis == 1 && d.source[k].line >= a && d.source[k].line <= b);
         for (int k = 0; k < d.n_monitors; ++k)
             edge = edge || (d.monitor[k].normal == 0 && d.monitor[k].position >= a && d.monitor[k].position <= b);
-        for (int k = 0; k < d.n_poles; ++k)   // the lean bulk loop has no pole update
-            edge = edge || (d.pole[k].i0 < i1 && d.pole[k].i1 > i0);
         if (edge) rows.push_back(ti);
     }
     for (int tj = 0; tj < s.tiles_j; ++tj) {

@@ -1,5 +1,7 @@
 // Running-DFT box monitors for gfx950: one launch per step (skipped at once except on sample steps) accumulates the
-// FP64 spectra of every DFT region of an instance, one lane per sampled cell and blockIdx.y per region.
+// FP64 spectra of every DFT region of an instance, blockIdx.y per region and a grid-stride loop over its sampled
+// cells. The grid is capped at one full occupancy of the device (2048 blocks of 256 lanes), so a step that takes no
+// sample costs a few microseconds however large the box is.
 //
 // On sample step n ((n + 1) % m == 0, sample s = (n + 1) / m - 1) each lane reads its components from the new state
 // set (one float each, or two for a face pair averaged to the cell centre) and adds v * phase[s][f] to its (re, im)
@@ -77,7 +79,9 @@ __global__ void __launch_bounds__(256) yee2d_dft_kernel(yee2d_desc desc, int32_t
     const int set = (int)((n & 1) ^ 1);
     const int32_t n_axis[3] = {d.nx, d.ny, 1};
     const int64_t axis_stride[3] = {d.pitch, 1, 0};
-    dft_cell(r, d.field[set], n_axis, axis_stride, s, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    const int64_t cells = (int64_t)r.count[0] * r.count[1] * r.count[2];
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cells; t += (int64_t)gridDim.x * blockDim.x)
+        dft_cell(r, d.field[set], n_axis, axis_stride, s, t);
 }
 
 // DFT regions of a 3D instance after step n = *clock + offset (cell index (i * ny + j) * nz + k).
@@ -94,5 +98,7 @@ __global__ void __launch_bounds__(256) yee3d_dft_kernel(yee3d_desc desc, int32_t
     const int set = (int)((n & 1) ^ 1);
     const int32_t n_axis[3] = {d.nx, d.ny, d.nz};
     const int64_t axis_stride[3] = {(int64_t)d.ny * d.nz, d.nz, 1};
-    dft_cell(r, d.field[set], n_axis, axis_stride, s, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    const int64_t cells = (int64_t)r.count[0] * r.count[1] * r.count[2];
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cells; t += (int64_t)gridDim.x * blockDim.x)
+        dft_cell(r, d.field[set], n_axis, axis_stride, s, t);
 }
