@@ -26,11 +26,11 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 7
+#define YEE_ABI_VERSION 8
 #define YEE_MAX_SOURCES 4
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
-#define YEE_MAX_POLES 4
+#define YEE_MAX_POLES 8
 #define YEE_DRUDE 1
 #define YEE_LORENTZ 2
 
@@ -143,20 +143,24 @@ typedef struct yee_dft_region {
 } yee_dft_region;
 
 /*
- * Dispersive pole of a 2D E_z instance on the cell box [i0, i1) x [j0, j1), in the fused E update of the step kernel
- * (auxiliary differential equations in velocity form, from E^n before the update; corrections after the Yee E update
- * with its CPML, before the sources):
+ * Dispersive pole of a 2D instance on one E component over the index box [i0, i1) x [j0, j1), in the fused E update
+ * of the step kernel (auxiliary differential equations in velocity form, from E^n before the update; corrections after
+ * the Yee E update with its CPML, before the sources):
  *   Drude (YEE_DRUDE):     J += damping J + drive E^n;                     E -= cb J
  *   Lorentz (YEE_LORENTZ): dP += damping dP - restoring P + drive E^n; P += dP;  E -= cb inv_dt dP
- * cb is the cell's E coefficient (0 in PEC cells); drive holds the pole's drive coefficient times its mask (0 outside
- * the pole's cells, so their state stays 0). The state is read and written in place by the owner of its cell.
- * Replaces physics/em/solvers/fdtd_ez_2d.py:123-163, 318-358 and the kernels of fdtd_ez_2d_kernels.py:176-264;
- * coefficients physics/em/materials.py:132-190.
+ * component 0: E_z at the cell centres (E_z instances); 1: E_x on the hi-y faces, 2: E_y on the hi-x faces (H_z
+ * instances: one record per face set of a pole). cb is the cell's or face's E coefficient (0 on PEC); drive holds the
+ * pole's drive coefficient times its cell mask, or the mask's face average (0 outside, so the state stays 0) and the
+ * PEC keep. The state is read and written in place by the owner of its cell or face.
+ * Replaces physics/em/solvers/fdtd_ez_2d.py:123-163, 318-358, fdtd_hz_2d.py:286-323, 441-473, 539-561 and the kernels
+ * of fdtd_ez_2d_kernels.py:176-264; coefficients physics/em/materials.py:132-190.
  */
 typedef struct yee_pole {
     int32_t kind;            /* YEE_DRUDE or YEE_LORENTZ */
+    int32_t component;       /* 0 E_z, 1 E_x faces, 2 E_y faces */
     int32_t i0, i1, j0, j1;
     float damping, restoring, inv_dt;
+    int32_t pad_;
     const float* drive;      /* (i1 - i0) * (j1 - j0), row-major */
     float* state[2];         /* Drude: J (state[1] unused); Lorentz: P, dP; (i1 - i0) * (j1 - j0) each */
 } yee_pole;
@@ -196,7 +200,7 @@ typedef struct yee2d_desc {
     int32_t n_dft;
     int32_t pad3_;
     yee_dft_region dft[YEE_MAX_DFT];
-    int32_t n_poles;         /* E_z only */
+    int32_t n_poles;
     int32_t pad4_;
     yee_pole pole[YEE_MAX_POLES];
     yee_probes probes;

@@ -6,11 +6,11 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 7
+ABI_VERSION = 8
 MAX_SOURCES = 4
 MAX_MONITORS = 4
 MAX_DFT = 4
-MAX_POLES = 4
+MAX_POLES = 8
 YEE_DRUDE, YEE_LORENTZ = 1, 2
 MAX_SOURCES_3D = 4
 MAX_PLANES_3D = 8
@@ -48,8 +48,9 @@ class DftRegionC(ctypes.Structure):
 
 
 class PoleC(ctypes.Structure):
-    _fields_ = [('kind', c_i32), ('i0', c_i32), ('i1', c_i32), ('j0', c_i32), ('j1', c_i32), ('damping', c_f32),
-                ('restoring', c_f32), ('inv_dt', c_f32), ('drive', c_ptr), ('state', c_ptr * 2)]
+    _fields_ = [('kind', c_i32), ('component', c_i32), ('i0', c_i32), ('i1', c_i32), ('j0', c_i32), ('j1', c_i32),
+                ('damping', c_f32), ('restoring', c_f32), ('inv_dt', c_f32), ('pad_', c_i32), ('drive', c_ptr),
+                ('state', c_ptr * 2)]
 
 
 class Desc2DC(ctypes.Structure):

@@ -267,16 +267,15 @@ __device__ __forceinline__ RowCpml row_cpml(const yee_cpml_axis& c, int i, bool 
     return out;
 }
 
-// Dispersive poles of row i (E_z): each pole whose box holds the row updates its state from E^n (e_n) and corrects
-// the updated E (ez) of its cells, before the sources (physics/em/solvers/fdtd_ez_2d.py:262-358). The loop is
-// uniform; rows outside every pole box skip it.
+// Pole records of one E component (0 E_z cells, 1 E_x faces, 2 E_y faces) on row i: each record whose box holds the
+// row updates its state from E^n (e_n) and corrects the updated E (e) with the cell's or face's coefficient cb.
 template <int VEC>
-__device__ __forceinline__ void apply_poles(const yee2d_desc& d, int i, int jb, const float e_n[VEC],
-                                            const float4 c[VEC], float ez[VEC])
+__device__ __forceinline__ void apply_pole_records(const yee2d_desc& d, int component, int i, int jb,
+                                                   const float e_n[VEC], const float cb[VEC], float e[VEC])
 {
     for (int p = 0; p < d.n_poles; ++p) {
         const yee_pole& pl = d.pole[p];
-        if (i < pl.i0 || i >= pl.i1) continue;
+        if (pl.component != component || i < pl.i0 || i >= pl.i1) continue;
         const int64_t base = (int64_t)(i - pl.i0) * (pl.j1 - pl.j0) - pl.j0;
 #pragma unroll
         for (int k = 0; k < VEC; ++k) {
@@ -285,20 +284,32 @@ __device__ __forceinline__ void apply_poles(const yee2d_desc& d, int i, int jb, 
             const int64_t q = base + j;
             const float drive = pl.drive[q];
             if (pl.kind == YEE_DRUDE) {
-                float jz = pl.state[0][q];
-                jz += pl.damping * jz + drive * e_n[k];
-                pl.state[0][q] = jz;
-                ez[k] -= c[k].y * jz;
+                float jc = pl.state[0][q];
+                jc += pl.damping * jc + drive * e_n[k];
+                pl.state[0][q] = jc;
+                e[k] -= cb[k] * jc;
             } else {
-                const float pz = pl.state[0][q];
-                float dpz = pl.state[1][q];
-                dpz += pl.damping * dpz - pl.restoring * pz + drive * e_n[k];
-                pl.state[0][q] = pz + dpz;
-                pl.state[1][q] = dpz;
-                ez[k] -= c[k].y * pl.inv_dt * dpz;
+                const float pc = pl.state[0][q];
+                float dpc = pl.state[1][q];
+                dpc += pl.damping * dpc - pl.restoring * pc + drive * e_n[k];
+                pl.state[0][q] = pc + dpc;
+                pl.state[1][q] = dpc;
+                e[k] -= cb[k] * pl.inv_dt * dpc;
             }
         }
     }
+}
+
+// Dispersive poles of row i (E_z cells), before the sources (physics/em/solvers/fdtd_ez_2d.py:262-358). The loop is
+// uniform; rows outside every pole box skip it.
+template <int VEC>
+__device__ __forceinline__ void apply_poles(const yee2d_desc& d, int i, int jb, const float e_n[VEC],
+                                            const float4 c[VEC], float ez[VEC])
+{
+    float cb[VEC];
+#pragma unroll
+    for (int k = 0; k < VEC; ++k) cb[k] = c[k].y;
+    apply_pole_records<VEC>(d, 0, i, jb, e_n, cb, ez);
 }
 
 __device__ __forceinline__ int wave_index()
@@ -704,7 +715,7 @@ struct HzTile {
 
         const float hz_sh = shfl_from_next(hz_cur[0]);
         const int id_sh = __shfl_down(id_at(cur.ids, 0), 1, YEE_WAVE);
-        float ex_v[VEC], ey_v[VEC];
+        float ex_v[VEC], ey_v[VEC], cb_x[VEC], cb_y[VEC];
         const float idx_h = d.inv_dx_h[i];
 #pragma unroll
         for (int q = 0; q < VEC; ++q) {
@@ -712,6 +723,7 @@ struct HzTile {
             const int id_q = id_at(cur.ids, q);
             // E_x on the hi-y face: H_z of the next column (+ y CPML 'h' rows, sign +)
             ex_v[q] = j < ny ? cur.ex[q] : 0.f;
+            cb_x[q] = cb_y[q] = 0.f;
             if (j < ny && (j < ny - 1 || py)) {
                 float hz_right;
                 int id_right;
@@ -720,6 +732,7 @@ struct HzTile {
                 else { hz_right = hz_cur_r; id_right = (int)cur.id_r; }   // tile edge or periodic wrap
                 float ca, cb;
                 face_coefficients<LDS>(d, id_q, id_right, 0, &ca, &cb);
+                cb_x[q] = cb;
                 const float dh = hz_right - hz_cur[q];
                 ex_v[q] = hz_ex(ca, cb, cur.ex[q], dh, col.idy_h[q]);
                 if (col.ry_h[q] >= 0) {
@@ -732,6 +745,7 @@ struct HzTile {
             if (ey_updated && j < ny) {
                 float ca, cb;
                 face_coefficients<LDS>(d, id_q, id_at(nxt.ids, q), 1, &ca, &cb);
+                cb_y[q] = cb;
                 const float dh = hz_next[q] - hz_cur[q];
                 ey_v[q] = hz_ey(ca, cb, cur.ey[q], dh, idx_h);
                 if (xh.r >= 0) {
@@ -739,6 +753,11 @@ struct HzTile {
                     ey_v[q] -= cb * psi_xh[q];
                 }
             }
+        }
+        // dispersive poles of the E faces from E^n, then their corrections (before the sources)
+        if (d.n_poles > 0) {
+            apply_pole_records<VEC>(d, 1, i, col.jb, cur.ex, cb_x, ex_v);
+            apply_pole_records<VEC>(d, 2, i, col.jb, cur.ey, cb_y, ey_v);
         }
         // sheet current on the hi faces at t + dt/2 (E_y for a line along y, E_x for a line along x)
         for (int s = 0; s < d.n_sources; ++s) {

@@ -150,10 +150,11 @@ int32_t check_2d(const yee2d_desc* d)
     const int32_t extents[3] = {d->nx, d->ny, 1};
     if (check_dft(d->dft, d->n_dft, extents, 2, 3, d->sample_interval) != YEE_OK) return YEE_EINVAL;
     if (d->n_poles < 0 || d->n_poles > YEE_MAX_POLES) return fail(YEE_EINVAL, "n_poles out of range");
-    if (d->n_poles > 0 && d->polarization != YEE_EZ) return fail(YEE_EINVAL, "dispersive poles are E_z only");
     for (int p = 0; p < d->n_poles; ++p) {
         const yee_pole& pl = d->pole[p];
         if (pl.kind != YEE_DRUDE && pl.kind != YEE_LORENTZ) return fail(YEE_EINVAL, "pole kind must be 1 or 2");
+        if (d->polarization == YEE_EZ ? pl.component != 0 : (pl.component != 1 && pl.component != 2))
+            return fail(YEE_EINVAL, "pole component must be 0 (E_z) or 1, 2 (E_x, E_y faces of H_z)");
         if (pl.i0 < 0 || pl.i1 > d->nx || pl.i0 >= pl.i1 || pl.j0 < 0 || pl.j1 > d->ny || pl.j0 >= pl.j1)
             return fail(YEE_EINVAL, "pole box outside the grid");
         if (!pl.drive || !pl.state[0] || (pl.kind == YEE_LORENTZ && !pl.state[1]))
@@ -321,7 +322,7 @@ void apply_defaults(yee2d_desc& d)
 
 // Edge tile rows and columns of a descriptor: tile rows whose rows (with the row before and the row after) reach an x
 // CPML layer or hold a source / monitor line along y, tile columns likewise for y. Too many lines: every tile is
-// an edge tile. (Dispersive poles run in both loops.)
+// an edge tile. E_z poles run in both loops; H_z pole rows are edge rows.
 yee2d_tile_sets tile_sets(const yee2d_desc& d)
 {
     yee2d_tile_sets s{};
@@ -337,6 +338,8 @@ yee2d_tile_sets tile_sets(const yee2d_desc& d)
             edge = edge || (d.source[k].axis == 1 && d.source[k].line >= a && d.source[k].line <= b);
         for (int k = 0; k < d.n_monitors; ++k)
             edge = edge || (d.monitor[k].normal == 0 && d.monitor[k].position >= a && d.monitor[k].position <= b);
+        for (int k = 0; d.polarization == YEE_HZ && k < d.n_poles; ++k)   // the lean H_z loop has no pole update
+            edge = edge || (d.pole[k].i0 < i1 && d.pole[k].i1 > i0);
         if (edge) rows.push_back(ti);
     }
     for (int tj = 0; tj < s.tiles_j; ++tj) {

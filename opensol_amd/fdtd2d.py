@@ -137,13 +137,12 @@ class Model2D:
     def _add_pole(self, pole, mask, background):
         if not isinstance(pole, (DrudePole, LorentzPole)):
             raise TypeError(f'add_dispersive needs a DrudePole or LorentzPole, got {type(pole).__name__}')
-        if self.polarization != 'ez':
-            raise ValueError('dispersive poles of this FDTD path are for the E_z polarisation')
         mask = np.asarray(mask)
         if mask.dtype != np.bool_ or mask.shape != self.shape:
             raise ValueError(f'pole mask must be a bool array of shape {self.shape}, got {mask.dtype} {mask.shape}')
-        if len(self.poles) >= _lib.MAX_POLES:
-            raise ValueError(f'at most {_lib.MAX_POLES} dispersive poles per 2D model')
+        limit = _lib.MAX_POLES if self.polarization == 'ez' else _lib.MAX_POLES // 2   # H_z: one record per face set
+        if len(self.poles) >= limit:
+            raise ValueError(f'at most {limit} dispersive poles per 2D {self.polarization} model')
         if background:   # eps_r = eps_inf under the mask, the other material values kept
             materials = self.materials
             for old in np.unique(materials.ids[mask]):
@@ -152,6 +151,28 @@ class Model2D:
                 materials.ids[mask & (materials.ids == old)] = new
         self.poles.append((pole, mask.copy()))
         self.materials.version += 1
+
+    def pole_weights(self):
+        """[(pole, component, float32 weight array)]: E_z, the pole's cell mask on its cells (component 0); H_z, the
+        mask's face average on the E_x (1, hi-y) and E_y (2, hi-x) faces (physics/em/yee_grid.py:128-142, uniform
+        cells: half of each adjacent cell; the last face of a non-periodic axis pairs its cell with itself), times the
+        PEC keep (a cell or face touching PEC carries no current)."""
+        pec = self.materials.pec_flags()[self.materials.ids]
+        out = []
+        for pole, mask in self.poles:
+            if self.polarization == 'ez':
+                out.append((pole, 0, (mask & ~pec).astype(DTYPE)))
+                continue
+            m = mask.astype(DTYPE)
+            for component, axis in ((1, 1), (2, 0)):
+                n = self.shape[axis]
+                partner = (np.arange(n) + 1) % n if self.bc.periodic('xy'[axis]) else np.minimum(np.arange(n) + 1,
+                                                                                                n - 1)
+                take = (slice(None), partner) if axis == 1 else (partner, slice(None))
+                weight = DTYPE(0.5) * m + DTYPE(0.5) * m[take]
+                weight[pec | pec[take]] = 0.0
+                out.append((pole, component, weight))
+        return out
 
     def pole_term(self):
         """float64 (nx, ny): the summed stability terms of the poles on each cell (0 without)."""
@@ -214,7 +235,8 @@ class Model2D:
         if self.poles:   # per cell: eps mu_r >= the light term + mu_r x the poles' terms (physics/em/__init__.py:242-270)
             cells = self.materials.values()[self.materials.ids]
             keep = ~self.materials.pec_flags()[self.materials.ids]
-            ratio = cells[..., 0] * cells[..., 4] / (required + cells[..., 4] * self.pole_term())
+            eps_cells = cells[..., 0] if self.polarization == 'ez' else np.minimum(cells[..., 0], cells[..., 1])
+            ratio = eps_cells * cells[..., 4] / (required + cells[..., 4] * self.pole_term())
             worst = float(ratio[keep].min()) if keep.any() else 1.0
             if worst < 1.0 - 1e-6:
                 factor = math.floor(float(self.grid.courant) * math.sqrt(max(worst, 0.0)) * 1e3) / 1e3
@@ -441,18 +463,17 @@ class Engine2D:
         self.phases = [None] * len(model.monitors)
         self.dft_phases = [None] * len(model.dft_monitors)
         self.poles = []
-        pec_cells = model.materials.pec_flags()[model.materials.ids]
-        for pole, mask in model.poles:
-            box = bounding_box(mask)
+        for pole, component, weight in model.pole_weights():
+            box = bounding_box(weight > 0)
             if box is None:
                 continue
             (i0, i1), (j0, j1) = box
             coefficients = pole.ade_coefficients(float(model.dt))
-            weight = (mask & ~pec_cells)[i0:i1, j0:j1].astype(np.float32)
-            drive = torch.from_numpy(np.ascontiguousarray(weight * coefficients[-1])).to(dev)
+            drive = torch.from_numpy(np.ascontiguousarray(weight[i0:i1, j0:j1] * coefficients[-1])).to(dev)
             states = [torch.zeros((i1 - i0, j1 - j0), dtype=f32, device=dev)
                       for _ in range(1 if isinstance(pole, DrudePole) else 2)]
-            self.poles.append(dict(pole=pole, box=box, coefficients=coefficients, drive=drive, states=states))
+            self.poles.append(dict(pole=pole, component=component, box=box, coefficients=coefficients, drive=drive,
+                                   states=states))
         components = COMPONENTS[model.polarization]
         self.probes = ProbeTable(model.probes, [p.cell[0] * pitch + p.cell[1] for p in model.probes],
                                  [components.index(p.field_component) for p in model.probes], dev)
@@ -566,6 +587,7 @@ class Engine2D:
             (pl.i0, pl.i1), (pl.j0, pl.j1) = entry['box']
             drude = isinstance(entry['pole'], DrudePole)
             pl.kind = _lib.YEE_DRUDE if drude else _lib.YEE_LORENTZ
+            pl.component = entry['component']
             pl.damping = float(entry['coefficients'][0])
             pl.restoring = 0.0 if drude else float(entry['coefficients'][1])
             pl.inv_dt = float(np.float32(1.0) / np.float32(model.dt))

@@ -235,7 +235,7 @@ class _Built:
         return self._shape(selection[0], kinds)
 
     def _paint(self, material, value, index):
-        """Paint one material in list order; returns its dispersive poles (E_z: DrudePole / LorentzPole from the
+        """Paint one material in list order; returns its dispersive poles (2D: DrudePole / LorentzPole from the
         'dispersion' list, definition_common.py:1079-1099), whose background permittivity is the eps_r property,
         else the first Drude eps_inf, else 1 (:1102-1111)."""
         from .dispersion import DrudePole, LorentzPole
@@ -245,8 +245,8 @@ class _Built:
             raise ValueError(f"material {material.get('name')!r}: only eps_r, sigma_e and dispersion are part of "
                              f"the FDTD path")
         dispersion = material.get('dispersion') or []
-        if dispersion and self.kind != 'em_fdtd_ez':
-            raise ValueError(f"material {material.get('name')!r}: dispersive poles are built for em_fdtd_ez only")
+        if dispersion and self.three_d:
+            raise ValueError(f"material {material.get('name')!r}: dispersive poles are built for the 2D types only")
         poles = []
         for pole in dispersion:
             model = pole.get('model') if isinstance(pole, dict) else None

@@ -537,11 +537,10 @@ class Yee2D:
         self._add_pole(('lorentz', delta_eps, omega_0, gamma), mask, None)
 
     def _add_pole(self, pole, mask, eps_inf):
-        if self.polarization != 'ez':
-            raise ValueError("the oracle's poles are E_z only")
         mask = np.asarray(mask, dtype=bool)
         if eps_inf is not None:
-            self.materials['eps_r'][mask] = F32(eps_inf)
+            for name in (('eps_r',) if self.polarization == 'ez' else ('eps_xx', 'eps_yy')):
+                self.materials[name][mask] = F32(eps_inf)
         self.poles.append({'pole': pole, 'mask': mask.copy()})
         self._finalized = False
 
@@ -700,6 +699,19 @@ class Yee2D:
             self.keep_x, self.keep_y = ~(pec | pec[:, pj]), ~(pec | pec[pi])   # faces touching a PEC cell
             self.ca_x, self.cb_x = self.ca_x * self.keep_x, self.cb_x * self.keep_x
             self.ca_y, self.cb_y = self.ca_y * self.keep_y, self.cb_y * self.keep_y
+            # poles on the E faces (fdtd_hz_2d.py:441-473, 539-561): drive = the mask's face average (yee_grid.py:
+            # 128-142, uniform cells) x the face keep x the drive coefficient
+            for entry in self.poles:
+                kind = entry['pole'][0]
+                coefficients = (drude_coefficients(*entry['pole'][1:], self.dt) if kind == 'drude'
+                                else lorentz_coefficients(*entry['pole'][1:], self.dt))
+                entry['coefficients'] = tuple(real(c) for c in coefficients)
+                m = entry['mask'].astype(real)
+                entry['drive'] = {'ex': (half * m + half * m[:, pj]) * self.keep_x * real(coefficients[-1]),
+                                  'ey': (half * m + half * m[pi]) * self.keep_y * real(coefficients[-1])}
+                if 'state' not in entry:
+                    entry['state'] = {name: [np.zeros(self.geo.shape, dtype=real)
+                                             for _ in range(1 if kind == 'drude' else 2)] for name in ('ex', 'ey')}
         self._calibrate()
         if self.n_step == 0:
             interval = dft_sample_interval(self.dt, self.sources, [mon.frequencies for mon in self.monitors])
@@ -816,6 +828,16 @@ class Yee2D:
         hz += curl
         for side in self.sides:   # apply_hz: x sides (ey, db, hz), y sides (ex, db, hz) (boundaries.py:766-768)
             side.apply('e', ey if side.axis == 'x' else ex, self.db, hz)
+        # pole updates on the E faces from E^n (fdtd_hz_2d.py:286-310)
+        for entry in self.poles:
+            for name, field in (('ex', ex), ('ey', ey)):
+                state, drive = entry['state'][name], entry['drive'][name]
+                if entry['pole'][0] == 'drude':
+                    state[0] += entry['coefficients'][0] * state[0] + drive * field
+                else:
+                    damping, restoring, _ = entry['coefficients']
+                    state[1] += damping * state[1] - restoring * state[0] + drive * field
+                    state[0] += state[1]
         # update_exy (fdtd_hz_2d_kernels.py:146-155)
         ca_x, cb_x, ca_y, cb_y = self.ca_x, self.cb_x, self.ca_y, self.cb_y
         ex[:, :-1] = ca_x[:, :-1] * ex[:, :-1] + cb_x[:, :-1] * (hz[:, 1:] - hz[:, :-1]) * self.inv_dy_h[:-1]
@@ -829,6 +851,15 @@ class Yee2D:
                 side.apply('h', hz, cb_y, ey)
             else:
                 side.apply('h', hz, cb_x, ex)
+        # pole corrections (fdtd_hz_2d.py:312-323): Drude E -= cb J, Lorentz E -= cb / dt dP, per face set
+        inv_dt = self.dtype(1.0) / self.dtype(self.dt)
+        for entry in self.poles:
+            for field, cb, name in ((ex, cb_x, 'ex'), (ey, cb_y, 'ey')):
+                state = entry['state'][name]
+                if entry['pole'][0] == 'drude':
+                    field -= cb * state[0]
+                else:
+                    field -= cb * inv_dt * state[1]
 
     def run(self, duration=None, n_steps=None):
         """Tick loop stopping at t >= t_end - dt/2 (coupling/runner.py:174-204); returns the steps run."""

@@ -104,11 +104,11 @@ POLES2D = dict(size_x=2e-6, size_y=1e-6, dx=25e-9, npml=10, boundaries='periodic
                steps=(300, 800))
 
 
-def build_poles2d(target, **options):
+def build_poles2d(target, polarization='ez', **options):
     """The pole case on the oracle Yee2D, opensol_amd.Simulation or Model2D; returns (sim, monitor)."""
     spec = POLES2D
     sim = target(spec['size_x'], spec['size_y'], spec['dx'], npml=spec['npml'], boundaries=spec['boundaries'],
-                 polarization='ez', **options)
+                 polarization=polarization, **options)
     (x0, y0, x1, y1), eps = spec['dielectric']
     sim.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
     (cx, cy), radius = spec['lorentz_disc']

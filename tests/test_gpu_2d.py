@@ -254,7 +254,8 @@ def test_dft_boxes_match_oracle(gpu, polarization):
             assert peak_error <= 1e-5 and l2 <= 1e-5, (index, name, peak_error, l2)
 
 
-def test_dispersive_poles_match_oracle(gpu):
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_dispersive_poles_match_oracle(gpu, polarization):
     """Drude rectangle (eps_inf background) under a PEC corner and a Lorentz disc on a dielectric, in the fused E_z
     update, against the FP64 oracle (which reproduces the reference's own run to 1e-12): fields <= 1e-5 relative L2
     at 300 and 800 steps, normalised by the field's largest norm so far (by step 800 the pulse has mostly left; the
@@ -263,10 +264,10 @@ def test_dispersive_poles_match_oracle(gpu):
     from tests.cases import POLES2D, build_poles2d
     from opensol_amd.batch import Batch2D
     Simulation = simulation_class()
-    ref, ref_monitor = build_poles2d(Yee2D, phase_dtype=np.complex128)
-    sim, monitor = build_poles2d(Simulation)
-    graphed, _ = build_poles2d(Simulation, graph_steps=16)
-    batched, _ = build_poles2d(Simulation)
+    ref, ref_monitor = build_poles2d(Yee2D, polarization, phase_dtype=np.complex128)
+    sim, monitor = build_poles2d(Simulation, polarization)
+    graphed, _ = build_poles2d(Simulation, polarization, graph_steps=16)
+    batched, _ = build_poles2d(Simulation, polarization)
     done = 0
     peak = {}
     for step in POLES2D['steps']:
@@ -277,10 +278,13 @@ def test_dispersive_poles_match_oracle(gpu):
         for name in sim.model.components:
             peak[name] = max(peak.get(name, 0.0), float(np.linalg.norm(ref.fields[name])))
             error = float(np.linalg.norm(sim.engine.field(name) - ref.fields[name])) / peak[name]
-            print(f'poles step {step} {name}: rel L2 {error:.3e}')
+            print(f'{polarization} poles step {step} {name}: rel L2 {error:.3e}')
             assert error <= FIELD_TOL, (step, name, error)
             np.testing.assert_array_equal(graphed.engine.field(name), sim.engine.field(name))
-    assert not np.any(sim.engine.field('ez')[ref.pec_mask()])
+    if polarization == 'ez':
+        assert not np.any(sim.engine.field('ez')[ref.pec_mask()])
+    else:
+        assert not np.any(sim.engine.field('ex')[~ref.keep_x]) and not np.any(sim.engine.field('ey')[~ref.keep_y])
     flux_error = np.max(np.abs(monitor.flux() / ref_monitor.flux() - 1.0))
     print(f'poles flux rel error {flux_error:.3e}')
     assert flux_error <= 1e-5

@@ -234,6 +234,17 @@ def test_pole_masks_background_and_drive_match_the_oracle():
         np.testing.assert_array_equal(mask, entry['mask'])
         coefficients = pole.ade_coefficients(float(model.dt))
         assert tuple(float(c) for c in coefficients) == tuple(float(c) for c in entry['coefficients'])
+    # H_z: the drive of each face set is the mask's face average times the face keep, as the oracle's
+    model_hz, _ = build_poles2d(Model2D, 'hz')
+    ref_hz, _ = build_poles2d(Yee2D, 'hz')
+    model_hz.finalize()
+    ref_hz.finalize()
+    weights = model_hz.pole_weights()
+    assert [c for _, c, _ in weights] == [1, 2, 1, 2]
+    for (pole, component, weight), entry in zip(weights, [e for e in ref_hz.poles for _ in (0, 1)]):
+        drive = pole.ade_coefficients(float(model_hz.dt))[-1]
+        want = entry['drive']['ex' if component == 1 else 'ey']
+        np.testing.assert_allclose(weight * np.float64(drive), want, rtol=1e-7, atol=0)
     pec = model.materials.pec_flags()[model.materials.ids]
     np.testing.assert_array_equal(pec, ref.pec_mask())
     np.testing.assert_array_equal(model.materials.property_array(0)[~pec], ref.materials['eps_r'][~pec])
@@ -242,6 +253,8 @@ def test_pole_masks_background_and_drive_match_the_oracle():
     unstable.add_drude_rectangle(0.2e-6, 0.2e-6, 0.6e-6, 0.6e-6, DrudePole(1.37e16, 1e14, 1.0))
     with pytest.raises(ValueError):
         unstable.check_courant()
-    with pytest.raises(ValueError):
-        Model2D(1e-6, 1e-6, 25e-9, npml=5, polarization='hz').add_drude_rectangle(0, 0, 1e-7, 1e-7,
-                                                                                    DrudePole(1e15, 1e13, 2.0))
+    hz = Model2D(1e-6, 1e-6, 25e-9, npml=5, polarization='hz')
+    for _ in range(4):
+        hz.add_drude_rectangle(0, 0, 1e-7, 1e-7, DrudePole(1e15, 1e13, 2.0))
+    with pytest.raises(ValueError):   # four H_z poles fill the eight face records
+        hz.add_drude_rectangle(0, 0, 1e-7, 1e-7, DrudePole(1e15, 1e13, 2.0))

@@ -153,7 +153,7 @@ def test_3d_model_builds_the_scripted_case():
 
 def test_dispersive_material_becomes_a_pole_on_its_cells():
     """A material 'dispersion' list (definition_common.py:1009-1111): one DrudePole on the film's cells, the film's
-    permittivity its eps_inf; refused for H_z and with unknown pole keys."""
+    permittivity its eps_inf; H_z takes it too; unknown pole keys are refused."""
     from opensol_amd import DrudePole
     golden, spec = golden_spec('drude')
     names = M.evaluate_parameters(spec['parameters'])
@@ -166,8 +166,7 @@ def test_dispersive_material_becomes_a_pole_on_its_cells():
     np.testing.assert_array_equal(model.materials.property_array(0)[film], np.float32(2.25))
     other = json.loads(json.dumps(spec))
     other['physics'][0]['type'] = 'em_fdtd_hz'
-    with pytest.raises(ValueError):
-        M._Built(other, names)
+    assert len(M._Built(other, names).sim.model.poles) == 1   # H_z: the pole on the film's E faces
     other = json.loads(json.dumps(spec))
     other['materials'][0]['dispersion'][0]['tau'] = 1.0
     with pytest.raises(ValueError):

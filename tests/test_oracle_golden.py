@@ -108,17 +108,19 @@ def test_oracle_reproduces_reference_pec_case(polarization):
             assert error <= 1e-6, (index, name, error)
 
 
-def test_oracle_reproduces_reference_pole_case():
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_oracle_reproduces_reference_pole_case(polarization):
     """Drude rectangle (eps_inf background) under a PEC corner and a Lorentz disc on a dielectric: the FP64 oracle
-    reproduces the reference's FP64 run (masks, eps_r, fields at two steps, flux spectrum)."""
+    reproduces the reference's FP64 run (masks, eps_r, fields at two steps, flux spectrum); H_z: poles on the E faces."""
     from tests.cases import POLES2D, build_poles2d
-    golden = np.load(os.path.join(GOLDEN, 'poles2d_ez.npz'))
-    sim, monitor = build_poles2d(Yee2D, phase_dtype=np.complex64)
+    golden = np.load(os.path.join(GOLDEN, f'poles2d_{polarization}.npz'))
+    sim, monitor = build_poles2d(Yee2D, polarization, phase_dtype=np.complex64)
     assert (sim.nx, sim.ny) == (int(golden['nx']), int(golden['ny']))
     np.testing.assert_array_equal(sim.poles[0]['mask'], golden['drude_mask'])
     np.testing.assert_array_equal(sim.poles[1]['mask'], golden['lorentz_mask'])
     np.testing.assert_array_equal(sim.pec_regions[0], golden['pec_mask'])
-    np.testing.assert_array_equal(sim.materials['eps_r'].astype(np.float64), golden['eps_r'])
+    eps_name = 'eps_r' if polarization == 'ez' else 'eps_xx'
+    np.testing.assert_array_equal(sim.materials[eps_name].astype(np.float64), golden['eps_r'])
     done = 0
     for step in POLES2D['steps']:
         sim.run(n_steps=step - done)
