@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 8
+#define YEE_ABI_VERSION 9
 #define YEE_MAX_SOURCES 4
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
@@ -282,6 +282,25 @@ typedef struct yee3d_cpml_axis {
  * coefficients of the id, the E face coefficients of a face between two cells of the id, and the float32 eps_r and
  * sigma_e from which a face between two ids forms its mean coefficients (kernels3d.py:61-85, shares 1/2).
  */
+/*
+ * Dispersive pole record of a 3D instance on one E component (0 E_x, 1 E_y, 2 E_z: the hi faces along that axis) over
+ * the index box [i0, i1) x [j0, j1) x [k0, k1): the updates and corrections of yee_pole, in the fused E update of the
+ * step kernel. drive holds the pole's drive coefficient times the face average of its cell mask along the component's
+ * axis. One pole is three records. Replaces physics/em3d/engine3d.py:431-530 (add_pole, _pole_region) and the kernels
+ * of kernels3d.py:967-1004.
+ */
+typedef struct yee3d_pole {
+    int32_t kind;            /* YEE_DRUDE or YEE_LORENTZ */
+    int32_t component;       /* 0 E_x, 1 E_y, 2 E_z */
+    int32_t i0, i1, j0, j1, k0, k1;
+    float damping, restoring, inv_dt;
+    int32_t pad_;
+    const float* drive;      /* (i1 - i0) * (j1 - j0) * (k1 - k0), k fastest */
+    float* state[2];         /* Drude: J; Lorentz: P, dP */
+} yee3d_pole;
+
+#define YEE3D_MAX_POLES 12
+
 typedef struct yee3d_desc {
     int32_t nx, ny, nz;
     int32_t periodic[3];
@@ -303,6 +322,9 @@ typedef struct yee3d_desc {
     int32_t n_dft;
     int32_t pad2_;
     yee_dft_region dft[YEE_MAX_DFT];
+    int32_t n_poles;
+    int32_t pad3_;
+    yee3d_pole pole[YEE3D_MAX_POLES];
     yee_probes probes;
     int64_t* clock;
     int64_t step_limit;
@@ -313,7 +335,8 @@ typedef struct yee_ctx yee_ctx;
 /* Library ABI version (YEE_ABI_VERSION) and sizeof checks for bindings. */
 int32_t yee_abi_version(void);
 /* 0: yee2d_desc, 1: yee_line_source, 2: yee_flux_line, 3: yee_cpml_axis, 4: yee3d_desc, 5: yee_point_source,
- * 6: yee_flux_plane, 7: yee3d_cpml_axis, 8: yee3d_axis_row, 9: yee_probes, 10: yee_dft_region, 11: yee_pole */
+ * 6: yee_flux_plane, 7: yee3d_cpml_axis, 8: yee3d_axis_row, 9: yee_probes, 10: yee_dft_region, 11: yee_pole,
+ * 12: yee3d_pole */
 size_t yee_sizeof(int32_t which);
 const char* yee_last_error(void);
 

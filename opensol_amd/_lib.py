@@ -6,11 +6,12 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 8
+ABI_VERSION = 9
 MAX_SOURCES = 4
 MAX_MONITORS = 4
 MAX_DFT = 4
 MAX_POLES = 8
+MAX_POLES_3D = 12
 YEE_DRUDE, YEE_LORENTZ = 1, 2
 MAX_SOURCES_3D = 4
 MAX_PLANES_3D = 8
@@ -86,17 +87,24 @@ class AxisRow3DC(ctypes.Structure):
                 ('psi_row_e', c_i32), ('b_e', c_f32), ('g_e', c_f32)]
 
 
+class Pole3DC(ctypes.Structure):
+    _fields_ = [('kind', c_i32), ('component', c_i32), ('i0', c_i32), ('i1', c_i32), ('j0', c_i32), ('j1', c_i32),
+                ('k0', c_i32), ('k1', c_i32), ('damping', c_f32), ('restoring', c_f32), ('inv_dt', c_f32),
+                ('pad_', c_i32), ('drive', c_ptr), ('state', c_ptr * 2)]
+
+
 class Desc3DC(ctypes.Structure):
     _fields_ = [('nx', c_i32), ('ny', c_i32), ('nz', c_i32), ('periodic', c_i32 * 3), ('rows_per_tile', c_i32),
                 ('planes_per_tile', c_i32), ('field', (c_ptr * 6) * 2), ('material', c_ptr), ('table', c_ptr),
                 ('n_materials', c_i32), ('dt', c_f32), ('axis_rows', c_ptr * 3), ('cpml', Cpml3DAxisC * 3), ('n_sources', c_i32), ('n_planes', c_i32), ('sample_interval', c_i32),
                 ('pad_', c_i32), ('source', PointSourceC * MAX_SOURCES_3D), ('plane', FluxPlaneC * MAX_PLANES_3D),
                 ('n_dft', c_i32), ('pad2_', c_i32), ('dft', DftRegionC * MAX_DFT),
+                ('n_poles', c_i32), ('pad3_', c_i32), ('pole', Pole3DC * MAX_POLES_3D),
                 ('probes', ProbesC), ('clock', c_ptr), ('step_limit', c_i64)]
 
 
 STRUCTS = (Desc2DC, LineSourceC, FluxLineC, CpmlAxisC, Desc3DC, PointSourceC, FluxPlaneC, Cpml3DAxisC, AxisRow3DC,
-           ProbesC, DftRegionC, PoleC)
+           ProbesC, DftRegionC, PoleC, Pole3DC)
 
 # Every entry point include/yee.h declares: (name, restype, argtypes)
 SIGNATURES = (

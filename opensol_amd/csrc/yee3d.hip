@@ -232,8 +232,35 @@ __device__ __forceinline__ HNew h_cell(const yee3d_desc& d, int par, bool store,
     return HNew{vx, vy, vz};
 }
 
-// E update of one owned cell with its CPML (psi in place) and point sources.
-template <bool LDS>
+// Pole records on cell (i, j, k): each record whose box holds the cell updates its state from E^n of its component
+// and corrects the updated value with that face's cb (physics/em3d/engine3d.py:508-530).
+__device__ __forceinline__ void apply_poles_3d(const yee3d_desc& d, int i, int j, int k, const float e_old[3],
+                                               const float cb[3], float e_new[3])
+{
+    for (int p = 0; p < d.n_poles; ++p) {
+        const yee3d_pole& pl = d.pole[p];
+        if (i < pl.i0 || i >= pl.i1 || j < pl.j0 || j >= pl.j1 || k < pl.k0 || k >= pl.k1) continue;
+        const int64_t q = ((int64_t)(i - pl.i0) * (pl.j1 - pl.j0) + (j - pl.j0)) * (pl.k1 - pl.k0) + (k - pl.k0);
+        const int c = pl.component;
+        const float drive = pl.drive[q];
+        if (pl.kind == YEE_DRUDE) {
+            float jc = pl.state[0][q];
+            jc += pl.damping * jc + drive * e_old[c];
+            pl.state[0][q] = jc;
+            e_new[c] -= cb[c] * jc;
+        } else {
+            const float pc = pl.state[0][q];
+            float dpc = pl.state[1][q];
+            dpc += pl.damping * dpc - pl.restoring * pc + drive * e_old[c];
+            pl.state[0][q] = pc + dpc;
+            pl.state[1][q] = dpc;
+            e_new[c] -= cb[c] * pl.inv_dt * dpc;
+        }
+    }
+}
+
+// E update of one owned cell with its CPML (psi in place), dispersive poles (POLES) and point sources.
+template <bool LDS, bool POLES>
 __device__ __forceinline__ void e_cell(const yee3d_desc& d, int64_t n, float& ex, float& ey, float& ez, float2 fx,
                                        float2 fy, float2 fz, float dhz_y, float dhy_z, float dhx_z, float dhz_x,
                                        float dhy_x, float dhx_y, float iex, float iey, float iez, const Row& rx,
@@ -275,6 +302,15 @@ __device__ __forceinline__ void e_cell(const yee3d_desc& d, int64_t n, float& ex
         vx -= fx.y * a;
         vy += fy.y * b;
     }
+    if constexpr (POLES) {
+        const float e_old[3] = {ex, ey, ez};
+        const float cb[3] = {fx.y, fy.y, fz.y};
+        float e_new[3] = {vx, vy, vz};
+        apply_poles_3d(d, i, j, k, e_old, cb, e_new);
+        vx = e_new[0];
+        vy = e_new[1];
+        vz = e_new[2];
+    }
     for (int s = 0; s < d.n_sources; ++s) {
         const yee_point_source& src = d.source[s];
         if (src.i == i && src.j == j && src.k == k && n < src.wave_len) {
@@ -313,7 +349,7 @@ __device__ __forceinline__ void stf(Rsrc r, uint32_t lane_bytes, uint32_t row_by
 }
 
 // One wavefront tile: planes [i0, i1), rows j0..j0+J-1, z cells tk*OWNED .. +OWNED-1.
-template <int J, bool LDS>
+template <int J, bool LDS, bool POLES>
 __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, int tk)
 {
     const int lane = threadIdx.x & (LANES - 1);
@@ -460,7 +496,8 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
                 const float2 fz = face<LDS>(d, id, id_zp);
                 const Row ry_e = e_row(uniform_row(d.axis_rows[1], jr[r]));
                 float ex = cur[r].ex, ey = cur[r].ey, ez = cur[r].ez;
-                e_cell<LDS>(d, n, ex, ey, ez, fx, fy, fz, dhz_y, dhy_z, dhx_z, dhz_x, dhy_x, dhx_y, ax.inv_e, iey[r],
+                e_cell<LDS, POLES>(d, n, ex, ey, ez, fx, fy, fz, dhz_y, dhy_z, dhx_z, dhz_x, dhy_x, dhx_y, ax.inv_e,
+                                   iey[r],
                             iez, rx_e, ry_e, rz_e, i, jr[r], k);
                 stf(wex, vk4, roff[r], ex);
                 stf(wey, vk4, roff[r], ey);
@@ -483,7 +520,7 @@ namespace yee3 {
 
 // The tiles of one block of one instance: tile t = (i chunk, j tile, k tile), k fastest, one per wavefront. Every
 // thread of the block reaches the table barrier before a wave without a tile leaves.
-template <int J>
+template <int J, bool POLES>
 __device__ __forceinline__ void step_block(const yee3d_desc& d, int64_t n, int tile, int tiles, int tiles_j,
                                            int tiles_k)
 {
@@ -497,21 +534,32 @@ __device__ __forceinline__ void step_block(const yee3d_desc& d, int64_t n, int t
     const int i0 = ic * d.planes_per_tile;
     const int i1 = min(d.nx, i0 + d.planes_per_tile);
     if (lds)
-        tile3d<J, true>(d, n, i0, i1, tj, tk);
+        tile3d<J, true, POLES>(d, n, i0, i1, tj, tk);
     else
-        tile3d<J, false>(d, n, i0, i1, tj, tk);
+        tile3d<J, false, POLES>(d, n, i0, i1, tj, tk);
 }
 
 }  // namespace yee3
 
-// One 3D instance, four tiles per block.
-template <int J>
-__global__ void __launch_bounds__(256) yee3d_step_kernel(yee3d_desc d, int32_t tiles_j, int32_t tiles_k,
+// One 3D instance, four tiles per block. POLES: the E update carries the dispersive-pole records; that instantiation
+// reads the descriptor in place at the kernel-argument pointer (constant address space), since the records are
+// indexed at run time.
+template <int J, bool POLES>
+__global__ void __launch_bounds__(256) yee3d_step_kernel(yee3d_desc desc, int32_t tiles_j, int32_t tiles_k,
                                                         int32_t tiles, int32_t offset)
 {
-    const int64_t n = *d.clock + offset;
-    if (n >= d.step_limit) return;
-    yee3::step_block<J>(d, n, blockIdx.x * 4 + yee3::wave_index(), tiles, tiles_j, tiles_k);
+    if constexpr (POLES) {
+        typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
+        const yee3d_desc& d = *(const yee3d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
+        if (d.clock != desc.clock) return;
+        const int64_t n = *d.clock + offset;
+        if (n >= d.step_limit) return;
+        yee3::step_block<J, true>(d, n, blockIdx.x * 4 + yee3::wave_index(), tiles, tiles_j, tiles_k);
+    } else {
+        const int64_t n = *desc.clock + offset;
+        if (n >= desc.step_limit) return;
+        yee3::step_block<J, false>(desc, n, blockIdx.x * 4 + yee3::wave_index(), tiles, tiles_j, tiles_k);
+    }
 }
 
 // Many 3D instances of uneven sizes in one launch: block b belongs to the instance whose block range
@@ -536,10 +584,11 @@ __global__ void __launch_bounds__(256) yee3d_batch_kernel(const yee3d_desc* __re
     const int tiles_j = (d.ny + J - 1) / J;
     const int tiles_k = (d.nz + yee3::OWNED - 1) / yee3::OWNED;
     const int tiles = ((d.nx + d.planes_per_tile - 1) / d.planes_per_tile) * tiles_j * tiles_k;
-    yee3::step_block<J>(d, n, (block - prefix[lo]) * 4 + yee3::wave_index(), tiles, tiles_j, tiles_k);
+    yee3::step_block<J, false>(d, n, (block - prefix[lo]) * 4 + yee3::wave_index(), tiles, tiles_j, tiles_k);
 }
 
-template __global__ void yee3d_step_kernel<4>(yee3d_desc, int32_t, int32_t, int32_t, int32_t);
+template __global__ void yee3d_step_kernel<4, false>(yee3d_desc, int32_t, int32_t, int32_t, int32_t);
+template __global__ void yee3d_step_kernel<4, true>(yee3d_desc, int32_t, int32_t, int32_t, int32_t);
 template __global__ void yee3d_batch_kernel<4>(const yee3d_desc* __restrict__, const int32_t* __restrict__, int32_t,
                                                int32_t);
 

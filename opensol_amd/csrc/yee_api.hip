@@ -18,7 +18,7 @@ __global__ void yee2d_step_kernel(yee2d_desc d, yee2d_tile_sets sets, int32_t of
 template <int VEC>
 __global__ void yee2d_batch_kernel(const yee2d_desc* __restrict__ descs, const int32_t* __restrict__ prefix,
                                    int32_t n_inst, int32_t offset);
-template <int J>
+template <int J, bool POLES>
 __global__ void yee3d_step_kernel(yee3d_desc d, int32_t tiles_j, int32_t tiles_k, int32_t tiles, int32_t offset);
 template <int J>
 __global__ void yee3d_batch_kernel(const yee3d_desc* __restrict__ descs, const int32_t* __restrict__ prefix,
@@ -214,6 +214,17 @@ int32_t check_3d(const yee3d_desc* d)
     for (int a = 0; a < 3; ++a)
         if (n[a] < 2) return fail(YEE_EINVAL, "3D grid needs nx, ny, nz >= 2");
     if (check_dft(d->dft, d->n_dft, n, 3, 6, d->sample_interval) != YEE_OK) return YEE_EINVAL;
+    if (d->n_poles < 0 || d->n_poles > YEE3D_MAX_POLES) return fail(YEE_EINVAL, "n_poles out of range");
+    for (int p = 0; p < d->n_poles; ++p) {
+        const yee3d_pole& pl = d->pole[p];
+        if (pl.kind != YEE_DRUDE && pl.kind != YEE_LORENTZ) return fail(YEE_EINVAL, "pole kind must be 1 or 2");
+        if (pl.component < 0 || pl.component > 2) return fail(YEE_EINVAL, "pole component must be 0, 1 or 2");
+        if (pl.i0 < 0 || pl.i1 > d->nx || pl.i0 >= pl.i1 || pl.j0 < 0 || pl.j1 > d->ny || pl.j0 >= pl.j1 ||
+            pl.k0 < 0 || pl.k1 > d->nz || pl.k0 >= pl.k1)
+            return fail(YEE_EINVAL, "pole box outside the grid");
+        if (!pl.drive || !pl.state[0] || (pl.kind == YEE_LORENTZ && !pl.state[1]))
+            return fail(YEE_EINVAL, "pole arrays missing");
+    }
     if ((int64_t)d->nx * d->ny * d->nz > ((int64_t)1 << 40)) return fail(YEE_EINVAL, "3D grid too large");
     if ((int64_t)d->ny * d->nz >= ((int64_t)1 << 29)) return fail(YEE_EINVAL, "3D plane (ny * nz) too large");
     if (d->rows_per_tile != kRows3D) return fail(YEE_EINVAL, "rows_per_tile must be 4 (or 0)");
@@ -277,6 +288,14 @@ int64_t tiles_2d(const yee2d_desc& d)
 using StepKernel = void (*)(yee2d_desc, yee2d_tile_sets, int32_t);
 
 // Bulk-tile (edge = false) or edge-tile kernel of a descriptor.
+// The 3D step kernel of an instance: the one with the pole update only when it has poles (its registers and its
+// descriptor reads differ).
+typedef void (*StepKernel3D)(yee3d_desc, int32_t, int32_t, int32_t, int32_t);
+StepKernel3D step_kernel_3d(const yee3d_desc& d)
+{
+    return d.n_poles > 0 ? yee3d_step_kernel<kRows3D, true> : yee3d_step_kernel<kRows3D, false>;
+}
+
 StepKernel step_kernel(const yee2d_desc& d, bool edge)
 {
     if (d.polarization == YEE_EZ) {
@@ -450,7 +469,7 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
     if (ctx->kind == kSingle3D) {
         const int sample_blocks = (int)((ctx->sample_cells + 255) / 256);
         for (int32_t k = 0; k < count; ++k) {
-            hipLaunchKernelGGL(yee3d_step_kernel<kRows3D>, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d3,
+            hipLaunchKernelGGL(step_kernel_3d(ctx->d3), dim3(ctx->blocks), dim3(256), 0, stream, ctx->d3,
                                ctx->tiles_j3, ctx->tiles_k3, (int32_t)ctx->tiles, k);
             if (sample_blocks > 0)
                 hipLaunchKernelGGL(yee3d_sample_kernel, dim3(sample_blocks), dim3(256), 0, stream, ctx->d3,
@@ -540,6 +559,7 @@ size_t yee_sizeof(int32_t which)
         case 9: return sizeof(yee_probes);
         case 10: return sizeof(yee_dft_region);
         case 11: return sizeof(yee_pole);
+        case 12: return sizeof(yee3d_pole);
         default: return 0;
     }
 }
@@ -666,7 +686,7 @@ int32_t step_timed_3d(yee_ctx* ctx, int64_t n_steps, hipStream_t s, double* kern
     int32_t status = YEE_OK;
     for (int64_t k = 0; k < n_steps && status == YEE_OK; ++k) {
         (void)hipEventRecord(ev[0], s);
-        hipLaunchKernelGGL(yee3d_step_kernel<kRows3D>, dim3(ctx->blocks), dim3(256), 0, s, d, ctx->tiles_j3,
+        hipLaunchKernelGGL(step_kernel_3d(ctx->d3), dim3(ctx->blocks), dim3(256), 0, s, d, ctx->tiles_j3,
                            ctx->tiles_k3, (int32_t)ctx->tiles, 0);
         (void)hipEventRecord(ev[1], s);
         if (sample_blocks > 0)
@@ -808,6 +828,7 @@ int32_t yee3d_batch_create(const yee3d_desc* descs, int32_t n, void* descs_dev, 
         if (host[k].rows_per_tile != kRows3D) return fail(YEE_EINVAL, "batched 3D instances use 4 rows per tile");
         if (host[k].probes.n_probes > 0) return fail(YEE_EINVAL, "batched instances take no field probes");
         if (host[k].n_dft > 0) return fail(YEE_EINVAL, "batched instances take no DFT regions");
+        if (host[k].n_poles > 0) return fail(YEE_EINVAL, "batched 3D instances take no dispersive poles");
         const int64_t next = block_prefix[k] + (tiles_3d(host[k]) + 3) / 4;
         if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
         block_prefix[k + 1] = (int32_t)next;

@@ -17,6 +17,7 @@ from .constants import C0, DTYPE, EPS0, MU0
 from .domain import Domain
 from .fdtd2d import clock_times, steps_for_duration
 from .materials import MaterialMap, semi_implicit
+from .dispersion import DrudePole, LorentzPole, pole_step_term
 from .monitors import DFTPlane3D, FieldProbe, FluxBox, FluxPlane, ProbeTable, dft_sample_interval
 from .sources import PointDipole, resolve_frequencies, resolve_frequency
 from .waveforms import make_temporal
@@ -71,6 +72,7 @@ class Model3D:
             self.inv_h.append(inv_h)
         self.sample_interval = 1
         self.pec_boxes = []   # index ranges of PEC boxes, painted over the dielectrics at finalize
+        self.poles = []       # (DrudePole or LorentzPole, bool (nx, ny, nz) mask)
         self.probes = []
         self.dft_monitors = []
 
@@ -142,9 +144,38 @@ class Model3D:
                 ids[key] = pec
             self.materials.version += 1
 
+    def add_dispersive(self, pole, window, mask):
+        """Drude or Lorentz pole on the bool mask of the cells of `window` (a slice per axis) over their own
+        permittivity (physics/em3d/physics3d.py:346-349)."""
+        if not isinstance(pole, (DrudePole, LorentzPole)):
+            raise TypeError(f'add_dispersive needs a DrudePole or LorentzPole, got {type(pole).__name__}')
+        full = np.zeros(self.shape, dtype=bool)
+        window = tuple(window)
+        mask = np.asarray(mask)
+        if len(window) != 3 or mask.dtype != np.bool_ or mask.shape != full[window].shape:
+            raise ValueError(f'pole mask must be a bool array of the window shape {full[window].shape}')
+        if 3 * (len(self.poles) + 1) > _lib.MAX_POLES_3D:
+            raise ValueError(f'at most {_lib.MAX_POLES_3D // 3} dispersive poles per 3D model')
+        full[window] = mask
+        self.poles.append((pole, full))
+        self.materials.version += 1
+
+    def pole_weights(self):
+        """[(pole, component, float32 weight)]: per E component the face average of the pole's cell mask along that
+        axis (physics/em3d/engine3d.py:461-488; uniform cells: half of each adjacent cell, the last face of a
+        non-periodic axis pairs its cell with itself)."""
+        out = []
+        for pole, mask in self.poles:
+            m = mask.astype(DTYPE)
+            for axis in range(3):
+                n = self.shape[axis]
+                partner = (np.arange(n) + 1) % n if self.periodic[axis] else np.minimum(np.arange(n) + 1, n - 1)
+                out.append((pole, axis, DTYPE(0.5) * m + DTYPE(0.5) * np.take(m, partner, axis=axis)))
+        return out
+
     def check_courant(self):
-        """ValueError when eps_r mu_r < (C0 dt)^2 sum 1/d^2 for a material in use (physics/em3d/engine3d.py:824-852);
-        PEC cells do not step E and set no limit."""
+        """ValueError when eps_r mu_r < (C0 dt)^2 sum 1/d^2 (+ mu_r x the poles' terms in their cells) for a material
+        in use (physics/em3d/engine3d.py:824-852); PEC cells do not step E and set no limit."""
         used = self.materials.used()
         values = self.materials.values()[used[~self.materials.pec_flags()[used]]]
         if values.size == 0:
@@ -152,6 +183,14 @@ class Model3D:
         d = float(self.grid.dx)
         required = (C0 * float(self.dt)) ** 2 * 3.0 / d ** 2
         worst = float((values[:, 0] * values[:, 4]).min() / required)
+        if self.poles:
+            term = np.zeros(self.shape)
+            for pole, mask in self.poles:
+                term[mask] += pole_step_term(pole, float(self.dt))
+            cells = self.materials.values()[self.materials.ids]
+            keep = ~self.materials.pec_flags()[self.materials.ids]
+            ratio = cells[..., 0] * cells[..., 4] / (required + cells[..., 4] * term)
+            worst = min(worst, float(ratio[keep].min()) if keep.any() else 1.0)
         if worst < 1.0 - 1e-6:
             factor = math.floor(float(self.grid.courant) * math.sqrt(max(worst, 0.0)) * 1e3) / 1e3
             raise ValueError(f'eps_r * mu_r is below (C0 dt)^2 sum 1/d^2, so the time step is unstable; lower '
@@ -348,6 +387,19 @@ class Engine3D:
         self.waves = [None] * len(model.sources)
         self.phases = [None] * len(model.monitors)
         self.dft_phases = [None] * len(model.dft_monitors)
+        self.poles = []
+        for pole, component, weight in model.pole_weights():
+            nonzero = [np.nonzero(np.any(weight > 0, axis=tuple(b for b in range(3) if b != a)))[0] for a in range(3)]
+            if nonzero[0].size == 0:
+                continue
+            box = tuple((int(v[0]), int(v[-1]) + 1) for v in nonzero)
+            coefficients = pole.ade_coefficients(float(model.dt))
+            crop = tuple(slice(lo, hi) for lo, hi in box)
+            drive = torch.from_numpy(np.ascontiguousarray(weight[crop] * coefficients[-1])).to(dev)
+            states = [torch.zeros(drive.shape, dtype=torch.float32, device=dev)
+                      for _ in range(1 if isinstance(pole, DrudePole) else 2)]
+            self.poles.append(dict(pole=pole, component=component, box=box, coefficients=coefficients, drive=drive,
+                                   states=states))
         ny, nz = shape[1], shape[2]
         self.probes = ProbeTable(model.probes, [(p.cell[0] * ny + p.cell[1]) * nz + p.cell[2] for p in model.probes],
                                  [COMPONENTS.index(p.field_component) for p in model.probes], dev)
@@ -441,6 +493,19 @@ class Engine3D:
         d.n_dft = len(model.dft_monitors)
         for k, monitor in enumerate(model.dft_monitors):
             monitor.fill(d.dft[k], COMPONENTS, self.dft_phases[k], self.dft_accumulators[k])
+        d.n_poles = len(self.poles)
+        for k, entry in enumerate(self.poles):
+            pl = d.pole[k]
+            (pl.i0, pl.i1), (pl.j0, pl.j1), (pl.k0, pl.k1) = entry['box']
+            drude = isinstance(entry['pole'], DrudePole)
+            pl.kind = _lib.YEE_DRUDE if drude else _lib.YEE_LORENTZ
+            pl.component = entry['component']
+            pl.damping = float(entry['coefficients'][0])
+            pl.restoring = 0.0 if drude else float(entry['coefficients'][1])
+            pl.inv_dt = float(np.float32(1.0) / np.float32(model.dt))
+            pl.drive = entry['drive'].data_ptr()
+            for q, state in enumerate(entry['states']):
+                pl.state[q] = state.data_ptr()
         self.probes.fill(d.probes)
         d.clock = self.clock.data_ptr()
         d.step_limit = _lib.INT64_MAX

@@ -214,7 +214,11 @@ class _Built:
                 dispersive.append((index, poles))
         for index, poles in dispersive:   # definition_common.py:1009-1041: each pole on its material's cells
             for pole in poles:
-                self.sim.add_dispersive(pole, self._owner == index)
+                if self.three_d:
+                    window = tuple(slice(0, n) for n in self.sim.model.shape)
+                    self.sim.add_dispersive(pole, window, self._owner == index)
+                else:
+                    self.sim.add_dispersive(pole, self._owner == index)
         self.features = []
         for feature in physics.get('features') or []:
             if feature.get('enabled', True):
@@ -235,7 +239,7 @@ class _Built:
         return self._shape(selection[0], kinds)
 
     def _paint(self, material, value, index):
-        """Paint one material in list order; returns its dispersive poles (2D: DrudePole / LorentzPole from the
+        """Paint one material in list order; returns its dispersive poles (DrudePole / LorentzPole from the
         'dispersion' list, definition_common.py:1079-1099), whose background permittivity is the eps_r property,
         else the first Drude eps_inf, else 1 (:1102-1111)."""
         from .dispersion import DrudePole, LorentzPole
@@ -245,8 +249,6 @@ class _Built:
             raise ValueError(f"material {material.get('name')!r}: only eps_r, sigma_e and dispersion are part of "
                              f"the FDTD path")
         dispersion = material.get('dispersion') or []
-        if dispersion and self.three_d:
-            raise ValueError(f"material {material.get('name')!r}: dispersive poles are built for the 2D types only")
         poles = []
         for pole in dispersion:
             model = pole.get('model') if isinstance(pole, dict) else None
@@ -276,7 +278,10 @@ class _Built:
                 self._owner[(x[:, None] - np.float32(p['cx'])) ** 2 + (y[None, :] - np.float32(p['cy'])) ** 2
                             <= np.float32(p['r']) ** 2] = index
             else:
-                self.sim.add_dielectric_box((p['x0'], p['y0'], p['z0']), (p['x1'], p['y1'], p['z1']), eps, sigma)
+                lo, hi = (p['x0'], p['y0'], p['z0']), (p['x1'], p['y1'], p['z1'])
+                self.sim.add_dielectric_box(lo, hi, eps, sigma)
+                ranges = self.sim.domain.box_ranges(lo, hi, clamp_to_physical=True)
+                self._owner[tuple(slice(a, b) for a, b in ranges)] = index
         return poles
 
     def _apply(self, feature, value):

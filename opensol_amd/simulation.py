@@ -266,6 +266,12 @@ class Simulation3D:
         self._require_unbuilt('add_field_probe')
         return self.model.add_field_probe(x, y, z, component)
 
+    def add_dispersive(self, pole, window, mask):
+        """Drude or Lorentz pole on a bool mask of the cells of `window` (one slice per axis), over their own
+        permittivity (physics/em3d/physics3d.py:346-349)."""
+        self._require_unbuilt('add_dispersive')
+        self.model.add_dispersive(pole, window, mask)
+
     def add_flux_box(self, lo, hi, wavelengths=None, frequencies=None):
         """Net outward flux through the box between the cell edges nearest the corners lo, hi (physics3d.py:447-451)."""
         self._require_unbuilt('add_flux_box')

@@ -307,6 +307,7 @@ class Yee3D:
                           'mu_r': np.ones(self.shape, F32)}
         self.sources, self.monitors, self.probes = [], [], []
         self.pec_boxes = []
+        self.poles = []
         self.n_step, self.time = 0, 0.0
         self._finalized = False
         self._build_spacings_and_cpml()
@@ -471,6 +472,36 @@ class Yee3D:
         self._finalized = False
         return FluxBox(faces)
 
+    def add_dispersive_drude(self, mask, omega_p, gamma):
+        """Drude pole on a bool (nx, ny, nz) cell mask over the cells' permittivity (physics3d.py:346-349)."""
+        self.poles.append({'pole': ('drude', omega_p, gamma), 'mask': np.asarray(mask, dtype=bool).copy()})
+        self._finalized = False
+
+    def add_dispersive_lorentz(self, mask, delta_eps, omega_0, gamma):
+        self.poles.append({'pole': ('lorentz', delta_eps, omega_0, gamma),
+                           'mask': np.asarray(mask, dtype=bool).copy()})
+        self._finalized = False
+
+    def _pole_setup(self):
+        """Per pole and E component: the face average of the mask along the component's axis (engine3d.py:461-488,
+        uniform cells) times the float32 drive coefficient, and zero states."""
+        from .yee2d import drude_coefficients, lorentz_coefficients
+        real = self.dtype
+        for entry in self.poles:
+            kind = entry['pole'][0]
+            coefficients = (drude_coefficients(*entry['pole'][1:], self.dt) if kind == 'drude'
+                            else lorentz_coefficients(*entry['pole'][1:], self.dt))
+            entry['coefficients'] = tuple(real(c) for c in coefficients)
+            m = entry['mask'].astype(real)
+            entry['drive'] = []
+            for axis, n in enumerate(self.shape):
+                partner = (np.arange(n) + 1) % n if self.periodic[axis] else np.minimum(np.arange(n) + 1, n - 1)
+                weight = real(0.5) * m + real(0.5) * np.take(m, partner, axis=axis)
+                entry['drive'].append(weight * real(coefficients[-1]))
+            if 'state' not in entry:
+                entry['state'] = [[np.zeros(self.shape, dtype=real) for _ in range(1 if kind == 'drude' else 2)]
+                                  for _ in range(3)]
+
     def add_dft_plane(self, z, wavelengths=None, frequencies=None):
         """DFT of E at the cell centres of the window layer containing z (physics3d.py:478-486)."""
         if (wavelengths is None) == (frequencies is None):
@@ -533,6 +564,7 @@ class Yee3D:
         for source in self.sources:
             axis = AXES.index(source.component[1])
             source.weight = float(self.cb[axis][source.cell]) * source.moment / source.volume
+        self._pole_setup()
         spectral = [m.frequencies for m in self.monitors]
         interval = dft_sample_interval(self.dt, self.sources, spectral)
         for monitor in self.monitors:
@@ -575,6 +607,16 @@ class Yee3D:
                 continue
             for which, (target, difference, sign) in enumerate(h_rows[layer.axis]):
                 layer.add(target, db, sign, layer.step(which, difference))
+        # pole J / P from E^n (engine3d.py:508-522)
+        for entry in self.poles:
+            for c, field in enumerate((ex, ey, ez)):
+                state, drive = entry['state'][c], entry['drive'][c]
+                if entry['pole'][0] == 'drude':
+                    state[0] += entry['coefficients'][0] * state[0] + drive * field
+                else:
+                    damping, restoring, _ = entry['coefficients']
+                    state[1] += damping * state[1] - restoring * state[0] + drive * field
+                    state[0] += state[1]
         # E (kernels3d.py:538-610)
         dhz_y, dhy_z = self._lo_difference(hz, 1), self._lo_difference(hy, 2)
         dhx_z, dhz_x = self._lo_difference(hx, 2), self._lo_difference(hz, 0)
@@ -590,6 +632,15 @@ class Yee3D:
                 continue
             for which, (target, coefficient, difference, sign) in enumerate(e_rows[layer.axis]):
                 layer.add(target, coefficient, sign, layer.step(which, difference))
+        # pole corrections (engine3d.py:524-530): Drude E -= cb J, Lorentz E -= cb / dt dP, before the sources
+        inv_dt = self.dtype(1.0) / self.dtype(self.dt)
+        for entry in self.poles:
+            for c, (field, cb) in enumerate(zip((ex, ey, ez), (cbx, cby, cbz))):
+                state = entry['state'][c]
+                if entry['pole'][0] == 'drude':
+                    field -= cb * state[0]
+                else:
+                    field -= cb * inv_dt * state[1]
         t = self.time
         for source in self.sources:
             source.inject(F, t + 0.5 * float(self.dt), self.dtype)

@@ -127,3 +127,43 @@ def build_poles2d(target, polarization='ez', **options):
     sim.add_line_source(**spec['source'])
     monitor = sim.add_flux_monitor(**spec['monitor'])
     return sim, monitor
+
+
+# 3D case with dispersive poles (tests/golden/make_poles3d_golden.py writes it from the reference): eps_r 2 on a box
+# with a Drude pole, a Lorentz pole on a second box, a z dipole and a flux plane normal to x; 8-cell CPML.
+POLES3D = dict(window=(0.6e-6, 0.5e-6, 0.45e-6), dx=25e-9, npml=8,
+               drude_box=((0.05e-6, 0.05e-6, 0.05e-6), (0.25e-6, 0.30e-6, 0.20e-6)), drude=(3e15, 1e14), drude_eps=2.0,
+               lorentz_box=((0.35e-6, 0.20e-6, 0.25e-6), (0.55e-6, 0.45e-6, 0.40e-6)), lorentz=(1.5, 3e15, 2e14),
+               dipole=(0.30e-6, 0.24e-6, 0.22e-6), plane_x=0.45e-6, wavelengths=[0.5e-6, 0.6e-6, 0.7e-6],
+               n_steps=250)
+
+
+def poles3d_box_mask(centres, lo, hi):
+    """Cells whose float32 centre (clamped to the window) lies in the half-open box."""
+    axis_masks = [(np.asarray(c, np.float32) >= np.float32(a)) & (np.asarray(c, np.float32) < np.float32(b))
+                  for c, a, b in zip(centres, lo, hi)]
+    return axis_masks[0][:, None, None] & axis_masks[1][None, :, None] & axis_masks[2][None, None, :]
+
+
+def build_poles3d(target, **options):
+    """The 3D pole case on the oracle Yee3D or on opensol_amd.Simulation3D; returns (sim, plane)."""
+    spec = POLES3D
+    sim = target(*spec['window'], spec['dx'], npml=spec['npml'], **options)
+    if hasattr(sim, 'host_centres'):   # the oracle
+        centres = sim.host_centres()
+        drude = poles3d_box_mask(centres, *spec['drude_box'])
+        lorentz = poles3d_box_mask(centres, *spec['lorentz_box'])
+        sim.materials['eps_r'][drude] = np.float32(spec['drude_eps'])
+        sim.add_dispersive_drude(drude, *spec['drude'])
+        sim.add_dispersive_lorentz(lorentz, *spec['lorentz'])
+    else:
+        from opensol_amd import DrudePole, LorentzPole
+        sim.add_dielectric_box(*spec['drude_box'], spec['drude_eps'])
+        centres = sim.model.domain.host_centres(True)
+        shape = sim.model.shape
+        window = tuple(slice(0, n) for n in shape)
+        sim.add_dispersive(DrudePole(*spec['drude']), window, poles3d_box_mask(centres, *spec['drude_box']))
+        sim.add_dispersive(LorentzPole(*spec['lorentz']), window, poles3d_box_mask(centres, *spec['lorentz_box']))
+    sim.add_dipole(*spec['dipole'], orientation='z', dipole_moment=1e-12, wavelength=0.6e-6, bandwidth=0.5)
+    plane = sim.add_flux_plane('x', spec['plane_x'], wavelengths=spec['wavelengths'])
+    return sim, plane

@@ -171,3 +171,17 @@ def test_dispersive_material_becomes_a_pole_on_its_cells():
     other['materials'][0]['dispersion'][0]['tau'] = 1.0
     with pytest.raises(ValueError):
         M._Built(other, names)
+
+
+def test_3d_dispersive_material_becomes_a_pole_on_its_box():
+    """em_fdtd_3d: a material's Drude dispersion becomes a pole on exactly the cells its box painted, with eps_inf
+    as their permittivity."""
+    golden = np.load(os.path.join(FIXTURES, 'model_dipole_3d.npz'))
+    spec = json.loads(str(golden['model_json']))
+    spec['materials'][0]['properties'] = {}
+    spec['materials'][0]['dispersion'] = [{'model': 'drude', 'omega_p': '2e15', 'gamma': '1e14', 'eps_inf': 2.25}]
+    model = M._Built(spec, M.evaluate_parameters(spec['parameters'])).sim.model
+    (pole, mask), = model.poles
+    painted = model.materials.property_array(0) == np.float32(2.25)
+    np.testing.assert_array_equal(mask, painted)
+    assert mask.sum() > 0 and float(pole.omega_p) == np.float32(2e15)
