@@ -26,8 +26,8 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 9
-#define YEE_MAX_SOURCES 4
+#define YEE_ABI_VERSION 10
+#define YEE_MAX_SOURCES 8
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
 #define YEE_MAX_POLES 8
@@ -48,11 +48,19 @@ extern "C" {
  * Replaces core/sources.py:221-229 (LineSource.inject -> _add_scaled_profile :146-155) and, for H_z,
  * physics/em/sources.py:469-480 (TangentialSheetCurrentSource.inject_electric on E_x / E_y).
  * axis 0: line along x at y-cell `line` (E_z, or E_x for H_z); axis 1: line along y at x-cell `line` (E_z, or E_y).
+ * A point current is a one-cell line: the E_z filament current (core/sources.py:162-188 PointSource.inject,
+ * physics/em/__init__.py:411-418), one face of an H_z in-plane electric dipole (physics/em/sources.py:447-466,
+ * :431-440) or, with field YEE_SOURCE_H, the H_z magnetic line current (physics/em/sources.py:439-444, :420-429),
+ * which adds wave[n] * profile to H_z after the H update of step n (wave sampled at t_n, not t_n + dt/2).
  */
+#define YEE_SOURCE_E 0   /* the E component named by the polarisation and axis, after the E update */
+#define YEE_SOURCE_H 1   /* H_z (H_z instances only), after the H update */
 typedef struct yee_line_source {
     int32_t axis;
     int32_t line;
     int32_t lo, hi;          /* half-open cell span along the line */
+    int32_t field;           /* YEE_SOURCE_E or YEE_SOURCE_H */
+    int32_t pad_;
     const float* profile;    /* (hi - lo) calibrated float32 profile */
     const float* wave;       /* float32 amplitude * s(t_n + dt/2) per absolute step n */
     int64_t wave_len;        /* entries in wave; steps n >= wave_len inject nothing */

@@ -6,8 +6,9 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 9
-MAX_SOURCES = 4
+ABI_VERSION = 10
+MAX_SOURCES = 8
+SOURCE_E, SOURCE_H = 0, 1   # yee_line_source.field
 MAX_MONITORS = 4
 MAX_DFT = 4
 MAX_POLES = 8
@@ -22,7 +23,7 @@ c_i32, c_i64, c_f32, c_ptr = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, cty
 
 
 class LineSourceC(ctypes.Structure):
-    _fields_ = [('axis', c_i32), ('line', c_i32), ('lo', c_i32), ('hi', c_i32),
+    _fields_ = [('axis', c_i32), ('line', c_i32), ('lo', c_i32), ('hi', c_i32), ('field', c_i32), ('pad_', c_i32),
                 ('profile', c_ptr), ('wave', c_ptr), ('wave_len', c_i64)]
 
 

@@ -656,11 +656,12 @@ struct HzTile {
         for (int q = 0; q < VEC; ++q) {
             const int j = col.jb + q;
             const bool layer_y = col.ry_e[q] >= 0;
-            const float v = hz_centre(tab_a<LDS>(d, id_at(r.ids, q)), r.hz[q],
-                                      r.ex[q] - (q == 0 ? ex_left : r.ex[q - 1]), r.ey[q] - ey_below[q],
-                                      col.idy_e[q], idx_e, xe.r >= 0, xe.b, xe.g, p.x[q], &p.x[q], layer_y,
-                                      layer_y ? cy.b_e[col.ry_e[q]] : 0.f, layer_y ? cy.g_e[col.ry_e[q]] : 0.f,
-                                      p.y[q], &p.y[q]);
+            float v = hz_centre(tab_a<LDS>(d, id_at(r.ids, q)), r.hz[q],
+                                r.ex[q] - (q == 0 ? ex_left : r.ex[q - 1]), r.ey[q] - ey_below[q],
+                                col.idy_e[q], idx_e, xe.r >= 0, xe.b, xe.g, p.x[q], &p.x[q], layer_y,
+                                layer_y ? cy.b_e[col.ry_e[q]] : 0.f, layer_y ? cy.g_e[col.ry_e[q]] : 0.f,
+                                p.y[q], &p.y[q]);
+            if (d.n_sources > 0) v = magnetic_currents(k, j, v);
             out[q] = j < ny ? v : 0.f;
         }
         if (store && col.any) {
@@ -680,7 +681,22 @@ struct HzTile {
                                idy_e_r, idx_e, xe.r >= 0, xe.b, xe.g, p.x_r, &unused_x, ry_e_r >= 0,
                                ry_e_r >= 0 ? cy.b_e[ry_e_r] : 0.f, ry_e_r >= 0 ? cy.g_e[ry_e_r] : 0.f, p.y_r,
                                &unused_y);
+            if (d.n_sources > 0) *out_r = magnetic_currents(k, jr, *out_r);
         }
+    }
+
+    // magnetic line currents on H_z cell (k, j) after the H update, at t_n: H_z += float32(value * profile). Every
+    // copy of the cell (owner row, lookahead row, right halo column) adds the same term.
+    __device__ __forceinline__ float magnetic_currents(int k, int j, float v) const
+    {
+        for (int s = 0; s < d.n_sources; ++s) {
+            const yee_line_source& src = d.source[s];
+            if (src.field != YEE_SOURCE_H) continue;
+            const bool on_line = src.axis == 1 ? (k == src.line && j >= src.lo && j < src.hi)
+                                               : (j == src.line && k >= src.lo && k < src.hi);
+            if (on_line) v += __fmul_rn(wave_value(src, n), src.profile[src.axis == 1 ? j - src.lo : k - src.lo]);
+        }
+        return v;
     }
 
     // one row i: psi reads of H_z row i+1 and of the faces of row i, the prefetch of row i+2, then H_z of row i+1
@@ -762,6 +778,7 @@ struct HzTile {
         // sheet current on the hi faces at t + dt/2 (E_y for a line along y, E_x for a line along x)
         for (int s = 0; s < d.n_sources; ++s) {
             const yee_line_source& src = d.source[s];
+            if (src.field != YEE_SOURCE_E) continue;
             const float w = wave_value(src, n);
 #pragma unroll
             for (int q = 0; q < VEC; ++q) {

@@ -190,6 +190,8 @@ int32_t check_2d(const yee2d_desc* d)
         if (src.line < 0 || src.line >= n_line) return fail(YEE_EINVAL, "source line outside the grid");
         if (src.lo < 0 || src.hi > n_along || src.lo >= src.hi) return fail(YEE_EINVAL, "source span out of range");
         if (!src.profile || !src.wave) return fail(YEE_EINVAL, "source profile / wave missing");
+        if (src.field != YEE_SOURCE_E && !(src.field == YEE_SOURCE_H && d->polarization == YEE_HZ))
+            return fail(YEE_EINVAL, "source field must be YEE_SOURCE_E (or YEE_SOURCE_H on an H_z instance)");
     }
     if (d->n_monitors < 0 || d->n_monitors > YEE_MAX_MONITORS) return fail(YEE_EINVAL, "n_monitors out of range");
     if (d->n_monitors > 0 && d->sample_interval < 1) return fail(YEE_EINVAL, "sample_interval must be >= 1");

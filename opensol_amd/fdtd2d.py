@@ -17,7 +17,8 @@ from .domain import Domain
 from .materials import MaterialMap, semi_implicit
 from .dispersion import DrudePole, LorentzPole, bounding_box, pole_step_term
 from .monitors import DFTMonitor, FieldProbe, FluxMonitor, ProbeTable, dft_sample_interval
-from .sources import LineSource, complex_permittivity, resolve_frequencies, resolve_frequency
+from .sources import (LineSource, PointCurrent, complex_permittivity, electric_dipole_faces, resolve_frequencies,
+                      resolve_frequency)
 from .waveforms import make_temporal
 
 POLARIZATIONS = ('ez', 'hz')
@@ -302,6 +303,58 @@ class Model2D:
         self.sources.append(source)
         return source
 
+    def _point_cell(self, x, y, label):
+        """(ix, iy, cell area from the float32 grid width) of the cell containing (x, y) (placement.py:41-52)."""
+        self.domain.validate_physical_pos((x, y), label)
+        ix, iy = self.domain.pos_to_cell(x, y)
+        width = float(self.grid.dx)
+        return ix, iy, width * width
+
+    def _require(self, polarization, name):
+        if self.polarization != polarization:
+            raise ValueError(f'{name} needs polarization {polarization!r}, this model is {self.polarization!r}')
+
+    def add_point_source(self, x, y, wavelength=None, frequency=None, bandwidth=0.5, current=1e-3,
+                         waveform='gaussian'):
+        """z filament current `current` [A] at the cell containing (x, y) (E_z only): E_z += cb * I / (dx dy) *
+        s(t + dt/2) after the E update (physics/em/__init__.py:411-418)."""
+        self._require('ez', 'add_point_source')
+        temporal = make_temporal(waveform, resolve_frequency(wavelength, frequency), bandwidth)
+        ix, iy, area = self._point_cell(x, y, 'Point source')
+        source = PointCurrent('x', iy, ix, temporal, float(current) / area, 'ez')
+        self.sources.append(source)
+        return source
+
+    def add_magnetic_line_current(self, x, y, wavelength=None, frequency=None, bandwidth=0.5, magnetic_current=1.0,
+                                  waveform='gaussian'):
+        """z magnetic line current K_m = `magnetic_current` [V] at the cell containing (x, y) (H_z only):
+        H_z += db * K_m / (dx dy) * s(t) after the H update (physics/em/__init__.py:420-429)."""
+        self._require('hz', 'add_magnetic_line_current')
+        temporal = make_temporal(waveform, resolve_frequency(wavelength, frequency), bandwidth)
+        ix, iy, area = self._point_cell(x, y, 'Magnetic line current')
+        source = PointCurrent('x', iy, ix, temporal, float(magnetic_current) / area, 'hz', field='h')
+        self.sources.append(source)
+        return source
+
+    def add_electric_dipole(self, x, y, wavelength=None, frequency=None, bandwidth=0.5, dipole_moment=1e-3,
+                            orientation=0.0, waveform='gaussian'):
+        """In-plane electric dipole M = `dipole_moment` [A] along `orientation` [deg, ccw from +x] in the cell
+        containing (x, y) (H_z only), shared over the E_x / E_y faces around the cell; one source record per face
+        (physics/em/__init__.py:431-440). Returns the list of face sources."""
+        self._require('hz', 'add_electric_dipole')
+        temporal = make_temporal(waveform, resolve_frequency(wavelength, frequency), bandwidth)
+        ix, iy, _ = self._point_cell(x, y, 'Electric dipole')
+        faces = electric_dipole_faces(ix, iy, self.grid.shape, (self.bc.periodic('x'), self.bc.periodic('y')),
+                                      float(self.grid.dx), dipole_moment, orientation)
+        added = []
+        for component, i, j, density in faces:
+            if component == 'ex':   # E_x face (i, j): hi y face of cell (i, j), a line along x at y-cell j
+                added.append(PointCurrent('x', j, i, temporal, density, 'ex'))
+            else:                   # E_y face (i, j): hi x face of cell (i, j), a line along y at x-cell i
+                added.append(PointCurrent('y', i, j, temporal, density, 'ey'))
+        self.sources.extend(added)
+        return added
+
     def add_flux_monitor(self, axis, position, span, wavelengths=None, frequencies=None):
         """Poynting flux through a line (axis 'x' = along x at y = position, flux > 0 toward +y); span = (start, end)
         along it [m] (placement.py:55-70, physics/em/__init__.py:511-518)."""
@@ -348,7 +401,7 @@ class Model2D:
         """Courant check, sheet calibration and the DFT sample interval shared by every flux and DFT monitor
         (physics/em/__init__.py:181-203)."""
         if len(self.sources) > _lib.MAX_SOURCES:
-            raise ValueError(f'at most {_lib.MAX_SOURCES} line sources per 2D model')
+            raise ValueError(f'at most {_lib.MAX_SOURCES} source records per 2D model (line sources, point currents, dipole faces)')
         if len(self.monitors) > _lib.MAX_MONITORS:
             raise ValueError(f'at most {_lib.MAX_MONITORS} flux monitors per 2D model')
         if len(self.dft_monitors) > _lib.MAX_DFT:
@@ -369,6 +422,12 @@ class Model2D:
         values[pec] = (1.0, 1.0, 1.0, 0.0, 1.0, 0.0)   # calibrated as vacuum, then zeroed by the keep mask
         dt = float(self.dt)
         for source in self.sources:
+            if source.field == 'h':   # H_z magnetic line current: db of the cell, from the cell's own material
+                ids = source.line(self.materials.ids).astype(np.int64)
+                raw = self.materials.values()
+                _, db = semi_implicit(raw[ids, 4], raw[ids, 5], dt, MU0)
+                source.calibrate(db)
+                continue
             omega = 2.0 * np.pi * float(source.temporal.f0)
             ids = source.line(self.materials.ids).astype(np.int64)
             mu, sigma_m = values[ids, 4], values[ids, 5]
@@ -508,7 +567,8 @@ class Engine2D:
         capacity = max(int(steps), 2 * self.capacity, 1024)
         times = clock_times(model.dt, capacity)
         half = times[:-1] + 0.5 * float(model.dt)
-        self.waves = [torch.from_numpy(s.wave_table(half)).to(self.device) for s in model.sources]
+        self.waves = [torch.from_numpy(s.wave_table(times[:-1] if s.field == 'h' else half)).to(self.device)
+                      for s in model.sources]
         m = model.sample_interval
         n_samples = capacity // m
         sample_times = times[m * np.arange(1, n_samples + 1)]
@@ -564,6 +624,7 @@ class Engine2D:
             src.axis = 0 if source.axis == 'x' else 1
             src.line = source.position
             src.lo, src.hi = source.span
+            src.field = _lib.SOURCE_H if source.field == 'h' else _lib.SOURCE_E
             src.profile = self.profiles[k].data_ptr()
             src.wave = self.waves[k].data_ptr()
             src.wave_len = self.waves[k].numel()

@@ -11,8 +11,9 @@ physics/em3d/definition.py, physics/em/definition_common.py:1167-1287):
     pml_sigma_factor, pml_kappa_max, pml_alpha_max, courant_factor (definition_common.py:139-190, 660-668);
   * geometry kinds rectangle, circle, line, point and box (model/geometry.py:111-262); materials with eps_r and
     sigma_e painted on rectangles, circles (2D) or boxes (3D), in the order the materials are listed;
-  * 2D features line_source, flux_monitor, field_probe, dft_monitor and pec_domain; 3D features dipole_source,
-    field_probe, flux_plane, flux_box, dft_plane and pec_domain;
+  * 2D features line_source, point_source (E_z filament current; H_z magnetic line current or in-plane dipole),
+    flux_monitor, field_probe, dft_monitor and pec_domain; 3D features dipole_source, field_probe, flux_plane,
+    flux_box, dft_plane and pec_domain;
   * the time_dependent study (duration; the runner stops at t >= t_end - dt/2) and its sweep over one parameter;
   * results: flux_spectrum (with power_per_cw where every source shares one waveform, physics/em/results.py:
     93-122), timeseries and dft_fields per feature, the last run's fields on the window, results.npz in the
@@ -318,6 +319,15 @@ class _Built:
                     return sim.add_line_source(axis, position, amplitude=get('amplitude', 1.0), span=span,
                                                **source_time)
                 return sim.add_flux_monitor(axis, position, tuple(sorted(span)), wavelengths=_wavelengths(params, get))
+            if kind == 'point_source':   # definition_common.py:1199-1217
+                _, p = self._single(feature, ('point',))
+                if self.kind.endswith('ez'):
+                    return sim.add_point_source(p['x'], p['y'], current=get('current', 1e-3), **source_time)
+                if text('excitation', 'magnetic_current') == 'magnetic_current':
+                    return sim.add_magnetic_line_current(p['x'], p['y'], magnetic_current=get('magnetic_current', 1.0),
+                                                         **source_time)
+                return sim.add_electric_dipole(p['x'], p['y'], dipole_moment=get('dipole_moment', 1e-3),
+                                               orientation=get('orientation', 0.0), **source_time)
             if kind == 'dft_monitor':
                 _, p = self._single(feature, ('rectangle',))
                 return sim.add_dft_monitor(p['x0'], p['y0'], p['x1'], p['y1'], wavelengths=_wavelengths(params, get),

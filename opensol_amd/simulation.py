@@ -103,6 +103,27 @@ class Simulation:
         self._require_unbuilt('add_line_source')
         return self.model.add_line_source(axis, position, wavelength, frequency, bandwidth, amplitude, waveform, span)
 
+    def add_point_source(self, x, y, wavelength=None, frequency=None, bandwidth=0.5, current=1e-3,
+                         waveform='gaussian'):
+        """z filament current [A] at the cell containing (x, y), E_z only (physics/em/__init__.py:411-418)."""
+        self._require_unbuilt('add_point_source')
+        return self.model.add_point_source(x, y, wavelength, frequency, bandwidth, current, waveform)
+
+    def add_magnetic_line_current(self, x, y, wavelength=None, frequency=None, bandwidth=0.5, magnetic_current=1.0,
+                                  waveform='gaussian'):
+        """z magnetic line current [V] at the cell containing (x, y), H_z only (physics/em/__init__.py:420-429)."""
+        self._require_unbuilt('add_magnetic_line_current')
+        return self.model.add_magnetic_line_current(x, y, wavelength, frequency, bandwidth, magnetic_current,
+                                                    waveform)
+
+    def add_electric_dipole(self, x, y, wavelength=None, frequency=None, bandwidth=0.5, dipole_moment=1e-3,
+                            orientation=0.0, waveform='gaussian'):
+        """In-plane electric dipole [A] along `orientation` [deg] in the cell containing (x, y), H_z only
+        (physics/em/__init__.py:431-440)."""
+        self._require_unbuilt('add_electric_dipole')
+        return self.model.add_electric_dipole(x, y, wavelength, frequency, bandwidth, dipole_moment, orientation,
+                                              waveform)
+
     def add_gaussian_beam(self, axis, position, waist, center, wavelength=None, frequency=None, bandwidth=0.5,
                           amplitude=1.0, waveform='gaussian', span=None):
         """Line source with transverse factor exp(-((s - center)/waist)^2) (physics/em/__init__.py:450-457)."""

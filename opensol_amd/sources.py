@@ -49,6 +49,8 @@ class LineSource:
     """Sheet current on one grid line: axis 'x' = line along x at y-cell `position` (axis 'y': along y at x-cell).
     component: 'ez' (E_z sheet through the line-cell centres) or 'ex' / 'ey' (H_z: hi faces of the line cells)."""
 
+    field = 'e'   # injected after the E update
+
     def __init__(self, axis, position, temporal, amplitude, span, normal_width, component, transverse_shape=None):
         if axis not in ('x', 'y'):
             raise ValueError(f"LineSource axis must be 'x' or 'y', got {axis!r}")
@@ -87,6 +89,55 @@ class LineSource:
         omega = 2.0 * np.pi * float(self.temporal.f0)
         admittance = sheet_admittance(line_eps, mu_line, sigma_m_line, omega, self.normal_width, dt)
         self.profile[...] = np.asarray(cb_line, dtype=np.float64) * admittance * self.weight
+
+
+class PointCurrent(LineSource):
+    """Current at one cell or face of a 2D model, kept as a one-cell line (axis, line cell, span (cell, cell + 1)):
+    profile = float32(coefficient * float32(current density)), the coefficient being cb of the E_z cell (filament
+    current I / cell area, physics/em/sources.py:544-561 PointCurrentCalibration), cb of the E_x / E_y face (one face
+    of an in-plane electric dipole, :605-616 DipoleCalibration) or db of the H_z cell (magnetic line current
+    K_m / cell area). field 'e' adds profile * float32(s(t_n + dt/2)) after the E update, field 'h' adds
+    profile * float32(s(t_n)) to H_z after the H update (physics/em/sources.py:439-444, core/sources.py:162-188)."""
+
+    def __init__(self, axis, line, cell, temporal, current_density, component, field='e'):
+        super().__init__(axis, line, temporal, 1.0, (cell, cell + 1), 1.0, component)
+        if field not in ('e', 'h'):
+            raise ValueError(f"point current field must be 'e' or 'h', got {field!r}")
+        self.current_density = DTYPE(current_density)
+        self.field = field
+
+    def calibrate(self, coefficient, *unused):
+        self.profile[...] = np.asarray(coefficient, dtype=np.float64) * np.float64(self.current_density)
+
+
+def dipole_faces(cell, n_cells, periodic):
+    """(face, moment fraction) of the hi face `cell` and the lo face `cell - 1` along one axis, half each; on a
+    non-periodic axis a face outside [0, n_cells - 2] (the outer face is never updated) passes its half to the other
+    (physics/em/placement.py:268-276)."""
+    if periodic:
+        return [(cell, 0.5), ((cell - 1) % n_cells, 0.5)]
+    faces = [face for face in (cell, cell - 1) if 0 <= face <= n_cells - 2]
+    if not faces:
+        raise ValueError(f'an electric dipole needs at least 2 cells along a non-periodic axis, got {n_cells}')
+    return [(face, 1.0 / len(faces)) for face in faces]
+
+
+def electric_dipole_faces(ix, iy, n_cells, periodic, width, dipole_moment, orientation):
+    """[('ex' | 'ey', i, j, J [A/m^2]), ...] of an in-plane dipole of moment M [A] along `orientation` [deg, ccw from
+    +x] in cell (ix, iy) of a uniform grid of cell width `width`: a face carrying the fraction f of M_axis has
+    J = f M_axis / (width * dual spacing); faces below 1e-12 of the largest are dropped (physics/em/placement.py:
+    279-299)."""
+    theta = math.radians(float(orientation))
+    moment = (float(dipole_moment) * math.cos(theta), float(dipole_moment) * math.sin(theta))
+    inv_dual = 1.0 / (0.5 * (width + width))
+    faces = [('ex', ix, j, f * moment[0] * inv_dual / width) for j, f in dipole_faces(iy, n_cells[1], periodic[1])]
+    faces += [('ey', i, iy, f * moment[1] * inv_dual / width) for i, f in dipole_faces(ix, n_cells[0], periodic[0])]
+    largest = max(abs(face[3]) for face in faces)
+    faces = [face for face in faces if abs(face[3]) > 1e-12 * largest]
+    if not faces:
+        raise ValueError(f'Electric dipole needs a finite non-zero dipole_moment and a finite orientation, got '
+                         f'dipole_moment={dipole_moment}, orientation={orientation}')
+    return faces
 
 
 def complex_permittivity(eps_rel, sigma_e, omega):

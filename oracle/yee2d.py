@@ -312,6 +312,42 @@ class LineSource:
         target += value * self.profile
 
 
+class CellCurrent:
+    """Soft current at one cell or face (core/sources.py:162-188 PointSource): weight = coefficient[i, j] times the
+    float32 current density in numpy's precision rules (physics/em/sources.py:544-561, 605-616), and each step
+    field[i, j] += weight * float32(s(t)). `field` 'e' sources are injected at t + dt/2 after the E update, 'h'
+    sources (the H_z magnetic line current, physics/em/sources.py:439-444) at t after the H update."""
+
+    def __init__(self, component, cell, temporal, current_density, coefficient, field):
+        self.component, self.cell, self.temporal = component, (int(cell[0]), int(cell[1])), temporal
+        self.current_density = F32(current_density)
+        self.coefficient, self.field = coefficient, field
+        self.weight = np.zeros(1, dtype=F32)
+
+    def calibrate(self, coefficients):
+        i, j = self.cell
+        np.multiply(coefficients[i:i + 1, j], self.current_density, out=self.weight, casting='same_kind')
+
+    def inject(self, fields, t):
+        value = F32(F32(1.0) * F32(self.temporal(t)))
+        if value == 0:
+            return
+        i, j = self.cell
+        fields[self.component][i:i + 1, j] += self.weight * value
+
+
+def dipole_face_shares(cell, n_cells, periodic):
+    """Faces carrying an in-plane dipole along one axis (physics/em/placement.py:268-276): the hi face `cell` and the
+    lo face `cell - 1`, half each; on a bounded axis only faces 0..n-2 exist and the survivors share the moment."""
+    candidates = [cell, cell - 1]
+    if periodic:
+        return [(candidates[0], 0.5), (candidates[1] % n_cells, 0.5)]
+    kept = [face for face in candidates if 0 <= face <= n_cells - 2]
+    if not kept:
+        raise ValueError('an electric dipole needs at least 2 cells along a bounded axis')
+    return [(face, 1.0 / len(kept)) for face in kept]
+
+
 class FluxMonitor:
     """Flux line with complex128 E and H spectra phased at the E sample time (em/monitors.py:203-245, 325-379)."""
 
@@ -607,6 +643,64 @@ class Yee2D:
         self._finalized = False
         return source
 
+    def _point(self, x, y, label):
+        self.geo.validate(x, y, label)
+        return self.geo.cell(0, x), self.geo.cell(1, y)
+
+    def add_point_source(self, x, y, wavelength=None, frequency=None, bandwidth=0.5, current=1e-3,
+                         waveform='gaussian'):
+        """E_z filament current I [A]: J = I / (dx dy) with the float32 kernel width, weight = cb J
+        (physics/em/__init__.py:411-418, placement.py:48-52)."""
+        if self.polarization != 'ez':
+            raise ValueError('add_point_source needs polarization ez')
+        temporal = make_temporal(waveform, resolve_frequency(wavelength, frequency), bandwidth)
+        cell = self._point(x, y, 'Point source')
+        w = float(self.geo.dx32)
+        source = CellCurrent('ez', cell, temporal, float(current) / (w * w), 'cb', 'e')
+        self.sources.append(source)
+        self._finalized = False
+        return source
+
+    def add_magnetic_line_current(self, x, y, wavelength=None, frequency=None, bandwidth=0.5, magnetic_current=1.0,
+                                  waveform='gaussian'):
+        """H_z magnetic line current K_m [V]: M = K_m / (dx dy), weight = db M, added to H_z after the H update at t
+        (physics/em/__init__.py:420-429)."""
+        if self.polarization != 'hz':
+            raise ValueError('add_magnetic_line_current needs polarization hz')
+        temporal = make_temporal(waveform, resolve_frequency(wavelength, frequency), bandwidth)
+        cell = self._point(x, y, 'Magnetic line current')
+        w = float(self.geo.dx32)
+        source = CellCurrent('hz', cell, temporal, float(magnetic_current) / (w * w), 'db', 'h')
+        self.sources.append(source)
+        self._finalized = False
+        return source
+
+    def add_electric_dipole(self, x, y, wavelength=None, frequency=None, bandwidth=0.5, dipole_moment=1e-3,
+                            orientation=0.0, waveform='gaussian'):
+        """H_z in-plane dipole M [A] along `orientation` [deg]: E_x faces (ix, j) carry f M cos / (dx_i dual dy_j),
+        E_y faces (i, iy) f M sin / (dual dx_i dy_j); weights cb_x / cb_y of the face (physics/em/__init__.py:431-440,
+        placement.py:279-299, sources.py:605-616)."""
+        if self.polarization != 'hz':
+            raise ValueError('add_electric_dipole needs polarization hz')
+        temporal = make_temporal(waveform, resolve_frequency(wavelength, frequency), bandwidth)
+        ix, iy = self._point(x, y, 'Electric dipole')
+        w = float(self.geo.dx32)
+        angle = math.radians(float(orientation))
+        mx, my = float(dipole_moment) * math.cos(angle), float(dipole_moment) * math.sin(angle)
+        faces = [('ex', (ix, j), share * mx * self._inv_bulk['y']['h'][j] / w)
+                 for j, share in dipole_face_shares(iy, self.ny, self.bc.periodic('y'))]
+        faces += [('ey', (i, iy), share * my * self._inv_bulk['x']['h'][i] / w)
+                  for i, share in dipole_face_shares(ix, self.nx, self.bc.periodic('x'))]
+        biggest = max(abs(density) for *_, density in faces)
+        faces = [face for face in faces if abs(face[2]) > 1e-12 * biggest]
+        if not faces:
+            raise ValueError('Electric dipole needs a finite non-zero moment and orientation')
+        added = [CellCurrent(name, cell, temporal, density, 'cb_x' if name == 'ex' else 'cb_y', 'e')
+                 for name, cell, density in faces]
+        self.sources.extend(added)
+        self._finalized = False
+        return added
+
     def add_field_probe(self, x, y, field_component=None):
         """Samples of one component at the cell containing (x, y); None = the out-of-plane field
         (physics/em/__init__.py:493-498, placement.py:16, 29-45)."""
@@ -723,6 +817,9 @@ class Yee2D:
         """profile = float32(cb_line admittance weight) (em/sources.py:383-392, 582-602)."""
         m = self.materials
         for source in self.sources:
+            if isinstance(source, CellCurrent):
+                source.calibrate(getattr(self, source.coefficient))
+                continue
             omega = 2.0 * np.pi * source.temporal.f0
             if self.polarization == 'ez':
                 cb_line = source.line(self.cb)
@@ -757,9 +854,10 @@ class Yee2D:
         if self.polarization == 'ez':
             self._step_ez()
         else:
-            self._step_hz()
+            self._step_hz(t_start)
         for source in self.sources:
-            source.inject(self.fields, t_half)
+            if getattr(source, 'field', 'e') == 'e':
+                source.inject(self.fields, t_half)
         self.n_step += 1
         self.time = t_full
         for probe in self.probes:
@@ -813,7 +911,7 @@ class Yee2D:
             if entry['pole'][0] == 'lorentz':
                 ez -= cb * inv_dt * entry['state'][1]
 
-    def _step_hz(self):
+    def _step_hz(self, t_start):
         f = self.fields
         hz, ex, ey = f['hz'], f['ex'], f['ey']
         px, py = self.bc.periodic('x'), self.bc.periodic('y')
@@ -828,6 +926,10 @@ class Yee2D:
         hz += curl
         for side in self.sides:   # apply_hz: x sides (ey, db, hz), y sides (ex, db, hz) (boundaries.py:766-768)
             side.apply('e', ey if side.axis == 'x' else ex, self.db, hz)
+        # magnetic line currents at t_start after the H update (fdtd_hz_2d.py:144-189, inject_magnetic)
+        for source in self.sources:
+            if getattr(source, 'field', 'e') == 'h':
+                source.inject(f, t_start)
         # pole updates on the E faces from E^n (fdtd_hz_2d.py:286-310)
         for entry in self.poles:
             for name, field in (('ex', ex), ('ey', ey)):

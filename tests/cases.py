@@ -47,6 +47,37 @@ def build_pec2d(target, polarization, boundary_type, **options):
     return sim, source, monitor
 
 
+# 2D cases driven by point currents (tests/golden/make_point2d_golden.py writes them from the reference): a dielectric
+# and a PEC rectangle; E_z filament currents (one in the dielectric, one in vacuum, one inside the PEC); H_z a magnetic
+# line current, an in-plane dipole at 30 degrees in y cell 0 (faces wrap) and one along y touching the PEC.
+POINT2D = dict(size_x=1.5e-6, size_y=1e-6, dx=25e-9, npml=10, faces=('cpml', 'cpml', 'periodic', 'periodic'),
+               dielectric=((0.2e-6, 0.0, 0.7e-6, 1e-6), 2.25), pec_rect=(1.0e-6, 0.4e-6, 1.2e-6, 0.6e-6),
+               wavelength=600e-9, bandwidth=0.6, steps=(120, 400),
+               ez_points=((0.4e-6, 0.5e-6, 2e-3), (0.85e-6, 0.3e-6, -1e-3), (1.1e-6, 0.5e-6, 1e-3)),
+               hz_magnetic=(0.45e-6, 0.55e-6, 2.0),
+               hz_dipoles=((0.8e-6, 0.01e-6, 1e-3, 30.0), (0.99e-6, 0.5e-6, 2e-3, 90.0)))
+
+
+def build_point2d(target, polarization, boundary_type, **options):
+    """The point-current case on `target`; returns (sim, sources) with one entry per reference source (a dipole's
+    entry is its list of face sources)."""
+    spec = POINT2D
+    sim = target(spec['size_x'], spec['size_y'], spec['dx'], npml=spec['npml'],
+                 boundaries=boundary_type(spec['faces'], spec['npml']), polarization=polarization, **options)
+    (x0, y0, x1, y1), eps = spec['dielectric']
+    sim.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
+    sim.add_pec_region(pec_rect_mask(sim, *spec['pec_rect']))
+    pulse = dict(wavelength=spec['wavelength'], bandwidth=spec['bandwidth'])
+    if polarization == 'ez':
+        sources = [sim.add_point_source(x, y, current=current, **pulse) for x, y, current in spec['ez_points']]
+    else:
+        x, y, km = spec['hz_magnetic']
+        sources = [sim.add_magnetic_line_current(x, y, magnetic_current=km, **pulse)]
+        sources += [sim.add_electric_dipole(x, y, dipole_moment=moment, orientation=angle, **pulse)
+                    for x, y, moment, angle in spec['hz_dipoles']]
+    return sim, sources
+
+
 def pec_rect_mask(sim, x0, y0, x1, y1):
     """Cells whose float64 centre lies in [x0, x1) x [y0, y1) (the reference's Domain.rect_mask, core/domain.py:
     255-260), on either target."""

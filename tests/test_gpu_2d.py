@@ -193,6 +193,32 @@ def test_pec_case_matches_oracle(gpu, polarization):
 
 
 @pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_point_currents_match_oracle(gpu, polarization):
+    """Point currents (E_z filament currents; H_z magnetic line current injected after the H update and in-plane
+    dipole faces, one wrapping at y cell 0, one touching PEC) against the FP64 oracle, which reproduces the
+    reference's own run to 1e-12: fields <= 1e-5 relative L2 at both steps; graph replay bit-identical to eager."""
+    from tests.cases import POINT2D, build_point2d
+    Simulation = simulation_class()
+    ref, _ = build_point2d(Yee2D, polarization, _oracle_boundaries)
+    sim, _ = build_point2d(Simulation, polarization, _product_boundaries)
+    graphed, _ = build_point2d(Simulation, polarization, _product_boundaries, graph_steps=16)
+    done = 0
+    for step in POINT2D['steps']:
+        print(f'point currents {polarization} to step {step}', flush=True)
+        ref.run(n_steps=step - done)
+        sim.run(n_steps=step - done, print_every=0)
+        graphed.run(n_steps=step - done, print_every=0)
+        done = step
+        for name in sim.model.components:
+            error = rel_l2(sim.engine.field(name), ref.fields[name])
+            print(f'{polarization} step {step} {name}: rel L2 {error:.3e}')
+            assert error <= FIELD_TOL, (step, name, error)
+            np.testing.assert_array_equal(graphed.engine.field(name), sim.engine.field(name))
+    if polarization == 'ez':
+        assert not np.any(sim.engine.field('ez')[ref.pec_mask()])
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
 def test_field_probes_match_oracle(gpu, polarization):
     """Field probes of the thin-film case: every sample within 1e-5 of the series' peak of the FP64 oracle's, times
     equal to the oracle's, the last sample equal to the engine's field at the probe's cell bit for bit, rows kept

@@ -160,6 +160,51 @@ def test_pec_cells_and_source_profile_match_the_oracle(polarization):
     assert np.all(source.profile[ref_source.profile == 0] == 0)
 
 
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_point_current_records_match_the_reference(polarization):
+    """Point currents: each record's cell or face, field and float32 weight equal the reference's own (the fixture of
+    its FP64 run), the H_z magnetic current's wave table samples s(t_n) and the others s(t_n + dt/2); the source
+    table is capped and the polarisation checked."""
+    import os
+    from opensol_amd.boundaries import BoundaryConfig
+    from opensol_amd.fdtd2d import clock_times
+    from tests.cases import build_point2d
+
+    def product_bc(faces, npml):
+        bc = BoundaryConfig(*faces)
+        bc.cpml_npml = npml
+        return bc
+
+    golden = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'fixtures',
+                                  f'point2d_{polarization}.npz'))
+    model, sources = build_point2d(Model2D, polarization, product_bc)
+    model.finalize()
+    for index, source in enumerate(sources):
+        records = source if isinstance(source, list) else [source]
+        weights = np.concatenate([record.profile for record in records]).astype(np.float64)
+        np.testing.assert_array_equal(weights, golden[f'source{index}_weights'])
+        if isinstance(source, list):
+            faces = [[0 if r.component == 'ex' else 1, r.span[0] if r.axis == 'x' else r.position,
+                      r.position if r.axis == 'x' else r.span[0]] for r in records]
+            assert faces == golden[f'source{index}_faces'].tolist()
+        else:
+            assert [source.span[0], source.position] == golden[f'source{index}_cell'].tolist()
+    times = clock_times(model.dt, 8)
+    for source in model.sources:
+        at = times[:-1] if source.field == 'h' else times[:-1] + 0.5 * float(model.dt)
+        expected = np.array([np.float32(source.temporal(t)) for t in at], dtype=np.float32)
+        table = source.wave_table(at)
+        np.testing.assert_array_equal(table, expected)
+    assert [s.field for s in model.sources] == (['e'] * 3 if polarization == 'ez' else ['h'] + ['e'] * 6)
+    wrong = 'add_magnetic_line_current' if polarization == 'ez' else 'add_point_source'
+    with pytest.raises(ValueError, match='needs polarization'):
+        getattr(model, wrong)(0.5e-6, 0.5e-6, wavelength=600e-9)
+    if polarization == 'hz':
+        model.add_electric_dipole(0.3e-6, 0.3e-6, wavelength=600e-9, orientation=45.0)   # 7 + 4 records > 8
+        with pytest.raises(ValueError, match='at most 8 source records'):
+            model.finalize()
+
+
 def test_pec_faces_are_e_z_only():
     from opensol_amd.boundaries import BoundaryConfig
     bc = BoundaryConfig('pec', 'cpml', 'periodic', 'periodic')

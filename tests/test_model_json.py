@@ -173,6 +173,30 @@ def test_dispersive_material_becomes_a_pole_on_its_cells():
         M._Built(other, names)
 
 
+def test_point_source_feature_places_the_point_currents():
+    """A 'point_source' feature on a point geometry (definition_common.py:1199-1217): E_z takes a filament current of
+    `current`; H_z a magnetic line current (the default excitation) or an in-plane dipole along `orientation`."""
+    golden, spec = golden_spec('ez')
+    spec['geometry'].append({'name': 'emitter', 'kind': 'point', 'params': {'x': '1.5*um', 'y': '1.2*um'}})
+    source = next(f for f in spec['physics'][0]['features'] if f['type'] == 'line_source')
+    source.update(type='point_source', selection=['emitter'])
+    source['params'] = {'waveform': 'gaussian', 'wavelength': '600*nm', 'bandwidth': 0.5, 'current': 2e-3}
+    names = M.evaluate_parameters(spec['parameters'])
+    model = M._Built(spec, names).sim.model
+    (point,) = model.sources
+    cell = model.domain.pos_to_cell(1.5e-6, 1.2e-6)
+    area = float(model.grid.dx) ** 2
+    assert (point.field, point.component, point.span[0], point.position) == ('e', 'ez', *cell)
+    assert point.current_density == np.float32(2e-3 / area)
+    spec['physics'][0]['type'] = 'em_fdtd_hz'
+    (magnetic,) = M._Built(spec, names).sim.model.sources
+    assert (magnetic.field, magnetic.component) == ('h', 'hz')
+    assert magnetic.current_density == np.float32(1.0 / area)   # magnetic_current defaults to 1 V
+    source['params'].update(excitation='electric_dipole', dipole_moment=1e-3, orientation=90.0)
+    faces = M._Built(spec, names).sim.model.sources
+    assert [f.component for f in faces] == ['ey', 'ey'] and all(f.field == 'e' for f in faces)
+
+
 def test_3d_dispersive_material_becomes_a_pole_on_its_box():
     """em_fdtd_3d: a material's Drude dispersion becomes a pole on exactly the cells its box painted, with eps_inf
     as their permittivity."""

@@ -9,6 +9,7 @@ from oracle.yee2d import Yee2D
 from tests.cases import SNAPSHOT_STEPS, THIN_FILM, add_thin_film_probes, build
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'fixtures')
 
 
 def run_oracle(polarization, with_slab):
@@ -106,6 +107,37 @@ def test_oracle_reproduces_reference_pec_case(polarization):
             assert values.shape == want.shape
             error = np.max(np.abs(values - want)) / np.max(np.abs(want))
             assert error <= 1e-6, (index, name, error)
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_oracle_reproduces_reference_point_currents(polarization):
+    """Point currents (E_z filament currents; H_z magnetic line current and in-plane dipoles, with the periodic face
+    wrap, the 1e-12 share cut and a PEC face): the FP64 oracle reproduces the reference's FP64 run (cells, faces and
+    float32 weights exactly; fields at two steps to 1e-12 of the peak)."""
+    from tests.cases import POINT2D, build_point2d
+    golden = np.load(os.path.join(FIXTURES, f'point2d_{polarization}.npz'))
+    sim, sources = build_point2d(Yee2D, polarization, _oracle_boundaries)
+    assert (sim.nx, sim.ny) == (int(golden['nx']), int(golden['ny']))
+    assert float(sim.dt) == float(golden['dt'])
+    done = 0
+    for step in POINT2D['steps']:
+        sim.run(n_steps=step - done)
+        done = step
+        for name, values in sim.fields.items():
+            want = golden[f'{name}_{step}']
+            assert np.abs(want).max() > 0
+            np.testing.assert_allclose(values, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    for index, source in enumerate(sources):
+        faces = source if isinstance(source, list) else [source]
+        want = golden[f'source{index}_weights']
+        np.testing.assert_array_equal(np.concatenate([face.weight for face in faces]).astype(np.float64), want)
+        if isinstance(source, list):
+            cells = [[0 if face.component == 'ex' else 1, *face.cell] for face in faces]
+            assert cells == golden[f'source{index}_faces'].tolist()
+        else:
+            assert list(source.cell) == golden[f'source{index}_cell'].tolist()
+    weights = [w for k, w in golden.items() if k.endswith('_weights')]
+    assert any(np.any(w == 0) for w in weights)   # a PEC cell or face injects nothing
 
 
 @pytest.mark.parametrize('polarization', ['ez', 'hz'])
