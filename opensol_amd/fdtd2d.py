@@ -303,6 +303,15 @@ class Model2D:
         self.sources.append(source)
         return source
 
+    def add_gaussian_beam(self, axis, position, waist, center, wavelength=None, frequency=None, bandwidth=0.5,
+                          amplitude=1.0, waveform='gaussian', span=None):
+        """Line source with the transverse factor exp(-((s - center) / waist)^2) over the float64 centres s of its
+        cells; `amplitude` is the peak field at the waist (physics/em/__init__.py:450-457, placement.py:246-249)."""
+        if not waist > 0:
+            raise ValueError(f'Gaussian beam waist must be > 0, got {waist}')
+        return self.add_line_source(axis, position, wavelength, frequency, bandwidth, amplitude, waveform, span,
+                                    waist=waist, center=center)
+
     def _point_cell(self, x, y, label):
         """(ix, iy, cell area from the float32 grid width) of the cell containing (x, y) (placement.py:41-52)."""
         self.domain.validate_physical_pos((x, y), label)

@@ -11,8 +11,8 @@ physics/em3d/definition.py, physics/em/definition_common.py:1167-1287):
     pml_sigma_factor, pml_kappa_max, pml_alpha_max, courant_factor (definition_common.py:139-190, 660-668);
   * geometry kinds rectangle, circle, line, point and box (model/geometry.py:111-262); materials with eps_r and
     sigma_e painted on rectangles, circles (2D) or boxes (3D), in the order the materials are listed;
-  * 2D features line_source, point_source (E_z filament current; H_z magnetic line current or in-plane dipole),
-    flux_monitor, field_probe, dft_monitor and pec_domain; 3D features dipole_source, field_probe, flux_plane,
+  * 2D features line_source, gaussian_beam, point_source (E_z filament current; H_z magnetic line current or
+    in-plane dipole), flux_monitor, field_probe, dft_monitor and pec_domain; 3D features dipole_source, field_probe, flux_plane,
     flux_box, dft_plane and pec_domain;
   * the time_dependent study (duration; the runner stops at t >= t_end - dt/2) and its sweep over one parameter;
   * results: flux_spectrum (with power_per_cw where every source shares one waveform, physics/em/results.py:
@@ -307,7 +307,7 @@ class _Built:
                 return sim.add_field_probe(p['x'], p['y'], p.get('z', 0.0), text('component', 'ez'))
             return sim.add_field_probe(p['x'], p['y'], text('component', self.kind[-2:]))
         if not self.three_d:
-            if kind in ('line_source', 'flux_monitor'):
+            if kind in ('line_source', 'gaussian_beam', 'flux_monitor'):
                 _, p = self._single(feature, ('line',))
                 if p['x0'] == p['x1']:
                     axis, position, span = 'y', 0.5 * (p['x0'] + p['x1']), (p['y0'], p['y1'])
@@ -318,6 +318,11 @@ class _Built:
                 if kind == 'line_source':
                     return sim.add_line_source(axis, position, amplitude=get('amplitude', 1.0), span=span,
                                                **source_time)
+                if kind == 'gaussian_beam':   # definition_common.py:1227-1232
+                    center = get('center')
+                    center = 0.5 * (span[0] + span[1]) if center is None else center
+                    return sim.add_gaussian_beam(axis, position, get('waist', 1e-6), center,
+                                                 amplitude=get('amplitude', 1.0), span=span, **source_time)
                 return sim.add_flux_monitor(axis, position, tuple(sorted(span)), wavelengths=_wavelengths(params, get))
             if kind == 'point_source':   # definition_common.py:1199-1217
                 _, p = self._single(feature, ('point',))

@@ -128,10 +128,8 @@ class Simulation:
                           amplitude=1.0, waveform='gaussian', span=None):
         """Line source with transverse factor exp(-((s - center)/waist)^2) (physics/em/__init__.py:450-457)."""
         self._require_unbuilt('add_gaussian_beam')
-        if not waist > 0:
-            raise ValueError(f'Gaussian beam waist must be > 0, got {waist}')
-        return self.model.add_line_source(axis, position, wavelength, frequency, bandwidth, amplitude, waveform, span,
-                                          waist=waist, center=center)
+        return self.model.add_gaussian_beam(axis, position, waist, center, wavelength, frequency, bandwidth, amplitude,
+                                            waveform, span)
 
     def add_field_probe(self, x, y, field_component=None):
         """Raw Yee samples of one component at the cell containing (x, y), one per step; get_data() -> (times,

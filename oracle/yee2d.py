@@ -287,13 +287,14 @@ def sheet_admittance(line_eps, mu_r, sigma_m, omega, normal_spacing, dt):
 class LineSource:
     """Calibrated sheet current (physics/em/__init__.py:442-473; core/sources.py:190-229)."""
 
-    def __init__(self, axis, line_cell, span_cells, temporal, amplitude, normal_width, component):
+    def __init__(self, axis, line_cell, span_cells, temporal, amplitude, normal_width, component, shape=None):
         self.axis, self.position, self.span = axis, int(line_cell), span_cells
         self.temporal, self.amplitude = temporal, F32(amplitude)
         self.normal_width = float(normal_width)
         self.component = component
         n_line = span_cells[1] - span_cells[0]
-        self.weight = np.full(n_line, 2.0 / (ETA0 * self.normal_width)).astype(F32)
+        shape = np.ones(n_line) if shape is None else shape   # Gaussian beam: exp(-((s - centre) / waist)^2)
+        self.weight = (2.0 * shape / (ETA0 * self.normal_width)).astype(F32)
         self.profile = np.zeros(n_line, dtype=F32)
 
     def line(self, array):
@@ -602,8 +603,17 @@ class Yee2D:
                     mask[:, index] = True
         return mask
 
+    def add_gaussian_beam(self, axis, position, waist, center, wavelength=None, frequency=None, bandwidth=0.5,
+                          amplitude=1.0, waveform='gaussian', span=None):
+        """Sheet current with the transverse factor exp(-((s - center) / waist)^2) over the float64 centres s of the
+        line cells (physics/em/__init__.py:450-457, placement.py:246-249)."""
+        if not waist > 0:
+            raise ValueError('Gaussian beam waist must be > 0')
+        return self.add_line_source(axis, position, wavelength, frequency, bandwidth, amplitude, waveform, span,
+                                    beam=(float(waist), float(center)))
+
     def add_line_source(self, axis, position, wavelength=None, frequency=None, bandwidth=0.5, amplitude=1.0,
-                        waveform='gaussian', span=None):
+                        waveform='gaussian', span=None, beam=None):
         """Sheet current of amplitude [V/m] per side (physics/em/__init__.py:442-473, placement.py:223-251)."""
         geo = self.geo
         temporal = make_temporal(waveform, resolve_frequency(wavelength, frequency), bandwidth)
@@ -638,7 +648,12 @@ class Yee2D:
                 raise ValueError('H_z sheet on the outer grid wall')
             normal_width = 1.0 / float(inv_face[line_cell])
             component = 'ex' if axis == 'x' else 'ey'
-        source = LineSource(axis, line_cell, span_cells, temporal, amplitude, normal_width, component)
+        shape = None
+        if beam is not None:
+            waist, centre = beam
+            along = geo.centres[along_axis][span_cells[0]:span_cells[1]]
+            shape = np.exp(-((along - centre) / waist) ** 2)
+        source = LineSource(axis, line_cell, span_cells, temporal, amplitude, normal_width, component, shape)
         self.sources.append(source)
         self._finalized = False
         return source

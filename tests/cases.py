@@ -55,12 +55,14 @@ POINT2D = dict(size_x=1.5e-6, size_y=1e-6, dx=25e-9, npml=10, faces=('cpml', 'cp
                wavelength=600e-9, bandwidth=0.6, steps=(120, 400),
                ez_points=((0.4e-6, 0.5e-6, 2e-3), (0.85e-6, 0.3e-6, -1e-3), (1.1e-6, 0.5e-6, 1e-3)),
                hz_magnetic=(0.45e-6, 0.55e-6, 2.0),
-               hz_dipoles=((0.8e-6, 0.01e-6, 1e-3, 30.0), (0.99e-6, 0.5e-6, 2e-3, 90.0)))
+               hz_dipoles=((0.8e-6, 0.01e-6, 1e-3, 30.0), (0.99e-6, 0.5e-6, 2e-3, 90.0)),
+               beam=dict(axis='y', position=1.35e-6, waist=0.2e-6, center=0.6e-6, amplitude=0.5,
+                         span=(0.1e-6, 0.9e-6)))
 
 
 def build_point2d(target, polarization, boundary_type, **options):
-    """The point-current case on `target`; returns (sim, sources) with one entry per reference source (a dipole's
-    entry is its list of face sources)."""
+    """The point-current case (with its Gaussian beam last) on `target`; returns (sim, sources) with one entry per
+    reference point source (a dipole's entry is its list of face sources)."""
     spec = POINT2D
     sim = target(spec['size_x'], spec['size_y'], spec['dx'], npml=spec['npml'],
                  boundaries=boundary_type(spec['faces'], spec['npml']), polarization=polarization, **options)
@@ -75,6 +77,7 @@ def build_point2d(target, polarization, boundary_type, **options):
         sources = [sim.add_magnetic_line_current(x, y, magnetic_current=km, **pulse)]
         sources += [sim.add_electric_dipole(x, y, dipole_moment=moment, orientation=angle, **pulse)
                     for x, y, moment, angle in spec['hz_dipoles']]
+    sim.add_gaussian_beam(**spec['beam'], **pulse)
     return sim, sources
 
 

@@ -9,7 +9,9 @@ Cases (window 1.5 um x 1 um, dx 25 nm, 10-cell CPML on x, y periodic; a dielectr
   hz: a magnetic line current in the dielectric; an in-plane dipole at 30 degrees in y cell 0 (its E_x faces wrap to
       the last row); a dipole along y next to the PEC rectangle (its E_x shares fall below the 1e-12 cut; one E_y
       face touches PEC: weight 0).
-Fields are recorded after 120 and 400 steps, with every source weight.
+Both cases also carry a Gaussian beam (a sheet source with the transverse factor exp(-((s - centre) / waist)^2))
+over part of a line, whose calibrated profile is recorded. Fields are recorded after 120 and 400 steps, with every
+source weight.
 
 Run in this container only (the reference is not present on the GPU box):
     python tests/golden/make_point2d_golden.py
@@ -32,6 +34,7 @@ STEPS = (120, 400)
 EZ_POINTS = ((0.4e-6, 0.5e-6, 2e-3), (0.85e-6, 0.3e-6, -1e-3), (1.1e-6, 0.5e-6, 1e-3))
 HZ_MAGNETIC = (0.45e-6, 0.55e-6, 2.0)
 HZ_DIPOLES = ((0.8e-6, 0.01e-6, 1e-3, 30.0), (0.99e-6, 0.5e-6, 2e-3, 90.0))
+BEAM = dict(axis='y', position=1.35e-6, waist=0.2e-6, center=0.6e-6, amplitude=0.5, span=(0.1e-6, 0.9e-6))
 
 
 def run_case(polarization):
@@ -59,6 +62,7 @@ def run_case(polarization):
         sources += [em.add_electric_dipole(x, y, wavelength=LAMBDA0, bandwidth=BANDWIDTH, dipole_moment=moment,
                                            orientation=angle)
                     for x, y, moment, angle in HZ_DIPOLES]
+    beam = em.add_gaussian_beam(wavelength=LAMBDA0, bandwidth=BANDWIDTH, **BEAM)
     names = ('ez', 'hx', 'hy') if polarization == 'ez' else ('hz', 'ex', 'ey')
     out = {'pec_mask': mask}
     done = 0
@@ -76,6 +80,8 @@ def run_case(polarization):
             out[f'source{index}_weights'] = np.asarray(source.face_weights, dtype=np.float64)
         else:
             out[f'source{index}_weights'] = np.asarray(source.weight, dtype=np.float64)
+    out['beam_span'] = np.array(beam.span)
+    out['beam_profile'] = np.asarray(beam.profile, dtype=np.float64)
     return out
 
 

@@ -192,15 +192,18 @@ def test_point_current_records_match_the_reference(polarization):
     times = clock_times(model.dt, 8)
     for source in model.sources:
         at = times[:-1] if source.field == 'h' else times[:-1] + 0.5 * float(model.dt)
-        expected = np.array([np.float32(source.temporal(t)) for t in at], dtype=np.float32)
+        expected = np.array([source.amplitude * np.float32(source.temporal(t)) for t in at], dtype=np.float32)
         table = source.wave_table(at)
         np.testing.assert_array_equal(table, expected)
-    assert [s.field for s in model.sources] == (['e'] * 3 if polarization == 'ez' else ['h'] + ['e'] * 6)
+    assert [s.field for s in model.sources] == (['e'] * 4 if polarization == 'ez' else ['h'] + ['e'] * 7)
+    beam = model.sources[-1]   # the Gaussian beam
+    assert list(beam.span) == golden['beam_span'].tolist()
+    np.testing.assert_allclose(beam.profile, golden['beam_profile'], rtol=1e-6, atol=0)
     wrong = 'add_magnetic_line_current' if polarization == 'ez' else 'add_point_source'
     with pytest.raises(ValueError, match='needs polarization'):
         getattr(model, wrong)(0.5e-6, 0.5e-6, wavelength=600e-9)
     if polarization == 'hz':
-        model.add_electric_dipole(0.3e-6, 0.3e-6, wavelength=600e-9, orientation=45.0)   # 7 + 4 records > 8
+        model.add_electric_dipole(0.3e-6, 0.3e-6, wavelength=600e-9, orientation=45.0)   # 8 + 4 records > 8
         with pytest.raises(ValueError, match='at most 8 source records'):
             model.finalize()
 

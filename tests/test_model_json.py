@@ -80,7 +80,7 @@ def test_unsupported_models_are_refused():
         with pytest.raises(ValueError):
             M._Built(other, names)
     other = json.loads(json.dumps(spec))
-    other['physics'][0]['features'].append({'name': 'beam', 'type': 'gaussian_beam', 'selection': ['source_line'],
+    other['physics'][0]['features'].append({'name': 'box', 'type': 'plane_wave_tfsf', 'selection': [],
                                             'params': {}, 'enabled': True})
     with pytest.raises(ValueError):
         M._Built(other, names)
@@ -195,6 +195,21 @@ def test_point_source_feature_places_the_point_currents():
     source['params'].update(excitation='electric_dipole', dipole_moment=1e-3, orientation=90.0)
     faces = M._Built(spec, names).sim.model.sources
     assert [f.component for f in faces] == ['ey', 'ey'] and all(f.field == 'e' for f in faces)
+
+
+def test_gaussian_beam_feature_is_a_shaped_line_source():
+    """A 'gaussian_beam' feature (definition_common.py:1227-1232): the line source of its line with the transverse
+    factor exp(-((s - centre) / waist)^2), centre defaulting to the middle of the line."""
+    golden, spec = golden_spec('ez')
+    source = next(f for f in spec['physics'][0]['features'] if f['type'] == 'line_source')
+    source['type'] = 'gaussian_beam'
+    source['params'] = dict(source['params'], waist='0.8*um')
+    names = M.evaluate_parameters(spec['parameters'])
+    (beam,) = M._Built(spec, names).sim.model.sources
+    line = beam.weight.astype(np.float64)
+    centres = M._Built(spec, names).sim.model.domain.cell_centers()[1][beam.span[0]:beam.span[1]]
+    expected = np.exp(-((centres - 1.5e-6) / 0.8e-6) ** 2)   # the line runs y = 0 .. Ly = 3 um: centre 1.5 um
+    np.testing.assert_allclose(line / line.max(), expected / expected.max(), rtol=1e-6, atol=0)
 
 
 def test_3d_dispersive_material_becomes_a_pole_on_its_box():

@@ -112,8 +112,8 @@ def test_oracle_reproduces_reference_pec_case(polarization):
 @pytest.mark.parametrize('polarization', ['ez', 'hz'])
 def test_oracle_reproduces_reference_point_currents(polarization):
     """Point currents (E_z filament currents; H_z magnetic line current and in-plane dipoles, with the periodic face
-    wrap, the 1e-12 share cut and a PEC face): the FP64 oracle reproduces the reference's FP64 run (cells, faces and
-    float32 weights exactly; fields at two steps to 1e-12 of the peak)."""
+    wrap, the 1e-12 share cut and a PEC face) and a Gaussian beam: the FP64 oracle reproduces the reference's FP64 run
+    (cells, faces, float32 weights and the beam profile exactly; fields at two steps to 1e-12 of the peak)."""
     from tests.cases import POINT2D, build_point2d
     golden = np.load(os.path.join(FIXTURES, f'point2d_{polarization}.npz'))
     sim, sources = build_point2d(Yee2D, polarization, _oracle_boundaries)
@@ -138,6 +138,10 @@ def test_oracle_reproduces_reference_point_currents(polarization):
             assert list(source.cell) == golden[f'source{index}_cell'].tolist()
     weights = [w for k, w in golden.items() if k.endswith('_weights')]
     assert any(np.any(w == 0) for w in weights)   # a PEC cell or face injects nothing
+    beam = sim.sources[-1]   # the Gaussian beam: span and calibrated float32 profile
+    assert list(beam.span) == golden['beam_span'].tolist()
+    np.testing.assert_array_equal(beam.profile.astype(np.float64), golden['beam_profile'])
+    assert golden['beam_profile'].max() > 4 * golden['beam_profile'][[0, -1]].max()   # a bell over the span
 
 
 @pytest.mark.parametrize('polarization', ['ez', 'hz'])
