@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 10
+#define YEE_ABI_VERSION 11
 #define YEE_MAX_SOURCES 8
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
@@ -212,6 +212,10 @@ typedef struct yee2d_desc {
     int32_t pad4_;
     yee_pole pole[YEE_MAX_POLES];
     yee_probes probes;
+    /* E_z outer-face rows pinned after each step, per face (x_lo, x_hi, y_lo, y_hi): 0 none, +1 the row copies its
+     * inner neighbour (pmc, symmetric mirror), -1 the negated copy (antisymmetric mirror); x faces first, a corner
+     * taking both factors. Replaces physics/em/boundaries.py:478-488 (apply_ez_walls, wall_centre_rows :105-128). */
+    int32_t wall_sign[4];
     int64_t* clock;          /* device int64: absolute step index n of the next step */
     int64_t step_limit;      /* steps n >= step_limit are skipped (batched instances); INT64_MAX for none */
 } yee2d_desc;

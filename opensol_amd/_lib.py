@@ -6,7 +6,7 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 10
+ABI_VERSION = 11
 MAX_SOURCES = 8
 SOURCE_E, SOURCE_H = 0, 1   # yee_line_source.field
 MAX_MONITORS = 4
@@ -65,7 +65,7 @@ class Desc2DC(ctypes.Structure):
                 ('pad2_', c_i32), ('source', LineSourceC * MAX_SOURCES), ('monitor', FluxLineC * MAX_MONITORS),
                 ('n_dft', c_i32), ('pad3_', c_i32), ('dft', DftRegionC * MAX_DFT),
                 ('n_poles', c_i32), ('pad4_', c_i32), ('pole', PoleC * MAX_POLES),
-                ('probes', ProbesC), ('clock', c_ptr), ('step_limit', c_i64)]
+                ('probes', ProbesC), ('wall_sign', c_i32 * 4), ('clock', c_ptr), ('step_limit', c_i64)]
 
 
 class PointSourceC(ctypes.Structure):

@@ -12,7 +12,9 @@ import numpy as np
 from .constants import DTYPE, EPS0, FACES, FACES_3D
 
 CPML, PERIODIC, PEC, PMC = 'cpml', 'periodic', 'pec', 'pmc'
-_KINDS = (CPML, PERIODIC, PEC, PMC)
+SYMMETRIC, ANTISYMMETRIC = 'symmetric', 'antisymmetric'
+MIRRORS = (SYMMETRIC, ANTISYMMETRIC)
+_KINDS = (CPML, PERIODIC, PEC, PMC, SYMMETRIC, ANTISYMMETRIC)
 
 
 class BoundaryConfig:
@@ -20,7 +22,9 @@ class BoundaryConfig:
 
     Face types: 'cpml', 'periodic', 'pec' (the outermost window cell layer is a perfect electric conductor: 3D and 2D
     E_z) and 'pmc' (3D: the natural grid edge, H tangential 0 half a cell beyond the outermost centres,
-    physics/em3d/engine3d.py:12-13). Mirrors and Bloch phases are not part of this FDTD path."""
+    physics/em3d/engine3d.py:12-13; 2D E_z: the outermost row copies its inner neighbour), 'symmetric' / 'antisymmetric'
+    (2D E_z mirrors: one ghost cell beyond the window copies the edge cell, negated for 'antisymmetric',
+    physics/em/boundaries.py:449-488, 283-291). Bloch phases are not part of this FDTD path."""
 
     def __init__(self, x_lo='cpml', x_hi='cpml', y_lo='cpml', y_hi='cpml', z_lo=None, z_hi=None):
         faces = {'x_lo': x_lo, 'x_hi': x_hi, 'y_lo': y_lo, 'y_hi': y_hi}
@@ -31,7 +35,7 @@ class BoundaryConfig:
             kind = getattr(kind, 'value', kind)
             if kind not in _KINDS:
                 raise ValueError(f"face {face}: type {kind!r} is not supported by this FDTD path "
-                                 f"(cpml, periodic, pec, pmc)")
+                                 f"(cpml, periodic, pec, pmc, symmetric, antisymmetric)")
             self.faces[face] = kind
         self.cpml_npml = 15
         self.cpml_order = 3
@@ -93,7 +97,10 @@ class BoundaryConfig:
 
     @property
     def npml_sides(self):
-        return {face: (self.cpml_npml if kind == CPML else 0) for face, kind in self.faces.items()}
+        """Padding cells per face: cpml_npml on cpml faces, one ghost cell on mirror faces, else 0
+        (physics/em/boundaries.py:283-286)."""
+        return {face: (self.cpml_npml if kind == CPML else 1 if kind in MIRRORS else 0)
+                for face, kind in self.faces.items()}
 
     @classmethod
     def all_pml(cls, npml=15, dimensions=2):

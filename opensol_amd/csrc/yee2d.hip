@@ -1356,6 +1356,36 @@ __global__ void __launch_bounds__(256) yee2d_probe_kernel(yee2d_desc desc, int32
     }
 }
 
+// E_z outer-face rows after step n = *clock + offset (physics/em/boundaries.py:105-128 wall_centre_rows, :478-488):
+// lane k < ny pins column k of the pinned x faces from row 1 (nx - 2), taking the y factor and the inner column at a
+// pinned y corner; lane ny + i pins the y faces of row i (corner rows belong to the x lanes). Every lane reads only
+// rows and columns no lane writes, so the lanes need no ordering.
+__global__ void __launch_bounds__(256) yee2d_wall_kernel(yee2d_desc desc, int32_t offset)
+{
+    const yee2d_desc& d = desc;
+    const int64_t n = *d.clock + offset;
+    if (n >= d.step_limit) return;
+    float* ez = d.field[(n & 1) ^ 1][0];
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t P = d.pitch;
+    const float s_xlo = (float)d.wall_sign[0], s_xhi = (float)d.wall_sign[1];
+    const float s_ylo = (float)d.wall_sign[2], s_yhi = (float)d.wall_sign[3];
+    if (k < d.ny) {
+        int64_t j = k, inner = k;
+        float yf = 1.0f;
+        if (j == 0 && d.wall_sign[2] != 0) { inner = 1; yf = s_ylo; }
+        else if (j == d.ny - 1 && d.wall_sign[3] != 0) { inner = d.ny - 2; yf = s_yhi; }
+        if (d.wall_sign[0] != 0) ez[j] = (s_xlo * yf) * ez[P + inner];
+        if (d.wall_sign[1] != 0) ez[(int64_t)(d.nx - 1) * P + j] = (s_xhi * yf) * ez[(int64_t)(d.nx - 2) * P + inner];
+    } else if (k < (int64_t)d.ny + d.nx) {
+        const int64_t i = k - d.ny;
+        if ((i == 0 && d.wall_sign[0] != 0) || (i == d.nx - 1 && d.wall_sign[1] != 0)) return;
+        const int64_t row = i * P;
+        if (d.wall_sign[2] != 0) ez[row] = s_ylo * ez[row + 1];
+        if (d.wall_sign[3] != 0) ez[row + d.ny - 1] = s_yhi * ez[row + d.ny - 2];
+    }
+}
+
 // Advance one clock by k steps (bounded by the instance's step limit); a single lane writes with a vector store.
 __global__ void yee_clock_advance(int64_t* clock, int64_t k, int64_t limit)
 {

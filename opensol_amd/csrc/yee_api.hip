@@ -29,6 +29,7 @@ __global__ void yee3d_batch_sample_kernel(const yee3d_desc* __restrict__ descs, 
 __global__ void yee_clock_advance_batch3d(const yee3d_desc* __restrict__ descs, int32_t n_inst, int64_t k);
 __global__ void yee_clock_advance(int64_t* clock, int64_t k, int64_t limit);
 __global__ void yee2d_probe_kernel(yee2d_desc d, int32_t offset);
+__global__ void yee2d_wall_kernel(yee2d_desc d, int32_t offset);
 __global__ void yee3d_probe_kernel(yee3d_desc d, int32_t offset);
 __global__ void yee2d_dft_kernel(yee2d_desc d, int32_t offset);
 __global__ void yee3d_dft_kernel(yee3d_desc d, int32_t offset);
@@ -192,6 +193,15 @@ int32_t check_2d(const yee2d_desc* d)
         if (!src.profile || !src.wave) return fail(YEE_EINVAL, "source profile / wave missing");
         if (src.field != YEE_SOURCE_E && !(src.field == YEE_SOURCE_H && d->polarization == YEE_HZ))
             return fail(YEE_EINVAL, "source field must be YEE_SOURCE_E (or YEE_SOURCE_H on an H_z instance)");
+    }
+    for (int f = 0; f < 4; ++f) {
+        const int32_t sign = d->wall_sign[f];
+        if (sign == 0) continue;
+        const int axis_periodic = f < 2 ? d->periodic_x : d->periodic_y;
+        const int n_axis = f < 2 ? d->nx : d->ny;
+        if (sign != 1 && sign != -1) return fail(YEE_EINVAL, "wall_sign must be -1, 0 or 1");
+        if (d->polarization != YEE_EZ) return fail(YEE_EINVAL, "wall rows are E_z only");
+        if (axis_periodic || n_axis < 2) return fail(YEE_EINVAL, "wall rows need a non-periodic axis of 2+ cells");
     }
     if (d->n_monitors < 0 || d->n_monitors > YEE_MAX_MONITORS) return fail(YEE_EINVAL, "n_monitors out of range");
     if (d->n_monitors > 0 && d->sample_interval < 1) return fail(YEE_EINVAL, "sample_interval must be >= 1");
@@ -444,6 +454,14 @@ void drop_graph(yee_ctx* ctx)
     }
 }
 
+bool has_walls(const yee2d_desc& d)
+{
+    return d.wall_sign[0] != 0 || d.wall_sign[1] != 0 || d.wall_sign[2] != 0 || d.wall_sign[3] != 0;
+}
+
+// One lane per column (x faces) and per row (y faces).
+int wall_blocks(const yee2d_desc& d) { return (int)(((int64_t)d.nx + d.ny + 255) / 256); }
+
 // Launch `count` steps (offsets 0..count-1 from the device clock) followed by the clock advance.
 int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
 {
@@ -456,6 +474,8 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
                 hipLaunchKernelGGL(bulk, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
             if (ctx->edge_blocks > 0)
                 hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
+            if (has_walls(ctx->d2))
+                hipLaunchKernelGGL(yee2d_wall_kernel, dim3(wall_blocks(ctx->d2)), dim3(256), 0, stream, ctx->d2, k);
             if (ctx->d2.n_dft > 0)
                 hipLaunchKernelGGL(yee2d_dft_kernel, dim3(dft_blocks(ctx->d2.dft, ctx->d2.n_dft), ctx->d2.n_dft),
                                    dim3(256), 0, stream, ctx->d2, k);
@@ -614,6 +634,8 @@ int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, 
             status = fail(YEE_EINVAL, "batched instances take no field probes");
         if (status == YEE_OK && host[k].n_dft > 0)
             status = fail(YEE_EINVAL, "batched instances take no DFT regions");
+        if (status == YEE_OK && has_walls(host[k]))
+            status = fail(YEE_EINVAL, "batched instances take no wall rows");
         if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
         const int64_t next = prefix[k] + tiles_2d(host[k]);
         if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
@@ -738,6 +760,8 @@ int32_t yee_step_timed(yee_ctx* ctx, int64_t n_steps, void* stream, double* kern
         if (ctx->edge_blocks > 0)
             hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, s, ctx->d2, ctx->sets, 0);
         (void)hipEventRecord(ev[2], s);
+        if (has_walls(ctx->d2))
+            hipLaunchKernelGGL(yee2d_wall_kernel, dim3(wall_blocks(ctx->d2)), dim3(256), 0, s, ctx->d2, 0);
         if (ctx->d2.n_dft > 0)
             hipLaunchKernelGGL(yee2d_dft_kernel, dim3(dft_blocks(ctx->d2.dft, ctx->d2.n_dft), ctx->d2.n_dft),
                                dim3(256), 0, s, ctx->d2, 0);

@@ -38,10 +38,10 @@ class Model2D:
         self.bc = resolve_boundaries(boundaries, npml)
         if self.bc.dimensions != 2:
             raise ValueError('a 2D model needs a 2D BoundaryConfig')
-        walls = [face for face, kind in self.bc.faces.items() if kind in ('pec', 'pmc')]
-        if walls and (polarization != 'ez' or any(self.bc.faces[face] == 'pmc' for face in walls)):
-            raise ValueError(f"2D faces {walls}: this FDTD path has 'pec' faces for the E_z polarisation only "
-                             f"(the E_z row of the wall is pinned to 0, physics/em/boundaries.py:57-61)")
+        walls = [face for face, kind in self.bc.faces.items() if kind not in ('cpml', 'periodic')]
+        if walls and polarization != 'ez':
+            raise ValueError(f"2D faces {walls}: this FDTD path has 'pec', 'pmc' and mirror faces for the E_z "
+                             f"polarisation only (physics/em/boundaries.py:449-488)")
         self.domain = Domain((size_x, size_y), dx, self.bc.npml_sides, courant)
         self.grid = self.domain.grid
         self.nx, self.ny = self.grid.shape
@@ -410,7 +410,13 @@ class Model2D:
         """Courant check, sheet calibration and the DFT sample interval shared by every flux and DFT monitor
         (physics/em/__init__.py:181-203)."""
         if len(self.sources) > _lib.MAX_SOURCES:
-            raise ValueError(f'at most {_lib.MAX_SOURCES} source records per 2D model (line sources, point currents, dipole faces)')
+            raise ValueError(f'at most {_lib.MAX_SOURCES} source records per 2D model (line sources, point currents, '
+                             f'dipole faces)')
+        pinned = wall_rows(self)
+        for monitor in self.monitors:   # the fused flux sample reads E before the wall rows are copied
+            normal = 'xy'.index(monitor.normal)
+            if monitor.position in pinned[normal] or any(c in pinned[1 - normal] for c in range(*monitor.span)):
+                raise ValueError('a flux monitor may not touch a pmc wall row (its E there is pinned after the step)')
         if len(self.monitors) > _lib.MAX_MONITORS:
             raise ValueError(f'at most {_lib.MAX_MONITORS} flux monitors per 2D model')
         if len(self.dft_monitors) > _lib.MAX_DFT:
@@ -463,6 +469,21 @@ class Model2D:
                             ) / (w + w)
                 keep = ~(pec[ids] | pec[upper_ids])
             source.calibrate(np.where(keep, cb, 0.0), line_eps, mu, sigma_m, dt)
+
+
+# E_z outer rows copied from their inner neighbour after each step: pmc and symmetric faces keep E_z even, an
+# antisymmetric face odd (physics/em/boundaries.py:478-488); 'pec' rows are PEC cells instead.
+WALL_SIGNS = {'pmc': 1, 'symmetric': 1, 'antisymmetric': -1}
+
+
+def wall_rows(model):
+    """{axis: set of pinned outer rows} of the model's copied wall rows."""
+    rows = {0: set(), 1: set()}
+    for face, kind in model.bc.faces.items():
+        if kind in WALL_SIGNS:
+            axis = 'xy'.index(face[0])
+            rows[axis].add(0 if face.endswith('lo') else model.grid.shape[axis] - 1)
+    return rows
 
 
 def clock_times(dt, n_steps):
@@ -637,6 +658,8 @@ class Engine2D:
             src.profile = self.profiles[k].data_ptr()
             src.wave = self.waves[k].data_ptr()
             src.wave_len = self.waves[k].numel()
+        for f, face in enumerate(('x_lo', 'x_hi', 'y_lo', 'y_hi')):
+            d.wall_sign[f] = WALL_SIGNS.get(model.bc.faces[face], 0)
         d.n_monitors = len(model.monitors)
         d.sample_interval = model.sample_interval
         for k, monitor in enumerate(model.monitors):

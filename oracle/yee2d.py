@@ -57,8 +57,8 @@ class Boundaries:
     def __init__(self, faces, npml=15, order=3, sigma_factor=1.0, kappa_max=15.0, alpha_max=0.0):
         self.faces = dict(faces)
         for face, kind in self.faces.items():
-            if kind not in ('cpml', 'periodic', 'pec'):
-                raise ValueError(f'oracle supports cpml, periodic and pec faces, got {face}={kind}')
+            if kind not in ('cpml', 'periodic', 'pec', 'pmc', 'symmetric', 'antisymmetric'):
+                raise ValueError(f'oracle supports cpml, periodic, pec, pmc and mirror faces, got {face}={kind}')
         for axis in 'xy':
             if (self.faces[f'{axis}_lo'] == 'periodic') != (self.faces[f'{axis}_hi'] == 'periodic'):
                 raise ValueError(f'{axis}: periodic faces must be paired')
@@ -83,7 +83,14 @@ class Boundaries:
         return self.faces[f'{axis}_lo'] == 'periodic'
 
     def npml_sides(self):
-        return {face: (self.npml if self.cpml(face) else 0) for face in FACES}
+        """cpml faces: npml cells; mirror faces: one ghost cell (physics/em/boundaries.py:283-286); walls: none."""
+        return {face: (self.npml if self.cpml(face) else 1 if self.faces[face] in ('symmetric', 'antisymmetric')
+                       else 0) for face in FACES}
+
+    def wall_sign(self, face):
+        """E_z outer-row copy factor of a face: +1 pmc / symmetric (E_z even), -1 antisymmetric, None otherwise
+        (physics/em/boundaries.py:478-488)."""
+        return {'pmc': 1.0, 'symmetric': 1.0, 'antisymmetric': -1.0}.get(self.faces[face])
 
 
 class Geometry2D:
@@ -517,8 +524,8 @@ class Yee2D:
         self.sources, self.monitors, self.probes = [], [], []
         self.pec_regions = []
         self.poles = []
-        if polarization == 'hz' and any(kind == 'pec' for kind in self.bc.faces.values()):
-            raise ValueError("the oracle's 'pec' faces are the E_z wall rows")
+        if polarization == 'hz' and any(kind not in ('cpml', 'periodic') for kind in self.bc.faces.values()):
+            raise ValueError("the oracle's wall and mirror faces are E_z rows")
         self.n_step, self.time = 0, 0.0
         self._finalized = False
         self._build_spacings_and_cpml()
@@ -873,6 +880,8 @@ class Yee2D:
         for source in self.sources:
             if getattr(source, 'field', 'e') == 'e':
                 source.inject(self.fields, t_half)
+        if self.polarization == 'ez':
+            self._ez_walls()
         self.n_step += 1
         self.time = t_full
         for probe in self.probes:
@@ -880,6 +889,16 @@ class Yee2D:
         for monitor in self.monitors:
             if self.n_step % monitor.sample_interval == 0:
                 monitor.update(self.fields, t_full)
+
+    def _ez_walls(self):
+        """Outer E_z rows after the sources (the re-pin of fdtd_ez_2d.py:105-117): x faces, then y faces, each row
+        set to its factor times its inner neighbour (physics/em/boundaries.py:130-141 host twin, :478-488)."""
+        ez = self.fields['ez']
+        for face, outer, inner in (('x_lo', np.s_[0, :], np.s_[1, :]), ('x_hi', np.s_[-1, :], np.s_[-2, :]),
+                                   ('y_lo', np.s_[:, 0], np.s_[:, 1]), ('y_hi', np.s_[:, -1], np.s_[:, -2])):
+            sign = self.bc.wall_sign(face)
+            if sign is not None:
+                ez[outer] = self.dtype(sign) * ez[inner]
 
     def _step_ez(self):
         f = self.fields
@@ -895,6 +914,11 @@ class Yee2D:
             hy[-1] = da[-1] * hy[-1] + db[-1] * (ez[0] - ez[-1]) * self.inv_dx_h[-1]
         for side in self.sides:   # apply_h: x sides (ez, db, hy), y sides (ez, db, hx) (boundaries.py:752-756)
             side.apply('h', ez, db, hy if side.axis == 'x' else hx)
+        # pmc planes: tangential H = 0 on face 0 / n-2 (apply_h_walls, boundaries.py:463-473)
+        for face, field, row in (('x_lo', hy, np.s_[0, :]), ('x_hi', hy, np.s_[-2, :]),
+                                 ('y_lo', hx, np.s_[:, 0]), ('y_hi', hx, np.s_[:, -2])):
+            if self.bc.faces[face] == 'pmc':
+                field[row] = 0
         # pole updates from E^n (fdtd_ez_2d.py:318-345)
         for entry in self.poles:
             if entry['pole'][0] == 'drude':

@@ -81,6 +81,28 @@ def build_point2d(target, polarization, boundary_type, **options):
     return sim, sources
 
 
+# 2D E_z case bounded by wall and mirror faces (tests/golden/make_walls2d_golden.py writes it from the reference):
+# x_lo symmetric, x_hi pmc, y_lo antisymmetric, y_hi CPML; a dielectric, a filament current near the (x_lo, y_lo)
+# corner and a line source over the whole grid line (into the pinned rows).
+WALLS2D = dict(size_x=1.2e-6, size_y=1e-6, dx=25e-9, npml=10, faces=('symmetric', 'pmc', 'antisymmetric', 'cpml'),
+               dielectric=((0.5e-6, 0.2e-6, 0.8e-6, 0.6e-6), 3.0), point=(0.1e-6, 0.12e-6, 1e-3),
+               line=dict(axis='y', position=0.95e-6), wavelength=600e-9, bandwidth=0.6, steps=(150, 400))
+
+
+def build_walls2d(target, boundary_type, **options):
+    """The wall and mirror case on `target` (oracle Yee2D or opensol_amd Simulation)."""
+    spec = WALLS2D
+    sim = target(spec['size_x'], spec['size_y'], spec['dx'], npml=spec['npml'],
+                 boundaries=boundary_type(spec['faces'], spec['npml']), polarization='ez', **options)
+    (x0, y0, x1, y1), eps = spec['dielectric']
+    sim.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
+    pulse = dict(wavelength=spec['wavelength'], bandwidth=spec['bandwidth'])
+    x, y, current = spec['point']
+    sim.add_point_source(x, y, current=current, **pulse)
+    sim.add_line_source(**spec['line'], **pulse)
+    return sim
+
+
 def pec_rect_mask(sim, x0, y0, x1, y1):
     """Cells whose float64 centre lies in [x0, x1) x [y0, y1) (the reference's Domain.rect_mask, core/domain.py:
     255-260), on either target."""

@@ -218,6 +218,32 @@ def test_point_currents_match_oracle(gpu, polarization):
         assert not np.any(sim.engine.field('ez')[ref.pec_mask()])
 
 
+def test_walls_and_mirrors_match_oracle(gpu):
+    """E_z with symmetric, pmc and antisymmetric faces against the FP64 oracle (which reproduces the reference's run to
+    1e-12): fields <= 1e-5 relative L2 at both steps; the pinned rows are exact copies (negated on the antisymmetric
+    face); graph replay bit-identical to eager stepping."""
+    from tests.cases import WALLS2D, build_walls2d
+    Simulation = simulation_class()
+    ref = build_walls2d(Yee2D, _oracle_boundaries)
+    sim = build_walls2d(Simulation, _product_boundaries)
+    graphed = build_walls2d(Simulation, _product_boundaries, graph_steps=16)
+    done = 0
+    for step in WALLS2D['steps']:
+        print(f'walls to step {step}', flush=True)
+        ref.run(n_steps=step - done)
+        sim.run(n_steps=step - done, print_every=0)
+        graphed.run(n_steps=step - done, print_every=0)
+        done = step
+        for name in sim.model.components:
+            error = rel_l2(sim.engine.field(name), ref.fields[name])
+            print(f'walls step {step} {name}: rel L2 {error:.3e}')
+            assert error <= FIELD_TOL, (step, name, error)
+            np.testing.assert_array_equal(graphed.engine.field(name), sim.engine.field(name))
+    ez = sim.engine.field('ez')
+    assert np.array_equal(ez[0, 1:], ez[1, 1:]) and np.array_equal(ez[-1], ez[-2])
+    assert np.array_equal(ez[1:, 0], -ez[1:, 1]) and np.abs(ez[0]).max() > 0
+
+
 @pytest.mark.parametrize('polarization', ['ez', 'hz'])
 def test_field_probes_match_oracle(gpu, polarization):
     """Field probes of the thin-film case: every sample within 1e-5 of the series' peak of the FP64 oracle's, times

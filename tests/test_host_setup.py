@@ -208,13 +208,27 @@ def test_point_current_records_match_the_reference(polarization):
             model.finalize()
 
 
-def test_pec_faces_are_e_z_only():
+def test_wall_faces_are_e_z_only_and_padded_like_the_oracle():
+    """Wall and mirror faces: E_z only; mirror faces pad one ghost cell and walls none (the grid equals the oracle's);
+    the descriptor's wall factors; a flux line touching a pmc row is refused."""
     from opensol_amd.boundaries import BoundaryConfig
-    bc = BoundaryConfig('pec', 'cpml', 'periodic', 'periodic')
-    with pytest.raises(ValueError, match='E_z polarisation only'):
-        Model2D(1e-6, 1e-6, 25e-9, npml=5, boundaries=bc, polarization='hz')
-    with pytest.raises(ValueError, match='E_z polarisation only'):
-        Model2D(1e-6, 1e-6, 25e-9, npml=5, boundaries=BoundaryConfig('pmc', 'cpml', 'cpml', 'cpml'))
+    from oracle.yee2d import FACES, Boundaries
+    for kind in ('pec', 'pmc', 'symmetric', 'antisymmetric'):
+        with pytest.raises(ValueError, match='E_z polarisation only'):
+            Model2D(1e-6, 1e-6, 25e-9, npml=5, boundaries=BoundaryConfig(kind, 'cpml', 'cpml', 'cpml'),
+                    polarization='hz')
+    faces = ('symmetric', 'pmc', 'antisymmetric', 'cpml')
+    bc = BoundaryConfig(*faces)
+    bc.cpml_npml = 10
+    model = Model2D(1.2e-6, 1e-6, 25e-9, npml=10, boundaries=bc)
+    ref = Yee2D(1.2e-6, 1e-6, 25e-9, npml=10, boundaries=Boundaries(dict(zip(FACES, faces)), npml=10))
+    assert model.shape == (ref.nx, ref.ny) == (49, 51)
+    from opensol_amd.fdtd2d import WALL_SIGNS, wall_rows
+    assert [WALL_SIGNS.get(k, 0) for k in faces] == [1, 1, -1, 0]
+    assert wall_rows(model) == {0: {0, 48}, 1: {0}}
+    model.add_flux_monitor('y', 1.2e-6 - 1e-9, (0.2e-6, 0.8e-6), wavelengths=[600e-9])   # x cell 48: the pmc row
+    with pytest.raises(ValueError, match='pmc wall row'):
+        model.finalize()
 
 
 @pytest.mark.parametrize('polarization', ['ez', 'hz'])

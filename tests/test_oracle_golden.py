@@ -144,6 +144,24 @@ def test_oracle_reproduces_reference_point_currents(polarization):
     assert golden['beam_profile'].max() > 4 * golden['beam_profile'][[0, -1]].max()   # a bell over the span
 
 
+def test_oracle_reproduces_reference_walls_and_mirrors():
+    """E_z with a symmetric, a pmc and an antisymmetric face (ghost cells, even / odd copies, the corner, the re-pin
+    after a source injects into a pinned row): the FP64 oracle reproduces the reference's FP64 run to 1e-12."""
+    from tests.cases import WALLS2D, build_walls2d
+    golden = np.load(os.path.join(FIXTURES, 'walls2d_ez.npz'))
+    sim = build_walls2d(Yee2D, _oracle_boundaries)
+    assert (sim.nx, sim.ny) == (int(golden['nx']), int(golden['ny']))
+    done = 0
+    for step in WALLS2D['steps']:
+        sim.run(n_steps=step - done)
+        done = step
+        for name, values in sim.fields.items():
+            want = golden[f'{name}_{step}']
+            np.testing.assert_allclose(values, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    ez = golden['ez_400']
+    assert np.array_equal(ez[0], ez[1]) and np.array_equal(ez[:, 0], -ez[:, 1]) and np.abs(ez[0]).max() > 0
+
+
 @pytest.mark.parametrize('polarization', ['ez', 'hz'])
 def test_oracle_reproduces_reference_pole_case(polarization):
     """Drude rectangle (eps_inf background) under a PEC corner and a Lorentz disc on a dielectric: the FP64 oracle
