@@ -51,6 +51,9 @@ class Model3D:
 
     def __init__(self, size_x, size_y, size_z, dx, npml=12, courant=None, boundaries=None):
         self.bc = resolve_boundaries_3d(boundaries, npml)
+        mirrors = [face for face, kind in self.bc.faces.items() if kind in ('symmetric', 'antisymmetric')]
+        if mirrors:
+            raise ValueError(f'3D faces {mirrors}: mirror faces are 2D E_z only in this FDTD path')
         self.domain = Domain((size_x, size_y, size_z), dx, self.bc.npml_sides, courant)
         self.grid = self.domain.grid
         self.shape = self.grid.shape
