@@ -53,6 +53,7 @@ extern "C" {
  * :431-440) or, with field YEE_SOURCE_H, the H_z magnetic line current (physics/em/sources.py:439-444, :420-429),
  * which adds wave[n] * profile to H_z after the H update of step n (wave sampled at t_n, not t_n + dt/2).
  */
+#define YEE_WALL_ZERO 2   /* wall_sign of an H_z pmc face */
 #define YEE_SOURCE_E 0   /* the E component named by the polarisation and axis, after the E update */
 #define YEE_SOURCE_H 1   /* H_z (H_z instances only), after the H update */
 typedef struct yee_line_source {
@@ -212,9 +213,10 @@ typedef struct yee2d_desc {
     int32_t pad4_;
     yee_pole pole[YEE_MAX_POLES];
     yee_probes probes;
-    /* E_z outer-face rows pinned after each step, per face (x_lo, x_hi, y_lo, y_hi): 0 none, +1 the row copies its
-     * inner neighbour (pmc, symmetric mirror), -1 the negated copy (antisymmetric mirror); x faces first, a corner
-     * taking both factors. Replaces physics/em/boundaries.py:478-488 (apply_ez_walls, wall_centre_rows :105-128). */
+    /* Outer-face rows per face (x_lo, x_hi, y_lo, y_hi). E_z: 0 none, +1 the row copies its inner neighbour after
+     * the step (pmc, symmetric mirror), -1 the negated copy (antisymmetric mirror); x faces first, a corner taking
+     * both factors (physics/em/boundaries.py:478-488, wall_centre_rows :105-128). H_z: 0 none, YEE_WALL_ZERO H_z = 0
+     * on the outermost cells after the H update (pmc, :517-527). */
     int32_t wall_sign[4];
     int64_t* clock;          /* device int64: absolute step index n of the next step */
     int64_t step_limit;      /* steps n >= step_limit are skipped (batched instances); INT64_MAX for none */

@@ -9,6 +9,7 @@ LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dir
 ABI_VERSION = 11
 MAX_SOURCES = 8
 SOURCE_E, SOURCE_H = 0, 1   # yee_line_source.field
+WALL_ZERO = 2   # yee2d_desc.wall_sign of an H_z pmc face
 MAX_MONITORS = 4
 MAX_DFT = 4
 MAX_POLES = 8

@@ -662,6 +662,7 @@ struct HzTile {
                                 layer_y ? cy.b_e[col.ry_e[q]] : 0.f, layer_y ? cy.g_e[col.ry_e[q]] : 0.f,
                                 p.y[q], &p.y[q]);
             if (d.n_sources > 0) v = magnetic_currents(k, j, v);
+            if (wall_cell(k, j)) v = 0.f;
             out[q] = j < ny ? v : 0.f;
         }
         if (store && col.any) {
@@ -682,7 +683,15 @@ struct HzTile {
                                ry_e_r >= 0 ? cy.b_e[ry_e_r] : 0.f, ry_e_r >= 0 ? cy.g_e[ry_e_r] : 0.f, p.y_r,
                                &unused_y);
             if (d.n_sources > 0) *out_r = magnetic_currents(k, jr, *out_r);
+            if (wall_cell(k, jr)) *out_r = 0.f;
         }
+    }
+
+    // an outermost H_z cell of a pmc face (pinned to 0 after the H update and the magnetic currents)
+    __device__ __forceinline__ bool wall_cell(int k, int j) const
+    {
+        return (d.wall_sign[0] != 0 && k == 0) || (d.wall_sign[1] != 0 && k == nx - 1) ||
+               (d.wall_sign[2] != 0 && j == 0) || (d.wall_sign[3] != 0 && j == ny - 1);
     }
 
     // magnetic line currents on H_z cell (k, j) after the H update, at t_n: H_z += float32(value * profile). Every

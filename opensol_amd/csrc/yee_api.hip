@@ -199,8 +199,10 @@ int32_t check_2d(const yee2d_desc* d)
         if (sign == 0) continue;
         const int axis_periodic = f < 2 ? d->periodic_x : d->periodic_y;
         const int n_axis = f < 2 ? d->nx : d->ny;
-        if (sign != 1 && sign != -1) return fail(YEE_EINVAL, "wall_sign must be -1, 0 or 1");
-        if (d->polarization != YEE_EZ) return fail(YEE_EINVAL, "wall rows are E_z only");
+        if (d->polarization == YEE_EZ && sign != 1 && sign != -1)
+            return fail(YEE_EINVAL, "E_z wall_sign must be -1, 0 or 1");
+        if (d->polarization == YEE_HZ && sign != YEE_WALL_ZERO)
+            return fail(YEE_EINVAL, "H_z wall_sign must be 0 or YEE_WALL_ZERO");
         if (axis_periodic || n_axis < 2) return fail(YEE_EINVAL, "wall rows need a non-periodic axis of 2+ cells");
     }
     if (d->n_monitors < 0 || d->n_monitors > YEE_MAX_MONITORS) return fail(YEE_EINVAL, "n_monitors out of range");
@@ -369,6 +371,8 @@ yee2d_tile_sets tile_sets(const yee2d_desc& d)
             edge = edge || (d.source[k].axis == 1 && d.source[k].line >= a && d.source[k].line <= b);
         for (int k = 0; k < d.n_monitors; ++k)
             edge = edge || (d.monitor[k].normal == 0 && d.monitor[k].position >= a && d.monitor[k].position <= b);
+        if (d.polarization == YEE_HZ)   // H_z wall rows are pinned in the general loop
+            edge = edge || (d.wall_sign[0] != 0 && a == 0) || (d.wall_sign[1] != 0 && b >= d.nx - 1);
         for (int k = 0; d.polarization == YEE_HZ && k < d.n_poles; ++k)   // the lean H_z loop has no pole update
             edge = edge || (d.pole[k].i0 < i1 && d.pole[k].i1 > i0);
         if (edge) rows.push_back(ti);
@@ -377,6 +381,8 @@ yee2d_tile_sets tile_sets(const yee2d_desc& d)
         const int j0 = tj * cols, j1 = std::min(d.ny, j0 + cols);
         const int a = j0 > 0 ? j0 - 1 : 0, b = j1 < d.ny ? j1 : d.ny - 1;
         bool edge = a < d.layer_y_lo || b >= d.layer_y_hi;
+        if (d.polarization == YEE_HZ)
+            edge = edge || (d.wall_sign[2] != 0 && a == 0) || (d.wall_sign[3] != 0 && b >= d.ny - 1);
         for (int k = 0; k < d.n_sources; ++k)
             edge = edge || (d.source[k].axis == 0 && d.source[k].line >= a && d.source[k].line <= b);
         for (int k = 0; k < d.n_monitors; ++k)
@@ -459,6 +465,12 @@ bool has_walls(const yee2d_desc& d)
     return d.wall_sign[0] != 0 || d.wall_sign[1] != 0 || d.wall_sign[2] != 0 || d.wall_sign[3] != 0;
 }
 
+// The E_z copy launch (H_z walls act inside the H update).
+bool has_ez_walls(const yee2d_desc& d)
+{
+    return d.polarization == YEE_EZ && has_walls(d);
+}
+
 // One lane per column (x faces) and per row (y faces).
 int wall_blocks(const yee2d_desc& d) { return (int)(((int64_t)d.nx + d.ny + 255) / 256); }
 
@@ -474,7 +486,7 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
                 hipLaunchKernelGGL(bulk, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
             if (ctx->edge_blocks > 0)
                 hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
-            if (has_walls(ctx->d2))
+            if (has_ez_walls(ctx->d2))
                 hipLaunchKernelGGL(yee2d_wall_kernel, dim3(wall_blocks(ctx->d2)), dim3(256), 0, stream, ctx->d2, k);
             if (ctx->d2.n_dft > 0)
                 hipLaunchKernelGGL(yee2d_dft_kernel, dim3(dft_blocks(ctx->d2.dft, ctx->d2.n_dft), ctx->d2.n_dft),
@@ -760,7 +772,7 @@ int32_t yee_step_timed(yee_ctx* ctx, int64_t n_steps, void* stream, double* kern
         if (ctx->edge_blocks > 0)
             hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, s, ctx->d2, ctx->sets, 0);
         (void)hipEventRecord(ev[2], s);
-        if (has_walls(ctx->d2))
+        if (has_ez_walls(ctx->d2))
             hipLaunchKernelGGL(yee2d_wall_kernel, dim3(wall_blocks(ctx->d2)), dim3(256), 0, s, ctx->d2, 0);
         if (ctx->d2.n_dft > 0)
             hipLaunchKernelGGL(yee2d_dft_kernel, dim3(dft_blocks(ctx->d2.dft, ctx->d2.n_dft), ctx->d2.n_dft),

@@ -39,9 +39,9 @@ class Model2D:
         if self.bc.dimensions != 2:
             raise ValueError('a 2D model needs a 2D BoundaryConfig')
         walls = [face for face, kind in self.bc.faces.items() if kind not in ('cpml', 'periodic')]
-        if walls and polarization != 'ez':
-            raise ValueError(f"2D faces {walls}: this FDTD path has 'pec', 'pmc' and mirror faces for the E_z "
-                             f"polarisation only (physics/em/boundaries.py:449-488)")
+        if walls and polarization != 'ez' and any(self.bc.faces[face] != 'pmc' for face in walls):
+            raise ValueError(f"2D faces {walls}: this FDTD path has 'pec' and mirror faces for the E_z "
+                             f"polarisation only, 'pmc' for both (physics/em/boundaries.py:449-527)")
         self.domain = Domain((size_x, size_y), dx, self.bc.npml_sides, courant)
         self.grid = self.domain.grid
         self.nx, self.ny = self.grid.shape
@@ -480,7 +480,7 @@ def wall_rows(model):
     """{axis: set of pinned outer rows} of the model's copied wall rows."""
     rows = {0: set(), 1: set()}
     for face, kind in model.bc.faces.items():
-        if kind in WALL_SIGNS:
+        if kind in WALL_SIGNS and (model.polarization == 'ez' or kind == 'pmc'):
             axis = 'xy'.index(face[0])
             rows[axis].add(0 if face.endswith('lo') else model.grid.shape[axis] - 1)
     return rows
@@ -659,7 +659,10 @@ class Engine2D:
             src.wave = self.waves[k].data_ptr()
             src.wave_len = self.waves[k].numel()
         for f, face in enumerate(('x_lo', 'x_hi', 'y_lo', 'y_hi')):
-            d.wall_sign[f] = WALL_SIGNS.get(model.bc.faces[face], 0)
+            if model.polarization == 'ez':
+                d.wall_sign[f] = WALL_SIGNS.get(model.bc.faces[face], 0)
+            else:   # H_z pmc: H_z = 0 on the outermost cells
+                d.wall_sign[f] = _lib.WALL_ZERO if model.bc.faces[face] == 'pmc' else 0
         d.n_monitors = len(model.monitors)
         d.sample_interval = model.sample_interval
         for k, monitor in enumerate(model.monitors):

@@ -524,8 +524,8 @@ class Yee2D:
         self.sources, self.monitors, self.probes = [], [], []
         self.pec_regions = []
         self.poles = []
-        if polarization == 'hz' and any(kind not in ('cpml', 'periodic') for kind in self.bc.faces.values()):
-            raise ValueError("the oracle's wall and mirror faces are E_z rows")
+        if polarization == 'hz' and any(kind not in ('cpml', 'periodic', 'pmc') for kind in self.bc.faces.values()):
+            raise ValueError("the oracle's H_z walls are pmc faces only")
         self.n_step, self.time = 0, 0.0
         self._finalized = False
         self._build_spacings_and_cpml()
@@ -969,6 +969,12 @@ class Yee2D:
         for source in self.sources:
             if getattr(source, 'field', 'e') == 'h':
                 source.inject(f, t_start)
+        # pmc walls: H_z = 0 on the outermost cells, after the magnetic injections (apply_hz_walls,
+        # boundaries.py:517-527; re-pinned after injection, fdtd_hz_2d.py:165-170)
+        for face, outer in (('x_lo', np.s_[0, :]), ('x_hi', np.s_[-1, :]), ('y_lo', np.s_[:, 0]),
+                            ('y_hi', np.s_[:, -1])):
+            if self.bc.faces[face] == 'pmc':
+                hz[outer] = 0
         # pole updates on the E faces from E^n (fdtd_hz_2d.py:286-310)
         for entry in self.poles:
             for name, field in (('ex', ex), ('ey', ey)):

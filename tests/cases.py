@@ -89,17 +89,23 @@ WALLS2D = dict(size_x=1.2e-6, size_y=1e-6, dx=25e-9, npml=10, faces=('symmetric'
                line=dict(axis='y', position=0.95e-6), wavelength=600e-9, bandwidth=0.6, steps=(150, 400))
 
 
-def build_walls2d(target, boundary_type, **options):
-    """The wall and mirror case on `target` (oracle Yee2D or opensol_amd Simulation)."""
+def build_walls2d(target, boundary_type, polarization='ez', **options):
+    """The wall and mirror case on `target` (oracle Yee2D or opensol_amd Simulation); H_z: x_lo and y_hi pmc, a
+    magnetic line current beside their corner and a 45 degree dipole."""
     spec = WALLS2D
+    faces = spec['faces'] if polarization == 'ez' else ('pmc', 'cpml', 'cpml', 'pmc')
     sim = target(spec['size_x'], spec['size_y'], spec['dx'], npml=spec['npml'],
-                 boundaries=boundary_type(spec['faces'], spec['npml']), polarization='ez', **options)
+                 boundaries=boundary_type(faces, spec['npml']), polarization=polarization, **options)
     (x0, y0, x1, y1), eps = spec['dielectric']
     sim.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
     pulse = dict(wavelength=spec['wavelength'], bandwidth=spec['bandwidth'])
-    x, y, current = spec['point']
-    sim.add_point_source(x, y, current=current, **pulse)
-    sim.add_line_source(**spec['line'], **pulse)
+    if polarization == 'ez':
+        x, y, current = spec['point']
+        sim.add_point_source(x, y, current=current, **pulse)
+        sim.add_line_source(**spec['line'], **pulse)
+    else:
+        sim.add_magnetic_line_current(0.05e-6, 0.95e-6, magnetic_current=2.0, **pulse)
+        sim.add_electric_dipole(0.6e-6, 0.5e-6, orientation=45.0, **pulse)
     return sim
 
 

@@ -2,7 +2,8 @@
 reference's own numpy path (EMPhysics precision='double', CPU backend). It pins the outer-row rules of the E_z
 polarisation end to end: the one ghost cell of a mirror face, the even copy of a 'symmetric' face, the odd copy of an
 'antisymmetric' face, the 'pmc' wall (the outermost window row copies its neighbour, tangential H = 0 on the plane),
-the corner taking both factors, and the re-pin after a source injects into a pinned row.
+the corner taking both factors, and the re-pin after a source injects into a pinned row; and, for H_z, the 'pmc'
+wall (H_z = 0 on the outermost cells, after a magnetic line current beside it).
 
 Case (window 1.2 um x 1 um, dx 25 nm, x_lo 'symmetric', x_hi 'pmc', y_lo 'antisymmetric', y_hi 10-cell CPML): a
 dielectric rectangle, a filament current near the (x_lo, y_lo) corner and a Gaussian line source along y whose span
@@ -30,6 +31,36 @@ LAMBDA0, BANDWIDTH = 600e-9, 0.6
 STEPS = (150, 400)
 
 
+HZ_FACES = ('pmc', 'cpml', 'cpml', 'pmc')
+HZ_MAGNETIC = (0.05e-6, 0.95e-6, 2.0)
+
+
+def run_hz():
+    """The H_z case: x_lo and y_hi 'pmc' (H_z = 0 on the outermost cells), the dielectric, a magnetic line current
+    next to the corner they share and a 45 degree dipole; fields after 150 and 400 steps under hz_* keys."""
+    from open_sol import Simulation
+    from open_sol.core.domain import Domain
+    from open_sol.physics.em import EMPhysics, BoundaryConfig
+
+    bc = BoundaryConfig(*HZ_FACES)
+    bc.cpml_npml = NPML
+    domain = Domain.uniform(WINDOW_X, WINDOW_Y, DX, npml=NPML, boundary_config=bc, verbose=False, backend='cpu')
+    sim = Simulation.multiphysics(domain)
+    em = sim.add_physics('em', EMPhysics(domain, polarization='hz', boundary_config=bc, precision='double'))
+    (x0, y0, x1, y1), eps = DIELECTRIC
+    em.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
+    x, y, km = HZ_MAGNETIC
+    em.add_magnetic_line_current(x, y, wavelength=LAMBDA0, bandwidth=BANDWIDTH, magnetic_current=km)
+    em.add_electric_dipole(0.6e-6, 0.5e-6, wavelength=LAMBDA0, bandwidth=BANDWIDTH, orientation=45.0)
+    out, done = {}, 0
+    for target in STEPS:
+        sim.run(n_steps=target - done, print_every=0, flush=False)
+        done = target
+        for name in ('hz', 'ex', 'ey'):
+            out[f'hzcase_{name}_{target}'] = np.array(getattr(em.fields, name), dtype=np.float64)
+    return out
+
+
 def main():
     sys.path.insert(0, REFERENCE)
     from open_sol import Simulation
@@ -55,6 +86,7 @@ def main():
             out[f'{name}_{target}'] = np.array(getattr(em.fields, name), dtype=np.float64)
     engine = em.engine
     out.update({'nx': engine.nx, 'ny': engine.ny, 'dt': float(engine.dt)})
+    out.update(run_hz())
     path = os.path.join(FIXTURES, 'walls2d_ez.npz')
     np.savez_compressed(path, **out)
     ez = out['ez_400']

@@ -213,7 +213,7 @@ def test_wall_faces_are_e_z_only_and_padded_like_the_oracle():
     the descriptor's wall factors; a flux line touching a pmc row is refused."""
     from opensol_amd.boundaries import BoundaryConfig
     from oracle.yee2d import FACES, Boundaries
-    for kind in ('pec', 'pmc', 'symmetric', 'antisymmetric'):
+    for kind in ('pec', 'symmetric', 'antisymmetric'):   # H_z takes 'pmc' only
         with pytest.raises(ValueError, match='E_z polarisation only'):
             Model2D(1e-6, 1e-6, 25e-9, npml=5, boundaries=BoundaryConfig(kind, 'cpml', 'cpml', 'cpml'),
                     polarization='hz')

@@ -160,6 +160,14 @@ def test_oracle_reproduces_reference_walls_and_mirrors():
             np.testing.assert_allclose(values, want, rtol=0, atol=1e-12 * np.abs(want).max())
     ez = golden['ez_400']
     assert np.array_equal(ez[0], ez[1]) and np.array_equal(ez[:, 0], -ez[:, 1]) and np.abs(ez[0]).max() > 0
+    hz_case = build_walls2d(Yee2D, _oracle_boundaries, 'hz')   # H_z pmc walls, a magnetic current beside them
+    done = 0
+    for step in WALLS2D['steps']:
+        hz_case.run(n_steps=step - done)
+        done = step
+        for name, values in hz_case.fields.items():
+            want = golden[f'hzcase_{name}_{step}']
+            np.testing.assert_allclose(values, want, rtol=0, atol=1e-12 * np.abs(want).max())
 
 
 @pytest.mark.parametrize('polarization', ['ez', 'hz'])
