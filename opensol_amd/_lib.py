@@ -6,7 +6,7 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 11
+ABI_VERSION = 12
 MAX_SOURCES = 8
 SOURCE_E, SOURCE_H = 0, 1   # yee_line_source.field
 WALL_ZERO = 2   # yee2d_desc.wall_sign of an H_z pmc face
@@ -117,6 +117,7 @@ SIGNATURES = (
     ('yee_step', c_i32, (c_ptr, c_i64, c_ptr)),
     ('yee_step_timed', c_i32, (c_ptr, c_i64, c_ptr, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(c_i64))),
     ('yee_set_graph_steps', c_i32, (c_ptr, c_i32)),
+    ('yee_build_graph', c_i32, (c_ptr,)),
     ('yee2d_update', c_i32, (c_ptr, ctypes.POINTER(Desc2DC))),
     ('yee2d_get_desc', c_i32, (c_ptr, ctypes.POINTER(Desc2DC))),
     ('yee3d_create', c_i32, (ctypes.POINTER(Desc3DC), ctypes.POINTER(c_ptr))),

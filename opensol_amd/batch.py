@@ -74,6 +74,7 @@ class _Batch:
                                 ctypes.byref(self._ctx)), self._create_name)
         if self.graph_steps:
             _lib.check(self.lib.yee_set_graph_steps(self._ctx, self.graph_steps), 'yee_set_graph_steps')
+            _lib.check(self.lib.yee_build_graph(self._ctx), 'yee_build_graph')   # capture here, not in launch()
         return longest
 
     def _cached_descriptor(self, engine):

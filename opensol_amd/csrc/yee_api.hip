@@ -692,6 +692,13 @@ int32_t yee_set_graph_steps(yee_ctx* ctx, int32_t graph_steps)
     return YEE_OK;
 }
 
+int32_t yee_build_graph(yee_ctx* ctx)
+{
+    if (!ctx || !ctx->kind) return fail(YEE_ESTATE, "null or destroyed context");
+    if (ctx->graph_steps <= 0 || ctx->graph) return YEE_OK;
+    return build_graph(ctx);
+}
+
 // Cells in the bulk tile set: the non-edge tile rows times the non-edge tile columns (the set is their product).
 int64_t bulk_cells(const yee2d_desc& d, const yee2d_tile_sets& s)
 {

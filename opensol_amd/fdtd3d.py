@@ -535,6 +535,14 @@ class Engine3D:
     def stream_handle(self):
         return self.torch.cuda.current_stream(self.device).cuda_stream
 
+    def prepare(self, n_steps):
+        """Grow the device tables to cover the next n_steps steps and instantiate the step graph now, so that stepping
+        them afterwards only launches (no table upload, no capture)."""
+        if self.model.materials.version != self._materials_version:
+            self.materials_changed()
+        self._ensure_capacity(self.n_step + int(n_steps))
+        _lib.check(self.lib.yee_build_graph(self._ctx), 'yee_build_graph')
+
     def step(self, n_steps):
         """Enqueue n_steps steps on the current stream (asynchronous)."""
         n_steps = int(n_steps)

@@ -5,9 +5,6 @@
         (KiB per dispatch) to HBM bytes for the Yee kernels.
     python -m opensol_amd.profiling summarize <counter_collection.csv> [kernel substring]
         per-kernel mean of each counter over the dispatches in a rocprofv3 --pmc output file.
-    python -m opensol_amd.profiling stream [--build-only]
-        builds csrc/stream_probe.hip (hipcc, gfx950) into opensol_amd/stream_probe and runs it: the HBM bandwidth a
-        plain 3-in / 3-out streaming kernel reaches, the practical roofline of the Yee kernels' read/write mix.
 """
 import argparse
 import csv
@@ -49,39 +46,18 @@ def summarize(path, kernel_filter=None):
               f'dispatches  {name}')
 
 
-def stream_probe(build_only=False):
-    import os
-    import subprocess
-    from .build import ARCH, hipcc
-    here = os.path.dirname(os.path.abspath(__file__))
-    source = os.path.join(here, 'csrc', 'stream_probe.hip')
-    binary = os.path.join(here, 'stream_probe')
-    cmd = [hipcc(), '-O3', f'--offload-arch={ARCH}', '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
-           source, '-o', binary]
-    result = subprocess.run(cmd, capture_output=True, text=True)
-    if result.returncode != 0:
-        raise RuntimeError(f'stream_probe build failed:\n{result.stderr}')
-    if build_only:
-        return 0
-    return subprocess.run([binary]).returncode
-
-
 def main(argv=None):
     parser = argparse.ArgumentParser()
     sub = parser.add_subparsers(dest='command', required=True)
     copy = sub.add_parser('copy')
     copy.add_argument('--mib', type=int, default=1024)
     copy.add_argument('--repeat', type=int, default=5)
-    stream = sub.add_parser('stream')
-    stream.add_argument('--build-only', action='store_true')
     summary = sub.add_parser('summarize')
     summary.add_argument('path')
     summary.add_argument('kernel', nargs='?')
     args = parser.parse_args(argv)
     if args.command == 'copy':
         copy_probe(args.mib, args.repeat)
-    elif args.command == 'stream':
-        return stream_probe(args.build_only)
     else:
         summarize(args.path, args.kernel)
 

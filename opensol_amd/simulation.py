@@ -67,6 +67,26 @@ class Simulation:
     def tick(self):
         return self.n_step
 
+    # ---- physics access (simulation.py:250-270) -------------------------------------------------------------------
+    @property
+    def em(self):
+        """The one attached physics: the 2D EM model (the reference's sim.em)."""
+        return self.model
+
+    def __getattr__(self, name):
+        # Only reached when normal lookup fails: names of the EM model (the sole physics) fall through to it. Dunder
+        # lookups (copy / pickle probes) never do.
+        if name.startswith('__'):
+            raise AttributeError(name)
+        model = self.__dict__.get('model')
+        if model is None:
+            raise AttributeError(f"'Simulation' has no attribute '{name}' and no physics are attached.")
+        try:
+            return getattr(model, name)
+        except AttributeError:
+            raise AttributeError(f"'Simulation' has no attribute '{name}' and neither has its only physics "
+                                 f"'em'.") from None
+
     # ---- model building -------------------------------------------------------------------------------------------
     def add_dielectric_rectangle(self, x0, y0, x1, y1, eps_r, sigma_e=0.0):
         self.model.add_dielectric_rectangle(x0, y0, x1, y1, eps_r, sigma_e)
@@ -265,6 +285,24 @@ class Simulation3D:
     @property
     def time(self):
         return 0.0 if self.engine is None else self.engine.time
+
+    @property
+    def em3d(self):
+        """The one attached physics: the 3D EM model."""
+        return self.model
+
+    def __getattr__(self, name):
+        # names of the 3D EM model (the sole physics) fall through to it, as Simulation's do (simulation.py:250-270)
+        if name.startswith('__'):
+            raise AttributeError(name)
+        model = self.__dict__.get('model')
+        if model is None:
+            raise AttributeError(f"'Simulation3D' has no attribute '{name}' and no physics are attached.")
+        try:
+            return getattr(model, name)
+        except AttributeError:
+            raise AttributeError(f"'Simulation3D' has no attribute '{name}' and neither has its only physics "
+                                 f"'em3d'.") from None
 
     def add_dielectric_box(self, lo, hi, eps_r, sigma_e=0.0):
         self.model.add_dielectric_box(lo, hi, eps_r, sigma_e)
