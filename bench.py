@@ -52,11 +52,13 @@ def window_for_cells(n_total, npml, dx):
     raise ValueError(f'no window size gives {n_total} cells')
 
 
-def build_ez_workload(n=8192, npml=20, dx=20e-9, rectangles=400, seed=SEED, **options):
-    """The 2D E_z roofline workload as a Simulation (not yet finalized) plus its description."""
-    from opensol_amd import Simulation
+def build_ez_workload(n=8192, npml=20, dx=20e-9, rectangles=400, seed=SEED, target=None, **options):
+    """The 2D E_z roofline workload as a Simulation (not yet finalized); `target` builds the same model on another
+    class with the Simulation constructor and methods (the oracle's Yee2D in the parity tests)."""
+    if target is None:
+        from opensol_amd import Simulation as target
     size = window_for_cells(n, npml, dx)
-    sim = Simulation(size, size, dx, npml=npml, **options)
+    sim = target(size, size, dx, npml=npml, **options)
     rng = np.random.default_rng(seed)
     for _ in range(rectangles):
         cx, cy = rng.uniform(0, size, 2)
@@ -64,20 +66,23 @@ def build_ez_workload(n=8192, npml=20, dx=20e-9, rectangles=400, seed=SEED, **op
         eps = rng.uniform(1.0, 12.0)
         sim.add_dielectric_rectangle(cx - wx / 2, cy - wy / 2, cx + wx / 2, cy + wy / 2, eps_r=eps)
     sim.add_line_source(axis='y', position=2e-6, wavelength=1.55e-6, bandwidth=0.5)
-    assert sim.model.shape == (n, n), sim.model.shape
+    shape = sim.model.shape if hasattr(sim, 'model') else (sim.nx, sim.ny)
+    assert shape == (n, n), shape
     return sim
 
 
-def build_batch_instances(count=256, seed=SEED, **options):
-    """The batched-2D workload: (simulations, steps per instance)."""
-    from opensol_amd import Simulation
+def build_batch_instances(count=256, seed=SEED, target=None, **options):
+    """The batched-2D workload: (simulations, steps per instance); the first k instances of a count are the same for
+    every count. `target` as in build_ez_workload."""
+    if target is None:
+        from opensol_amd import Simulation as target
     rng = np.random.default_rng(seed)
     dx, npml = 25e-9, 12
     sims, steps = [], []
     for k in range(count):
         nx, ny = (int(round(np.exp(rng.uniform(np.log(128), np.log(1024))))) for _ in range(2))
         sx, sy = window_for_cells(nx, npml, dx), window_for_cells(ny, npml, dx)
-        sim = Simulation(sx, sy, dx, npml=npml, polarization='ez' if k % 2 == 0 else 'hz', **options)
+        sim = target(sx, sy, dx, npml=npml, polarization='ez' if k % 2 == 0 else 'hz', **options)
         for _ in range(int(rng.integers(20, 101))):
             cx, cy = rng.uniform(0, sx), rng.uniform(0, sy)
             wx, wy = rng.uniform(0.05, 0.4) * sx, rng.uniform(0.05, 0.4) * sy
@@ -175,11 +180,13 @@ def run_serve(args, torch):
             'rates': rates, 'roofline': None, 'cpu_baseline': None}
 
 
-def build_3d_workload(n=512, npml=12, dx=25e-9, boxes=200, seed=SEED, **options):
-    """The 3D roofline workload as a Simulation3D (not yet finalized)."""
-    from opensol_amd import Simulation3D
+def build_3d_workload(n=512, npml=12, dx=25e-9, boxes=200, seed=SEED, target=None, **options):
+    """The 3D roofline workload as a Simulation3D (not yet finalized); `target` as in build_ez_workload (the
+    oracle's Yee3D)."""
+    if target is None:
+        from opensol_amd import Simulation3D as target
     size = window_for_cells(n, npml, dx)
-    sim = Simulation3D(size, size, size, dx, npml=npml, **options)
+    sim = target(size, size, size, dx, npml=npml, **options)
     rng = np.random.default_rng(seed)
     for _ in range(boxes):
         centre = rng.uniform(0, size, 3)
@@ -189,7 +196,8 @@ def build_3d_workload(n=512, npml=12, dx=25e-9, boxes=200, seed=SEED, **options)
     sim.add_dipole(c, c, c, orientation='z', dipole_moment=1e-12, wavelength=1e-6, bandwidth=0.5)
     for x in (c - 4e-6, c + 4e-6):
         sim.add_flux_plane('x', x, wavelengths=[0.8e-6, 0.9e-6, 1.0e-6, 1.1e-6])
-    assert sim.model.shape == (n, n, n), sim.model.shape
+    shape = sim.model.shape if hasattr(sim, 'model') else sim.shape
+    assert tuple(shape) == (n, n, n), shape
     return sim
 
 

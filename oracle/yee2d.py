@@ -124,13 +124,16 @@ class Geometry2D:
         window = self.windows[axis]
         return containing_cell(coord, self.edges[axis], window.start, window.stop - 1)
 
-    def rect_mask(self, x0, y0, x1, y1):
-        """Cells whose float32 centre lies in [x_min, x_max) x [y_min, y_max) (core/domain.py:255-260)."""
+    def rect_axes(self, x0, y0, x1, y1):
+        """The two axis factors of rect_mask: (cells along x, cells along y) whose float32 centre lies in the range."""
         cx, cy = (c.astype(F32) for c in self.centres)
         x_min, x_max = F32(min(x0, x1)), F32(max(x0, x1))
         y_min, y_max = F32(min(y0, y1)), F32(max(y0, y1))
-        mx = (cx >= x_min) & (cx < x_max)
-        my = (cy >= y_min) & (cy < y_max)
+        return (cx >= x_min) & (cx < x_max), (cy >= y_min) & (cy < y_max)
+
+    def rect_mask(self, x0, y0, x1, y1):
+        """Cells whose float32 centre lies in [x_min, x_max) x [y_min, y_max) (core/domain.py:255-260)."""
+        mx, my = self.rect_axes(x0, y0, x1, y1)
         return mx[:, None] & my[None, :]
 
 
@@ -562,11 +565,18 @@ class Yee2D:
 
     def add_dielectric_rectangle(self, x0, y0, x1, y1, eps_r, sigma_e=0.0):
         """physics/em/__init__.py:317-327, core/material_grid.py:56-83 (later writes win)."""
-        mask = self.geo.rect_mask(x0, y0, x1, y1)
+        # the mask mx x my is a block of cells (centres increase along each axis): write it through index ranges,
+        # which sets the same cells as the boolean mask in a fraction of the time on large grids
+        mx, my = self.geo.rect_axes(x0, y0, x1, y1)
+        rows, columns = np.nonzero(mx)[0], np.nonzero(my)[0]
+        if rows.size and columns.size:
+            block = (slice(rows[0], rows[-1] + 1), slice(columns[0], columns[-1] + 1))
+        else:
+            block = (slice(0, 0), slice(0, 0))
         names = ('eps_r',) if self.polarization == 'ez' else ('eps_xx', 'eps_yy')
         for name in names:
-            self.materials[name][mask] = F32(eps_r)
-        self.materials['sigma_e'][mask] = F32(sigma_e)
+            self.materials[name][block] = F32(eps_r)
+        self.materials['sigma_e'][block] = F32(sigma_e)
         self._finalized = False
 
     def add_drude_rectangle(self, x0, y0, x1, y1, omega_p, gamma, eps_inf=1.0):

@@ -229,3 +229,61 @@ def build_poles3d(target, **options):
     sim.add_dipole(*spec['dipole'], orientation='z', dipole_moment=1e-12, wavelength=0.6e-6, bandwidth=0.5)
     plane = sim.add_flux_plane('x', spec['plane_x'], wavelengths=spec['wavelengths'])
     return sim, plane
+
+
+# 2D cases with conductive media and the ricker / cw waveforms (tests/golden/make_lossy_golden.py writes them from the
+# reference): three rectangles with sigma_e > 0 painted in order (the first one around the Ricker line source, so its
+# sheet calibration sees sigma_e), a cw line source over part of a line, a flux line behind the slab.
+LOSSY2D = dict(size_x=2e-6, size_y=1e-6, dx=25e-9, npml=10, faces=('cpml', 'cpml', 'periodic', 'periodic'),
+               rectangles=((0.3e-6, 0.0, 0.7e-6, 1e-6, 2.0, 5e3), (0.9e-6, 0.0, 1.5e-6, 1e-6, 3.0, 2e4),
+                           (1.1e-6, 0.2e-6, 1.3e-6, 0.6e-6, 1.5, 3e5)),
+               sources=(dict(axis='y', position=0.5e-6, wavelength=600e-9, waveform='ricker'),
+                        dict(axis='y', position=0.35e-6, wavelength=700e-9, waveform='cw', amplitude=0.3,
+                             span=(0.2e-6, 0.7e-6))),
+               monitor=dict(axis='y', position=1.7e-6, span=(0.0, 1e-6), wavelengths=[500e-9, 600e-9, 700e-9]),
+               steps=(150, 500), waveform_samples=600)
+
+
+def build_lossy2d(target, polarization, boundary_type, **options):
+    """The conductive-media case on `target` (oracle Yee2D with oracle Boundaries, or Simulation with BoundaryConfig);
+    returns (sim, sources, monitor)."""
+    spec = LOSSY2D
+    sim = target(spec['size_x'], spec['size_y'], spec['dx'], npml=spec['npml'],
+                 boundaries=boundary_type(spec['faces'], spec['npml']), polarization=polarization, **options)
+    for x0, y0, x1, y1, eps, sigma in spec['rectangles']:
+        sim.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps, sigma_e=sigma)
+    sources = [sim.add_line_source(**source) for source in spec['sources']]
+    monitor = sim.add_flux_monitor(**spec['monitor'])
+    return sim, sources, monitor
+
+
+# 3D case with conductive boxes, a Ricker z dipole and a cw x dipole (tests/golden/make_lossy_golden.py).
+LOSSY3D = dict(window=(0.6e-6, 0.5e-6, 0.45e-6), dx=25e-9, npml=8,
+               boxes=(((0.05e-6, 0.05e-6, 0.05e-6), (0.35e-6, 0.30e-6, 0.25e-6), 3.0, 2e4),
+                      ((0.30e-6, 0.20e-6, 0.20e-6), (0.50e-6, 0.45e-6, 0.40e-6), 2.0, 3e5)),
+               dipoles=(((0.27e-6, 0.24e-6, 0.22e-6), 'z', 1e-12, 0.6e-6, 'ricker'),
+                        ((0.20e-6, 0.30e-6, 0.30e-6), 'x', 5e-13, 0.7e-6, 'cw')),
+               plane_x=0.45e-6, wavelengths=(0.5e-6, 0.6e-6, 0.7e-6), n_steps=250)
+
+
+def lossy3d_masks(centres):
+    """One mask per LOSSY3D box: cells whose float32 centre (clamped to the window) lies in the half-open box."""
+    return [poles3d_box_mask(centres, lo, hi) for lo, hi, _, _ in LOSSY3D['boxes']]
+
+
+def build_lossy3d(target, **options):
+    """The 3D conductive case on the oracle Yee3D or on opensol_amd.Simulation3D; returns (sim, dipoles, plane)."""
+    spec = LOSSY3D
+    sim = target(*spec['window'], spec['dx'], npml=spec['npml'], **options)
+    if hasattr(sim, 'host_centres'):   # the oracle: paint the arrays as the golden script does
+        for mask, (_, _, eps, sigma) in zip(lossy3d_masks(sim.host_centres()), spec['boxes']):
+            sim.materials['eps_r'][mask] = np.float32(eps)
+            sim.materials['sigma_e'][mask] = np.float32(sigma)
+    else:
+        for lo, hi, eps, sigma in spec['boxes']:
+            sim.add_dielectric_box(lo, hi, eps, sigma_e=sigma)
+    dipoles = [sim.add_dipole(*position, orientation=orientation, dipole_moment=moment, wavelength=wavelength,
+                              bandwidth=0.5, waveform=waveform)
+               for position, orientation, moment, wavelength, waveform in spec['dipoles']]
+    plane = sim.add_flux_plane('x', spec['plane_x'], wavelengths=list(spec['wavelengths']))
+    return sim, dipoles, plane

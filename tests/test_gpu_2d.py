@@ -82,8 +82,11 @@ def _same_model(target, polarization, case, rng_seed, **options):
     sim = target(wx * dx, wy * dx, dx, npml=max(npml, 1), boundaries=boundaries, polarization=polarization,
                  **options)
     rng = np.random.default_rng(rng_seed)
-    for x0, y0, x1, y1, eps in random_rectangles(rng, wx * dx, wy * dx, 6):
-        sim.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
+    # every other rectangle conductive (sigma_e up to 6e4 S/m): the semi-implicit E coefficients and, for H_z, the
+    # float32 face means of eps and sigma across two ids
+    sigmas = np.random.default_rng(rng_seed + 7).uniform(0.0, 6e4, 6)
+    for k, (x0, y0, x1, y1, eps) in enumerate(random_rectangles(rng, wx * dx, wy * dx, 6)):
+        sim.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps, sigma_e=float(sigmas[k]) if k % 2 else 0.0)
     sim.add_line_source(axis='x', position=0.4 * wy * dx, wavelength=700e-9, bandwidth=0.6)
     sim.add_line_source(axis='y', position=0.6 * wx * dx, wavelength=500e-9, bandwidth=0.4, amplitude=0.5)
     sim.add_flux_monitor(axis='y', position=0.7 * wx * dx, span=(0.1 * wy * dx, 0.9 * wy * dx),
@@ -190,6 +193,35 @@ def test_pec_case_matches_oracle(gpu, polarization):
     flux_error = np.max(np.abs(monitor.flux() / ref_monitor.flux() - 1.0))
     print(f'{polarization} PEC flux rel error {flux_error:.3e}')
     assert flux_error <= 1e-5
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_lossy_case_matches_oracle(gpu, polarization):
+    """Conductive rectangles (sigma_e 5e3 to 3e5 S/m; the Ricker source line inside one), a cw line source over part of
+    a line and a flux line sampled every step, against the FP64 oracle (which reproduces the reference's own run to
+    1e-12): fields <= 1e-5 relative L2 at both steps, flux <= 1e-5; graph replay bit-identical to eager."""
+    from tests.cases import LOSSY2D, build_lossy2d
+    Simulation = simulation_class()
+    ref, _, ref_monitor = build_lossy2d(Yee2D, polarization, _oracle_boundaries, phase_dtype=np.complex128)
+    sim, _, monitor = build_lossy2d(Simulation, polarization, _product_boundaries)
+    graphed, _, graphed_monitor = build_lossy2d(Simulation, polarization, _product_boundaries, graph_steps=16)
+    done = 0
+    for step in LOSSY2D['steps']:
+        print(f'lossy {polarization} to step {step}', flush=True)
+        ref.run(n_steps=step - done)
+        sim.run(n_steps=step - done, print_every=0)
+        graphed.run(n_steps=step - done, print_every=0)
+        done = step
+        for name in sim.model.components:
+            error = rel_l2(sim.engine.field(name), ref.fields[name])
+            print(f'lossy {polarization} step {step} {name}: rel L2 {error:.3e}')
+            assert error <= FIELD_TOL, (step, name, error)
+            np.testing.assert_array_equal(graphed.engine.field(name), sim.engine.field(name))
+    assert monitor.sample_interval == ref_monitor.sample_interval == 1
+    flux_error = np.max(np.abs(monitor.flux() / ref_monitor.flux() - 1.0))
+    print(f'lossy {polarization} flux rel error {flux_error:.3e}')
+    assert flux_error <= 1e-5
+    np.testing.assert_array_equal(graphed_monitor.flux(), monitor.flux())
 
 
 @pytest.mark.parametrize('polarization', ['ez', 'hz'])
