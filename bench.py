@@ -373,7 +373,7 @@ def run_features(args, torch):
     per_step = elapsed_s / args.steps
     m = sim.model.sample_interval
     dft_cells = box.mx * box.my
-    dft_bytes = dft_cells * (4 + 32 * box.n_freqs)   # E_z read, (re, im) FP64 read + write per frequency
+    dft_bytes = dft_cells * (4 + 64 * box.n_freqs)   # E_z read, Kahan (sum, comp) FP64 (re, im) read + write per frequency
     print(f'# features: {per_step * 1e3:.4f} ms/step against {plain_s * 1e3:.4f} plain; DFT box {box.mx}x{box.my} x '
           f'{box.n_freqs} every {m} steps ({dft_bytes / 1e9:.2f} GB per sample)', file=sys.stderr, flush=True)
     value = cells * args.steps / elapsed_s

@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 12
+#define YEE_ABI_VERSION 13
 #define YEE_MAX_SOURCES 8
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
@@ -148,7 +148,8 @@ typedef struct yee_dft_region {
     int32_t pair_wrap[3];
     const double* phase;     /* [sample s][n_freq][2]: exp(-j omega t) m dt at t = t_{(s+1) m} */
     int64_t n_samples;
-    double* acc;             /* [n_comp][n_freq][count0 * count1 * count2][2 (re, im)] */
+    double* acc;             /* [n_comp][n_freq][count0 * count1 * count2][4 (sum re, sum im, comp re, comp im)]:
+                                FP64 Kahan pairs; the spectrum is sum - comp */
 } yee_dft_region;
 
 /*

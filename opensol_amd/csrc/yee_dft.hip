@@ -4,10 +4,11 @@
 // sample costs a few microseconds however large the box is.
 //
 // On sample step n ((n + 1) % m == 0, sample s = (n + 1) / m - 1) each lane reads its components from the new state
-// set (one float each, or two for a face pair averaged to the cell centre) and adds v * phase[s][f] to its (re, im)
-// accumulator of every frequency f. Accumulators are laid out [component][frequency][cell][2], so a wavefront reads
-// and writes 64 consecutive 16-byte records: the kernel streams 32 B per cell, component and frequency plus the field
-// reads, and is HBM bound on the sample steps.
+// set (one float each, or two for a face pair averaged to the cell centre) and adds v * phase[s][f] to its FP64 Kahan
+// (sum, compensation) pairs of the real and imaginary parts at every frequency f. Accumulators are laid out
+// [component][frequency][cell][4 (sum re, sum im, comp re, comp im)], so a wavefront reads and writes 64 consecutive
+// 32-byte records: the kernel streams 64 B per cell, component and frequency plus the field reads, and is HBM bound
+// on the sample steps.
 //
 // The descriptor is read in place at the kernel-argument pointer (constant address space): the field table is
 // indexed by the runtime component, which would otherwise copy the whole descriptor to private memory.
@@ -20,6 +21,16 @@
 #include <hip/hip_runtime.h>
 
 namespace {
+
+// FP64 Kahan-compensated accumulation of x into (sum, comp) held in registers: the true running sum is sum - comp
+// (Kahan 1965; hipcc compiles without fast-math, so the compensation survives).
+__device__ __forceinline__ void kahan(double& sum, double& comp, double x)
+{
+    const double y = x - comp;
+    const double t = sum + y;
+    comp = (t - sum) - y;
+    sum = t;
+}
 
 // Accumulate one sampled cell `t` of region `r`. field: the new state set's component pointers; n_axis: grid extent
 // per axis; axis_stride: element stride per axis.
@@ -46,13 +57,13 @@ __device__ __forceinline__ void dft_cell(const yee_dft_region& r, float* const* 
                                                                  : cell);
             v = 0.5f * (v + f[lower]);
         }
-        double2* acc = reinterpret_cast<double2*>(r.acc) + (int64_t)c * r.n_freq * cells + t;
+        double4* acc = reinterpret_cast<double4*>(r.acc) + (int64_t)c * r.n_freq * cells + t;
         const double x = (double)v;
         for (int q = 0; q < r.n_freq; ++q) {
-            double2 sum = acc[(int64_t)q * cells];
-            sum.x += x * phase[2 * q];
-            sum.y += x * phase[2 * q + 1];
-            acc[(int64_t)q * cells] = sum;
+            double4 a = acc[(int64_t)q * cells];   // (sum re, sum im, comp re, comp im)
+            kahan(a.x, a.z, x * phase[2 * q]);
+            kahan(a.y, a.w, x * phase[2 * q + 1]);
+            acc[(int64_t)q * cells] = a;
         }
     }
 }

@@ -80,6 +80,29 @@ class MaterialMap:
         self.ids[mask] = material
         self.version += 1
 
+    def set_property(self, columns, values):
+        """Write the per-cell float32 array `values` into material columns `columns` (e.g. (0, 1, 2) for an isotropic
+        eps_r, (3,) for sigma_e) of every cell, as a direct write into the reference's material arrays followed by
+        materials_changed does (core/material_grid.py:38-83). Each distinct (old material, new value) pair becomes one
+        id; PEC cells stay PEC."""
+        values = np.broadcast_to(np.asarray(values, dtype=DTYPE), self.shape)
+        key = (self.ids.astype(np.uint64) << np.uint64(32)) | values.view(np.uint32).astype(np.uint64)
+        unique, inverse = np.unique(key.ravel(), return_inverse=True)
+        pec = self.pec_flags()
+        mapped = np.empty(unique.size, dtype=np.uint16)
+        for u, packed in enumerate(unique):
+            old = int(packed >> np.uint64(32))
+            if pec[old]:
+                mapped[u] = old
+                continue
+            value = np.array([int(packed & np.uint64(0xFFFFFFFF))], dtype=np.uint32).view(DTYPE)[0]
+            entry = list(self.materials[old])
+            for column in columns:
+                entry[column] = value
+            mapped[u] = self.material_id(*entry)
+        self.ids = mapped[inverse].reshape(self.shape)
+        self.version += 1
+
     def values(self):
         """(n_materials, 6) float64 array of the float32 material values."""
         return np.array(self.materials, dtype=np.float64)

@@ -255,8 +255,8 @@ def dft_sample_slice(start, stop, stride):
 
 
 class _DFTRegion:
-    """Device record of a running DFT over a box of sampled cells (the yee_dft_region record): FP64 (re, im)
-    accumulators [component][frequency][cells], phases exp(-j omega t) m dt at the sample times (the flux monitors'
+    """Device record of a running DFT over a box of sampled cells (the yee_dft_region record): FP64 Kahan (sum re,
+    sum im, compensation re, compensation im) accumulators [component][frequency][cells], phases exp(-j omega t) m dt at the sample times (the flux monitors'
     tables). Subclasses name the components and their pairing."""
 
     names = ()
@@ -278,7 +278,7 @@ class _DFTRegion:
         return tuple(len(range(s.start, s.stop, s.step)) for s in self.samples)
 
     def accumulator_shape(self):
-        return (len(self.names), self.n_freqs) + self.sample_counts + (2,)
+        return (len(self.names), self.n_freqs) + self.sample_counts + (4,)
 
     def phase_table(self, sample_times):
         phase = -np.outer(np.asarray(sample_times, dtype=np.float64), self.omega)
@@ -308,7 +308,7 @@ class _DFTRegion:
         if self.engine is None:
             return np.zeros(self.accumulator_shape()[:-1], dtype=np.complex128)
         acc = self.engine.read_dft_accumulator(self.index)
-        return acc[..., 0] + 1j * acc[..., 1]
+        return (acc[..., 0] - acc[..., 2]) + 1j * (acc[..., 1] - acc[..., 3])
 
 
 class DFTMonitor(_DFTRegion):

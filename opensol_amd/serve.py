@@ -12,18 +12,26 @@ device state fits the HBM budget, then stay resident until they finish. Work adv
 A finished job is read out (its monitors' flux spectra), its buffers are freed and its latency (arrival to result) is
 recorded. The plain form, for comparison, runs the same jobs one at a time in arrival order.
 
+Failure handling. A job whose device state alone exceeds the HBM budget is refused at arrival (status 'rejected',
+with the reason in job.error), as the reference's memory budget refuses an over-sized model instead of spilling
+(core/gpu_memory.py:177, 271-290). Job.cancel() may be called from any thread; the server honours it at the next
+chunk boundary, like the reference's RunControl.checkpoint between study steps (model/study.py:194-251): a pending
+job is dropped, a running one stops after the chunk in flight and its state is freed (status 'cancelled'). A job
+whose spectra come out non-finite is reported as 'failed'.
+
 Every job's result is the same, bit for bit, whether it runs alone or in the mix: the batched launches and the
 single-instance kernels give identical bits, and every instance keeps its own clock.
 
 The reference has no counterpart: it runs one model per call (model/study.py:437-447).
 """
+import threading
 import time
 from dataclasses import dataclass, field
 
 import numpy as np
 
 
-@dataclass
+@dataclass(eq=False)
 class Job:
     """One FDTD job: `build()` returns a Simulation / Simulation3D with its sources and monitors, `steps` the steps
     to run, `arrival` the arrival time [s] after the stream starts."""
@@ -37,6 +45,17 @@ class Job:
     finished: float = None
     cells: int = 0
     result: list = field(default_factory=list)
+    status: str = 'pending'   # pending, running, done, rejected, cancelled, failed
+    error: str = ''
+    _cancel: threading.Event = field(default_factory=threading.Event, repr=False)
+
+    def cancel(self):
+        """Ask the server to drop this job at its next chunk boundary (thread-safe)."""
+        self._cancel.set()
+
+    @property
+    def cancelled(self):
+        return self._cancel.is_set()
 
     @property
     def latency(self):
@@ -65,6 +84,17 @@ def _readout(sim):
     return [np.asarray(m.flux()) for m in sim.model.monitors]
 
 
+def _finish(job, now):
+    """Read out a job that ran all its steps; non-finite spectra mark it failed."""
+    job.result = _readout(job.sim)
+    job.finished = now
+    if all(np.all(np.isfinite(r)) for r in job.result):
+        job.status = 'done'
+    else:
+        job.status = 'failed'
+        job.error = 'non-finite flux spectra (unstable or diverging fields)'
+
+
 class FDTDServer:
     """Continuous-batching server over one GPU. hbm_budget [B] bounds the resident device state; chunk_steps is the
     granularity at which jobs join and leave."""
@@ -77,16 +107,28 @@ class FDTDServer:
         import torch
         self.torch = torch
 
-    def _admit(self, pending, active, start, clock):
-        """Admit every arrived job whose state fits the budget (at least one job when none is active). The new
-        engines are created and zeroed on the current stream."""
+    def _admit(self, pending, active, start, clock, closed):
+        """Admit every arrived job whose state fits the budget. A job larger than the whole budget is refused and a
+        cancelled one dropped (both go to `closed`). The new engines are created and zeroed on the current stream."""
         now = clock() - start
         while pending and pending[0].arrival <= now:
             job = pending[0]
+            if job.cancelled:
+                pending.pop(0)
+                job.status, job.finished = 'cancelled', now
+                closed.append(job)
+                continue
             if job.sim is None:
                 job.sim = job.build()
             need = state_bytes(job.sim)
-            if active and self._used + need > self.hbm_budget:
+            if need > self.hbm_budget:
+                pending.pop(0)
+                job.sim = None
+                job.status, job.finished = 'rejected', now
+                job.error = f'device state {need:.4g} B exceeds the HBM budget {self.hbm_budget:.4g} B'
+                closed.append(job)
+                continue
+            if self._used + need > self.hbm_budget:
                 break
             pending.pop(0)
             job.sim.finalize()
@@ -95,11 +137,20 @@ class FDTDServer:
             job.cells = int(np.prod(job.sim.model.shape))
             job.admitted = now
             job.bytes = need
+            job.status = 'running'
             self._used += need
             active.append(job)
 
-    def run(self, jobs, clock=time.perf_counter):
-        """Serve `jobs` (sorted by arrival); returns the stats dict. Arrivals are real time from the call.
+    def _release(self, job, active, done):
+        job.sim.close()
+        job.sim = None
+        self._used -= job.bytes
+        active.remove(job)
+        done.append(job)
+
+    def run(self, jobs, clock=time.perf_counter, on_chunk=None):
+        """Serve `jobs` (sorted by arrival); returns the stats dict. Arrivals are real time from the call. on_chunk, if
+        given, is called with the list of running jobs at every chunk boundary (a caller may cancel jobs there).
 
         Each chunk enqueues one batched launch per step for the admitted 2D jobs and one for the admitted 3D jobs, on
         two streams so the two batches overlap on the device. Chunks are pipelined: the host enqueues chunk k, then
@@ -112,21 +163,31 @@ class FDTDServer:
         if any(j.steps < 1 for j in jobs):
             raise ValueError('every job needs at least one step')
         pending = sorted(jobs, key=lambda j: j.arrival)
-        active, done = [], []
+        active, done, closed = [], [], []
         self._used = 0.0
         start = clock()
         cell_updates = 0
-        streams = [(torch.cuda.Stream(), torch.cuda.Stream()) for _ in range(2)]   # default priority
+        streams = None              # two stream pairs (default priority), made when the first chunk runs
         last_event = [None, None]   # end of the previous chunk's 2D and 3D batches
-        inflight = None             # (events, batches, jobs finishing in that chunk)
+        inflight = None             # (events, batches, jobs finishing in that chunk, jobs cancelled before it)
         parity = 0
         phases = dict(admit=0.0, enqueue=0.0, wait=0.0, readout=0.0, chunks=0)
         while pending or active or inflight:
             t0 = clock()
-            self._admit(pending, active, start, clock)
+            self._admit(pending, active, start, clock, closed)
             t1 = clock()
+            if on_chunk is not None:
+                on_chunk([j for j in active if j.done_steps < j.steps])
+            stopping = [j for j in active if j.cancelled and j.done_steps < j.steps]
+            for j in stopping:   # no further steps; freed once the chunk in flight (which may hold it) is done
+                j.done_steps = j.steps
+                j.status = 'cancelled'
             runnable = [j for j in active if j.done_steps < j.steps]
             current = None
+            if runnable and streams is None:
+                streams = [(torch.cuda.Stream(), torch.cuda.Stream()) for _ in range(2)]
+            if runnable or stopping:
+                current = ([], [], [], stopping)
             if runnable:
                 main = torch.cuda.current_stream()
                 events, batches, finishing = [], [], []
@@ -153,38 +214,40 @@ class FDTDServer:
                         cell_updates += j.cells * c
                         if j.done_steps >= j.steps:
                             finishing.append(j)
-                current = (events, batches, finishing)
+                current = (events, batches, finishing, stopping)
                 parity ^= 1
                 phases['chunks'] += 1
-            elif inflight is None:
-                wait = pending[0].arrival - (clock() - start)
-                if wait > 0:
-                    time.sleep(min(wait, 0.01))
+            elif current is None and inflight is None:
+                if pending:
+                    wait = pending[0].arrival - (clock() - start)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
                 phases['admit'] += t1 - t0
                 continue
             t2 = clock()
             if inflight is not None:
-                events, batches, finishing = inflight
+                events, batches, finishing, cancelled = inflight
                 for event in events:
                     event.synchronize()
                 t3 = clock()
                 for batch in batches:
                     batch.close()
                 for j in finishing:
-                    j.result = _readout(j.sim)
-                    j.finished = clock() - start
-                    j.sim.close()
-                    j.sim = None
-                    self._used -= j.bytes
-                    active.remove(j)
-                    done.append(j)
+                    if j.status != 'cancelled':
+                        _finish(j, clock() - start)
+                    self._release(j, active, done)
+                for j in cancelled:
+                    if j in active:
+                        j.finished = clock() - start
+                        self._release(j, active, done)
                 phases['wait'] += t3 - t2
                 phases['readout'] += clock() - t3
             inflight = current
             phases['admit'] += t1 - t0
             phases['enqueue'] += t2 - t1
-        torch.cuda.synchronize()
-        stats = self._stats(done, clock() - start, cell_updates)
+        if streams is not None:
+            torch.cuda.synchronize()
+        stats = self._stats(done + closed, clock() - start, cell_updates)
         stats['host_phases_s'] = phases
         return stats
 
@@ -207,21 +270,24 @@ class FDTDServer:
             torch.cuda.synchronize()
             job.done_steps = job.steps
             cell_updates += job.cells * job.steps
-            job.result = _readout(job.sim)
-            job.finished = clock() - start
+            _finish(job, clock() - start)
             job.sim.close()
             job.sim = None
             done.append(job)
         return self._stats(done, clock() - start, cell_updates)
 
     @staticmethod
-    def _stats(done, makespan, cell_updates):
+    def _stats(jobs, makespan, cell_updates):
+        """Latency percentiles over the jobs that completed; every job's status, and the errors of the others."""
+        done = [j for j in jobs if j.status == 'done']
         latencies = np.array([j.latency for j in done])
         return {'jobs': len(done), 'makespan_s': makespan, 'cell_updates': cell_updates,
                 'cell_updates_per_s': cell_updates / makespan if makespan > 0 else 0.0,
                 'p50_latency_s': float(np.percentile(latencies, 50)) if latencies.size else None,
                 'p99_latency_s': float(np.percentile(latencies, 99)) if latencies.size else None,
-                'results': {j.name: j.result for j in done}}
+                'results': {j.name: j.result for j in done},
+                'status': {j.name: j.status for j in jobs},
+                'errors': {j.name: j.error for j in jobs if j.error}}
 
 
 __all__ = ['Job', 'FDTDServer', 'state_bytes']

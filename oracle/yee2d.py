@@ -347,6 +347,133 @@ class CellCurrent:
         fields[self.component][i:i + 1, j] += self.weight * value
 
 
+def aux_line_terms(width, dt, half_omega_dt):
+    """(Courant number c dt / width, cos(kappa0 / 2), Mur coefficient) of a uniform vacuum 1D line at the carrier
+    (physics/em/sources.py:189-204): sin(kappa0/2) = sin(omega0 dt/2) / S, Mur S' = S cos(kappa0/2) / cos(omega0 dt/2)."""
+    courant = C0 * float(dt) / float(width)
+    sin_half_kappa = math.sin(half_omega_dt) / courant
+    if sin_half_kappa >= 1.0:
+        raise ValueError('TFSF: the centre frequency has no propagating wave on the 1D grid')
+    cos_half_kappa = math.sqrt(1.0 - sin_half_kappa ** 2)
+    mur_courant = courant * cos_half_kappa / math.cos(half_omega_dt)
+    return courant, cos_half_kappa, (mur_courant - 1.0) / (mur_courant + 1.0)
+
+
+class TFSFBox:
+    """+x vacuum plane wave on the total-field box [x0, x1) x [y0, y1) of a uniform grid (physics/em/sources.py:207-378,
+    _PlaneWaveBox, TFSFSource, TFSFSourceHz). A 1D incident line (20 margin cells, the box columns, 20 margin cells;
+    the soft source at aux cell 10, first-order Mur ends matched to the carrier's numerical phase velocity) runs beside
+    the 2D grid; after the H update its incident E (time n) corrects the H faces on the box edges and the line's H
+    advances; after the E update its incident H (time n + 1/2) corrects the E cells / faces on the box edges and the
+    line's E advances with the source value float(2 S cos(kappa0/2) amplitude s(t + dt/2)). E_z: incident (E_z, H_y);
+    H_z: incident (E_y, H_z = E_y / eta0). Side coefficients dt / (mu0 or eps0) / dx, 0 on an open (mirror) side."""
+
+    MARGIN, SOURCE_INDEX = 20, 10
+
+    def __init__(self, cells, temporal, dx, dt, amplitude, polarization, open_faces, dtype):
+        x0, x1, y0, y1 = (int(v) for v in cells)
+        if not (1 <= x0 < x1 and 1 <= y0 < y1):
+            raise ValueError(f'TFSF box [{x0}, {x1}) x [{y0}, {y1}) must keep a scattered-field cell on the lo sides')
+        self.x0, self.x1, self.y0, self.y1 = x0, x1, y0, y1
+        self.cells = (x0, x1, y0, y1)
+        self.temporal, self.amplitude = temporal, float(amplitude)
+        self.polarization, self.open_faces = polarization, frozenset(open_faces)
+        self.real = real = dtype
+        self.field = 'tfsf'
+        dt = float(dt)
+        margin = self.MARGIN
+        widths = np.full(2 * margin + (x1 - x0), float(dx))
+        face_spacings = 0.5 * (widths[:-1] + widths[1:])
+        f0 = temporal.f_peak if isinstance(temporal, RickerWavelet) else temporal.f0
+        courant, cos_half_kappa, mur = aux_line_terms(float(dx), dt, 0.5 * 2.0 * np.pi * float(f0) * dt)
+        self.source_scale = 2.0 * courant * cos_half_kappa * self.amplitude
+        self.mur_lo = self.mur_hi = real(mur)
+        self.aux_x0, self.n = margin, widths.size
+
+        def sides(constant, spacings):
+            names = ('x_lo', 'x_hi', 'y_lo', 'y_hi')[:len(spacings)]
+            return tuple(real(0.0 if name in self.open_faces else dt / (constant * spacing))
+                         for name, spacing in zip(names, spacings))
+        side_spacings = (float(dx),) * 4
+        side_widths = (float(dx),) * 2
+        if polarization == 'ez':
+            self.aux_e = np.zeros(self.n, dtype=real)   # E_z at the aux cell centres
+            self.aux_h = np.zeros(self.n, dtype=real)   # H_y on the aux hi faces
+            self.coef_h = sides(MU0, side_spacings)     # H_y rows x0-1 / x1-1, H_x columns y0-1 / y1-1
+            self.coef_e = sides(EPS0, side_widths)      # E_z rows x0 / x1-1
+            self.aux_coef_h = (dt / (MU0 * face_spacings)).astype(real)
+            self.aux_coef_e = (dt / (EPS0 * widths)).astype(real)
+        else:
+            self.aux_h = np.zeros(self.n, dtype=real)   # H_z at the aux centres (index 0 unused)
+            self.aux_e = np.zeros(self.n, dtype=real)   # E_y on the aux hi faces: the soft source and Mur ends
+            self.coef_h = sides(MU0, side_widths)       # H_z rows x0 / x1-1
+            self.coef_e = sides(EPS0, side_spacings)    # E_y rows x0-1 / x1-1, E_x columns y0-1 / y1-1
+            self.aux_coef_h = (-dt / (MU0 * widths[1:])).astype(real)
+            self.aux_coef_e = (-dt / (EPS0 * np.append(face_spacings, widths[-1]))).astype(real)
+
+    def _aux_e_update(self, e, h, value):
+        """Interior E of the line, the soft source, Mur ends (the tfsf_aux_e_graded kernel, physics/em/sources.py:
+        163-187): e[k] += ce[k] (h[k] - h[k-1]) for 1 <= k <= n-2."""
+        n = self.n
+        old = (e[0], e[1], e[n - 2], e[n - 1])
+        e[1:n - 1] += self.aux_coef_e[1:n - 1] * (h[1:n - 1] - h[0:n - 2])
+        e[self.SOURCE_INDEX] += value
+        e[0] = old[1] + self.mur_lo * (e[1] - old[0])
+        e[n - 1] = old[2] + self.mur_hi * (e[n - 2] - old[3])
+
+    def inject_h(self, fields):
+        """After the H update: H-side corrections from the incident E at time n, then the line's H update."""
+        x0, x1, y0, y1, a0 = self.x0, self.x1, self.y0, self.y1, self.aux_x0
+        last = a0 + (x1 - 1 - x0)
+        if self.polarization == 'ez':
+            hx, hy, inc = fields['hx'], fields['hy'], self.aux_e
+            c_xlo, c_xhi, c_ylo, c_yhi = self.coef_h
+            hy[x0 - 1, y0:y1] -= c_xlo * inc[a0]
+            hy[x1 - 1, y0:y1] += c_xhi * inc[last]
+            incident = inc[a0:a0 + (x1 - x0)]
+            hx[x0:x1, y0 - 1] += c_ylo * incident
+            hx[x0:x1, y1 - 1] -= c_yhi * incident
+            self.aux_h[:-1] += self.aux_coef_h * (self.aux_e[1:] - self.aux_e[:-1])
+        else:
+            hz, inc = fields['hz'], self.aux_e
+            c_xlo, c_xhi = self.coef_h
+            hz[x0, y0:y1] += c_xlo * inc[a0 - 1]
+            hz[x1 - 1, y0:y1] -= c_xhi * inc[last]
+            self.aux_h[1:] += self.aux_coef_h * (self.aux_e[1:] - self.aux_e[:-1])
+
+    def inject_e(self, fields, t, keep):
+        """After the E update: E-side corrections from the incident H at n + 1/2 (0 in PEC cells / faces, the keep
+        re-pin), then the line's E update with the source at t."""
+        x0, x1, y0, y1, a0 = self.x0, self.x1, self.y0, self.y1, self.aux_x0
+        last = a0 + (x1 - 1 - x0)
+        value = self.real(self.source_scale * self.temporal(t))
+        if self.polarization == 'ez':
+            ez, inc = fields['ez'], self.aux_h
+            c_xlo, c_xhi = self.coef_e
+            ez[x0, y0:y1] -= c_xlo * inc[a0 - 1]
+            ez[x1 - 1, y0:y1] += c_xhi * inc[last]
+            if keep is not None:
+                ez[x0, y0:y1] *= keep[x0, y0:y1]
+                ez[x1 - 1, y0:y1] *= keep[x1 - 1, y0:y1]
+            self._aux_e_update(self.aux_e, self.aux_h, value)
+        else:
+            ex, ey, inc = fields['ex'], fields['ey'], self.aux_h
+            c_xlo, c_xhi, c_ylo, c_yhi = self.coef_e
+            ey[x0 - 1, y0:y1] += c_xlo * inc[a0]
+            ey[x1 - 1, y0:y1] -= c_xhi * inc[last]
+            incident = inc[a0:a0 + (x1 - x0)]
+            ex[x0:x1, y0 - 1] -= c_ylo * incident
+            ex[x0:x1, y1 - 1] += c_yhi * incident
+            if keep is not None:
+                keep_x, keep_y = keep
+                for field, k, rows in ((ey, keep_y, (x0 - 1, x1 - 1)),):
+                    for row in rows:
+                        field[row, y0:y1] *= k[row, y0:y1]
+                for column in (y0 - 1, y1 - 1):
+                    ex[x0:x1, column] *= keep_x[x0:x1, column]
+            self._aux_e_update(self.aux_e, self.aux_h[1:], value)
+
+
 def dipole_face_shares(cell, n_cells, periodic):
     """Faces carrying an in-plane dipole along one axis (physics/em/placement.py:268-276): the hi face `cell` and the
     lo face `cell - 1`, half each; on a bounded axis only faces 0..n-2 exist and the survivors share the moment."""
@@ -675,6 +802,32 @@ class Yee2D:
         self._finalized = False
         return source
 
+    def add_tfsf(self, x0, y0, x1, y1, wavelength=None, frequency=None, bandwidth=0.5, amplitude=1.0,
+                 waveform='gaussian'):
+        """+x plane wave of incident peak `amplitude` [V/m] (E_z on ez; E_y on hz) inside the total-field box of cells
+        [cell(x_lo), cell(x_hi) + 1) x [cell(y_lo), cell(y_hi) + 1) (physics/em/__init__.py:475-490, placement.py:
+        97-150): one scattered-field cell must stay inside the window on every side, except a y mirror face of the
+        wave's parity (E_z symmetric, H_z antisymmetric), which the box opens onto (no correction on that side)."""
+        geo = self.geo
+        temporal = make_temporal(waveform, resolve_frequency(wavelength, frequency), bandwidth)
+        for x, y, label in ((x0, y0, 'corner 0'), (x1, y1, 'corner 1')):
+            geo.validate(x, y, f'TFSF box {label}')
+        x_lo, x_hi = sorted((x0, x1))
+        y_lo, y_hi = sorted((y0, y1))
+        cells = (geo.cell(0, x_lo), geo.cell(0, x_hi) + 1, geo.cell(1, y_lo), geo.cell(1, y_hi) + 1)
+        wx, wy = geo.windows
+        reached = {face for face, hit in (('x_lo', cells[0] <= wx.start), ('x_hi', cells[1] >= wx.stop),
+                                          ('y_lo', cells[2] <= wy.start), ('y_hi', cells[3] >= wy.stop)) if hit}
+        parity = 'symmetric' if self.polarization == 'ez' else 'antisymmetric'
+        open_faces = {face for face in reached if face[0] == 'y' and self.bc.faces[face] == parity}
+        if {'x_lo', 'x_hi'} & reached or (reached - open_faces):
+            raise ValueError(f'TFSF box cells {cells} need a scattered-field cell inside the window on every side')
+        source = TFSFBox(cells, temporal, float(geo.dx32), float(self.dt), amplitude, self.polarization, open_faces,
+                         self.dtype)
+        self.sources.append(source)
+        self._finalized = False
+        return source
+
     def _point(self, x, y, label):
         self.geo.validate(x, y, label)
         return self.geo.cell(0, x), self.geo.cell(1, y)
@@ -849,6 +1002,8 @@ class Yee2D:
         """profile = float32(cb_line admittance weight) (em/sources.py:383-392, 582-602)."""
         m = self.materials
         for source in self.sources:
+            if isinstance(source, TFSFBox):   # vacuum plane wave: no calibration
+                continue
             if isinstance(source, CellCurrent):
                 source.calibrate(getattr(self, source.coefficient))
                 continue
@@ -887,9 +1042,14 @@ class Yee2D:
             self._step_ez()
         else:
             self._step_hz(t_start)
+        pec = self.pec_mask()
+        keep = None if not pec.any() else (self.keep if self.polarization == 'ez' else (self.keep_x, self.keep_y))
         for source in self.sources:
-            if getattr(source, 'field', 'e') == 'e':
+            field = getattr(source, 'field', 'e')
+            if field == 'e':
                 source.inject(self.fields, t_half)
+            elif field == 'tfsf':
+                source.inject_e(self.fields, t_half, keep)
         if self.polarization == 'ez':
             self._ez_walls()
         self.n_step += 1
@@ -928,6 +1088,14 @@ class Yee2D:
         for face, field, row in (('x_lo', hy, np.s_[0, :]), ('x_hi', hy, np.s_[-2, :]),
                                  ('y_lo', hx, np.s_[:, 0]), ('y_hi', hx, np.s_[:, -2])):
             if self.bc.faces[face] == 'pmc':
+                field[row] = 0
+        # TFSF H-side corrections after the H update, then the pmc planes again (fdtd_ez_2d.py:89-98)
+        tfsf = [source for source in self.sources if getattr(source, 'field', 'e') == 'tfsf']
+        for source in tfsf:
+            source.inject_h(f)
+        for face, field, row in (('x_lo', hy, np.s_[0, :]), ('x_hi', hy, np.s_[-2, :]),
+                                 ('y_lo', hx, np.s_[:, 0]), ('y_hi', hx, np.s_[:, -2])):
+            if tfsf and self.bc.faces[face] == 'pmc':
                 field[row] = 0
         # pole updates from E^n (fdtd_ez_2d.py:318-345)
         for entry in self.poles:
@@ -977,8 +1145,11 @@ class Yee2D:
             side.apply('e', ey if side.axis == 'x' else ex, self.db, hz)
         # magnetic line currents at t_start after the H update (fdtd_hz_2d.py:144-189, inject_magnetic)
         for source in self.sources:
-            if getattr(source, 'field', 'e') == 'h':
+            field = getattr(source, 'field', 'e')
+            if field == 'h':
                 source.inject(f, t_start)
+            elif field == 'tfsf':
+                source.inject_h(f)
         # pmc walls: H_z = 0 on the outermost cells, after the magnetic injections (apply_hz_walls,
         # boundaries.py:517-527; re-pinned after injection, fdtd_hz_2d.py:165-170)
         for face, outer in (('x_lo', np.s_[0, :]), ('x_hi', np.s_[-1, :]), ('y_lo', np.s_[:, 0]),

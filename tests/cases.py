@@ -287,3 +287,29 @@ def build_lossy3d(target, **options):
                for position, orientation, moment, wavelength, waveform in spec['dipoles']]
     plane = sim.add_flux_plane('x', spec['plane_x'], wavelengths=list(spec['wavelengths']))
     return sim, dipoles, plane
+
+
+# 2D TFSF plane-wave boxes (tests/golden/make_tfsf_golden.py writes them from the reference): a +x Gaussian plane wave
+# in the total-field box around a dielectric scatterer, CPML around; a flux line in the scattered-field region behind
+# the box and one inside it. Case 'ez_mirror' puts a symmetric mirror on y_lo, which the box opens onto.
+TFSF2D = dict(size_x=2e-6, size_y=1.4e-6, dx=25e-9, npml=10, box=(0.4e-6, 0.2e-6, 1.6e-6, 1.2e-6),
+              scatterer=((0.8e-6, 0.5e-6, 1.1e-6, 0.9e-6), 4.0), wavelength=600e-9, bandwidth=0.5, amplitude=1.0,
+              monitors=(dict(axis='y', position=1.8e-6, span=(0.0, 1.4e-6), wavelengths=[500e-9, 600e-9, 700e-9]),
+                        dict(axis='y', position=1.3e-6, span=(0.3e-6, 1.1e-6), wavelengths=[500e-9, 600e-9, 700e-9])),
+              faces={'ez': ('cpml',) * 4, 'hz': ('cpml',) * 4, 'ez_mirror': ('cpml', 'cpml', 'symmetric', 'cpml')},
+              mirror_box=(0.4e-6, 0.0, 1.6e-6, 1.0e-6), steps=(150, 400))
+
+
+def build_tfsf2d(target, case, boundary_type, **options):
+    """The TFSF case `case` ('ez', 'hz', 'ez_mirror') on `target`; returns (sim, source, monitors)."""
+    spec = TFSF2D
+    polarization = 'hz' if case == 'hz' else 'ez'
+    sim = target(spec['size_x'], spec['size_y'], spec['dx'], npml=spec['npml'],
+                 boundaries=boundary_type(spec['faces'][case], spec['npml']), polarization=polarization, **options)
+    (x0, y0, x1, y1), eps = spec['scatterer']
+    sim.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
+    box = spec['mirror_box'] if case == 'ez_mirror' else spec['box']
+    source = sim.add_tfsf(*box, wavelength=spec['wavelength'], bandwidth=spec['bandwidth'],
+                          amplitude=spec['amplitude'])
+    monitors = [sim.add_flux_monitor(**monitor) for monitor in spec['monitors']]
+    return sim, source, monitors

@@ -63,7 +63,7 @@ def test_ez8192_roofline_grid_matches_oracle(gpu):
 
 def test_yee3d_generator_at_128_matches_oracle(gpu):
     """The 512^3 workload's generator at 128^3 (boxes, the E_z dipole at the centre, two flux planes at 4 wavelengths,
-    m = 17): 320 steps of the pulse against the FP64 oracle (fields relative to their largest norm, flux), then 3
+    m = 17): 220 steps of the pulse against the FP64 oracle (fields relative to their largest norm, flux), then 3
     steps from random fields (every tile of the 3D step, the CPML on all six faces)."""
     from oracle.yee3d import Yee3D
     bench = _bench()
@@ -74,7 +74,7 @@ def test_yee3d_generator_at_128_matches_oracle(gpu):
     assert sim.model.sample_interval == ref.sample_interval == 17
     names = ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')
     norms = {}
-    for chunk in (160, 160):
+    for chunk in (110, 110):
         print(f'yee3d 128: {chunk} more steps', flush=True)
         sim.run(n_steps=chunk)
         ref.run(n_steps=chunk)

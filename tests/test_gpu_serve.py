@@ -75,3 +75,32 @@ def test_budget_holds_jobs_until_memory_frees(gpu):
     for job in jobs:   # the jobs resident when `job` was admitted fit the budget together
         resident = [o for o in jobs if o.admitted <= job.admitted < o.finished]
         assert sum(o.bytes for o in resident) <= budget
+
+
+def test_cancelled_job_stops_at_a_chunk_boundary(gpu):
+    """A running job cancelled at a chunk boundary stops after the chunk in flight, its state is freed and it reports
+    'cancelled'; the other jobs' spectra are unchanged (equal to their solo runs bit for bit); a job cancelled while
+    pending never runs."""
+    from opensol_amd.serve import FDTDServer
+    jobs = _jobs()
+    victim, never = jobs[1], jobs[7]
+    never.cancel()
+
+    def on_chunk(running):   # cancel the victim once it has run one chunk (of four)
+        if any(j is victim for j in running) and victim.done_steps > 0:
+            victim.cancel()
+
+    stats = FDTDServer(hbm_budget=1e9, chunk_steps=25).run(jobs, on_chunk=on_chunk)
+    assert stats['status'][victim.name] == 'cancelled' and victim.done_steps == victim.steps and victim.sim is None
+    assert stats['status'][never.name] == 'cancelled' and never.admitted is None
+    assert stats['jobs'] == len(jobs) - 2
+    for job in _jobs():
+        if job.name in (victim.name, never.name):
+            assert job.name not in stats['results']
+            continue
+        sim = job.build()
+        sim.run(n_steps=job.steps, print_every=0)
+        solo = [np.asarray(m.flux()) for m in sim.model.monitors]
+        sim.close()
+        for a, b in zip(solo, stats['results'][job.name]):
+            assert np.array_equal(a, b), job.name

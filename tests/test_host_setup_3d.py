@@ -211,7 +211,7 @@ def test_dft_plane_layer_matches_the_oracle(z):
     model, ref = _pair(faces)
     plane, ref_plane = model.add_dft_plane(z, wavelengths=[6e-7]), ref.add_dft_plane(z, wavelengths=[6e-7])
     assert plane.k_grid == ref_plane.k
-    assert plane.accumulator_shape() == (3, 1, ref_plane.xs.size, ref_plane.ys.size, 1, 2)
+    assert plane.accumulator_shape() == (3, 1, ref_plane.xs.size, ref_plane.ys.size, 1, 4)   # Kahan (sum, comp) pairs
     assert plane.pairs == ((0, False), (1, True), (2, False))
     with pytest.raises(ValueError):
         model.add_dft_plane(0.7e-6, wavelengths=[6e-7])

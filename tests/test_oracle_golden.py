@@ -218,3 +218,42 @@ def test_pole_oracle_bulk_plasma_oscillation():
     t_span = (crossings[-3] - crossings[0]) * float(sim.dt)
     decay = -np.log(peaks[-1] / peaks[0]) / t_span
     assert abs(decay / (gamma / 2.0) - 1.0) < 5e-2
+
+
+@pytest.mark.parametrize('case', ['ez', 'hz', 'ez_mirror'])
+def test_oracle_reproduces_reference_tfsf(case):
+    """TFSF plane-wave boxes around a dielectric (E_z, H_z, and E_z opening onto a symmetric y_lo mirror): the FP64
+    oracle reproduces the reference's FP64 run (box cells, fields at two steps to 1e-12 of the peak, the scattered-
+    and total-field flux to 1e-12). The scattered-field region holds only the scatterer's field (no incident leak)."""
+    from tests.cases import TFSF2D, build_tfsf2d
+    golden = np.load(os.path.join(GOLDEN, f'tfsf2d_{case}.npz'))
+    sim, source, monitors = build_tfsf2d(Yee2D, case, _oracle_boundaries)
+    assert list(source.cells) == golden['cells'].tolist()
+    assert (sim.nx, sim.ny) == (int(golden['nx']), int(golden['ny']))
+    done = 0
+    for step in TFSF2D['steps']:
+        sim.run(n_steps=step - done)
+        done = step
+        for name, values in sim.fields.items():
+            want = golden[f'{name}_{step}']
+            assert np.abs(want).max() > 0
+            np.testing.assert_allclose(values, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    assert monitors[0].sample_interval == int(golden['sample_interval'])
+    for index, monitor in enumerate(monitors):
+        np.testing.assert_allclose(monitor.flux(), golden[f'flux{index}'], rtol=1e-12)
+
+
+def test_tfsf_without_scatterer_leaks_nothing():
+    """An empty TFSF box in vacuum: the scattered-field region stays below 1e-6 of the incident peak (the exact 1D /
+    2D consistency of the incident line), while the total-field region carries the full wave; a wrong side
+    coefficient, sign or timing leaks at the 1e-2 level."""
+    for polarization, name in (('ez', 'ez'), ('hz', 'ey')):
+        sim = Yee2D(2e-6, 1.4e-6, 25e-9, npml=10, polarization=polarization)
+        source = sim.add_tfsf(0.4e-6, 0.2e-6, 1.6e-6, 1.2e-6, wavelength=600e-9, bandwidth=0.5)
+        x0, x1, y0, y1 = source.cells
+        sim.run(n_steps=150)
+        field = sim.fields[name]
+        inside = np.abs(field[x0 + 2:x1 - 2, y0 + 2:y1 - 2]).max()
+        outside = max(np.abs(field[:x0 - 2]).max(), np.abs(field[x1 + 2:]).max(), np.abs(field[:, :y0 - 2]).max(),
+                      np.abs(field[:, y1 + 2:]).max())
+        assert inside > 0.5 and outside < 1e-6 * inside, (polarization, inside, outside)
