@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 13
+#define YEE_ABI_VERSION 14
 #define YEE_MAX_SOURCES 8
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
@@ -175,6 +175,34 @@ typedef struct yee_pole {
     float* state[2];         /* Drude: J (state[1] unused); Lorentz: P, dP; (i1 - i0) * (j1 - j0) each */
 } yee_pole;
 
+/*
+ * TFSF plane-wave box of a 2D instance: a +x vacuum plane wave in the total-field cells [x0, x1) x [y0, y1). The
+ * incident values come from the reference's 1D incident line run on the host for every step (the line depends on
+ * nothing in the grid): `line` holds one value per box column, `ends` the two values at the faces x0 - 1 and x1 - 1.
+ * The corrections are added to every copy of a corrected value inside the step kernel:
+ *   E_z (incident E_z, H_y): after the H update, H_y of rows x0-1 / x1-1 (columns [y0, y1)) += coef_h[0] / coef_h[1] *
+ *     line[n][0] / line[n][L-1], H_x of columns y0-1 / y1-1 (rows i in [x0, x1)) += coef_h[2] / coef_h[3] *
+ *     line[n][i - x0] (line = incident E_z at time n); after the E update, E_z of rows x0 / x1-1 += coef_e[0] /
+ *     coef_e[1] * ends[n][0] / ends[n][1] (incident H_y at n + 1/2);
+ *   H_z (incident E_y, H_z): after the H update, H_z of rows x0 / x1-1 += coef_h[0] / coef_h[1] * ends[n][0] /
+ *     ends[n][1] (incident E_y at n); after the E update, E_y of rows x0-1 / x1-1 += coef_e[0] / coef_e[1] *
+ *     line[n][0] / line[n][L-1], E_x of columns y0-1 / y1-1 += coef_e[2] / coef_e[3] * line[n][i - x0] (incident H_z
+ *     at n + 1/2).
+ * Coefficients carry their signs (0 on a side that opens onto a mirror face); E-side corrections skip PEC cells and
+ * pinned faces (cb = 0). Replaces physics/em/sources.py:207-378 (TFSFSource, TFSFSourceHz and their correction
+ * kernels :27-130).
+ */
+typedef struct yee_tfsf {
+    int32_t x0, x1, y0, y1;
+    float coef_h[4];
+    float coef_e[4];
+    const float* line;       /* [step][x1 - x0] */
+    const float* ends;       /* [step][2] */
+    int64_t n_steps;         /* rows of both tables; steps past them add nothing */
+} yee_tfsf;
+
+#define YEE_MAX_TFSF 2
+
 typedef struct yee2d_desc {
     int32_t nx, ny;
     int32_t pitch;           /* row stride in elements of every (nx, *) array: a multiple of 4, >= ny */
@@ -219,6 +247,9 @@ typedef struct yee2d_desc {
      * both factors (physics/em/boundaries.py:478-488, wall_centre_rows :105-128). H_z: 0 none, YEE_WALL_ZERO H_z = 0
      * on the outermost cells after the H update (pmc, :517-527). */
     int32_t wall_sign[4];
+    int32_t n_tfsf;
+    int32_t pad5_;
+    yee_tfsf tfsf[YEE_MAX_TFSF];
     int64_t* clock;          /* device int64: absolute step index n of the next step */
     int64_t step_limit;      /* steps n >= step_limit are skipped (batched instances); INT64_MAX for none */
 } yee2d_desc;
@@ -351,7 +382,7 @@ typedef struct yee_ctx yee_ctx;
 int32_t yee_abi_version(void);
 /* 0: yee2d_desc, 1: yee_line_source, 2: yee_flux_line, 3: yee_cpml_axis, 4: yee3d_desc, 5: yee_point_source,
  * 6: yee_flux_plane, 7: yee3d_cpml_axis, 8: yee3d_axis_row, 9: yee_probes, 10: yee_dft_region, 11: yee_pole,
- * 12: yee3d_pole */
+ * 12: yee3d_pole, 13: yee_tfsf */
 size_t yee_sizeof(int32_t which);
 const char* yee_last_error(void);
 

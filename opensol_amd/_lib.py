@@ -6,13 +6,14 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 13
+ABI_VERSION = 14
 MAX_SOURCES = 8
 SOURCE_E, SOURCE_H = 0, 1   # yee_line_source.field
 WALL_ZERO = 2   # yee2d_desc.wall_sign of an H_z pmc face
 MAX_MONITORS = 4
 MAX_DFT = 4
 MAX_POLES = 8
+MAX_TFSF = 2
 MAX_POLES_3D = 12
 YEE_DRUDE, YEE_LORENTZ = 1, 2
 MAX_SOURCES_3D = 4
@@ -56,6 +57,11 @@ class PoleC(ctypes.Structure):
                 ('state', c_ptr * 2)]
 
 
+class TfsfC(ctypes.Structure):
+    _fields_ = [('x0', c_i32), ('x1', c_i32), ('y0', c_i32), ('y1', c_i32), ('coef_h', c_f32 * 4), ('coef_e', c_f32 * 4),
+                ('line', c_ptr), ('ends', c_ptr), ('n_steps', c_i64)]
+
+
 class Desc2DC(ctypes.Structure):
     _fields_ = [('nx', c_i32), ('ny', c_i32), ('pitch', c_i32), ('polarization', c_i32), ('periodic_x', c_i32),
                 ('periodic_y', c_i32), ('rows_per_tile', c_i32), ('lane_columns', c_i32), ('field', (c_ptr * 3) * 2),
@@ -66,7 +72,8 @@ class Desc2DC(ctypes.Structure):
                 ('pad2_', c_i32), ('source', LineSourceC * MAX_SOURCES), ('monitor', FluxLineC * MAX_MONITORS),
                 ('n_dft', c_i32), ('pad3_', c_i32), ('dft', DftRegionC * MAX_DFT),
                 ('n_poles', c_i32), ('pad4_', c_i32), ('pole', PoleC * MAX_POLES),
-                ('probes', ProbesC), ('wall_sign', c_i32 * 4), ('clock', c_ptr), ('step_limit', c_i64)]
+                ('probes', ProbesC), ('wall_sign', c_i32 * 4), ('n_tfsf', c_i32), ('pad5_', c_i32),
+                ('tfsf', TfsfC * MAX_TFSF), ('clock', c_ptr), ('step_limit', c_i64)]
 
 
 class PointSourceC(ctypes.Structure):
@@ -106,7 +113,7 @@ class Desc3DC(ctypes.Structure):
 
 
 STRUCTS = (Desc2DC, LineSourceC, FluxLineC, CpmlAxisC, Desc3DC, PointSourceC, FluxPlaneC, Cpml3DAxisC, AxisRow3DC,
-           ProbesC, DftRegionC, PoleC, Pole3DC)
+           ProbesC, DftRegionC, PoleC, Pole3DC, TfsfC)
 
 # Every entry point include/yee.h declares: (name, restype, argtypes)
 SIGNATURES = (

@@ -312,6 +312,90 @@ __device__ __forceinline__ void apply_poles(const yee2d_desc& d, int i, int jb, 
     apply_pole_records<VEC>(d, 0, i, jb, e_n, cb, ez);
 }
 
+// ------------------------------------------------------------------------------------------------ TFSF boxes
+// Box-edge corrections of the TFSF records (physics/em/sources.py:27-130); the incident tables come from the host's
+// run of the 1D incident line. Every copy of a corrected H value (owner, prologue row, halo column) goes through these
+// helpers, so the copies stay bit-identical. E-side corrections skip PEC cells and pinned faces (cb = 0), as the
+// reference's keep re-pin after injection does.
+__device__ __forceinline__ float tfsf_line(const yee_tfsf& t, int64_t n, int k)
+{
+    return n < t.n_steps ? t.line[n * (t.x1 - t.x0) + k] : 0.f;
+}
+__device__ __forceinline__ float tfsf_end(const yee_tfsf& t, int64_t n, int e)
+{
+    return n < t.n_steps ? t.ends[2 * n + e] : 0.f;
+}
+
+// E_z: H_y on the face row i (rows x0-1, x1-1; columns [y0, y1)) from the incident E_z at time n
+__device__ __forceinline__ float tfsf_ez_hy(const yee2d_desc& d, int64_t n, int i, int j, float v)
+{
+    for (int b = 0; b < d.n_tfsf; ++b) {
+        const yee_tfsf& t = d.tfsf[b];
+        if (j < t.y0 || j >= t.y1) continue;
+        if (i == t.x0 - 1) v += t.coef_h[0] * tfsf_line(t, n, 0);
+        if (i == t.x1 - 1) v += t.coef_h[1] * tfsf_line(t, n, t.x1 - 1 - t.x0);
+    }
+    return v;
+}
+// E_z: H_x on the face column j (columns y0-1, y1-1; rows [x0, x1)) from the incident E_z at time n
+__device__ __forceinline__ float tfsf_ez_hx(const yee2d_desc& d, int64_t n, int i, int j, float v)
+{
+    for (int b = 0; b < d.n_tfsf; ++b) {
+        const yee_tfsf& t = d.tfsf[b];
+        if (i < t.x0 || i >= t.x1) continue;
+        if (j == t.y0 - 1) v += t.coef_h[2] * tfsf_line(t, n, i - t.x0);
+        if (j == t.y1 - 1) v += t.coef_h[3] * tfsf_line(t, n, i - t.x0);
+    }
+    return v;
+}
+// E_z: E_z of rows x0, x1-1 (columns [y0, y1)) from the incident H_y at time n + 1/2
+__device__ __forceinline__ float tfsf_ez_e(const yee2d_desc& d, int64_t n, int i, int j, float v, float cb)
+{
+    if (cb == 0.f) return v;
+    for (int b = 0; b < d.n_tfsf; ++b) {
+        const yee_tfsf& t = d.tfsf[b];
+        if (j < t.y0 || j >= t.y1) continue;
+        if (i == t.x0) v += t.coef_e[0] * tfsf_end(t, n, 0);
+        if (i == t.x1 - 1) v += t.coef_e[1] * tfsf_end(t, n, 1);
+    }
+    return v;
+}
+// H_z: H_z of rows x0, x1-1 (columns [y0, y1)) from the incident E_y at time n
+__device__ __forceinline__ float tfsf_hz_h(const yee2d_desc& d, int64_t n, int i, int j, float v)
+{
+    for (int b = 0; b < d.n_tfsf; ++b) {
+        const yee_tfsf& t = d.tfsf[b];
+        if (j < t.y0 || j >= t.y1) continue;
+        if (i == t.x0) v += t.coef_h[0] * tfsf_end(t, n, 0);
+        if (i == t.x1 - 1) v += t.coef_h[1] * tfsf_end(t, n, 1);
+    }
+    return v;
+}
+// H_z: E_y on the face rows x0-1, x1-1 (columns [y0, y1)) from the incident H_z at time n + 1/2
+__device__ __forceinline__ float tfsf_hz_ey(const yee2d_desc& d, int64_t n, int i, int j, float v, float cb)
+{
+    if (cb == 0.f) return v;
+    for (int b = 0; b < d.n_tfsf; ++b) {
+        const yee_tfsf& t = d.tfsf[b];
+        if (j < t.y0 || j >= t.y1) continue;
+        if (i == t.x0 - 1) v += t.coef_e[0] * tfsf_line(t, n, 0);
+        if (i == t.x1 - 1) v += t.coef_e[1] * tfsf_line(t, n, t.x1 - 1 - t.x0);
+    }
+    return v;
+}
+// H_z: E_x on the face columns y0-1, y1-1 (rows [x0, x1)) from the incident H_z at time n + 1/2
+__device__ __forceinline__ float tfsf_hz_ex(const yee2d_desc& d, int64_t n, int i, int j, float v, float cb)
+{
+    if (cb == 0.f) return v;
+    for (int b = 0; b < d.n_tfsf; ++b) {
+        const yee_tfsf& t = d.tfsf[b];
+        if (i < t.x0 || i >= t.x1) continue;
+        if (j == t.y0 - 1) v += t.coef_e[2] * tfsf_line(t, n, i - t.x0);
+        if (j == t.y1 - 1) v += t.coef_e[3] * tfsf_line(t, n, i - t.x0);
+    }
+    return v;
+}
+
 __device__ __forceinline__ int wave_index()
 {
     return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -412,16 +496,19 @@ struct EzTile {
 #pragma unroll
         for (int k = 0; k < VEC; ++k) {
             hy_v[k] = cur.hy[k];
-            if (hy_updated)
+            const int j = col.jb + k;
+            if (hy_updated) {
                 hy_v[k] = ez_hy(c[k], cur.hy[k], cur.e_next[k] - e_cur[k], idx_h, xh.r >= 0, xh.b, xh.g, psi_xh[k],
                                 &psi_xh[k]);
-            const int j = col.jb + k;
+                if (d.n_tfsf > 0) hy_v[k] = tfsf_ez_hy(d, n, i, j, hy_v[k]);
+            }
             hx_v[k] = j < ny ? cur.hx[k] : 0.f;
             if (j < ny && (j < ny - 1 || py)) {
                 const bool layer = col.ry_h[k] >= 0;
                 hx_v[k] = ez_hx(c[k], cur.hx[k], e_right[k] - e_cur[k], col.idy_h[k], layer,
                                 layer ? cy.b_h[col.ry_h[k]] : 0.f, layer ? cy.g_h[col.ry_h[k]] : 0.f, psi_yh[k],
                                 &psi_yh[k]);
+                if (d.n_tfsf > 0) hx_v[k] = tfsf_ez_hx(d, n, i, j, hx_v[k]);
             }
         }
         // H_x left of the first column: the previous lane, or lane 0's recompute of column jl
@@ -430,6 +517,7 @@ struct EzTile {
             float unused;
             hx_left = ez_hx(tab_a<LDS>(d, cur.id_l), cur.hx_l, e_cur[0] - cur.e_l, idy_h_l, ry_l >= 0,
                             ry_l >= 0 ? cy.b_h[ry_l] : 0.f, ry_l >= 0 ? cy.g_h[ry_l] : 0.f, psi_l, &unused);
+            if (d.n_tfsf > 0) hx_left = tfsf_ez_hx(d, n, i, jl, hx_left);
         } else if (lane == 0) {
             hx_left = 0.f;   // column 0 of a non-periodic axis reads H_x = 0 below it
         }
@@ -453,6 +541,12 @@ struct EzTile {
         }
         // dispersive poles from E^n, then their corrections of the updated E (before the sources)
         apply_poles<VEC>(d, i, col.jb, e_cur, c, ez_v);
+        // TFSF E-side corrections from the incident H at t + dt/2
+        if (d.n_tfsf > 0) {
+#pragma unroll
+            for (int k = 0; k < VEC; ++k)
+                if (col.jb + k < ny) ez_v[k] = tfsf_ez_e(d, n, i, col.jb + k, ez_v[k], c[k].y);
+        }
         // sheet sources at t + dt/2: E_z += float32(value * profile)
         for (int s = 0; s < d.n_sources; ++s) {
             const yee_line_source& src = d.source[s];
@@ -556,6 +650,7 @@ struct EzTile {
                 float unused;
                 hy_prev[k] = ez_hy(tab_a<LDS>(d, id_at(idp, k)), hy_o[k], e_cur[k] - e_p[k], idx, xh.r >= 0, xh.b,
                                    xh.g, psi_o[k], &unused);
+                if (d.n_tfsf > 0) hy_prev[k] = tfsf_ez_hy(d, n, ip, jb + k, hy_prev[k]);
             }
         }
 
@@ -662,6 +757,7 @@ struct HzTile {
                                 layer_y ? cy.b_e[col.ry_e[q]] : 0.f, layer_y ? cy.g_e[col.ry_e[q]] : 0.f,
                                 p.y[q], &p.y[q]);
             if (d.n_sources > 0) v = magnetic_currents(k, j, v);
+            if (d.n_tfsf > 0) v = tfsf_hz_h(d, n, k, j, v);
             if (wall_cell(k, j)) v = 0.f;
             out[q] = j < ny ? v : 0.f;
         }
@@ -683,6 +779,7 @@ struct HzTile {
                                ry_e_r >= 0 ? cy.b_e[ry_e_r] : 0.f, ry_e_r >= 0 ? cy.g_e[ry_e_r] : 0.f, p.y_r,
                                &unused_y);
             if (d.n_sources > 0) *out_r = magnetic_currents(k, jr, *out_r);
+            if (d.n_tfsf > 0) *out_r = tfsf_hz_h(d, n, k, jr, *out_r);
             if (wall_cell(k, jr)) *out_r = 0.f;
         }
     }
@@ -783,6 +880,16 @@ struct HzTile {
         if (d.n_poles > 0) {
             apply_pole_records<VEC>(d, 1, i, col.jb, cur.ex, cb_x, ex_v);
             apply_pole_records<VEC>(d, 2, i, col.jb, cur.ey, cb_y, ey_v);
+        }
+        // TFSF E-side corrections from the incident H_z at t + dt/2
+        if (d.n_tfsf > 0) {
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) {
+                const int j = col.jb + q;
+                if (j >= ny) continue;
+                ex_v[q] = tfsf_hz_ex(d, n, i, j, ex_v[q], cb_x[q]);
+                if (ey_updated) ey_v[q] = tfsf_hz_ey(d, n, i, j, ey_v[q], cb_y[q]);
+            }
         }
         // sheet current on the hi faces at t + dt/2 (E_y for a line along y, E_x for a line along x)
         for (int s = 0; s < d.n_sources; ++s) {

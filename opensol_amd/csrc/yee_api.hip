@@ -205,6 +205,14 @@ int32_t check_2d(const yee2d_desc* d)
             return fail(YEE_EINVAL, "H_z wall_sign must be 0 or YEE_WALL_ZERO");
         if (axis_periodic || n_axis < 2) return fail(YEE_EINVAL, "wall rows need a non-periodic axis of 2+ cells");
     }
+    if (d->n_tfsf < 0 || d->n_tfsf > YEE_MAX_TFSF) return fail(YEE_EINVAL, "n_tfsf out of range");
+    for (int b = 0; b < d->n_tfsf; ++b) {
+        const yee_tfsf& t = d->tfsf[b];
+        // one scattered-field cell on every side: the corrected rows x0-1 .. x1-1 and columns y0-1 .. y1-1 exist
+        if (t.x0 < 1 || t.x1 > d->nx - 1 || t.x0 >= t.x1 || t.y0 < 1 || t.y1 > d->ny - 1 || t.y0 >= t.y1)
+            return fail(YEE_EINVAL, "TFSF box needs 1 <= x0 < x1 <= nx - 1 and 1 <= y0 < y1 <= ny - 1");
+        if (!t.line || !t.ends || t.n_steps < 0) return fail(YEE_EINVAL, "TFSF incident tables missing");
+    }
     if (d->n_monitors < 0 || d->n_monitors > YEE_MAX_MONITORS) return fail(YEE_EINVAL, "n_monitors out of range");
     if (d->n_monitors > 0 && d->sample_interval < 1) return fail(YEE_EINVAL, "sample_interval must be >= 1");
     for (int m = 0; m < d->n_monitors; ++m) {
@@ -375,6 +383,10 @@ yee2d_tile_sets tile_sets(const yee2d_desc& d)
             edge = edge || (d.wall_sign[0] != 0 && a == 0) || (d.wall_sign[1] != 0 && b >= d.nx - 1);
         for (int k = 0; d.polarization == YEE_HZ && k < d.n_poles; ++k)   // the lean H_z loop has no pole update
             edge = edge || (d.pole[k].i0 < i1 && d.pole[k].i1 > i0);
+        for (int k = 0; k < d.n_tfsf; ++k) {   // TFSF corrected rows x0-1, x0, x1-1
+            const yee_tfsf& t = d.tfsf[k];
+            for (const int r : {t.x0 - 1, t.x0, t.x1 - 1}) edge = edge || (r >= a && r <= b);
+        }
         if (edge) rows.push_back(ti);
     }
     for (int tj = 0; tj < s.tiles_j; ++tj) {
@@ -387,6 +399,10 @@ yee2d_tile_sets tile_sets(const yee2d_desc& d)
             edge = edge || (d.source[k].axis == 0 && d.source[k].line >= a && d.source[k].line <= b);
         for (int k = 0; k < d.n_monitors; ++k)
             edge = edge || (d.monitor[k].normal == 1 && d.monitor[k].position >= a && d.monitor[k].position <= b);
+        for (int k = 0; k < d.n_tfsf; ++k) {   // TFSF corrected columns y0-1, y1-1
+            const yee_tfsf& t = d.tfsf[k];
+            for (const int c : {t.y0 - 1, t.y1 - 1}) edge = edge || (c >= a && c <= b);
+        }
         if (edge) columns.push_back(tj);
     }
     if ((int)rows.size() > YEE_MAX_EDGE_LINES || (int)columns.size() > YEE_MAX_EDGE_LINES) {
@@ -594,6 +610,7 @@ size_t yee_sizeof(int32_t which)
         case 10: return sizeof(yee_dft_region);
         case 11: return sizeof(yee_pole);
         case 12: return sizeof(yee3d_pole);
+        case 13: return sizeof(yee_tfsf);
         default: return 0;
     }
 }
@@ -648,6 +665,8 @@ int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, 
             status = fail(YEE_EINVAL, "batched instances take no DFT regions");
         if (status == YEE_OK && has_walls(host[k]))
             status = fail(YEE_EINVAL, "batched instances take no wall rows");
+        else if (status == YEE_OK && host[k].n_tfsf > 0)
+            status = fail(YEE_EINVAL, "batched instances take no TFSF boxes");
         if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
         const int64_t next = prefix[k] + tiles_2d(host[k]);
         if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");

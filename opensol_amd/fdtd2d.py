@@ -17,8 +17,8 @@ from .domain import Domain
 from .materials import MaterialMap, semi_implicit
 from .dispersion import DrudePole, LorentzPole, bounding_box, pole_step_term
 from .monitors import DFTMonitor, FieldProbe, FluxMonitor, ProbeTable, dft_sample_interval
-from .sources import (LineSource, PointCurrent, complex_permittivity, electric_dipole_faces, resolve_frequencies,
-                      resolve_frequency)
+from .sources import (LineSource, PlaneWaveBox, PointCurrent, complex_permittivity, electric_dipole_faces,
+                      resolve_frequencies, resolve_frequency)
 from .waveforms import make_temporal
 
 POLARIZATIONS = ('ez', 'hz')
@@ -48,6 +48,7 @@ class Model2D:
         self.dt = self.grid.dt
         self.materials = MaterialMap((self.nx, self.ny))
         self.sources = []
+        self.tfsf = []      # PlaneWaveBox records (TFSF boxes)
         self.monitors = []
         dx32 = float(self.grid.dx)
         self.cpml = {axis: CPMLAxisTables(self.bc, axis, n, dx32, float(self.dt))
@@ -303,6 +304,37 @@ class Model2D:
         self.sources.append(source)
         return source
 
+    def add_tfsf(self, x0, y0, x1, y1, wavelength=None, frequency=None, bandwidth=0.5, amplitude=1.0,
+                 waveform='gaussian'):
+        """+x plane wave of incident peak `amplitude` [V/m] (E_z on ez; E_y on hz, H_z = E_y / eta0) inside the
+        total-field box of cells [cell(x_lo), cell(x_hi) + 1) x [cell(y_lo), cell(y_hi) + 1) (physics/em/__init__.py:
+        475-490, placement.py:97-150): one scattered-field cell must stay inside the window on every side, except a y
+        mirror face of the wave's parity (E_z symmetric), which the box opens onto."""
+        temporal = make_temporal(waveform, resolve_frequency(wavelength, frequency), bandwidth)
+        domain = self.domain
+        for x, y, label in ((x0, y0, 'corner 0'), (x1, y1, 'corner 1')):
+            domain.validate_physical_pos((x, y), f'TFSF box {label}')
+        x_lo, x_hi = sorted((x0, x1))
+        y_lo, y_hi = sorted((y0, y1))
+        cells = (domain.pos_to_cell_axis(0, x_lo), domain.pos_to_cell_axis(0, x_hi) + 1,
+                 domain.pos_to_cell_axis(1, y_lo), domain.pos_to_cell_axis(1, y_hi) + 1)
+        wx, wy = domain.physical_window()
+        reached = {face for face, hit in (('x_lo', cells[0] <= wx.start), ('x_hi', cells[1] >= wx.stop),
+                                          ('y_lo', cells[2] <= wy.start), ('y_hi', cells[3] >= wy.stop)) if hit}
+        parity = 'symmetric' if self.polarization == 'ez' else 'antisymmetric'
+        open_faces = {face for face in reached if face[0] == 'y' and self.bc.faces[face] == parity}
+        if {'x_lo', 'x_hi'} & reached or (reached - open_faces):
+            raise ValueError(f'TFSF box cells [{cells[0]}, {cells[1]}) x [{cells[2]}, {cells[3]}) need at least one '
+                             f'scattered-field cell inside the physical window [{wx.start}, {wx.stop}) x '
+                             f'[{wy.start}, {wy.stop}) on every side, except a y mirror face (ez symmetric, hz '
+                             f'antisymmetric) the box opens onto')
+        if len(self.tfsf) >= _lib.MAX_TFSF:
+            raise ValueError(f'at most {_lib.MAX_TFSF} TFSF boxes per 2D model')
+        box = PlaneWaveBox(cells, temporal, float(self.grid.dx), float(self.dt), amplitude, self.polarization,
+                           open_faces)
+        self.tfsf.append(box)
+        return box
+
     def add_gaussian_beam(self, axis, position, waist, center, wavelength=None, frequency=None, bandwidth=0.5,
                           amplitude=1.0, waveform='gaussian', span=None):
         """Line source with the transverse factor exp(-((s - center) / waist)^2) over the float64 centres s of its
@@ -425,7 +457,8 @@ class Model2D:
         self.check_courant()
         self.calibrate()
         spectral = self.monitors + self.dft_monitors
-        self.sample_interval = dft_sample_interval(float(self.dt), self.sources, [m.frequencies for m in spectral])
+        self.sample_interval = dft_sample_interval(float(self.dt), self.sources + self.tfsf,
+                                                   [m.frequencies for m in spectral])
         for monitor in spectral:
             monitor.sample_interval = self.sample_interval
 
@@ -549,6 +582,7 @@ class Engine2D:
         for index, monitor in enumerate(model.dft_monitors):
             monitor.engine, monitor.index = self, index
         self.waves = [None] * len(model.sources)
+        self.tfsf_tables = [None] * len(model.tfsf)
         self.phases = [None] * len(model.monitors)
         self.dft_phases = [None] * len(model.dft_monitors)
         self.poles = []
@@ -599,6 +633,7 @@ class Engine2D:
         half = times[:-1] + 0.5 * float(model.dt)
         self.waves = [torch.from_numpy(s.wave_table(times[:-1] if s.field == 'h' else half)).to(self.device)
                       for s in model.sources]
+        self.tfsf_tables = [tuple(torch.from_numpy(a).to(self.device) for a in box.tables(half)) for box in model.tfsf]
         m = model.sample_interval
         n_samples = capacity // m
         sample_times = times[m * np.arange(1, n_samples + 1)]
@@ -691,6 +726,14 @@ class Engine2D:
             for q, state in enumerate(entry['states']):
                 pl.state[q] = state.data_ptr()
         self.probes.fill(d.probes)
+        d.n_tfsf = len(model.tfsf)
+        for k, (box, (line, ends)) in enumerate(zip(model.tfsf, self.tfsf_tables)):
+            t = d.tfsf[k]
+            t.x0, t.x1, t.y0, t.y1 = box.cells
+            for q in range(4):
+                t.coef_h[q], t.coef_e[q] = float(box.coef_h[q]), float(box.coef_e[q])
+            t.line, t.ends = line.data_ptr(), ends.data_ptr()
+            t.n_steps = line.shape[0]
         d.clock = self.clock.data_ptr()
         d.step_limit = _lib.INT64_MAX
         return d

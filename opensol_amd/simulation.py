@@ -157,6 +157,13 @@ class Simulation:
         self._require_unbuilt('add_field_probe')
         return self.model.add_field_probe(x, y, field_component)
 
+    def add_tfsf(self, x0, y0, x1, y1, wavelength=None, frequency=None, bandwidth=0.5, amplitude=1.0,
+                 waveform='gaussian'):
+        """+x plane wave of incident peak `amplitude` [V/m] inside the total-field box (x0, y0)-(x1, y1)
+        (physics/em/__init__.py:475-490)."""
+        self._require_unbuilt('add_tfsf')
+        return self.model.add_tfsf(x0, y0, x1, y1, wavelength, frequency, bandwidth, amplitude, waveform)
+
     def add_dft_monitor(self, x0, y0, x1, y1, wavelengths=None, frequencies=None, downsample=1):
         """Running DFT over the cell box of (x0, y0)-(x1, y1); downsample s > 1 samples every s-th cell along x and y
         (physics/em/__init__.py:500-509)."""

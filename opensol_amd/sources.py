@@ -167,3 +167,97 @@ class PointDipole:
     def wave_table(self, half_step_times):
         """float32 s(t_n + dt/2) per step."""
         return np.array([self.temporal(t) for t in half_step_times], dtype=np.float64).astype(DTYPE)
+
+
+class PlaneWaveBox:
+    """+x vacuum plane wave in the total-field cells [x0, x1) x [y0, y1) of a uniform 2D grid (physics/em/sources.py:
+    207-378: _PlaneWaveBox, TFSFSource, TFSFSourceHz). A 1D incident line (20 margin cells, the box columns, 20 margin
+    cells; the soft source at line cell 10 with value float32(2 S cos(kappa0/2) amplitude s(t + dt/2)); first-order
+    Mur ends matched to the carrier's numerical phase velocity) runs on the host in float32 with the reference's
+    update order, since it depends on nothing in the grid; tables() records, per step n, the incident values the box
+    edges need: E_z: line = E_z^n of the box columns, ends = H_y^(n+1/2) at the faces x0-1, x1-1; H_z: line =
+    H_z^(n+1/2) of the box columns, ends = E_y^n at the faces x0-1, x1-1. The kernel adds them with the signed side
+    coefficients dt / (mu0 or eps0) / dx (0 on a side that opens onto a mirror face)."""
+
+    MARGIN, SOURCE_INDEX = 20, 10
+    field = 'tfsf'
+
+    def __init__(self, cells, temporal, dx, dt, amplitude, polarization, open_faces=()):
+        x0, x1, y0, y1 = (int(v) for v in cells)
+        if not (1 <= x0 < x1 and 1 <= y0 < y1):
+            raise ValueError(f'TFSF box [{x0}, {x1}) x [{y0}, {y1}) must be non-empty and keep one scattered-field '
+                             f'cell on the lo sides (x0 >= 1, y0 >= 1)')
+        self.x0, self.x1, self.y0, self.y1 = x0, x1, y0, y1
+        self.temporal, self.amplitude = temporal, float(amplitude)
+        self.polarization, self.open_faces = polarization, frozenset(open_faces)
+        dt, dx = float(dt), float(dx)
+        margin = self.MARGIN
+        widths = np.full(2 * margin + (x1 - x0), dx)
+        face_spacings = 0.5 * (widths[:-1] + widths[1:])
+        f0 = getattr(temporal, 'f_peak', None) or temporal.f0
+        half_omega_dt = math.pi * float(f0) * dt
+        courant = C0 * dt / dx
+        sin_half_kappa = math.sin(half_omega_dt) / courant
+        if sin_half_kappa >= 1.0:
+            raise ValueError(f'TFSF: the centre frequency has no propagating wave on the 1D grid (sin(k dx/2) = '
+                             f'{sin_half_kappa:.3g} >= 1); refine the mesh')
+        cos_half_kappa = math.sqrt(1.0 - sin_half_kappa ** 2)
+        mur_courant = courant * cos_half_kappa / math.cos(half_omega_dt)
+        self.mur = DTYPE((mur_courant - 1.0) / (mur_courant + 1.0))
+        self.source_scale = 2.0 * courant * cos_half_kappa * self.amplitude
+        self.aux_x0, self.n = margin, widths.size
+
+        def sides(constant, count):
+            names = ('x_lo', 'x_hi', 'y_lo', 'y_hi')[:count]
+            return [DTYPE(0.0 if name in self.open_faces else dt / (constant * dx)) for name in names]
+        if polarization == 'ez':
+            h, e = sides(MU0, 4), sides(EPS0, 2)
+            # H_y rows x0-1 (-) / x1-1 (+), H_x columns y0-1 (+) / y1-1 (-); E_z rows x0 (-) / x1-1 (+)
+            self.coef_h = (-h[0], h[1], h[2], -h[3])
+            self.coef_e = (-e[0], e[1], DTYPE(0.0), DTYPE(0.0))
+            self.aux_coef_h = (dt / (MU0 * face_spacings)).astype(DTYPE)
+            self.aux_coef_e = (dt / (EPS0 * widths)).astype(DTYPE)
+        else:
+            h, e = sides(MU0, 2), sides(EPS0, 4)
+            # H_z rows x0 (+) / x1-1 (-); E_y rows x0-1 (+) / x1-1 (-), E_x columns y0-1 (-) / y1-1 (+)
+            self.coef_h = (h[0], -h[1], DTYPE(0.0), DTYPE(0.0))
+            self.coef_e = (e[0], -e[1], -e[2], e[3])
+            self.aux_coef_h = (-dt / (MU0 * widths[1:])).astype(DTYPE)
+            self.aux_coef_e = (-dt / (EPS0 * np.append(face_spacings, widths[-1]))).astype(DTYPE)
+
+    @property
+    def cells(self):
+        return self.x0, self.x1, self.y0, self.y1
+
+    def _aux_e_update(self, e, h, value):
+        n = self.n
+        old = (e[0], e[1], e[n - 2], e[n - 1])
+        e[1:n - 1] += self.aux_coef_e[1:n - 1] * (h[1:n - 1] - h[0:n - 2])
+        e[self.SOURCE_INDEX] += value
+        e[0] = old[1] + self.mur * (e[1] - old[0])
+        e[n - 1] = old[2] + self.mur * (e[n - 2] - old[3])
+
+    def tables(self, half_step_times):
+        """(line (steps, x1 - x0), ends (steps, 2)) float32 incident values for the steps whose half-step times are
+        given (the source is evaluated at t_n + dt/2 on the engine's float64 clock)."""
+        steps = len(half_step_times)
+        length = self.x1 - self.x0
+        a0, last = self.aux_x0, self.aux_x0 + length - 1
+        line = np.zeros((steps, length), dtype=DTYPE)
+        ends = np.zeros((steps, 2), dtype=DTYPE)
+        aux_e = np.zeros(self.n, dtype=DTYPE)
+        aux_h = np.zeros(self.n, dtype=DTYPE)
+        scale = self.source_scale
+        for step, t in enumerate(half_step_times):
+            value = DTYPE(scale * self.temporal(float(t)))
+            if self.polarization == 'ez':
+                line[step] = aux_e[a0:a0 + length]
+                aux_h[:-1] += self.aux_coef_h * (aux_e[1:] - aux_e[:-1])
+                ends[step] = (aux_h[a0 - 1], aux_h[last])
+                self._aux_e_update(aux_e, aux_h, value)
+            else:
+                ends[step] = (aux_e[a0 - 1], aux_e[last])
+                aux_h[1:] += self.aux_coef_h * (aux_e[1:] - aux_e[:-1])
+                line[step] = aux_h[a0:a0 + length]
+                self._aux_e_update(aux_e, aux_h[1:], value)
+        return line, ends

@@ -407,3 +407,49 @@ def test_dispersive_poles_match_oracle(gpu, polarization):
     batch.close()
     for name in sim.model.components:
         np.testing.assert_array_equal(batched.engine.field(name), sim.engine.field(name))
+
+
+@pytest.mark.parametrize('case', ['ez', 'hz', 'ez_mirror'])
+def test_tfsf_matches_oracle(gpu, case):
+    """TFSF plane-wave boxes around a dielectric (E_z, H_z, and E_z opening onto a symmetric y_lo mirror) against the
+    FP64 oracle, which reproduces the reference's own runs to 1e-12: fields <= 1e-5 relative L2 at both steps, flux
+    behind and inside the box <= 1e-5; graph replay bit-identical to eager."""
+    from tests.cases import TFSF2D, build_tfsf2d
+    Simulation = simulation_class()
+    ref, _, ref_monitors = build_tfsf2d(Yee2D, case, _oracle_boundaries, phase_dtype=np.complex128)
+    sim, _, monitors = build_tfsf2d(Simulation, case, _product_boundaries)
+    graphed, _, graphed_monitors = build_tfsf2d(Simulation, case, _product_boundaries, graph_steps=16)
+    done = 0
+    for step in TFSF2D['steps']:
+        print(f'tfsf {case} to step {step}', flush=True)
+        ref.run(n_steps=step - done)
+        sim.run(n_steps=step - done, print_every=0)
+        graphed.run(n_steps=step - done, print_every=0)
+        done = step
+        for name in sim.model.components:
+            error = rel_l2(sim.engine.field(name), ref.fields[name])
+            print(f'tfsf {case} step {step} {name}: rel L2 {error:.3e}')
+            assert error <= FIELD_TOL, (step, name, error)
+            np.testing.assert_array_equal(graphed.engine.field(name), sim.engine.field(name))
+    for monitor, ref_monitor, graphed_monitor in zip(monitors, ref_monitors, graphed_monitors):
+        flux_error = np.max(np.abs(monitor.flux() / ref_monitor.flux() - 1.0))
+        print(f'tfsf {case} flux rel error {flux_error:.3e}')
+        assert flux_error <= 1e-5
+        np.testing.assert_array_equal(graphed_monitor.flux(), monitor.flux())
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_empty_tfsf_box_leaks_nothing(gpu, polarization):
+    """An empty TFSF box in vacuum on the GPU: the scattered-field region stays below 1e-5 of the incident peak
+    (float32 rounding of the 2D update against the host's 1D incident line), the total-field region carries the wave."""
+    Simulation = simulation_class()
+    sim = Simulation(2e-6, 1.4e-6, 25e-9, npml=10, polarization=polarization)
+    box = sim.add_tfsf(0.4e-6, 0.2e-6, 1.6e-6, 1.2e-6, wavelength=600e-9, bandwidth=0.5)
+    sim.run(n_steps=150, print_every=0)
+    field = sim.engine.field('ez' if polarization == 'ez' else 'ey')
+    x0, x1, y0, y1 = box.cells
+    inside = np.abs(field[x0 + 2:x1 - 2, y0 + 2:y1 - 2]).max()
+    outside = max(np.abs(field[:x0 - 2]).max(), np.abs(field[x1 + 2:]).max(), np.abs(field[:, :y0 - 2]).max(),
+                  np.abs(field[:, y1 + 2:]).max())
+    print(f'tfsf {polarization}: inside {inside:.3e}, outside {outside:.3e}')
+    assert inside > 0.5 and outside < 1e-5 * inside

@@ -323,3 +323,35 @@ def test_pole_masks_background_and_drive_match_the_oracle():
         hz.add_drude_rectangle(0, 0, 1e-7, 1e-7, DrudePole(1e15, 1e13, 2.0))
     with pytest.raises(ValueError):   # four H_z poles fill the eight face records
         hz.add_drude_rectangle(0, 0, 1e-7, 1e-7, DrudePole(1e15, 1e13, 2.0))
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_tfsf_incident_tables_match_the_oracle_line(polarization):
+    """The product's host run of the TFSF incident line (float32) equals the oracle's float32 line step by step, bit
+    for bit: per step the box columns' incident values and the two face values the box edges read; same cells and
+    signed side coefficients."""
+    from opensol_amd.fdtd2d import clock_times
+    model = Model2D(2e-6, 1.4e-6, 25e-9, npml=10, polarization=polarization)
+    box = model.add_tfsf(0.4e-6, 0.2e-6, 1.6e-6, 1.2e-6, wavelength=600e-9, bandwidth=0.5)
+    ref = Yee2D(2e-6, 1.4e-6, 25e-9, npml=10, polarization=polarization, dtype=np.float32)
+    ref_box = ref.add_tfsf(0.4e-6, 0.2e-6, 1.6e-6, 1.2e-6, wavelength=600e-9, bandwidth=0.5)
+    assert box.cells == ref_box.cells
+    signs = (-1, 1, 1, -1) if polarization == 'ez' else (1, -1)
+    np.testing.assert_array_equal(np.abs(box.coef_h[:len(ref_box.coef_h)]), ref_box.coef_h)
+    assert [np.sign(c) for c in box.coef_h[:len(signs)]] == list(signs)
+    steps = 260
+    times = clock_times(model.dt, steps)
+    line, ends = box.tables(times[:-1] + 0.5 * float(model.dt))
+    a0, length = ref_box.aux_x0, ref_box.x1 - ref_box.x0
+    ref.finalize()
+    for n in range(steps):
+        if polarization == 'ez':
+            np.testing.assert_array_equal(line[n], ref_box.aux_e[a0:a0 + length])
+        else:
+            np.testing.assert_array_equal(ends[n], ref_box.aux_e[[a0 - 1, a0 + length - 1]])
+        ref.step()
+        if polarization == 'ez':
+            np.testing.assert_array_equal(ends[n], ref_box.aux_h[[a0 - 1, a0 + length - 1]])
+        else:
+            np.testing.assert_array_equal(line[n], ref_box.aux_h[a0:a0 + length])
+    assert np.abs(line).max() > 0.5 if polarization == 'ez' else np.abs(ends).max() > 0.5
