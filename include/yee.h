@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 14
+#define YEE_ABI_VERSION 15
 #define YEE_MAX_SOURCES 8
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
@@ -318,6 +318,8 @@ typedef struct yee3d_cpml_axis {
     float* psi_h[2][2];      /* [set][first, second component] */
 } yee3d_cpml_axis;
 
+#define YEE_MIRROR_SYMMETRIC 1
+#define YEE_MIRROR_ANTISYMMETRIC 2
 #define YEE3D_MAX_SOURCES 4
 #define YEE3D_MAX_PLANES 8     /* a flux box is six bounded planes */
 
@@ -372,6 +374,13 @@ typedef struct yee3d_desc {
     int32_t pad3_;
     yee3d_pole pole[YEE3D_MAX_POLES];
     yee_probes probes;
+    /* Mirror faces (x_lo, x_hi, y_lo, y_hi, z_lo, z_hi): 0 none, YEE_MIRROR_SYMMETRIC, YEE_MIRROR_ANTISYMMETRIC. The
+     * face pads one ghost cell (index 0 on a lo face, n - 1 on a hi face); the plane is the hi face of cell 0 / n - 2.
+     * Symmetric: the tangential H on the plane is 0 after the H update (in the step kernel) and the normal E on the
+     * plane 0 after the sources; antisymmetric: the ghost's tangential E = minus its inner neighbour's after the
+     * sources (yee3d_mirror_kernel, zeros first, then images in face order). Replaces physics/em3d/engine3d.py:
+     * 114-126, 513-540 (_mirror_rows and the mirror-row writes). */
+    int32_t mirror[6];
     int64_t* clock;
     int64_t step_limit;
 } yee3d_desc;

@@ -6,7 +6,7 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 14
+ABI_VERSION = 15
 MAX_SOURCES = 8
 SOURCE_E, SOURCE_H = 0, 1   # yee_line_source.field
 WALL_ZERO = 2   # yee2d_desc.wall_sign of an H_z pmc face
@@ -19,6 +19,7 @@ YEE_DRUDE, YEE_LORENTZ = 1, 2
 MAX_SOURCES_3D = 4
 MAX_PLANES_3D = 8
 YEE_EZ, YEE_HZ = 0, 1
+MIRROR_SYMMETRIC, MIRROR_ANTISYMMETRIC = 1, 2
 INT64_MAX = (1 << 63) - 1
 
 c_i32, c_i64, c_f32, c_ptr = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_void_p
@@ -109,7 +110,7 @@ class Desc3DC(ctypes.Structure):
                 ('pad_', c_i32), ('source', PointSourceC * MAX_SOURCES_3D), ('plane', FluxPlaneC * MAX_PLANES_3D),
                 ('n_dft', c_i32), ('pad2_', c_i32), ('dft', DftRegionC * MAX_DFT),
                 ('n_poles', c_i32), ('pad3_', c_i32), ('pole', Pole3DC * MAX_POLES_3D),
-                ('probes', ProbesC), ('clock', c_ptr), ('step_limit', c_i64)]
+                ('probes', ProbesC), ('mirror', c_i32 * 6), ('clock', c_ptr), ('step_limit', c_i64)]
 
 
 STRUCTS = (Desc2DC, LineSourceC, FluxLineC, CpmlAxisC, Desc3DC, PointSourceC, FluxPlaneC, Cpml3DAxisC, AxisRow3DC,

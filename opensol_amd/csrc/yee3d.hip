@@ -229,6 +229,15 @@ __device__ __forceinline__ HNew h_cell(const yee3d_desc& d, int par, bool store,
         vx += db * a;
         vy -= db * b;
     }
+    if (d.mirror[0] | d.mirror[1] | d.mirror[2] | d.mirror[3] | d.mirror[4] | d.mirror[5]) {
+        // symmetric mirror planes (the hi face of cell 0 / n - 2): tangential H = 0 (engine3d.py:513-518)
+        if ((d.mirror[0] == YEE_MIRROR_SYMMETRIC && i == 0) || (d.mirror[1] == YEE_MIRROR_SYMMETRIC && i == d.nx - 2))
+            vy = vz = 0.f;
+        if ((d.mirror[2] == YEE_MIRROR_SYMMETRIC && j == 0) || (d.mirror[3] == YEE_MIRROR_SYMMETRIC && j == d.ny - 2))
+            vx = vz = 0.f;
+        if ((d.mirror[4] == YEE_MIRROR_SYMMETRIC && k == 0) || (d.mirror[5] == YEE_MIRROR_SYMMETRIC && k == d.nz - 2))
+            vx = vy = 0.f;
+    }
     return HNew{vx, vy, vz};
 }
 
@@ -668,6 +677,40 @@ __global__ void __launch_bounds__(256) yee3d_batch_sample_kernel(const yee3d_des
     typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
     const yee3d_desc& d = *(const yee3d_desc*)((ConstDesc*)descs + lo);
     yee3::sample_cell(d, *d.clock + offset, t - prefix[lo]);
+}
+
+// Mirror face `face` (0 x_lo .. 5 z_hi) after step n = *clock + offset, one lane per cell of the face: symmetric, the
+// normal E on the plane (the hi face of cell 0 / n - 2) = 0; antisymmetric, the ghost cell's (0 / n - 1) tangential E =
+// minus its inner neighbour's (1 / n - 2) (physics/em3d/engine3d.py:531-540). The host launches the symmetric faces
+// first, then the antisymmetric ones in face order, as the reference writes its zero rows before its images.
+__global__ void __launch_bounds__(256) yee3d_mirror_kernel(yee3d_desc desc, int32_t face, int32_t offset)
+{
+    // the descriptor in place at the kernel-argument pointer: the field table is indexed by runtime axes
+    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
+    const yee3d_desc& d = *(const yee3d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
+    if (d.clock != desc.clock) return;
+    const int64_t n = *d.clock + offset;
+    if (n >= d.step_limit) return;
+    const int a = face >> 1, hi = face & 1;
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;   // the two tangential axes, in grid order
+    const int32_t ext[3] = {d.nx, d.ny, d.nz};
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)ext[b] * ext[c]) return;
+    int idx[3];
+    idx[b] = (int)(t / ext[c]);
+    idx[c] = (int)(t % ext[c]);
+    const int64_t stride[3] = {(int64_t)d.ny * d.nz, d.nz, 1};
+    const int set = (int)((n & 1) ^ 1);
+    if (d.mirror[face] == YEE_MIRROR_SYMMETRIC) {
+        idx[a] = hi ? ext[a] - 2 : 0;
+        d.field[set][a][idx[0] * stride[0] + idx[1] * stride[1] + idx[2]] = 0.f;
+    } else if (d.mirror[face] == YEE_MIRROR_ANTISYMMETRIC) {
+        idx[a] = hi ? ext[a] - 1 : 0;
+        const int64_t ghost = idx[0] * stride[0] + idx[1] * stride[1] + idx[2];
+        const int64_t inner = ghost + (hi ? -stride[a] : stride[a]);
+        d.field[set][b][ghost] = -d.field[set][b][inner];
+        d.field[set][c][ghost] = -d.field[set][c][inner];
+    }
 }
 
 // Field probes after step n = *clock + offset: samples[n * n_probes + k] = field[new set][component[k]][cell[k]]

@@ -31,6 +31,7 @@ __global__ void yee_clock_advance(int64_t* clock, int64_t k, int64_t limit);
 __global__ void yee2d_probe_kernel(yee2d_desc d, int32_t offset);
 __global__ void yee2d_wall_kernel(yee2d_desc d, int32_t offset);
 __global__ void yee3d_probe_kernel(yee3d_desc d, int32_t offset);
+__global__ void yee3d_mirror_kernel(yee3d_desc d, int32_t face, int32_t offset);
 __global__ void yee2d_dft_kernel(yee2d_desc d, int32_t offset);
 __global__ void yee3d_dft_kernel(yee3d_desc d, int32_t offset);
 __global__ void yee_clock_advance_batch(const yee2d_desc* __restrict__ descs, int32_t n_inst, int64_t k);
@@ -267,6 +268,13 @@ int32_t check_3d(const yee3d_desc* d)
             if (c.n_rows > n[a]) return fail(YEE_EINVAL, "more CPML rows than cells");
         }
     }
+    for (int f = 0; f < 6; ++f) {
+        const int32_t kind = d->mirror[f];
+        if (kind == 0) continue;
+        if (kind != YEE_MIRROR_SYMMETRIC && kind != YEE_MIRROR_ANTISYMMETRIC)
+            return fail(YEE_EINVAL, "mirror must be 0, YEE_MIRROR_SYMMETRIC or YEE_MIRROR_ANTISYMMETRIC");
+        if (d->periodic[f >> 1] || n[f >> 1] < 3) return fail(YEE_EINVAL, "a mirror face needs a non-periodic axis of 3+ cells");
+    }
     if (d->n_sources < 0 || d->n_sources > YEE3D_MAX_SOURCES) return fail(YEE_EINVAL, "n_sources out of range");
     for (int s = 0; s < d->n_sources; ++s) {
         const yee_point_source& src = d->source[s];
@@ -476,6 +484,20 @@ void drop_graph(yee_ctx* ctx)
     }
 }
 
+// The mirror-row launches of a 3D instance after its step kernel: symmetric faces first, then antisymmetric ones.
+void launch_mirrors_3d(const yee3d_desc& d, int32_t offset, hipStream_t stream)
+{
+    for (const int kind : {YEE_MIRROR_SYMMETRIC, YEE_MIRROR_ANTISYMMETRIC}) {
+        for (int face = 0; face < 6; ++face) {
+            if (d.mirror[face] != kind) continue;
+            const int a = face >> 1;
+            const int64_t cells = (a == 0 ? (int64_t)d.ny * d.nz : a == 1 ? (int64_t)d.nx * d.nz : (int64_t)d.nx * d.ny);
+            hipLaunchKernelGGL(yee3d_mirror_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, stream, d,
+                               face, offset);
+        }
+    }
+}
+
 bool has_walls(const yee2d_desc& d)
 {
     return d.wall_sign[0] != 0 || d.wall_sign[1] != 0 || d.wall_sign[2] != 0 || d.wall_sign[3] != 0;
@@ -521,6 +543,7 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
         for (int32_t k = 0; k < count; ++k) {
             hipLaunchKernelGGL(step_kernel_3d(ctx->d3), dim3(ctx->blocks), dim3(256), 0, stream, ctx->d3,
                                ctx->tiles_j3, ctx->tiles_k3, (int32_t)ctx->tiles, k);
+            launch_mirrors_3d(ctx->d3, k, stream);
             if (sample_blocks > 0)
                 hipLaunchKernelGGL(yee3d_sample_kernel, dim3(sample_blocks), dim3(256), 0, stream, ctx->d3,
                                    ctx->sample_cells, k);
@@ -751,6 +774,7 @@ int32_t step_timed_3d(yee_ctx* ctx, int64_t n_steps, hipStream_t s, double* kern
         hipLaunchKernelGGL(step_kernel_3d(ctx->d3), dim3(ctx->blocks), dim3(256), 0, s, d, ctx->tiles_j3,
                            ctx->tiles_k3, (int32_t)ctx->tiles, 0);
         (void)hipEventRecord(ev[1], s);
+        launch_mirrors_3d(d, 0, s);
         if (sample_blocks > 0)
             hipLaunchKernelGGL(yee3d_sample_kernel, dim3(sample_blocks), dim3(256), 0, s, d, ctx->sample_cells, 0);
         (void)hipEventRecord(ev[2], s);
@@ -893,6 +917,8 @@ int32_t yee3d_batch_create(const yee3d_desc* descs, int32_t n, void* descs_dev, 
         if (host[k].probes.n_probes > 0) return fail(YEE_EINVAL, "batched instances take no field probes");
         if (host[k].n_dft > 0) return fail(YEE_EINVAL, "batched instances take no DFT regions");
         if (host[k].n_poles > 0) return fail(YEE_EINVAL, "batched 3D instances take no dispersive poles");
+        for (int f = 0; f < 6; ++f)
+            if (host[k].mirror[f] != 0) return fail(YEE_EINVAL, "batched 3D instances take no mirror faces");
         const int64_t next = block_prefix[k] + (tiles_3d(host[k]) + 3) / 4;
         if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
         block_prefix[k + 1] = (int32_t)next;

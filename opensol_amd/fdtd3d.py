@@ -27,8 +27,9 @@ COMPONENTS = ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')
 
 
 def resolve_boundaries_3d(boundaries, npml):
-    """None: CPML on every face; a dict of face -> 'cpml' / 'periodic' / 'pec' / 'pmc' (missing faces: cpml); or a
-    BoundaryConfig with z faces (physics/em3d/boundaries3d.py:17-60)."""
+    """None: CPML on every face; a dict of face -> 'cpml' / 'periodic' / 'pec' / 'pmc' / 'symmetric' /
+    'antisymmetric' (missing faces: cpml); or a BoundaryConfig with z faces (physics/em3d/boundaries3d.py:17-60). A
+    mirror face pads one ghost cell beyond the window (physics/em3d/engine3d.py:14-17)."""
     if boundaries is None:
         return BoundaryConfig.all_pml(npml, dimensions=3)
     if isinstance(boundaries, BoundaryConfig):
@@ -51,9 +52,6 @@ class Model3D:
 
     def __init__(self, size_x, size_y, size_z, dx, npml=12, courant=None, boundaries=None):
         self.bc = resolve_boundaries_3d(boundaries, npml)
-        mirrors = [face for face, kind in self.bc.faces.items() if kind in ('symmetric', 'antisymmetric')]
-        if mirrors:
-            raise ValueError(f'3D faces {mirrors}: mirror faces are 2D E_z only in this FDTD path')
         self.domain = Domain((size_x, size_y, size_z), dx, self.bc.npml_sides, courant)
         self.grid = self.domain.grid
         self.shape = self.grid.shape
@@ -510,6 +508,9 @@ class Engine3D:
             for q, state in enumerate(entry['states']):
                 pl.state[q] = state.data_ptr()
         self.probes.fill(d.probes)
+        kinds = {'symmetric': _lib.MIRROR_SYMMETRIC, 'antisymmetric': _lib.MIRROR_ANTISYMMETRIC}
+        for f, face in enumerate(('x_lo', 'x_hi', 'y_lo', 'y_hi', 'z_lo', 'z_hi')):
+            d.mirror[f] = kinds.get(model.bc.faces[face], 0)
         d.clock = self.clock.data_ptr()
         d.step_limit = _lib.INT64_MAX
         return d

@@ -46,16 +46,17 @@ def courant_time_step_3d(dx, dy, dz, courant=None):
 
 
 class Boundaries3D:
-    """Face types ('cpml' / 'periodic' / 'pec' / 'pmc') and CPML settings (physics/em3d/boundaries3d.py:17-60). A
-    'pmc' face is the bare grid edge (engine3d.py:12-13); a 'pec' face makes the outermost window cell layer PEC
-    (engine3d.py:13, boundaries3d.py:149-158)."""
+    """Face types ('cpml' / 'periodic' / 'pec' / 'pmc' / 'symmetric' / 'antisymmetric') and CPML settings
+    (physics/em3d/boundaries3d.py:17-60). A 'pmc' face is the bare grid edge (engine3d.py:12-13); a 'pec' face makes
+    the outermost window cell layer PEC (engine3d.py:13, boundaries3d.py:149-158); a mirror face pads one ghost cell
+    outside the window (engine3d.py:14-17)."""
 
     def __init__(self, faces=None, npml=12, order=3, sigma_factor=1.0, kappa_max=15.0, alpha_max=0.0):
         faces = dict(faces or {})
         self.faces = {face: faces.get(face, 'cpml') for face in FACES_3D}
         for face, kind in self.faces.items():
-            if kind not in ('cpml', 'periodic', 'pec', 'pmc'):
-                raise ValueError(f'oracle supports cpml, periodic, pec and pmc faces, got {face}={kind}')
+            if kind not in ('cpml', 'periodic', 'pec', 'pmc', 'symmetric', 'antisymmetric'):
+                raise ValueError(f'oracle supports cpml, periodic, pec, pmc and mirror faces, got {face}={kind}')
         for axis in AXES:
             if (self.faces[f'{axis}_lo'] == 'periodic') != (self.faces[f'{axis}_hi'] == 'periodic'):
                 raise ValueError(f'{axis}: periodic faces must be paired')
@@ -69,7 +70,25 @@ class Boundaries3D:
         return self.faces[f'{axis}_lo'] == 'periodic'
 
     def npml_sides(self):
-        return {face: (self.npml if self.cpml(face) else 0) for face in FACES_3D}
+        return {face: (self.npml if self.cpml(face) else 1 if self.faces[face] in MIRRORS_3D else 0)
+                for face in FACES_3D}
+
+    def mirror_planes(self, windows):
+        """[(axis, kind, ghost row, inner row, plane-face row)] of the mirror faces (boundaries3d.py:133-146): the
+        plane is the window edge, the hi face of the lo ghost / of the last window cell."""
+        planes = []
+        for axis, name in enumerate(AXES):
+            for side in ('lo', 'hi'):
+                kind = self.faces[f'{name}_{side}']
+                if kind not in MIRRORS_3D:
+                    continue
+                cells = windows[axis]
+                ghost, inner = (cells.start - 1, cells.start) if side == 'lo' else (cells.stop, cells.stop - 1)
+                planes.append((axis, kind, ghost, inner, cells.start - 1 if side == 'lo' else cells.stop - 1))
+        return planes
+
+
+MIRRORS_3D = ('symmetric', 'antisymmetric')
 
 
 def axis_profiles(bc, axis, n, pml_dx):
@@ -607,6 +626,13 @@ class Yee3D:
                 continue
             for which, (target, difference, sign) in enumerate(h_rows[layer.axis]):
                 layer.add(target, db, sign, layer.step(which, difference))
+        # symmetric mirrors: the tangential H on the plane face is pinned to 0 (engine3d.py:114-126, 513-518)
+        mirrors = self.bc.mirror_planes(self.windows)
+        for axis, kind, _, _, plane in mirrors:
+            if kind == 'symmetric':
+                for name in AXES:
+                    if name != AXES[axis]:
+                        F['h' + name][along(axis, plane)] = 0
         # pole J / P from E^n (engine3d.py:508-522)
         for entry in self.poles:
             for c, field in enumerate((ex, ey, ez)):
@@ -644,6 +670,16 @@ class Yee3D:
         t = self.time
         for source in self.sources:
             source.inject(F, t + 0.5 * float(self.dt), self.dtype)
+        # mirror E rows after the sources (engine3d.py:531-540): symmetric, the normal E on the plane face = 0;
+        # antisymmetric, the ghost's tangential E = minus its inner neighbour's
+        for axis, kind, ghost, inner, plane in mirrors:
+            if kind == 'symmetric':
+                F['e' + AXES[axis]][along(axis, plane)] = 0
+            else:
+                for name in AXES:
+                    if name != AXES[axis]:
+                        component = F['e' + name]
+                        component[along(axis, ghost)] = -component[along(axis, inner)]
         self.time = t + float(self.dt)
         self.n_step += 1
         for probe in self.probes:

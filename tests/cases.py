@@ -313,3 +313,34 @@ def build_tfsf2d(target, case, boundary_type, **options):
                           amplitude=spec['amplitude'])
     monitors = [sim.add_flux_monitor(**monitor) for monitor in spec['monitors']]
     return sim, source, monitors
+
+
+# 3D cases with mirror faces (tests/golden/make_mirrors3d_golden.py writes them from the reference): a ghost cell
+# padded beyond each mirror face, CPML on the other faces, two dielectric boxes, a dipole near the mirrors' corner and
+# a flux plane normal to x.
+MIRRORS3D = dict(window=(0.55e-6, 0.5e-6, 0.45e-6), dx=25e-9, npml=8,
+                 boxes=(((0.05e-6, 0.05e-6, 0.05e-6), (0.25e-6, 0.25e-6, 0.20e-6), 3.0),
+                        ((0.30e-6, 0.25e-6, 0.20e-6), (0.50e-6, 0.45e-6, 0.40e-6), 2.0)),
+                 wavelength=0.6e-6, plane_x=0.4e-6, wavelengths=(0.5e-6, 0.6e-6, 0.7e-6), n_steps=200,
+                 cases={'a': dict(faces={'x_lo': 'symmetric', 'y_hi': 'antisymmetric', 'z_lo': 'antisymmetric'},
+                                  dipole=(0.08e-6, 0.22e-6, 0.07e-6), orientation='z'),
+                        'b': dict(faces={'y_lo': 'symmetric', 'z_hi': 'symmetric', 'x_hi': 'antisymmetric'},
+                                  dipole=(0.2e-6, 0.06e-6, 0.38e-6), orientation='x')})
+
+
+def build_mirrors3d(target, name, **options):
+    """Mirror case `name` on the oracle Yee3D or on opensol_amd.Simulation3D; returns (sim, dipole, plane)."""
+    spec = MIRRORS3D
+    case = spec['cases'][name]
+    faces = {face: case['faces'].get(face, 'cpml') for face in ('x_lo', 'x_hi', 'y_lo', 'y_hi', 'z_lo', 'z_hi')}
+    sim = target(*spec['window'], spec['dx'], npml=spec['npml'], boundaries=faces, **options)
+    if hasattr(sim, 'host_centres'):   # the oracle: paint the array as the golden script does
+        for lo, hi, value in spec['boxes']:
+            sim.materials['eps_r'][poles3d_box_mask(sim.host_centres(), lo, hi)] = np.float32(value)
+    else:
+        for lo, hi, value in spec['boxes']:
+            sim.add_dielectric_box(lo, hi, value)
+    dipole = sim.add_dipole(*case['dipole'], orientation=case['orientation'], dipole_moment=1e-12,
+                            wavelength=spec['wavelength'], bandwidth=0.5)
+    plane = sim.add_flux_plane('x', spec['plane_x'], wavelengths=list(spec['wavelengths']))
+    return sim, dipole, plane

@@ -226,9 +226,8 @@ def test_wall_faces_are_e_z_only_and_padded_like_the_oracle():
     from opensol_amd.fdtd2d import WALL_SIGNS, wall_rows
     assert [WALL_SIGNS.get(k, 0) for k in faces] == [1, 1, -1, 0]
     assert wall_rows(model) == {0: {0, 48}, 1: {0}}
-    from opensol_amd.fdtd3d import Model3D
-    with pytest.raises(ValueError, match='2D E_z only'):
-        Model3D(0.5e-6, 0.5e-6, 0.5e-6, 25e-9, npml=4, boundaries={'x_lo': 'symmetric'})
+    from opensol_amd.fdtd3d import Model3D   # 3D mirrors pad one ghost cell too
+    assert Model3D(0.5e-6, 0.5e-6, 0.5e-6, 25e-9, npml=4, boundaries={'x_lo': 'symmetric'}).shape == (25, 28, 28)
     model.add_flux_monitor('y', 1.2e-6 - 1e-9, (0.2e-6, 0.8e-6), wavelengths=[600e-9])   # x cell 48: the pmc row
     with pytest.raises(ValueError, match='pmc wall row'):
         model.finalize()

@@ -234,3 +234,20 @@ def test_flux_box_faces_match_the_oracle():
             np.testing.assert_array_equal(pair['area'], ref_pair['area'])
     with pytest.raises(ValueError):
         model.add_flux_box((0.2e-6, 0.1e-6, 0.1e-6), (0.21e-6, 0.3e-6, 0.3e-6), wavelengths=[6e-7])
+
+
+@pytest.mark.parametrize('name', ['a', 'b'])
+def test_mirror_setup_matches_golden(name):
+    """Mirror faces pad one ghost cell: grid shape, painting, dipole cell and weight and the flux plane edge equal the
+    reference's golden run (tests/golden/mirrors3d.npz)."""
+    import os
+    from opensol_amd.fdtd3d import Model3D
+    from tests.cases import build_mirrors3d
+    golden = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden', 'mirrors3d.npz'))
+    model, dipole, plane = build_mirrors3d(Model3D, name)
+    model.finalize()
+    assert model.shape == tuple(golden[f'{name}_shape'])
+    np.testing.assert_array_equal(model.materials.property_array(0), golden[f'{name}_eps_r'])
+    assert dipole.cell == tuple(golden[f'{name}_dipole_cell'])
+    assert abs(float(dipole.weight) / float(golden[f'{name}_dipole_weight'][0]) - 1.0) < 3e-7
+    assert plane.edge == int(golden[f'{name}_plane_edge'])

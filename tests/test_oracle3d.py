@@ -217,3 +217,35 @@ def test_oracle_reproduces_reference_pole_case():
         error = _rel(sim.fields[f], golden[f].astype(np.float64))
         assert error < 5e-5, (f, error)
     np.testing.assert_allclose(plane.flux(), golden['flux'], rtol=1e-6)
+
+
+@pytest.mark.parametrize('name', ['a', 'b'])
+def test_mirror_faces_match_golden(name):
+    """Mirror faces (symmetric: tangential H and normal E on the plane pinned to 0; antisymmetric: the ghost cell's
+    tangential E = minus its inner neighbour's) on two corners: the FP64 oracle against the reference's float32 run
+    (fields within 5e-4 relative L2 after 200 steps, flux within 2e-5); the pinned rows are exactly 0 / exact images
+    in both."""
+    from tests.cases import MIRRORS3D, build_mirrors3d
+    golden = np.load(os.path.join(GOLDEN, 'mirrors3d.npz'))
+    sim, dipole, plane = build_mirrors3d(Yee3D, name)
+    sim.finalize()
+    assert sim.shape == tuple(golden[f'{name}_shape'])
+    np.testing.assert_array_equal(sim.materials['eps_r'], golden[f'{name}_eps_r'])
+    assert dipole.cell == tuple(golden[f'{name}_dipole_cell'])
+    assert plane.edge == int(golden[f'{name}_plane_edge'])
+    sim.run(n_steps=MIRRORS3D['n_steps'])
+    for f in ('ex', 'ey', 'ez', 'hx', 'hy', 'hz'):
+        assert _rel(golden[f"{name}_{f}"], sim.fields[f]) < 5e-5, f   # measured 1e-6 to 1.3e-5
+    np.testing.assert_allclose(plane.flux(), golden[f'{name}_flux'], rtol=2e-5)
+    for axis, kind, ghost, inner, face in sim.bc.mirror_planes(sim.windows):
+        others = [a for a in 'xyz' if a != 'xyz'[axis]]
+        for source in (golden, None):
+            get = (lambda c: golden[f'{name}_{c}']) if source is not None else (lambda c: sim.fields[c])
+            if kind == 'symmetric':
+                assert not np.any(np.take(get('e' + 'xyz'[axis]), face, axis=axis))
+                for o in others:
+                    assert not np.any(np.take(get('h' + o), face, axis=axis))
+            else:
+                for o in others:
+                    np.testing.assert_array_equal(np.take(get('e' + o), ghost, axis=axis),
+                                                  -np.take(get('e' + o), inner, axis=axis))
