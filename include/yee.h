@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 15
+#define YEE_ABI_VERSION 16
 #define YEE_MAX_SOURCES 8
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
@@ -244,8 +244,11 @@ typedef struct yee2d_desc {
     yee_probes probes;
     /* Outer-face rows per face (x_lo, x_hi, y_lo, y_hi). E_z: 0 none, +1 the row copies its inner neighbour after
      * the step (pmc, symmetric mirror), -1 the negated copy (antisymmetric mirror); x faces first, a corner taking
-     * both factors (physics/em/boundaries.py:478-488, wall_centre_rows :105-128). H_z: 0 none, YEE_WALL_ZERO H_z = 0
-     * on the outermost cells after the H update (pmc, :517-527). */
+     * both factors (physics/em/boundaries.py:478-488, wall_centre_rows :105-128). H_z (inside the step kernel, after
+     * the H update, the magnetic currents and TFSF): 0 none, YEE_WALL_ZERO H_z = 0 on the outermost cells (pmc),
+     * +1 the outermost H_z cells copy their inner neighbour and the tangential E on the plane face (face row 0 / n - 2)
+     * is 0 after the sources (pec, antisymmetric mirror), -1 the negated copy (symmetric mirror); a corner takes both
+     * faces' factors (:82-93, 500-538). */
     int32_t wall_sign[4];
     int32_t n_tfsf;
     int32_t pad5_;

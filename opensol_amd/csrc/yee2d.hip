@@ -682,12 +682,22 @@ struct HzPsi {
     float x[VEC], y[VEC], x_r, y_r;
 };
 
-template <int VEC, bool LDS>
+// H_z outer-face rules of a face (wall_sign): YEE_WALL_ZERO pins the outermost H_z cells to 0 (pmc); +1 copies the
+// inner neighbour into them (pec, antisymmetric mirror: H_z even) and pins the tangential E on the plane face (face
+// row 0 / n - 2, the inner face of the outermost cell); -1 copies it negated (symmetric mirror: H_z odd)
+// (physics/em/boundaries.py:82-93, 500-538).
+__device__ __forceinline__ float hz_copy_sign(int32_t wall) { return wall == 1 ? 1.f : (wall == -1 ? -1.f : 0.f); }
+
+// LEFT: the tile whose first column is the y_hi copy-wall column ny - 1 (a tile edge) recomputes H_z of column
+// ny - 2 in lane 0 from memory, for the copy (a separate kernel instantiation; the host selects it).
+template <int VEC, bool LDS, bool LEFT = false>
 struct HzTile {
     const yee2d_desc& d;
     int64_t n;
     int par, lane, nx, ny, P, jb_c, jl, jr, kr, nry, i0, i1;
     bool px, py, left_halo, right_halo, sample;
+    float s_xlo, s_xhi, s_ylo, s_yhi;   // copy-wall signs (0: no copy wall)
+    bool left_hz;                       // LEFT: this lane recomputes H_z of column jl = ny - 2
     int64_t sample_index;
     Columns<VEC> col;
     float idy_e_r;
@@ -761,6 +771,22 @@ struct HzTile {
             if (wall_cell(k, j)) v = 0.f;
             out[q] = j < ny ? v : 0.f;
         }
+        // copy walls on the y faces: column 0 from column 1 (the same lane), column ny - 1 from column ny - 2 (the same
+        // lane, the previous lane, or lane 0's recompute when ny - 1 starts a tile)
+        if (s_ylo != 0.f && col.jb == 0) out[0] = s_ylo * out[1];
+        if (s_yhi != 0.f) {
+            const float prev_lane = shfl_from_prev(out[VEC - 1]);
+            float left_value = 0.f;
+            if constexpr (LEFT) {
+                if (left_hz) left_value = hz_recompute(k, jl, xe);
+            }
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) {
+                if (col.jb + q != ny - 1) continue;
+                const float inner = q > 0 ? out[q > 0 ? q - 1 : 0] : (lane > 0 ? prev_lane : left_value);
+                out[q] = s_yhi * inner;
+            }
+        }
         if (store && col.any) {
             const int64_t row = (int64_t)k * P + col.jb;
             stv<VEC>(d.field[par ^ 1][0] + row, out);
@@ -781,7 +807,36 @@ struct HzTile {
             if (d.n_sources > 0) *out_r = magnetic_currents(k, jr, *out_r);
             if (d.n_tfsf > 0) *out_r = tfsf_hz_h(d, n, k, jr, *out_r);
             if (wall_cell(k, jr)) *out_r = 0.f;
+            if (s_yhi != 0.f && jr == ny - 1) *out_r = s_yhi * pickv<VEC>(out, VEC - 1);   // the copy of column ny - 2
         }
+    }
+
+    // H_z of cell (k, j) after this step from memory (old values), its CPML, magnetic currents, TFSF and pmc rows:
+    // the LEFT recompute of column ny - 2 (the same operands as hz_row, so the same bits).
+    __device__ __forceinline__ float hz_recompute(int k, int j, const RowCpml& xe) const
+    {
+        const yee_cpml_axis& cx = d.cpml[0];
+        const yee_cpml_axis& cy = d.cpml[1];
+        const int64_t row = (int64_t)k * P;
+        const float hz_o = d.field[par][0][row + j];
+        const float ex_j = d.field[par][1][row + j];
+        const int jm = j > 0 ? j - 1 : (py ? ny - 1 : -1);
+        const float ex_m = jm >= 0 ? d.field[par][1][row + jm] : 0.f;
+        const float ey_j = d.field[par][2][row + j];
+        const int kb = k > 0 ? k - 1 : (px ? nx - 1 : -1);
+        const float ey_b = kb >= 0 ? d.field[par][2][(int64_t)kb * P + j] : 0.f;
+        const int id = d.material[row + j];
+        const int ry = nry > 0 ? cy.row_e[j] : -1;
+        const float psx = xe.r >= 0 ? cx.psi_e[par][(int64_t)xe.r * P + j] : 0.f;
+        const float psy = ry >= 0 ? cy.psi_e[par][(int64_t)k * nry + ry] : 0.f;
+        float unused_x, unused_y;
+        float v = hz_centre(tab_a<LDS>(d, id), hz_o, ex_j - ex_m, ey_j - ey_b, d.inv_dy_e[j], d.inv_dx_e[k],
+                            xe.r >= 0, xe.b, xe.g, psx, &unused_x, ry >= 0, ry >= 0 ? cy.b_e[ry] : 0.f,
+                            ry >= 0 ? cy.g_e[ry] : 0.f, psy, &unused_y);
+        if (d.n_sources > 0) v = magnetic_currents(k, j, v);
+        if (d.n_tfsf > 0) v = tfsf_hz_h(d, n, k, j, v);
+        if (wall_cell(k, j)) v = 0.f;
+        return v;
     }
 
     // an outermost H_z cell of a pmc face (pinned to 0 after the H update and the magnetic currents)
@@ -834,6 +889,20 @@ struct HzTile {
         // (3) H_z of row k1 (stored when inside the tile), then the faces of row i
         float hz_next[VEC] = {}, hz_next_r = 0.f;
         if (ey_updated) hz_row(k1, nxt, cur.ey, cur.ey_r, xe, p, i + 1 < i1, hz_next, &hz_next_r);
+        // copy walls on the x faces: row 0 from row 1, row nx - 1 from row nx - 2 (after their y-face copies, so a
+        // corner takes both factors), stored again over the rows written above
+        if (s_xlo != 0.f && i == 0 && ey_updated) {
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) hz_cur[q] = s_xlo * hz_next[q];
+            hz_cur_r = s_xlo * hz_next_r;
+            if (col.any) stv<VEC>(d.field[par ^ 1][0] + row + col.jb, hz_cur);
+        }
+        if (s_xhi != 0.f && k1 == nx - 1) {
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) hz_next[q] = s_xhi * hz_cur[q];
+            hz_next_r = s_xhi * hz_cur_r;
+            if (i + 1 < i1 && col.any) stv<VEC>(d.field[par ^ 1][0] + (int64_t)k1 * P + col.jb, hz_next);
+        }
 
         const float hz_sh = shfl_from_next(hz_cur[0]);
         const int id_sh = __shfl_down(id_at(cur.ids, 0), 1, YEE_WAVE);
@@ -889,6 +958,18 @@ struct HzTile {
                 if (j >= ny) continue;
                 ex_v[q] = tfsf_hz_ex(d, n, i, j, ex_v[q], cb_x[q]);
                 if (ey_updated) ey_v[q] = tfsf_hz_ey(d, n, i, j, ey_v[q], cb_y[q]);
+            }
+        }
+        // tangential E pinned on the plane face of the pec / antisymmetric faces, after the sources
+        if ((d.wall_sign[0] == 1 && i == 0) || (d.wall_sign[1] == 1 && i == nx - 2)) {
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) ey_v[q] = 0.f;
+        }
+        if (d.wall_sign[2] == 1 || d.wall_sign[3] == 1) {
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) {
+                const int j = col.jb + q;
+                if ((d.wall_sign[2] == 1 && j == 0) || (d.wall_sign[3] == 1 && j == ny - 2)) ex_v[q] = 0.f;
             }
         }
         // sheet current on the hi faces at t + dt/2 (E_y for a line along y, E_x for a line along x)
@@ -969,6 +1050,11 @@ struct HzTile {
             idy_e_r = d.inv_dy_e[jr];
             if (nry > 0) ry_e_r = cy.row_e[jr];
         }
+        s_xlo = hz_copy_sign(d.wall_sign[0]);
+        s_xhi = hz_copy_sign(d.wall_sign[1]);
+        s_ylo = hz_copy_sign(d.wall_sign[2]);
+        s_yhi = hz_copy_sign(d.wall_sign[3]);
+        left_hz = LEFT && lane == 0 && col.jb == ny - 1 && s_yhi != 0.f;
         sample = d.n_monitors > 0 && ((n + 1) % d.sample_interval) == 0;
         sample_index = (n + 1) / (d.sample_interval > 0 ? d.sample_interval : 1) - 1;
 
@@ -1340,21 +1426,27 @@ __device__ __forceinline__ bool bulk_tile(const yee2d_tile_sets& s, int item, in
 // few, slow tiles would otherwise finish long after the bulk).
 constexpr int EDGE_SPLIT = 8;
 
-template <int POL, int VEC, bool EDGE, bool LDS, bool POLES = false>
+template <int POL, int VEC, bool EDGE, bool LDS, bool POLES = false, bool LEFT = false>
 __device__ __forceinline__ void tile_2d(const yee2d_desc& d, const TileBox& t, int64_t n, int part = 0,
                                         int parts = 1)
 {
     const bool ez = POL == YEE_EZ || (POL == POL_ANY && d.polarization == YEE_EZ);
     if constexpr (EDGE) {
         const int chunk = (t.i1 - t.i0 + parts - 1) / parts;
-        const int r0 = t.i0 + part * chunk;
-        const int r1 = min(t.i1, r0 + chunk);
+        int r0 = t.i0 + part * chunk;
+        int r1 = min(t.i1, r0 + chunk);
+        if (!ez && (d.wall_sign[1] == 1 || d.wall_sign[1] == -1)) {
+            // an H_z x_hi copy wall: row nx - 1 copies row nx - 2, so the chunk holding row nx - 2 takes row nx - 1
+            // too and no chunk starts at it
+            if (r0 == d.nx - 1 && r0 > 0) return;
+            if (r1 == d.nx - 1) r1 = d.nx;
+        }
         if (r0 >= r1) return;
         if (ez) {
             EzTile<VEC, LDS> tile(d, n);
             tile.run(t.tj, r0, r1);
         } else {
-            HzTile<VEC, LDS> tile(d, n);
+            HzTile<VEC, LDS, LEFT> tile(d, n);
             tile.run(t.tj, r0, r1);
         }
     } else {
@@ -1370,7 +1462,7 @@ __device__ __forceinline__ void tile_2d(const yee2d_desc& d, const TileBox& t, i
 namespace yee {
 
 // One block of the bulk (EDGE = false) or edge (EDGE = true) tile set; `block` counts blocks within that set.
-template <int POL, int VEC, bool EDGE, bool POLES>
+template <int POL, int VEC, bool EDGE, bool POLES, bool LEFT = false>
 __device__ __forceinline__ void step_block(const yee2d_desc& d, const yee2d_tile_sets& sets, int64_t n, int block)
 {
     const int parts = EDGE ? EDGE_SPLIT : 1;
@@ -1383,9 +1475,9 @@ __device__ __forceinline__ void step_block(const yee2d_desc& d, const yee2d_tile
     if (!mine) return;
     const TileBox t = tile_box<VEC>(d, ti * sets.tiles_j + tj);
     if (lds)
-        tile_2d<POL, VEC, EDGE, true, POLES>(d, t, n, slot % parts, parts);
+        tile_2d<POL, VEC, EDGE, true, POLES, LEFT>(d, t, n, slot % parts, parts);
     else
-        tile_2d<POL, VEC, EDGE, false, POLES>(d, t, n, slot % parts, parts);
+        tile_2d<POL, VEC, EDGE, false, POLES, LEFT>(d, t, n, slot % parts, parts);
 }
 
 }  // namespace yee
@@ -1400,7 +1492,7 @@ __device__ __forceinline__ void step_block(const yee2d_desc& d, const yee2d_tile
 // constant address space. Bound to the argument itself, the general H_z path took its address in a way that made the
 // compiler copy all 736 bytes to scratch and turn every field access into a flat load (168 VGPRs + 744 B/lane of
 // scratch); read in place, the fields are scalar loads (127 VGPRs, no scratch).
-template <int POL, int VEC, bool EDGE, bool POLES>
+template <int POL, int VEC, bool EDGE, bool POLES, bool LEFT>
 __global__ void __launch_bounds__(256) yee2d_step_kernel(yee2d_desc desc, yee2d_tile_sets sets, int32_t offset)
 {
     typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
@@ -1408,20 +1500,24 @@ __global__ void __launch_bounds__(256) yee2d_step_kernel(yee2d_desc desc, yee2d_
     if (d.clock != desc.clock) return;   // the argument layout is not the one assumed: do nothing
     const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
-    yee::step_block<POL, VEC, EDGE, POLES>(d, sets, n, blockIdx.x);
+    yee::step_block<POL, VEC, EDGE, POLES, LEFT>(d, sets, n, blockIdx.x);
 }
 
 // POLES: the E_z bulk loop with the dispersive-pole update (a separate instantiation, so the loop without poles keeps
-// its own register allocation); the general loop always carries it.
-#define YEE_STEP_KERNEL(POL, VEC, EDGE, POLES) \
-    template __global__ void yee2d_step_kernel<POL, VEC, EDGE, POLES>(yee2d_desc, yee2d_tile_sets, int32_t);
-#define YEE_STEP_KERNELS(POL, VEC) YEE_STEP_KERNEL(POL, VEC, false, false) YEE_STEP_KERNEL(POL, VEC, true, false)
+// its own register allocation); the general loop always carries it. LEFT: the H_z general loop with the lane-0
+// recompute of column ny - 2 for a y_hi copy wall on a tile edge.
+#define YEE_STEP_KERNEL(POL, VEC, EDGE, POLES, LEFT) \
+    template __global__ void yee2d_step_kernel<POL, VEC, EDGE, POLES, LEFT>(yee2d_desc, yee2d_tile_sets, int32_t);
+#define YEE_STEP_KERNELS(POL, VEC) \
+    YEE_STEP_KERNEL(POL, VEC, false, false, false) YEE_STEP_KERNEL(POL, VEC, true, false, false)
 YEE_STEP_KERNELS(YEE_EZ, 2)
 YEE_STEP_KERNELS(YEE_EZ, 4)
 YEE_STEP_KERNELS(YEE_HZ, 2)
 YEE_STEP_KERNELS(YEE_HZ, 4)
-YEE_STEP_KERNEL(YEE_EZ, 2, false, true)
-YEE_STEP_KERNEL(YEE_EZ, 4, false, true)
+YEE_STEP_KERNEL(YEE_HZ, 2, true, false, true)
+YEE_STEP_KERNEL(YEE_HZ, 4, true, false, true)
+YEE_STEP_KERNEL(YEE_EZ, 2, false, true, false)
+YEE_STEP_KERNEL(YEE_EZ, 4, false, true, false)
 
 // Many instances (E_z and H_z mixed) in one launch: tile t belongs to the instance whose tile range
 // [prefix[k], prefix[k+1]) holds it. Tables are read from global memory (they stay in L1/L2).

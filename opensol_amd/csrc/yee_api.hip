@@ -13,7 +13,7 @@
 #include "yee.h"
 #include "yee_launch.h"
 
-template <int POL, int VEC, bool EDGE, bool POLES>
+template <int POL, int VEC, bool EDGE, bool POLES, bool LEFT>
 __global__ void yee2d_step_kernel(yee2d_desc d, yee2d_tile_sets sets, int32_t offset);
 template <int VEC>
 __global__ void yee2d_batch_kernel(const yee2d_desc* __restrict__ descs, const int32_t* __restrict__ prefix,
@@ -202,9 +202,10 @@ int32_t check_2d(const yee2d_desc* d)
         const int n_axis = f < 2 ? d->nx : d->ny;
         if (d->polarization == YEE_EZ && sign != 1 && sign != -1)
             return fail(YEE_EINVAL, "E_z wall_sign must be -1, 0 or 1");
-        if (d->polarization == YEE_HZ && sign != YEE_WALL_ZERO)
-            return fail(YEE_EINVAL, "H_z wall_sign must be 0 or YEE_WALL_ZERO");
-        if (axis_periodic || n_axis < 2) return fail(YEE_EINVAL, "wall rows need a non-periodic axis of 2+ cells");
+        if (d->polarization == YEE_HZ && sign != YEE_WALL_ZERO && sign != 1 && sign != -1)
+            return fail(YEE_EINVAL, "H_z wall_sign must be -1, 0, 1 or YEE_WALL_ZERO");
+        if (axis_periodic || n_axis < (d->polarization == YEE_HZ ? 3 : 2))
+            return fail(YEE_EINVAL, "wall rows need a non-periodic axis (3+ cells for H_z, 2+ for E_z)");
     }
     if (d->n_tfsf < 0 || d->n_tfsf > YEE_MAX_TFSF) return fail(YEE_EINVAL, "n_tfsf out of range");
     for (int b = 0; b < d->n_tfsf; ++b) {
@@ -326,19 +327,33 @@ StepKernel3D step_kernel_3d(const yee3d_desc& d)
     return d.n_poles > 0 ? yee3d_step_kernel<kRows3D, true> : yee3d_step_kernel<kRows3D, false>;
 }
 
+// An H_z instance whose y_hi copy wall (pec or mirror) column ny - 1 is the first column of a tile: its edge tiles
+// run the general loop with lane 0's recompute of column ny - 2.
+bool needs_left_hz(const yee2d_desc& d)
+{
+    const int32_t w = d.wall_sign[3];
+    return d.polarization == YEE_HZ && (w == 1 || w == -1) && (d.ny - 1) % (64 * d.lane_columns) == 0;
+}
+
 StepKernel step_kernel(const yee2d_desc& d, bool edge)
 {
     if (d.polarization == YEE_EZ) {
         const bool poles = d.n_poles > 0;
         if (d.lane_columns == 2)
-            return edge ? yee2d_step_kernel<YEE_EZ, 2, true, false>
-                        : (poles ? yee2d_step_kernel<YEE_EZ, 2, false, true> : yee2d_step_kernel<YEE_EZ, 2, false, false>);
-        return edge ? yee2d_step_kernel<YEE_EZ, 4, true, false>
-                    : (poles ? yee2d_step_kernel<YEE_EZ, 4, false, true> : yee2d_step_kernel<YEE_EZ, 4, false, false>);
+            return edge ? yee2d_step_kernel<YEE_EZ, 2, true, false, false>
+                        : (poles ? yee2d_step_kernel<YEE_EZ, 2, false, true, false>
+                                 : yee2d_step_kernel<YEE_EZ, 2, false, false, false>);
+        return edge ? yee2d_step_kernel<YEE_EZ, 4, true, false, false>
+                    : (poles ? yee2d_step_kernel<YEE_EZ, 4, false, true, false>
+                             : yee2d_step_kernel<YEE_EZ, 4, false, false, false>);
     }
+    const bool left = needs_left_hz(d);
     if (d.lane_columns == 2)
-        return edge ? yee2d_step_kernel<YEE_HZ, 2, true, false> : yee2d_step_kernel<YEE_HZ, 2, false, false>;
-    return edge ? yee2d_step_kernel<YEE_HZ, 4, true, false> : yee2d_step_kernel<YEE_HZ, 4, false, false>;
+        return edge ? (left ? yee2d_step_kernel<YEE_HZ, 2, true, false, true>
+                            : yee2d_step_kernel<YEE_HZ, 2, true, false, false>)
+                    : yee2d_step_kernel<YEE_HZ, 2, false, false, false>;
+    return edge ? (left ? yee2d_step_kernel<YEE_HZ, 4, true, false, true> : yee2d_step_kernel<YEE_HZ, 4, true, false, false>)
+                : yee2d_step_kernel<YEE_HZ, 4, false, false, false>;
 }
 
 // Wavefront slots of the current device for a 256-thread kernel: resident blocks per CU x CUs x 4 waves.

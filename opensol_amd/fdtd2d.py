@@ -38,10 +38,6 @@ class Model2D:
         self.bc = resolve_boundaries(boundaries, npml)
         if self.bc.dimensions != 2:
             raise ValueError('a 2D model needs a 2D BoundaryConfig')
-        walls = [face for face, kind in self.bc.faces.items() if kind not in ('cpml', 'periodic')]
-        if walls and polarization != 'ez' and any(self.bc.faces[face] != 'pmc' for face in walls):
-            raise ValueError(f"2D faces {walls}: this FDTD path has 'pec' and mirror faces for the E_z "
-                             f"polarisation only, 'pmc' for both (physics/em/boundaries.py:449-527)")
         self.domain = Domain((size_x, size_y), dx, self.bc.npml_sides, courant)
         self.grid = self.domain.grid
         self.nx, self.ny = self.grid.shape
@@ -200,12 +196,13 @@ class Model2D:
         self.add_pec_region(mask)
 
     def pec_mask(self):
-        """bool (nx, ny): PEC regions and the outermost window row of every 'pec' face."""
+        """bool (nx, ny): PEC regions and, for E_z, the outermost window row of every 'pec' face (an H_z 'pec' face is
+        a wall copy in the step kernel instead, physics/em/boundaries.py:500-538)."""
         mask = np.zeros(self.shape, dtype=bool)
         for region in self.pec_regions:
             mask |= region
         window = self.domain.physical_window()
-        for face in self.bc.pec_faces():
+        for face in (self.bc.pec_faces() if self.polarization == 'ez' else ()):
             axis = 'xy'.index(face[0])
             index = window[axis].start if face.endswith('lo') else window[axis].stop - 1
             if axis == 0:
@@ -507,13 +504,18 @@ class Model2D:
 # E_z outer rows copied from their inner neighbour after each step: pmc and symmetric faces keep E_z even, an
 # antisymmetric face odd (physics/em/boundaries.py:478-488); 'pec' rows are PEC cells instead.
 WALL_SIGNS = {'pmc': 1, 'symmetric': 1, 'antisymmetric': -1}
+# H_z outer rows inside the step kernel (physics/em/boundaries.py:82-93, 500-538): pmc zeroes them, pec and
+# antisymmetric faces copy the inner row (H_z even, tangential E on the plane face pinned), symmetric faces copy it
+# negated.
+HZ_WALL_SIGNS = {'pmc': _lib.WALL_ZERO, 'pec': 1, 'antisymmetric': 1, 'symmetric': -1}
 
 
 def wall_rows(model):
-    """{axis: set of pinned outer rows} of the model's copied wall rows."""
+    """{axis: set of outer rows} of the model's E_z wall rows copied after the step kernel (the fused flux sample reads
+    E before that copy); H_z walls act inside the step kernel, before any sample."""
     rows = {0: set(), 1: set()}
     for face, kind in model.bc.faces.items():
-        if kind in WALL_SIGNS and (model.polarization == 'ez' or kind == 'pmc'):
+        if kind in WALL_SIGNS and model.polarization == 'ez':
             axis = 'xy'.index(face[0])
             rows[axis].add(0 if face.endswith('lo') else model.grid.shape[axis] - 1)
     return rows
@@ -696,8 +698,8 @@ class Engine2D:
         for f, face in enumerate(('x_lo', 'x_hi', 'y_lo', 'y_hi')):
             if model.polarization == 'ez':
                 d.wall_sign[f] = WALL_SIGNS.get(model.bc.faces[face], 0)
-            else:   # H_z pmc: H_z = 0 on the outermost cells
-                d.wall_sign[f] = _lib.WALL_ZERO if model.bc.faces[face] == 'pmc' else 0
+            else:
+                d.wall_sign[f] = HZ_WALL_SIGNS.get(model.bc.faces[face], 0)
         d.n_monitors = len(model.monitors)
         d.sample_interval = model.sample_interval
         for k, monitor in enumerate(model.monitors):

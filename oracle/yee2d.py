@@ -654,8 +654,6 @@ class Yee2D:
         self.sources, self.monitors, self.probes = [], [], []
         self.pec_regions = []
         self.poles = []
-        if polarization == 'hz' and any(kind not in ('cpml', 'periodic', 'pmc') for kind in self.bc.faces.values()):
-            raise ValueError("the oracle's H_z walls are pmc faces only")
         self.n_step, self.time = 0, 0.0
         self._finalized = False
         self._build_spacings_and_cpml()
@@ -739,7 +737,7 @@ class Yee2D:
         for region in self.pec_regions:
             mask |= region
         for face, kind in self.bc.faces.items():
-            if kind == 'pec':
+            if kind == 'pec' and self.polarization == 'ez':   # H_z 'pec' faces are wall copies (_hz_walls)
                 index = 0 if face.endswith('lo') else -1
                 if face[0] == 'x':
                     mask[index, :] = True
@@ -1052,6 +1050,8 @@ class Yee2D:
                 source.inject_e(self.fields, t_half, keep)
         if self.polarization == 'ez':
             self._ez_walls()
+        else:
+            self._exy_walls()
         self.n_step += 1
         self.time = t_full
         for probe in self.probes:
@@ -1059,6 +1059,27 @@ class Yee2D:
         for monitor in self.monitors:
             if self.n_step % monitor.sample_interval == 0:
                 monitor.update(self.fields, t_full)
+
+    HZ_CENTRE_ROWS = {'x_lo': (np.s_[0, :], np.s_[1, :]), 'x_hi': (np.s_[-1, :], np.s_[-2, :]),
+                      'y_lo': (np.s_[:, 0], np.s_[:, 1]), 'y_hi': (np.s_[:, -1], np.s_[:, -2])}
+    PLANE_FACE_ROWS = {'x_lo': np.s_[0, :], 'x_hi': np.s_[-2, :], 'y_lo': np.s_[:, 0], 'y_hi': np.s_[:, -2]}
+
+    def _hz_walls(self):
+        """Outer H_z rows (boundaries.py:82-93, 517-527): pec / antisymmetric faces H_z even, then pmc faces 0, then
+        symmetric faces H_z odd, each kind over the faces in order."""
+        hz = self.fields['hz']
+        for kinds, sign in ((('pec', 'antisymmetric'), 1.0), (('pmc',), 0.0), (('symmetric',), -1.0)):
+            for face in FACES:
+                if self.bc.faces[face] in kinds:
+                    outer, inner = self.HZ_CENTRE_ROWS[face]
+                    hz[outer] = 0 if sign == 0 else self.dtype(sign) * hz[inner]
+
+    def _exy_walls(self):
+        """Tangential E = 0 on the plane face of the pec / antisymmetric faces (E_y rows 0 / nx-2 on x faces, E_x
+        columns 0 / ny-2 on y faces; apply_exy_walls, boundaries.py:529-538), after the sources."""
+        for face in FACES:
+            if self.bc.faces[face] in ('pec', 'antisymmetric'):
+                self.fields['ey' if face[0] == 'x' else 'ex'][self.PLANE_FACE_ROWS[face]] = 0
 
     def _ez_walls(self):
         """Outer E_z rows after the sources (the re-pin of fdtd_ez_2d.py:105-117): x faces, then y faces, each row
@@ -1150,12 +1171,10 @@ class Yee2D:
                 source.inject(f, t_start)
             elif field == 'tfsf':
                 source.inject_h(f)
-        # pmc walls: H_z = 0 on the outermost cells, after the magnetic injections (apply_hz_walls,
-        # boundaries.py:517-527; re-pinned after injection, fdtd_hz_2d.py:165-170)
-        for face, outer in (('x_lo', np.s_[0, :]), ('x_hi', np.s_[-1, :]), ('y_lo', np.s_[:, 0]),
-                            ('y_hi', np.s_[:, -1])):
-            if self.bc.faces[face] == 'pmc':
-                hz[outer] = 0
+        # H_z walls after the magnetic injections (apply_hz_walls, boundaries.py:517-527; re-pinned after injection,
+        # fdtd_hz_2d.py:165-170): pec / antisymmetric faces copy the inner row (H_z even), pmc faces zero it,
+        # symmetric faces copy it negated (H_z odd); each kind over the faces in order (_set_outer_rows :82-93)
+        self._hz_walls()
         # pole updates on the E faces from E^n (fdtd_hz_2d.py:286-310)
         for entry in self.poles:
             for name, field in (('ex', ex), ('ey', ey)):

@@ -344,3 +344,36 @@ def build_mirrors3d(target, name, **options):
                             wavelength=spec['wavelength'], bandwidth=0.5)
     plane = sim.add_flux_plane('x', spec['plane_x'], wavelengths=list(spec['wavelengths']))
     return sim, dipole, plane
+
+
+# 2D H_z cases bounded by pec and mirror faces (tests/golden/make_hzwalls2d_golden.py writes them from the reference):
+# 'a' x_lo pec, x_hi symmetric, y_lo antisymmetric, y_hi pec around a dielectric, with a magnetic line current by the
+# (x_lo, y_lo) corner, an in-plane dipole by the y_hi wall and a line source along y; 'b' CPML on the lo faces, x_hi
+# symmetric and a y_hi pec wall at column 128 of 129 (the first column of a wavefront tile of 128 columns), a
+# magnetic line current beside that wall and a line source along x.
+HZWALLS2D = dict(dx=25e-9, npml=10, wavelength=600e-9, bandwidth=0.6, steps=(150, 400),
+                 cases={'a': dict(size=(1.0e-6, 1.0e-6), faces=('pec', 'symmetric', 'antisymmetric', 'pec'),
+                                  dielectric=((0.3e-6, 0.3e-6, 0.6e-6, 0.7e-6), 3.0),
+                                  magnetic=(0.04e-6, 0.06e-6, 2.0), dipole=(0.5e-6, 0.96e-6, 1e-3, 60.0),
+                                  line=dict(axis='y', position=0.8e-6, span=(0.1e-6, 0.9e-6))),
+                        'b': dict(size=(1.0e-6, 2.975e-6), faces=('cpml', 'symmetric', 'cpml', 'pec'),
+                                  dielectric=((0.2e-6, 1.0e-6, 0.8e-6, 2.5e-6), 2.5),
+                                  magnetic=(0.5e-6, 2.94e-6, 2.0), dipole=(0.7e-6, 2.96e-6, 1e-3, 0.0),
+                                  line=dict(axis='x', position=2.0e-6, span=(0.0, 1.0e-6)))})
+
+
+def build_hzwalls2d(target, name, boundary_type, **options):
+    """H_z wall case `name` on `target` (oracle Yee2D or opensol_amd Simulation)."""
+    spec = HZWALLS2D
+    case = spec['cases'][name]
+    sim = target(*case['size'], spec['dx'], npml=spec['npml'], boundaries=boundary_type(case['faces'], spec['npml']),
+                 polarization='hz', **options)
+    (x0, y0, x1, y1), eps = case['dielectric']
+    sim.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
+    pulse = dict(wavelength=spec['wavelength'], bandwidth=spec['bandwidth'])
+    x, y, km = case['magnetic']
+    sim.add_magnetic_line_current(x, y, magnetic_current=km, **pulse)
+    x, y, moment, angle = case['dipole']
+    sim.add_electric_dipole(x, y, dipole_moment=moment, orientation=angle, **pulse)
+    sim.add_line_source(**case['line'], **pulse)
+    return sim

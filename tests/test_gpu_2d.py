@@ -453,3 +453,76 @@ def test_empty_tfsf_box_leaks_nothing(gpu, polarization):
                   np.abs(field[:, y1 + 2:]).max())
     print(f'tfsf {polarization}: inside {inside:.3e}, outside {outside:.3e}')
     assert inside > 0.5 and outside < 1e-5 * inside
+
+
+@pytest.mark.parametrize('name', ['a', 'b'])
+def test_hz_walls_and_mirrors_match_oracle(gpu, name):
+    """H_z pec and mirror faces (even / odd H_z copies inside the step kernel, the tangential E pinned on the plane of
+    the pec and antisymmetric faces, ghost cells on mirrors) against the FP64 oracle, which reproduces the reference's
+    own runs to 1e-12: fields <= 1e-5 relative L2 at both steps; the copied rows exact; graph replay bit-identical.
+    Case 'b' puts the y_hi wall on column 128 of 129, the first column of a tile of 128."""
+    from tests.cases import HZWALLS2D, build_hzwalls2d
+    Simulation = simulation_class()
+    for lanes in (2, 4):
+        sim = build_hzwalls2d(Simulation, name, _product_boundaries, lane_columns=lanes)
+        graphed = build_hzwalls2d(Simulation, name, _product_boundaries, lane_columns=lanes, graph_steps=16)
+        ref = build_hzwalls2d(Yee2D, name, _oracle_boundaries)
+        done = 0
+        for step in HZWALLS2D['steps']:
+            print(f'H_z walls {name} lanes {lanes} to step {step}', flush=True)
+            ref.run(n_steps=step - done)
+            sim.run(n_steps=step - done, print_every=0)
+            graphed.run(n_steps=step - done, print_every=0)
+            done = step
+            for field in sim.model.components:
+                error = rel_l2(sim.engine.field(field), ref.fields[field])
+                print(f'H_z walls {name} lanes {lanes} step {step} {field}: rel L2 {error:.3e}')
+                assert error <= FIELD_TOL, (lanes, step, field, error)
+                np.testing.assert_array_equal(graphed.engine.field(field), sim.engine.field(field))
+        hz = sim.engine.field('hz')
+        faces = HZWALLS2D['cases'][name]['faces']
+        signs = {'pec': 1.0, 'antisymmetric': 1.0, 'symmetric': -1.0}
+        for face, (outer, inner) in zip(('x_lo', 'x_hi', 'y_lo', 'y_hi'),
+                                        ((np.s_[0], np.s_[1]), (np.s_[-1], np.s_[-2]), (np.s_[:, 0], np.s_[:, 1]),
+                                         (np.s_[:, -1], np.s_[:, -2]))):
+            kind = faces[('x_lo', 'x_hi', 'y_lo', 'y_hi').index(face)]
+            if kind in signs:
+                np.testing.assert_array_equal(hz[outer], signs[kind] * hz[inner])
+
+
+WALL_SIZES = [(17, 20), (18, 21), (16, 129), (15, 257)]
+
+
+@pytest.mark.parametrize('lanes', [2, 4])
+@pytest.mark.parametrize('shape', WALL_SIZES, ids=[f'{a}x{b}' for a, b in WALL_SIZES])
+def test_hz_wall_columns_random_fields(gpu, shape, lanes):
+    """Random fields in an H_z grid bounded by pec (x_lo), symmetric (x_hi), antisymmetric (y_lo) and pec (y_hi) faces
+    at widths that put column ny - 1 in the same lane as ny - 2, in the next lane, and on a tile edge (129 and 257
+    columns): 9 steps against the oracle, <= 5e-6 of each field's peak."""
+    Simulation = simulation_class()
+    wx, wy = shape
+    dx = 25e-9
+    faces = ('pec', 'symmetric', 'antisymmetric', 'pec')
+    sizes = ((wx - 1) * dx, (wy - 1) * dx)   # one ghost cell on each mirror face
+    ref = Yee2D(*sizes, dx, npml=1, boundaries=_oracle_boundaries(faces, 1), polarization='hz')
+    sim = Simulation(*sizes, dx, npml=1, boundaries=_product_boundaries(faces, 1), polarization='hz',
+                     lane_columns=lanes)
+    rng = np.random.default_rng(wy + lanes)
+    for x0, y0, x1, y1, eps in random_rectangles(rng, *sizes, 4):
+        for target in (ref, sim):
+            target.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
+    ref.finalize()
+    sim.finalize()
+    assert sim.model.shape == (ref.nx, ref.ny) == shape
+    for name in sim.model.components:
+        values = rng.standard_normal(shape).astype(np.float32)
+        ref.fields[name][...] = values
+        sim.engine.set_field(name, values)
+    for _ in range(9):
+        ref.step()
+    sim.run(n_steps=9, print_every=0)
+    for name in sim.model.components:
+        got, want = sim.engine.field(name), ref.fields[name]
+        error = np.max(np.abs(got - want)) / np.max(np.abs(want))
+        print(f'H_z walls {shape} lanes {lanes} {name}: {error:.3e}')
+        assert error <= 5e-6, (name, error)

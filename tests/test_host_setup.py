@@ -208,15 +208,21 @@ def test_point_current_records_match_the_reference(polarization):
             model.finalize()
 
 
-def test_wall_faces_are_e_z_only_and_padded_like_the_oracle():
-    """Wall and mirror faces: E_z only; mirror faces pad one ghost cell and walls none (the grid equals the oracle's);
-    the descriptor's wall factors; a flux line touching a pmc row is refused."""
+def test_wall_faces_are_padded_like_the_oracle():
+    """Wall and mirror faces: mirror faces pad one ghost cell and walls none (the grid equals the oracle's, both
+    polarisations); the descriptor's wall factors; an E_z flux line touching a pmc row is refused; an H_z 'pec' face
+    is a wall copy, not PEC cells."""
     from opensol_amd.boundaries import BoundaryConfig
+    from opensol_amd.fdtd2d import HZ_WALL_SIGNS
     from oracle.yee2d import FACES, Boundaries
-    for kind in ('pec', 'symmetric', 'antisymmetric'):   # H_z takes 'pmc' only
-        with pytest.raises(ValueError, match='E_z polarisation only'):
-            Model2D(1e-6, 1e-6, 25e-9, npml=5, boundaries=BoundaryConfig(kind, 'cpml', 'cpml', 'cpml'),
-                    polarization='hz')
+    for kind in ('pec', 'symmetric', 'antisymmetric', 'pmc'):
+        bc = BoundaryConfig(kind, 'cpml', 'cpml', 'cpml')
+        bc.cpml_npml = 5
+        hz = Model2D(1e-6, 1e-6, 25e-9, npml=5, boundaries=bc, polarization='hz')
+        ref = Yee2D(1e-6, 1e-6, 25e-9, npml=5, polarization='hz',
+                    boundaries=Boundaries(dict(zip(FACES, (kind, 'cpml', 'cpml', 'cpml'))), npml=5))
+        assert hz.shape == (ref.nx, ref.ny) and not hz.pec_mask().any()
+    assert HZ_WALL_SIGNS == {'pmc': 2, 'pec': 1, 'antisymmetric': 1, 'symmetric': -1}
     faces = ('symmetric', 'pmc', 'antisymmetric', 'cpml')
     bc = BoundaryConfig(*faces)
     bc.cpml_npml = 10

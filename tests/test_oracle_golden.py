@@ -257,3 +257,29 @@ def test_tfsf_without_scatterer_leaks_nothing():
         outside = max(np.abs(field[:x0 - 2]).max(), np.abs(field[x1 + 2:]).max(), np.abs(field[:, :y0 - 2]).max(),
                       np.abs(field[:, y1 + 2:]).max())
         assert inside > 0.5 and outside < 1e-6 * inside, (polarization, inside, outside)
+
+
+@pytest.mark.parametrize('name', ['a', 'b'])
+def test_oracle_reproduces_reference_hz_walls(name):
+    """H_z pec and mirror faces (even copies on pec / antisymmetric faces with the tangential E pinned on their plane,
+    odd copies on symmetric faces, ghost cells on mirrors, corners with both factors; 'b' with the y_hi wall on column
+    128 of 129): the FP64 oracle reproduces the reference's FP64 run to 1e-12 of the peak; the golden rows are exact
+    copies."""
+    from tests.cases import HZWALLS2D, build_hzwalls2d
+    golden = np.load(os.path.join(GOLDEN, 'hzwalls2d.npz'))
+    sim = build_hzwalls2d(Yee2D, name, _oracle_boundaries)
+    assert (sim.nx, sim.ny) == tuple(golden[f'{name}_shape'])
+    done = 0
+    for step in HZWALLS2D['steps']:
+        sim.run(n_steps=step - done)
+        done = step
+        for field, values in sim.fields.items():
+            want = golden[f'{name}_{field}_{step}']
+            assert np.abs(want).max() > 0
+            np.testing.assert_allclose(values, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    hz = golden[f'{name}_hz_400']
+    faces = HZWALLS2D['cases'][name]['faces']
+    if faces[3] == 'pec':
+        assert np.array_equal(hz[:, -1], hz[:, -2]) and np.abs(hz[:, -1]).max() > 0
+    if faces[1] == 'symmetric':
+        assert np.array_equal(hz[-1], -hz[-2])
