@@ -960,18 +960,6 @@ struct HzTile {
                 if (ey_updated) ey_v[q] = tfsf_hz_ey(d, n, i, j, ey_v[q], cb_y[q]);
             }
         }
-        // tangential E pinned on the plane face of the pec / antisymmetric faces, after the sources
-        if ((d.wall_sign[0] == 1 && i == 0) || (d.wall_sign[1] == 1 && i == nx - 2)) {
-#pragma unroll
-            for (int q = 0; q < VEC; ++q) ey_v[q] = 0.f;
-        }
-        if (d.wall_sign[2] == 1 || d.wall_sign[3] == 1) {
-#pragma unroll
-            for (int q = 0; q < VEC; ++q) {
-                const int j = col.jb + q;
-                if ((d.wall_sign[2] == 1 && j == 0) || (d.wall_sign[3] == 1 && j == ny - 2)) ex_v[q] = 0.f;
-            }
-        }
         // sheet current on the hi faces at t + dt/2 (E_y for a line along y, E_x for a line along x)
         for (int s = 0; s < d.n_sources; ++s) {
             const yee_line_source& src = d.source[s];
@@ -985,6 +973,19 @@ struct HzTile {
                 } else if (j == src.line && i >= src.lo && i < src.hi) {
                     ex_v[q] += __fmul_rn(w, src.profile[i - src.lo]);
                 }
+            }
+        }
+        // tangential E pinned on the plane face of the pec / antisymmetric faces, after the TFSF corrections and
+        // the sheet currents (a dipole face on the plane is zeroed again)
+        if ((d.wall_sign[0] == 1 && i == 0) || (d.wall_sign[1] == 1 && i == nx - 2)) {
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) ey_v[q] = 0.f;
+        }
+        if (d.wall_sign[2] == 1 || d.wall_sign[3] == 1) {
+#pragma unroll
+            for (int q = 0; q < VEC; ++q) {
+                const int j = col.jb + q;
+                if ((d.wall_sign[2] == 1 && j == 0) || (d.wall_sign[3] == 1 && j == ny - 2)) ex_v[q] = 0.f;
             }
         }
         // (4) stores

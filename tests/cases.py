@@ -155,6 +155,24 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / norm) if norm > 0 else float(np.linalg.norm(a))
 
 
+def update_vector_scales(fields, scales):
+    """Running maximum, over the checked steps, of the L2 norm of the whole E and of the whole H field (all their
+    components in `fields`). Float32 rounding is relative to the whole field, so a component that symmetry keeps
+    near zero (H_x of a z dipole between mirrors, H_z of a centred z dipole) or a field that has decayed since its
+    peak is measured against this scale, not against its own current norm."""
+    for kind in ('e', 'h'):
+        names = [name for name in fields if name[0] == kind]
+        if names:
+            norm = float(np.sqrt(sum(np.linalg.norm(np.asarray(fields[n], np.float64)) ** 2 for n in names)))
+            scales[kind] = max(scales.get(kind, 0.0), norm)
+    return scales
+
+
+def scaled_error(a, b, scales, name):
+    """||a - b|| over the running scale of the field vector that component `name` belongs to."""
+    diff = np.asarray(a, np.float64) - np.asarray(b, np.float64)
+    return float(np.linalg.norm(diff) / scales[name[0]])
+
 # 2D E_z case with dispersive poles (tests/golden/make_poles2d_golden.py writes it from the reference): a Drude
 # rectangle (eps_inf 2) under a PEC corner, a Lorentz disc over a dielectric rectangle, a line source and a flux line.
 POLES2D = dict(size_x=2e-6, size_y=1e-6, dx=25e-9, npml=10, boundaries='periodic_y',
