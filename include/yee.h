@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 16
+#define YEE_ABI_VERSION 17
 #define YEE_MAX_SOURCES 8
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
@@ -465,6 +465,21 @@ int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, 
  */
 size_t yee3d_batch_bytes(int32_t n);
 int32_t yee3d_batch_create(const yee3d_desc* descs, int32_t n, void* descs_dev, void* stream, yee_ctx** out);
+
+/*
+ * yee2d_batch_step_static: the no-allocation form of yee2d_batch_create + yee_step. No counterpart in the reference;
+ * it is for callers that cannot allocate device memory (tight serving loops, harnesses that forbid hipMalloc): it
+ * calls no hipMalloc / hipFree and takes no workspace. Steps n batched 2D instances (n <= YEE_STATIC_BATCH_MAX) by
+ * n_steps launches on `stream`, as a batch context would: each instance stops at its step_limit and keeps its own
+ * clock. The descriptors (lane_columns and rows_per_tile set, as yee2d_get_desc resolves them; the batch's other
+ * rules apply) and the tile prefix are copied on `stream` into a static device array of the library, one per GPU
+ * (sizeof(yee2d_desc) * YEE_STATIC_BATCH_MAX + 4 * (YEE_STATIC_BATCH_MAX + 1) bytes, held until the process ends),
+ * and `stream` is synchronized once after the copies, before the launches are enqueued. Every call on a GPU shares
+ * that GPU's array: calls on one stream are ordered, but calls that overlap on one GPU (other streams or threads)
+ * race. The current device must be the stream's device.
+ */
+#define YEE_STATIC_BATCH_MAX 256
+int32_t yee2d_batch_step_static(const yee2d_desc* descs, int32_t n, int64_t n_steps, void* stream);
 
 #ifdef __cplusplus
 }

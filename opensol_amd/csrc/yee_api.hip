@@ -7,6 +7,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -527,6 +528,22 @@ bool has_ez_walls(const yee2d_desc& d)
 // One lane per column (x faces) and per row (y faces).
 int wall_blocks(const yee2d_desc& d) { return (int)(((int64_t)d.nx + d.ny + 255) / 256); }
 
+// n_steps launches of the 2D batch kernel over descriptors and tile prefix already on the device, then the clocks.
+int32_t launch_batch_2d(const yee2d_desc* descs_dev, const int32_t* prefix_dev, int32_t n_inst, int64_t tiles,
+                        int32_t lane_columns, int32_t count, hipStream_t stream)
+{
+    auto kernel = lane_columns == 2 ? yee2d_batch_kernel<2> : yee2d_batch_kernel<4>;
+    const int blocks = (int)((tiles + 3) / 4);
+    for (int32_t k = 0; k < count; ++k)
+        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, stream, descs_dev, prefix_dev, n_inst, k);
+    YEE_HIP(hipGetLastError());
+    const int threads = 256;
+    hipLaunchKernelGGL(yee_clock_advance_batch, dim3((n_inst + threads - 1) / threads), dim3(threads), 0, stream,
+                       descs_dev, n_inst, (int64_t)count);
+    YEE_HIP(hipGetLastError());
+    return YEE_OK;
+}
+
 // Launch `count` steps (offsets 0..count-1 from the device clock) followed by the clock advance.
 int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
 {
@@ -590,18 +607,9 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
         YEE_HIP(hipGetLastError());
         return YEE_OK;
     }
-    if (ctx->kind == kBatch2D) {
-        auto kernel = ctx->descs[0].lane_columns == 2 ? yee2d_batch_kernel<2> : yee2d_batch_kernel<4>;
-        for (int32_t k = 0; k < count; ++k)
-            hipLaunchKernelGGL(kernel, dim3(ctx->blocks), dim3(256), 0, stream, ctx->descs_dev, ctx->prefix_dev,
-                               ctx->n_inst, k);
-        YEE_HIP(hipGetLastError());
-        const int threads = 256;
-        hipLaunchKernelGGL(yee_clock_advance_batch, dim3((ctx->n_inst + threads - 1) / threads), dim3(threads), 0,
-                           stream, ctx->descs_dev, ctx->n_inst, (int64_t)count);
-        YEE_HIP(hipGetLastError());
-        return YEE_OK;
-    }
+    if (ctx->kind == kBatch2D)
+        return launch_batch_2d(ctx->descs_dev, ctx->prefix_dev, ctx->n_inst, ctx->tiles, ctx->descs[0].lane_columns,
+                               count, stream);
     return fail(YEE_ESTATE, "context has no instances");
 }
 
@@ -623,6 +631,58 @@ int32_t build_graph(yee_ctx* ctx)
         ctx->graph = nullptr;
         return fail(YEE_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(inst));
     }
+    return YEE_OK;
+}
+
+// One member k of a 2D batch (defaults already resolved): the single-instance checks, the batch's restrictions, and
+// the tile prefix after it (*next = prefix + its tiles).
+int32_t check_batch_member_2d(const yee2d_desc& d, int32_t lane_columns, int32_t k, int32_t prefix, int32_t* next)
+{
+    int32_t status = check_2d(&d);
+    if (status == YEE_OK && d.lane_columns != lane_columns)
+        status = fail(YEE_EINVAL, "batched instances must share lane_columns");
+    if (status == YEE_OK && d.probes.n_probes > 0)
+        status = fail(YEE_EINVAL, "batched instances take no field probes");
+    if (status == YEE_OK && d.n_dft > 0) status = fail(YEE_EINVAL, "batched instances take no DFT regions");
+    if (status == YEE_OK && has_walls(d))
+        status = fail(YEE_EINVAL, "batched instances take no wall rows");
+    else if (status == YEE_OK && d.n_tfsf > 0)
+        status = fail(YEE_EINVAL, "batched instances take no TFSF boxes");
+    if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
+    const int64_t total = (int64_t)prefix + tiles_2d(d);
+    if (total > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
+    *next = (int32_t)total;
+    return YEE_OK;
+}
+
+// ---- the no-allocation batch entry point -------------------------------------------------------------------------
+// Descriptors and tile prefix of yee2d_batch_step_static live in this file-scope device array, laid out as
+// yee2d_batch_bytes(n) lays out a caller's buffer. Every GPU holds its own copy until the process ends. In host code
+// `scratch` names a host-side stand-in, not a device address: the address comes from hipGetSymbolAddress, looked up
+// once per device (with that device current) on the first call there, and only that address is used.
+constexpr size_t kStaticBatchBytes =
+    sizeof(yee2d_desc) * (size_t)YEE_STATIC_BATCH_MAX + sizeof(int32_t) * (size_t)(YEE_STATIC_BATCH_MAX + 1);
+static __device__ __align__(256) unsigned char scratch[kStaticBatchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_address[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device.
+int32_t scratch_address(void** out)
+{
+    int device = -1;
+    YEE_HIP(hipGetDevice(&device));
+    if (device < 0 || device >= kMaxDevices) return fail(YEE_EINVAL, "device index beyond the library's table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_address[device]) {
+        void* ptr = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess || !ptr)
+            return fail(YEE_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc));
+        g_scratch_address[device] = ptr;
+    }
+    *out = g_scratch_address[device];
     return YEE_OK;
 }
 
@@ -694,21 +754,8 @@ int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, 
     for (int32_t k = 0; k < n; ++k) {
         if (host[k].lane_columns == 0) host[k].lane_columns = host[0].lane_columns ? host[0].lane_columns : 2;
         if (host[k].rows_per_tile == 0) host[k].rows_per_tile = 32;
-        int32_t status = check_2d(&host[k]);
-        if (status == YEE_OK && host[k].lane_columns != host[0].lane_columns)
-            status = fail(YEE_EINVAL, "batched instances must share lane_columns");
-        if (status == YEE_OK && host[k].probes.n_probes > 0)
-            status = fail(YEE_EINVAL, "batched instances take no field probes");
-        if (status == YEE_OK && host[k].n_dft > 0)
-            status = fail(YEE_EINVAL, "batched instances take no DFT regions");
-        if (status == YEE_OK && has_walls(host[k]))
-            status = fail(YEE_EINVAL, "batched instances take no wall rows");
-        else if (status == YEE_OK && host[k].n_tfsf > 0)
-            status = fail(YEE_EINVAL, "batched instances take no TFSF boxes");
-        if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
-        const int64_t next = prefix[k] + tiles_2d(host[k]);
-        if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
-        prefix[k + 1] = (int32_t)next;
+        int32_t status = check_batch_member_2d(host[k], host[0].lane_columns, k, prefix[k], &prefix[k + 1]);
+        if (status != YEE_OK) return status;
     }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     auto* dev_descs = reinterpret_cast<yee2d_desc*>(descs_dev);
@@ -731,6 +778,43 @@ int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, 
 size_t yee2d_batch_bytes(int32_t n)
 {
     return n < 1 ? 0 : sizeof(yee2d_desc) * (size_t)n + sizeof(int32_t) * (size_t)(n + 1);
+}
+
+int32_t yee2d_batch_step_static(const yee2d_desc* descs, int32_t n, int64_t n_steps, void* stream)
+{
+    if (!descs || n < 1) return fail(YEE_EINVAL, "batch needs descriptors");
+    if (n > YEE_STATIC_BATCH_MAX)
+        return fail(YEE_EINVAL, "yee2d_batch_step_static takes at most " + std::to_string(YEE_STATIC_BATCH_MAX) +
+                                    " instances (its static device array), got " + std::to_string(n));
+    if (n_steps < 0) return fail(YEE_EINVAL, "n_steps must be >= 0");
+    int32_t prefix[YEE_STATIC_BATCH_MAX + 1];
+    prefix[0] = 0;
+    for (int32_t k = 0; k < n; ++k) {
+        if (descs[k].lane_columns == 0 || descs[k].rows_per_tile == 0)
+            return fail(YEE_EINVAL, "instance " + std::to_string(k) +
+                                        ": yee2d_batch_step_static copies descriptors as given and needs lane_columns "
+                                        "and rows_per_tile set");
+        int32_t status = check_batch_member_2d(descs[k], descs[0].lane_columns, k, prefix[k], &prefix[k + 1]);
+        if (status != YEE_OK) return status;
+    }
+    if (n_steps == 0) return YEE_OK;
+    void* base = nullptr;
+    int32_t status = scratch_address(&base);
+    if (status != YEE_OK) return status;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    auto* dev_descs = reinterpret_cast<yee2d_desc*>(base);
+    auto* dev_prefix = reinterpret_cast<int32_t*>(static_cast<char*>(base) + sizeof(yee2d_desc) * n);
+    // stream-ordered after this stream's earlier work (an earlier call's launches read the same array)
+    YEE_HIP(hipMemcpyAsync(dev_descs, descs, sizeof(yee2d_desc) * n, hipMemcpyHostToDevice, s));
+    YEE_HIP(hipMemcpyAsync(dev_prefix, prefix, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
+    YEE_HIP(hipStreamSynchronize(s));   // the prefix lives in this frame; the caller may reuse descs on return
+    for (int64_t remaining = n_steps; remaining > 0;) {
+        const int32_t chunk = remaining > 4096 ? 4096 : (int32_t)remaining;
+        status = launch_batch_2d(dev_descs, dev_prefix, n, prefix[n], descs[0].lane_columns, chunk, s);
+        if (status != YEE_OK) return status;
+        remaining -= chunk;
+    }
+    return YEE_OK;
 }
 
 int32_t yee2d_get_desc(const yee_ctx* ctx, yee2d_desc* out)
