@@ -92,6 +92,9 @@ struct yee_ctx {
     int32_t graph_steps = 0;
     hipGraphExec_t graph = nullptr;
     hipStream_t capture_stream = nullptr;
+    // 2D: the edge tile kernel runs beside the bulk one on a second stream (fork / join events)
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 };
 
 namespace {
@@ -544,6 +547,19 @@ int32_t launch_batch_2d(const yee2d_desc* descs_dev, const int32_t* prefix_dev, 
     return YEE_OK;
 }
 
+// The second stream and fork / join events of a 2D context whose step has both tile sets (made once, before any
+// capture). Default priority, non-blocking.
+bool concurrent_sets(const yee_ctx* ctx) { return ctx->blocks > 0 && ctx->edge_blocks > 0; }
+
+int32_t ensure_side(yee_ctx* ctx)
+{
+    if (ctx->kind != kSingle2D || !concurrent_sets(ctx) || ctx->side) return YEE_OK;
+    YEE_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+    YEE_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+    YEE_HIP(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
+    return YEE_OK;
+}
+
 // Launch `count` steps (offsets 0..count-1 from the device clock) followed by the clock advance.
 int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
 {
@@ -551,11 +567,27 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
     if (ctx->kind == kSingle2D) {
         const StepKernel bulk = step_kernel(ctx->d2, false);
         const StepKernel edge = step_kernel(ctx->d2, true);
+        const bool fork = concurrent_sets(ctx);
+        if (fork) {
+            int32_t status = ensure_side(ctx);
+            if (status != YEE_OK) return status;
+        }
         for (int32_t k = 0; k < count; ++k) {
-            if (ctx->blocks > 0)
+            if (fork) {
+                // the two tile sets are disjoint and both read state set n, write set n + 1: the edge tiles (few,
+                // latency-bound) run beside the bulk ones; everything after the step waits for both
+                YEE_HIP(hipEventRecord(ctx->fork_ev, stream));
+                YEE_HIP(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+                hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, ctx->side, ctx->d2, ctx->sets, k);
                 hipLaunchKernelGGL(bulk, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
-            if (ctx->edge_blocks > 0)
-                hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
+                YEE_HIP(hipEventRecord(ctx->join_ev, ctx->side));
+                YEE_HIP(hipStreamWaitEvent(stream, ctx->join_ev, 0));
+            } else {
+                if (ctx->blocks > 0)
+                    hipLaunchKernelGGL(bulk, dim3(ctx->blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
+                if (ctx->edge_blocks > 0)
+                    hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, stream, ctx->d2, ctx->sets, k);
+            }
             if (has_ez_walls(ctx->d2))
                 hipLaunchKernelGGL(yee2d_wall_kernel, dim3(wall_blocks(ctx->d2)), dim3(256), 0, stream, ctx->d2, k);
             if (ctx->d2.n_dft > 0)
@@ -616,6 +648,8 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
 int32_t build_graph(yee_ctx* ctx)
 {
     if (!ctx->capture_stream) YEE_HIP(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+    int32_t side_status = ensure_side(ctx);
+    if (side_status != YEE_OK) return side_status;
     YEE_HIP(hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal));
     int32_t status = launch_steps(ctx, ctx->graph_steps, ctx->capture_stream);
     hipGraph_t graph = nullptr;
@@ -1063,6 +1097,9 @@ int32_t yee_destroy(yee_ctx* ctx)
     if (!ctx) return YEE_OK;
     drop_graph(ctx);
     if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
     ctx->kind = 0;
     delete ctx;
     return YEE_OK;
