@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 17
+#define YEE_ABI_VERSION 18
 #define YEE_MAX_SOURCES 8
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
@@ -448,8 +448,10 @@ int32_t yee_destroy(yee_ctx* ctx);
 
 /*
  * Batched 2D instances (mixed E_z / H_z, uneven sizes) stepped by one launch per step over their tiles.
- * descs_dev: caller device buffer of yee2d_batch_bytes(n) bytes that receives the descriptors and the tile prefix
- * (copied on `stream`, which is synchronized once before the call returns). Each instance keeps its own clock and
+ * descs_dev: caller device buffer of yee2d_batch_bytes(n) bytes that receives the descriptors, each member's tile sets
+ * and the bulk and edge tile prefixes (copied on `stream`, which is synchronized once before the call returns). A step
+ * is two launches: the members' bulk tiles in the lean loops and their edge tiles (CPML, source and monitor lines,
+ * E_z and H_z pole rows) in the general loops. Each instance keeps its own clock and
  * step_limit, so instances of different lengths share launches and finished ones skip their tiles. No counterpart in
  * the reference, which steps one model at a time (model/study.py:437-447 builds one model per sweep value).
  */
@@ -472,8 +474,8 @@ int32_t yee3d_batch_create(const yee3d_desc* descs, int32_t n, void* descs_dev, 
  * calls no hipMalloc / hipFree and takes no workspace. Steps n batched 2D instances (n <= YEE_STATIC_BATCH_MAX) by
  * n_steps launches on `stream`, as a batch context would: each instance stops at its step_limit and keeps its own
  * clock. The descriptors (lane_columns and rows_per_tile set, as yee2d_get_desc resolves them; the batch's other
- * rules apply) and the tile prefix are copied on `stream` into a static device array of the library, one per GPU
- * (sizeof(yee2d_desc) * YEE_STATIC_BATCH_MAX + 4 * (YEE_STATIC_BATCH_MAX + 1) bytes, held until the process ends),
+ * rules apply), their tile sets and the tile prefixes are copied on `stream` into a static device array of the
+ * library, one per GPU (yee2d_batch_bytes(YEE_STATIC_BATCH_MAX) bytes, held until the process ends),
  * and `stream` is synchronized once after the copies, before the launches are enqueued. Every call on a GPU shares
  * that GPU's array: calls on one stream are ordered, but calls that overlap on one GPU (other streams or threads)
  * race. The current device must be the stream's device.

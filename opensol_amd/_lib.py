@@ -6,7 +6,7 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 17
+ABI_VERSION = 18
 MAX_SOURCES = 8
 SOURCE_E, SOURCE_H = 0, 1   # yee_line_source.field
 WALL_ZERO = 2   # yee2d_desc.wall_sign of an H_z pmc face

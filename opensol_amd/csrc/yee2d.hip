@@ -1520,10 +1520,14 @@ YEE_STEP_KERNEL(YEE_HZ, 4, true, false, true)
 YEE_STEP_KERNEL(YEE_EZ, 2, false, true, false)
 YEE_STEP_KERNEL(YEE_EZ, 4, false, true, false)
 
-// Many instances (E_z and H_z mixed) in one launch: tile t belongs to the instance whose tile range
-// [prefix[k], prefix[k+1]) holds it. Tables are read from global memory (they stay in L1/L2).
-template <int VEC>
+// Many instances (E_z and H_z mixed) in one launch, one kernel per tile class: with EDGE = false every member's bulk
+// tiles run the lean loops, with EDGE = true its edge tiles run the general loops, one wavefront per tile. Tile t of
+// the class belongs to the member whose range [prefix[k], prefix[k+1]) holds it; the member's tile sets (the host's
+// lists for that member, E_z pole rows among the edge rows) give the tile. Tables are read from global memory (they
+// stay in L1/L2).
+template <int VEC, bool EDGE>
 __global__ void __launch_bounds__(256) yee2d_batch_kernel(const yee2d_desc* __restrict__ descs,
+                                                         const yee2d_tile_sets* __restrict__ sets,
                                                          const int32_t* __restrict__ prefix, int32_t n_inst,
                                                          int32_t offset)
 {
@@ -1534,19 +1538,29 @@ __global__ void __launch_bounds__(256) yee2d_batch_kernel(const yee2d_desc* __re
         const int mid = (lo + hi) >> 1;
         if (prefix[mid] <= tile) lo = mid; else hi = mid;
     }
-    // Read the descriptor through the constant address space: its fields become scalar loads, as a kernel argument's.
+    // Read the descriptor and the tile sets through the constant address space: their fields become scalar loads, as
+    // a kernel argument's.
     typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
+    typedef const __attribute__((address_space(4))) yee2d_tile_sets ConstSets;
     const yee2d_desc& d = *(const yee2d_desc*)((ConstDesc*)descs + lo);
+    const yee2d_tile_sets& s = *(const yee2d_tile_sets*)((ConstSets*)sets + lo);
     const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
-    const yee::TileBox t = yee::tile_box<VEC>(d, tile - prefix[lo]);
-    yee::tile_2d<yee::POL_ANY, VEC, true, false>(d, t, n);
+    int ti = 0, tj = 0;
+    const int item = tile - prefix[lo];
+    if (!(EDGE ? yee::edge_tile(s, item, &ti, &tj) : yee::bulk_tile(s, item, &ti, &tj))) return;
+    const yee::TileBox t = yee::tile_box<VEC>(d, ti * s.tiles_j + tj);
+    yee::tile_2d<yee::POL_ANY, VEC, EDGE, false>(d, t, n);
 }
 
-template __global__ void yee2d_batch_kernel<2>(const yee2d_desc* __restrict__, const int32_t* __restrict__, int32_t,
-                                              int32_t);
-template __global__ void yee2d_batch_kernel<4>(const yee2d_desc* __restrict__, const int32_t* __restrict__, int32_t,
-                                              int32_t);
+#define YEE_BATCH_KERNEL(VEC, EDGE)                                                                              \
+    template __global__ void yee2d_batch_kernel<VEC, EDGE>(const yee2d_desc* __restrict__,                      \
+                                                           const yee2d_tile_sets* __restrict__,                 \
+                                                           const int32_t* __restrict__, int32_t, int32_t);
+YEE_BATCH_KERNEL(2, false)
+YEE_BATCH_KERNEL(2, true)
+YEE_BATCH_KERNEL(4, false)
+YEE_BATCH_KERNEL(4, true)
 
 // Field probes after step n = *clock + offset: samples[n * n_probes + k] = field[new set][component[k]][cell[k]]
 // (core/monitors.py:42-130), cell = i * pitch + j. One block; the descriptor is read in place as in yee2d_step_kernel.

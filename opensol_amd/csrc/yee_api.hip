@@ -16,9 +16,9 @@
 
 template <int POL, int VEC, bool EDGE, bool POLES, bool LEFT>
 __global__ void yee2d_step_kernel(yee2d_desc d, yee2d_tile_sets sets, int32_t offset);
-template <int VEC>
-__global__ void yee2d_batch_kernel(const yee2d_desc* __restrict__ descs, const int32_t* __restrict__ prefix,
-                                   int32_t n_inst, int32_t offset);
+template <int VEC, bool EDGE>
+__global__ void yee2d_batch_kernel(const yee2d_desc* __restrict__ descs, const yee2d_tile_sets* __restrict__ sets,
+                                   const int32_t* __restrict__ prefix, int32_t n_inst, int32_t offset);
 template <int J, bool POLES>
 __global__ void yee3d_step_kernel(yee3d_desc d, int32_t tiles_j, int32_t tiles_k, int32_t tiles, int32_t offset);
 template <int J>
@@ -74,6 +74,10 @@ struct yee_ctx {
     yee2d_desc* descs_dev = nullptr;
     int32_t* prefix_dev = nullptr;
     int32_t n_inst = 0;
+    // 2D batch: member tile sets and the bulk / general tile prefixes in the device buffer, and the tile counts
+    yee2d_tile_sets* sets_dev = nullptr;
+    int32_t* general_prefix_dev = nullptr;
+    int64_t general_tiles = 0;
     // launch geometry
     int64_t tiles = 0;
     int blocks = 0;        // blocks of the bulk kernel (4 tiles each)
@@ -391,7 +395,8 @@ void apply_defaults(yee2d_desc& d)
 // Edge tile rows and columns of a descriptor: tile rows whose rows (with the row before and the row after) reach an x
 // CPML layer or hold a source / monitor line along y, tile columns likewise for y. Too many lines: every tile is
 // an edge tile. E_z poles run in both loops; H_z pole rows are edge rows.
-yee2d_tile_sets tile_sets(const yee2d_desc& d)
+// batch: the member of a batched launch, whose lean loop has no pole update, so E_z pole rows are edge rows too.
+yee2d_tile_sets tile_sets(const yee2d_desc& d, bool batch = false)
 {
     yee2d_tile_sets s{};
     const int cols = 64 * d.lane_columns;
@@ -408,7 +413,7 @@ yee2d_tile_sets tile_sets(const yee2d_desc& d)
             edge = edge || (d.monitor[k].normal == 0 && d.monitor[k].position >= a && d.monitor[k].position <= b);
         if (d.polarization == YEE_HZ)   // H_z wall rows are pinned in the general loop
             edge = edge || (d.wall_sign[0] != 0 && a == 0) || (d.wall_sign[1] != 0 && b >= d.nx - 1);
-        for (int k = 0; d.polarization == YEE_HZ && k < d.n_poles; ++k)   // the lean H_z loop has no pole update
+        for (int k = 0; (d.polarization == YEE_HZ || batch) && k < d.n_poles; ++k)   // lean loops without poles
             edge = edge || (d.pole[k].i0 < i1 && d.pole[k].i1 > i0);
         for (int k = 0; k < d.n_tfsf; ++k) {   // TFSF corrected rows x0-1, x0, x1-1
             const yee_tfsf& t = d.tfsf[k];
@@ -531,29 +536,69 @@ bool has_ez_walls(const yee2d_desc& d)
 // One lane per column (x faces) and per row (y faces).
 int wall_blocks(const yee2d_desc& d) { return (int)(((int64_t)d.nx + d.ny + 255) / 256); }
 
-// n_steps launches of the 2D batch kernel over descriptors and tile prefix already on the device, then the clocks.
-int32_t launch_batch_2d(const yee2d_desc* descs_dev, const int32_t* prefix_dev, int32_t n_inst, int64_t tiles,
-                        int32_t lane_columns, int32_t count, hipStream_t stream)
+// Device buffer of a 2D batch: the member descriptors, their tile sets, then the bulk and the edge tile prefixes
+// (tile counts of members 0..k-1, n + 1 entries each).
+struct Batch2DLayout {
+    size_t sets, bulk_prefix, edge_prefix, bytes;
+};
+
+constexpr Batch2DLayout batch2d_layout(int32_t n)
 {
-    auto kernel = lane_columns == 2 ? yee2d_batch_kernel<2> : yee2d_batch_kernel<4>;
-    const int blocks = (int)((tiles + 3) / 4);
-    for (int32_t k = 0; k < count; ++k)
-        hipLaunchKernelGGL(kernel, dim3(blocks), dim3(256), 0, stream, descs_dev, prefix_dev, n_inst, k);
+    const size_t sets = sizeof(yee2d_desc) * (size_t)n;
+    const size_t bulk = sets + sizeof(yee2d_tile_sets) * (size_t)n;
+    const size_t edge = bulk + sizeof(int32_t) * (size_t)(n + 1);
+    return Batch2DLayout{sets, bulk, edge, edge + sizeof(int32_t) * (size_t)(n + 1)};
+}
+
+// A 2D batch on the device: where its arrays are and how many tiles of each class it has.
+struct Batch2DArgs {
+    const yee2d_desc* descs;
+    const yee2d_tile_sets* sets;
+    const int32_t* bulk_prefix;
+    const int32_t* edge_prefix;
+    int32_t n;
+    int64_t bulk_tiles, edge_tiles;
+    int32_t lane_columns;
+};
+
+Batch2DArgs batch2d_args(void* base, int32_t n, int64_t bulk_tiles, int64_t edge_tiles, int32_t lane_columns)
+{
+    const Batch2DLayout l = batch2d_layout(n);
+    char* b = static_cast<char*>(base);
+    return Batch2DArgs{reinterpret_cast<const yee2d_desc*>(b), reinterpret_cast<const yee2d_tile_sets*>(b + l.sets),
+                       reinterpret_cast<const int32_t*>(b + l.bulk_prefix),
+                       reinterpret_cast<const int32_t*>(b + l.edge_prefix), n, bulk_tiles, edge_tiles, lane_columns};
+}
+
+// count launches of the batched bulk-tile kernel (the lean loops) and of the batched edge-tile kernel (the general
+// loops, one wavefront per tile: the many members keep the device busy), then the clocks.
+int32_t launch_batch_2d(const Batch2DArgs& b, int32_t count, hipStream_t stream)
+{
+    auto bulk = b.lane_columns == 2 ? yee2d_batch_kernel<2, false> : yee2d_batch_kernel<4, false>;
+    auto edge = b.lane_columns == 2 ? yee2d_batch_kernel<2, true> : yee2d_batch_kernel<4, true>;
+    const int bulk_blocks = (int)((b.bulk_tiles + 3) / 4);
+    const int edge_blocks = (int)((b.edge_tiles + 3) / 4);
+    for (int32_t k = 0; k < count; ++k) {
+        if (bulk_blocks > 0)
+            hipLaunchKernelGGL(bulk, dim3(bulk_blocks), dim3(256), 0, stream, b.descs, b.sets, b.bulk_prefix, b.n, k);
+        if (edge_blocks > 0)
+            hipLaunchKernelGGL(edge, dim3(edge_blocks), dim3(256), 0, stream, b.descs, b.sets, b.edge_prefix, b.n, k);
+    }
     YEE_HIP(hipGetLastError());
     const int threads = 256;
-    hipLaunchKernelGGL(yee_clock_advance_batch, dim3((n_inst + threads - 1) / threads), dim3(threads), 0, stream,
-                       descs_dev, n_inst, (int64_t)count);
+    hipLaunchKernelGGL(yee_clock_advance_batch, dim3((b.n + threads - 1) / threads), dim3(threads), 0, stream,
+                       b.descs, b.n, (int64_t)count);
     YEE_HIP(hipGetLastError());
     return YEE_OK;
 }
 
 // The second stream and fork / join events of a 2D context whose step has both tile sets (made once, before any
 // capture). Default priority, non-blocking.
-bool concurrent_sets(const yee_ctx* ctx) { return ctx->blocks > 0 && ctx->edge_blocks > 0; }
+bool concurrent_sets(const yee_ctx* ctx) { return ctx->kind == kSingle2D && ctx->blocks > 0 && ctx->edge_blocks > 0; }
 
 int32_t ensure_side(yee_ctx* ctx)
 {
-    if (ctx->kind != kSingle2D || !concurrent_sets(ctx) || ctx->side) return YEE_OK;
+    if (!concurrent_sets(ctx) || ctx->side) return YEE_OK;
     YEE_HIP(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
     YEE_HIP(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
     YEE_HIP(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
@@ -640,7 +685,8 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
         return YEE_OK;
     }
     if (ctx->kind == kBatch2D)
-        return launch_batch_2d(ctx->descs_dev, ctx->prefix_dev, ctx->n_inst, ctx->tiles, ctx->descs[0].lane_columns,
+        return launch_batch_2d(batch2d_args(ctx->descs_dev, ctx->n_inst, ctx->tiles, ctx->general_tiles,
+                                            ctx->descs[0].lane_columns),
                                count, stream);
     return fail(YEE_ESTATE, "context has no instances");
 }
@@ -668,9 +714,8 @@ int32_t build_graph(yee_ctx* ctx)
     return YEE_OK;
 }
 
-// One member k of a 2D batch (defaults already resolved): the single-instance checks, the batch's restrictions, and
-// the tile prefix after it (*next = prefix + its tiles).
-int32_t check_batch_member_2d(const yee2d_desc& d, int32_t lane_columns, int32_t k, int32_t prefix, int32_t* next)
+// One member k of a 2D batch (defaults already resolved): the single-instance checks and the batch's restrictions.
+int32_t check_batch_member_2d(const yee2d_desc& d, int32_t lane_columns, int32_t k)
 {
     int32_t status = check_2d(&d);
     if (status == YEE_OK && d.lane_columns != lane_columns)
@@ -683,20 +728,46 @@ int32_t check_batch_member_2d(const yee2d_desc& d, int32_t lane_columns, int32_t
     else if (status == YEE_OK && d.n_tfsf > 0)
         status = fail(YEE_EINVAL, "batched instances take no TFSF boxes");
     if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
-    const int64_t total = (int64_t)prefix + tiles_2d(d);
-    if (total > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
-    *next = (int32_t)total;
+    return YEE_OK;
+}
+
+// The device image of a 2D batch (batch2d_layout(n).bytes bytes at `image`) from member descriptors whose defaults
+// are resolved: checks every member, then writes the descriptors, their tile sets and the two tile prefixes.
+int32_t stage_batch_2d(const yee2d_desc* descs, int32_t n, char* image, int64_t* bulk_tiles, int64_t* edge_tiles)
+{
+    const Batch2DLayout l = batch2d_layout(n);
+    auto* sets = reinterpret_cast<yee2d_tile_sets*>(image + l.sets);
+    auto* bulk = reinterpret_cast<int32_t*>(image + l.bulk_prefix);
+    auto* edge = reinterpret_cast<int32_t*>(image + l.edge_prefix);
+    bulk[0] = edge[0] = 0;
+    for (int32_t k = 0; k < n; ++k) {
+        int32_t status = check_batch_member_2d(descs[k], descs[0].lane_columns, k);
+        if (status != YEE_OK) return status;
+        std::memcpy(image + sizeof(yee2d_desc) * (size_t)k, &descs[k], sizeof(yee2d_desc));
+        sets[k] = tile_sets(descs[k], true);
+        const yee2d_tile_sets& t = sets[k];
+        const int64_t all = (int64_t)t.tiles_i * t.tiles_j;
+        const int64_t b = t.all_edge ? 0 : (int64_t)(t.tiles_i - t.n_rows) * std::max(0, t.tiles_j - t.n_cols);
+        const int64_t next_bulk = (int64_t)bulk[k] + b, next_edge = (int64_t)edge[k] + (all - b);
+        if (next_bulk > INT32_MAX / 4 || next_edge > INT32_MAX / 4)
+            return fail(YEE_EINVAL, "batch has too many tiles");
+        bulk[k + 1] = (int32_t)next_bulk;
+        edge[k + 1] = (int32_t)next_edge;
+    }
+    *bulk_tiles = bulk[n];
+    *edge_tiles = edge[n];
     return YEE_OK;
 }
 
 // ---- the no-allocation batch entry point -------------------------------------------------------------------------
-// Descriptors and tile prefix of yee2d_batch_step_static live in this file-scope device array, laid out as
+// Descriptors, tile sets and tile prefixes of yee2d_batch_step_static live in this file-scope device array, laid out as
 // yee2d_batch_bytes(n) lays out a caller's buffer. Every GPU holds its own copy until the process ends. In host code
 // `scratch` names a host-side stand-in, not a device address: the address comes from hipGetSymbolAddress, looked up
 // once per device (with that device current) on the first call there, and only that address is used.
-constexpr size_t kStaticBatchBytes =
-    sizeof(yee2d_desc) * (size_t)YEE_STATIC_BATCH_MAX + sizeof(int32_t) * (size_t)(YEE_STATIC_BATCH_MAX + 1);
+constexpr size_t kStaticBatchBytes = batch2d_layout(YEE_STATIC_BATCH_MAX).bytes;
 static __device__ __align__(256) unsigned char scratch[kStaticBatchBytes];
+// the host image the static entry point copies from (no heap allocation; used under g_scratch_mutex)
+alignas(256) char g_static_image[kStaticBatchBytes];
 
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mutex;
@@ -784,35 +855,31 @@ int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, 
     *out = nullptr;
     if (!descs || n < 1 || !descs_dev) return fail(YEE_EINVAL, "batch needs descriptors and a device buffer");
     std::vector<yee2d_desc> host(descs, descs + n);
-    std::vector<int32_t> prefix(n + 1, 0);
     for (int32_t k = 0; k < n; ++k) {
         if (host[k].lane_columns == 0) host[k].lane_columns = host[0].lane_columns ? host[0].lane_columns : 2;
         if (host[k].rows_per_tile == 0) host[k].rows_per_tile = 32;
-        int32_t status = check_batch_member_2d(host[k], host[0].lane_columns, k, prefix[k], &prefix[k + 1]);
-        if (status != YEE_OK) return status;
     }
+    const Batch2DLayout l = batch2d_layout(n);
+    std::vector<char> image(l.bytes);
+    int64_t bulk_tiles = 0, edge_tiles = 0;
+    int32_t status = stage_batch_2d(host.data(), n, image.data(), &bulk_tiles, &edge_tiles);
+    if (status != YEE_OK) return status;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    auto* dev_descs = reinterpret_cast<yee2d_desc*>(descs_dev);
-    auto* dev_prefix = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(descs_dev) + sizeof(yee2d_desc) * n);
-    YEE_HIP(hipMemcpyAsync(dev_descs, host.data(), sizeof(yee2d_desc) * n, hipMemcpyHostToDevice, s));
-    YEE_HIP(hipMemcpyAsync(dev_prefix, prefix.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
-    YEE_HIP(hipStreamSynchronize(s));   // the host vectors are the copies' sources
+    YEE_HIP(hipMemcpyAsync(descs_dev, image.data(), l.bytes, hipMemcpyHostToDevice, s));
+    YEE_HIP(hipStreamSynchronize(s));   // the host image is the copy's source
     yee_ctx* ctx = new yee_ctx();
     ctx->kind = kBatch2D;
     ctx->descs = std::move(host);
-    ctx->descs_dev = dev_descs;
-    ctx->prefix_dev = dev_prefix;
+    ctx->descs_dev = reinterpret_cast<yee2d_desc*>(descs_dev);
     ctx->n_inst = n;
-    ctx->tiles = prefix[n];
-    ctx->blocks = (int)((ctx->tiles + 3) / 4);
+    ctx->tiles = bulk_tiles;
+    ctx->general_tiles = edge_tiles;
+    ctx->blocks = (int)((bulk_tiles + 3) / 4);
     *out = ctx;
     return YEE_OK;
 }
 
-size_t yee2d_batch_bytes(int32_t n)
-{
-    return n < 1 ? 0 : sizeof(yee2d_desc) * (size_t)n + sizeof(int32_t) * (size_t)(n + 1);
-}
+size_t yee2d_batch_bytes(int32_t n) { return n < 1 ? 0 : batch2d_layout(n).bytes; }
 
 int32_t yee2d_batch_step_static(const yee2d_desc* descs, int32_t n, int64_t n_steps, void* stream)
 {
@@ -821,30 +888,30 @@ int32_t yee2d_batch_step_static(const yee2d_desc* descs, int32_t n, int64_t n_st
         return fail(YEE_EINVAL, "yee2d_batch_step_static takes at most " + std::to_string(YEE_STATIC_BATCH_MAX) +
                                     " instances (its static device array), got " + std::to_string(n));
     if (n_steps < 0) return fail(YEE_EINVAL, "n_steps must be >= 0");
-    int32_t prefix[YEE_STATIC_BATCH_MAX + 1];
-    prefix[0] = 0;
-    for (int32_t k = 0; k < n; ++k) {
+    for (int32_t k = 0; k < n; ++k)
         if (descs[k].lane_columns == 0 || descs[k].rows_per_tile == 0)
             return fail(YEE_EINVAL, "instance " + std::to_string(k) +
                                         ": yee2d_batch_step_static copies descriptors as given and needs lane_columns "
                                         "and rows_per_tile set");
-        int32_t status = check_batch_member_2d(descs[k], descs[0].lane_columns, k, prefix[k], &prefix[k + 1]);
-        if (status != YEE_OK) return status;
-    }
     if (n_steps == 0) return YEE_OK;
     void* base = nullptr;
     int32_t status = scratch_address(&base);
     if (status != YEE_OK) return status;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    auto* dev_descs = reinterpret_cast<yee2d_desc*>(base);
-    auto* dev_prefix = reinterpret_cast<int32_t*>(static_cast<char*>(base) + sizeof(yee2d_desc) * n);
-    // stream-ordered after this stream's earlier work (an earlier call's launches read the same array)
-    YEE_HIP(hipMemcpyAsync(dev_descs, descs, sizeof(yee2d_desc) * n, hipMemcpyHostToDevice, s));
-    YEE_HIP(hipMemcpyAsync(dev_prefix, prefix, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
-    YEE_HIP(hipStreamSynchronize(s));   // the prefix lives in this frame; the caller may reuse descs on return
+    int64_t bulk_tiles = 0, edge_tiles = 0;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        status = stage_batch_2d(descs, n, g_static_image, &bulk_tiles, &edge_tiles);
+        if (status != YEE_OK) return status;
+        // stream-ordered after this stream's earlier work (an earlier call's launches read the same array); the
+        // host image is reused by the next call, so the copy completes before the lock is released
+        YEE_HIP(hipMemcpyAsync(base, g_static_image, batch2d_layout(n).bytes, hipMemcpyHostToDevice, s));
+        YEE_HIP(hipStreamSynchronize(s));
+    }
+    const Batch2DArgs args = batch2d_args(base, n, bulk_tiles, edge_tiles, descs[0].lane_columns);
     for (int64_t remaining = n_steps; remaining > 0;) {
         const int32_t chunk = remaining > 4096 ? 4096 : (int32_t)remaining;
-        status = launch_batch_2d(dev_descs, dev_prefix, n, prefix[n], descs[0].lane_columns, chunk, s);
+        status = launch_batch_2d(args, chunk, s);
         if (status != YEE_OK) return status;
         remaining -= chunk;
     }
