@@ -156,7 +156,11 @@ def _serve_3d(seed):
 
 def run_serve(args, torch):
     from opensol_amd.serve import FDTDServer
-    server = FDTDServer(hbm_budget=200e9, chunk_steps=100, graph_steps=0)
+    server = FDTDServer(hbm_budget=200e9, chunk_steps=100, graph_steps=0, autotune=True)
+    tune_start = time.perf_counter()   # the job classes' batch-member shapes, ahead of the timed streams
+    classes = server.tune_classes([job for rate in (50, 100, 200, 400) for job in serve_jobs(rate, args.jobs)])
+    tune_s = time.perf_counter() - tune_start
+    print(f'# autotuned {len(classes)} job classes in {tune_s:.1f} s', file=sys.stderr, flush=True)
     rates = {}
     for rate in (50, 100, 200, 400):
         served = server.run(serve_jobs(rate, args.jobs))
@@ -176,7 +180,7 @@ def run_serve(args, torch):
             'ms_per_step': None, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32',
             'data': 'synthetic (seeded Poisson job stream, 80 % 2D / 20 % 3D)',
             'config': {'workload': 'serve_2d3d_poisson', 'jobs_per_rate': args.jobs, 'hbm_budget_B': 200e9,
-                       'chunk_steps': 100, 'seed': SEED},
+                       'chunk_steps': 100, 'seed': SEED, 'autotune': {'classes': len(classes), 'seconds': tune_s}},
             'rates': rates, 'roofline': None, 'cpu_baseline': None}
 
 
@@ -452,7 +456,14 @@ def run_batch(args, torch):
     torch.cuda.synchronize()
     plain_s = start.elapsed_time(stop) / 1e3
     print(f'# plain: {plain_s:.3f} s, {cell_updates / plain_s:.4e} cell-updates/s', file=sys.stderr, flush=True)
-    batch = Batch2D(batch_sims, graph_steps=args.graph_steps)
+    tune_start = time.perf_counter()   # each member's rows per tile from its size class (tuned once per device)
+    batch = Batch2D(batch_sims, graph_steps=args.graph_steps, autotune=True)
+    tune_s = time.perf_counter() - tune_start
+    chosen = {}
+    for sim in batch_sims:
+        rows = int(sim._engine_options.get('rows_per_tile') or 0)
+        chosen[str(rows)] = chosen.get(str(rows), 0) + 1
+    print(f'# autotuned members in {tune_s:.2f} s: rows per tile {chosen}', file=sys.stderr, flush=True)
     batch.prepare(steps)   # descriptors and the step graph, outside the timed region
     torch.cuda.synchronize()
     start.record()
@@ -473,7 +484,8 @@ def run_batch(args, torch):
             'ms_per_step': batch_s / launches * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
             'dtype': 'f32', 'data': 'synthetic (seeded instances: rectangles, Gaussian line source, flux line)',
             'config': {'workload': 'batch256_2d_ez_hz_128to1024', 'instances': len(steps), 'seed': SEED,
-                       'cell_updates': cell_updates, 'graph_steps': args.graph_steps},
+                       'cell_updates': cell_updates, 'graph_steps': args.graph_steps,
+                       'autotune': {'seconds': tune_s, 'rows_per_tile': chosen}},
             'plain': {'value': cell_updates / plain_s, 'seconds': plain_s},
             'batched': {'value': value, 'seconds': batch_s, 'launches': launches},
             'speedup_vs_plain': value / (cell_updates / plain_s),

@@ -20,7 +20,7 @@ class _Batch:
 
     three_d = False
 
-    def __init__(self, simulations, graph_steps=0):
+    def __init__(self, simulations, graph_steps=0, autotune=False):
         if not simulations:
             raise ValueError('a batch needs at least one simulation')
         self.simulations = list(simulations)
@@ -34,6 +34,8 @@ class _Batch:
             if (len(sim.model.shape) == 3) != self.three_d:
                 raise ValueError(f'{type(self).__name__} steps {"3D" if self.three_d else "2D"} simulations')
             self._set_options(sim)
+            if autotune and sim.engine is None:   # the member's size class, tuned once per device (autotune.py)
+                self._tune(sim)
             sim.finalize()
         self.lib = _lib.load()
 
@@ -115,15 +117,22 @@ class _Batch:
 
 
 class Batch2D(_Batch):
-    """2D Simulations (E_z and H_z mixed) stepped together; every instance uses `lane_columns` columns per lane."""
+    """2D Simulations (E_z and H_z mixed) stepped together; every instance uses `lane_columns` columns per lane.
+    autotune=True chooses each not-yet-finalized member's rows per tile by its size class (autotune.py)."""
 
     _desc_type = _lib.Desc2DC
     _create_name = 'yee2d_batch_create'
 
-    def __init__(self, simulations, graph_steps=0, lane_columns=2, rows_per_tile=0):
+    def __init__(self, simulations, graph_steps=0, lane_columns=2, rows_per_tile=0, autotune=False):
         self.lane_columns = int(lane_columns)
         self.rows_per_tile = int(rows_per_tile)
-        super().__init__(simulations, graph_steps)
+        if autotune and self.rows_per_tile:
+            raise ValueError('Batch2D: autotune chooses rows_per_tile; pass one or the other')
+        super().__init__(simulations, graph_steps, autotune)
+
+    def _tune(self, sim):
+        from .autotune import tune_batch_member
+        tune_batch_member(sim, lane_columns=self.lane_columns)
 
     def _set_options(self, sim):
         if getattr(sim.model, 'polarization', None) not in ('ez', 'hz'):
@@ -151,15 +160,22 @@ class Batch2D(_Batch):
 
 
 class Batch3D(_Batch):
-    """Simulation3Ds stepped together; planes_per_tile (0: each instance's own) sets the x planes of every tile."""
+    """Simulation3Ds stepped together; planes_per_tile (0: each instance's own) sets the x planes of every tile.
+    autotune=True chooses each not-yet-finalized member's planes per tile by its size class (autotune.py)."""
 
     three_d = True
     _desc_type = _lib.Desc3DC
     _create_name = 'yee3d_batch_create'
 
-    def __init__(self, simulations, graph_steps=0, planes_per_tile=0):
+    def __init__(self, simulations, graph_steps=0, planes_per_tile=0, autotune=False):
         self.planes_per_tile = int(planes_per_tile)
-        super().__init__(simulations, graph_steps)
+        if autotune and self.planes_per_tile:
+            raise ValueError('Batch3D: autotune chooses planes_per_tile; pass one or the other')
+        super().__init__(simulations, graph_steps, autotune)
+
+    def _tune(self, sim):
+        from .autotune import tune_batch_member
+        tune_batch_member(sim)
 
     def _set_options(self, sim):
         pass

@@ -10,7 +10,8 @@ device state fits the HBM budget, then stay resident until they finish. Work adv
     chunk while the current one runs on the device.
 
 A finished job is read out (its monitors' flux spectra), its buffers are freed and its latency (arrival to result) is
-recorded. The plain form, for comparison, runs the same jobs one at a time in arrival order.
+recorded. With autotune (the default) every admitted job's batch-member launch shape is the one tuned for its size
+class on this device (autotune.py; tune_classes() does the tuning ahead of a stream). The plain form, for comparison, runs the same jobs one at a time in arrival order.
 
 Failure handling. A job whose device state alone exceeds the HBM budget is refused at arrival (status 'rejected',
 with the reason in job.error), as the reference's memory budget refuses an over-sized model instead of spilling
@@ -99,10 +100,11 @@ class FDTDServer:
     """Continuous-batching server over one GPU. hbm_budget [B] bounds the resident device state; chunk_steps is the
     granularity at which jobs join and leave."""
 
-    def __init__(self, hbm_budget=200e9, chunk_steps=100, graph_steps=0):
+    def __init__(self, hbm_budget=200e9, chunk_steps=100, graph_steps=0, autotune=True):
         self.hbm_budget = float(hbm_budget)
         self.chunk_steps = int(chunk_steps)
         self.graph_steps = int(graph_steps)
+        self.autotune = bool(autotune)
         self._used = 0.0
         import torch
         self.torch = torch
@@ -131,6 +133,9 @@ class FDTDServer:
             if self._used + need > self.hbm_budget:
                 break
             pending.pop(0)
+            if self.autotune and job.sim.engine is None:   # batch-member launch shape of the job's size class
+                from .autotune import tune_batch_member
+                tune_batch_member(job.sim)
             job.sim.finalize()
             # source and phase tables for every step of the job now, so no table is replaced while a chunk runs
             job.sim.engine._ensure_capacity(job.steps)
@@ -140,6 +145,20 @@ class FDTDServer:
             job.status = 'running'
             self._used += need
             active.append(job)
+
+    def tune_classes(self, jobs):
+        """Tune the batch-member launch shape of every size class among `jobs` ahead of serving them (each job is
+        built once for it and dropped; a class already in the cache costs nothing). Returns the classes seen."""
+        from .autotune import class_key, tune_batch_member
+        seen = set()
+        for job in jobs:
+            sim = job.build()
+            key = class_key('batch', sim.model)
+            if key not in seen:
+                seen.add(key)
+                tune_batch_member(sim)
+            sim.close()
+        return sorted(seen)
 
     def _release(self, job, active, done):
         job.sim.close()

@@ -20,6 +20,19 @@ def gpu_available():
         return False
 
 
+@pytest.fixture(scope='session', autouse=True)
+def tune_cache(tmp_path_factory):
+    """Launch shapes tuned during the tests go to a cache file of the session, not the user's."""
+    path = str(tmp_path_factory.mktemp('tune') / 'autotune.json')
+    previous = os.environ.get('OPENSOL_AMD_TUNE_CACHE')
+    os.environ['OPENSOL_AMD_TUNE_CACHE'] = path
+    yield path
+    if previous is None:
+        os.environ.pop('OPENSOL_AMD_TUNE_CACHE', None)
+    else:
+        os.environ['OPENSOL_AMD_TUNE_CACHE'] = previous
+
+
 @pytest.fixture(scope='session')
 def gpu():
     """Skip-free guard: a gpu-marked test on a machine without a GPU fails loudly instead of passing silently."""

@@ -131,6 +131,49 @@ def test_batch3d_matches_solo_runs_bit_for_bit(gpu, planes):
         sim.close()
 
 
+def test_autotuned_batches_give_the_same_bits_and_fill_the_cache(gpu, tune_cache):
+    """Batch2D / Batch3D with autotune=True: every member's launch shape comes from its size class (tuned as a
+    one-member batch, then cached in the session's JSON file), and the fields and flux equal an untuned batch's bit
+    for bit; a second member of a tuned class takes the cached shape without timing."""
+    import json
+    from opensol_amd.autotune import tune_batch_member
+    from opensol_amd.batch import Batch2D, Batch3D
+    runs = []
+    for tuned in (True, False):
+        made = [_make(spec, k) for k, spec in enumerate(INSTANCES)]
+        batch = Batch2D([sim for sim, _ in made], autotune=tuned)
+        batch.run([spec[4] for spec in INSTANCES])
+        runs.append([_snapshot(sim, monitor) for sim, monitor in made])
+        if tuned:
+            assert all(sim._engine_options.get('rows_per_tile') for sim, _ in made)
+        batch.close()
+        for sim, _ in made:
+            sim.close()
+    for (fa, xa), (fb, xb) in zip(*runs):
+        for name in fa:
+            assert np.array_equal(fa[name], fb[name]), name
+        assert np.array_equal(xa, xb)
+    runs = []
+    for tuned in (True, False):
+        made = [_make_3d(spec, k) for k, spec in enumerate(INSTANCES_3D)]
+        batch = Batch3D([sim for sim, _ in made], autotune=tuned)
+        batch.run([spec[3] for spec in INSTANCES_3D])
+        runs.append([_snapshot_3d(sim, plane) for sim, plane in made])
+        batch.close()
+        for sim, _ in made:
+            sim.close()
+    for (fa, xa), (fb, xb) in zip(*runs):
+        for name in fa:
+            assert np.array_equal(fa[name], fb[name]), name
+        assert np.array_equal(xa, xb)
+    with open(tune_cache) as f:
+        entries = next(iter(json.load(f).values()))
+    assert any(key.startswith('batch:ez:') for key in entries) and any(key.startswith('batch:3d:') for key in entries)
+    again, _ = _make(INSTANCES[0], 0)
+    options, timings = tune_batch_member(again)
+    assert timings == {} and options == {'rows_per_tile': again._engine_options['rows_per_tile']}
+
+
 def test_tall_grid_single_and_batched_kernels_agree(gpu):
     """A tall grid (77 y tiles, 2 z tiles) stepped by the single-instance kernel and by the batched kernel gives the
     same bits."""
