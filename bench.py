@@ -269,12 +269,13 @@ def _cpu_model():
     return 'unknown CPU'
 
 
-def measured_traffic(workload):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE), or None."""
+def measured_traffic(workload, field='bytes_per_launch'):
+    """HBM bytes per launch (or per step's tile-kernel pair, field 'tiles_bytes_per_step') from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE), or None."""
     try:
         with open(TRAFFIC_FILE) as f:
             entry = json.load(f).get(workload)
-        return None if entry is None else float(entry['bytes_per_launch'])
+        return None if entry is None else float(entry[field])
     except (OSError, ValueError, KeyError):
         return None
 
@@ -327,7 +328,18 @@ def run_2d(args, torch):
                 'traffic': measured_traffic(workload), 'kernel': 'yee2d_step_kernel<EZ, bulk tiles>',
                 'bytes_per_cell_update': BYTES_PER_CELL_2D, 'step_achieved': step_achieved,
                 'step_frac': step_achieved / HBM_PEAK_GBS}
-    if kernels is not None:
+    if kernels is not None and 'tiles' in kernels:
+        # The bulk and edge tile kernels run side by side and together are the step (plus one clock kernel per
+        # graph): the measured unit is the pair, timed over the timed region as the step period (fork to fork).
+        # yee_step_timed's eager fork-to-join time, with a host synchronization every step, is reported beside it.
+        tiles_ms, tiles_cells = kernels['tiles']
+        roofline.update(achieved=step_achieved, frac=step_achieved / HBM_PEAK_GBS, launch_ms=per_step * 1e3,
+                        cells_per_launch=cells, algorithmic_bytes_per_launch=BYTES_PER_CELL_2D * cells,
+                        kernel='yee2d_step_kernel<EZ> bulk + edge tile sets, side by side (step period)',
+                        traffic=measured_traffic(workload, 'tiles_bytes_per_step'), eager_tiles_ms=tiles_ms)
+        print(f'# kernels: bulk + edge tiles side by side, {per_step * 1e6:.1f} us per step in the timed region '
+              f'({step_achieved:.1f} GB/s); eager fork-to-join {tiles_ms * 1e3:.1f} us', file=sys.stderr, flush=True)
+    elif kernels is not None:
         bulk_ms, bulk_cells = kernels['bulk']
         achieved = BYTES_PER_CELL_2D * bulk_cells / (bulk_ms * 1e-3) / 1e9
         roofline.update(achieved=achieved, frac=achieved / HBM_PEAK_GBS, launch_ms=bulk_ms, cells_per_launch=bulk_cells,

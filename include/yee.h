@@ -414,8 +414,9 @@ int32_t yee_step(yee_ctx* ctx, int64_t n_steps, void* stream);
 /*
  * Profiling form of yee_step: the same launches, one step at a time and never from a graph, with a HIP event pair
  * around every kernel. kernel_ms[c] receives the summed kernel time of class c over the n_steps steps and cells[c]
- * the cells one launch of that class updates (single 2D instance: class 0 = bulk tiles, 1 = edge tiles; 3D: class 0
- * = the fused step kernel, every cell, 1 = the plane-sample kernel). Synchronizes `stream`. No counterpart in the
+ * the cells one launch of that class updates (single 2D instance: class 0 = bulk tiles, 1 = edge tiles, or, when both
+ * sets exist and run side by side as yee_step launches them, class 0 = both, every cell, from the fork to the join,
+ * and class 1 empty; 3D: class 0 = the fused step kernel, every cell, 1 = the plane-sample kernel). Synchronizes `stream`. No counterpart in the
  * reference (its timing is wall clock around Simulation.run).
  */
 int32_t yee_step_timed(yee_ctx* ctx, int64_t n_steps, void* stream, double* kernel_ms, int64_t* cells);

@@ -1008,19 +1008,35 @@ int32_t yee_step_timed(yee_ctx* ctx, int64_t n_steps, void* stream, double* kern
     if (ctx->kind == kSingle3D) return step_timed_3d(ctx, n_steps, s, kernel_ms, cells);
     const StepKernel bulk = step_kernel(ctx->d2, false);
     const StepKernel edge = step_kernel(ctx->d2, true);
-    cells[0] = bulk_cells(ctx->d2, ctx->sets);
-    cells[1] = (int64_t)ctx->d2.nx * ctx->d2.ny - cells[0];
+    // tile sets launched side by side (as yee_step launches them): one class, every cell, from the fork to the join
+    const bool fork = concurrent_sets(ctx);
+    if (fork) {
+        int32_t status = ensure_side(ctx);
+        if (status != YEE_OK) return status;
+    }
+    cells[0] = fork ? (int64_t)ctx->d2.nx * ctx->d2.ny : bulk_cells(ctx->d2, ctx->sets);
+    cells[1] = fork ? 0 : (int64_t)ctx->d2.nx * ctx->d2.ny - cells[0];
     kernel_ms[0] = kernel_ms[1] = 0.0;
     hipEvent_t ev[3];
     for (auto& e : ev) YEE_HIP(hipEventCreate(&e));
     int32_t status = YEE_OK;
     for (int64_t k = 0; k < n_steps && status == YEE_OK; ++k) {
         (void)hipEventRecord(ev[0], s);
-        if (ctx->blocks > 0)
+        if (fork) {
+            (void)hipEventRecord(ctx->fork_ev, s);
+            (void)hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0);
+            hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, ctx->side, ctx->d2, ctx->sets, 0);
             hipLaunchKernelGGL(bulk, dim3(ctx->blocks), dim3(256), 0, s, ctx->d2, ctx->sets, 0);
-        (void)hipEventRecord(ev[1], s);
-        if (ctx->edge_blocks > 0)
-            hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, s, ctx->d2, ctx->sets, 0);
+            (void)hipEventRecord(ctx->join_ev, ctx->side);
+            (void)hipStreamWaitEvent(s, ctx->join_ev, 0);
+            (void)hipEventRecord(ev[1], s);
+        } else {
+            if (ctx->blocks > 0)
+                hipLaunchKernelGGL(bulk, dim3(ctx->blocks), dim3(256), 0, s, ctx->d2, ctx->sets, 0);
+            (void)hipEventRecord(ev[1], s);
+            if (ctx->edge_blocks > 0)
+                hipLaunchKernelGGL(edge, dim3(ctx->edge_blocks), dim3(256), 0, s, ctx->d2, ctx->sets, 0);
+        }
         (void)hipEventRecord(ev[2], s);
         if (has_ez_walls(ctx->d2))
             hipLaunchKernelGGL(yee2d_wall_kernel, dim3(wall_blocks(ctx->d2)), dim3(256), 0, s, ctx->d2, 0);
@@ -1040,7 +1056,7 @@ int32_t yee_step_timed(yee_ctx* ctx, int64_t n_steps, void* stream, double* kern
             break;
         }
         kernel_ms[0] += a;
-        kernel_ms[1] += b;
+        if (!fork) kernel_ms[1] += b;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     if (status == YEE_OK) YEE_HIP(hipStreamSynchronize(s));
