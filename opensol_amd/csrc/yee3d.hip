@@ -27,11 +27,12 @@
 //   plane flux         physics/em3d/monitors3d.py:90-101; physics/em/monitors.py:22-34, 92-96
 //   step order         physics/em3d/engine3d.py:497-554
 #include "yee_device.h"
+#include "yee3d_index.h"
 
 namespace yee3 {
 
-constexpr int LANES = 64;
-constexpr int OWNED = 62;          // z cells a wavefront tile owns (lanes 1..62)
+constexpr int LANES = yee3idx::kLanes;
+constexpr int OWNED = yee3idx::kOwned;   // z cells a wavefront tile owns (lanes 1..62)
 constexpr int LDS_IDS = 512;       // ids whose table is staged in LDS: (da, db, ca, cb) + (eps_r, sigma_e), 12 KiB
 constexpr float EPS0F = 8.8541878128e-12f;
 
@@ -97,16 +98,7 @@ __device__ __forceinline__ float from_next(float v) { return __shfl_down(v, 1, L
 __device__ __forceinline__ float from_prev(float v) { return __shfl_up(v, 1, LANES); }
 __device__ __forceinline__ int from_next_i(int v) { return __shfl_down(v, 1, LANES); }
 
-// Wrapped (periodic) or clamped index along an axis of n cells. A tile can span several periods of a short periodic
-// axis (64 lanes over nz = 5), so the wrap is a full modulo.
-__device__ __forceinline__ int fold(int v, int n, bool periodic)
-{
-    if (periodic) {
-        const int m = v % n;
-        return m < 0 ? m + n : m;
-    }
-    return v < 0 ? 0 : (v >= n ? n - 1 : v);
-}
+using yee3idx::fold;
 
 // psi recursion of one layer cell: b psi + g dF.
 __device__ __forceinline__ float psi_next(float b, float psi, float g, float difference) { return b * psi + g * difference; }
@@ -361,6 +353,7 @@ __device__ __forceinline__ void stf(Rsrc r, uint32_t lane_bytes, uint32_t row_by
 template <int J, bool LDS, bool POLES>
 __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, int tk)
 {
+    static_assert(J == yee3idx::kRows, "the row index math assumes kRows rows per lane");
     const int lane = threadIdx.x & (LANES - 1);
     const int nx = d.nx, ny = d.ny, nz = d.nz;
     const bool px = d.periodic[0] != 0, py = d.periodic[1] != 0, pz = d.periodic[2] != 0;
@@ -369,11 +362,11 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
     const int par = (int)(n & 1);
 
     // z of this lane: byte offsets within a row
-    const int kraw = tk * OWNED - 1 + lane;
+    const int kraw = yee3idx::lane_z(tk, lane);
     const int k = fold(kraw, nz, pz);
     const uint32_t vk4 = (uint32_t)k * 4u, vk2 = (uint32_t)k * 2u;
     const bool k_zero = !pz && kraw < 0;                      // lo neighbour of cell 0: H reads 0
-    const bool k_own = lane >= 1 && lane <= OWNED && kraw < nz;
+    const bool k_own = yee3idx::lane_owns(lane, kraw, nz);
     const yee3d_axis_row az = d.axis_rows[2][k];
     const float ihz = az.inv_h, iez = az.inv_e;
     const Row rz_h = h_row(az), rz_e = e_row(az);
@@ -385,7 +378,7 @@ __device__ void tile3d(const yee3d_desc& d, int64_t n, int i0, int i1, int tj, i
     bool own[J + 2];
 #pragma unroll
     for (int r = 0; r < J + 2; ++r) {
-        const int jraw = tj * J - 1 + r;
+        const int jraw = yee3idx::row_y(tj, r);
         jr[r] = fold(jraw, ny, py);
         roff[r] = (uint32_t)jr[r] * (uint32_t)nz * 4u;
         own[r] = r >= 1 && r <= J && jraw < ny;
@@ -537,9 +530,8 @@ __device__ __forceinline__ void step_block(const yee3d_desc& d, int64_t n, int t
     if (!__syncthreads_or(mine)) return;
     const bool lds = stage_tables(d);
     if (!mine) return;
-    const int tk = tile % tiles_k;
-    const int tj = (tile / tiles_k) % tiles_j;
-    const int ic = tile / (tiles_k * tiles_j);
+    const yee3idx::TileCoords c = yee3idx::tile_coords(tile, tiles_j, tiles_k);
+    const int tk = c.tk, tj = c.tj, ic = c.ic;
     const int i0 = ic * d.planes_per_tile;
     const int i1 = min(d.nx, i0 + d.planes_per_tile);
     if (lds)

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "yee.h"
+#include "yee3d_index.h"
 #include "yee_launch.h"
 
 template <int POL, int VEC, bool EDGE, bool POLES, bool LEFT>
@@ -60,8 +61,8 @@ constexpr int kEdgeSplit = 8;     // row chunks per edge tile (yee2d.hip EDGE_SP
 
 enum CtxKind { kSingle2D = 1, kBatch2D = 2, kSingle3D = 3, kBatch3D = 4 };
 
-constexpr int kRows3D = 4;         // y rows per 3D wavefront tile (yee3d.hip J)
-constexpr int kOwned3D = 62;       // z cells a 3D wavefront tile owns (yee3d.hip OWNED)
+constexpr int kRows3D = yee3idx::kRows;    // y rows per 3D wavefront tile (yee3d_index.h)
+constexpr int kOwned3D = yee3idx::kOwned;  // z cells a 3D wavefront tile owns (yee3d_index.h)
 constexpr int kAutoPlanes = 4;     // default x planes per 3D tile
 
 }  // namespace
