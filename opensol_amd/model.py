@@ -179,6 +179,9 @@ class _Built:
         self.three_d = self.kind == 'em_fdtd_3d'
         value = lambda v: evaluate(v, names)   # noqa: E731
         settings = physics.get('settings') or {}
+        track = settings.get('track_absorption', False)
+        if track if isinstance(track, bool) else value(track):
+            raise ValueError('absorption tracking (absorbed_power) is not part of the FDTD path')
 
         def setting(key, default):
             raw = settings.get(key, default)
@@ -443,6 +446,167 @@ class _Built:
         return out
 
 
+# ---- stored fields and snapshots (model/study.py:916-967, 2041-2130, 2623-2645; physics/em/renderer.py:12-312) -----
+_SNAPSHOT_PIXELS = 250_000        # window cells per stored field (model/study.py:44)
+_SNAPSHOT_BYTE_LIMIT = 1 << 30    # intermediate snapshots stop past this many bytes (:45)
+
+
+def _field_info(name, label, unit, cmap, signed=False, primary=False, parity='', staggered=''):
+    """A FieldInfo record (core/renderer.py) as results.npz stores it."""
+    return {'name': name, 'label': label, 'unit': unit, 'signed': signed, 'cmap': cmap, 'primary': primary,
+            'requires_study': '', 'parity': parity, 'staggered': staggered, 'plane_signs': ''}
+
+
+_MODULI = [_field_info('e_abs', '|E|', 'V/m', 'inferno'), _field_info('e_intensity', '|E|^2', 'V^2/m^2', 'inferno'),
+           _field_info('h_abs', '|H|', 'A/m', 'viridis')]
+_MEDIA = [_field_info('absorbed_power', 'Absorbed power density', 'W/m^3', 'magma'),
+          _field_info('carrier_density', 'Free-carrier density', '1/m^3', 'viridis'),
+          _field_info('population_inversion', 'Population inversion N2 - N1', '1/m^3', 'RdBu_r', signed=True,
+                      parity='even'),
+          _field_info('crystallinity', 'Crystallinity', '1', 'viridis')]
+_SIGMA = _field_info('sigma_e', 'Electric conductivity', 'S/m', 'cividis')
+# The fields a time-dependent study stores of a 2D FDTD physics (every output field but the frequency-domain ones),
+# primary first, in the renderer's order (physics/em/renderer.py:159-215).
+STORED_FIELDS = {
+    'ez': [_field_info('ez', 'E_z', 'V/m', 'RdBu_r', signed=True, primary=True, parity='polar_z'),
+           _field_info('hx', 'H_x (+y face)', 'A/m', 'RdBu_r', signed=True, parity='axial_x', staggered='y'),
+           _field_info('hy', 'H_y (+x face)', 'A/m', 'RdBu_r', signed=True, parity='axial_y', staggered='x'),
+           *_MODULI, _field_info('eps_r', 'Relative permittivity', '1', 'YlOrBr'), _SIGMA, *_MEDIA],
+    'hz': [_field_info('hz', 'H_z', 'A/m', 'RdBu_r', signed=True, primary=True, parity='axial_z'),
+           _field_info('ex', 'E_x', 'V/m', 'RdBu_r', signed=True, parity='polar_x'),
+           _field_info('ey', 'E_y', 'V/m', 'RdBu_r', signed=True, parity='polar_y'),
+           *_MODULI, _field_info('eps_xx', 'Relative permittivity xx', '1', 'YlOrBr'),
+           _field_info('eps_yy', 'Relative permittivity yy', '1', 'YlOrBr'), _SIGMA, *_MEDIA],
+}
+
+
+def _next_output_time(t_now, t_start, interval, dt):
+    """The first output time after t_now on the interval's grid, at least one tick ahead (model/study.py:2688-2691)."""
+    k = math.floor((t_now + 0.5 * dt - t_start) / interval) + 1
+    return max(t_start + k * interval, t_now + dt)
+
+
+def snapshot_schedule(dt, duration, output_interval=0.0, n_snapshots=20):
+    """[(tick, t)] of the snapshots a time-dependent study records, the last one the final state: the study's tick
+    loop (t += dt in float64, stopping at t >= t_end - dt/2) with a record whenever t >= next output - dt/2
+    (model/study.py:916-967). output_interval 0 spaces n_snapshots evenly over the duration."""
+    dt = float(dt)
+    half = 0.5 * dt
+    interval = float(output_interval)
+    if interval <= 0.0 and int(n_snapshots) > 1:
+        interval = float(duration) / (int(n_snapshots) - 1)
+    t, tick, t_final = 0.0, 0, float(duration)
+    next_output = 0.0 if interval > 0.0 else math.inf
+    out = []
+    while t < t_final - half:
+        if t >= next_output - half:
+            out.append((tick, t))
+            next_output = _next_output_time(t, 0.0, interval, dt)
+        stop = min(t_final, next_output) - half
+        while t < stop:
+            t += dt
+            tick += 1
+    out.append((tick, t))
+    return out
+
+
+def _to_centres(face, axis, periodic):
+    """Hi-face samples averaged with their lower neighbour onto the cell centres, in float32; the first cell pairs
+    with the last on a periodic axis, else with itself (physics/em/fields.py:53-66)."""
+    face = np.asarray(face, dtype=np.float32)
+    lower = np.roll(face, 1, axis=axis)
+    if not periodic:
+        first = [slice(None)] * face.ndim
+        first[axis] = 0
+        lower[tuple(first)] = face[tuple(first)]
+    return (face + lower) * np.float32(0.5)
+
+
+class _Recorder:
+    """Stored-field snapshots of a 2D run (model/study.py:2041-2130): each stored field over the native window cells,
+    float32, with the cell centres; intermediate snapshots stop once they hold more than _SNAPSHOT_BYTE_LIMIT bytes
+    (the final one is always taken). Windows above _SNAPSHOT_PIXELS cells are stored natively too, where the reference
+    resamples them to nearest cells. `requested`: the study's stored_fields keys, else every stored field."""
+
+    def __init__(self, built, requested=()):
+        sim = built.sim
+        self.sim = sim
+        self.physics = built.physics_name
+        self.polarization = sim.model.polarization
+        infos = {f'{self.physics}.{info["name"]}': info for info in STORED_FIELDS[self.polarization]}
+        for key in requested:
+            if key not in infos:
+                raise ValueError(f"stored field '{key}' is not an output field of this model; available: "
+                                 f"{sorted(infos)}")
+        self.keys = list(requested) or list(infos)
+        self.infos = {key: infos[key] for key in self.keys}
+        self.window = tuple(sim.domain.physical_window())
+        edges, centres = sim.domain.cell_edges(), sim.domain.cell_centers()
+        self.edges = [np.asarray(e[w.start:w.stop + 1], dtype=np.float64) for e, w in zip(edges, self.window)]
+        self.centres = [np.asarray(c[w], dtype=np.float64) for c, w in zip(centres, self.window)]
+        self.extent = [float(self.edges[0][0]), float(self.edges[0][-1]), float(self.edges[1][0]),
+                       float(self.edges[1][-1])]
+        self.snapshots = []
+        self.stored_bytes = 0
+        self.full = False
+
+    def _values(self):
+        """{component: float32 window array} of the current state (physics/em/renderer.py:159-240)."""
+        sim, w = self.sim, self.window
+        bc = sim.model.bc
+        px, py = bc.periodic('x'), bc.periodic('y')
+        materials = sim.model.materials
+        out = {}
+        if self.polarization == 'ez':
+            ez, hx, hy = (sim.engine.field(n) for n in ('ez', 'hx', 'hy'))
+            out['ez'], out['hx'], out['hy'] = ez[w], hx[w], hy[w]
+            out['e_abs'] = np.abs(ez[w])
+            out['e_intensity'] = ez[w] * ez[w]
+            hx_c, hy_c = _to_centres(hx, 1, py)[w], _to_centres(hy, 0, px)[w]
+            out['h_abs'] = np.sqrt(hx_c * hx_c + hy_c * hy_c)
+            out['eps_r'] = materials.property_array(0)[w]
+        else:
+            hz, ex, ey = (sim.engine.field(n) for n in ('hz', 'ex', 'ey'))
+            ex_c, ey_c = _to_centres(ex, 1, py)[w], _to_centres(ey, 0, px)[w]
+            out['hz'], out['ex'], out['ey'] = hz[w], ex_c, ey_c
+            intensity = ex_c * ex_c + ey_c * ey_c
+            out['e_abs'] = np.sqrt(intensity)
+            out['e_intensity'] = intensity
+            out['h_abs'] = np.abs(hz[w])
+            out['eps_xx'] = materials.property_array(0)[w]
+            out['eps_yy'] = materials.property_array(1)[w]
+        out['sigma_e'] = materials.property_array(3)[w]
+        shape = tuple(s.stop - s.start for s in w)
+        # no absorption tracking, carriers, gain or phase-change media on this path: these read zero, as the
+        # reference's do without them (physics/em/renderer.py:62-69, 122-150)
+        for name in ('absorbed_power', 'carrier_density', 'population_inversion', 'crystallinity'):
+            out[name] = np.zeros(shape, dtype=np.float32)
+        return {name: np.ascontiguousarray(values, dtype=np.float32) for name, values in out.items()}
+
+    def record(self, final=False):
+        """Store the current state (skipped once full unless final)."""
+        if self.full and not final:
+            return
+        values = self._values()
+        fields = {key: values[key.split('.', 1)[1]] for key in self.keys}
+        self.snapshots.append({'time': float(self.sim.time), 'label': None, 'time_label': self.time_label(),
+                               'fields': fields})
+        self.stored_bytes += sum(v.nbytes for v in fields.values())
+        if not (final or self.full) and self.stored_bytes > _SNAPSHOT_BYTE_LIMIT:
+            self.full = True
+
+    def time_label(self):
+        return f't = {float(self.sim.time) * 1e15:.1f} fs'
+
+    def layout(self):
+        """What every snapshot of this run shares: infos, extent, centres and edges."""
+        return {'infos': self.infos, 'extent': self.extent, 'centres': self.centres, 'edges': self.edges}
+
+
+def _snapshot_bytes(snapshots):
+    return sum(v.nbytes for snapshot in snapshots for v in snapshot['fields'].values())
+
+
 # ---- digests (io/results_io.py:156-225) ----------------------------------------------------------------------------
 def _finite(values):
     values = np.asarray(values, dtype=np.float64).ravel()
@@ -512,17 +676,37 @@ def _split(result, prefix, arrays):
     return entries
 
 
-class Results:
-    """Outcome of run(): status, the last run's results and fields, every swept run's results, summary() and
-    save(). result(physics, feature) / sweep_result(physics, feature) as the reference's Results."""
+def _snapshot_entry(snapshot, layout, prefix, arrays):
+    """Metadata entry {time, label, fields, cuts, segments} of one snapshot; its values and native cell centres go
+    into arrays under prefix + key (io/results_io.py:441-481)."""
+    fields = {}
+    for key, values in snapshot['fields'].items():
+        arrays[f'{prefix}{key}/values'] = values
+        arrays[f'{prefix}{key}/x_centers'] = layout['centres'][0]
+        arrays[f'{prefix}{key}/y_centers'] = layout['centres'][1]
+        fields[key] = {'extent': layout['extent'], 'info': layout['infos'][key], 'time_label': snapshot['time_label'],
+                       'native_shape': None, 'plane': None, 'axes': ['x', 'y'], 'unfolded': False, 'mesh': 'mesh',
+                       'triangulation': None}
+    return {'time': snapshot['time'], 'label': snapshot['label'], 'fields': fields, 'cuts': [], 'segments': {}}
 
-    def __init__(self, status, spec, results, sweep_results, sweep, fields, summary):
+
+class Results:
+    """Outcome of run(): status, the last run's results, stored fields and snapshots, every swept run's results (and,
+    with keep_fields, snapshots), summary() and save(). result(physics, feature) / sweep_result(physics, feature) as
+    the reference's Results. fields: {key: (values, x edges, y edges)} of the final state; snapshots: [{time, label,
+    time_label, fields {key: values}}]; sweep_snapshots: [(layout, snapshots)] per swept value."""
+
+    def __init__(self, status, spec, results, sweep_results, sweep, fields, summary, snapshots=(), layout=None,
+                 sweep_snapshots=()):
         self.status = status
         self.spec = spec
         self.results = results
         self.sweep_results = sweep_results
         self.sweep = sweep
         self.fields = fields
+        self.snapshots = list(snapshots)
+        self.layout = layout
+        self.sweep_snapshots = list(sweep_snapshots)
         self._summary = summary
 
     def summary(self):
@@ -536,23 +720,33 @@ class Results:
         return [run[key] for run in self.sweep_results]
 
     def save(self, directory):
-        """results.npz (format 'open_sol.results' version 1), model.json and summary.json; returns the files dict."""
+        """results.npz (format 'open_sol.results' version 1; io/results_io.py:398-438), model.json and summary.json;
+        returns the files dict."""
         os.makedirs(directory, exist_ok=True)
         arrays = {}
         metadata = {'format': RESULTS_FORMAT, 'version': RESULTS_FORMAT_VERSION, 'fields': {}, 'results': {},
                     'sweep_results': [], 'sector_results': [], 'sector_combined_results': {}, 'snapshots': [],
                     'sweep_snapshots': []}
+        t_end = self._summary['t_end']
         for key, (values, x_edges, y_edges) in self.fields.items():
             arrays[f'field/{key}/values'] = np.asarray(values)
             arrays[f'field/{key}/x_edges'] = np.asarray(x_edges, dtype=np.float64)
             arrays[f'field/{key}/y_edges'] = np.asarray(y_edges, dtype=np.float64)
             physics, component = key.split('.', 1)
-            metadata['fields'][key] = {'physics': physics, 'component': component, 'time': self._summary['t_end']}
+            metadata['fields'][key] = {'physics': physics, 'component': component,
+                                       'info': self.layout['infos'][key] if self.layout else None, 'time': t_end,
+                                       'time_label': f't = {t_end * 1e15:.1f} fs', 'plane': None, 'mesh': 'mesh',
+                                       'triangulation': None}
         for key, result in self.results.items():
             metadata['results'][key] = _split(result, f'result/{key}/', arrays)
         for index, run in enumerate(self.sweep_results if self.sweep else []):
             metadata['sweep_results'].append({key: _split(result, f'sweep/{index}/{key}/', arrays)
                                               for key, result in run.items()})
+        metadata['snapshots'] = [_snapshot_entry(snapshot, self.layout, f'snapshot/{index}/', arrays)
+                                 for index, snapshot in enumerate(self.snapshots)]
+        metadata['sweep_snapshots'] = [[_snapshot_entry(snapshot, layout, f'sweep_snapshot/{value}/{index}/', arrays)
+                                        for index, snapshot in enumerate(snapshots)]
+                                       for value, (layout, snapshots) in enumerate(self.sweep_snapshots)]
         files = {'summary': 'summary.json', 'arrays': 'results.npz', 'model': 'model.json'}
         with open(os.path.join(directory, 'results.npz'), 'wb') as handle:
             np.savez(handle, **arrays, metadata=np.array(json.dumps(_json_safe(metadata))))
@@ -565,9 +759,54 @@ class Results:
         return files
 
 
+def _advance(built, settings, names, requested):
+    """One time-dependent study of a build (model/study.py:916-967): the snapshot ticks of snapshot_schedule, with a
+    record at each and the final state last (2D); a 3D build runs the duration in one go and stores no snapshots.
+    Returns the recorder (None in 3D)."""
+    sim = built.sim
+    duration = evaluate(settings.get('duration', 200e-15), names)
+    if built.three_d:
+        sim.run(duration=duration, print_every=0)
+        return None
+    interval = evaluate(settings.get('output_interval', 0.0), names)
+    count = int(evaluate(settings.get('n_snapshots', 20), names))
+    if not float(interval) >= 0.0 or count < 1:
+        raise ValueError(f'output_interval must be >= 0 and n_snapshots >= 1, got {interval!r}, {count!r}')
+    sim.finalize()
+    recorder = _Recorder(built, requested)
+    schedule = snapshot_schedule(sim.model.dt, duration, interval, count)
+    for index, (tick, _) in enumerate(schedule):
+        if tick > sim.n_step:
+            sim.run(n_steps=tick - sim.n_step, print_every=0)
+        last = index == len(schedule) - 1
+        if last or not recorder.full:
+            recorder.record(final=last)
+    return recorder
+
+
+class _KeptSweepSnapshots:
+    """keep_fields: every swept value's snapshots while they total at most _SNAPSHOT_BYTE_LIMIT bytes, from the value
+    that passes it on only each value's final snapshot (model/study.py:674-693)."""
+
+    def __init__(self):
+        self.stored_bytes = 0
+        self.finals_only = False
+        self.kept = []
+
+    def take(self, recorder):
+        snapshots = recorder.snapshots
+        if not self.finals_only and self.stored_bytes + _snapshot_bytes(snapshots) > _SNAPSHOT_BYTE_LIMIT:
+            self.finals_only = True
+        if self.finals_only:
+            snapshots = snapshots[-1:]
+        self.stored_bytes += _snapshot_bytes(snapshots)
+        self.kept.append((recorder.layout(), snapshots))
+
+
 def run(spec, out_dir=None, parameters=None, progress=None):
     """Build and run a model (a dict or a path to its JSON) on the HIP path; returns Results and, with out_dir, saves
-    them there. parameters {name: expression} override the model's. A study sweep runs one build per value."""
+    them there. parameters {name: expression} override the model's. A study sweep runs one build per value; 2D runs
+    store the stored-field snapshots of the study's output schedule (model/study.py:916-967, 2041-2130)."""
     if isinstance(spec, (str, os.PathLike)):
         spec = load(spec)
     study = spec.get('study') or {'type': 'time_dependent', 'settings': {'duration': 200e-15}}
@@ -575,41 +814,62 @@ def run(spec, out_dir=None, parameters=None, progress=None):
         raise ValueError(f"study type {study.get('type')!r}: only 'time_dependent' is part of the FDTD path")
     if spec.get('derived_values'):
         raise ValueError('derived values are not part of the FDTD path')
+    if study.get('stored_cuts'):
+        raise ValueError('stored cuts (3D planes) are not part of the FDTD path')
+    settings = study.get('settings') or {}
+    requested = list(study.get('stored_fields') or [])
     overrides = dict(parameters or {})
     base = evaluate_parameters(spec.get('parameters') or [], overrides)
     sweep = study.get('sweep')
     runs = [overrides]
+    kept = None
     if sweep:
         if sweep.get('parameter') not in base:
             raise ValueError(f"sweep parameter {sweep.get('parameter')!r} is not a model parameter")
         values = [evaluate(v, base) for v in sweep.get('values') or []]
         runs = [dict(overrides, **{sweep['parameter']: v}) for v in values]
+        kept = _KeptSweepSnapshots() if sweep.get('keep_fields', True) else None
         sweep = {'parameter': sweep['parameter'], 'values': values}
     wall = time.perf_counter()
-    sweep_results, built, names = [], None, base
+    sweep_results, built, names, recorder = [], None, base, None
     for index, run_overrides in enumerate(runs):
         names = evaluate_parameters(spec.get('parameters') or [], run_overrides)
         if built is not None:
             built.sim.close()
         built = _Built(spec, names)
-        duration = evaluate((study.get('settings') or {}).get('duration', 200e-15), names)
-        built.sim.run(duration=duration)
+        recorder = _advance(built, settings, names, requested)
         sweep_results.append(built.results())
+        if kept is not None and recorder is not None:
+            kept.take(recorder)
         if progress is not None:
             progress({'run': index, 'of': len(runs), 'steps': built.sim.engine.n_step})
     results = sweep_results[-1]
+    snapshots = recorder.snapshots if recorder is not None else []
     summary = {'status': 'completed', 'message': '', 'parameters': names,
                'results': {key: result_digest(result) for key, result in results.items()},
-               't_end': float(built.sim.time), 'ticks': int(built.sim.engine.n_step),
+               't_end': float(built.sim.time), 'ticks': int(built.sim.engine.n_step), 'tick_unit': 'step',
+               'snapshot_times': [s['time'] for s in snapshots], 'snapshot_labels': [s['label'] for s in snapshots],
+               'stored_fields': list(recorder.keys) if recorder is not None else [],
                'wall_time': time.perf_counter() - wall}
     if sweep:
-        summary['sweep'] = dict(sweep, results=[{key: result_digest(r) for key, r in run.items()}
-                                                for run in sweep_results])
-    out = Results('completed', spec, results, sweep_results, sweep, built.fields(), summary)
+        summary['sweep'] = dict(sweep, derived={}, reference={}, error={}, curve_error={},
+                                results=[{key: result_digest(r) for key, r in run.items()} for run in sweep_results],
+                                failed=[], converged=[None] * len(runs), derived_series=[None] * len(runs))
+        if kept is not None and kept.kept:
+            summary['sweep']['snapshot_times'] = [[s['time'] for s in snaps] for _, snaps in kept.kept]
+            summary['sweep']['snapshot_labels'] = [[s['label'] for s in snaps] for _, snaps in kept.kept]
+    if recorder is not None:
+        final = recorder.snapshots[-1]['fields']
+        fields = {key: (values, recorder.edges[0], recorder.edges[1]) for key, values in final.items()}
+        out = Results('completed', spec, results, sweep_results, sweep, fields, summary, snapshots, recorder.layout(),
+                      kept.kept if kept is not None else ())
+    else:
+        out = Results('completed', spec, results, sweep_results, sweep, built.fields(), summary)
     built.sim.close()
     if out_dir is not None:
         out.save(out_dir)
     return out
 
 
-__all__ = ['CONSTANTS', 'Results', 'evaluate', 'evaluate_parameters', 'load', 'result_digest', 'run']
+__all__ = ['CONSTANTS', 'Results', 'STORED_FIELDS', 'evaluate', 'evaluate_parameters', 'load', 'result_digest', 'run',
+           'snapshot_schedule']

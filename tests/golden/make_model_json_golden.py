@@ -5,11 +5,12 @@ type switched to em_fdtd_hz and once with the film a Drude medium (E_z), and for
 selection, feature placement, the study duration and sweep, and the results.npz layout.
 
 Saved per case: the model JSON text (input), each swept run's flux wavelengths, power and power_per_cw per monitor
-feature, the results.npz array names and its metadata JSON text (the file layout), and the summary's sweep record,
-tick count and end time.
+feature, the results.npz array names and its metadata JSON text (the file layout), the summary's sweep record, tick
+count and end time, and (E_z, H_z) a sample of the results.npz values: the stored fields of one snapshot and of the
+final field group, their cell centres and edges, and the empty run's primary field at that snapshot.
 
 Run in this container only (the reference is not present on the GPU box):
-    python tests/golden/make_model_json_golden.py
+    python tests/golden/make_model_json_golden.py [3d] [ez] [hz] [drude]   (no argument: every case)
 The fixture holds inputs and outputs only, no source text of the reference.
 """
 import json
@@ -24,6 +25,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURES = os.path.join(os.path.dirname(HERE), 'fixtures')
 EXAMPLE = os.path.join(REFERENCE, 'examples', 'em', 'thin_film.json')
 MONITORS = ('reflection_side_flux', 'transmitted_flux')
+# results.npz values kept (besides the names and metadata): the stored fields of snapshot KEPT_SNAPSHOT and of the
+# final field group, with their centres and edges, and the first swept run's primary field at that snapshot
+KEPT_SNAPSHOT = 20
+KEPT_FIELDS = {'ez': ('ez', 'hx', 'hy', 'e_abs', 'e_intensity', 'h_abs', 'eps_r', 'sigma_e'),
+               'hz': ('hz', 'ex', 'ey', 'e_abs', 'e_intensity', 'h_abs', 'eps_xx', 'eps_yy', 'sigma_e')}
 
 
 # A small 3D model of the em_fdtd_3d vocabulary (no example of the reference uses only features of this path): a
@@ -147,6 +153,18 @@ def run_case(polarization):
         with np.load(os.path.join(directory, 'out', 'results.npz'), allow_pickle=False) as npz:
             out['npz_names'] = np.array(sorted(npz.files))
             out['npz_metadata'] = np.array(str(npz['metadata']))
+            if polarization in KEPT_FIELDS:   # a sample of the field and snapshot values
+                primary = KEPT_FIELDS[polarization][0]
+                for name in KEPT_FIELDS[polarization]:
+                    out[f'npz/snapshot/{KEPT_SNAPSHOT}/em.{name}/values'] = npz[
+                        f'snapshot/{KEPT_SNAPSHOT}/em.{name}/values']
+                    out[f'npz/field/em.{name}/values'] = npz[f'field/em.{name}/values']
+                for axis in 'xy':
+                    out[f'npz/snapshot/{KEPT_SNAPSHOT}/{axis}_centers'] = npz[
+                        f'snapshot/{KEPT_SNAPSHOT}/em.{primary}/{axis}_centers']
+                    out[f'npz/field/{axis}_edges'] = npz[f'field/em.{primary}/{axis}_edges']
+                out[f'npz/sweep_snapshot/0/{KEPT_SNAPSHOT}/em.{primary}/values'] = npz[
+                    f'sweep_snapshot/0/{KEPT_SNAPSHOT}/em.{primary}/values']
         with open(os.path.join(directory, 'out', 'summary.json'), encoding='utf-8') as handle:
             summary = json.load(handle)
         out['summary_sweep'] = np.array(json.dumps(summary.get('sweep')))
@@ -157,12 +175,13 @@ def run_case(polarization):
     return out
 
 
-def main():
-    out = run_3d()
-    path = os.path.join(FIXTURES, 'model_dipole_3d.npz')
-    np.savez_compressed(path, **out)
-    print(path, 'ticks', int(out['summary_ticks']), 'flux_x', out['flux_x/power'])
-    for polarization in ('ez', 'hz', 'drude'):
+def main(cases=('3d', 'ez', 'hz', 'drude')):
+    if '3d' in cases:
+        out = run_3d()
+        path = os.path.join(FIXTURES, 'model_dipole_3d.npz')
+        np.savez_compressed(path, **out)
+        print(path, 'ticks', int(out['summary_ticks']), 'flux_x', out['flux_x/power'])
+    for polarization in [case for case in ('ez', 'hz', 'drude') if case in cases]:
         out = run_case(polarization)
         path = os.path.join(FIXTURES, f'model_thin_film_{polarization}.npz')
         np.savez_compressed(path, **out)
@@ -170,4 +189,4 @@ def main():
 
 
 if __name__ == '__main__':
-    main()
+    main(tuple(sys.argv[1:]) or ('3d', 'ez', 'hz', 'drude'))

@@ -1,7 +1,8 @@
 """The reference's thin-film example model (examples/em/thin_film.json, E_z and H_z) run through the model-JSON front
 end on the HIP path, against what the reference's own headless API returned for it (tests/golden/
-make_model_json_golden.py, its float32 CPU path; fixtures in tests/fixtures): every swept run's flux spectra and per-cw powers, the transmission,
-and the results.npz array layout."""
+make_model_json_golden.py, its float32 CPU path; fixtures in tests/fixtures): every swept run's flux spectra and per-cw
+powers, the transmission, and results.npz: every array name, the stored-field and snapshot metadata, the snapshot times
+of both swept runs, and the stored fields of one snapshot and of the final state."""
 import json
 import os
 
@@ -12,6 +13,9 @@ pytestmark = pytest.mark.gpu
 
 FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'fixtures')
 MONITORS = ('reflection_side_flux', 'transmitted_flux')
+FIELDS = {'ez': ['ez', 'hx', 'hy', 'e_abs', 'e_intensity', 'h_abs'],
+          'hz': ['hz', 'ex', 'ey', 'e_abs', 'e_intensity', 'h_abs']}
+MATERIALS = {'ez': ['eps_r', 'sigma_e'], 'hz': ['eps_xx', 'eps_yy', 'sigma_e']}
 
 
 @pytest.mark.parametrize('polarization', ['ez', 'hz', 'drude'])
@@ -40,9 +44,51 @@ def test_thin_film_model_matches_the_reference_run(gpu, polarization, tmp_path):
     t_ref = golden['transmitted_flux/1/power'] / golden['transmitted_flux/0/power']
     print(f'{polarization} transmission max |diff| {np.max(np.abs(t_gpu - t_ref)):.2e}')
     assert np.max(np.abs(t_gpu - t_ref)) <= 1e-5
+    # results.npz: every array name, the field / snapshot / sweep_snapshot metadata entries, the snapshot times
+    want_meta = json.loads(str(golden['npz_metadata']))
     with np.load(tmp_path / 'results.npz', allow_pickle=False) as npz:
-        ours = {n for n in npz.files if n.split('/')[0] in ('result', 'sweep')}
-    assert ours == {str(n) for n in golden['npz_names'] if str(n).split('/')[0] in ('result', 'sweep')}
+        assert set(npz.files) == {str(n) for n in golden['npz_names']}
+        meta = json.loads(str(npz['metadata']))
+        for group in ('fields', 'snapshots', 'sweep_snapshots'):
+            assert meta[group] == want_meta[group], group
+        assert sorted(meta['results']) == sorted(want_meta['results'])
+        summary = results.summary()
+        sweep = json.loads(str(golden['summary_sweep']))
+        assert summary['sweep']['snapshot_times'] == sweep['snapshot_times']
+        assert summary['snapshot_times'] == sweep['snapshot_times'][-1]
+        if polarization == 'drude':
+            return
+        # stored-field values of snapshot 20 and of the final field group, and the empty run's primary field at
+        # snapshot 20: materials exact; fields within 1e-5 of the field's running peak (its largest magnitude in this
+        # run up to that snapshot). The pulse leaves through the CPML and the fields decay by six orders between 8 and
+        # 80 fs, while float32 rounding differences stay at the size the peak left them (about 4e-7 of it), so they
+        # are measured against the running peak, not against each snapshot's own.
+        snap = 20   # the snapshot whose values the fixture keeps
+        primary = 'ez' if polarization == 'ez' else 'hz'
+        last = len(results.snapshots) - 1
+
+        def running_peak(snapshots, key, upto):
+            return max(float(np.max(np.abs(s['fields'][key]))) for s in snapshots[:upto + 1])
+
+        checks = []
+        for name in MATERIALS[polarization] + FIELDS[polarization]:
+            checks.append((f'snapshot/{snap}/em.{name}/values', results.snapshots, f'em.{name}', snap))
+            checks.append((f'field/em.{name}/values', results.snapshots, f'em.{name}', last))
+        checks.append((f'sweep_snapshot/0/{snap}/em.{primary}/values', results.sweep_snapshots[0][1],
+                       f'em.{primary}', snap))
+        for key, snapshots, field, upto in checks:
+            ours, want = npz[key], golden[f'npz/{key}']
+            assert ours.dtype == np.float32 and ours.shape == want.shape, key
+            if field.split('.')[1] in MATERIALS[polarization]:
+                np.testing.assert_array_equal(ours, want, err_msg=key)
+                continue
+            peak = running_peak(snapshots, field, upto)
+            if peak == 0.0:   # H_x (E_z) / E_x (H_z) of a wave along x stay zero: the reference's are zero too
+                np.testing.assert_array_equal(want, 0.0, err_msg=key)
+                continue
+            error = float(np.max(np.abs(ours.astype(np.float64) - want))) / peak
+            print(f'{polarization} {key}: max err / running peak {error:.2e}')
+            assert error <= 1e-5, (key, error)
 
 
 def test_3d_model_matches_the_reference_run(gpu):

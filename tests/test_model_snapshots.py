@@ -1,0 +1,181 @@
+"""Snapshot side of the model-JSON front end, on the CPU: the output schedule, the stored-field table, the face-to-centre
+averaging and the results.npz snapshot / field layout, against what the reference's own run of its thin-film example
+saved (tests/fixtures/model_thin_film_{ez,hz}.npz, written by tests/golden/make_model_json_golden.py).
+
+  * snapshot_schedule() reproduces the reference run's 42 snapshot times bit for bit (output_interval 2 fs: ticks of
+    58.4 as, so the times are 0, 1.985, 4.028, ... 79.976 fs and the final 80.034 fs) and its tick count, and with
+    output_interval 0 spaces n_snapshots over the duration (first at 0, last the final state, one per grid point);
+  * STORED_FIELDS equals the FieldInfo records, in order, that the reference stored for E_z and H_z;
+  * _to_centres() is checked against a per-cell loop (periodic and not), so a swapped neighbour or a dropped wrap
+    fails;
+  * Results.save() of hand-made snapshots writes the reference's array names and metadata entries for a 2D run."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+FIXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'fixtures')
+
+
+def _golden(polarization):
+    return np.load(os.path.join(FIXTURES, f'model_thin_film_{polarization}.npz'))
+
+
+def _dt(spec):
+    from opensol_amd import model as M
+    names = M.evaluate_parameters(spec['parameters'])
+    built = M._Built(spec, names)
+    return built, names
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_schedule_matches_the_reference_snapshot_times(polarization):
+    from opensol_amd import model as M
+    from opensol_amd.fdtd2d import steps_for_duration
+    golden = _golden(polarization)
+    spec = json.loads(str(golden['model_json']))
+    built, names = _dt(spec)
+    settings = spec['study']['settings']
+    duration = M.evaluate(settings['duration'], names)
+    schedule = M.snapshot_schedule(built.sim.model.dt, duration, M.evaluate(settings['output_interval'], names))
+    want = json.loads(str(golden['summary_sweep']))['snapshot_times']
+    assert [t for _, t in schedule] == want[0] == want[1]
+    assert schedule[-1][0] == steps_for_duration(0.0, built.sim.model.dt, duration)
+    assert 2 * schedule[-1][0] == int(golden['summary_ticks'])   # the summary counts both swept runs' ticks
+    metadata = json.loads(str(golden['npz_metadata']))
+    assert [s['time'] for s in metadata['snapshots']] == want[1]
+
+
+def test_even_schedule_closed_form():
+    """output_interval 0: n_snapshots grid points k * duration / (n - 1), each taken at the first tick within half a
+    tick of it, the last one the final state."""
+    from opensol_amd import model as M
+    dt, duration, n = 1.0e-3, 1.0, 11
+    schedule = M.snapshot_schedule(dt, duration, 0.0, n)
+    assert len(schedule) == n
+    ticks = [tick for tick, _ in schedule]
+    assert ticks == [round(k * 100) for k in range(n)]
+    assert schedule[-1][0] == 1000
+    # one snapshot: only the final state
+    assert M.snapshot_schedule(dt, duration, 0.0, 1) == [(1000, schedule[-1][1])]
+    # an interval longer than the run: the start and the final state
+    assert [tick for tick, _ in M.snapshot_schedule(dt, duration, 5.0)] == [0, 1000]
+    # intervals shorter than a tick record every tick
+    assert [tick for tick, _ in M.snapshot_schedule(dt, 10 * dt, dt / 4)] == list(range(11))
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_stored_field_table_matches_the_reference(polarization):
+    from opensol_amd import model as M
+    metadata = json.loads(str(_golden(polarization)['npz_metadata']))
+    stored = metadata['snapshots'][0]['fields']
+    assert [key.split('.', 1)[1] for key in stored] == [info['name'] for info in M.STORED_FIELDS[polarization]]
+    for info in M.STORED_FIELDS[polarization]:
+        assert stored[f'em.{info["name"]}']['info'] == info
+        assert metadata['fields'][f'em.{info["name"]}']['info'] == info
+
+
+@pytest.mark.parametrize('periodic', [False, True])
+@pytest.mark.parametrize('axis', [0, 1])
+def test_face_to_centre_average_against_a_loop(axis, periodic):
+    from opensol_amd import model as M
+    rng = np.random.default_rng(7 + axis + 2 * periodic)
+    face = rng.standard_normal((7, 5)).astype(np.float32)
+    got = M._to_centres(face, axis, periodic)
+    want = np.empty_like(face)
+    for i in range(face.shape[0]):
+        for j in range(face.shape[1]):
+            index = (i, j)[axis]
+            lower = list((i, j))
+            lower[axis] = index - 1 if index > 0 else (face.shape[axis] - 1 if periodic else 0)
+            want[i, j] = np.float32(np.float32(face[i, j] + face[tuple(lower)]) * np.float32(0.5))
+    assert got.dtype == np.float32
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize('polarization', ['ez', 'hz'])
+def test_saved_layout_matches_the_reference_names_and_entries(polarization, tmp_path):
+    """The recorder's window layout (built on the CPU, no engine) and Results.save() of snapshots at the reference
+    run's times: the window edges and centres, the field / snapshot / sweep_snapshot array names and those groups'
+    metadata entries equal the reference's (the values are hand-made here; the GPU test compares a real run's)."""
+    from opensol_amd import model as M
+    golden = _golden(polarization)
+    metadata = json.loads(str(golden['npz_metadata']))
+    spec = json.loads(str(golden['model_json']))
+    built, _ = _dt(spec)
+    recorder = M._Recorder(built)
+    np.testing.assert_array_equal(recorder.edges[0], golden['npz/field/x_edges'])
+    np.testing.assert_array_equal(recorder.edges[1], golden['npz/field/y_edges'])
+    np.testing.assert_array_equal(recorder.centres[0], golden['npz/snapshot/20/x_centers'])
+    np.testing.assert_array_equal(recorder.centres[1], golden['npz/snapshot/20/y_centers'])
+    shape = (recorder.edges[0].size - 1, recorder.edges[1].size - 1)
+    rng = np.random.default_rng(3)
+    for snapshot in metadata['snapshots']:
+        t = snapshot['time']
+        recorder.snapshots.append({'time': t, 'label': None, 'time_label': f't = {t * 1e15:.1f} fs',
+                                   'fields': {k: rng.standard_normal(shape).astype(np.float32)
+                                              for k in recorder.keys}})
+    final = recorder.snapshots[-1]['fields']
+    fields = {key: (values, recorder.edges[0], recorder.edges[1]) for key, values in final.items()}
+    kept = [(recorder.layout(), recorder.snapshots)] * 2
+    out = M.Results('completed', {}, {}, [{}, {}], {'parameter': 'n_film', 'values': [1.0, 1.5]}, fields,
+                    {'t_end': recorder.snapshots[-1]['time']}, recorder.snapshots, recorder.layout(), kept)
+    out.save(tmp_path)
+    groups = ('field', 'snapshot', 'sweep_snapshot')
+    with np.load(tmp_path / 'results.npz', allow_pickle=False) as npz:
+        ours = {n for n in npz.files if n.split('/')[0] in groups}
+        saved = json.loads(str(npz['metadata']))
+    assert ours == {str(n) for n in golden['npz_names'] if str(n).split('/')[0] in groups}
+    assert saved['fields'] == metadata['fields']
+    assert saved['snapshots'] == metadata['snapshots']
+    assert saved['sweep_snapshots'] == metadata['sweep_snapshots']
+
+
+def test_stored_fields_request_is_checked():
+    from opensol_amd import model as M
+    golden = _golden('ez')
+    spec = json.loads(str(golden['model_json']))
+    built, _ = _dt(spec)
+    recorder = M._Recorder(built, ['em.h_abs', 'em.ez'])
+    assert recorder.keys == ['em.h_abs', 'em.ez']
+    with pytest.raises(ValueError, match="stored field 'em.hz' is not an output field"):
+        M._Recorder(built, ['em.hz'])
+
+
+def test_absorption_tracking_is_refused():
+    from opensol_amd import model as M
+    spec = json.loads(str(_golden('ez')['model_json']))
+    spec['physics'][0].setdefault('settings', {})['track_absorption'] = True
+    with pytest.raises(ValueError, match='absorption tracking'):
+        M._Built(spec, M.evaluate_parameters(spec['parameters']))
+
+
+def test_snapshot_byte_limit_keeps_the_final_state(monkeypatch):
+    """Past the byte limit intermediate records stop and the final one is still taken (model/study.py:2121-2129);
+    a keep_fields sweep past it keeps only each later value's final snapshot (:683-693)."""
+    from opensol_amd import model as M
+
+    class Fake:
+        def __init__(self):
+            self.keys = ['em.ez']
+            self.snapshots = []
+            self.full = False
+            self.stored_bytes = 0
+
+        def layout(self):
+            return {}
+
+    monkeypatch.setattr(M, '_SNAPSHOT_BYTE_LIMIT', 100)
+    recorder = Fake()
+    recorder._values = lambda: {'ez': np.zeros(10, np.float32)}   # 40 bytes a record
+    recorder.sim = type('S', (), {'time': 0.0})()
+    recorder.time_label = lambda: 't = 0.0 fs'
+    for index in range(6):
+        M._Recorder.record(recorder, final=index == 5)
+    assert len(recorder.snapshots) == 4 and recorder.full   # 3 x 40 passes 100 after the third; final appended
+    kept = M._KeptSweepSnapshots()
+    kept.take(recorder)
+    kept.take(recorder)
+    assert [len(s) for _, s in kept.kept] == [1, 1] and math.isclose(kept.stored_bytes, 80)
