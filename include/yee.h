@@ -113,8 +113,8 @@ typedef struct yee_cpml_axis {
 /*
  * Field probes of an instance: after step n (on the device clock), samples[n * n_probes + k] = the field component
  * component[k] at the flat element index cell[k] of the new state set, for n < capacity (NaN for a record outside
- * the grid or the components). Batched instances take no probes. One small launch per step
- * (also inside the step graphs). Replaces the FieldProbe rows of the reference (core/monitors.py:42-130: one
+ * the grid or the components). One small launch per step (also inside the step graphs; a batch samples all its
+ * members' probes in one launch). Replaces the FieldProbe rows of the reference (core/monitors.py:42-130: one
  * probe_rows launch per step; physics/em/__init__.py:493-498, physics/em3d/physics3d.py:425-433).
  */
 typedef struct yee_probes {
@@ -451,8 +451,10 @@ int32_t yee_destroy(yee_ctx* ctx);
  * Batched 2D instances (mixed E_z / H_z, uneven sizes) stepped by one launch per step over their tiles.
  * descs_dev: caller device buffer of yee2d_batch_bytes(n) bytes that receives the descriptors, each member's tile sets
  * and the bulk and edge tile prefixes (copied on `stream`, which is synchronized once before the call returns). A step
- * is two launches: the members' bulk tiles in the lean loops and their edge tiles (CPML, source and monitor lines,
- * E_z and H_z pole rows) in the general loops. Each instance keeps its own clock and
+ * is two launches: the members' bulk tiles in the lean loops and their edge tiles (CPML, source, monitor and TFSF
+ * lines, E_z and H_z pole rows, H_z wall rows) in the general loops; then, when any member has them, one launch each
+ * over all members for E_z wall rows, DFT regions and field probes, in the single-instance order. Members share
+ * lane_columns; every other feature of a single instance is allowed. Each instance keeps its own clock and
  * step_limit, so instances of different lengths share launches and finished ones skip their tiles. No counterpart in
  * the reference, which steps one model at a time (model/study.py:437-447 builds one model per sweep value).
  */
@@ -460,11 +462,13 @@ size_t yee2d_batch_bytes(int32_t n);
 int32_t yee2d_batch_create(const yee2d_desc* descs, int32_t n, void* descs_dev, void* stream, yee_ctx** out);
 
 /*
- * Batched 3D instances of uneven sizes stepped by one launch per step over their tiles (plus one sample launch over
- * all their flux-plane cells). descs_dev: caller device buffer of yee3d_batch_bytes(n) bytes that receives the
- * descriptors and the block and plane-cell prefixes (copied on `stream`, which is synchronized once before the call
- * returns). Each instance keeps its own clock, step_limit and planes_per_tile. No counterpart in the reference, which
- * steps one model at a time (model/study.py:437-447).
+ * Batched 3D instances of uneven sizes stepped by one launch per step over their tiles (members with dispersive poles
+ * take the E update that carries them), then one launch per mirror kind and face present in any member (symmetric
+ * faces first, as a single instance orders them), one sample launch over all their flux-plane cells, and one launch
+ * each over all members for DFT planes and field probes. descs_dev: caller device buffer of yee3d_batch_bytes(n) bytes
+ * that receives the descriptors and the block and plane-cell prefixes (copied on `stream`, which is synchronized once
+ * before the call returns). Each instance keeps its own clock, step_limit and planes_per_tile (4 rows per tile). No
+ * counterpart in the reference, which steps one model at a time (model/study.py:437-447).
  */
 size_t yee3d_batch_bytes(int32_t n);
 int32_t yee3d_batch_create(const yee3d_desc* descs, int32_t n, void* descs_dev, void* stream, yee_ctx** out);

@@ -1521,11 +1521,12 @@ YEE_STEP_KERNEL(YEE_EZ, 2, false, true, false)
 YEE_STEP_KERNEL(YEE_EZ, 4, false, true, false)
 
 // Many instances (E_z and H_z mixed) in one launch, one kernel per tile class: with EDGE = false every member's bulk
-// tiles run the lean loops, with EDGE = true its edge tiles run the general loops, one wavefront per tile. Tile t of
+// tiles run the lean loops, with EDGE = true its edge tiles run the general loops (sources, monitors, CPML, poles,
+// TFSF and H_z walls), one wavefront per tile. Tile t of
 // the class belongs to the member whose range [prefix[k], prefix[k+1]) holds it; the member's tile sets (the host's
 // lists for that member, E_z pole rows among the edge rows) give the tile. Tables are read from global memory (they
 // stay in L1/L2).
-template <int VEC, bool EDGE>
+template <int VEC, bool EDGE, bool LEFT>
 __global__ void __launch_bounds__(256) yee2d_batch_kernel(const yee2d_desc* __restrict__ descs,
                                                          const yee2d_tile_sets* __restrict__ sets,
                                                          const int32_t* __restrict__ prefix, int32_t n_inst,
@@ -1550,26 +1551,36 @@ __global__ void __launch_bounds__(256) yee2d_batch_kernel(const yee2d_desc* __re
     const int item = tile - prefix[lo];
     if (!(EDGE ? yee::edge_tile(s, item, &ti, &tj) : yee::bulk_tile(s, item, &ti, &tj))) return;
     const yee::TileBox t = yee::tile_box<VEC>(d, ti * s.tiles_j + tj);
+    if constexpr (EDGE && LEFT) {
+        // an H_z y_hi copy wall on a tile edge: the general loop with the lane-0 recompute of column ny - 2
+        const int32_t w = d.wall_sign[3];
+        if (d.polarization == YEE_HZ && (w == 1 || w == -1) && (d.ny - 1) % (64 * VEC) == 0) {
+            yee::tile_2d<yee::POL_ANY, VEC, true, false, false, true>(d, t, n);
+            return;
+        }
+    }
     yee::tile_2d<yee::POL_ANY, VEC, EDGE, false>(d, t, n);
 }
 
-#define YEE_BATCH_KERNEL(VEC, EDGE)                                                                              \
-    template __global__ void yee2d_batch_kernel<VEC, EDGE>(const yee2d_desc* __restrict__,                      \
-                                                           const yee2d_tile_sets* __restrict__,                 \
-                                                           const int32_t* __restrict__, int32_t, int32_t);
-YEE_BATCH_KERNEL(2, false)
-YEE_BATCH_KERNEL(2, true)
-YEE_BATCH_KERNEL(4, false)
-YEE_BATCH_KERNEL(4, true)
+// LEFT: the edge kernel of a batch with an H_z member that has a y_hi copy wall on a tile edge; its members take the
+// general loop with the lane-0 recompute (a separate instantiation: that loop's registers stay out of the plain one).
+#define YEE_BATCH_KERNEL(VEC, EDGE, LEFT)                                                                        \
+    template __global__ void yee2d_batch_kernel<VEC, EDGE, LEFT>(const yee2d_desc* __restrict__,                \
+                                                                 const yee2d_tile_sets* __restrict__,           \
+                                                                 const int32_t* __restrict__, int32_t, int32_t);
+YEE_BATCH_KERNEL(2, false, false)
+YEE_BATCH_KERNEL(2, true, false)
+YEE_BATCH_KERNEL(2, true, true)
+YEE_BATCH_KERNEL(4, false, false)
+YEE_BATCH_KERNEL(4, true, false)
+YEE_BATCH_KERNEL(4, true, true)
 
-// Field probes after step n = *clock + offset: samples[n * n_probes + k] = field[new set][component[k]][cell[k]]
-// (core/monitors.py:42-130), cell = i * pitch + j. One block; the descriptor is read in place as in yee2d_step_kernel.
-__global__ void __launch_bounds__(256) yee2d_probe_kernel(yee2d_desc desc, int32_t offset)
+namespace yee {
+
+// Field probes of one instance after step n: samples[n * n_probes + k] = field[new set][component[k]][cell[k]]
+// (core/monitors.py:42-130), cell = i * pitch + j; the block's lanes stride over the probes.
+__device__ __forceinline__ void probes_2d(const yee2d_desc& d, int64_t n)
 {
-    typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
-    const yee2d_desc& d = *(const yee2d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
-    if (d.clock != desc.clock) return;
-    const int64_t n = *d.clock + offset;
     const yee_probes& p = d.probes;
     if (n >= d.step_limit || n >= p.capacity) return;
     const int set = (int)((n & 1) ^ 1);
@@ -1583,17 +1594,14 @@ __global__ void __launch_bounds__(256) yee2d_probe_kernel(yee2d_desc desc, int32
     }
 }
 
-// E_z outer-face rows after step n = *clock + offset (physics/em/boundaries.py:105-128 wall_centre_rows, :478-488):
+// E_z outer-face rows of one instance after step n (physics/em/boundaries.py:105-128 wall_centre_rows, :478-488):
 // lane k < ny pins column k of the pinned x faces from row 1 (nx - 2), taking the y factor and the inner column at a
-// pinned y corner; lane ny + i pins the y faces of row i (corner rows belong to the x lanes). Every lane reads only
-// rows and columns no lane writes, so the lanes need no ordering.
-__global__ void __launch_bounds__(256) yee2d_wall_kernel(yee2d_desc desc, int32_t offset)
+// pinned y corner; lane ny + i pins the y faces of row i (corner rows belong to the x lanes); lanes beyond do
+// nothing. Every lane reads only rows and columns no lane writes, so the lanes need no ordering.
+__device__ __forceinline__ void ez_walls(const yee2d_desc& d, int64_t n, int64_t k)
 {
-    const yee2d_desc& d = desc;
-    const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
     float* ez = d.field[(n & 1) ^ 1][0];
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t P = d.pitch;
     const float s_xlo = (float)d.wall_sign[0], s_xhi = (float)d.wall_sign[1];
     const float s_ylo = (float)d.wall_sign[2], s_yhi = (float)d.wall_sign[3];
@@ -1611,6 +1619,47 @@ __global__ void __launch_bounds__(256) yee2d_wall_kernel(yee2d_desc desc, int32_
         if (d.wall_sign[2] != 0) ez[row] = s_ylo * ez[row + 1];
         if (d.wall_sign[3] != 0) ez[row + d.ny - 1] = s_yhi * ez[row + d.ny - 2];
     }
+}
+
+__device__ __forceinline__ bool has_ez_walls(const yee2d_desc& d)
+{
+    return d.polarization == YEE_EZ &&
+           (d.wall_sign[0] != 0 || d.wall_sign[1] != 0 || d.wall_sign[2] != 0 || d.wall_sign[3] != 0);
+}
+
+}  // namespace yee
+
+// Field probes after step n = *clock + offset. One block; the descriptor is read in place as in yee2d_step_kernel.
+__global__ void __launch_bounds__(256) yee2d_probe_kernel(yee2d_desc desc, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
+    const yee2d_desc& d = *(const yee2d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
+    if (d.clock != desc.clock) return;
+    yee::probes_2d(d, *d.clock + offset);
+}
+
+// E_z outer-face rows after step n = *clock + offset, one lane per column and row.
+__global__ void __launch_bounds__(256) yee2d_wall_kernel(yee2d_desc desc, int32_t offset)
+{
+    yee::ez_walls(desc, *desc.clock + offset, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The probes of a batch's members after their step *clock + offset: block k serves member k (members without probes
+// return at once). Descriptors are read through the constant address space.
+__global__ void __launch_bounds__(256) yee2d_batch_probe_kernel(const yee2d_desc* __restrict__ descs, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
+    const yee2d_desc& d = *(const yee2d_desc*)((ConstDesc*)descs + blockIdx.x);
+    if (d.probes.n_probes > 0) yee::probes_2d(d, *d.clock + offset);
+}
+
+// The E_z walls of a batch's members: blockIdx.y is the member, blockIdx.x * 256 + lane its column / row lane (the
+// grid spans the largest member; members without E_z walls return at once).
+__global__ void __launch_bounds__(256) yee2d_batch_wall_kernel(const yee2d_desc* __restrict__ descs, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
+    const yee2d_desc& d = *(const yee2d_desc*)((ConstDesc*)descs + blockIdx.y);
+    if (yee::has_ez_walls(d)) yee::ez_walls(d, *d.clock + offset, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Advance one clock by k steps (bounded by the instance's step limit); a single lane writes with a vector store.

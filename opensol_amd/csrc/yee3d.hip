@@ -565,7 +565,7 @@ __global__ void __launch_bounds__(256) yee3d_step_kernel(yee3d_desc desc, int32_
 
 // Many 3D instances of uneven sizes in one launch: block b belongs to the instance whose block range
 // [prefix[k], prefix[k+1]) holds it, so a block stages one instance's table. Each instance runs on its own clock
-// and stops at its own step limit.
+// and stops at its own step limit; members with dispersive poles take the E update that carries them.
 template <int J>
 __global__ void __launch_bounds__(256) yee3d_batch_kernel(const yee3d_desc* __restrict__ descs,
                                                          const int32_t* __restrict__ prefix, int32_t n_inst,
@@ -585,7 +585,11 @@ __global__ void __launch_bounds__(256) yee3d_batch_kernel(const yee3d_desc* __re
     const int tiles_j = (d.ny + J - 1) / J;
     const int tiles_k = (d.nz + yee3::OWNED - 1) / yee3::OWNED;
     const int tiles = ((d.nx + d.planes_per_tile - 1) / d.planes_per_tile) * tiles_j * tiles_k;
-    yee3::step_block<J, false>(d, n, (block - prefix[lo]) * 4 + yee3::wave_index(), tiles, tiles_j, tiles_k);
+    const int tile = (block - prefix[lo]) * 4 + yee3::wave_index();
+    if (d.n_poles > 0)   // the E update with the member's dispersive-pole records (block-uniform: one member per block)
+        yee3::step_block<J, true>(d, n, tile, tiles, tiles_j, tiles_k);
+    else
+        yee3::step_block<J, false>(d, n, tile, tiles, tiles_j, tiles_k);
 }
 
 template __global__ void yee3d_step_kernel<4, false>(yee3d_desc, int32_t, int32_t, int32_t, int32_t);
@@ -671,22 +675,18 @@ __global__ void __launch_bounds__(256) yee3d_batch_sample_kernel(const yee3d_des
     yee3::sample_cell(d, *d.clock + offset, t - prefix[lo]);
 }
 
-// Mirror face `face` (0 x_lo .. 5 z_hi) after step n = *clock + offset, one lane per cell of the face: symmetric, the
-// normal E on the plane (the hi face of cell 0 / n - 2) = 0; antisymmetric, the ghost cell's (0 / n - 1) tangential E =
-// minus its inner neighbour's (1 / n - 2) (physics/em3d/engine3d.py:531-540). The host launches the symmetric faces
-// first, then the antisymmetric ones in face order, as the reference writes its zero rows before its images.
-__global__ void __launch_bounds__(256) yee3d_mirror_kernel(yee3d_desc desc, int32_t face, int32_t offset)
+namespace yee3 {
+
+// Mirror face `face` (0 x_lo .. 5 z_hi) of one instance after step n, cell t of the face: symmetric, the normal E on
+// the plane (the hi face of cell 0 / n - 2) = 0; antisymmetric, the ghost cell's (0 / n - 1) tangential E = minus its
+// inner neighbour's (1 / n - 2) (physics/em3d/engine3d.py:531-540). The host launches the symmetric faces first, then
+// the antisymmetric ones in face order, as the reference writes its zero rows before its images.
+__device__ __forceinline__ void mirror_cell(const yee3d_desc& d, int64_t n, int face, int64_t t)
 {
-    // the descriptor in place at the kernel-argument pointer: the field table is indexed by runtime axes
-    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
-    const yee3d_desc& d = *(const yee3d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
-    if (d.clock != desc.clock) return;
-    const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
     const int a = face >> 1, hi = face & 1;
     const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;   // the two tangential axes, in grid order
     const int32_t ext[3] = {d.nx, d.ny, d.nz};
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)ext[b] * ext[c]) return;
     int idx[3];
     idx[b] = (int)(t / ext[c]);
@@ -705,14 +705,10 @@ __global__ void __launch_bounds__(256) yee3d_mirror_kernel(yee3d_desc desc, int3
     }
 }
 
-// Field probes after step n = *clock + offset: samples[n * n_probes + k] = field[new set][component[k]][cell[k]]
-// (core/monitors.py:42-130, physics/em3d/physics3d.py:425-433). One block; the descriptor is read in place.
-__global__ void __launch_bounds__(256) yee3d_probe_kernel(yee3d_desc desc, int32_t offset)
+// Field probes of one instance after step n: samples[n * n_probes + k] = field[new set][component[k]][cell[k]]
+// (core/monitors.py:42-130, physics/em3d/physics3d.py:425-433); the block's lanes stride over the probes.
+__device__ __forceinline__ void probes_3d(const yee3d_desc& d, int64_t n)
 {
-    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
-    const yee3d_desc& d = *(const yee3d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
-    if (d.clock != desc.clock) return;
-    const int64_t n = *d.clock + offset;
     const yee_probes& p = d.probes;
     if (n >= d.step_limit || n >= p.capacity) return;
     const int set = (int)((n & 1) ^ 1);
@@ -724,6 +720,47 @@ __global__ void __launch_bounds__(256) yee3d_probe_kernel(yee3d_desc desc, int32
         p.samples[n * p.n_probes + k] = (c >= 0 && c < 6 && cell >= 0 && cell < cells)
                                             ? d.field[set][c][cell] : __builtin_nanf("");
     }
+}
+
+}  // namespace yee3
+
+// Mirror face `face` after step n = *clock + offset, one lane per cell of the face. The descriptor is read in place at
+// the kernel-argument pointer: the field table is indexed by runtime axes.
+__global__ void __launch_bounds__(256) yee3d_mirror_kernel(yee3d_desc desc, int32_t face, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
+    const yee3d_desc& d = *(const yee3d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
+    if (d.clock != desc.clock) return;
+    yee3::mirror_cell(d, *d.clock + offset, face, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Field probes after step n = *clock + offset. One block; the descriptor is read in place.
+__global__ void __launch_bounds__(256) yee3d_probe_kernel(yee3d_desc desc, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
+    const yee3d_desc& d = *(const yee3d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
+    if (d.clock != desc.clock) return;
+    yee3::probes_3d(d, *d.clock + offset);
+}
+
+// Face `face` of the batch's members whose mirror there is of `kind`: blockIdx.y is the member, blockIdx.x * 256 +
+// lane the face cell (the grid spans the largest face; the rest return at once). The host launches the kinds and faces
+// in the single-instance order.
+__global__ void __launch_bounds__(256) yee3d_batch_mirror_kernel(const yee3d_desc* __restrict__ descs, int32_t face,
+                                                                int32_t kind, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
+    const yee3d_desc& d = *(const yee3d_desc*)((ConstDesc*)descs + blockIdx.y);
+    if (d.mirror[face] == kind)
+        yee3::mirror_cell(d, *d.clock + offset, face, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The probes of a batch's members: block k serves member k (members without probes return at once).
+__global__ void __launch_bounds__(256) yee3d_batch_probe_kernel(const yee3d_desc* __restrict__ descs, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
+    const yee3d_desc& d = *(const yee3d_desc*)((ConstDesc*)descs + blockIdx.x);
+    if (d.probes.n_probes > 0) yee3::probes_3d(d, *d.clock + offset);
 }
 
 // Advance the clocks of a 3D batch by k steps, each bounded by its instance's step limit.

@@ -17,7 +17,7 @@
 
 template <int POL, int VEC, bool EDGE, bool POLES, bool LEFT>
 __global__ void yee2d_step_kernel(yee2d_desc d, yee2d_tile_sets sets, int32_t offset);
-template <int VEC, bool EDGE>
+template <int VEC, bool EDGE, bool LEFT>
 __global__ void yee2d_batch_kernel(const yee2d_desc* __restrict__ descs, const yee2d_tile_sets* __restrict__ sets,
                                    const int32_t* __restrict__ prefix, int32_t n_inst, int32_t offset);
 template <int J, bool POLES>
@@ -37,6 +37,13 @@ __global__ void yee3d_mirror_kernel(yee3d_desc d, int32_t face, int32_t offset);
 __global__ void yee2d_dft_kernel(yee2d_desc d, int32_t offset);
 __global__ void yee3d_dft_kernel(yee3d_desc d, int32_t offset);
 __global__ void yee_clock_advance_batch(const yee2d_desc* __restrict__ descs, int32_t n_inst, int64_t k);
+__global__ void yee2d_batch_probe_kernel(const yee2d_desc* __restrict__ descs, int32_t offset);
+__global__ void yee2d_batch_wall_kernel(const yee2d_desc* __restrict__ descs, int32_t offset);
+__global__ void yee2d_batch_dft_kernel(const yee2d_desc* __restrict__ descs, int32_t offset);
+__global__ void yee3d_batch_dft_kernel(const yee3d_desc* __restrict__ descs, int32_t offset);
+__global__ void yee3d_batch_mirror_kernel(const yee3d_desc* __restrict__ descs, int32_t face, int32_t kind,
+                                          int32_t offset);
+__global__ void yee3d_batch_probe_kernel(const yee3d_desc* __restrict__ descs, int32_t offset);
 
 namespace {
 
@@ -523,6 +530,36 @@ void launch_mirrors_3d(const yee3d_desc& d, int32_t offset, hipStream_t stream)
     }
 }
 
+// The launches a 3D batch's members need after each step: per mirror kind (symmetric, antisymmetric) and face, the
+// blocks of the largest such face of any member (0: none has it); DFT regions and probes as in BatchAux2D.
+struct BatchAux3D {
+    int mirror_blocks[2][6] = {};
+    int dft_blocks = 0, dft_regions = 0;
+    bool probes = false;
+};
+
+BatchAux3D batch_aux_3d(const yee3d_desc* descs, int32_t n)
+{
+    BatchAux3D a;
+    for (int32_t k = 0; k < n; ++k) {
+        const yee3d_desc& d = descs[k];
+        for (int face = 0; face < 6; ++face) {
+            const int kind = d.mirror[face] == YEE_MIRROR_SYMMETRIC ? 0 : d.mirror[face] == YEE_MIRROR_ANTISYMMETRIC ? 1 : -1;
+            if (kind < 0) continue;
+            const int axis = face >> 1;
+            const int64_t cells = axis == 0 ? (int64_t)d.ny * d.nz : axis == 1 ? (int64_t)d.nx * d.nz
+                                                                               : (int64_t)d.nx * d.ny;
+            a.mirror_blocks[kind][face] = std::max(a.mirror_blocks[kind][face], (int)((cells + 255) / 256));
+        }
+        if (d.n_dft > 0) {
+            a.dft_blocks = std::max(a.dft_blocks, dft_blocks(d.dft, d.n_dft));
+            a.dft_regions = std::max(a.dft_regions, (int)d.n_dft);
+        }
+        a.probes = a.probes || d.probes.n_probes > 0;
+    }
+    return a;
+}
+
 bool has_walls(const yee2d_desc& d)
 {
     return d.wall_sign[0] != 0 || d.wall_sign[1] != 0 || d.wall_sign[2] != 0 || d.wall_sign[3] != 0;
@@ -551,7 +588,32 @@ constexpr Batch2DLayout batch2d_layout(int32_t n)
     return Batch2DLayout{sets, bulk, edge, edge + sizeof(int32_t) * (size_t)(n + 1)};
 }
 
-// A 2D batch on the device: where its arrays are and how many tiles of each class it has.
+// The launches a 2D batch's members need after each step, over all members at once: E_z walls (blocks along x of the
+// largest member's wall lanes), DFT regions (blocks of the largest region, most regions of a member) and probes. Zero
+// blocks: no member needs that launch.
+struct BatchAux2D {
+    int wall_blocks = 0, dft_blocks = 0, dft_regions = 0;
+    bool probes = false;
+    bool left = false;   // an H_z member with a y_hi copy wall on a tile edge: the edge kernel's LEFT instantiation
+};
+
+BatchAux2D batch_aux_2d(const yee2d_desc* descs, int32_t n)
+{
+    BatchAux2D a;
+    for (int32_t k = 0; k < n; ++k) {
+        const yee2d_desc& d = descs[k];
+        if (has_ez_walls(d)) a.wall_blocks = std::max(a.wall_blocks, wall_blocks(d));
+        if (d.n_dft > 0) {
+            a.dft_blocks = std::max(a.dft_blocks, dft_blocks(d.dft, d.n_dft));
+            a.dft_regions = std::max(a.dft_regions, (int)d.n_dft);
+        }
+        a.probes = a.probes || d.probes.n_probes > 0;
+        a.left = a.left || needs_left_hz(d);
+    }
+    return a;
+}
+
+// A 2D batch on the device: where its arrays are, how many tiles of each class it has and its after-step launches.
 struct Batch2DArgs {
     const yee2d_desc* descs;
     const yee2d_tile_sets* sets;
@@ -560,30 +622,43 @@ struct Batch2DArgs {
     int32_t n;
     int64_t bulk_tiles, edge_tiles;
     int32_t lane_columns;
+    BatchAux2D aux;
 };
 
-Batch2DArgs batch2d_args(void* base, int32_t n, int64_t bulk_tiles, int64_t edge_tiles, int32_t lane_columns)
+Batch2DArgs batch2d_args(void* base, int32_t n, int64_t bulk_tiles, int64_t edge_tiles, int32_t lane_columns,
+                         const BatchAux2D& aux)
 {
     const Batch2DLayout l = batch2d_layout(n);
     char* b = static_cast<char*>(base);
     return Batch2DArgs{reinterpret_cast<const yee2d_desc*>(b), reinterpret_cast<const yee2d_tile_sets*>(b + l.sets),
                        reinterpret_cast<const int32_t*>(b + l.bulk_prefix),
-                       reinterpret_cast<const int32_t*>(b + l.edge_prefix), n, bulk_tiles, edge_tiles, lane_columns};
+                       reinterpret_cast<const int32_t*>(b + l.edge_prefix), n, bulk_tiles, edge_tiles, lane_columns,
+                       aux};
 }
 
 // count launches of the batched bulk-tile kernel (the lean loops) and of the batched edge-tile kernel (the general
-// loops, one wavefront per tile: the many members keep the device busy), then the clocks.
+// loops, one wavefront per tile: the many members keep the device busy), each step followed by the members' E_z
+// walls, DFT regions and probes in the single-instance order, then the clocks.
 int32_t launch_batch_2d(const Batch2DArgs& b, int32_t count, hipStream_t stream)
 {
-    auto bulk = b.lane_columns == 2 ? yee2d_batch_kernel<2, false> : yee2d_batch_kernel<4, false>;
-    auto edge = b.lane_columns == 2 ? yee2d_batch_kernel<2, true> : yee2d_batch_kernel<4, true>;
+    auto bulk = b.lane_columns == 2 ? yee2d_batch_kernel<2, false, false> : yee2d_batch_kernel<4, false, false>;
+    auto edge = b.lane_columns == 2
+                    ? (b.aux.left ? yee2d_batch_kernel<2, true, true> : yee2d_batch_kernel<2, true, false>)
+                    : (b.aux.left ? yee2d_batch_kernel<4, true, true> : yee2d_batch_kernel<4, true, false>);
     const int bulk_blocks = (int)((b.bulk_tiles + 3) / 4);
     const int edge_blocks = (int)((b.edge_tiles + 3) / 4);
+    const BatchAux2D& a = b.aux;
     for (int32_t k = 0; k < count; ++k) {
         if (bulk_blocks > 0)
             hipLaunchKernelGGL(bulk, dim3(bulk_blocks), dim3(256), 0, stream, b.descs, b.sets, b.bulk_prefix, b.n, k);
         if (edge_blocks > 0)
             hipLaunchKernelGGL(edge, dim3(edge_blocks), dim3(256), 0, stream, b.descs, b.sets, b.edge_prefix, b.n, k);
+        if (a.wall_blocks > 0)
+            hipLaunchKernelGGL(yee2d_batch_wall_kernel, dim3(a.wall_blocks, b.n), dim3(256), 0, stream, b.descs, k);
+        if (a.dft_blocks > 0)
+            hipLaunchKernelGGL(yee2d_batch_dft_kernel, dim3(a.dft_blocks, a.dft_regions, b.n), dim3(256), 0, stream,
+                               b.descs, k);
+        if (a.probes) hipLaunchKernelGGL(yee2d_batch_probe_kernel, dim3(b.n), dim3(256), 0, stream, b.descs, k);
     }
     YEE_HIP(hipGetLastError());
     const int threads = 256;
@@ -671,12 +746,26 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
     }
     if (ctx->kind == kBatch3D) {
         const int sample_blocks = (int)((ctx->sample_cells + 255) / 256);
+        const BatchAux3D a = batch_aux_3d(ctx->descs3.data(), ctx->n_inst);
         for (int32_t k = 0; k < count; ++k) {
             hipLaunchKernelGGL(yee3d_batch_kernel<kRows3D>, dim3(ctx->blocks), dim3(256), 0, stream, ctx->descs3_dev,
                                ctx->block_prefix_dev, ctx->n_inst, k);
+            // mirror faces as a single instance orders them: every symmetric face, then the antisymmetric ones
+            for (int kind = 0; kind < 2; ++kind)
+                for (int face = 0; face < 6; ++face)
+                    if (a.mirror_blocks[kind][face] > 0)
+                        hipLaunchKernelGGL(yee3d_batch_mirror_kernel, dim3(a.mirror_blocks[kind][face], ctx->n_inst),
+                                           dim3(256), 0, stream, ctx->descs3_dev, face,
+                                           kind == 0 ? YEE_MIRROR_SYMMETRIC : YEE_MIRROR_ANTISYMMETRIC, k);
             if (sample_blocks > 0)
                 hipLaunchKernelGGL(yee3d_batch_sample_kernel, dim3(sample_blocks), dim3(256), 0, stream,
                                    ctx->descs3_dev, ctx->sample_prefix_dev, ctx->n_inst, k);
+            if (a.dft_blocks > 0)
+                hipLaunchKernelGGL(yee3d_batch_dft_kernel, dim3(a.dft_blocks, a.dft_regions, ctx->n_inst), dim3(256),
+                                   0, stream, ctx->descs3_dev, k);
+            if (a.probes)
+                hipLaunchKernelGGL(yee3d_batch_probe_kernel, dim3(ctx->n_inst), dim3(256), 0, stream,
+                                   ctx->descs3_dev, k);
         }
         YEE_HIP(hipGetLastError());
         const int threads = 256;
@@ -687,7 +776,7 @@ int32_t launch_steps(yee_ctx* ctx, int32_t count, hipStream_t stream)
     }
     if (ctx->kind == kBatch2D)
         return launch_batch_2d(batch2d_args(ctx->descs_dev, ctx->n_inst, ctx->tiles, ctx->general_tiles,
-                                            ctx->descs[0].lane_columns),
+                                            ctx->descs[0].lane_columns, batch_aux_2d(ctx->descs.data(), ctx->n_inst)),
                                count, stream);
     return fail(YEE_ESTATE, "context has no instances");
 }
@@ -715,19 +804,13 @@ int32_t build_graph(yee_ctx* ctx)
     return YEE_OK;
 }
 
-// One member k of a 2D batch (defaults already resolved): the single-instance checks and the batch's restrictions.
+// One member k of a 2D batch (defaults already resolved): the single-instance checks and the batch's one restriction,
+// a shared lane_columns (every member's probes, DFT regions, walls, TFSF boxes and poles are stepped as alone).
 int32_t check_batch_member_2d(const yee2d_desc& d, int32_t lane_columns, int32_t k)
 {
     int32_t status = check_2d(&d);
     if (status == YEE_OK && d.lane_columns != lane_columns)
         status = fail(YEE_EINVAL, "batched instances must share lane_columns");
-    if (status == YEE_OK && d.probes.n_probes > 0)
-        status = fail(YEE_EINVAL, "batched instances take no field probes");
-    if (status == YEE_OK && d.n_dft > 0) status = fail(YEE_EINVAL, "batched instances take no DFT regions");
-    if (status == YEE_OK && has_walls(d))
-        status = fail(YEE_EINVAL, "batched instances take no wall rows");
-    else if (status == YEE_OK && d.n_tfsf > 0)
-        status = fail(YEE_EINVAL, "batched instances take no TFSF boxes");
     if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
     return YEE_OK;
 }
@@ -909,7 +992,8 @@ int32_t yee2d_batch_step_static(const yee2d_desc* descs, int32_t n, int64_t n_st
         YEE_HIP(hipMemcpyAsync(base, g_static_image, batch2d_layout(n).bytes, hipMemcpyHostToDevice, s));
         YEE_HIP(hipStreamSynchronize(s));
     }
-    const Batch2DArgs args = batch2d_args(base, n, bulk_tiles, edge_tiles, descs[0].lane_columns);
+    const Batch2DArgs args = batch2d_args(base, n, bulk_tiles, edge_tiles, descs[0].lane_columns,
+                                          batch_aux_2d(descs, n));
     for (int64_t remaining = n_steps; remaining > 0;) {
         const int32_t chunk = remaining > 4096 ? 4096 : (int32_t)remaining;
         status = launch_batch_2d(args, chunk, s);
@@ -1131,11 +1215,6 @@ int32_t yee3d_batch_create(const yee3d_desc* descs, int32_t n, void* descs_dev, 
         const int32_t status = check_3d(&host[k]);
         if (status != YEE_OK) return fail(status, "instance " + std::to_string(k) + ": " + g_last_error);
         if (host[k].rows_per_tile != kRows3D) return fail(YEE_EINVAL, "batched 3D instances use 4 rows per tile");
-        if (host[k].probes.n_probes > 0) return fail(YEE_EINVAL, "batched instances take no field probes");
-        if (host[k].n_dft > 0) return fail(YEE_EINVAL, "batched instances take no DFT regions");
-        if (host[k].n_poles > 0) return fail(YEE_EINVAL, "batched 3D instances take no dispersive poles");
-        for (int f = 0; f < 6; ++f)
-            if (host[k].mirror[f] != 0) return fail(YEE_EINVAL, "batched 3D instances take no mirror faces");
         const int64_t next = block_prefix[k] + (tiles_3d(host[k]) + 3) / 4;
         if (next > INT32_MAX / 2) return fail(YEE_EINVAL, "batch has too many tiles");
         block_prefix[k + 1] = (int32_t)next;

@@ -74,17 +74,11 @@ __device__ __forceinline__ int64_t sample_of(int64_t n, int32_t m)
     return ((n + 1) % m) == 0 ? (n + 1) / m - 1 : -1;
 }
 
-}  // namespace
-
-// DFT regions of a 2D instance after step n = *clock + offset (cell index i * pitch + j).
-__global__ void __launch_bounds__(256) yee2d_dft_kernel(yee2d_desc desc, int32_t offset)
+// Region `region` of a 2D instance after step n (cell index i * pitch + j); the grid's x blocks stride over its cells.
+__device__ __forceinline__ void dft_region_2d(const yee2d_desc& d, int64_t n, int region)
 {
-    typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
-    const yee2d_desc& d = *(const yee2d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
-    if (d.clock != desc.clock) return;
-    const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
-    const yee_dft_region& r = d.dft[blockIdx.y];
+    const yee_dft_region& r = d.dft[region];
     const int64_t s = sample_of(n, d.sample_interval);
     if (s < 0 || s >= r.n_samples) return;
     const int set = (int)((n & 1) ^ 1);
@@ -95,15 +89,11 @@ __global__ void __launch_bounds__(256) yee2d_dft_kernel(yee2d_desc desc, int32_t
         dft_cell(r, d.field[set], n_axis, axis_stride, s, t);
 }
 
-// DFT regions of a 3D instance after step n = *clock + offset (cell index (i * ny + j) * nz + k).
-__global__ void __launch_bounds__(256) yee3d_dft_kernel(yee3d_desc desc, int32_t offset)
+// Region `region` of a 3D instance after step n (cell index (i * ny + j) * nz + k).
+__device__ __forceinline__ void dft_region_3d(const yee3d_desc& d, int64_t n, int region)
 {
-    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
-    const yee3d_desc& d = *(const yee3d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
-    if (d.clock != desc.clock) return;
-    const int64_t n = *d.clock + offset;
     if (n >= d.step_limit) return;
-    const yee_dft_region& r = d.dft[blockIdx.y];
+    const yee_dft_region& r = d.dft[region];
     const int64_t s = sample_of(n, d.sample_interval);
     if (s < 0 || s >= r.n_samples) return;
     const int set = (int)((n & 1) ^ 1);
@@ -112,4 +102,41 @@ __global__ void __launch_bounds__(256) yee3d_dft_kernel(yee3d_desc desc, int32_t
     const int64_t cells = (int64_t)r.count[0] * r.count[1] * r.count[2];
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < cells; t += (int64_t)gridDim.x * blockDim.x)
         dft_cell(r, d.field[set], n_axis, axis_stride, s, t);
+}
+
+}  // namespace
+
+// DFT regions of a 2D instance after step n = *clock + offset, blockIdx.y per region.
+__global__ void __launch_bounds__(256) yee2d_dft_kernel(yee2d_desc desc, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
+    const yee2d_desc& d = *(const yee2d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
+    if (d.clock != desc.clock) return;
+    dft_region_2d(d, *d.clock + offset, blockIdx.y);
+}
+
+// DFT regions of a 3D instance after step n = *clock + offset, blockIdx.y per region.
+__global__ void __launch_bounds__(256) yee3d_dft_kernel(yee3d_desc desc, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
+    const yee3d_desc& d = *(const yee3d_desc*)(ConstDesc*)__builtin_amdgcn_kernarg_segment_ptr();
+    if (d.clock != desc.clock) return;
+    dft_region_3d(d, *d.clock + offset, blockIdx.y);
+}
+
+// The DFT regions of a batch's members: blockIdx.z is the member, blockIdx.y its region (the grid spans the most
+// regions and the largest region of any member; the rest return at once). Descriptors are read through the constant
+// address space.
+__global__ void __launch_bounds__(256) yee2d_batch_dft_kernel(const yee2d_desc* __restrict__ descs, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee2d_desc ConstDesc;
+    const yee2d_desc& d = *(const yee2d_desc*)((ConstDesc*)descs + blockIdx.z);
+    if ((int)blockIdx.y < d.n_dft) dft_region_2d(d, *d.clock + offset, blockIdx.y);
+}
+
+__global__ void __launch_bounds__(256) yee3d_batch_dft_kernel(const yee3d_desc* __restrict__ descs, int32_t offset)
+{
+    typedef const __attribute__((address_space(4))) yee3d_desc ConstDesc;
+    const yee3d_desc& d = *(const yee3d_desc*)((ConstDesc*)descs + blockIdx.z);
+    if ((int)blockIdx.y < d.n_dft) dft_region_3d(d, *d.clock + offset, blockIdx.y);
 }
