@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define YEE_ABI_VERSION 18
+#define YEE_ABI_VERSION 19
 #define YEE_MAX_SOURCES 8
 #define YEE_MAX_MONITORS 4
 #define YEE_MAX_DFT 4
@@ -487,6 +487,15 @@ int32_t yee3d_batch_create(const yee3d_desc* descs, int32_t n, void* descs_dev, 
  */
 #define YEE_STATIC_BATCH_MAX 256
 int32_t yee2d_batch_step_static(const yee2d_desc* descs, int32_t n, int64_t n_steps, void* stream);
+
+/*
+ * yee_flag_nonfinite: enqueue on `stream` a check of data[0 .. count) (device floats, e.g. an instance's current field
+ * set) that sets *flag (a device int32) to 1 when any value is NaN or infinite; the flag is never cleared, so the
+ * caller zeroes it and reads it after the stream has passed the check. No counterpart in the reference, which checks
+ * Courant stability before a run and nothing during it (physics/em/__init__.py:242-281); it lets a server stop a
+ * diverging job at the next chunk boundary instead of at its end.
+ */
+int32_t yee_flag_nonfinite(const float* data, int64_t count, int32_t* flag, void* stream);
 
 #ifdef __cplusplus
 }

@@ -6,7 +6,7 @@ import os
 # OPENSOL_AMD_LIBRARY names another build of the same library (e.g. one compiled with other launch options)
 LIBRARY_PATH = os.environ.get('OPENSOL_AMD_LIBRARY') or os.path.join(os.path.dirname(os.path.abspath(__file__)),
                                                                      'libyee.so')
-ABI_VERSION = 18
+ABI_VERSION = 19
 MAX_SOURCES = 8
 SOURCE_E, SOURCE_H = 0, 1   # yee_line_source.field
 WALL_ZERO = 2   # yee2d_desc.wall_sign of an H_z pmc face
@@ -138,6 +138,7 @@ SIGNATURES = (
     ('yee3d_batch_bytes', ctypes.c_size_t, (c_i32,)),
     ('yee3d_batch_create', c_i32, (ctypes.POINTER(Desc3DC), c_i32, c_ptr, c_ptr, ctypes.POINTER(c_ptr))),
     ('yee2d_batch_step_static', c_i32, (ctypes.POINTER(Desc2DC), c_i32, c_i64, c_ptr)),
+    ('yee_flag_nonfinite', c_i32, (c_ptr, c_i64, c_ptr, c_ptr)),
 )
 
 _LIB = None

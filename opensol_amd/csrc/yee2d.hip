@@ -1662,6 +1662,17 @@ __global__ void __launch_bounds__(256) yee2d_batch_wall_kernel(const yee2d_desc*
     if (yee::has_ez_walls(d)) yee::ez_walls(d, *d.clock + offset, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
+// flag[0] = 1 when any of data[0 .. count) is NaN or infinite (grid-stride; lanes that find one store the same value,
+// so the stores need no ordering). The flag is never cleared here.
+__global__ void __launch_bounds__(256) yee_nonfinite_kernel(const float* __restrict__ data, int64_t count,
+                                                           int32_t* flag)
+{
+    bool bad = false;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (int64_t)gridDim.x * blockDim.x)
+        bad = bad || !__builtin_isfinite(data[t]);
+    if (bad) flag[0] = 1;
+}
+
 // Advance one clock by k steps (bounded by the instance's step limit); a single lane writes with a vector store.
 __global__ void yee_clock_advance(int64_t* clock, int64_t k, int64_t limit)
 {

@@ -44,6 +44,7 @@ __global__ void yee3d_batch_dft_kernel(const yee3d_desc* __restrict__ descs, int
 __global__ void yee3d_batch_mirror_kernel(const yee3d_desc* __restrict__ descs, int32_t face, int32_t kind,
                                           int32_t offset);
 __global__ void yee3d_batch_probe_kernel(const yee3d_desc* __restrict__ descs, int32_t offset);
+__global__ void yee_nonfinite_kernel(const float* __restrict__ data, int64_t count, int32_t* flag);
 
 namespace {
 
@@ -1000,6 +1001,18 @@ int32_t yee2d_batch_step_static(const yee2d_desc* descs, int32_t n, int64_t n_st
         if (status != YEE_OK) return status;
         remaining -= chunk;
     }
+    return YEE_OK;
+}
+
+int32_t yee_flag_nonfinite(const float* data, int64_t count, int32_t* flag, void* stream)
+{
+    if (count < 0 || (count > 0 && (!data || !flag)))
+        return fail(YEE_EINVAL, "yee_flag_nonfinite needs data, flag and count >= 0");
+    if (count == 0) return YEE_OK;
+    const int64_t blocks = std::min<int64_t>(2048, (count + 255) / 256);
+    hipLaunchKernelGGL(yee_nonfinite_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), data, count, flag);
+    YEE_HIP(hipGetLastError());
     return YEE_OK;
 }
 

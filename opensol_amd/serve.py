@@ -17,8 +17,17 @@ Failure handling. A job whose device state alone exceeds the HBM budget is refus
 with the reason in job.error), as the reference's memory budget refuses an over-sized model instead of spilling
 (core/gpu_memory.py:177, 271-290). Job.cancel() may be called from any thread; the server honours it at the next
 chunk boundary, like the reference's RunControl.checkpoint between study steps (model/study.py:194-251): a pending
-job is dropped, a running one stops after the chunk in flight and its state is freed (status 'cancelled'). A job
-whose spectra come out non-finite is reported as 'failed'.
+job is dropped, a running one stops after the chunk in flight and its state is freed (status 'cancelled'). After
+every chunk each job's current field set is checked on the device for NaN / Inf (yee_flag_nonfinite, read when the
+chunk is collected); a flagged job stops there and is reported as 'failed', as is one whose spectra come out
+non-finite at the end.
+
+Preemption (preempt_after=seconds). When the job at the head of the queue fits the budget but not the memory left,
+and has waited that long, the most recently admitted running jobs are preempted until it fits: each one's evolving
+state is copied to the host (solver.solution_state: both field sets, psi, poles, accumulators, probes, clock), its
+device state is freed, and it goes back to the queue right behind the new job; re-admitted, it gets a new engine, its
+state is written back (restore_solution) and it continues bit for bit. A job is preempted at most once and only a job
+that has not run yet triggers preemption, so jobs cannot preempt each other in turn.
 
 Every job's result is the same, bit for bit, whether it runs alone or in the mix: the batched launches and the
 single-instance kernels give identical bits, and every instance keeps its own clock.
@@ -48,6 +57,8 @@ class Job:
     result: list = field(default_factory=list)
     status: str = 'pending'   # pending, running, done, rejected, cancelled, failed
     error: str = ''
+    preemptions: int = 0
+    saved: dict = field(default=None, repr=False)   # host state while preempted
     _cancel: threading.Event = field(default_factory=threading.Event, repr=False)
 
     def cancel(self):
@@ -96,16 +107,35 @@ def _finish(job, now):
         job.error = 'non-finite flux spectra (unstable or diverging fields)'
 
 
+def choose_victims(head_bytes, free_bytes, running):
+    """The running jobs to preempt so that a job of head_bytes fits: the most recently admitted first (the later arrival
+    first among jobs admitted together), each at most once (jobs already preempted are skipped), until free_bytes plus
+    theirs covers it; [] when even all of them would not."""
+    chosen, free = [], float(free_bytes)
+    for job in sorted(running, key=lambda j: (j.admitted, j.arrival), reverse=True):
+        if free >= head_bytes:
+            break
+        if job.preemptions == 0:
+            chosen.append(job)
+            free += job.bytes
+    return chosen if free >= head_bytes else []
+
+
 class FDTDServer:
     """Continuous-batching server over one GPU. hbm_budget [B] bounds the resident device state; chunk_steps is the
-    granularity at which jobs join and leave."""
+    granularity at which jobs join and leave; preempt_after [s], if set, lets a queued job that has waited that long
+    preempt running ones; finite_check runs the device NaN / Inf check after every chunk."""
 
-    def __init__(self, hbm_budget=200e9, chunk_steps=100, graph_steps=0, autotune=True):
+    def __init__(self, hbm_budget=200e9, chunk_steps=100, graph_steps=0, autotune=True, preempt_after=None,
+                 finite_check=True):
         self.hbm_budget = float(hbm_budget)
         self.chunk_steps = int(chunk_steps)
         self.graph_steps = int(graph_steps)
         self.autotune = bool(autotune)
+        self.preempt_after = None if preempt_after is None else float(preempt_after)
+        self.finite_check = bool(finite_check)
         self._used = 0.0
+        self.preemptions = 0
         import torch
         self.torch = torch
 
@@ -133,14 +163,20 @@ class FDTDServer:
             if self._used + need > self.hbm_budget:
                 break
             pending.pop(0)
-            if self.autotune and job.sim.engine is None:   # batch-member launch shape of the job's size class
+            resumed = job.saved is not None
+            if self.autotune and job.sim.engine is None and not resumed:   # the launch shape of the size class
                 from .autotune import tune_batch_member
                 tune_batch_member(job.sim)
             job.sim.finalize()
             # source and phase tables for every step of the job now, so no table is replaced while a chunk runs
             job.sim.engine._ensure_capacity(job.steps)
+            if resumed:   # a preempted job: its state back into the new engine
+                from .solver import restore_solution
+                restore_solution(job.sim.engine, job.saved)
+                job.saved = None
             job.cells = int(np.prod(job.sim.model.shape))
-            job.admitted = now
+            if not resumed:
+                job.admitted = now
             job.bytes = need
             job.status = 'running'
             self._used += need
@@ -167,6 +203,71 @@ class FDTDServer:
         active.remove(job)
         done.append(job)
 
+    def _preemption(self, pending, active, start, clock):
+        """The running jobs to preempt for the head of the queue now ([] when none is due)."""
+        if self.preempt_after is None or not pending:
+            return []
+        head = pending[0]
+        now = clock() - start
+        if head.arrival > now or head.cancelled or head.preemptions or head.saved is not None \
+                or now - head.arrival < self.preempt_after:
+            return []
+        if head.sim is None:
+            head.sim = head.build()
+        need = state_bytes(head.sim)
+        if need > self.hbm_budget or self._used + need <= self.hbm_budget:
+            return []   # refused at admission, or it fits anyway
+        running = [j for j in active if j.done_steps < j.steps and j.status == 'running']
+        return choose_victims(need, self.hbm_budget - self._used, running)
+
+    def _preempt(self, job, pending, active):
+        """Move a running job's state to the host, free its device state and queue it behind the head job."""
+        from .solver import solution_state
+        job.saved = solution_state(job.sim.engine)
+        job.sim.close()
+        self._used -= job.bytes
+        active.remove(job)
+        job.preemptions += 1
+        job.status = 'pending'
+        self.preemptions += 1
+        pending.insert(1, job)
+
+    def _collect(self, chunk, active, done, start, clock, phases):
+        """Wait for a chunk, then read out its finished jobs, fail the jobs its finite check flagged and free the
+        state of every job that stops with it. Returns the flagged jobs that are still in a later chunk."""
+        t2 = clock()
+        for event in chunk['events']:
+            event.synchronize()
+        t3 = clock()
+        flagged = []
+        for members, steps, flags in chunk['checks']:
+            for job, step, bad in zip(members, steps, flags.cpu().numpy()):
+                if bad and job.status == 'running':
+                    job.status = 'failed'
+                    job.error = f'non-finite fields after step {step} (device check at a chunk boundary)'
+                    job.finished = clock() - start
+                    flagged.append(job)
+        for batch in chunk['batches']:
+            batch.close()
+        stopping = set(id(j) for j in chunk['stopping'])
+        for job in chunk['finishing']:
+            if job.status == 'running':
+                _finish(job, clock() - start)
+            if job in active:
+                self._release(job, active, done)
+        for job in chunk['stopping']:
+            if job in active:
+                job.finished = job.finished if job.finished is not None else clock() - start
+                self._release(job, active, done)
+        later = []
+        for job in flagged:   # not finishing in this chunk: stop stepping it; free it when no chunk holds it
+            if job in active and id(job) not in stopping:
+                job.done_steps = job.steps
+                later.append(job)
+        phases['wait'] += t3 - t2
+        phases['readout'] += clock() - t3
+        return later
+
     def run(self, jobs, clock=time.perf_counter, on_chunk=None):
         """Serve `jobs` (sorted by arrival); returns the stats dict. Arrivals are real time from the call. on_chunk, if
         given, is called with the list of running jobs at every chunk boundary (a caller may cancel jobs there).
@@ -178,21 +279,33 @@ class FDTDServer:
         job is never part of a later chunk. Consecutive chunks alternate between two stream pairs, and each chunk's
         streams wait on the previous chunk's events, which keeps every instance's steps in order."""
         from .batch import Batch2D, Batch3D
+        from . import _lib
         torch = self.torch
         if any(j.steps < 1 for j in jobs):
             raise ValueError('every job needs at least one step')
+        lib = _lib.load() if self.finite_check else None
         pending = sorted(jobs, key=lambda j: j.arrival)
         active, done, closed = [], [], []
         self._used = 0.0
+        self.preemptions = 0
         start = clock()
         cell_updates = 0
         streams = None              # two stream pairs (default priority), made when the first chunk runs
         last_event = [None, None]   # end of the previous chunk's 2D and 3D batches
-        inflight = None             # (events, batches, jobs finishing in that chunk, jobs cancelled before it)
+        inflight = None             # the chunk on the device: events, batches, finite checks, finishing / stopping jobs
         parity = 0
         phases = dict(admit=0.0, enqueue=0.0, wait=0.0, readout=0.0, chunks=0)
         while pending or active or inflight:
             t0 = clock()
+            victims = self._preemption(pending, active, start, clock)
+            if victims:   # settle every victim's state first: the chunk in flight may hold it
+                if inflight is not None:
+                    held = self._collect(inflight, active, done, start, clock, phases)
+                    for job in held:
+                        self._release(job, active, done)
+                    inflight = None
+                for job in self._preemption(pending, active, start, clock):
+                    self._preempt(job, pending, active)
             self._admit(pending, active, start, clock, closed)
             t1 = clock()
             if on_chunk is not None:
@@ -206,10 +319,9 @@ class FDTDServer:
             if runnable and streams is None:
                 streams = [(torch.cuda.Stream(), torch.cuda.Stream()) for _ in range(2)]
             if runnable or stopping:
-                current = ([], [], [], stopping)
+                current = dict(events=[], batches=[], checks=[], finishing=[], stopping=stopping, members=[])
             if runnable:
                 main = torch.cuda.current_stream()
-                events, batches, finishing = [], [], []
                 for kind_index, kind in enumerate((Batch2D, Batch3D)):
                     members = [j for j in runnable if j.three_d == kind.three_d]
                     if not members:
@@ -223,17 +335,27 @@ class FDTDServer:
                         if last_event[kind_index] is not None:
                             stream.wait_event(last_event[kind_index])
                         batch.launch()
+                        if lib is not None:   # NaN / Inf in each member's state after the chunk
+                            flags = torch.zeros(len(members), dtype=torch.int32, device=stream.device)
+                            handle = stream.cuda_stream
+                            for k, j in enumerate(members):
+                                engine = j.sim.engine
+                                state = engine.fields[engine.n_step & 1]
+                                _lib.check(lib.yee_flag_nonfinite(state.data_ptr(), state.numel(),
+                                                                  flags.data_ptr() + 4 * k, handle),
+                                           'yee_flag_nonfinite')
+                            current['checks'].append((members, [j.sim.engine.n_step for j in members], flags))
                         event = torch.cuda.Event()
                         event.record(stream)
                     last_event[kind_index] = event
-                    events.append(event)
-                    batches.append(batch)
+                    current['events'].append(event)
+                    current['batches'].append(batch)
+                    current['members'].extend(members)
                     for j, c in zip(members, counts):
                         j.done_steps += c
                         cell_updates += j.cells * c
                         if j.done_steps >= j.steps:
-                            finishing.append(j)
-                current = (events, batches, finishing, stopping)
+                            current['finishing'].append(j)
                 parity ^= 1
                 phases['chunks'] += 1
             elif current is None and inflight is None:
@@ -245,22 +367,13 @@ class FDTDServer:
                 continue
             t2 = clock()
             if inflight is not None:
-                events, batches, finishing, cancelled = inflight
-                for event in events:
-                    event.synchronize()
-                t3 = clock()
-                for batch in batches:
-                    batch.close()
-                for j in finishing:
-                    if j.status != 'cancelled':
-                        _finish(j, clock() - start)
-                    self._release(j, active, done)
-                for j in cancelled:
-                    if j in active:
-                        j.finished = clock() - start
-                        self._release(j, active, done)
-                phases['wait'] += t3 - t2
-                phases['readout'] += clock() - t3
+                held = self._collect(inflight, active, done, start, clock, phases)
+                in_current = set(id(j) for j in current['members']) if current is not None else set()
+                for job in held:
+                    if id(job) in in_current:
+                        current['stopping'].append(job)   # freed when the chunk just enqueued is collected
+                    else:
+                        self._release(job, active, done)
             inflight = current
             phases['admit'] += t1 - t0
             phases['enqueue'] += t2 - t1
@@ -268,6 +381,7 @@ class FDTDServer:
             torch.cuda.synchronize()
         stats = self._stats(done + closed, clock() - start, cell_updates)
         stats['host_phases_s'] = phases
+        stats['preemptions'] = self.preemptions
         return stats
 
     def run_plain(self, jobs, clock=time.perf_counter):
@@ -309,4 +423,4 @@ class FDTDServer:
                 'errors': {j.name: j.error for j in jobs if j.error}}
 
 
-__all__ = ['Job', 'FDTDServer', 'state_bytes']
+__all__ = ['Job', 'FDTDServer', 'choose_victims', 'state_bytes']

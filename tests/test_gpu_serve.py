@@ -104,3 +104,72 @@ def test_cancelled_job_stops_at_a_chunk_boundary(gpu):
         sim.close()
         for a, b in zip(solo, stats['results'][job.name]):
             assert np.array_equal(a, b), job.name
+
+
+def _big_job():
+    from opensol_amd import Simulation
+    dx = 25e-9
+    sim = Simulation(150 * dx, 120 * dx, dx, npml=8, polarization='ez')
+    sim.add_dielectric_rectangle(1.0e-6, 1.0e-6, 2.5e-6, 2.0e-6, eps_r=4.0)
+    sim.add_line_source(axis='y', position=0.6e-6, wavelength=0.6e-6, bandwidth=0.5)
+    sim.add_flux_monitor(axis='y', position=3.2e-6, span=(0.0, 120 * dx), wavelengths=[0.55e-6, 0.65e-6])
+    return sim
+
+
+def test_preempted_job_resumes_bit_for_bit(gpu):
+    """Two long jobs fill the budget; a larger job arrives and, with preempt_after=0, preempts the later-admitted one:
+    its state goes to the host and its device state is freed, the new job runs, and the preempted job is restored
+    into a new engine when memory frees and finishes. All three spectra equal their solo runs bit for bit."""
+    from opensol_amd.serve import FDTDServer, Job, state_bytes
+
+    def jobs():
+        return [Job(build=_job_2d(0), steps=3000, arrival=0.0, name='first'),
+                Job(build=_job_2d(1), steps=3000, arrival=0.005, name='second'),
+                Job(build=_big_job, steps=300, arrival=0.05, name='big')]
+
+    sizes = {job.name: state_bytes(job.build()) for job in jobs()}
+    assert sizes['big'] > sizes['second']
+    budget = sizes['first'] + sizes['big'] + 1.0   # both small jobs fit; the big one only without the second
+    stream = jobs()
+    stats = FDTDServer(hbm_budget=budget, chunk_steps=25, autotune=False, preempt_after=0.0).run(stream)
+    first, second, big = stream
+    print('preemptions', stats['preemptions'], 'admitted', [j.admitted for j in stream],
+          'finished', [j.finished for j in stream], flush=True)
+    assert stats['jobs'] == 3 and stats['preemptions'] == 1
+    assert second.preemptions == 1 and first.preemptions == 0 and big.preemptions == 0
+    assert big.finished < second.finished
+    for job in jobs():
+        sim = job.build()
+        sim.run(n_steps=job.steps, print_every=0)
+        solo = [np.asarray(m.flux()) for m in sim.model.monitors]
+        sim.close()
+        for a, b in zip(solo, stats['results'][job.name]):
+            assert np.array_equal(a, b), job.name
+
+
+def _nan_job():
+    """A job whose state holds a NaN before it is served (planted from the host into one E_z cell)."""
+    sim = _job_2d(2)()
+    sim.finalize()
+    values = sim.engine.field('ez')
+    values[values.shape[0] // 2, values.shape[1] // 2] = np.nan
+    sim.engine.set_field('ez', values)
+    return sim
+
+
+def test_diverging_job_fails_at_the_first_chunk_boundary(gpu):
+    """The device finite check after every chunk flags a job whose fields hold a NaN: it stops after the chunk that
+    showed it (status 'failed', the step in the error) instead of running to its end; the other job is unchanged."""
+    from opensol_amd.serve import FDTDServer, Job
+    good = Job(build=_job_2d(3), steps=200, arrival=0.0, name='good')
+    bad = Job(build=_nan_job, steps=200, arrival=0.0, name='bad')
+    stats = FDTDServer(hbm_budget=1e9, chunk_steps=25, autotune=False).run([good, bad])
+    print(stats['status'], stats['errors'], flush=True)
+    assert stats['status'] == {'good': 'done', 'bad': 'failed'}
+    assert 'after step 25 (device check' in stats['errors']['bad']
+    assert bad.sim is None and stats['jobs'] == 1
+    sim = good.build()
+    sim.run(n_steps=good.steps, print_every=0)
+    for a, b in zip([np.asarray(m.flux()) for m in sim.model.monitors], stats['results']['good']):
+        assert np.array_equal(a, b)
+    sim.close()

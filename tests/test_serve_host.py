@@ -1,28 +1,26 @@
-"""Serving on the host side (no GPU needed): a job whose device state alone exceeds the HBM budget is refused at
-arrival with the reason, and a job cancelled before it is admitted is dropped; neither touches the device."""
-import numpy as np
+"""Host logic of the server that needs no GPU: which running jobs a queued job preempts."""
+from types import SimpleNamespace
 
-from opensol_amd.serve import FDTDServer, Job, state_bytes
-
-
-def _job(k, cancel=False):
-    def build():
-        from opensol_amd import Simulation
-        sim = Simulation(2e-6, 1e-6, 25e-9, npml=8)
-        sim.add_line_source(axis='y', position=0.5e-6, wavelength=0.6e-6)
-        sim.add_flux_monitor(axis='y', position=1.5e-6, span=(0.0, 1e-6), wavelengths=[0.6e-6])
-        return sim
-    job = Job(build=build, steps=100, arrival=0.0, name=f'job{k}')
-    if cancel:
-        job.cancel()
-    return job
+from opensol_amd.serve import choose_victims
 
 
-def test_over_budget_job_is_rejected_and_cancelled_job_dropped():
-    need = state_bytes(_job(0).build())
-    jobs = [_job(0), _job(1, cancel=True)]
-    stats = FDTDServer(hbm_budget=need / 2).run(jobs)
-    assert stats['jobs'] == 0 and stats['cell_updates'] == 0
-    assert stats['status'] == {'job0': 'rejected', 'job1': 'cancelled'}
-    assert 'exceeds the HBM budget' in stats['errors']['job0']
-    assert jobs[0].sim is None and np.isfinite(jobs[0].latency)
+def _job(name, admitted, size, preemptions=0, arrival=0.0):
+    return SimpleNamespace(name=name, admitted=admitted, bytes=size, preemptions=preemptions, arrival=arrival)
+
+
+def test_latest_admitted_jobs_are_preempted_first_until_the_job_fits():
+    running = [_job('a', 0.0, 40), _job('b', 1.0, 30), _job('c', 2.0, 20)]
+    assert [j.name for j in choose_victims(45, 10, running)] == ['c', 'b']   # 10 + 20 < 45 <= 10 + 20 + 30
+    assert [j.name for j in choose_victims(30, 10, running)] == ['c']
+    assert choose_victims(5, 10, running) == []                               # fits already
+
+
+def test_a_job_is_preempted_at_most_once_and_partial_sets_are_not_taken():
+    running = [_job('a', 0.0, 40), _job('b', 1.0, 30, preemptions=1), _job('c', 2.0, 20)]
+    assert [j.name for j in choose_victims(60, 5, running)] == ['c', 'a']   # b was preempted before
+    assert choose_victims(100, 5, running) == []                            # even all would not make room
+
+
+def test_jobs_admitted_together_give_way_in_reverse_arrival_order():
+    running = [_job('early', 1.0, 30, arrival=0.0), _job('late', 1.0, 30, arrival=0.5)]
+    assert [j.name for j in choose_victims(35, 10, running)] == ['late']
