@@ -408,6 +408,8 @@ class _Built:
         return result
 
     def _dft_result(self, monitor):
+        """'dft_fields' of a DFT box (physics/em/results.py:125-150) or 3D DFT plane (physics/em3d/monitors3d.py:
+        273-295), with 'intensity_per_cw_factor' = 1 / |S(omega)|^2 when every source shares one waveform."""
         frequency = np.array(monitor.frequencies, dtype=np.float64)
         centres = self.sim.domain.cell_centers()
         edges = self.sim.domain.cell_edges()
@@ -416,22 +418,30 @@ class _Built:
             intensity = np.stack([monitor.intensity(i) for i in range(monitor.n_freqs)])
             extent = (float(edges[0][xs.start]), float(edges[0][xs.stop]), float(edges[1][ys.start]),
                       float(edges[1][ys.stop]))
-            return {'kind': 'dft_fields', 'frequency': frequency, 'wavelength': C0 / frequency,
-                    'x_centers': np.array(centres[0][xs]), 'y_centers': np.array(centres[1][ys]), 'extent': extent,
-                    'shape': tuple(monitor.shape), 'quantity': 'E intensity', 'z': float(centres[2][monitor.k_grid]),
-                    'intensity_all': intensity}
-        if monitor.polarization == 'ez':
-            intensity = np.abs(monitor.ez_dft) ** 2
+            result = {'kind': 'dft_fields', 'frequency': frequency, 'wavelength': C0 / frequency,
+                      'x_centers': np.array(centres[0][xs]), 'y_centers': np.array(centres[1][ys]), 'extent': extent,
+                      'shape': tuple(monitor.shape), 'quantity': 'abs(E_x)^2 + abs(E_y)^2 + abs(E_z)^2',
+                      'z': float(centres[2][monitor.k_grid]), 'intensity_all': intensity}
         else:
-            intensity = np.abs(monitor.ex_dft) ** 2 + np.abs(monitor.ey_dft) ** 2
-        xs, ys = monitor.x_slice, monitor.y_slice
-        extent = (float(edges[0][xs.start]), float(edges[0][xs.stop]), float(edges[1][ys.start]),
-                  float(edges[1][ys.stop]))
-        return {'kind': 'dft_fields', 'frequency': frequency, 'wavelength': C0 / frequency,
-                'x_centers': np.array(centres[0][monitor.x_samples]), 'y_centers': np.array(centres[1][monitor.y_samples]),
-                'extent': extent, 'shape': (int(monitor.mx), int(monitor.my)), 'downsample': int(monitor.downsample),
-                'quantity': 'E_z intensity' if monitor.polarization == 'ez' else 'in-plane E intensity',
-                'intensity_all': intensity}
+            if monitor.polarization == 'ez':
+                intensity = np.abs(monitor.ez_dft) ** 2
+            else:
+                intensity = np.abs(monitor.ex_dft) ** 2 + np.abs(monitor.ey_dft) ** 2
+            xs, ys = monitor.x_slice, monitor.y_slice
+            extent = (float(edges[0][xs.start]), float(edges[0][xs.stop]), float(edges[1][ys.start]),
+                      float(edges[1][ys.stop]))
+            result = {'kind': 'dft_fields', 'frequency': frequency, 'wavelength': C0 / frequency,
+                      'x_centers': np.array(centres[0][monitor.x_samples]),
+                      'y_centers': np.array(centres[1][monitor.y_samples]), 'extent': extent,
+                      'shape': (int(monitor.mx), int(monitor.my)), 'downsample': int(monitor.downsample),
+                      'quantity': 'abs(E_z)^2' if monitor.polarization == 'ez' else 'abs(E_x)^2 + abs(E_y)^2',
+                      'intensity_all': intensity}
+        shared = self._source_spectrum(frequency, monitor.sample_interval)
+        if shared is not None:
+            values, valid = shared
+            result['intensity_per_cw_factor'] = np.where(valid, 1.0 / np.where(valid, np.abs(values) ** 2, 1.0),
+                                                         np.nan)
+        return result
 
     def fields(self):
         """{'<physics>.<component>': (window values, x edges, y edges)} of the final state (3D: full arrays)."""
@@ -478,6 +488,19 @@ STORED_FIELDS = {
            *_MODULI, _field_info('eps_xx', 'Relative permittivity xx', '1', 'YlOrBr'),
            _field_info('eps_yy', 'Relative permittivity yy', '1', 'YlOrBr'), _SIGMA, *_MEDIA],
 }
+
+
+# em_fdtd_3d (physics/em3d/renderer3d.py:14-16, 79-92): the raw Yee components, the moduli at the cell centres, the
+# E_z permittivity array and the absorbed power, stored primary first.
+_RAW_3D = (('ez', 'E_z (+z face)', ''), ('ex', 'E_x (+x face)', 'x'), ('ey', 'E_y (+y face)', 'y'),
+           ('hx', 'H_x (+y +z edge)', 'y'), ('hy', 'H_y (+x +z edge)', 'x'), ('hz', 'H_z (+x +y edge)', 'xy'))
+STORED_FIELDS['3d'] = [
+    *(_field_info(name, label, 'V/m' if name[0] == 'e' else 'A/m', 'RdBu_r', signed=True, primary=name == 'ez',
+                  parity=f"{'polar' if name[0] == 'e' else 'axial'}_{name[1]}", staggered=staggered)
+      for name, label, staggered in _RAW_3D),
+    *_MODULI, _field_info('eps_r', 'Relative permittivity (eps_zz)', '1', 'YlOrBr'), _MEDIA[0]]
+# grid axes a raw 3D component is averaged along to reach the cell centre (E: its own axis; H: its edge's two axes)
+CENTRE_AXES_3D = {'ex': (0,), 'ey': (1,), 'ez': (2,), 'hx': (1, 2), 'hy': (0, 2), 'hz': (0, 1)}
 
 
 def _next_output_time(t_now, t_start, interval, dt):
@@ -598,9 +621,88 @@ class _Recorder:
     def time_label(self):
         return f't = {float(self.sim.time) * 1e15:.1f} fs'
 
+    plane = None
+
     def layout(self):
-        """What every snapshot of this run shares: infos, extent, centres and edges."""
-        return {'infos': self.infos, 'extent': self.extent, 'centres': self.centres, 'edges': self.edges}
+        """What every snapshot of this run shares: infos, extent, centres, edges and the cut plane (3D)."""
+        return {'infos': self.infos, 'extent': self.extent, 'centres': self.centres, 'edges': self.edges,
+                'plane': self.plane}
+
+
+class _Recorder3D(_Recorder):
+    """Stored-field snapshots of a 3D run: the mid-height z cut of each stored field (model/field_access_tensor.py:
+    186-204: the window cell containing size_z / 2), float32 over the window's x and y cells. Only the two z layers the
+    cut needs are copied from the device."""
+
+    def __init__(self, built, requested=()):
+        sim = built.sim
+        self.sim = sim
+        self.physics = built.physics_name
+        domain = sim.model.domain
+        infos = {f'{self.physics}.{info["name"]}': info for info in STORED_FIELDS['3d']}
+        for key in requested:
+            if key not in infos:
+                raise ValueError(f"stored field '{key}' is not an output field of this model; available: "
+                                 f"{sorted(infos)}")
+        self.keys = list(requested) or list(infos)
+        self.infos = {key: infos[key] for key in self.keys}
+        self.window = tuple(domain.physical_window())
+        self.k = domain.pos_to_cell_axis(2, 0.5 * domain.size_z)   # grid index of the cut
+        edges, centres = domain.cell_edges(), domain.cell_centers()
+        self.plane = ['z', float(centres[2][self.k])]
+        self.edges = [np.asarray(e[w.start:w.stop + 1], dtype=np.float64) for e, w in zip(edges[:2], self.window)]
+        self.centres = [np.asarray(c[w], dtype=np.float64) for c, w in zip(centres[:2], self.window)]
+        self.extent = [float(self.edges[0][0]), float(self.edges[0][-1]), float(self.edges[1][0]),
+                       float(self.edges[1][-1])]
+        self.snapshots = []
+        self.stored_bytes = 0
+        self.full = False
+
+    def _centre(self, layers, name):
+        """float32 window cut of a raw component averaged to the cell centres (physics/em3d/renderer3d.py:115-147):
+        the window plus the lower row along each averaged axis (cell 0: the wrapped last row on a periodic axis, else
+        its own), one pair sum per averaged axis in axis order, then the product with 1 / 2^axes."""
+        model = self.sim.model
+        axes = CENTRE_AXES_3D[name]
+        index = []
+        for axis, cells in enumerate(self.window[:2]):
+            rows = list(range(cells.start, cells.stop))
+            if axis in axes:
+                lower = cells.start - 1 if cells.start > 0 else (model.shape[axis] - 1 if model.periodic[axis] else 0)
+                rows = [lower] + rows
+            index.append(np.asarray(rows))
+        component = layers[name]   # (nx, ny, 2): z layers (lower, cut)
+        values = component[np.ix_(index[0], index[1], [0, 1] if 2 in axes else [1])].astype(np.float32)
+        for axis in axes:
+            upper, lower = [slice(None)] * 3, [slice(None)] * 3
+            upper[axis], lower[axis] = slice(1, None), slice(None, -1)
+            values = values[tuple(upper)] + values[tuple(lower)]
+        values = values * np.float32(1.0 / 2 ** len(axes))
+        return values[:, :, 0]
+
+    def _values(self):
+        """{component: float32 (wx, wy) cut} of the current state (physics/em3d/renderer3d.py:96-113)."""
+        from .fdtd3d import COMPONENTS
+        sim, model = self.sim, self.sim.model
+        engine = sim.engine
+        k = self.k
+        k_lower = k - 1 if k > 0 else (model.shape[2] - 1 if model.periodic[2] else k)
+        state = engine.fields[engine.n_step & 1]
+        cut = state[:, :, :, [k_lower, k]].cpu().numpy()   # (6, nx, ny, 2)
+        layers = {name: cut[c] for c, name in enumerate(COMPONENTS)}
+        wx, wy = self.window[0], self.window[1]
+        out = {name: layers[name][wx, wy, 1] for name in COMPONENTS}
+        e2 = sum(self._centre(layers, name) ** 2 for name in ('ex', 'ey', 'ez'))
+        h2 = sum(self._centre(layers, name) ** 2 for name in ('hx', 'hy', 'hz'))
+        out['e_intensity'] = e2
+        out['e_abs'] = np.sqrt(e2)
+        out['h_abs'] = np.sqrt(h2)
+        materials = model.materials   # the E_z permittivity of the cut's cells only (no full-grid array)
+        eps_z = np.array([m[2] for m in materials.materials], dtype=np.float32)
+        out['eps_r'] = eps_z[materials.ids[wx, wy, k]]
+        # no absorption tracking on this path: zeros, as the reference's without it (renderer3d.py:104-113)
+        out['absorbed_power'] = np.zeros((wx.stop - wx.start, wy.stop - wy.start), dtype=np.float32)
+        return {name: np.ascontiguousarray(values, dtype=np.float32) for name, values in out.items()}
 
 
 def _snapshot_bytes(snapshots):
@@ -685,8 +787,8 @@ def _snapshot_entry(snapshot, layout, prefix, arrays):
         arrays[f'{prefix}{key}/x_centers'] = layout['centres'][0]
         arrays[f'{prefix}{key}/y_centers'] = layout['centres'][1]
         fields[key] = {'extent': layout['extent'], 'info': layout['infos'][key], 'time_label': snapshot['time_label'],
-                       'native_shape': None, 'plane': None, 'axes': ['x', 'y'], 'unfolded': False, 'mesh': 'mesh',
-                       'triangulation': None}
+                       'native_shape': None, 'plane': layout['plane'], 'axes': ['x', 'y'], 'unfolded': False,
+                       'mesh': 'mesh', 'triangulation': None}
     return {'time': snapshot['time'], 'label': snapshot['label'], 'fields': fields, 'cuts': [], 'segments': {}}
 
 
@@ -735,7 +837,8 @@ class Results:
             physics, component = key.split('.', 1)
             metadata['fields'][key] = {'physics': physics, 'component': component,
                                        'info': self.layout['infos'][key] if self.layout else None, 'time': t_end,
-                                       'time_label': f't = {t_end * 1e15:.1f} fs', 'plane': None, 'mesh': 'mesh',
+                                       'time_label': f't = {t_end * 1e15:.1f} fs',
+                                       'plane': self.layout['plane'] if self.layout else None, 'mesh': 'mesh',
                                        'triangulation': None}
         for key, result in self.results.items():
             metadata['results'][key] = _split(result, f'result/{key}/', arrays)
@@ -761,19 +864,15 @@ class Results:
 
 def _advance(built, settings, names, requested):
     """One time-dependent study of a build (model/study.py:916-967): the snapshot ticks of snapshot_schedule, with a
-    record at each and the final state last (2D); a 3D build runs the duration in one go and stores no snapshots.
-    Returns the recorder (None in 3D)."""
+    record at each and the final state last (3D: the mid-height z cut). Returns the recorder."""
     sim = built.sim
     duration = evaluate(settings.get('duration', 200e-15), names)
-    if built.three_d:
-        sim.run(duration=duration, print_every=0)
-        return None
     interval = evaluate(settings.get('output_interval', 0.0), names)
     count = int(evaluate(settings.get('n_snapshots', 20), names))
     if not float(interval) >= 0.0 or count < 1:
         raise ValueError(f'output_interval must be >= 0 and n_snapshots >= 1, got {interval!r}, {count!r}')
     sim.finalize()
-    recorder = _Recorder(built, requested)
+    recorder = (_Recorder3D if built.three_d else _Recorder)(built, requested)
     schedule = snapshot_schedule(sim.model.dt, duration, interval, count)
     for index, (tick, _) in enumerate(schedule):
         if tick > sim.n_step:

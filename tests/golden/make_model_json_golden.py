@@ -6,8 +6,9 @@ selection, feature placement, the study duration and sweep, and the results.npz 
 
 Saved per case: the model JSON text (input), each swept run's flux wavelengths, power and power_per_cw per monitor
 feature, the results.npz array names and its metadata JSON text (the file layout), the summary's sweep record, tick
-count and end time, and (E_z, H_z) a sample of the results.npz values: the stored fields of one snapshot and of the
-final field group, their cell centres and edges, and the empty run's primary field at that snapshot.
+count and end time, and a sample of the results.npz values: the stored fields of one snapshot and of the final
+field group (3D: their mid-height z cuts), their cell centres and edges, and (E_z, H_z) the empty run's primary field
+at that snapshot.
 
 Run in this container only (the reference is not present on the GPU box):
     python tests/golden/make_model_json_golden.py [3d] [ez] [hz] [drude]   (no argument: every case)
@@ -30,6 +31,9 @@ MONITORS = ('reflection_side_flux', 'transmitted_flux')
 KEPT_SNAPSHOT = 20
 KEPT_FIELDS = {'ez': ('ez', 'hx', 'hy', 'e_abs', 'e_intensity', 'h_abs', 'eps_r', 'sigma_e'),
                'hz': ('hz', 'ex', 'ey', 'e_abs', 'e_intensity', 'h_abs', 'eps_xx', 'eps_yy', 'sigma_e')}
+# 3D: the stored fields' mid-height z cut in snapshot KEPT_SNAPSHOT_3D and in the final field group
+KEPT_SNAPSHOT_3D = 2
+KEPT_FIELDS_3D = ('ex', 'ey', 'ez', 'hx', 'hy', 'hz', 'e_abs', 'e_intensity', 'h_abs', 'eps_r', 'absorbed_power')
 
 
 # A small 3D model of the em_fdtd_3d vocabulary (no example of the reference uses only features of this path): a
@@ -88,7 +92,8 @@ MODEL_3D = {
 
 
 def run_3d():
-    """The 3D model through the reference's api.run: flux, per-cw power, DFT layer intensity and the probe series."""
+    """The 3D model through the reference's api.run: flux, per-cw power, DFT layer intensity, the probe series and the
+    results.npz layout with a sample of its snapshot and field values (mid-height z cuts)."""
     sys.path.insert(0, REFERENCE)
     from open_sol import api
 
@@ -98,9 +103,24 @@ def run_3d():
         path = os.path.join(directory, 'model.json')
         with open(path, 'w', encoding='utf-8') as handle:
             handle.write(text)
-        results = api.run(api.load(path), backend='cpu')
+        results = api.run(api.load(path), out_dir=os.path.join(directory, 'out'), backend='cpu')
         if results.status != 'completed':
             raise RuntimeError(f'reference run ended {results.status}')
+        with np.load(os.path.join(directory, 'out', 'results.npz'), allow_pickle=False) as npz:
+            out['npz_names'] = np.array(sorted(npz.files))
+            out['npz_metadata'] = np.array(str(npz['metadata']))
+            for name in KEPT_FIELDS_3D:   # the mid-height cut of snapshot KEPT_SNAPSHOT_3D and of the final fields
+                for group in (f'snapshot/{KEPT_SNAPSHOT_3D}', 'field'):
+                    out[f'npz/{group}/em.{name}/values'] = npz[f'{group}/em.{name}/values']
+            out['npz/result/em.layer/intensity_per_cw_factor'] = npz['result/em.layer/intensity_per_cw_factor']
+            for axis in 'xy':
+                out[f'npz/snapshot/{KEPT_SNAPSHOT_3D}/{axis}_centers'] = npz[
+                    f'snapshot/{KEPT_SNAPSHOT_3D}/em.ez/{axis}_centers']
+                out[f'npz/field/{axis}_edges'] = npz[f'field/em.ez/{axis}_edges']
+        with open(os.path.join(directory, 'out', 'summary.json'), encoding='utf-8') as handle:
+            summary = json.load(handle)
+        out['summary_snapshot_times'] = np.array(summary['snapshot_times'], dtype=np.float64)
+        out['summary_t_end'] = np.float64(summary['t_end'])
         for name in ('flux_x', 'flux_z', 'box'):
             result = results.result('em', name)
             for key in ('wavelength', 'power', 'power_per_cw'):

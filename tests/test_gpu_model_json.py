@@ -91,13 +91,42 @@ def test_thin_film_model_matches_the_reference_run(gpu, polarization, tmp_path):
             assert error <= 1e-5, (key, error)
 
 
-def test_3d_model_matches_the_reference_run(gpu):
+def test_3d_model_matches_the_reference_run(gpu, tmp_path):
     """A 3D model (dipole over a glass block, a PEC box, two flux planes, a flux box, a DFT layer, a probe) against the
     reference's api.run of the same file (float32 CPU path): flux and per-cw power within 1e-4, the layer intensity
-    within 1e-4 relative L2 per wavelength, the probe within 1e-4 of its peak."""
+    within 1e-4 relative L2 per wavelength, the probe within 1e-4 of its peak; results.npz with every array name and
+    the field / snapshot / result metadata of the reference's, the snapshot times bit for bit, the layer's per-cw factor
+    within 1e-4, and the mid-height z cuts of snapshot 2 and of the final state within 1e-5 of the running peak of
+    the field's vector (raw components) or of the field itself (moduli); materials exact."""
     from opensol_amd import model as M
     golden = np.load(os.path.join(FIXTURES, 'model_dipole_3d.npz'))
-    results = M.run(json.loads(str(golden['model_json'])))
+    results = M.run(json.loads(str(golden['model_json'])), out_dir=tmp_path)
+    assert results.summary()['snapshot_times'] == list(golden['summary_snapshot_times'])
+    want_meta = json.loads(str(golden['npz_metadata']))
+    with np.load(tmp_path / 'results.npz', allow_pickle=False) as npz:
+        assert set(npz.files) == {str(n) for n in golden['npz_names']}
+        meta = json.loads(str(npz['metadata']))
+        for group in ('fields', 'snapshots', 'sweep_snapshots', 'results'):
+            assert meta[group] == want_meta[group], group
+        factor = npz['result/em.layer/intensity_per_cw_factor']
+        want = golden['npz/result/em.layer/intensity_per_cw_factor']
+        np.testing.assert_allclose(factor, want, rtol=1e-4)
+        snap, last = 2, len(results.snapshots) - 1
+        # a raw component is scaled by its field vector's running peak (H_z of a z dipole is near zero by symmetry
+        # on the cut); a modulus by its own
+        vectors = {c: tuple(c[0] + a for a in 'xyz') for c in ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')}
+        for name in ('ex', 'ey', 'ez', 'hx', 'hy', 'hz', 'e_abs', 'e_intensity', 'h_abs', 'eps_r', 'absorbed_power'):
+            for key, upto in ((f'snapshot/{snap}/em.{name}/values', snap), (f'field/em.{name}/values', last)):
+                ours, want = npz[key], golden[f'npz/{key}']
+                assert ours.dtype == np.float32 and ours.shape == want.shape, key
+                peak = max(float(np.max(np.abs(s['fields'][f'em.{c}']))) for c in vectors.get(name, (name,))
+                           for s in results.snapshots[:upto + 1])
+                if name in ('eps_r', 'absorbed_power') or peak == 0.0:
+                    np.testing.assert_array_equal(ours, want, err_msg=key)
+                    continue
+                error = float(np.max(np.abs(ours.astype(np.float64) - want))) / peak
+                print(f'3D {key}: max err / running peak {error:.2e}')
+                assert error <= 1e-5, (key, error)
     assert results.summary()['ticks'] == int(golden['summary_ticks'])
     for name in ('flux_x', 'flux_z', 'box'):
         result = results.result('em', name)

@@ -179,3 +179,104 @@ def test_snapshot_byte_limit_keeps_the_final_state(monkeypatch):
     kept.take(recorder)
     kept.take(recorder)
     assert [len(s) for _, s in kept.kept] == [1, 1] and math.isclose(kept.stored_bytes, 80)
+
+
+# ---- 3D: the mid-height z cut ----------------------------------------------------------------------------------
+
+def _golden_3d():
+    return np.load(os.path.join(FIXTURES, 'model_dipole_3d.npz'))
+
+
+def test_3d_schedule_and_field_table_match_the_reference():
+    """The 3D model's snapshot times (output_interval 5 fs over 15 fs) and the stored-field records, primary first."""
+    from opensol_amd import model as M
+    golden = _golden_3d()
+    spec = json.loads(str(golden['model_json']))
+    built, names = _dt(spec)
+    settings = spec['study']['settings']
+    schedule = M.snapshot_schedule(built.sim.model.dt, M.evaluate(settings['duration'], names),
+                                   M.evaluate(settings['output_interval'], names))
+    assert [t for _, t in schedule] == list(golden['summary_snapshot_times'])
+    assert schedule[-1][0] == int(golden['summary_ticks'])
+    metadata = json.loads(str(golden['npz_metadata']))
+    stored = metadata['snapshots'][0]['fields']
+    assert list(stored) == [f'em.{info["name"]}' for info in M.STORED_FIELDS['3d']]
+    for info in M.STORED_FIELDS['3d']:
+        assert stored[f'em.{info["name"]}']['info'] == info
+
+
+def test_3d_saved_layout_matches_the_reference(tmp_path):
+    """A 3D recorder built without an engine: the cut plane (the window cell holding mid-height, by its centre), the
+    x / y edges and centres, and Results.save()'s snapshot and field names and metadata equal the reference's."""
+    from opensol_amd import model as M
+    golden = _golden_3d()
+    metadata = json.loads(str(golden['npz_metadata']))
+    built, _ = _dt(json.loads(str(golden['model_json'])))
+    recorder = M._Recorder3D(built)
+    assert recorder.plane == metadata['fields']['em.ez']['plane']
+    np.testing.assert_array_equal(recorder.edges[0], golden['npz/field/x_edges'])
+    np.testing.assert_array_equal(recorder.edges[1], golden['npz/field/y_edges'])
+    np.testing.assert_array_equal(recorder.centres[0], golden['npz/snapshot/2/x_centers'])
+    np.testing.assert_array_equal(recorder.centres[1], golden['npz/snapshot/2/y_centers'])
+    shape = (recorder.edges[0].size - 1, recorder.edges[1].size - 1)
+    for snapshot in metadata['snapshots']:
+        t = snapshot['time']
+        recorder.snapshots.append({'time': t, 'label': None, 'time_label': f't = {t * 1e15:.1f} fs',
+                                   'fields': {k: np.zeros(shape, np.float32) for k in recorder.keys}})
+    fields = {key: (values, recorder.edges[0], recorder.edges[1])
+              for key, values in recorder.snapshots[-1]['fields'].items()}
+    out = M.Results('completed', {}, {}, [{}], None, fields, {'t_end': recorder.snapshots[-1]['time']},
+                    recorder.snapshots, recorder.layout())
+    out.save(tmp_path)
+    groups = ('field', 'snapshot', 'sweep_snapshot')
+    with np.load(tmp_path / 'results.npz', allow_pickle=False) as npz:
+        ours = {n for n in npz.files if n.split('/')[0] in groups}
+        saved = json.loads(str(npz['metadata']))
+    assert ours == {str(n) for n in golden['npz_names'] if str(n).split('/')[0] in groups}
+    for group in ('fields', 'snapshots', 'sweep_snapshots'):
+        assert saved[group] == metadata[group], group
+
+
+class _CutModel:
+    def __init__(self, shape, periodic):
+        self.shape, self.periodic = shape, periodic
+
+
+@pytest.mark.parametrize('periodic', [(False, False, False), (True, True, False)])
+@pytest.mark.parametrize('window_start', [0, 2])
+@pytest.mark.parametrize('k', [0, 3])
+def test_3d_centre_average_against_a_loop(periodic, window_start, k):
+    """_Recorder3D._centre against a per-cell loop of the reference's rule: pair sums along each averaged axis in
+    axis order (the lower neighbour of window cell 0 wraps on a periodic axis, else is the cell itself), times
+    1 / 2^axes, in float32."""
+    from opensol_amd import model as M
+    shape = (7, 6, 5)
+    rng = np.random.default_rng(11 + window_start + k)
+    volume = {name: rng.standard_normal(shape).astype(np.float32) for name in M.CENTRE_AXES_3D}
+    recorder = M._Recorder3D.__new__(M._Recorder3D)
+    recorder.sim = type('S', (), {'model': _CutModel(shape, periodic)})()
+    recorder.window = (slice(window_start, shape[0] - 1), slice(window_start, shape[1]), slice(0, shape[2]))
+    k_lower = k - 1 if k > 0 else (shape[2] - 1 if periodic[2] else k)
+    layers = {name: v[:, :, [k_lower, k]] for name, v in volume.items()}
+
+    def lower(axis, index):   # grid row 0 pairs with the last row on a periodic axis, else with itself
+        return index - 1 if index > 0 else (shape[axis] - 1 if periodic[axis] else index)
+
+    for name, axes in M.CENTRE_AXES_3D.items():
+        got = recorder._centre(layers, name)
+        want = np.empty_like(got)
+        for a, i in enumerate(range(recorder.window[0].start, recorder.window[0].stop)):
+            for b, j in enumerate(range(recorder.window[1].start, recorder.window[1].stop)):
+                def value(p):
+                    return volume[name][p]
+
+                def summed(p, remaining):   # pair sums along the axes, the first axis innermost
+                    if not remaining:
+                        return value(p)
+                    axis = remaining[-1]
+                    q = list(p)
+                    q[axis] = lower(axis, p[axis])
+                    return np.float32(summed(p, remaining[:-1]) + summed(tuple(q), remaining[:-1]))
+
+                want[a, b] = np.float32(summed((i, j, k), list(axes)) * np.float32(1.0 / 2 ** len(axes)))
+        np.testing.assert_array_equal(got, want, err_msg=name)
