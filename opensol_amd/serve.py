@@ -240,6 +240,7 @@ class FDTDServer:
             event.synchronize()
         t3 = clock()
         flagged = []
+        t_flags = clock()
         for members, steps, flags in chunk['checks']:
             for job, step, bad in zip(members, steps, flags.cpu().numpy()):
                 if bad and job.status == 'running':
@@ -247,14 +248,23 @@ class FDTDServer:
                     job.error = f'non-finite fields after step {step} (device check at a chunk boundary)'
                     job.finished = clock() - start
                     flagged.append(job)
+        t_close = clock()
         for batch in chunk['batches']:
             batch.close()
+        t_finish = clock()
         stopping = set(id(j) for j in chunk['stopping'])
         for job in chunk['finishing']:
             if job.status == 'running':
                 _finish(job, clock() - start)
+        t_release = clock()
+        for job in chunk['finishing']:
             if job in active:
                 self._release(job, active, done)
+        t_end = clock()
+        phases['readout_flags'] = phases.get('readout_flags', 0.0) + t_close - t_flags
+        phases['readout_close'] = phases.get('readout_close', 0.0) + t_finish - t_close
+        phases['readout_spectra'] = phases.get('readout_spectra', 0.0) + t_release - t_finish
+        phases['readout_release'] = phases.get('readout_release', 0.0) + t_end - t_release
         for job in chunk['stopping']:
             if job in active:
                 job.finished = job.finished if job.finished is not None else clock() - start
