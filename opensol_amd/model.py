@@ -443,18 +443,6 @@ class _Built:
                                                          np.nan)
         return result
 
-    def fields(self):
-        """{'<physics>.<component>': (window values, x edges, y edges)} of the final state (3D: full arrays)."""
-        sim = self.sim
-        window = sim.domain.physical_window()
-        edges = sim.domain.cell_edges()
-        out = {}
-        for name in sim.model.components if not self.three_d else ('ex', 'ey', 'ez', 'hx', 'hy', 'hz'):
-            values = sim.engine.field(name)[tuple(window)]
-            out[f'{self.physics_name}.{name}'] = (values, edges[0][window[0].start:window[0].stop + 1],
-                                                  edges[1][window[1].start:window[1].stop + 1])
-        return out
-
 
 # ---- stored fields and snapshots (model/study.py:916-967, 2041-2130, 2623-2645; physics/em/renderer.py:12-312) -----
 _SNAPSHOT_PIXELS = 250_000        # window cells per stored field (model/study.py:44)
@@ -926,6 +914,8 @@ def run(spec, out_dir=None, parameters=None, progress=None):
         if sweep.get('parameter') not in base:
             raise ValueError(f"sweep parameter {sweep.get('parameter')!r} is not a model parameter")
         values = [evaluate(v, base) for v in sweep.get('values') or []]
+        if not values:
+            raise ValueError(f"sweep of {sweep['parameter']!r} has no values")
         runs = [dict(overrides, **{sweep['parameter']: v}) for v in values]
         kept = _KeptSweepSnapshots() if sweep.get('keep_fields', True) else None
         sweep = {'parameter': sweep['parameter'], 'values': values}
@@ -938,17 +928,17 @@ def run(spec, out_dir=None, parameters=None, progress=None):
         built = _Built(spec, names)
         recorder = _advance(built, settings, names, requested)
         sweep_results.append(built.results())
-        if kept is not None and recorder is not None:
+        if kept is not None:
             kept.take(recorder)
         if progress is not None:
             progress({'run': index, 'of': len(runs), 'steps': built.sim.engine.n_step})
     results = sweep_results[-1]
-    snapshots = recorder.snapshots if recorder is not None else []
+    snapshots = recorder.snapshots
     summary = {'status': 'completed', 'message': '', 'parameters': names,
                'results': {key: result_digest(result) for key, result in results.items()},
                't_end': float(built.sim.time), 'ticks': int(built.sim.engine.n_step), 'tick_unit': 'step',
                'snapshot_times': [s['time'] for s in snapshots], 'snapshot_labels': [s['label'] for s in snapshots],
-               'stored_fields': list(recorder.keys) if recorder is not None else [],
+               'stored_fields': list(recorder.keys),
                'wall_time': time.perf_counter() - wall}
     if sweep:
         summary['sweep'] = dict(sweep, derived={}, reference={}, error={}, curve_error={},
@@ -957,13 +947,10 @@ def run(spec, out_dir=None, parameters=None, progress=None):
         if kept is not None and kept.kept:
             summary['sweep']['snapshot_times'] = [[s['time'] for s in snaps] for _, snaps in kept.kept]
             summary['sweep']['snapshot_labels'] = [[s['label'] for s in snaps] for _, snaps in kept.kept]
-    if recorder is not None:
-        final = recorder.snapshots[-1]['fields']
-        fields = {key: (values, recorder.edges[0], recorder.edges[1]) for key, values in final.items()}
-        out = Results('completed', spec, results, sweep_results, sweep, fields, summary, snapshots, recorder.layout(),
-                      kept.kept if kept is not None else ())
-    else:
-        out = Results('completed', spec, results, sweep_results, sweep, built.fields(), summary)
+    final = recorder.snapshots[-1]['fields']   # the field group: the stored fields of the final state
+    fields = {key: (values, recorder.edges[0], recorder.edges[1]) for key, values in final.items()}
+    out = Results('completed', spec, results, sweep_results, sweep, fields, summary, snapshots, recorder.layout(),
+                  kept.kept if kept is not None else ())
     built.sim.close()
     if out_dir is not None:
         out.save(out_dir)

@@ -280,3 +280,11 @@ def test_3d_centre_average_against_a_loop(periodic, window_start, k):
 
                 want[a, b] = np.float32(summed((i, j, k), list(axes)) * np.float32(1.0 / 2 ** len(axes)))
         np.testing.assert_array_equal(got, want, err_msg=name)
+
+
+def test_a_sweep_without_values_is_refused():
+    from opensol_amd import model as M
+    spec = json.loads(str(_golden('ez')['model_json']))
+    spec['study']['sweep']['values'] = []
+    with pytest.raises(ValueError, match="sweep of 'n_film' has no values"):
+        M.run(spec)
