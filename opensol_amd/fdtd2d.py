@@ -214,12 +214,8 @@ class Model2D:
     def apply_pec(self):
         """Paint the PEC cells over the dielectrics; the material version changes only when a cell changes."""
         mask = self.pec_mask()
-        if not mask.any():
-            return
-        pec = self.materials.pec_id()
-        if np.any(self.materials.ids[mask] != pec):
-            self.materials.ids[mask] = pec
-            self.materials.version += 1
+        if mask.any():
+            self.materials.paint_pec([mask])
 
     def check_courant(self):
         """ValueError when eps * mu_r < (C0 dt)^2 (1/dx^2 + 1/dy^2) for a material in use (physics/em/__init__.py:

@@ -137,13 +137,8 @@ class Model3D:
         version changes only when a cell changes."""
         if not self.pec_boxes and not self.bc.pec_faces():
             return
-        pec = self.materials.pec_id()
         keys = [tuple(slice(lo, hi) for lo, hi in ranges) for ranges in self.pec_boxes] + self.pec_wall_layers()
-        ids = self.materials.ids
-        if any(ids[key].size and np.any(ids[key] != pec) for key in keys):
-            for key in keys:
-                ids[key] = pec
-            self.materials.version += 1
+        self.materials.paint_pec([key for key in keys if self.materials.ids[key].size])
 
     def add_dispersive(self, pole, window, mask):
         """Drude or Lorentz pole on the bool mask of the cells of `window` (a slice per axis) over their own

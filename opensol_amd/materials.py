@@ -35,6 +35,9 @@ class MaterialMap:
         self.materials = [(DTYPE(1.0),) * 3 + (DTYPE(0.0), DTYPE(1.0), DTYPE(0.0))]
         self._lookup = {self.materials[0]: 0}
         self.version = 0
+        # the dielectric ids painted under PEC cells (None until PEC is painted): the reference keeps PEC in its
+        # engine's own index and its material arrays hold the dielectric (physics/em3d/engine3d.py:351-362)
+        self.dielectric_ids = None
 
     def material_id(self, eps_x, eps_y=None, eps_z=None, sigma_e=0.0, mu_r=1.0, sigma_m=0.0):
         eps_y = eps_x if eps_y is None else eps_y
@@ -80,13 +83,49 @@ class MaterialMap:
         self.ids[mask] = material
         self.version += 1
 
+    def paint_pec(self, keys):
+        """Paint PEC over the cells of every index expression in `keys` (slice tuples or bool masks), keeping the
+        dielectric painted under them in dielectric_ids; returns whether a cell changed."""
+        pec = self.pec_id()
+        if not any(np.any(self.ids[key] != pec) for key in keys):
+            return False
+        under = self.ids.copy() if self.dielectric_ids is None else self.dielectric_ids
+        fresh = self.ids != pec   # cells painted since the last PEC pass hold the newest dielectric
+        under[fresh] = self.ids[fresh]
+        self.dielectric_ids = under
+        for key in keys:
+            self.ids[key] = pec
+        self.version += 1
+        return True
+
+    def display_ids(self, key=Ellipsis):
+        """The ids of the cells `key` with every PEC cell showing the dielectric painted under it: the material grid
+        the reference's renderers show (its PEC lives only in the engine's index)."""
+        ids = self.ids[key]
+        if self.dielectric_ids is None:
+            return ids
+        return np.where(self.pec_flags()[ids], self.dielectric_ids[key], ids)
+
+    def display_array(self, column, key=Ellipsis):
+        """Per-cell float32 values of one material column as displayed (display_ids)."""
+        table = np.array([m[column] for m in self.materials], dtype=DTYPE)
+        return table[self.display_ids(key)]
+
     def set_property(self, columns, values):
         """Write the per-cell float32 array `values` into material columns `columns` (e.g. (0, 1, 2) for an isotropic
         eps_r, (3,) for sigma_e) of every cell, as a direct write into the reference's material arrays followed by
         materials_changed does (core/material_grid.py:38-83). Each distinct (old material, new value) pair becomes one
-        id; PEC cells stay PEC."""
+        id; PEC cells stay PEC, and the dielectric under them takes the values too."""
         values = np.broadcast_to(np.asarray(values, dtype=DTYPE), self.shape)
-        key = (self.ids.astype(np.uint64) << np.uint64(32)) | values.view(np.uint32).astype(np.uint64)
+        new_ids = self._with_property(self.ids, columns, values)
+        if self.dielectric_ids is not None:
+            self.dielectric_ids = self._with_property(self.dielectric_ids, columns, values)
+        self.ids = new_ids
+        self.version += 1
+
+    def _with_property(self, ids, columns, values):
+        """ids remapped so that each cell's material takes `values` in `columns` (PEC entries kept)."""
+        key = (ids.astype(np.uint64) << np.uint64(32)) | values.view(np.uint32).astype(np.uint64)
         unique, inverse = np.unique(key.ravel(), return_inverse=True)
         pec = self.pec_flags()
         mapped = np.empty(unique.size, dtype=np.uint16)
@@ -100,8 +139,7 @@ class MaterialMap:
             for column in columns:
                 entry[column] = value
             mapped[u] = self.material_id(*entry)
-        self.ids = mapped[inverse].reshape(self.shape)
-        self.version += 1
+        return mapped[inverse].reshape(self.shape)
 
     def values(self):
         """(n_materials, 6) float64 array of the float32 material values."""

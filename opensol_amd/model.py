@@ -575,7 +575,7 @@ class _Recorder:
             out['e_intensity'] = ez[w] * ez[w]
             hx_c, hy_c = _to_centres(hx, 1, py)[w], _to_centres(hy, 0, px)[w]
             out['h_abs'] = np.sqrt(hx_c * hx_c + hy_c * hy_c)
-            out['eps_r'] = materials.property_array(0)[w]
+            out['eps_r'] = materials.display_array(0, w)
         else:
             hz, ex, ey = (sim.engine.field(n) for n in ('hz', 'ex', 'ey'))
             ex_c, ey_c = _to_centres(ex, 1, py)[w], _to_centres(ey, 0, px)[w]
@@ -584,9 +584,9 @@ class _Recorder:
             out['e_abs'] = np.sqrt(intensity)
             out['e_intensity'] = intensity
             out['h_abs'] = np.abs(hz[w])
-            out['eps_xx'] = materials.property_array(0)[w]
-            out['eps_yy'] = materials.property_array(1)[w]
-        out['sigma_e'] = materials.property_array(3)[w]
+            out['eps_xx'] = materials.display_array(0, w)
+            out['eps_yy'] = materials.display_array(1, w)
+        out['sigma_e'] = materials.display_array(3, w)
         shape = tuple(s.stop - s.start for s in w)
         # no absorption tracking, carriers, gain or phase-change media on this path: these read zero, as the
         # reference's do without them (physics/em/renderer.py:62-69, 122-150)
@@ -600,9 +600,10 @@ class _Recorder:
             return
         values = self._values()
         fields = {key: values[key.split('.', 1)[1]] for key in self.keys}
+        cuts = [{key: cut[key.split('.', 1)[1]] for key in self.keys} for cut in self._cut_values()]
         self.snapshots.append({'time': float(self.sim.time), 'label': None, 'time_label': self.time_label(),
-                               'fields': fields})
-        self.stored_bytes += sum(v.nbytes for v in fields.values())
+                               'fields': fields, 'cuts': cuts})
+        self.stored_bytes += _snapshot_bytes(self.snapshots[-1:])
         if not (final or self.full) and self.stored_bytes > _SNAPSHOT_BYTE_LIMIT:
             self.full = True
 
@@ -611,18 +612,26 @@ class _Recorder:
 
     plane = None
 
+    def _cut_values(self):
+        return []   # 2D: no cut planes
+
+    def cut_records(self):
+        return []
+
     def layout(self):
-        """What every snapshot of this run shares: infos, extent, centres, edges and the cut plane (3D)."""
+        """What every snapshot of this run shares: infos, extent, centres, edges, the cut plane (3D) and the stored
+        cuts' own layouts."""
         return {'infos': self.infos, 'extent': self.extent, 'centres': self.centres, 'edges': self.edges,
-                'plane': self.plane}
+                'plane': self.plane, 'axes': ['x', 'y'], 'cuts': []}
 
 
 class _Recorder3D(_Recorder):
     """Stored-field snapshots of a 3D run: the mid-height z cut of each stored field (model/field_access_tensor.py:
-    186-204: the window cell containing size_z / 2), float32 over the window's x and y cells. Only the two z layers the
-    cut needs are copied from the device."""
+    186-204: the window cell containing size_z / 2), float32 over the window's x and y cells, plus one cut per stored
+    cut plane ([axis, position]: the window cell containing the position, model/study.py:2250-2258), over the other
+    two axes in grid order. Only the two layers a cut needs are copied from the device."""
 
-    def __init__(self, built, requested=()):
+    def __init__(self, built, requested=(), cuts=()):
         sim = built.sim
         self.sim = sim
         self.physics = built.physics_name
@@ -635,66 +644,99 @@ class _Recorder3D(_Recorder):
         self.keys = list(requested) or list(infos)
         self.infos = {key: infos[key] for key in self.keys}
         self.window = tuple(domain.physical_window())
-        self.k = domain.pos_to_cell_axis(2, 0.5 * domain.size_z)   # grid index of the cut
         edges, centres = domain.cell_edges(), domain.cell_centers()
-        self.plane = ['z', float(centres[2][self.k])]
-        self.edges = [np.asarray(e[w.start:w.stop + 1], dtype=np.float64) for e, w in zip(edges[:2], self.window)]
-        self.centres = [np.asarray(c[w], dtype=np.float64) for c, w in zip(centres[:2], self.window)]
-        self.extent = [float(self.edges[0][0]), float(self.edges[0][-1]), float(self.edges[1][0]),
-                       float(self.edges[1][-1])]
+        self.planes = []
+        for axis_name, position in [('z', 0.5 * domain.size_z)] + list(cuts):
+            axis = 'xyz'.index(axis_name)
+            k = domain.pos_to_cell_axis(axis, position)
+            others = [b for b in range(3) if b != axis]
+            plane_edges = [np.asarray(edges[b][self.window[b].start:self.window[b].stop + 1], dtype=np.float64)
+                           for b in others]
+            self.planes.append({'axis': axis, 'k': k, 'plane': [axis_name, float(centres[axis][k])],
+                                'axes': ['xyz'[b] for b in others], 'edges': plane_edges,
+                                'centres': [np.asarray(centres[b][self.window[b]], dtype=np.float64) for b in others],
+                                'extent': [float(plane_edges[0][0]), float(plane_edges[0][-1]),
+                                           float(plane_edges[1][0]), float(plane_edges[1][-1])]})
+        default = self.planes[0]
+        self.plane, self.edges, self.centres, self.extent = (default['plane'], default['edges'], default['centres'],
+                                                             default['extent'])
         self.snapshots = []
         self.stored_bytes = 0
         self.full = False
 
-    def _centre(self, layers, name):
-        """float32 window cut of a raw component averaged to the cell centres (physics/em3d/renderer3d.py:115-147):
-        the window plus the lower row along each averaged axis (cell 0: the wrapped last row on a periodic axis, else
-        its own), one pair sum per averaged axis in axis order, then the product with 1 / 2^axes."""
+    def _centre(self, layers, name, axis=2):
+        """float32 window cut normal to `axis` of a raw component averaged to the cell centres (physics/em3d/
+        renderer3d.py:115-147): the window plus the lower row along each averaged axis (cell 0: the wrapped last row on
+        a periodic axis, else its own), one pair sum per averaged axis in axis order, then the product with 1 / 2^axes.
+        layers[name] holds the grid with two layers along `axis`: (lower, cut)."""
         model = self.sim.model
-        axes = CENTRE_AXES_3D[name]
+        averaged = CENTRE_AXES_3D[name]
         index = []
-        for axis, cells in enumerate(self.window[:2]):
+        for b in range(3):
+            if b == axis:
+                index.append(np.asarray([0, 1] if b in averaged else [1]))
+                continue
+            cells = self.window[b]
             rows = list(range(cells.start, cells.stop))
-            if axis in axes:
-                lower = cells.start - 1 if cells.start > 0 else (model.shape[axis] - 1 if model.periodic[axis] else 0)
+            if b in averaged:
+                lower = cells.start - 1 if cells.start > 0 else (model.shape[b] - 1 if model.periodic[b] else 0)
                 rows = [lower] + rows
             index.append(np.asarray(rows))
-        component = layers[name]   # (nx, ny, 2): z layers (lower, cut)
-        values = component[np.ix_(index[0], index[1], [0, 1] if 2 in axes else [1])].astype(np.float32)
-        for axis in axes:
+        values = layers[name][np.ix_(*index)].astype(np.float32)
+        for b in averaged:
             upper, lower = [slice(None)] * 3, [slice(None)] * 3
-            upper[axis], lower[axis] = slice(1, None), slice(None, -1)
+            upper[b], lower[b] = slice(1, None), slice(None, -1)
             values = values[tuple(upper)] + values[tuple(lower)]
-        values = values * np.float32(1.0 / 2 ** len(axes))
-        return values[:, :, 0]
+        values = values * np.float32(1.0 / 2 ** len(averaged))
+        return np.take(values, 0, axis=axis)
 
-    def _values(self):
-        """{component: float32 (wx, wy) cut} of the current state (physics/em3d/renderer3d.py:96-113)."""
+    def _plane_values(self, plane):
+        """{component: float32 cut} of the current state on one plane (physics/em3d/renderer3d.py:96-113)."""
         from .fdtd3d import COMPONENTS
-        sim, model = self.sim, self.sim.model
-        engine = sim.engine
-        k = self.k
-        k_lower = k - 1 if k > 0 else (model.shape[2] - 1 if model.periodic[2] else k)
+        model = self.sim.model
+        engine = self.sim.engine
+        axis, k = plane['axis'], plane['k']
+        k_lower = k - 1 if k > 0 else (model.shape[axis] - 1 if model.periodic[axis] else k)
         state = engine.fields[engine.n_step & 1]
-        cut = state[:, :, :, [k_lower, k]].cpu().numpy()   # (6, nx, ny, 2)
+        pick = [slice(None)] * 4
+        pick[1 + axis] = [k_lower, k]
+        cut = state[tuple(pick)].cpu().numpy()   # (6, ...) with two layers along the cut axis
         layers = {name: cut[c] for c, name in enumerate(COMPONENTS)}
-        wx, wy = self.window[0], self.window[1]
-        out = {name: layers[name][wx, wy, 1] for name in COMPONENTS}
-        e2 = sum(self._centre(layers, name) ** 2 for name in ('ex', 'ey', 'ez'))
-        h2 = sum(self._centre(layers, name) ** 2 for name in ('hx', 'hy', 'hz'))
+        window = list(self.window)
+        window[axis] = 1   # the cut layer
+        out = {name: layers[name][tuple(window)] for name in COMPONENTS}
+        e2 = sum(self._centre(layers, name, axis) ** 2 for name in ('ex', 'ey', 'ez'))
+        h2 = sum(self._centre(layers, name, axis) ** 2 for name in ('hx', 'hy', 'hz'))
         out['e_intensity'] = e2
         out['e_abs'] = np.sqrt(e2)
         out['h_abs'] = np.sqrt(h2)
-        materials = model.materials   # the E_z permittivity of the cut's cells only (no full-grid array)
-        eps_z = np.array([m[2] for m in materials.materials], dtype=np.float32)
-        out['eps_r'] = eps_z[materials.ids[wx, wy, k]]
+        cells = list(self.window)   # the E_z permittivity of the cut's cells only (no full-grid array)
+        cells[axis] = k
+        out['eps_r'] = model.materials.display_array(2, tuple(cells))
         # no absorption tracking on this path: zeros, as the reference's without it (renderer3d.py:104-113)
-        out['absorbed_power'] = np.zeros((wx.stop - wx.start, wy.stop - wy.start), dtype=np.float32)
+        out['absorbed_power'] = np.zeros(out['ez'].shape, dtype=np.float32)
         return {name: np.ascontiguousarray(values, dtype=np.float32) for name, values in out.items()}
+
+    def _values(self):
+        return self._plane_values(self.planes[0])
+
+    def _cut_values(self):
+        return [self._plane_values(plane) for plane in self.planes[1:]]
+
+    def layout(self):
+        out = super().layout()
+        out['cuts'] = [{'infos': self.infos, 'extent': p['extent'], 'centres': p['centres'], 'plane': p['plane'],
+                        'axes': p['axes']} for p in self.planes[1:]]
+        return out
+
+    def cut_records(self):
+        return [list(plane['plane']) for plane in self.planes[1:]]
 
 
 def _snapshot_bytes(snapshots):
-    return sum(v.nbytes for snapshot in snapshots for v in snapshot['fields'].values())
+    """Host bytes of the stored values of snapshots, cuts included (model/study.py:662-671)."""
+    return sum(v.nbytes for snapshot in snapshots for fields in [snapshot['fields'], *snapshot.get('cuts', [])]
+               for v in fields.values())
 
 
 # ---- digests (io/results_io.py:156-225) ----------------------------------------------------------------------------
@@ -766,18 +808,29 @@ def _split(result, prefix, arrays):
     return entries
 
 
-def _snapshot_entry(snapshot, layout, prefix, arrays):
-    """Metadata entry {time, label, fields, cuts, segments} of one snapshot; its values and native cell centres go
-    into arrays under prefix + key (io/results_io.py:441-481)."""
-    fields = {}
-    for key, values in snapshot['fields'].items():
+def _field_entries(fields, layout, time_label, prefix, arrays):
+    """Metadata entries of one snapshot cut's {key: values}; values and native cell centres go into arrays under
+    prefix + key (io/results_io.py:467-481)."""
+    entries = {}
+    for key, values in fields.items():
         arrays[f'{prefix}{key}/values'] = values
         arrays[f'{prefix}{key}/x_centers'] = layout['centres'][0]
         arrays[f'{prefix}{key}/y_centers'] = layout['centres'][1]
-        fields[key] = {'extent': layout['extent'], 'info': layout['infos'][key], 'time_label': snapshot['time_label'],
-                       'native_shape': None, 'plane': layout['plane'], 'axes': ['x', 'y'], 'unfolded': False,
-                       'mesh': 'mesh', 'triangulation': None}
-    return {'time': snapshot['time'], 'label': snapshot['label'], 'fields': fields, 'cuts': [], 'segments': {}}
+        entries[key] = {'extent': layout['extent'], 'info': layout['infos'][key], 'time_label': time_label,
+                        'native_shape': None, 'plane': layout['plane'], 'axes': list(layout['axes']),
+                        'unfolded': False, 'mesh': 'mesh', 'triangulation': None}
+    return entries
+
+
+def _snapshot_entry(snapshot, layout, prefix, arrays):
+    """Metadata entry {time, label, fields, cuts, segments} of one snapshot (io/results_io.py:441-464): the default
+    cut's fields under prefix, stored cut c under prefix + 'cut<c>/'."""
+    time_label = snapshot['time_label']
+    return {'time': snapshot['time'], 'label': snapshot['label'],
+            'fields': _field_entries(snapshot['fields'], layout, time_label, prefix, arrays),
+            'cuts': [_field_entries(fields, cut_layout, time_label, f'{prefix}cut{c}/', arrays)
+                     for c, (fields, cut_layout) in enumerate(zip(snapshot.get('cuts', []), layout['cuts']))],
+            'segments': {}}
 
 
 class Results:
@@ -850,7 +903,17 @@ class Results:
         return files
 
 
-def _advance(built, settings, names, requested):
+def _stored_cuts(study, names):
+    """[(axis, position [m])] of the study's stored_cuts (model/study.py:2250-2258)."""
+    planes = []
+    for cut in study.get('stored_cuts') or []:
+        if not (isinstance(cut, (list, tuple)) and len(cut) == 2 and cut[0] in ('x', 'y', 'z')):
+            raise ValueError(f"stored cut must be ['x' | 'y' | 'z', position in m], got {cut!r}")
+        planes.append((cut[0], evaluate(cut[1], names)))
+    return planes
+
+
+def _advance(built, settings, names, requested, cuts=()):
     """One time-dependent study of a build (model/study.py:916-967): the snapshot ticks of snapshot_schedule, with a
     record at each and the final state last (3D: the mid-height z cut). Returns the recorder."""
     sim = built.sim
@@ -860,7 +923,9 @@ def _advance(built, settings, names, requested):
     if not float(interval) >= 0.0 or count < 1:
         raise ValueError(f'output_interval must be >= 0 and n_snapshots >= 1, got {interval!r}, {count!r}')
     sim.finalize()
-    recorder = (_Recorder3D if built.three_d else _Recorder)(built, requested)
+    if cuts and not built.three_d:
+        raise ValueError('stored cuts need a 3D model (em_fdtd_3d); this model is 2D')
+    recorder = _Recorder3D(built, requested, cuts) if built.three_d else _Recorder(built, requested)
     schedule = snapshot_schedule(sim.model.dt, duration, interval, count)
     for index, (tick, _) in enumerate(schedule):
         if tick > sim.n_step:
@@ -901,8 +966,6 @@ def run(spec, out_dir=None, parameters=None, progress=None):
         raise ValueError(f"study type {study.get('type')!r}: only 'time_dependent' is part of the FDTD path")
     if spec.get('derived_values'):
         raise ValueError('derived values are not part of the FDTD path')
-    if study.get('stored_cuts'):
-        raise ValueError('stored cuts (3D planes) are not part of the FDTD path')
     settings = study.get('settings') or {}
     requested = list(study.get('stored_fields') or [])
     overrides = dict(parameters or {})
@@ -926,7 +989,7 @@ def run(spec, out_dir=None, parameters=None, progress=None):
         if built is not None:
             built.sim.close()
         built = _Built(spec, names)
-        recorder = _advance(built, settings, names, requested)
+        recorder = _advance(built, settings, names, requested, _stored_cuts(study, names))
         sweep_results.append(built.results())
         if kept is not None:
             kept.take(recorder)
@@ -938,7 +1001,7 @@ def run(spec, out_dir=None, parameters=None, progress=None):
                'results': {key: result_digest(result) for key, result in results.items()},
                't_end': float(built.sim.time), 'ticks': int(built.sim.engine.n_step), 'tick_unit': 'step',
                'snapshot_times': [s['time'] for s in snapshots], 'snapshot_labels': [s['label'] for s in snapshots],
-               'stored_fields': list(recorder.keys),
+               'stored_fields': list(recorder.keys), 'stored_cuts': recorder.cut_records(),
                'wall_time': time.perf_counter() - wall}
     if sweep:
         summary['sweep'] = dict(sweep, derived={}, reference={}, error={}, curve_error={},

@@ -31,8 +31,8 @@ def copy_probe(mib, repeat):
 
 def stream_rates(mib, repeat):
     """HBM streaming rates of the runtime's own kernels on buffers of `mib` MiB, timed with HIP events: device copy
-    (read 1 : write 1, the Yee kernels' mix is 14 : 12 in 2D and 26 : 24 in 3D), elementwise add (2 : 1), fill
-    (0 : 1) and sum (1 : 0). Rates count bytes read plus bytes written."""
+    (read 1 : write 1, the Yee kernels' mix is 14 : 12 in 2D and 26 : 24 in 3D), elementwise scale (1 : 1), add
+    (2 : 1), fill (0 : 1) and sum (1 : 0). Rates count bytes read plus bytes written."""
     import torch
     if not torch.cuda.is_available():
         raise RuntimeError('stream rates need a GPU')
@@ -41,7 +41,8 @@ def stream_rates(mib, repeat):
     b = torch.full_like(a, 2.0)
     c = torch.empty_like(a)
     size = n * 4
-    cases = {'copy (1:1)': (lambda: c.copy_(a), 2 * size), 'add (2:1)': (lambda: torch.add(a, b, out=c), 3 * size),
+    cases = {'copy (1:1)': (lambda: c.copy_(a), 2 * size), 'scale (1:1)': (lambda: torch.mul(a, 2.0, out=c), 2 * size),
+             'add (2:1)': (lambda: torch.add(a, b, out=c), 3 * size),
              'fill (0:1)': (lambda: c.fill_(3.0), size), 'sum (1:0)': (lambda: a.sum(), size)}
     for name, (op, moved) in cases.items():
         for _ in range(3):

@@ -231,7 +231,7 @@ class Simulation:
         component = component or self.model.components[0]
         if component in ('eps_r', 'eps_xx', 'eps_yy', 'sigma_e', 'mu_r', 'sigma_m'):
             column = {'eps_r': 0, 'eps_xx': 0, 'eps_yy': 1, 'sigma_e': 3, 'mu_r': 4, 'sigma_m': 5}[component]
-            return self.model.materials.property_array(column)[self.domain.physical_window()]
+            return self.model.materials.display_array(column, self.domain.physical_window())
         if component not in self.model.components:
             raise KeyError(f"no field '{component}'; available: {self.model.components}")
         return np.asarray(self.engine.field(component)[self.domain.physical_window()], dtype=DTYPE)

@@ -131,10 +131,10 @@ class FDTDSolver:
             out[name] = current[index] if self.kind == '3d' else current[index, :, :self.model.ny]
         materials = self.model.materials
         if self.kind == 'hz':
-            out['eps_xx'], out['eps_yy'] = materials.property_array(0), materials.property_array(1)
+            out['eps_xx'], out['eps_yy'] = materials.display_array(0), materials.display_array(1)
         else:
-            out['eps_r'] = materials.property_array(0)
-        out['sigma_e'] = materials.property_array(3)
+            out['eps_r'] = materials.display_array(0)
+        out['sigma_e'] = materials.display_array(3)
         return out
 
     def inject_source(self, name: str, array) -> None:

@@ -31,7 +31,8 @@ MONITORS = ('reflection_side_flux', 'transmitted_flux')
 KEPT_SNAPSHOT = 20
 KEPT_FIELDS = {'ez': ('ez', 'hx', 'hy', 'e_abs', 'e_intensity', 'h_abs', 'eps_r', 'sigma_e'),
                'hz': ('hz', 'ex', 'ey', 'e_abs', 'e_intensity', 'h_abs', 'eps_xx', 'eps_yy', 'sigma_e')}
-# 3D: the stored fields' mid-height z cut in snapshot KEPT_SNAPSHOT_3D and in the final field group
+# 3D: the stored fields' mid-height z cut in snapshot KEPT_SNAPSHOT_3D and in the final field group, and that
+# snapshot's two stored cuts (MODEL_3D's study stores an x and a y plane besides the z cut)
 KEPT_SNAPSHOT_3D = 2
 KEPT_FIELDS_3D = ('ex', 'ey', 'ez', 'hx', 'hy', 'hz', 'e_abs', 'e_intensity', 'h_abs', 'eps_r', 'absorbed_power')
 
@@ -86,7 +87,7 @@ MODEL_3D = {
     'mesh': {'mode': 'uniform', 'dx': '25*nm', 'max_ratio': 1.15, 'wavelength_min': None,
              'points_per_wavelength': 20.0, 'refinements': []},
     'study': {'type': 'time_dependent', 'settings': {'duration': '15*fs', 'output_interval': '5*fs'},
-              'stored_fields': [], 'sweep': None},
+              'stored_fields': [], 'stored_cuts': [['x', '0.31*um'], ['y', 'L/2']], 'sweep': None},
     'derived_values': [],
 }
 
@@ -113,6 +114,13 @@ def run_3d():
                 for group in (f'snapshot/{KEPT_SNAPSHOT_3D}', 'field'):
                     out[f'npz/{group}/em.{name}/values'] = npz[f'{group}/em.{name}/values']
             out['npz/result/em.layer/intensity_per_cw_factor'] = npz['result/em.layer/intensity_per_cw_factor']
+            for cut in (0, 1):   # the stored cuts (x, then y) of that snapshot
+                for name in KEPT_FIELDS_3D:
+                    key = f'snapshot/{KEPT_SNAPSHOT_3D}/cut{cut}/em.{name}/values'
+                    out[f'npz/{key}'] = npz[key]
+                for axis in 'xy':
+                    key = f'snapshot/{KEPT_SNAPSHOT_3D}/cut{cut}/em.ez/{axis}_centers'
+                    out[f'npz/{key}'] = npz[key]
             for axis in 'xy':
                 out[f'npz/snapshot/{KEPT_SNAPSHOT_3D}/{axis}_centers'] = npz[
                     f'snapshot/{KEPT_SNAPSHOT_3D}/em.ez/{axis}_centers']
@@ -120,6 +128,7 @@ def run_3d():
         with open(os.path.join(directory, 'out', 'summary.json'), encoding='utf-8') as handle:
             summary = json.load(handle)
         out['summary_snapshot_times'] = np.array(summary['snapshot_times'], dtype=np.float64)
+        out['summary_stored_cuts'] = np.array(json.dumps(summary['stored_cuts']))
         out['summary_t_end'] = np.float64(summary['t_end'])
         for name in ('flux_x', 'flux_z', 'box'):
             result = results.result('em', name)

@@ -96,12 +96,14 @@ def test_3d_model_matches_the_reference_run(gpu, tmp_path):
     reference's api.run of the same file (float32 CPU path): flux and per-cw power within 1e-4, the layer intensity
     within 1e-4 relative L2 per wavelength, the probe within 1e-4 of its peak; results.npz with every array name and
     the field / snapshot / result metadata of the reference's, the snapshot times bit for bit, the layer's per-cw factor
-    within 1e-4, and the mid-height z cuts of snapshot 2 and of the final state within 1e-5 of the running peak of
-    the field's vector (raw components) or of the field itself (moduli); materials exact."""
+    within 1e-4, the summary's stored cuts, and the mid-height z cuts of snapshot 2 and of the final state and
+    snapshot 2's stored x and y cuts within 1e-5 of the running peak (on that cut) of the field's vector (raw
+    components) or of the field itself (moduli); materials exact."""
     from opensol_amd import model as M
     golden = np.load(os.path.join(FIXTURES, 'model_dipole_3d.npz'))
     results = M.run(json.loads(str(golden['model_json'])), out_dir=tmp_path)
     assert results.summary()['snapshot_times'] == list(golden['summary_snapshot_times'])
+    assert results.summary()['stored_cuts'] == json.loads(str(golden['summary_stored_cuts']))
     want_meta = json.loads(str(golden['npz_metadata']))
     with np.load(tmp_path / 'results.npz', allow_pickle=False) as npz:
         assert set(npz.files) == {str(n) for n in golden['npz_names']}
@@ -116,11 +118,13 @@ def test_3d_model_matches_the_reference_run(gpu, tmp_path):
         # on the cut); a modulus by its own
         vectors = {c: tuple(c[0] + a for a in 'xyz') for c in ('ex', 'ey', 'ez', 'hx', 'hy', 'hz')}
         for name in ('ex', 'ey', 'ez', 'hx', 'hy', 'hz', 'e_abs', 'e_intensity', 'h_abs', 'eps_r', 'absorbed_power'):
-            for key, upto in ((f'snapshot/{snap}/em.{name}/values', snap), (f'field/em.{name}/values', last)):
+            cases = [(f'snapshot/{snap}/em.{name}/values', snap, None), (f'field/em.{name}/values', last, None)]
+            cases += [(f'snapshot/{snap}/cut{c}/em.{name}/values', snap, c) for c in (0, 1)]
+            for key, upto, cut in cases:
                 ours, want = npz[key], golden[f'npz/{key}']
                 assert ours.dtype == np.float32 and ours.shape == want.shape, key
-                peak = max(float(np.max(np.abs(s['fields'][f'em.{c}']))) for c in vectors.get(name, (name,))
-                           for s in results.snapshots[:upto + 1])
+                peak = max(float(np.max(np.abs((s['fields'] if cut is None else s['cuts'][cut])[f'em.{c}'])))
+                           for c in vectors.get(name, (name,)) for s in results.snapshots[:upto + 1])
                 if name in ('eps_r', 'absorbed_power') or peak == 0.0:
                     np.testing.assert_array_equal(ours, want, err_msg=key)
                     continue

@@ -170,6 +170,7 @@ def test_snapshot_byte_limit_keeps_the_final_state(monkeypatch):
     monkeypatch.setattr(M, '_SNAPSHOT_BYTE_LIMIT', 100)
     recorder = Fake()
     recorder._values = lambda: {'ez': np.zeros(10, np.float32)}   # 40 bytes a record
+    recorder._cut_values = lambda: []
     recorder.sim = type('S', (), {'time': 0.0})()
     recorder.time_label = lambda: 't = 0.0 fs'
     for index in range(6):
@@ -207,22 +208,30 @@ def test_3d_schedule_and_field_table_match_the_reference():
 
 def test_3d_saved_layout_matches_the_reference(tmp_path):
     """A 3D recorder built without an engine: the cut plane (the window cell holding mid-height, by its centre), the
-    x / y edges and centres, and Results.save()'s snapshot and field names and metadata equal the reference's."""
+    stored x and y cuts (their planes and in-plane centres), the x / y edges and centres, and Results.save()'s snapshot,
+    cut and field names and metadata equal the reference's."""
     from opensol_amd import model as M
     golden = _golden_3d()
     metadata = json.loads(str(golden['npz_metadata']))
-    built, _ = _dt(json.loads(str(golden['model_json'])))
-    recorder = M._Recorder3D(built)
+    spec = json.loads(str(golden['model_json']))
+    built, names = _dt(spec)
+    recorder = M._Recorder3D(built, (), M._stored_cuts(spec['study'], names))
     assert recorder.plane == metadata['fields']['em.ez']['plane']
+    assert recorder.cut_records() == json.loads(str(golden['summary_stored_cuts']))
+    for c, plane in enumerate(recorder.planes[1:]):
+        for a, axis in enumerate('xy'):
+            np.testing.assert_array_equal(plane['centres'][a], golden[f'npz/snapshot/2/cut{c}/em.ez/{axis}_centers'])
     np.testing.assert_array_equal(recorder.edges[0], golden['npz/field/x_edges'])
     np.testing.assert_array_equal(recorder.edges[1], golden['npz/field/y_edges'])
     np.testing.assert_array_equal(recorder.centres[0], golden['npz/snapshot/2/x_centers'])
     np.testing.assert_array_equal(recorder.centres[1], golden['npz/snapshot/2/y_centers'])
     shape = (recorder.edges[0].size - 1, recorder.edges[1].size - 1)
+    cut_shapes = [tuple(c.size for c in plane['centres']) for plane in recorder.planes[1:]]
     for snapshot in metadata['snapshots']:
         t = snapshot['time']
         recorder.snapshots.append({'time': t, 'label': None, 'time_label': f't = {t * 1e15:.1f} fs',
-                                   'fields': {k: np.zeros(shape, np.float32) for k in recorder.keys}})
+                                   'fields': {k: np.zeros(shape, np.float32) for k in recorder.keys},
+                                   'cuts': [{k: np.zeros(cs, np.float32) for k in recorder.keys} for cs in cut_shapes]})
     fields = {key: (values, recorder.edges[0], recorder.edges[1])
               for key, values in recorder.snapshots[-1]['fields'].items()}
     out = M.Results('completed', {}, {}, [{}], None, fields, {'t_end': recorder.snapshots[-1]['time']},
@@ -242,10 +251,11 @@ class _CutModel:
         self.shape, self.periodic = shape, periodic
 
 
-@pytest.mark.parametrize('periodic', [(False, False, False), (True, True, False)])
+@pytest.mark.parametrize('cut_axis', [0, 1, 2])
+@pytest.mark.parametrize('periodic', [(False, False, False), (True, True, True)])
 @pytest.mark.parametrize('window_start', [0, 2])
 @pytest.mark.parametrize('k', [0, 3])
-def test_3d_centre_average_against_a_loop(periodic, window_start, k):
+def test_3d_centre_average_against_a_loop(cut_axis, periodic, window_start, k):
     """_Recorder3D._centre against a per-cell loop of the reference's rule: pair sums along each averaged axis in
     axis order (the lower neighbour of window cell 0 wraps on a periodic axis, else is the cell itself), times
     1 / 2^axes, in float32."""
@@ -255,18 +265,22 @@ def test_3d_centre_average_against_a_loop(periodic, window_start, k):
     volume = {name: rng.standard_normal(shape).astype(np.float32) for name in M.CENTRE_AXES_3D}
     recorder = M._Recorder3D.__new__(M._Recorder3D)
     recorder.sim = type('S', (), {'model': _CutModel(shape, periodic)})()
-    recorder.window = (slice(window_start, shape[0] - 1), slice(window_start, shape[1]), slice(0, shape[2]))
-    k_lower = k - 1 if k > 0 else (shape[2] - 1 if periodic[2] else k)
-    layers = {name: v[:, :, [k_lower, k]] for name, v in volume.items()}
+    recorder.window = (slice(window_start, shape[0] - 1), slice(window_start, shape[1]), slice(window_start, shape[2]))
+    k_lower = k - 1 if k > 0 else (shape[cut_axis] - 1 if periodic[cut_axis] else k)
+    pick = [slice(None)] * 3
+    pick[cut_axis] = [k_lower, k]
+    layers = {name: v[tuple(pick)] for name, v in volume.items()}
+    in_plane = [b for b in range(3) if b != cut_axis]
 
     def lower(axis, index):   # grid row 0 pairs with the last row on a periodic axis, else with itself
         return index - 1 if index > 0 else (shape[axis] - 1 if periodic[axis] else index)
 
     for name, axes in M.CENTRE_AXES_3D.items():
-        got = recorder._centre(layers, name)
+        got = recorder._centre(layers, name, cut_axis)
         want = np.empty_like(got)
-        for a, i in enumerate(range(recorder.window[0].start, recorder.window[0].stop)):
-            for b, j in enumerate(range(recorder.window[1].start, recorder.window[1].stop)):
+        u, v = in_plane
+        for a, i in enumerate(range(recorder.window[u].start, recorder.window[u].stop)):
+            for b, j in enumerate(range(recorder.window[v].start, recorder.window[v].stop)):
                 def value(p):
                     return volume[name][p]
 
@@ -278,7 +292,9 @@ def test_3d_centre_average_against_a_loop(periodic, window_start, k):
                     q[axis] = lower(axis, p[axis])
                     return np.float32(summed(p, remaining[:-1]) + summed(tuple(q), remaining[:-1]))
 
-                want[a, b] = np.float32(summed((i, j, k), list(axes)) * np.float32(1.0 / 2 ** len(axes)))
+                point = [0, 0, 0]
+                point[u], point[v], point[cut_axis] = i, j, k
+                want[a, b] = np.float32(summed(tuple(point), list(axes)) * np.float32(1.0 / 2 ** len(axes)))
         np.testing.assert_array_equal(got, want, err_msg=name)
 
 
@@ -288,3 +304,44 @@ def test_a_sweep_without_values_is_refused():
     spec['study']['sweep']['values'] = []
     with pytest.raises(ValueError, match="sweep of 'n_film' has no values"):
         M.run(spec)
+
+
+def test_3d_material_cuts_show_the_dielectric_under_pec():
+    """The displayed eps_r of the z cut and both stored cuts equals the reference's on the CPU (no fields needed),
+    including the y cut through the PEC box, where the reference shows the dielectric painted under the PEC (its PEC
+    lives in the engine's index only, physics/em3d/engine3d.py:351-362)."""
+    from opensol_amd import model as M
+    golden = _golden_3d()
+    spec = json.loads(str(golden['model_json']))
+    built, names = _dt(spec)
+    built.sim.model.finalize()   # paints the PEC box
+    recorder = M._Recorder3D(built, (), M._stored_cuts(spec['study'], names))
+    materials = built.sim.model.materials
+    crossed = 0
+    for c, plane in enumerate(recorder.planes):
+        cells = list(recorder.window)
+        cells[plane['axis']] = plane['k']
+        shown = materials.display_array(2, tuple(cells))
+        want = golden['npz/field/em.eps_r/values' if c == 0 else f'npz/snapshot/2/cut{c - 1}/em.eps_r/values']
+        np.testing.assert_array_equal(shown, want, err_msg=str(plane['plane']))
+        crossed += int(np.count_nonzero(materials.pec_flags()[materials.ids[tuple(cells)]]))
+    assert crossed > 0   # a cut does cross the PEC box
+
+
+def test_pec_keeps_the_dielectric_under_it_for_display():
+    from opensol_amd.materials import MaterialMap
+    m = MaterialMap((6, 5))
+    glass = m.material_id(2.25)
+    m.paint_box(((0, 4), (0, 5)), glass)
+    mask = np.zeros((6, 5), dtype=bool)
+    mask[2:5, 1:3] = True
+    assert m.paint_pec([mask])
+    assert np.all(m.property_array(0)[mask] == -1.0)                  # the kernels' ids: PEC
+    shown = m.display_array(0)
+    assert np.all(shown[2:4, 1:3] == np.float32(2.25)) and np.all(shown[4, 1:3] == 1.0)   # glass, vacuum under PEC
+    assert not m.paint_pec([mask])                                    # nothing changes on a second pass
+    m.paint_box(((0, 6), (2, 3)), m.material_id(4.0))                 # a later dielectric over part of the PEC
+    assert m.paint_pec([mask])                                        # PEC again on top of it
+    assert np.all(m.display_array(0)[2:5, 2] == 4.0) and np.all(m.display_array(0)[2:4, 1] == np.float32(2.25))
+    m.set_property((3,), np.full((6, 5), 7.0, np.float32))            # a write into every cell reaches the dielectric
+    assert np.all(m.display_array(3) == 7.0) and np.all(m.property_array(0)[mask] == -1.0)
