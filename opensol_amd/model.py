@@ -533,11 +533,43 @@ def _to_centres(face, axis, periodic):
     return (face + lower) * np.float32(0.5)
 
 
+def _nearest_cells(edges, n_pixels):
+    """int64 index of the cell containing each of n_pixels uniform pixel centres over [edges[0], edges[-1]]
+    (model/field_access_tensor.py:835-841)."""
+    span = float(edges[-1] - edges[0])
+    centres = edges[0] + (np.arange(n_pixels) + 0.5) * (span / n_pixels)
+    return np.clip(np.searchsorted(edges, centres, side='right') - 1, 0, len(edges) - 2).astype(np.int64)
+
+
+def _display_view(u_edges, v_edges, max_pixels=_SNAPSHOT_PIXELS):
+    """None when a window plane of these cell edges is stored natively (at most max_pixels cells), else (u index,
+    v index, native shape) of its nearest-cell resampling on a uniform grid of at most max_pixels pixels
+    (model/field_access_tensor.py:74-89, 245-267)."""
+    n_u, n_v = len(u_edges) - 1, len(v_edges) - 1
+    if n_u * n_v <= max_pixels:
+        return None
+    width, height = float(u_edges[-1] - u_edges[0]), float(v_edges[-1] - v_edges[0])
+    finest = float(min(np.diff(u_edges).min(), np.diff(v_edges).min()))
+    pixel = max(finest, math.sqrt(width * height / max_pixels))
+    pixels_u, pixels_v = max(1, int(width / pixel)), max(1, int(height / pixel))
+    if pixels_u * pixels_v > max_pixels:   # float rounding guard
+        pixels_u = max(1, max_pixels // pixels_v)
+    return _nearest_cells(u_edges, pixels_u), _nearest_cells(v_edges, pixels_v), [n_u, n_v]
+
+
+def _shown(values, view):
+    """A window plane as a snapshot stores it: itself, or its nearest-cell resampling."""
+    if view is None:
+        return values
+    return np.ascontiguousarray(values[view[0][:, None], view[1][None, :]])
+
+
 class _Recorder:
     """Stored-field snapshots of a 2D run (model/study.py:2041-2130): each stored field over the native window cells,
-    float32, with the cell centres; intermediate snapshots stop once they hold more than _SNAPSHOT_BYTE_LIMIT bytes
-    (the final one is always taken). Windows above _SNAPSHOT_PIXELS cells are stored natively too, where the reference
-    resamples them to nearest cells. `requested`: the study's stored_fields keys, else every stored field."""
+    float32, with the cell centres, or, for a window above _SNAPSHOT_PIXELS cells, resampled to the nearest cell of a
+    uniform grid of at most that many pixels (no centres); intermediate snapshots stop once they hold more than
+    _SNAPSHOT_BYTE_LIMIT bytes (the final one is always taken, and its native values make the field group).
+    `requested`: the study's stored_fields keys, else every stored field."""
 
     def __init__(self, built, requested=()):
         sim = built.sim
@@ -557,7 +589,9 @@ class _Recorder:
         self.centres = [np.asarray(c[w], dtype=np.float64) for c, w in zip(centres, self.window)]
         self.extent = [float(self.edges[0][0]), float(self.edges[0][-1]), float(self.edges[1][0]),
                        float(self.edges[1][-1])]
+        self.view = _display_view(*self.edges)
         self.snapshots = []
+        self.final_fields = None
         self.stored_bytes = 0
         self.full = False
 
@@ -599,10 +633,14 @@ class _Recorder:
         if self.full and not final:
             return
         values = self._values()
-        fields = {key: values[key.split('.', 1)[1]] for key in self.keys}
-        cuts = [{key: cut[key.split('.', 1)[1]] for key in self.keys} for cut in self._cut_values()]
+        native = {key: values[key.split('.', 1)[1]] for key in self.keys}
+        fields = {key: _shown(v, self.view) for key, v in native.items()}
+        cuts = [{key: _shown(cut[key.split('.', 1)[1]], view) for key in self.keys}
+                for cut, view in zip(self._cut_values(), self._cut_views())]
         self.snapshots.append({'time': float(self.sim.time), 'label': None, 'time_label': self.time_label(),
                                'fields': fields, 'cuts': cuts})
+        if final:
+            self.final_fields = native   # the field group: native window cells (api.py:3560-3575)
         self.stored_bytes += _snapshot_bytes(self.snapshots[-1:])
         if not (final or self.full) and self.stored_bytes > _SNAPSHOT_BYTE_LIMIT:
             self.full = True
@@ -615,14 +653,18 @@ class _Recorder:
     def _cut_values(self):
         return []   # 2D: no cut planes
 
+    def _cut_views(self):
+        return []
+
     def cut_records(self):
         return []
 
     def layout(self):
-        """What every snapshot of this run shares: infos, extent, centres, edges, the cut plane (3D) and the stored
-        cuts' own layouts."""
+        """What every snapshot of this run shares: infos, extent, centres, edges, the cut plane (3D), the native shape
+        of a resampled plane (None when native) and the stored cuts' own layouts."""
         return {'infos': self.infos, 'extent': self.extent, 'centres': self.centres, 'edges': self.edges,
-                'plane': self.plane, 'axes': ['x', 'y'], 'cuts': []}
+                'plane': self.plane, 'axes': ['x', 'y'], 'native_shape': None if self.view is None else self.view[2],
+                'cuts': []}
 
 
 class _Recorder3D(_Recorder):
@@ -656,10 +698,12 @@ class _Recorder3D(_Recorder):
                                 'axes': ['xyz'[b] for b in others], 'edges': plane_edges,
                                 'centres': [np.asarray(centres[b][self.window[b]], dtype=np.float64) for b in others],
                                 'extent': [float(plane_edges[0][0]), float(plane_edges[0][-1]),
-                                           float(plane_edges[1][0]), float(plane_edges[1][-1])]})
+                                           float(plane_edges[1][0]), float(plane_edges[1][-1])],
+                                'view': _display_view(*plane_edges)})
         default = self.planes[0]
-        self.plane, self.edges, self.centres, self.extent = (default['plane'], default['edges'], default['centres'],
-                                                             default['extent'])
+        self.plane, self.edges, self.centres, self.extent, self.view = (
+            default['plane'], default['edges'], default['centres'], default['extent'], default['view'])
+        self.final_fields = None
         self.snapshots = []
         self.stored_bytes = 0
         self.full = False
@@ -723,10 +767,14 @@ class _Recorder3D(_Recorder):
     def _cut_values(self):
         return [self._plane_values(plane) for plane in self.planes[1:]]
 
+    def _cut_views(self):
+        return [plane['view'] for plane in self.planes[1:]]
+
     def layout(self):
         out = super().layout()
         out['cuts'] = [{'infos': self.infos, 'extent': p['extent'], 'centres': p['centres'], 'plane': p['plane'],
-                        'axes': p['axes']} for p in self.planes[1:]]
+                        'axes': p['axes'], 'native_shape': None if p['view'] is None else p['view'][2]}
+                       for p in self.planes[1:]]
         return out
 
     def cut_records(self):
@@ -814,11 +862,12 @@ def _field_entries(fields, layout, time_label, prefix, arrays):
     entries = {}
     for key, values in fields.items():
         arrays[f'{prefix}{key}/values'] = values
-        arrays[f'{prefix}{key}/x_centers'] = layout['centres'][0]
-        arrays[f'{prefix}{key}/y_centers'] = layout['centres'][1]
+        if layout['native_shape'] is None:   # native cells: their centres (a resampled plane has none)
+            arrays[f'{prefix}{key}/x_centers'] = layout['centres'][0]
+            arrays[f'{prefix}{key}/y_centers'] = layout['centres'][1]
         entries[key] = {'extent': layout['extent'], 'info': layout['infos'][key], 'time_label': time_label,
-                        'native_shape': None, 'plane': layout['plane'], 'axes': list(layout['axes']),
-                        'unfolded': False, 'mesh': 'mesh', 'triangulation': None}
+                        'native_shape': layout['native_shape'], 'plane': layout['plane'],
+                        'axes': list(layout['axes']), 'unfolded': False, 'mesh': 'mesh', 'triangulation': None}
     return entries
 
 
@@ -1010,8 +1059,8 @@ def run(spec, out_dir=None, parameters=None, progress=None):
         if kept is not None and kept.kept:
             summary['sweep']['snapshot_times'] = [[s['time'] for s in snaps] for _, snaps in kept.kept]
             summary['sweep']['snapshot_labels'] = [[s['label'] for s in snaps] for _, snaps in kept.kept]
-    final = recorder.snapshots[-1]['fields']   # the field group: the stored fields of the final state
-    fields = {key: (values, recorder.edges[0], recorder.edges[1]) for key, values in final.items()}
+    # the field group: the stored fields of the final state on the native window cells
+    fields = {key: (values, recorder.edges[0], recorder.edges[1]) for key, values in recorder.final_fields.items()}
     out = Results('completed', spec, results, sweep_results, sweep, fields, summary, snapshots, recorder.layout(),
                   kept.kept if kept is not None else ())
     built.sim.close()

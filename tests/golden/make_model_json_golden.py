@@ -11,7 +11,7 @@ field group (3D: their mid-height z cuts), their cell centres and edges, and (E_
 at that snapshot.
 
 Run in this container only (the reference is not present on the GPU box):
-    python tests/golden/make_model_json_golden.py [3d] [ez] [hz] [drude]   (no argument: every case)
+    python tests/golden/make_model_json_golden.py [3d] [ez] [hz] [drude] [big]   (no argument: every case)
 The fixture holds inputs and outputs only, no source text of the reference.
 """
 import json
@@ -204,12 +204,60 @@ def run_case(polarization):
     return out
 
 
-def main(cases=('3d', 'ez', 'hz', 'drude')):
+def big_model_text():
+    """The thin-film example widened to a 553 x 459 cell window (253 827 cells, past the reference's 250 000-cell
+    snapshot limit, so its snapshots hold nearest-cell resampled fields), the film over the middle 40 % of y, a 3 fs
+    run with 2 snapshots and no sweep."""
+    with open(EXAMPLE, encoding='utf-8') as handle:
+        spec = json.load(handle)
+    for parameter in spec['parameters']:
+        if parameter['name'] == 'Lx':
+            parameter['expression'] = '13.825*um'
+        if parameter['name'] == 'Ly':
+            parameter['expression'] = '11.475*um'
+    for item in spec['geometry']:
+        if item['name'] == 'slab':
+            item['params']['y0'], item['params']['y1'] = '0.3*Ly', '0.7*Ly'
+    spec['study'] = {'type': 'time_dependent', 'settings': {'duration': '3*fs', 'n_snapshots': 2},
+                     'stored_fields': [], 'sweep': None}
+    return json.dumps(spec)
+
+
+def run_big():
+    """The widened example through api.run: results.npz names and metadata, and the resampled snapshot fields E_z
+    and eps_r of the final snapshot."""
+    sys.path.insert(0, REFERENCE)
+    from open_sol import api
+
+    text = big_model_text()
+    out = {'model_json': np.array(text)}
+    with tempfile.TemporaryDirectory() as directory:
+        path = os.path.join(directory, 'model.json')
+        with open(path, 'w', encoding='utf-8') as handle:
+            handle.write(text)
+        results = api.run(api.load(path), out_dir=os.path.join(directory, 'out'), backend='cpu')
+        if results.status != 'completed':
+            raise RuntimeError(f'reference run ended {results.status}')
+        with np.load(os.path.join(directory, 'out', 'results.npz'), allow_pickle=False) as npz:
+            out['npz_names'] = np.array(sorted(npz.files))
+            out['npz_metadata'] = np.array(str(npz['metadata']))
+            for name in ('ez', 'eps_r'):
+                out[f'npz/snapshot/1/em.{name}/values'] = npz[f'snapshot/1/em.{name}/values']
+        results.close()
+    return out
+
+
+def main(cases=('3d', 'ez', 'hz', 'drude', 'big')):
     if '3d' in cases:
         out = run_3d()
         path = os.path.join(FIXTURES, 'model_dipole_3d.npz')
         np.savez_compressed(path, **out)
         print(path, 'ticks', int(out['summary_ticks']), 'flux_x', out['flux_x/power'])
+    if 'big' in cases:
+        out = run_big()
+        path = os.path.join(FIXTURES, 'model_big_window_ez.npz')
+        np.savez_compressed(path, **out)
+        print(path, 'snapshot shape', out['npz/snapshot/1/em.ez/values'].shape)
     for polarization in [case for case in ('ez', 'hz', 'drude') if case in cases]:
         out = run_case(polarization)
         path = os.path.join(FIXTURES, f'model_thin_film_{polarization}.npz')
@@ -218,4 +266,4 @@ def main(cases=('3d', 'ez', 'hz', 'drude')):
 
 
 if __name__ == '__main__':
-    main(tuple(sys.argv[1:]) or ('3d', 'ez', 'hz', 'drude'))
+    main(tuple(sys.argv[1:]) or ('3d', 'ez', 'hz', 'drude', 'big'))

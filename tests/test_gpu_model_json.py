@@ -154,3 +154,25 @@ def test_3d_model_matches_the_reference_run(gpu, tmp_path):
     error = np.max(np.abs(probe['values'] - golden['probe/values'])) / peak
     print(f'3D probe: max err / peak {error:.2e}')
     assert error <= 1e-4
+
+
+def test_big_window_model_matches_the_reference_run(gpu, tmp_path):
+    """The thin-film example widened to 553 x 459 window cells (past the 250 000-cell snapshot limit): every array
+    name and the field / snapshot metadata equal the reference run's, and the resampled E_z of the final snapshot is
+    within 1e-5 of its peak of the reference's."""
+    from opensol_amd import model as M
+    golden = np.load(os.path.join(FIXTURES, 'model_big_window_ez.npz'))
+    M.run(json.loads(str(golden['model_json'])), out_dir=tmp_path)
+    want_meta = json.loads(str(golden['npz_metadata']))
+    with np.load(tmp_path / 'results.npz', allow_pickle=False) as npz:
+        assert set(npz.files) == {str(n) for n in golden['npz_names']}
+        meta = json.loads(str(npz['metadata']))
+        for group in ('fields', 'snapshots', 'sweep_snapshots'):
+            assert meta[group] == want_meta[group], group
+        ours, want = npz['snapshot/1/em.ez/values'], golden['npz/snapshot/1/em.ez/values']
+        assert ours.shape == want.shape == (548, 455)
+        peak = float(np.max(np.abs(want)))
+        error = float(np.max(np.abs(ours.astype(np.float64) - want))) / peak
+        print(f'big window resampled E_z: max err / peak {error:.2e}')
+        assert peak > 0 and error <= 1e-5
+        np.testing.assert_array_equal(npz['snapshot/1/em.eps_r/values'], golden['npz/snapshot/1/em.eps_r/values'])

@@ -171,6 +171,8 @@ def test_snapshot_byte_limit_keeps_the_final_state(monkeypatch):
     recorder = Fake()
     recorder._values = lambda: {'ez': np.zeros(10, np.float32)}   # 40 bytes a record
     recorder._cut_values = lambda: []
+    recorder._cut_views = lambda: []
+    recorder.view = None
     recorder.sim = type('S', (), {'time': 0.0})()
     recorder.time_label = lambda: 't = 0.0 fs'
     for index in range(6):
@@ -345,3 +347,42 @@ def test_pec_keeps_the_dielectric_under_it_for_display():
     assert np.all(m.display_array(0)[2:5, 2] == 4.0) and np.all(m.display_array(0)[2:4, 1] == np.float32(2.25))
     m.set_property((3,), np.full((6, 5), 7.0, np.float32))            # a write into every cell reaches the dielectric
     assert np.all(m.display_array(3) == 7.0) and np.all(m.property_array(0)[mask] == -1.0)
+
+
+
+def test_big_window_snapshots_are_resampled_like_the_reference(tmp_path):
+    """A 553 x 459 cell window (past the 250 000-cell snapshot limit): the snapshots hold the nearest-cell resampling
+    on a 548 x 455 display grid with native_shape [553, 459] and no centre arrays, the field group keeps the native
+    cells. The displayed eps_r of the resampled snapshot equals the reference's (the material needs no GPU), and the
+    saved names and snapshot / field metadata equal the reference run's."""
+    from opensol_amd import model as M
+    golden = np.load(os.path.join(FIXTURES, 'model_big_window_ez.npz'))
+    metadata = json.loads(str(golden['npz_metadata']))
+    spec = json.loads(str(golden['model_json']))
+    built, names = _dt(spec)
+    built.sim.model.finalize()
+    recorder = M._Recorder(built)
+    assert recorder.view is not None and recorder.view[2] == [553, 459]
+    shown = M._shown(built.sim.model.materials.display_array(0, recorder.window), recorder.view)
+    np.testing.assert_array_equal(shown, golden['npz/snapshot/1/em.eps_r/values'])
+    native = tuple(e.size - 1 for e in recorder.edges)
+    for snapshot in metadata['snapshots']:
+        t = snapshot['time']
+        values = {k: np.zeros(native, np.float32) for k in recorder.keys}
+        recorder.snapshots.append({'time': t, 'label': None, 'time_label': f't = {t * 1e15:.1f} fs',
+                                   'fields': {k: M._shown(v, recorder.view) for k, v in values.items()},
+                                   'cuts': []})
+        recorder.final_fields = values
+    fields = {key: (v, recorder.edges[0], recorder.edges[1]) for key, v in recorder.final_fields.items()}
+    out = M.Results('completed', {}, {}, [{}], None, fields, {'t_end': recorder.snapshots[-1]['time']},
+                    recorder.snapshots, recorder.layout())
+    out.save(tmp_path)
+    groups = ('field', 'snapshot', 'sweep_snapshot')
+    with np.load(tmp_path / 'results.npz', allow_pickle=False) as npz:
+        ours = {n for n in npz.files if n.split('/')[0] in groups}
+        saved = json.loads(str(npz['metadata']))
+        assert npz['snapshot/1/em.ez/values'].shape == golden['npz/snapshot/1/em.ez/values'].shape == (548, 455)
+        assert npz['field/em.ez/values'].shape == (553, 459)
+    assert ours == {str(n) for n in golden['npz_names'] if str(n).split('/')[0] in groups}
+    for group in ('fields', 'snapshots', 'sweep_snapshots'):
+        assert saved[group] == metadata[group], group
