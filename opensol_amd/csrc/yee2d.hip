@@ -1423,10 +1423,6 @@ __device__ __forceinline__ bool bulk_tile(const yee2d_tile_sets& s, int item, in
     return true;
 }
 
-// Edge tiles are split along x into EDGE_SPLIT row chunks, one wavefront each (the general path is latency-bound:
-// few, slow tiles would otherwise finish long after the bulk).
-constexpr int EDGE_SPLIT = 8;
-
 template <int POL, int VEC, bool EDGE, bool LDS, bool POLES = false, bool LEFT = false>
 __device__ __forceinline__ void tile_2d(const yee2d_desc& d, const TileBox& t, int64_t n, int part = 0,
                                         int parts = 1)
@@ -1466,7 +1462,7 @@ namespace yee {
 template <int POL, int VEC, bool EDGE, bool POLES, bool LEFT = false>
 __device__ __forceinline__ void step_block(const yee2d_desc& d, const yee2d_tile_sets& sets, int64_t n, int block)
 {
-    const int parts = EDGE ? EDGE_SPLIT : 1;
+    const int parts = EDGE ? yee_edge_parts(d.rows_per_tile) : 1;
     const int slot = block * 4 + wave_index();
     const int item = slot / parts;
     int ti = 0, tj = 0;
@@ -1486,8 +1482,8 @@ __device__ __forceinline__ void step_block(const yee2d_desc& d, const yee2d_tile
 // One instance, two kernels over disjoint tile sets. The host lists the edge tile rows and columns (those touching a
 // CPML layer, with the halo row / column, or a source or monitor line); every other tile is a bulk tile. EDGE = false
 // steps the bulk tiles with the lean loop, EDGE = true the edge tiles with the general one, each edge tile split into
-// EDGE_SPLIT row chunks. Tiles are enumerated compactly: no block is launched without work. (One launch holding both
-// paths runs slower: it carries the general path's registers into the bulk tiles.)
+// yee_edge_parts(rows_per_tile) row chunks. Tiles are enumerated compactly: no block is launched without work. (One
+// launch holding both paths runs slower: it carries the general path's registers into the bulk tiles.)
 //
 // The descriptor is read in place in the kernel argument segment (the first argument sits at offset 0) through the
 // constant address space. Bound to the argument itself, the general H_z path took its address in a way that made the

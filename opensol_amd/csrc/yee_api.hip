@@ -67,7 +67,6 @@ int32_t fail(int32_t code, const std::string& message)
 // 8192^2 E_z step (2 columns per lane) 8 rows took 353.3 us against 359.3 at 16, 370.7 at 32, 370.8 at 6 and
 // 410.8 at 4 (and 371-377 us at 10, 12, 20 and 24 rows).
 constexpr int kAutoRows = 8;
-constexpr int kEdgeSplit = 8;     // row chunks per edge tile (yee2d.hip EDGE_SPLIT)
 
 enum CtxKind { kSingle2D = 1, kBatch2D = 2, kSingle3D = 3, kBatch3D = 4 };
 
@@ -92,7 +91,7 @@ struct yee_ctx {
     // launch geometry
     int64_t tiles = 0;
     int blocks = 0;        // blocks of the bulk kernel (4 tiles each)
-    int edge_blocks = 0;   // blocks of the edge kernel (kEdgeSplit wavefronts per tile, 4 per block)
+    int edge_blocks = 0;   // blocks of the edge kernel (yee_edge_parts wavefronts per tile, 4 per block)
     yee2d_tile_sets sets{};
     // 3D
     yee3d_desc d3{};
@@ -452,7 +451,7 @@ void set_geometry(yee_ctx* ctx)
     const int64_t bulk = s.all_edge ? 0 : (int64_t)(s.tiles_i - s.n_rows) * std::max(0, s.tiles_j - s.n_cols);
     const int64_t edge = ctx->tiles - bulk;
     ctx->blocks = (int)((bulk + 3) / 4);
-    ctx->edge_blocks = (int)((edge * kEdgeSplit + 3) / 4);
+    ctx->edge_blocks = (int)((edge * yee_edge_parts(d.rows_per_tile) + 3) / 4);
 }
 
 // Defaults of a 3D descriptor: 4 rows per tile and kAutoPlanes x planes per tile (balanced over the x extent).
