@@ -57,6 +57,30 @@ class LorentzPole:
         return complex(chi) if np.ndim(chi) == 0 else chi
 
 
+def pole_from_dict(pole, number=float):
+    """DrudePole / LorentzPole of a dispersion entry {'model': 'drude', 'omega_p', 'gamma'[, 'eps_inf']} or
+    {'model': 'lorentz', 'delta_eps', 'omega_0', 'gamma'} (physics/em/definition_common.py:1079-1099); `number`
+    evaluates each value. Unknown or missing keys, negative values and eps_inf <= 0 are refused."""
+    model = pole.get('model') if isinstance(pole, dict) else None
+    keys = {'drude': (('omega_p', 'gamma'), ('eps_inf',)),
+            'lorentz': (('delta_eps', 'omega_0', 'gamma'), ())}.get(model)
+    if keys is None:
+        raise ValueError(f"dispersion pole model must be 'drude' or 'lorentz', got {model!r}")
+    required, optional = keys
+    if set(pole) - {'model', *required, *optional} or any(k not in pole for k in required):
+        raise ValueError(f'{model} pole needs {required} (optional {optional}), got {sorted(pole)}')
+    numbers = {k: number(pole[k]) for k in (*required, *optional) if k in pole}
+    if any(v < 0 for v in numbers.values()) or numbers.get('eps_inf', 1.0) <= 0:
+        raise ValueError(f'{model} pole values must be >= 0 (eps_inf > 0), got {numbers}')
+    return DrudePole(**numbers) if model == 'drude' else LorentzPole(**numbers)
+
+
+def background_permittivity(poles):
+    """eps_r of cells whose material gives poles but no eps_r: the first Drude pole's eps_inf, else 1
+    (physics/em/definition_common.py:1102-1111)."""
+    return next((float(p.eps_inf) for p in poles if isinstance(p, DrudePole)), 1.0)
+
+
 def pole_step_term(pole, dt):
     """Pole term of the stability bound (physics/em/materials.py:193-205): Drude (omega_p dt)^2 / 4, Lorentz
     delta_eps (omega_0 dt)^2 / (4 - (omega_0 dt)^2); ValueError when omega_0 dt >= 2."""

@@ -155,6 +155,27 @@ class Domain:
             out.append(np.clip(centres, centres[window.start], centres[window.stop - 1]).astype(DTYPE))
         return tuple(out)
 
+    def rect_mask(self, x0, y0, x1, y1):
+        """Bool host mask of the cells whose float32 centre lies in [x_min, x_max) x [y_min, y_max), corners in
+        any order (core/domain.py:255-260)."""
+        x, y = self._centres32[:2]
+        inside = [(c >= DTYPE(min(a, b))) & (c < DTYPE(max(a, b))) for c, a, b in ((x, x0, x1), (y, y0, y1))]
+        return inside[0][:, None] & inside[1][None, :]
+
+    def circle_mask(self, cx, cy, radius):
+        """Bool host mask of the cells whose float32 centre lies within `radius` of (cx, cy), boundary included
+        (core/domain.py:262-267)."""
+        if not radius >= 0:
+            raise ValueError(f'circle radius must be >= 0, got {radius}')
+        x, y = self._centres32[:2]
+        return (x[:, None] - DTYPE(cx)) ** 2 + (y[None, :] - DTYPE(cy)) ** 2 <= DTYPE(radius) ** 2
+
+    def box_mask(self, lo, hi):
+        """Bool host mask of the cells of box_ranges(lo, hi) on a 3D grid (the 3D counterpart of rect_mask)."""
+        mask = np.zeros(self.shape, dtype=bool)
+        mask[tuple(slice(a, b) for a, b in self.box_ranges(lo, hi))] = True
+        return mask
+
     def box_ranges(self, lows, highs, clamp_to_physical=False):
         """Per-axis half-open index ranges of the cells whose float32 centre lies in [min, max) (the reference's
         rect_mask / box mask compares float32 centres, core/domain.py:90-97, 255-260; the 3D model builder's box raster

@@ -104,20 +104,12 @@ class Model2D:
 
     # ---- dispersive poles (E_z) ----------------------------------------------------------------------------------
     def rect_mask(self, x0, y0, x1, y1):
-        """bool (nx, ny): cells whose float32 centre lies in [x_min, x_max) x [y_min, y_max), compared in float32
-        (core/domain.py:100-112, 255-260)."""
-        xc, yc = (c.astype(DTYPE) for c in self.domain.cell_centers())
-        x_lo, x_hi = DTYPE(min(x0, x1)), DTYPE(max(x0, x1))
-        y_lo, y_hi = DTYPE(min(y0, y1)), DTYPE(max(y0, y1))
-        return ((xc >= x_lo) & (xc < x_hi))[:, None] & ((yc >= y_lo) & (yc < y_hi))[None, :]
+        """bool (nx, ny): cells whose float32 centre lies in [x_min, x_max) x [y_min, y_max) (core/domain.py:255-260)."""
+        return self.domain.rect_mask(x0, y0, x1, y1)
 
     def circle_mask(self, cx, cy, radius):
-        """bool (nx, ny): cells whose float32 centre lies within `radius` of (cx, cy), in float32 (core/domain.py:
-        262-268)."""
-        if not radius >= 0:
-            raise ValueError(f'circle radius must be >= 0, got {radius}')
-        xc, yc = (c.astype(DTYPE) for c in self.domain.cell_centers())
-        return (xc[:, None] - DTYPE(cx)) ** 2 + (yc[None, :] - DTYPE(cy)) ** 2 <= DTYPE(radius) ** 2
+        """bool (nx, ny): cells whose float32 centre lies within `radius` of (cx, cy) (core/domain.py:262-267)."""
+        return self.domain.circle_mask(cx, cy, radius)
 
     def add_drude_rectangle(self, x0, y0, x1, y1, drude):
         """Drude metal on the rectangle's cells, its eps_inf written as their permittivity

@@ -246,29 +246,14 @@ class _Built:
         """Paint one material in list order; returns its dispersive poles (DrudePole / LorentzPole from the
         'dispersion' list, definition_common.py:1079-1099), whose background permittivity is the eps_r property,
         else the first Drude eps_inf, else 1 (:1102-1111)."""
-        from .dispersion import DrudePole, LorentzPole
+        from .dispersion import background_permittivity, pole_from_dict
         properties = material.get('properties') or {}
         unknown = set(properties) - {'eps_r', 'sigma_e'}
         if unknown:
             raise ValueError(f"material {material.get('name')!r}: only eps_r, sigma_e and dispersion are part of "
                              f"the FDTD path")
-        dispersion = material.get('dispersion') or []
-        poles = []
-        for pole in dispersion:
-            model = pole.get('model') if isinstance(pole, dict) else None
-            keys = {'drude': (('omega_p', 'gamma'), ('eps_inf',)),
-                    'lorentz': (('delta_eps', 'omega_0', 'gamma'), ())}.get(model)
-            if keys is None:
-                raise ValueError(f"dispersion pole model must be 'drude' or 'lorentz', got {model!r}")
-            required, optional = keys
-            if set(pole) - {'model', *required, *optional} or any(k not in pole for k in required):
-                raise ValueError(f'{model} pole needs {required} (optional {optional}), got {sorted(pole)}')
-            numbers = {k: value(pole[k]) for k in (*required, *optional) if k in pole}
-            if any(v < 0 for v in numbers.values()) or numbers.get('eps_inf', 1.0) <= 0:
-                raise ValueError(f'{model} pole values must be >= 0 (eps_inf > 0), got {numbers}')
-            poles.append(DrudePole(**numbers) if model == 'drude' else LorentzPole(**numbers))
-        background = next((p.get('eps_inf', 1.0) for p in dispersion if p.get('model') == 'drude'), 1.0)
-        eps = value(properties.get('eps_r', background))
+        poles = [pole_from_dict(pole, value) for pole in material.get('dispersion') or []]
+        eps = value(properties['eps_r']) if 'eps_r' in properties else background_permittivity(poles)
         sigma = value(properties.get('sigma_e', 0.0))
         for name in material.get('selection') or []:
             kind, p = self._shape(name, ('box',) if self.three_d else ('rectangle', 'circle'))

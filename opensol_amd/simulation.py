@@ -12,11 +12,144 @@ Stepping runs on the GPU only (libyee.so through ctypes); the host keeps the flo
 flux readout, as the reference's host code does.
 """
 import time as _time
+from types import MappingProxyType
 
 import numpy as np
 
 from .constants import DTYPE
 from .fdtd2d import Engine2D, Model2D, steps_for_duration
+
+MATERIAL_PROPERTIES = ('eps_r', 'sigma_e')   # the material properties of the FDTD path
+
+
+def material_values(material, dispersion, properties):
+    """(eps_r, sigma_e, poles) of add_material's arguments (simulation.py:104-177) on the FDTD path: numeric eps_r /
+    sigma_e and a dispersion list of DrudePole / LorentzPole objects or their model-JSON dicts; without eps_r the
+    cells take the background permittivity of the poles. Library materials are refused."""
+    from .dispersion import DrudePole, LorentzPole, background_permittivity, pole_from_dict
+    if material is not None:
+        raise ValueError('add_material: library materials are not part of the FDTD path; give eps_r, sigma_e and '
+                         'dispersion')
+    unknown = sorted(set(properties) - set(MATERIAL_PROPERTIES))
+    if unknown:
+        raise ValueError(f'unknown material properties {unknown}; valid: {", ".join(MATERIAL_PROPERTIES)}')
+    for name, v in properties.items():
+        if v is not None and (isinstance(v, (str, bytes, bool)) or np.ndim(v) != 0):
+            raise ValueError(f'material property {name}={v!r}: add_material takes numbers')
+    poles = [pole if isinstance(pole, (DrudePole, LorentzPole)) else pole_from_dict(pole) for pole in dispersion or []]
+    eps = properties.get('eps_r')
+    sigma = properties.get('sigma_e')
+    return (background_permittivity(poles) if eps is None else float(eps),
+            0.0 if sigma is None else float(sigma), poles)
+
+
+def _device_bytes(obj, seen, visited=None):
+    """Bytes of the torch device tensors reachable from obj through containers and this package's objects, each
+    storage once (seen holds counted storage addresses); modules, libraries and foreign objects are not walked."""
+    import torch
+    visited = set() if visited is None else visited
+    if isinstance(obj, torch.Tensor):
+        if not obj.is_cuda:
+            return 0
+        storage = obj.untyped_storage()
+        if storage.data_ptr() in seen:
+            return 0
+        seen.add(storage.data_ptr())
+        return int(storage.nbytes())
+    if id(obj) in visited:
+        return 0
+    if isinstance(obj, dict):
+        items = list(obj.values())
+    elif isinstance(obj, (list, tuple)):
+        items = obj
+    elif type(obj).__module__.startswith(__package__ + '.') and hasattr(obj, '__dict__'):
+        items = list(vars(obj).values())
+    else:
+        return 0
+    visited.add(id(obj))
+    return sum(_device_bytes(item, seen, visited) for item in items)
+
+
+def _format_bytes(n):
+    return f'{n / 2**30:.2f} GiB' if n >= 2**30 else f'{n / 2**20:.1f} MiB'
+
+
+def memory_report(sim, file=None, physics='em'):
+    """Print (to file, default stdout) and return the memory status and the device bytes this simulation owns
+    (simulation.py:205-245): 'materials' the per-cell material ids and coefficient tables, `physics` ('em', 'em3d')
+    every other array of the engine (both field sets, CPML psi, pole, source and monitor tables, accumulators, probes), 'couplings' 0 (none
+    on this path); a storage shared by two owners counts for the first. The status dict has the reference's keys
+    (pool_used, pool_reserved, device_free, device_total, budget, other, host_available, kind); no budget is set on
+    this path, and host_owners is empty (the FDTD path keeps no CPU factorisations)."""
+    import psutil
+    import torch
+    if sim.model is None:
+        raise RuntimeError('simulation cleared')
+    engine = sim.engine
+    seen = set()
+    owners = {'materials': 0, physics: 0, 'couplings': 0}
+    if engine is not None:
+        for name in ('material', 'table', 'table2'):
+            owners['materials'] += _device_bytes(getattr(engine, name, None), seen)
+        owners[physics] = _device_bytes(engine, seen)
+    host = psutil.virtual_memory()
+    if torch.cuda.is_available():
+        free, total = torch.cuda.mem_get_info()
+        reserved = int(torch.cuda.memory_reserved())
+        status = {'pool_used': int(torch.cuda.memory_allocated()), 'pool_reserved': reserved, 'device_free': int(free),
+                  'device_total': int(total), 'budget': None, 'other': max(0, int(total) - int(free) - reserved),
+                  'host_available': int(host.available), 'kind': 'gpu'}
+        lines = ['GPU memory (pool counters cover the whole process)',
+                 *(f'  {key.replace("_", " "):<14} {_format_bytes(status[key])}'
+                   for key in ('pool_used', 'pool_reserved', 'other', 'device_free', 'device_total')),
+                 '  budget         not set', 'Owned by this simulation (device arrays)']
+    else:
+        status = {'pool_used': 0, 'pool_reserved': 0, 'device_free': int(host.available), 'device_total':
+                  int(host.total), 'budget': None, 'other': int(host.total - host.available),
+                  'host_available': int(host.available), 'kind': 'cpu'}
+        lines = ['Host memory (no GPU visible)', f'  available      {_format_bytes(status["device_free"])}',
+                 f'  total          {_format_bytes(status["device_total"])}', 'Owned by this simulation (device arrays)']
+    lines += [f'  {name:<14} {_format_bytes(size)}' for name, size in owners.items()]
+    print('\n'.join(lines), file=file)
+    return {'status': status, 'owners': owners, 'host_owners': {}}
+
+
+def _clear(sim):
+    """Close the engine, unbind the model's monitors and probes from it (so that handles the caller keeps hold no
+    device arrays) and drop the model."""
+    sim.close()
+    model = sim.model
+    if model is not None:
+        for group in ('monitors', 'dft_monitors', 'probes'):
+            for item in getattr(model, group, ()):
+                item.engine = None
+    sim.model = None
+
+
+def _add_material(sim, mask, material, dispersion, properties, add_pole):
+    if sim.model is None:
+        raise RuntimeError('simulation cleared')
+    model = sim.model
+    mask = np.asarray(mask)
+    if mask.dtype != np.bool_ or mask.shape != tuple(model.shape):
+        raise ValueError(f'add_material needs a bool mask of the full grid {tuple(model.shape)} (domain.rect_mask / '
+                         f'circle_mask), got {mask.dtype} {mask.shape}')
+    eps, sigma, poles = material_values(material, dispersion, properties)
+    if poles and sim.engine is not None:
+        raise ValueError('add_material with dispersion: add dispersive materials before the first run()')
+    if any(np.any(cells & mask) for _, cells in model.poles):
+        raise ValueError('add_material: the mask overlaps dispersive cells (poles are never removed): paint '
+                         'dispersive materials after the materials they replace')
+    materials = model.materials
+    material = materials.material_id(eps, eps, eps, sigma)
+    count = len(model.poles)
+    try:   # the poles first, so that a refused one (the per-model pole limit) leaves the model as it was
+        for pole in poles:
+            add_pole(pole, mask)
+    except Exception:
+        del model.poles[count:]
+        raise
+    materials.paint_mask(mask, material)
 
 
 class Simulation:
@@ -118,6 +251,14 @@ class Simulation:
         """PEC on the rectangle raster of add_dielectric_rectangle (this build's convenience over add_pec_region)."""
         self.model.add_pec_rectangle(x0, y0, x1, y1)
 
+    def add_material(self, mask, material=None, dispersion=None, **properties):
+        """Paint eps_r / sigma_e (numbers) and optional Drude / Lorentz dispersion on a bool (nx, ny) full-grid cell
+        mask (domain.rect_mask / circle_mask); later calls win cells (simulation.py:104-177). Poles need an unbuilt
+        simulation and may not overlap earlier dispersive cells, as the reference refuses (its poles are never
+        removed)."""
+        _add_material(self, mask, material, dispersion, properties, lambda pole, cells: self.model.add_dispersive(
+            pole, cells))
+
     def add_line_source(self, axis, position, wavelength=None, frequency=None, bandwidth=0.5, amplitude=1.0,
                         waveform='gaussian', span=None):
         self._require_unbuilt('add_line_source')
@@ -182,6 +323,8 @@ class Simulation:
     def finalize(self):
         """Courant check, calibration, DFT interval and (first call) the device engine."""
         model = self.model
+        if model is None:
+            raise RuntimeError('simulation cleared')
         if self.engine is None:
             model.finalize()
             self.engine = Engine2D(model, **self._engine_options)
@@ -238,6 +381,19 @@ class Simulation:
 
     def get_eps_r_snapshot(self):
         return self.get_field_snapshot('eps_r' if self.polarization == 'ez' else 'eps_xx')
+
+    @property
+    def physics(self):
+        """Read-only name -> physics mapping (simulation.py:72-75): the one EM model."""
+        return MappingProxyType({} if self.model is None else {'em': self.model})
+
+    def memory_report(self, file=None) -> dict:
+        return memory_report(self, file)
+
+    def clear(self):
+        """Free the device state and drop the model; a cleared Simulation is dead (simulation.py:191-203)."""
+        _clear(self)
+        self.callbacks = []
 
     def state(self):
         """Host copies of the current fields."""
@@ -319,6 +475,24 @@ class Simulation3D:
         (physics/em3d/physics3d.py:351-353)."""
         self.model.add_pec_box(lo, hi)
 
+    def add_material(self, mask, material=None, dispersion=None, **properties):
+        """Paint eps_r / sigma_e (numbers) and optional Drude / Lorentz dispersion on a bool (nx, ny, nz) full-grid
+        cell mask; later calls win cells (simulation.py:104-177). Poles as in Simulation.add_material."""
+        window = tuple(slice(0, n) for n in self.model.shape)
+        _add_material(self, mask, material, dispersion, properties,
+                      lambda pole, cells: self.model.add_dispersive(pole, window, cells))
+
+    @property
+    def physics(self):
+        """Read-only name -> physics mapping: the one 3D EM model, 'em3d' as its attribute name."""
+        return MappingProxyType({} if self.model is None else {'em3d': self.model})
+
+    def memory_report(self, file=None) -> dict:
+        return memory_report(self, file, 'em3d')
+
+    def clear(self):
+        _clear(self)
+
     def add_dipole(self, x, y, z, orientation='z', dipole_moment=1e-12, wavelength=None, frequency=None,
                    bandwidth=0.5, waveform='gaussian'):
         self._require_unbuilt('add_dipole')
@@ -357,6 +531,8 @@ class Simulation3D:
     def finalize(self):
         from .fdtd3d import Engine3D
         model = self.model
+        if model is None:
+            raise RuntimeError('simulation cleared')
         if self.engine is None:
             model.finalize()
             self.engine = Engine3D(model, **self._engine_options)
@@ -364,7 +540,7 @@ class Simulation3D:
             model.finalize()
             self.engine.materials_changed()
         self._materials_version = model.materials.version
-        return {'em': []}
+        return {'em3d': []}
 
     def run(self, duration: float = None, n_steps: int = None, print_every: int = 0, flush: bool = True):
         """Advance by `duration` [s] (stopping once t >= t_end - dt/2) or exactly `n_steps` steps."""

@@ -1,0 +1,163 @@
+"""Scripting API of the simulation (simulation.py:18-245): add_material on full-grid masks with the model-JSON
+painter's rules, the physics mapping, clear() and memory_report without a device. The painted grids are compared
+with the ones the existing painters (add_dielectric_rectangle / circle, add_drude_rectangle, add_dispersive) write,
+which the oracle and golden tests already pin; add_material must give the same material ids and pole cells."""
+import io
+
+import numpy as np
+import pytest
+
+from tests.cases import POLES2D
+
+
+def _sim(polarization='ez'):
+    from opensol_amd import Simulation
+    spec = POLES2D
+    return Simulation(spec['size_x'], spec['size_y'], spec['dx'], npml=spec['npml'], boundaries=spec['boundaries'],
+                      polarization=polarization)
+
+
+def _eps(sim):
+    materials = sim.model.materials
+    return materials.values()[materials.ids][..., 0]
+
+
+def test_masks_are_the_painters_rasters():
+    a, b = _sim(), _sim()
+    a.add_dielectric_rectangle(0.31e-6, 0.87e-6, 0.1e-6, 0.2e-6, eps_r=3.0)   # corners in any order
+    b.add_material(b.domain.rect_mask(0.31e-6, 0.87e-6, 0.1e-6, 0.2e-6), eps_r=3.0)
+    a.model.add_dielectric_circle(1.2e-6, 0.5e-6, 0.2e-6, eps_r=5.0, sigma_e=0.0)
+    b.add_material(b.domain.circle_mask(1.2e-6, 0.5e-6, 0.2e-6), eps_r=5.0)
+    assert np.array_equal(a.model.materials.ids, b.model.materials.ids)
+    # the rectangle's cells by brute force over float32 centres: half-open on both axes
+    x, y = (c.astype(np.float32) for c in b.domain.cell_centers())
+    want = np.array([[np.float32(0.1e-6) <= xi < np.float32(0.31e-6) and np.float32(0.2e-6) <= yj < np.float32(0.87e-6)
+                      for yj in y] for xi in x])
+    assert np.array_equal(b.domain.rect_mask(0.1e-6, 0.2e-6, 0.31e-6, 0.87e-6), want)
+    with pytest.raises(ValueError, match='radius'):
+        b.domain.circle_mask(0.0, 0.0, -1.0)
+
+
+def test_dispersive_materials_match_the_pole_painters():
+    """The pole case painted with add_material equals the one add_drude_rectangle / add_dispersive paint: the Drude
+    cells take eps_inf (the background rule), the Lorentz disc keeps the eps_r it is given."""
+    from opensol_amd import DrudePole, LorentzPole
+    spec = POLES2D
+    (x0, y0, x1, y1), eps = spec['dielectric']
+    (cx, cy), radius = spec['lorentz_disc']
+    a, b = _sim(), _sim()
+    a.add_dielectric_rectangle(x0, y0, x1, y1, eps_r=eps)
+    a.add_drude_rectangle(*spec['drude_rect'], DrudePole(**spec['drude']))
+    a.add_dispersive(LorentzPole(**spec['lorentz']), a.model.circle_mask(cx, cy, radius))
+    b.add_material(b.domain.rect_mask(x0, y0, x1, y1), eps_r=eps)
+    b.add_material(b.domain.rect_mask(*spec['drude_rect']), dispersion=[dict(model='drude', **spec['drude'])])
+    b.add_material(b.domain.circle_mask(cx, cy, radius), eps_r=eps, dispersion=[LorentzPole(**spec['lorentz'])])
+    assert np.array_equal(a.model.materials.ids, b.model.materials.ids)
+    assert len(a.model.poles) == len(b.model.poles) == 2
+    for (pa, ma), (pb, mb) in zip(a.model.poles, b.model.poles):
+        assert type(pa) is type(pb) and vars(pa) == vars(pb)
+        assert np.array_equal(ma, mb)
+    drude = b.domain.rect_mask(*spec['drude_rect'])
+    assert np.all(_eps(b)[drude] == np.float32(spec['drude']['eps_inf']))
+
+
+def test_background_permittivity_rule():
+    sim = _sim()
+    lorentz, drude = sim.domain.rect_mask(0.1e-6, 0.1e-6, 0.3e-6, 0.3e-6), sim.domain.rect_mask(0.5e-6, 0.1e-6,
+                                                                                             0.7e-6, 0.3e-6)
+    sim.add_material(lorentz, dispersion=[dict(model='lorentz', delta_eps=1.0, omega_0=2e15, gamma=1e14)])
+    sim.add_material(drude, dispersion=[dict(model='lorentz', delta_eps=1.0, omega_0=2e15, gamma=1e14),
+                                        dict(model='drude', omega_p=4e15, gamma=1e14, eps_inf=3.5),
+                                        dict(model='drude', omega_p=4e15, gamma=1e14, eps_inf=7.0)])
+    eps = _eps(sim)
+    assert np.all(eps[lorentz] == 1.0)                  # no Drude pole: 1
+    assert np.all(eps[drude] == np.float32(3.5))        # the first Drude pole's eps_inf
+    assert len(sim.model.poles) == 4
+    plain = sim.domain.rect_mask(1.5e-6, 0.1e-6, 1.7e-6, 0.3e-6)
+    sim.add_material(plain, sigma_e=2.5)
+    values = sim.model.materials.values()[sim.model.materials.ids]
+    assert np.all(values[plain][:, :3] == 1.0) and np.all(values[plain][:, 3] == np.float32(2.5))
+
+
+def test_add_material_refusals():
+    from opensol_amd import DrudePole
+    sim = _sim()
+    full = np.zeros(sim.model.shape, dtype=bool)
+    full[5:9, 5:9] = True
+    with pytest.raises(ValueError, match='library materials'):
+        sim.add_material(full, 'Si')
+    with pytest.raises(ValueError, match=r"unknown material properties \['k'\]"):
+        sim.add_material(full, k=0.1)
+    with pytest.raises(ValueError, match='takes numbers'):
+        sim.add_material(full, eps_r='2*eps0')
+    with pytest.raises(ValueError, match='full grid'):
+        sim.add_material(full[:-1], eps_r=2.0)
+    with pytest.raises(ValueError, match='full grid'):
+        sim.add_material(full.astype(np.uint8), eps_r=2.0)
+    with pytest.raises(ValueError, match="'drude' or 'lorentz'"):
+        sim.add_material(full, dispersion=[dict(model='debye', tau=1e-15)])
+    with pytest.raises(ValueError, match='pole needs'):
+        sim.add_material(full, dispersion=[dict(model='drude', omega_p=1e15)])
+    with pytest.raises(ValueError, match='> 0'):
+        sim.add_material(full, eps_r=-1.0)
+    sim.add_material(full, dispersion=[DrudePole(3e15, 1e14, 2.0)])
+    overlap = np.zeros_like(full)
+    overlap[8:12, 8:12] = True
+    with pytest.raises(ValueError, match='overlaps dispersive cells'):
+        sim.add_material(overlap, eps_r=4.0)
+    sim.engine = object()   # stands for a built engine: dispersion is refused before the engine is touched
+    beside = np.zeros_like(full)
+    beside[20:22, 20:22] = True
+    with pytest.raises(ValueError, match='before the first run'):
+        sim.add_material(beside, dispersion=[DrudePole(3e15, 1e14, 2.0)])
+    sim.engine = None
+
+
+def test_physics_mapping_clear_and_host_memory_report():
+    sim = _sim('hz')
+    physics = sim.physics
+    assert list(physics) == ['em'] and physics['em'] is sim.model
+    with pytest.raises(TypeError):
+        physics['other'] = sim.model
+    out = io.StringIO()
+    report = sim.memory_report(file=out)
+    assert report['owners'] == {'materials': 0, 'em': 0, 'couplings': 0}   # nothing on the device before the build
+    assert report['host_owners'] == {}
+    assert set(report['status']) == {'pool_used', 'pool_reserved', 'device_free', 'device_total', 'budget', 'other',
+                                     'host_available', 'kind'}
+    assert report['status']['kind'] == 'cpu' and report['status']['device_total'] > 0
+    assert 'Owned by this simulation' in out.getvalue() and 'materials' in out.getvalue()
+    sim.clear()
+    assert dict(sim.physics) == {}
+    for call in (lambda: sim.run(n_steps=1), sim.finalize, sim.memory_report,
+                 lambda: sim.add_material(np.zeros((4, 4), bool), eps_r=2.0)):
+        with pytest.raises(RuntimeError, match='simulation cleared'):
+            call()
+
+
+def test_3d_add_material_matches_the_box_painter():
+    from opensol_amd import DrudePole, Simulation3D
+    a, b = (Simulation3D(0.4e-6, 0.35e-6, 0.3e-6, 25e-9, npml=4) for _ in range(2))
+    lo, hi = (0.05e-6, 0.1e-6, 0.05e-6), (0.2e-6, 0.3e-6, 0.25e-6)
+    a.add_dielectric_box(lo, hi, 2.5)
+    b.add_material(b.domain.box_mask(lo, hi), eps_r=2.5)
+    assert np.array_equal(a.model.materials.ids, b.model.materials.ids)
+    b.add_material(b.domain.box_mask((0.25e-6, 0.0, 0.0), (0.4e-6, 0.1e-6, 0.1e-6)),
+                   dispersion=[DrudePole(3e15, 1e14, 2.0)])
+    assert len(b.model.poles) == 1 and b.model.poles[0][1].sum() == np.count_nonzero(
+        b.domain.box_mask((0.25e-6, 0.0, 0.0), (0.4e-6, 0.1e-6, 0.1e-6)))
+    assert list(b.physics) == ['em3d'] and b.physics['em3d'] is b.em3d
+    b.clear()
+    with pytest.raises(RuntimeError, match='simulation cleared'):
+        b.run(n_steps=1)
+
+
+def test_refused_pole_leaves_the_model_unchanged():
+    from opensol_amd import _lib, LorentzPole
+    sim = _sim()
+    mask = sim.domain.rect_mask(0.1e-6, 0.1e-6, 0.3e-6, 0.3e-6)
+    ids = sim.model.materials.ids.copy()
+    poles = [LorentzPole(1.0, 2e15, 1e14)] * (_lib.MAX_POLES + 1)
+    with pytest.raises(ValueError, match='at most'):
+        sim.add_material(mask, eps_r=3.0, dispersion=poles)
+    assert sim.model.poles == [] and np.array_equal(sim.model.materials.ids, ids)
