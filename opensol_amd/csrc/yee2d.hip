@@ -1462,7 +1462,7 @@ namespace yee {
 template <int POL, int VEC, bool EDGE, bool POLES, bool LEFT = false>
 __device__ __forceinline__ void step_block(const yee2d_desc& d, const yee2d_tile_sets& sets, int64_t n, int block)
 {
-    const int parts = EDGE ? yee_edge_parts(d.rows_per_tile) : 1;
+    const int parts = EDGE ? sets.edge_parts : 1;
     const int slot = block * 4 + wave_index();
     const int item = slot / parts;
     int ti = 0, tj = 0;
@@ -1482,7 +1482,7 @@ __device__ __forceinline__ void step_block(const yee2d_desc& d, const yee2d_tile
 // One instance, two kernels over disjoint tile sets. The host lists the edge tile rows and columns (those touching a
 // CPML layer, with the halo row / column, or a source or monitor line); every other tile is a bulk tile. EDGE = false
 // steps the bulk tiles with the lean loop, EDGE = true the edge tiles with the general one, each edge tile split into
-// yee_edge_parts(rows_per_tile) row chunks. Tiles are enumerated compactly: no block is launched without work. (One
+// sets.edge_parts row chunks (yee_edge_split). Tiles are enumerated compactly: no block is launched without work. (One
 // launch holding both paths runs slower: it carries the general path's registers into the bulk tiles.)
 //
 // The descriptor is read in place in the kernel argument segment (the first argument sits at offset 0) through the

@@ -91,7 +91,7 @@ struct yee_ctx {
     // launch geometry
     int64_t tiles = 0;
     int blocks = 0;        // blocks of the bulk kernel (4 tiles each)
-    int edge_blocks = 0;   // blocks of the edge kernel (yee_edge_parts wavefronts per tile, 4 per block)
+    int edge_blocks = 0;   // blocks of the edge kernel (sets.edge_parts wavefronts per tile, 4 per block)
     yee2d_tile_sets sets{};
     // 3D
     yee3d_desc d3{};
@@ -451,7 +451,8 @@ void set_geometry(yee_ctx* ctx)
     const int64_t bulk = s.all_edge ? 0 : (int64_t)(s.tiles_i - s.n_rows) * std::max(0, s.tiles_j - s.n_cols);
     const int64_t edge = ctx->tiles - bulk;
     ctx->blocks = (int)((bulk + 3) / 4);
-    ctx->edge_blocks = (int)((edge * yee_edge_parts(d.rows_per_tile) + 3) / 4);
+    ctx->sets.edge_parts = yee_edge_split(d.rows_per_tile, edge);
+    ctx->edge_blocks = (int)((edge * ctx->sets.edge_parts + 3) / 4);
 }
 
 // Defaults of a 3D descriptor: 4 rows per tile and kAutoPlanes x planes per tile (balanced over the x extent).
