@@ -114,6 +114,25 @@ def memory_report(sim, file=None, physics='em'):
     return {'status': status, 'owners': owners, 'host_owners': {}}
 
 
+def mesh_info(sim):
+    """Print the window and grid cell counts, the CPML padding per face, the cell sizes and the grid's time step
+    (simulation.py:359-378; the grid here is always uniform)."""
+    if sim.model is None:
+        raise RuntimeError('simulation cleared')
+    domain = sim.domain
+    counts = [w.stop - w.start for w in domain.physical_window()]
+    extent = domain.physical_extent()[1::2]
+    axes = domain.axes
+    print('\n--- Mesh info (uniform) ---')
+    print(f"  Physical domain: {' x '.join(f'{e * 1e6:.3f}' for e in extent)} µm, "
+          f"{' x '.join(str(n) for n in counts)} = {int(np.prod(counts)):,} cells")
+    print(f"  Total grid: {' x '.join(str(n) for n in domain.shape)} = {int(np.prod(domain.shape)):,} cells, "
+          f'PML cells per side {domain.npml_sides}')
+    print('  ' + ', '.join(f'd{a}: {w.min() * 1e9:.2f}-{w.max() * 1e9:.2f} nm' for a, w in zip(axes,
+                                                                                         domain.cell_widths())))
+    print(f'  Grid dt = {float(domain.grid.dt):.4e} s')
+
+
 def _clear(sim):
     """Close the engine, unbind the model's monitors and probes from it (so that handles the caller keeps hold no
     device arrays) and drop the model."""
@@ -390,6 +409,9 @@ class Simulation:
     def memory_report(self, file=None) -> dict:
         return memory_report(self, file)
 
+    def mesh_info(self):
+        mesh_info(self)
+
     def clear(self):
         """Free the device state and drop the model; a cleared Simulation is dead (simulation.py:191-203)."""
         _clear(self)
@@ -489,6 +511,9 @@ class Simulation3D:
 
     def memory_report(self, file=None) -> dict:
         return memory_report(self, file, 'em3d')
+
+    def mesh_info(self):
+        mesh_info(self)
 
     def clear(self):
         _clear(self)

@@ -161,3 +161,14 @@ def test_refused_pole_leaves_the_model_unchanged():
     with pytest.raises(ValueError, match='at most'):
         sim.add_material(mask, eps_r=3.0, dispersion=poles)
     assert sim.model.poles == [] and np.array_equal(sim.model.materials.ids, ids)
+
+
+def test_mesh_info_lines(capsys):
+    from opensol_amd import Simulation
+    sim = Simulation(6e-6, 3e-6, 25e-9, npml=15, boundaries='periodic_y')
+    sim.mesh_info()
+    out = capsys.readouterr().out
+    assert '240 x 120 = 28,800 cells' in out and '270 x 120 = 32,400 cells' in out
+    assert "'x_lo': 15" in out and "'y_lo': 0" in out
+    assert 'dx: 25.00-25.00 nm, dy: 25.00-25.00 nm' in out
+    assert f'Grid dt = {float(sim.grid.dt):.4e} s' in out
