@@ -344,6 +344,9 @@ __device__ __forceinline__ int ldid(Rsrc r, uint32_t lane_bytes, uint32_t row_by
 {
     return (int)__builtin_amdgcn_raw_buffer_load_b16(r, (int)lane_bytes, (int)row_bytes, 0);
 }
+// Default cache policy: non-temporal stores (nt) measured 2.240 against 2.085 ms per 512^3 step, while the 2D stores
+// gain from nt. (Unlike the 2D rows, a 62-cell z row writes partial 64 B segments at both ends; that difference is
+// the likely reason, not a measured one.)
 __device__ __forceinline__ void stf(Rsrc r, uint32_t lane_bytes, uint32_t row_bytes, float v)
 {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)lane_bytes, (int)row_bytes, 0);
