@@ -513,7 +513,7 @@ def main():
     parser.add_argument('--steps', type=int, default=None, help='timed steps (default 2000 2D, 500 3D)')
     parser.add_argument('--warmup', type=int, default=None, help='untimed steps (default 50 2D, 20 3D)')
     parser.add_argument('--graph-steps', type=int, default=None, help='steps per HIP graph (default 50 2D, 10 3D)')
-    parser.add_argument('--rows-per-tile', type=int, default=0, help='2D: 0 fills the device wave slots')
+    parser.add_argument('--rows-per-tile', type=int, default=0, help='2D: rows per tile (0: the library default, 8)')
     parser.add_argument('--lane-columns', type=int, default=0, choices=(0, 2, 4), help='2D: 0 = library default')
     parser.add_argument('--planes-per-tile', type=int, default=0, help='3D: x planes per tile (0: the library default, 4)')
     parser.add_argument('--cpu-steps', type=int, default=None,
