@@ -54,6 +54,7 @@ class Job:
     admitted: float = None
     finished: float = None
     cells: int = 0
+    bytes: int = 0        # device state counted against the HBM budget while resident
     result: list = field(default_factory=list)
     status: str = 'pending'   # pending, running, done, rejected, cancelled, failed
     error: str = ''
