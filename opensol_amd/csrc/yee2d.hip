@@ -1437,9 +1437,9 @@ __device__ __forceinline__ void tile_2d(const yee2d_desc& d, const TileBox& t, i
 {
     const bool ez = POL == YEE_EZ || (POL == POL_ANY && d.polarization == YEE_EZ);
     if constexpr (EDGE) {
-        const int chunk = (t.i1 - t.i0 + parts - 1) / parts;
-        int r0 = t.i0 + part * chunk;
-        int r1 = min(t.i1, r0 + chunk);
+        const RowSpan span = edge_chunk(t.i0, t.i1, part, parts);
+        const int r0 = span.r0;
+        int r1 = span.r1;
         if (!ez && (d.wall_sign[1] == 1 || d.wall_sign[1] == -1)) {
             // an H_z x_hi copy wall: row nx - 1 copies row nx - 2, so the chunk holding row nx - 2 takes row nx - 1
             // too and no chunk starts at it

@@ -27,6 +27,18 @@ inline int yee_edge_split(int rows_per_tile, int64_t edge_tiles)
     return parts;
 }
 
+// Rows [r0, r1) of chunk `part` of `parts` of the tile rows [i0, i1): ceil((i1 - i0) / parts) rows per chunk, so the
+// non-empty chunks cover the tile once; trailing chunks may be empty (r0 >= r1).
+struct RowSpan {
+    int r0, r1;
+};
+constexpr RowSpan edge_chunk(int i0, int i1, int part, int parts)
+{
+    const int chunk = (i1 - i0 + parts - 1) / parts;
+    const int r0 = i0 + part * chunk;
+    return RowSpan{r0, r0 + chunk < i1 ? r0 + chunk : i1};
+}
+
 // Edge tile rows / columns of one 2D instance (sorted, unique); every other tile is a bulk tile. all_edge: the lists
 // did not fit, every tile is an edge tile. edge_parts: row chunks per edge tile in the single-instance edge kernel
 // (yee_edge_split; the batch kernel runs whole tiles).
